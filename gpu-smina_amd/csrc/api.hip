@@ -1,0 +1,1366 @@
+/*
+ * The C ABI of include/sgpu.h over the gfx950 kernels.  A context owns one
+ * device's memory: tables, receptors, maps, model batches and per-call
+ * scratch.  Every entry point validates its arguments, launches on the
+ * context's stream and synchronises before returning (as the reference's
+ * run_* functions do, src/gpu/gpu_kernels_impl.cuh:312, :549, :996).
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/sgpu.h"
+#include "chain.h"
+#include "launch.h"
+
+using namespace sgpu;
+
+namespace
+{
+
+struct Fail
+{
+	sgpu_status code;
+	std::string msg;
+};
+
+void hip_check(hipError_t e, const char* what)
+{
+	if (e == hipErrorOutOfMemory)
+		throw Fail{SGPU_E_OOM, std::string(what) + ": out of device memory"};
+	if (e != hipSuccess)
+		throw Fail{SGPU_E_HIP, std::string(what) + ": " + hipGetErrorString(e)};
+}
+void require(bool ok, const std::string& msg)
+{
+	if (!ok)
+		throw Fail{SGPU_E_INVALID, msg};
+}
+
+template<typename T> T Real_host_max();
+template<> float Real_host_max<float>() { return FLT_MAX; }
+template<> double Real_host_max<double>() { return DBL_MAX; }
+
+// device buffer owned by the context
+struct DBuf
+{
+	void* p = nullptr;
+	size_t n = 0;
+	DBuf() = default;
+	DBuf(const DBuf&) = delete;
+	DBuf& operator=(const DBuf&) = delete;
+	~DBuf()
+	{
+		if (p)
+			(void)hipFree(p);
+	}
+	void alloc(size_t bytes)
+	{
+		if (p)
+			(void)hipFree(p);
+		p = nullptr;
+		n = bytes;
+		if (bytes)
+			hip_check(hipMalloc(&p, bytes), "hipMalloc");
+	}
+	template<typename X> X* as() const { return static_cast<X*>(p); }
+};
+template<typename X>
+void upload(DBuf& b, const std::vector<X>& v, hipStream_t st)
+{
+	b.alloc(std::max<size_t>(1, v.size()) * sizeof(X));
+	if (!v.empty())
+		hip_check(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(X), hipMemcpyHostToDevice, st), "upload");
+}
+
+struct Tables
+{
+	int n = 0;
+	double factor = 0, cutoff_sqr = 0;
+	DBuf smooth32, smooth64, fast;
+	bool ok = false;
+};
+
+struct Cells
+{
+	int off[3], dim[3];
+	double begin[3], end[3];
+	int enabled[3];
+	DBuf start, atoms;
+};
+
+struct Receptor
+{
+	int n = 0;
+	std::vector<double> xyz;
+	std::vector<int> type;
+	DBuf xyz64, xyz32, typ;
+	// direct-receptor cells per box (full 3 A bricks)
+	std::map<std::vector<double>, std::unique_ptr<Cells>> cells;
+};
+
+struct Grid
+{
+	int receptor = -1;
+	int precision = 0;
+	sgpu_grid_dims gd;
+	int dim[3];
+	double init[3], factor[3], factor_inv[3], dim_fl_minus_1[3];
+	std::vector<int> types;
+	int slot_of[NUM_ATOM_TYPES];
+	DBuf points; // FP64 point maps
+	DBuf cells;  // corner-packed, precision
+	DBuf slot_dev;
+};
+
+struct HostBatch
+{
+	std::vector<ModelHdr> hdr;
+	std::vector<int> atom_type, atom_frame, nbr_start, nbr_code, frame_parent, frame_kind, frame_begin, frame_end,
+			frame_conf, frame_change, child_start, child_count, child_list, level_start, level_frames, pair_a, pair_b,
+			pair_tri, heavy, tree_root, tree_conf, tree_change, tree_ligand, tree_ntors, tree_atom_begin,
+			tree_atom_end, tree_pair_begin, tree_pair_end, maps;
+	std::vector<double> atom_xyz0, atom_local, frame_origin, frame_axis;
+	WavePlan plan{};
+};
+
+struct Batch
+{
+	int precision = 0;
+	int n = 0;
+	HostBatch h;
+	DBuf hdr, atom_type, atom_frame, nbr_start, nbr_code, frame_parent, frame_kind, frame_begin, frame_end,
+			frame_conf, frame_change, child_start, child_count, child_list, level_start, level_frames, pair_a,
+			pair_b, pair_tri, heavy, tree_root, tree_conf, tree_change, tree_ligand, tree_ntors, tree_atom_begin,
+			tree_atom_end, tree_pair_begin, tree_pair_end, maps, atom_xyz0, atom_local, frame_origin, frame_axis;
+	// FP64 copy for the final rescoring when the batch searches in FP32
+	std::unique_ptr<Batch> fp64;
+};
+
+} // namespace
+
+struct sgpu_ctx
+{
+	int device = 0;
+	hipStream_t stream = nullptr;
+	hipEvent_t ev0 = nullptr, ev1 = nullptr;
+	std::mutex mu;
+	std::string err;
+	Tables tab;
+	std::map<int, std::unique_ptr<Receptor>> receptors;
+	std::map<int, std::unique_ptr<Grid>> grids;
+	std::map<int, std::unique_ptr<Batch>> batches;
+	int next_id = 1;
+	std::map<std::string, double> last_ms;
+	DBuf scratch; // per-call scratch
+	DBuf scratch2;
+	DBuf err_flag;
+};
+
+namespace
+{
+
+void pack_batch(const sgpu_flat_model* ms, int n, HostBatch& B)
+{
+	int nmax = 0, fmax = 0, cmax = 0, gmax = 0, pmax = 0;
+	for (int mi = 0; mi < n; mi++)
+	{
+		const sgpu_flat_model& m = ms[mi];
+		require(m.natoms > 0 && m.natoms <= MAX_ATOMS, "model atoms out of range (1.." + std::to_string(MAX_ATOMS) + ")");
+		require(m.nframes > 0 && m.nframes <= MAX_FRAMES, "model frames out of range");
+		require(m.conf_size <= MAX_CONF && m.change_size <= MAX_CHANGE, "model conf/change size over 64");
+		require(m.ntrees > 0 && m.ntrees <= MAX_TREES, "model trees out of range");
+		ModelHdr h{};
+		h.natoms = m.natoms;
+		h.ninflex = m.ninflex;
+		h.nframes = m.nframes;
+		h.ntrees = m.ntrees;
+		h.nligands = m.nligands;
+		h.npairs = m.npairs;
+		h.nother = m.nother;
+		h.conf_size = m.conf_size;
+		h.change_size = m.change_size;
+		h.num_tors = m.num_tors;
+		const int na = m.natoms + m.ninflex;
+		h.atom_off = int(B.atom_type.size());
+		for (int i = 0; i < na; i++)
+		{
+			B.atom_type.push_back(m.atom_type[i]);
+			for (int d = 0; d < 3; d++)
+				B.atom_xyz0.push_back(m.atom_init[3 * i + d]);
+		}
+		h.mov_off = int(B.atom_frame.size());
+		for (int i = 0; i < m.natoms; i++)
+		{
+			require(m.atom_frame[i] >= 0 && m.atom_frame[i] < m.nframes, "atom frame out of range");
+			B.atom_frame.push_back(m.atom_frame[i]);
+			for (int d = 0; d < 3; d++)
+				B.atom_local.push_back(m.atom_local[3 * i + d]);
+		}
+		// pair lists per movable atom, in evaluation (pair) order
+		std::vector<std::vector<int>> lists(m.natoms);
+		for (int p = 0; p < m.npairs; p++)
+		{
+			const int a = m.pair_a[p], b = m.pair_b[p];
+			require(a >= 0 && a < m.natoms && b > a && b < na, "pair atoms out of range");
+			require(m.pair_tri[p] >= 0 && m.pair_tri[p] < SGPU_NUM_TYPE_PAIRS, "pair type index out of range");
+			lists[a].push_back(p << 1);
+			if (b < m.natoms)
+				lists[b].push_back((p << 1) | 1);
+		}
+		h.nbr_off = int(B.nbr_start.size());
+		h.code_off = int(B.nbr_code.size());
+		int k = 0;
+		for (int i = 0; i < m.natoms; i++)
+		{
+			B.nbr_start.push_back(k);
+			for (int c : lists[i])
+				B.nbr_code.push_back(c);
+			k += int(lists[i].size());
+		}
+		B.nbr_start.push_back(k);
+		h.pair_off = int(B.pair_a.size());
+		for (int p = 0; p < m.npairs; p++)
+		{
+			B.pair_a.push_back(m.pair_a[p]);
+			B.pair_b.push_back(m.pair_b[p]);
+			B.pair_tri.push_back(m.pair_tri[p]);
+		}
+		// frames, children, levels
+		h.frame_off = int(B.frame_parent.size());
+		h.child_off = int(B.child_list.size());
+		std::vector<int> depth(m.nframes, 0);
+		int nlevels = 0;
+		for (int f = 0; f < m.nframes; f++)
+		{
+			const int p = m.frame_parent[f];
+			require(p < f, "frames must be in preorder");
+			depth[f] = p < 0 ? 0 : depth[p] + 1;
+			nlevels = std::max(nlevels, depth[f] + 1);
+			require(m.frame_begin[f] >= 0 && m.frame_end[f] <= m.natoms && m.frame_begin[f] <= m.frame_end[f],
+					"frame atom range out of range");
+		}
+		for (int f = 0; f < m.nframes; f++)
+		{
+			B.frame_parent.push_back(m.frame_parent[f]);
+			B.frame_kind.push_back(m.frame_kind[f]);
+			B.frame_begin.push_back(m.frame_begin[f]);
+			B.frame_end.push_back(m.frame_end[f]);
+			B.frame_conf.push_back(m.frame_conf[f]);
+			B.frame_change.push_back(m.frame_change[f]);
+			B.child_start.push_back(int(B.child_list.size()) - h.child_off);
+			int cnt = 0;
+			for (int c = f + 1; c < m.nframes; c++)
+				if (m.frame_parent[c] == f)
+				{
+					B.child_list.push_back(c);
+					++cnt;
+				}
+			B.child_count.push_back(cnt);
+			for (int d = 0; d < 3; d++)
+			{
+				B.frame_origin.push_back(m.frame_origin[3 * f + d]);
+				B.frame_axis.push_back(m.frame_axis[3 * f + d]);
+			}
+		}
+		h.nlevels = nlevels;
+		h.level_off = int(B.level_start.size());
+		h.lvf_off = int(B.level_frames.size());
+		int cntl = 0;
+		for (int L = 0; L < nlevels; L++)
+		{
+			B.level_start.push_back(cntl);
+			for (int f = 0; f < m.nframes; f++)
+				if (depth[f] == L)
+				{
+					B.level_frames.push_back(f);
+					++cntl;
+				}
+		}
+		B.level_start.push_back(cntl);
+		// heavy atoms
+		h.heavy_off = int(B.heavy.size());
+		int nh = 0;
+		for (int i = 0; i < m.natoms; i++)
+			if (!is_hydrogen_type(m.atom_type[i]))
+			{
+				B.heavy.push_back(i);
+				++nh;
+			}
+		h.nheavy = nh;
+		// trees
+		h.tree_off = int(B.tree_root.size());
+		std::vector<int> troot(m.ntrees, -1);
+		for (int f = 0; f < m.nframes; f++)
+			if (m.frame_parent[f] < 0)
+			{
+				require(m.frame_tree[f] >= 0 && m.frame_tree[f] < m.ntrees, "frame tree out of range");
+				troot[m.frame_tree[f]] = f;
+			}
+		std::vector<int> tab(m.ntrees, m.natoms), tae(m.ntrees, 0), tn(m.ntrees, 0);
+		for (int f = 0; f < m.nframes; f++)
+		{
+			const int t = m.frame_tree[f];
+			tab[t] = std::min(tab[t], m.frame_begin[f]);
+			tae[t] = std::max(tae[t], m.frame_end[f]);
+			if (m.frame_kind[f] != SGPU_FRAME_RIGID_ROOT)
+				++tn[t];
+		}
+		std::vector<int> tconf(m.ntrees), tchg(m.ntrees);
+		for (int t = 0; t < m.ntrees; t++)
+		{
+			require(troot[t] >= 0, "tree without a root frame");
+			const int r = troot[t];
+			tconf[t] = m.frame_conf[r];
+			tchg[t] = m.frame_change[r];
+		}
+		for (int t = 0; t < m.ntrees; t++)
+		{
+			const bool lig = t < m.nligands;
+			B.tree_root.push_back(troot[t]);
+			B.tree_conf.push_back(tconf[t]);
+			B.tree_change.push_back(tchg[t]);
+			B.tree_ligand.push_back(lig ? 1 : 0);
+			B.tree_ntors.push_back(tn[t]);
+			B.tree_atom_begin.push_back(tab[t]);
+			B.tree_atom_end.push_back(tae[t]);
+			// the ligand's internal pairs: contiguous after the other pairs
+			int pb = m.npairs, pe = m.npairs;
+			if (lig)
+			{
+				pb = m.npairs;
+				pe = m.nother;
+				for (int p = m.nother; p < m.npairs; p++)
+					if (m.pair_a[p] >= tab[t] && m.pair_a[p] < tae[t])
+					{
+						pb = std::min(pb, p);
+						pe = std::max(pe, p + 1);
+					}
+				if (pb > pe)
+					pb = pe;
+			}
+			B.tree_pair_begin.push_back(pb);
+			B.tree_pair_end.push_back(pe);
+		}
+		// conf / change maps
+		h.map_off = int(B.maps.size());
+		std::vector<int> ck(m.conf_size, -1), cc(m.conf_size, 0), gk(m.change_size, -1), gc(m.change_size, 0);
+		for (int f = 0; f < m.nframes; f++)
+		{
+			const int ci = m.frame_conf[f], gi = m.frame_change[f];
+			if (m.frame_kind[f] == SGPU_FRAME_RIGID_ROOT)
+			{
+				require(ci + 7 <= m.conf_size && gi + 6 <= m.change_size, "root conf index out of range");
+				for (int d = 0; d < 3; d++)
+				{
+					ck[ci + d] = CK_POSITION;
+					cc[ci + d] = gi + d;
+					gk[gi + d] = GK_POSITION;
+					gc[gi + d] = ci + d;
+				}
+				for (int d = 0; d < 4; d++)
+				{
+					ck[ci + 3 + d] = CK_QUAT0 + d;
+					cc[ci + 3 + d] = gi + 3;
+				}
+				for (int d = 0; d < 3; d++)
+				{
+					gk[gi + 3 + d] = GK_ORIENT0 + d;
+					gc[gi + 3 + d] = ci + 3;
+				}
+			}
+			else
+			{
+				require(ci < m.conf_size && gi < m.change_size, "torsion index out of range");
+				ck[ci] = CK_TORSION;
+				cc[ci] = gi;
+				gk[gi] = GK_TORSION;
+				gc[gi] = ci;
+			}
+		}
+		for (int v : ck)
+			require(v >= 0, "conf element without a frame");
+		for (int v : gk)
+			require(v >= 0, "change element without a frame");
+		B.maps.insert(B.maps.end(), ck.begin(), ck.end());
+		B.maps.insert(B.maps.end(), cc.begin(), cc.end());
+		B.maps.insert(B.maps.end(), gk.begin(), gk.end());
+		B.maps.insert(B.maps.end(), gc.begin(), gc.end());
+		B.hdr.push_back(h);
+		nmax = std::max(nmax, m.natoms);
+		fmax = std::max(fmax, m.nframes);
+		cmax = std::max(cmax, m.conf_size);
+		gmax = std::max(gmax, m.change_size);
+		pmax = std::max(pmax, m.npairs);
+	}
+	WavePlan& p = B.plan;
+	p.nmax = std::max(WAVE, (nmax + WAVE - 1) / WAVE * WAVE);
+	p.fmax = fmax;
+	p.cmax = WAVE;
+	p.gmax = WAVE;
+	p.pmax = std::max(1, pmax);
+	p.hmax = gmax * (gmax + 1) / 2;
+	p.bytes = 0;
+}
+
+template<typename T>
+std::vector<T> cast(const std::vector<double>& v)
+{
+	return std::vector<T>(v.begin(), v.end());
+}
+
+void upload_batch(Batch& b, hipStream_t st)
+{
+	const HostBatch& h = b.h;
+	b.hdr.alloc(h.hdr.size() * sizeof(ModelHdr));
+	hip_check(hipMemcpyAsync(b.hdr.p, h.hdr.data(), h.hdr.size() * sizeof(ModelHdr), hipMemcpyHostToDevice, st), "hdr");
+	upload(b.atom_type, h.atom_type, st);
+	upload(b.atom_frame, h.atom_frame, st);
+	upload(b.nbr_start, h.nbr_start, st);
+	upload(b.nbr_code, h.nbr_code, st);
+	upload(b.frame_parent, h.frame_parent, st);
+	upload(b.frame_kind, h.frame_kind, st);
+	upload(b.frame_begin, h.frame_begin, st);
+	upload(b.frame_end, h.frame_end, st);
+	upload(b.frame_conf, h.frame_conf, st);
+	upload(b.frame_change, h.frame_change, st);
+	upload(b.child_start, h.child_start, st);
+	upload(b.child_count, h.child_count, st);
+	upload(b.child_list, h.child_list, st);
+	upload(b.level_start, h.level_start, st);
+	upload(b.level_frames, h.level_frames, st);
+	upload(b.pair_a, h.pair_a, st);
+	upload(b.pair_b, h.pair_b, st);
+	upload(b.pair_tri, h.pair_tri, st);
+	upload(b.heavy, h.heavy, st);
+	upload(b.tree_root, h.tree_root, st);
+	upload(b.tree_conf, h.tree_conf, st);
+	upload(b.tree_change, h.tree_change, st);
+	upload(b.tree_ligand, h.tree_ligand, st);
+	upload(b.tree_ntors, h.tree_ntors, st);
+	upload(b.tree_atom_begin, h.tree_atom_begin, st);
+	upload(b.tree_atom_end, h.tree_atom_end, st);
+	upload(b.tree_pair_begin, h.tree_pair_begin, st);
+	upload(b.tree_pair_end, h.tree_pair_end, st);
+	upload(b.maps, h.maps, st);
+	if (b.precision == SGPU_FP64)
+	{
+		upload(b.atom_xyz0, h.atom_xyz0, st);
+		upload(b.atom_local, h.atom_local, st);
+		upload(b.frame_origin, h.frame_origin, st);
+		upload(b.frame_axis, h.frame_axis, st);
+	}
+	else
+	{
+		upload(b.atom_xyz0, cast<float>(h.atom_xyz0), st);
+		upload(b.atom_local, cast<float>(h.atom_local), st);
+		upload(b.frame_origin, cast<float>(h.frame_origin), st);
+		upload(b.frame_axis, cast<float>(h.frame_axis), st);
+	}
+	hip_check(hipStreamSynchronize(st), "upload models");
+}
+
+template<typename T>
+BatchPtrs<T> ptrs(const Batch& b)
+{
+	BatchPtrs<T> p;
+	p.hdr = b.hdr.as<ModelHdr>();
+	p.atom_type = b.atom_type.as<int>();
+	p.atom_xyz0 = b.atom_xyz0.as<T>();
+	p.atom_frame = b.atom_frame.as<int>();
+	p.atom_local = b.atom_local.as<T>();
+	p.nbr_start = b.nbr_start.as<int>();
+	p.nbr_code = b.nbr_code.as<int>();
+	p.frame_parent = b.frame_parent.as<int>();
+	p.frame_kind = b.frame_kind.as<int>();
+	p.frame_begin = b.frame_begin.as<int>();
+	p.frame_end = b.frame_end.as<int>();
+	p.frame_conf = b.frame_conf.as<int>();
+	p.frame_change = b.frame_change.as<int>();
+	p.child_start = b.child_start.as<int>();
+	p.child_count = b.child_count.as<int>();
+	p.child_list = b.child_list.as<int>();
+	p.frame_origin = b.frame_origin.as<T>();
+	p.frame_axis = b.frame_axis.as<T>();
+	p.level_start = b.level_start.as<int>();
+	p.level_frames = b.level_frames.as<int>();
+	p.pair_a = b.pair_a.as<int>();
+	p.pair_b = b.pair_b.as<int>();
+	p.pair_tri = b.pair_tri.as<int>();
+	p.heavy = b.heavy.as<int>();
+	p.tree_root = b.tree_root.as<int>();
+	p.tree_conf = b.tree_conf.as<int>();
+	p.tree_change = b.tree_change.as<int>();
+	p.tree_ligand = b.tree_ligand.as<int>();
+	p.tree_ntors = b.tree_ntors.as<int>();
+	p.tree_atom_begin = b.tree_atom_begin.as<int>();
+	p.tree_atom_end = b.tree_atom_end.as<int>();
+	p.tree_pair_begin = b.tree_pair_begin.as<int>();
+	p.tree_pair_end = b.tree_pair_end.as<int>();
+	p.maps = b.maps.as<int>();
+	return p;
+}
+
+int plan_bytes(const WavePlan& p, int prec)
+{
+	const int elems = prec == SGPU_FP64 ? wave_lds_elems<double>(p) : wave_lds_elems<float>(p);
+	return elems * (prec == SGPU_FP64 ? 8 : 4);
+}
+
+template<typename T>
+TablePtrs<T> table_ptrs(const sgpu_ctx* c)
+{
+	TablePtrs<T> t;
+	t.smooth = (sizeof(T) == 8 ? c->tab.smooth64 : c->tab.smooth32).template as<T>();
+	t.fast = c->tab.fast.as<double>();
+	t.n = c->tab.n;
+	t.factor = c->tab.factor;
+	t.cutoff_sqr = c->tab.cutoff_sqr;
+	return t;
+}
+
+template<typename T>
+GridPtrs<T> grid_ptrs(const Grid& g, double slope)
+{
+	GridPtrs<T> p;
+	p.cells = g.cells.as<T>();
+	p.slot_of = g.slot_dev.as<int>();
+	for (int d = 0; d < 3; d++)
+	{
+		p.dim[d] = g.dim[d];
+		p.init[d] = g.init[d];
+		p.factor[d] = g.factor[d];
+		p.factor_inv[d] = g.factor_inv[d];
+		p.dim_fl_minus_1[d] = g.dim_fl_minus_1[d];
+	}
+	p.ncells = (g.dim[0] - 1) * (g.dim[1] - 1) * (g.dim[2] - 1);
+	p.slope = slope;
+	return p;
+}
+
+// grid::init (src/lib/grid.cpp:60-75)
+void grid_geometry(const sgpu_grid_dims& gd, int* dim, double* init, double* factor, double* finv, double* dm1)
+{
+	for (int i = 0; i < 3; i++)
+	{
+		dim[i] = gd.n[i] + 1;
+		init[i] = gd.begin[i];
+		const double range = gd.end[i] - gd.begin[i];
+		dm1[i] = dim[i] - 1.0;
+		factor[i] = dm1[i] / range;
+		finv[i] = 1 / factor[i];
+	}
+}
+
+void check_gd(const sgpu_grid_dims* gd)
+{
+	require(gd != nullptr, "grid dims missing");
+	for (int i = 0; i < 3; i++)
+	{
+		require(gd->n[i] >= 1 && gd->n[i] <= 4096, "grid intervals out of range");
+		require(gd->end[i] > gd->begin[i], "grid box end must exceed begin");
+	}
+}
+
+// cells of the direct back end over box gd: full 3 A bricks
+Cells& receptor_cells(sgpu_ctx* c, Receptor& r, const sgpu_grid_dims& gd)
+{
+	std::vector<double> key;
+	for (int i = 0; i < 3; i++)
+	{
+		key.push_back(gd.begin[i]);
+		key.push_back(gd.end[i]);
+		key.push_back(gd.n[i]);
+	}
+	auto it = r.cells.find(key);
+	if (it != r.cells.end())
+		return *it->second;
+	std::unique_ptr<Cells> cl(new Cells);
+	GridBuildArgs a{};
+	a.rec_xyz = r.xyz64.as<double>();
+	a.rec_type = r.typ.as<int>();
+	a.rec_n = r.n;
+	a.cutoff_sqr = c->tab.cutoff_sqr;
+	std::vector<double> lo, hi;
+	for (int d = 0; d < 3; d++)
+	{
+		cl->off[d] = int(std::floor(gd.begin[d] / 3.0));
+		cl->dim[d] = int(std::ceil(gd.end[d] / 3.0)) - cl->off[d] + 1;
+		cl->begin[d] = gd.begin[d];
+		cl->end[d] = gd.end[d];
+		cl->enabled[d] = gd.n[d] > 0;
+		a.cell_off[d] = cl->off[d];
+		a.cell_dim[d] = cl->dim[d];
+		a.box_lo[d] = gd.begin[d];
+		a.box_hi[d] = gd.end[d];
+	}
+	for (int d = 0; d < 3; d++)
+		for (int i = 0; i < cl->dim[d]; i++)
+		{
+			lo.push_back((cl->off[d] + i) * 3.0);
+			hi.push_back((cl->off[d] + i + 1) * 3.0);
+		}
+	DBuf dlo, dhi;
+	upload(dlo, lo, c->stream);
+	upload(dhi, hi, c->stream);
+	a.cell_lo = dlo.as<double>();
+	a.cell_hi = dhi.as<double>();
+	const int ncells = cl->dim[0] * cl->dim[1] * cl->dim[2];
+	cl->start.alloc(size_t(ncells + 1) * sizeof(int));
+	a.cell_start = cl->start.as<int>();
+	hip_check(launch_cell_count(a, c->stream), "cell count");
+	int total = 0;
+	hip_check(hipMemcpyAsync(&total, a.cell_start + ncells, sizeof(int), hipMemcpyDeviceToHost, c->stream), "cells");
+	hip_check(hipStreamSynchronize(c->stream), "cells");
+	cl->atoms.alloc(size_t(std::max(1, total)) * sizeof(int));
+	a.cell_atoms = cl->atoms.as<int>();
+	hip_check(launch_cell_fill(a, c->stream), "cell fill");
+	hip_check(hipStreamSynchronize(c->stream), "cells");
+	Cells& ref = *cl;
+	r.cells[key] = std::move(cl);
+	return ref;
+}
+
+template<typename T>
+ReceptorPtrs<T> receptor_ptrs(const Receptor& r, const Cells& cl)
+{
+	ReceptorPtrs<T> p;
+	p.xyz = (sizeof(T) == 8 ? r.xyz64 : r.xyz32).template as<T>();
+	p.type = r.typ.as<int>();
+	p.natoms = r.n;
+	p.cell_start = cl.start.as<int>();
+	p.cell_atoms = cl.atoms.as<int>();
+	for (int d = 0; d < 3; d++)
+	{
+		p.cell_off[d] = cl.off[d];
+		p.cell_dim[d] = cl.dim[d];
+		p.begin[d] = cl.begin[d];
+		p.end[d] = cl.end[d];
+		p.enabled[d] = cl.enabled[d];
+	}
+	return p;
+}
+
+template<typename F>
+sgpu_status guarded(sgpu_ctx* c, F&& f)
+{
+	if (!c)
+		return SGPU_E_INVALID;
+	std::lock_guard<std::mutex> lock(c->mu);
+	try
+	{
+		hip_check(hipSetDevice(c->device), "hipSetDevice");
+		f();
+		return SGPU_OK;
+	}
+	catch (Fail& e)
+	{
+		c->err = e.msg;
+		return e.code;
+	}
+	catch (std::exception& e)
+	{
+		c->err = e.what();
+		return SGPU_E_INVALID;
+	}
+}
+
+struct Timer
+{
+	sgpu_ctx* c;
+	const char* kind;
+	Timer(sgpu_ctx* c_, const char* k) : c(c_), kind(k) { hip_check(hipEventRecord(c->ev0, c->stream), "event"); }
+	void stop()
+	{
+		hip_check(hipEventRecord(c->ev1, c->stream), "event");
+		hip_check(hipEventSynchronize(c->ev1), "event");
+		float ms = 0;
+		hip_check(hipEventElapsedTime(&ms, c->ev0, c->ev1), "event");
+		c->last_ms[kind] = ms;
+	}
+};
+
+Batch& find_batch(sgpu_ctx* c, int id)
+{
+	auto it = c->batches.find(id);
+	if (it == c->batches.end())
+		throw Fail{SGPU_E_NOT_FOUND, "unknown model batch id"};
+	return *it->second;
+}
+Receptor& find_receptor(sgpu_ctx* c, int id)
+{
+	auto it = c->receptors.find(id);
+	if (it == c->receptors.end())
+		throw Fail{SGPU_E_NOT_FOUND, "unknown receptor id"};
+	return *it->second;
+}
+Grid& find_grid(sgpu_ctx* c, int id)
+{
+	auto it = c->grids.find(id);
+	if (it == c->grids.end())
+		throw Fail{SGPU_E_NOT_FOUND, "unknown grid id"};
+	return *it->second;
+}
+
+void ensure_scratch(DBuf& b, size_t bytes)
+{
+	if (b.n < bytes)
+		b.alloc(bytes);
+}
+
+} // namespace
+
+extern "C" {
+
+sgpu_status sgpu_create(int device, sgpu_ctx** out)
+{
+	if (!out)
+		return SGPU_E_INVALID;
+	*out = nullptr;
+	int n = 0;
+	if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+		return SGPU_E_NO_DEVICE;
+	if (device < 0 || device >= n)
+		return SGPU_E_NO_DEVICE;
+	std::unique_ptr<sgpu_ctx> c(new sgpu_ctx);
+	c->device = device;
+	if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+			hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
+		return SGPU_E_HIP;
+	*out = c.release();
+	return SGPU_OK;
+}
+
+void sgpu_destroy(sgpu_ctx* c)
+{
+	if (!c)
+		return;
+	(void)hipSetDevice(c->device);
+	(void)hipStreamSynchronize(c->stream);
+	c->batches.clear();
+	c->grids.clear();
+	c->receptors.clear();
+	(void)hipEventDestroy(c->ev0);
+	(void)hipEventDestroy(c->ev1);
+	(void)hipStreamDestroy(c->stream);
+	delete c;
+}
+
+const char* sgpu_last_error(const sgpu_ctx* c) { return c ? c->err.c_str() : "no context"; }
+
+sgpu_status sgpu_device_info(sgpu_ctx* c, char* name, int name_len, int* cus)
+{
+	return guarded(c, [&]() {
+		hipDeviceProp_t p;
+		hip_check(hipGetDeviceProperties(&p, c->device), "hipGetDeviceProperties");
+		if (name && name_len > 0)
+		{
+			std::snprintf(name, size_t(name_len), "%s (%s)", p.name, p.gcnArchName);
+		}
+		if (cus)
+			*cus = p.multiProcessorCount;
+	});
+}
+
+sgpu_status sgpu_upload_tables(sgpu_ctx* c, const sgpu_tables* t)
+{
+	return guarded(c, [&]() {
+		require(t && t->smooth && t->fast && t->n >= 4 && t->factor > 0, "invalid tables");
+		require(size_t(t->cutoff_sqr * t->factor) + 1 < size_t(t->n), "table too short for the cutoff");
+		const size_t ne = size_t(SGPU_NUM_TYPE_PAIRS) * t->n;
+		std::vector<double> sm(t->smooth, t->smooth + 2 * ne), fa(t->fast, t->fast + ne);
+		upload(c->tab.smooth64, sm, c->stream);
+		upload(c->tab.smooth32, cast<float>(sm), c->stream);
+		upload(c->tab.fast, fa, c->stream);
+		c->tab.n = t->n;
+		c->tab.factor = t->factor;
+		c->tab.cutoff_sqr = t->cutoff_sqr;
+		c->tab.ok = true;
+		hip_check(hipStreamSynchronize(c->stream), "upload tables");
+	});
+}
+
+sgpu_status sgpu_upload_receptor(sgpu_ctx* c, const sgpu_receptor* r, int* id)
+{
+	return guarded(c, [&]() {
+		require(r && id && r->natoms > 0 && r->xyz && r->type, "invalid receptor");
+		std::unique_ptr<Receptor> R(new Receptor);
+		R->n = r->natoms;
+		R->xyz.assign(r->xyz, r->xyz + 3 * size_t(r->natoms));
+		R->type.assign(r->type, r->type + r->natoms);
+		for (int t : R->type)
+			require(t >= 0 && t < SGPU_NUM_TYPES, "receptor atom type out of range");
+		upload(R->xyz64, R->xyz, c->stream);
+		upload(R->xyz32, cast<float>(R->xyz), c->stream);
+		upload(R->typ, R->type, c->stream);
+		hip_check(hipStreamSynchronize(c->stream), "upload receptor");
+		*id = c->next_id++;
+		c->receptors[*id] = std::move(R);
+	});
+}
+
+sgpu_status sgpu_build_grids(sgpu_ctx* c, int receptor_id, const sgpu_grid_dims* gd, const int* types, int ntypes,
+		int precision, int* grid_id)
+{
+	return guarded(c, [&]() {
+		require(c->tab.ok, "tables not uploaded");
+		check_gd(gd);
+		require(grid_id && types && ntypes > 0 && ntypes <= 16, "1..16 map types");
+		require(precision == SGPU_FP32 || precision == SGPU_FP64, "invalid precision");
+		Receptor& r = find_receptor(c, receptor_id);
+		std::unique_ptr<Grid> g(new Grid);
+		g->receptor = receptor_id;
+		g->precision = precision;
+		g->gd = *gd;
+		grid_geometry(*gd, g->dim, g->init, g->factor, g->factor_inv, g->dim_fl_minus_1);
+		std::fill(g->slot_of, g->slot_of + NUM_ATOM_TYPES, -1);
+		for (int j = 0; j < ntypes; j++)
+		{
+			require(types[j] >= 0 && types[j] < NUM_ATOM_TYPES, "map type out of range");
+			require(g->slot_of[types[j]] < 0, "duplicate map type");
+			g->slot_of[types[j]] = j;
+			g->types.push_back(types[j]);
+		}
+		// cells with smina's first-probe bricks, per dimension
+		GridBuildArgs a{};
+		a.rec_xyz = r.xyz64.as<double>();
+		a.rec_type = r.typ.as<int>();
+		a.rec_n = r.n;
+		a.fast = c->tab.fast.as<double>();
+		a.table_n = c->tab.n;
+		a.factor = c->tab.factor;
+		a.cutoff_sqr = c->tab.cutoff_sqr;
+		std::vector<double> lo, hi;
+		for (int d = 0; d < 3; d++)
+		{
+			a.cell_off[d] = int(std::floor(gd->begin[d] / 3.0));
+			a.cell_dim[d] = int(std::ceil(gd->end[d] / 3.0)) - a.cell_off[d] + 1;
+			a.box_lo[d] = gd->begin[d];
+			a.box_hi[d] = gd->end[d];
+			a.dim[d] = g->dim[d];
+			a.init[d] = g->init[d];
+			a.factor_inv[d] = g->factor_inv[d];
+		}
+		for (int d = 0; d < 3; d++)
+		{
+			std::vector<double> l(a.cell_dim[d], 0), h(a.cell_dim[d], 0);
+			std::vector<char> seen(a.cell_dim[d], 0);
+			for (int i = 0; i <= gd->n[d]; i++)
+			{
+				const double cval = g->init[d] + g->factor_inv[d] * i;
+				const int cell = int(std::floor(cval / 3.0)) - a.cell_off[d];
+				require(cell >= 0 && cell < a.cell_dim[d], "probe outside the cell range");
+				if (!seen[cell])
+				{
+					seen[cell] = 1;
+					l[cell] = std::floor(cval / 3.0) * 3.0;
+					h[cell] = std::ceil(cval / 3.0) * 3.0;
+				}
+			}
+			lo.insert(lo.end(), l.begin(), l.end());
+			hi.insert(hi.end(), h.begin(), h.end());
+		}
+		DBuf dlo, dhi, dstart, datoms, dtypes;
+		upload(dlo, lo, c->stream);
+		upload(dhi, hi, c->stream);
+		upload(dtypes, g->types, c->stream);
+		a.cell_lo = dlo.as<double>();
+		a.cell_hi = dhi.as<double>();
+		a.types = dtypes.as<int>();
+		a.ntypes = ntypes;
+		const int ncells = a.cell_dim[0] * a.cell_dim[1] * a.cell_dim[2];
+		dstart.alloc(size_t(ncells + 1) * sizeof(int));
+		a.cell_start = dstart.as<int>();
+		Timer tm(c, "grid_build");
+		hip_check(launch_cell_count(a, c->stream), "cell count");
+		int total = 0;
+		hip_check(hipMemcpyAsync(&total, a.cell_start + ncells, sizeof(int), hipMemcpyDeviceToHost, c->stream), "cells");
+		hip_check(hipStreamSynchronize(c->stream), "cells");
+		datoms.alloc(size_t(std::max(1, total)) * sizeof(int));
+		a.cell_atoms = datoms.as<int>();
+		hip_check(launch_cell_fill(a, c->stream), "cell fill");
+		const size_t npts = size_t(g->dim[0]) * g->dim[1] * g->dim[2];
+		g->points.alloc(npts * ntypes * sizeof(double));
+		hip_check(launch_grid_points(a, g->points.as<double>(), c->stream), "grid points");
+		const size_t ncell = size_t(g->dim[0] - 1) * (g->dim[1] - 1) * (g->dim[2] - 1) * ntypes * 8;
+		if (precision == SGPU_FP64)
+		{
+			g->cells.alloc(ncell * sizeof(double));
+			hip_check(launch_grid_pack<double>(g->points.as<double>(), ntypes, g->dim, g->cells.as<double>(), c->stream),
+					"grid pack");
+		}
+		else
+		{
+			g->cells.alloc(ncell * sizeof(float));
+			hip_check(launch_grid_pack<float>(g->points.as<double>(), ntypes, g->dim, g->cells.as<float>(), c->stream),
+					"grid pack");
+		}
+		std::vector<int> slots(g->slot_of, g->slot_of + NUM_ATOM_TYPES);
+		upload(g->slot_dev, slots, c->stream);
+		tm.stop();
+		hip_check(hipStreamSynchronize(c->stream), "build grids");
+		*grid_id = c->next_id++;
+		c->grids[*grid_id] = std::move(g);
+	});
+}
+
+sgpu_status sgpu_download_grid(sgpu_ctx* c, int grid_id, int type, double* out)
+{
+	return guarded(c, [&]() {
+		Grid& g = find_grid(c, grid_id);
+		require(out && type >= 0 && type < NUM_ATOM_TYPES && g.slot_of[type] >= 0, "type has no map");
+		const size_t npts = size_t(g.dim[0]) * g.dim[1] * g.dim[2];
+		std::vector<double> pts(npts);
+		hip_check(hipMemcpyAsync(pts.data(), g.points.as<double>() + size_t(g.slot_of[type]) * npts, npts * sizeof(double),
+						  hipMemcpyDeviceToHost, c->stream),
+				"download grid");
+		hip_check(hipStreamSynchronize(c->stream), "download grid");
+		// stored precision: round through it
+		for (size_t k = 0; k < npts; k++)
+			out[k] = g.precision == SGPU_FP64 ? pts[k] : double(float(pts[k]));
+	});
+}
+
+sgpu_status sgpu_release_grid(sgpu_ctx* c, int grid_id)
+{
+	return guarded(c, [&]() {
+		find_grid(c, grid_id);
+		c->grids.erase(grid_id);
+	});
+}
+
+sgpu_status sgpu_upload_models(sgpu_ctx* c, const sgpu_flat_model* models, int n, int precision, int* batch_id)
+{
+	return guarded(c, [&]() {
+		require(models && n > 0 && batch_id, "no models");
+		require(precision == SGPU_FP32 || precision == SGPU_FP64, "invalid precision");
+		std::unique_ptr<Batch> b(new Batch);
+		b->precision = precision;
+		b->n = n;
+		pack_batch(models, n, b->h);
+		b->h.plan.bytes = plan_bytes(b->h.plan, precision);
+		require(b->h.plan.bytes * WAVES_PER_BLOCK <= 160 * 1024, "model too large for LDS");
+		upload_batch(*b, c->stream);
+		if (precision == SGPU_FP32)
+		{
+			b->fp64.reset(new Batch);
+			b->fp64->precision = SGPU_FP64;
+			b->fp64->n = n;
+			b->fp64->h = b->h;
+			b->fp64->h.plan.bytes = plan_bytes(b->h.plan, SGPU_FP64);
+			upload_batch(*b->fp64, c->stream);
+		}
+		*batch_id = c->next_id++;
+		c->batches[*batch_id] = std::move(b);
+	});
+}
+
+sgpu_status sgpu_release_models(sgpu_ctx* c, int batch_id)
+{
+	return guarded(c, [&]() {
+		find_batch(c, batch_id);
+		c->batches.erase(batch_id);
+	});
+}
+
+} // extern "C"
+
+namespace
+{
+
+template<typename T>
+void run_eval(sgpu_ctx* c, const sgpu_eval_request* q)
+{
+	Batch& b = find_batch(c, q->batch_id);
+	const int n = q->nconfs;
+	require(n > 0 && q->model && q->conf && q->e, "eval request incomplete");
+	require(q->conf_stride >= 1, "conf stride");
+	for (int i = 0; i < n; i++)
+	{
+		require(q->model[i] >= 0 && q->model[i] < b.n, "model index out of range");
+		require(q->conf_stride >= b.h.hdr[q->model[i]].conf_size, "conf stride below conf size");
+		if (q->g)
+			require(q->change_stride >= b.h.hdr[q->model[i]].change_size, "change stride below change size");
+		if (q->coords)
+			require(q->coord_stride >= 3 * b.h.hdr[q->model[i]].natoms, "coord stride below 3 * atoms");
+	}
+	EvalCtx<T> ec{};
+	ec.tab = table_ptrs<T>(c);
+	for (int d = 0; d < 3; d++)
+		ec.v[d] = T(q->v[d]);
+	ensure_scratch(c->err_flag, sizeof(int));
+	hip_check(hipMemsetAsync(c->err_flag.p, 0, sizeof(int), c->stream), "memset");
+	ec.error = c->err_flag.as<int>();
+	if (q->grid_id >= 0)
+	{
+		Grid& g = find_grid(c, q->grid_id);
+		require(g.precision == b.precision, "grid and model precision differ");
+		ec.backend = 0;
+		ec.grid = grid_ptrs<T>(g, q->slope);
+		ec.slope = T(q->slope);
+		for (int i = 0; i < n; i++)
+		{
+			const int mi = q->model[i];
+			const ModelHdr& h = b.h.hdr[mi];
+			for (int a = 0; a < h.natoms; a++)
+			{
+				const int t = b.h.atom_type[h.atom_off + a];
+				require(is_hydrogen_type(t) || g.slot_of[t] >= 0, "a ligand atom type has no map");
+			}
+		}
+	}
+	else
+	{
+		check_gd(q->gd);
+		Receptor& r = find_receptor(c, q->receptor_id);
+		Cells& cl = receptor_cells(c, r, *q->gd);
+		ec.backend = 1;
+		ec.rec = receptor_ptrs<T>(r, cl);
+		ec.slope = T(q->slope);
+	}
+	const int cs = b.h.plan.cmax;
+	std::vector<T> conf(size_t(n) * cs, T(0));
+	for (int i = 0; i < n; i++)
+		for (int k = 0; k < b.h.hdr[q->model[i]].conf_size; k++)
+			conf[size_t(i) * cs + k] = T(q->conf[size_t(i) * q->conf_stride + k]);
+	const int gs = b.h.plan.gmax, xs = 3 * b.h.plan.nmax, ps = b.h.plan.pmax;
+	DBuf dm, dc, de, dg, dx;
+	upload(dm, std::vector<int>(q->model, q->model + n), c->stream);
+	upload(dc, conf, c->stream);
+	de.alloc(size_t(n) * sizeof(T));
+	dg.alloc(size_t(n) * gs * sizeof(T));
+	dx.alloc(size_t(n) * xs * sizeof(T));
+	ensure_scratch(c->scratch, size_t(n) * ps * sizeof(T));
+	Timer tm(c, "eval");
+	hip_check(launch_eval<T>(ptrs<T>(b), ec, b.h.plan, n, dm.as<int>(), dc.as<T>(), cs, de.as<T>(), dg.as<T>(), gs,
+					  dx.as<T>(), xs, c->scratch.as<T>(), ps, c->stream),
+			"eval kernel");
+	tm.stop();
+	std::vector<T> e(n), g(size_t(n) * gs), x(size_t(n) * xs);
+	int flag = 0;
+	hip_check(hipMemcpyAsync(e.data(), de.p, e.size() * sizeof(T), hipMemcpyDeviceToHost, c->stream), "download");
+	hip_check(hipMemcpyAsync(g.data(), dg.p, g.size() * sizeof(T), hipMemcpyDeviceToHost, c->stream), "download");
+	hip_check(hipMemcpyAsync(x.data(), dx.p, x.size() * sizeof(T), hipMemcpyDeviceToHost, c->stream), "download");
+	hip_check(hipMemcpyAsync(&flag, c->err_flag.p, sizeof(int), hipMemcpyDeviceToHost, c->stream), "download");
+	hip_check(hipStreamSynchronize(c->stream), "eval");
+	for (int i = 0; i < n; i++)
+	{
+		const ModelHdr& h = b.h.hdr[q->model[i]];
+		q->e[i] = double(e[i]);
+		if (q->g)
+			for (int k = 0; k < h.change_size; k++)
+				q->g[size_t(i) * q->change_stride + k] = double(g[size_t(i) * gs + k]);
+		if (q->coords)
+			for (int k = 0; k < 3 * h.natoms; k++)
+				q->coords[size_t(i) * q->coord_stride + k] = double(x[size_t(i) * xs + k]);
+	}
+	if (flag)
+		throw Fail{SGPU_E_ATOM_OVERLAP, "Ligand atom exactly overlaps receptor atom.  I can't deal with this."};
+}
+
+template<typename T>
+void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, const sgpu_chains* ch,
+		sgpu_chain_results* out)
+{
+	Batch& b = find_batch(c, batch_id);
+	Grid& g = find_grid(c, grid_id);
+	require(g.precision == b.precision, "grid and model precision differ");
+	require(p && ch && out && ch->nchains > 0, "mc request incomplete");
+	require(p->num_saved_mins >= 1 && p->num_saved_mins <= MAX_POSES, "num_saved_mins out of range");
+	require(out->max_poses >= p->num_saved_mins, "result arrays hold fewer poses than num_saved_mins");
+	const int n = ch->nchains;
+	for (int i = 0; i < n; i++)
+	{
+		require(ch->model[i] >= 0 && ch->model[i] < b.n, "chain model out of range");
+		const ModelHdr& h = b.h.hdr[ch->model[i]];
+		require(ch->conf_stride >= h.conf_size && out->conf_stride >= h.conf_size, "conf stride");
+		require(out->coord_stride >= 3 * h.nheavy, "coord stride");
+		require(ch->num_steps[i] >= 0 && ch->maxiters[i] >= 0, "steps / maxiters");
+		require(ch->rng[size_t(i) * 625 + 624] <= 624, "rng position");
+		for (int a = 0; a < h.natoms; a++)
+		{
+			const int t = b.h.atom_type[h.atom_off + a];
+			require(is_hydrogen_type(t) || g.slot_of[t] >= 0, "a ligand atom type has no map");
+		}
+	}
+	ChainLayout L{};
+	L.conf_stride = b.h.plan.cmax;
+	int hmax = 1;
+	for (const ModelHdr& h : b.h.hdr)
+		hmax = std::max(hmax, h.nheavy);
+	L.coord_stride = 3 * hmax;
+	L.atom_stride = 3 * b.h.plan.nmax;
+	L.pair_stride = b.h.plan.pmax;
+	L.max_poses = MAX_POSES;
+	size_t off = 0;
+	auto take = [&](size_t bytes) {
+		off = (off + 63) / 64 * 64;
+		size_t o = off;
+		off += bytes;
+		return o;
+	};
+	L.off_mt = take(MT_N * sizeof(uint32_t));
+	L.off_ints = take((3 + MAX_POSES) * sizeof(int));
+	L.off_tmp = take(L.conf_stride * sizeof(T));
+	L.off_scal = take(2 * sizeof(T));
+	L.off_oute = take(MAX_POSES * sizeof(T));
+	L.off_outc = take(size_t(MAX_POSES) * L.conf_stride * sizeof(T));
+	L.off_outx = take(size_t(MAX_POSES) * L.coord_stride * sizeof(T));
+	L.off_mc = take(L.atom_stride * sizeof(T));
+	L.off_ro = take(MAX_TREES * 3 * sizeof(T));
+	L.off_pbuf = take(size_t(L.pair_stride) * sizeof(T));
+	L.off_ev = take(sizeof(int64_t));
+	L.bytes = (off + 63) / 64 * 64;
+	// initial chain state
+	std::vector<char> init(L.bytes * n, 0);
+	for (int i = 0; i < n; i++)
+	{
+		char* base = init.data() + L.bytes * i;
+		std::memcpy(base + L.off_mt, ch->rng + size_t(i) * 625, MT_N * sizeof(uint32_t));
+		int* ints = reinterpret_cast<int*>(base + L.off_ints);
+		ints[0] = int(ch->rng[size_t(i) * 625 + 624]);
+		ints[1] = 0;
+		ints[2] = 0;
+		T* tmp = reinterpret_cast<T*>(base + L.off_tmp);
+		for (int k = 0; k < b.h.hdr[ch->model[i]].conf_size; k++)
+			tmp[k] = T(ch->conf[size_t(i) * ch->conf_stride + k]);
+		T* scal = reinterpret_cast<T*>(base + L.off_scal);
+		scal[0] = 0;
+		scal[1] = Real_host_max<T>();
+	}
+	DBuf dchains, dmodel, dsteps, dmax;
+	upload(dchains, init, c->stream);
+	upload(dmodel, std::vector<int>(ch->model, ch->model + n), c->stream);
+	upload(dsteps, std::vector<int>(ch->num_steps, ch->num_steps + n), c->stream);
+	upload(dmax, std::vector<int>(ch->maxiters, ch->maxiters + n), c->stream);
+	EvalCtx<T> ec{};
+	ec.tab = table_ptrs<T>(c);
+	ec.backend = 0;
+	ec.grid = grid_ptrs<T>(g, p->cache_slope);
+	ec.slope = T(p->cache_slope);
+	McParams P{};
+	P.temperature = p->temperature;
+	P.amplitude = p->mutation_amplitude;
+	P.min_rmsd = p->min_rmsd;
+	P.num_saved = p->num_saved_mins;
+	for (int d = 0; d < 3; d++)
+	{
+		P.hunt[d] = p->hunt_cap[d];
+		P.authentic[d] = p->authentic_v[d];
+	}
+	P.accurate = p->accurate_line_search;
+	P.early_term = p->early_term;
+	int max_steps = 0;
+	for (int i = 0; i < n; i++)
+		max_steps = std::max(max_steps, ch->num_steps[i]);
+	Timer tm(c, "mc");
+	hip_check(launch_mc<T>(ptrs<T>(b), ec, b.h.plan, P, L, dchains.as<char>(), n, dmodel.as<int>(), dsteps.as<int>(),
+					  dmax.as<int>(), 0, max_steps, c->stream),
+			"mc kernel");
+	tm.stop();
+	hip_check(hipMemcpyAsync(init.data(), dchains.p, init.size(), hipMemcpyDeviceToHost, c->stream), "download");
+	hip_check(hipStreamSynchronize(c->stream), "mc");
+	for (int i = 0; i < n; i++)
+	{
+		const char* base = init.data() + L.bytes * i;
+		const ModelHdr& h = b.h.hdr[ch->model[i]];
+		const int* ints = reinterpret_cast<const int*>(base + L.off_ints);
+		const int cnt = std::min(ints[1], out->max_poses);
+		const T* oe = reinterpret_cast<const T*>(base + L.off_oute);
+		const T* oc = reinterpret_cast<const T*>(base + L.off_outc);
+		const T* ox = reinterpret_cast<const T*>(base + L.off_outx);
+		out->count[i] = cnt;
+		for (int k = 0; k < cnt; k++)
+		{
+			const int slot = ints[3 + k];
+			out->e[size_t(i) * out->max_poses + k] = double(oe[slot]);
+			for (int j = 0; j < h.conf_size; j++)
+				out->conf[(size_t(i) * out->max_poses + k) * out->conf_stride + j] = double(oc[size_t(slot) * L.conf_stride + j]);
+			for (int j = 0; j < 3 * h.nheavy; j++)
+				out->coords[(size_t(i) * out->max_poses + k) * out->coord_stride + j] =
+						double(ox[size_t(slot) * L.coord_stride + j]);
+		}
+		if (out->evals)
+			out->evals[i] = *reinterpret_cast<const int64_t*>(base + L.off_ev);
+	}
+}
+
+template<typename T>
+void run_refine(sgpu_ctx* c, const sgpu_refine_request* q)
+{
+	Batch& b = find_batch(c, q->batch_id);
+	Receptor& r = find_receptor(c, q->receptor_id);
+	check_gd(q->gd);
+	const int n = q->nposes;
+	require(n > 0 && q->model && q->conf && q->maxiters && q->e, "refine request incomplete");
+	for (int i = 0; i < n; i++)
+	{
+		require(q->model[i] >= 0 && q->model[i] < b.n, "pose model out of range");
+		require(q->conf_stride >= b.h.hdr[q->model[i]].conf_size, "conf stride");
+		if (q->coords)
+			require(q->coord_stride >= 3 * b.h.hdr[q->model[i]].nheavy, "coord stride");
+	}
+	Cells& cl = receptor_cells(c, r, *q->gd);
+	EvalCtx<T> ec{};
+	ec.tab = table_ptrs<T>(c);
+	ec.backend = 1;
+	ec.rec = receptor_ptrs<T>(r, cl);
+	for (int d = 0; d < 3; d++)
+		ec.v[d] = T(q->v[d]);
+	ensure_scratch(c->err_flag, sizeof(int));
+	hip_check(hipMemsetAsync(c->err_flag.p, 0, sizeof(int), c->stream), "memset");
+	ec.error = c->err_flag.as<int>();
+	const int cs = b.h.plan.cmax, ps = b.h.plan.pmax;
+	int hmax = 1;
+	for (const ModelHdr& h : b.h.hdr)
+		hmax = std::max(hmax, h.nheavy);
+	const int xs = 3 * hmax;
+	std::vector<T> conf(size_t(n) * cs, T(0));
+	for (int i = 0; i < n; i++)
+		for (int k = 0; k < b.h.hdr[q->model[i]].conf_size; k++)
+			conf[size_t(i) * cs + k] = T(q->conf[size_t(i) * q->conf_stride + k]);
+	DBuf dm, dc, dmi, de, dx, dev;
+	upload(dm, std::vector<int>(q->model, q->model + n), c->stream);
+	upload(dc, conf, c->stream);
+	upload(dmi, std::vector<int>(q->maxiters, q->maxiters + n), c->stream);
+	de.alloc(size_t(n) * sizeof(T));
+	dx.alloc(size_t(n) * xs * sizeof(T));
+	dev.alloc(size_t(n) * sizeof(int64_t));
+	ensure_scratch(c->scratch, size_t(n) * ps * sizeof(T));
+	Timer tm(c, "refine");
+	hip_check(launch_refine<T>(ptrs<T>(b), ec, b.h.plan, n, dm.as<int>(), dc.as<T>(), cs, dmi.as<int>(),
+					  q->accurate_line_search, q->early_term, de.as<T>(), dx.as<T>(), xs, c->scratch.as<T>(), ps,
+					  dev.as<int64_t>(), c->stream),
+			"refine kernel");
+	tm.stop();
+	std::vector<T> e(n), x(size_t(n) * xs);
+	int flag = 0;
+	hip_check(hipMemcpyAsync(conf.data(), dc.p, conf.size() * sizeof(T), hipMemcpyDeviceToHost, c->stream), "download");
+	hip_check(hipMemcpyAsync(e.data(), de.p, e.size() * sizeof(T), hipMemcpyDeviceToHost, c->stream), "download");
+	hip_check(hipMemcpyAsync(x.data(), dx.p, x.size() * sizeof(T), hipMemcpyDeviceToHost, c->stream), "download");
+	hip_check(hipMemcpyAsync(&flag, c->err_flag.p, sizeof(int), hipMemcpyDeviceToHost, c->stream), "download");
+	hip_check(hipStreamSynchronize(c->stream), "refine");
+	for (int i = 0; i < n; i++)
+	{
+		const ModelHdr& h = b.h.hdr[q->model[i]];
+		for (int k = 0; k < h.conf_size; k++)
+			q->conf[size_t(i) * q->conf_stride + k] = double(conf[size_t(i) * cs + k]);
+		q->e[i] = e[i] >= Real_host_max<T>() ? 1.7976931348623157e308 : double(e[i]);
+		if (q->coords)
+			for (int k = 0; k < 3 * h.nheavy; k++)
+				q->coords[size_t(i) * q->coord_stride + k] = double(x[size_t(i) * xs + k]);
+	}
+	if (flag)
+		throw Fail{SGPU_E_ATOM_OVERLAP, "Ligand atom exactly overlaps receptor atom.  I can't deal with this."};
+}
+
+void run_score(sgpu_ctx* c, const sgpu_score_request* q)
+{
+	Batch& b0 = find_batch(c, q->batch_id);
+	Batch& b = b0.precision == SGPU_FP64 ? b0 : *b0.fp64;
+	Receptor& r = find_receptor(c, q->receptor_id);
+	check_gd(q->gd);
+	const int n = q->nconfs;
+	require(n > 0 && q->model && q->conf && q->e && (q->mode == 0 || q->mode == 1), "score request incomplete");
+	for (int i = 0; i < n; i++)
+	{
+		require(q->model[i] >= 0 && q->model[i] < b.n, "model index out of range");
+		require(q->conf_stride >= b.h.hdr[q->model[i]].conf_size, "conf stride");
+	}
+	Cells& cl = receptor_cells(c, r, *q->gd);
+	EvalCtx<double> ec{};
+	ec.tab = table_ptrs<double>(c);
+	ec.backend = 1;
+	ec.rec = receptor_ptrs<double>(r, cl);
+	ec.slope = q->slope;
+	for (int d = 0; d < 3; d++)
+		ec.v[d] = q->v[d];
+	const int cs = b.h.plan.cmax, ps = std::max(b.h.plan.pmax, 64);
+	std::vector<double> conf(size_t(n) * cs, 0.0);
+	for (int i = 0; i < n; i++)
+		for (int k = 0; k < b.h.hdr[q->model[i]].conf_size; k++)
+			conf[size_t(i) * cs + k] = q->conf[size_t(i) * q->conf_stride + k];
+	DBuf dm, dc, de;
+	upload(dm, std::vector<int>(q->model, q->model + n), c->stream);
+	upload(dc, conf, c->stream);
+	de.alloc(size_t(n) * sizeof(double));
+	ensure_scratch(c->scratch2, size_t(n) * ps * sizeof(double));
+	Timer tm(c, "score");
+	hip_check(launch_score<double>(ptrs<double>(b), ec, b.h.plan, q->mode, n, dm.as<int>(), dc.as<double>(), cs,
+					  de.as<double>(), c->scratch2.as<double>(), ps, c->stream),
+			"score kernel");
+	tm.stop();
+	hip_check(hipMemcpyAsync(q->e, de.p, size_t(n) * sizeof(double), hipMemcpyDeviceToHost, c->stream), "download");
+	hip_check(hipStreamSynchronize(c->stream), "score");
+}
+
+} // namespace
+
+extern "C" {
+
+sgpu_status sgpu_eval_confs(sgpu_ctx* c, const sgpu_eval_request* q)
+{
+	return guarded(c, [&]() {
+		require(q != nullptr, "no request");
+		require(c->tab.ok, "tables not uploaded");
+		Batch& b = find_batch(c, q->batch_id);
+		if (b.precision == SGPU_FP64)
+			run_eval<double>(c, q);
+		else
+			run_eval<float>(c, q);
+	});
+}
+
+sgpu_status sgpu_run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, const sgpu_chains* ch,
+		sgpu_chain_results* out)
+{
+	return guarded(c, [&]() {
+		require(c->tab.ok, "tables not uploaded");
+		Batch& b = find_batch(c, batch_id);
+		if (b.precision == SGPU_FP64)
+			run_mc<double>(c, batch_id, grid_id, p, ch, out);
+		else
+			run_mc<float>(c, batch_id, grid_id, p, ch, out);
+	});
+}
+
+sgpu_status sgpu_run_refine(sgpu_ctx* c, const sgpu_refine_request* q)
+{
+	return guarded(c, [&]() {
+		require(q != nullptr, "no request");
+		require(c->tab.ok, "tables not uploaded");
+		Batch& b = find_batch(c, q->batch_id);
+		if (b.precision == SGPU_FP64)
+			run_refine<double>(c, q);
+		else
+			run_refine<float>(c, q);
+	});
+}
+
+sgpu_status sgpu_score(sgpu_ctx* c, const sgpu_score_request* q)
+{
+	return guarded(c, [&]() {
+		require(q != nullptr, "no request");
+		require(c->tab.ok, "tables not uploaded");
+		run_score(c, q);
+	});
+}
+
+sgpu_status sgpu_last_kernel_ms(sgpu_ctx* c, const char* kind, double* ms)
+{
+	return guarded(c, [&]() {
+		require(kind && ms, "arguments");
+		auto it = c->last_ms.find(kind);
+		require(it != c->last_ms.end(), "no launch of that kind yet");
+		*ms = it->second;
+	});
+}
+
+} // extern "C"

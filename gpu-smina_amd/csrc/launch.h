@@ -1,0 +1,68 @@
+// Host-side declarations of the kernel launches (one definition per
+// precision in kernels_f32.hip / kernels_f64.hip; grid build in grid.hip).
+#ifndef SGPU_LAUNCH_H
+#define SGPU_LAUNCH_H
+
+#include <hip/hip_runtime.h>
+#include "layout.h"
+
+namespace sgpu
+{
+
+constexpr int WAVES_PER_BLOCK = 1;
+
+struct WavePlan;
+template<typename T> struct EvalCtx;
+struct McParams;
+struct ChainLayout;
+
+template<typename T>
+hipError_t launch_eval(const BatchPtrs<T>& B, const EvalCtx<T>& ec, const WavePlan& plan, int n, const int* model,
+		const T* conf, int conf_stride, T* e, T* g, int g_stride, T* x, int x_stride, T* scratch, int pstride,
+		hipStream_t st);
+template<typename T>
+hipError_t launch_mc(const BatchPtrs<T>& B, const EvalCtx<T>& ec, const WavePlan& plan, const McParams& P,
+		const ChainLayout& L, char* chains, int nchains, const int* chain_model, const int* chain_steps,
+		const int* chain_maxiters, int step_lo, int step_hi, hipStream_t st);
+template<typename T>
+hipError_t launch_refine(const BatchPtrs<T>& B, const EvalCtx<T>& ec, const WavePlan& plan, int n, const int* model,
+		T* conf, int conf_stride, const int* maxiters, int accurate, int early_term, T* e, T* x, int x_stride,
+		T* scratch, int pstride, int64_t* evals, hipStream_t st);
+template<typename T>
+hipError_t launch_score(const BatchPtrs<T>& B, const EvalCtx<T>& ec, const WavePlan& plan, int mode, int n,
+		const int* model, const T* conf, int conf_stride, T* e, T* scratch, int pstride, hipStream_t st);
+
+// grid build: cell lists (per-dimension bricks lo/hi of each cell), then
+// the point maps in FP64 sums, then the corner-packed cells
+struct GridBuildArgs
+{
+	const double* rec_xyz;
+	const int* rec_type;
+	int rec_n;
+	const double* fast;
+	int table_n;
+	double factor;
+	double cutoff_sqr;
+	int cell_off[3];
+	int cell_dim[3];
+	const double* cell_lo; // [cell_dim[0] + cell_dim[1] + cell_dim[2]] per dimension
+	const double* cell_hi;
+	double box_lo[3];
+	double box_hi[3];
+	int* cell_start;       // [ncells + 1]
+	int* cell_atoms;       // sized by the count pass
+	int dim[3];            // grid points per axis
+	double init[3];
+	double factor_inv[3];
+	const int* types;      // [ntypes] ligand types
+	int ntypes;
+};
+hipError_t launch_cell_count(const GridBuildArgs& a, hipStream_t st);
+hipError_t launch_cell_fill(const GridBuildArgs& a, hipStream_t st);
+hipError_t launch_grid_points(const GridBuildArgs& a, double* points, hipStream_t st);
+template<typename T>
+hipError_t launch_grid_pack(const double* points, int ntypes, const int* dim, T* cells, hipStream_t st);
+
+} // namespace sgpu
+
+#endif

@@ -1,0 +1,256 @@
+/*
+ * Wave64 helpers and the smina geometry kernels in device form, templated on
+ * the arithmetic type T (float for the served search, double for the
+ * oracle-parity search).  Expressions keep smina's operand order
+ * (src/lib/common.h vec / mat, src/lib/quaternion.cpp, src/lib/curl.h) so the
+ * double instantiation, built with -ffp-contract=off, reproduces the CPU
+ * bit for bit.
+ */
+#ifndef SGPU_WAVE_H
+#define SGPU_WAVE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <float.h>
+#include "pmath.h"
+#include "layout.h"
+
+namespace sgpu
+{
+
+__device__ inline int lane_id() { return int(threadIdx.x) & (WAVE - 1); }
+__device__ inline int wave_in_block() { return int(threadIdx.x) / WAVE; }
+
+// order LDS and global accesses of one wavefront (all of its lanes) before
+// the accesses that follow; waves never wait for each other
+__device__ inline void wsync()
+{
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+__device__ inline float readlane(float v, int l)
+{
+	return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+__device__ inline double readlane(double v, int l)
+{
+	const uint64_t u = __builtin_bit_cast(uint64_t, v);
+	const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(u)), l));
+	const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(u >> 32)), l));
+	return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
+}
+__device__ inline int readlane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+
+// wave-wide sum in a fixed butterfly order (deterministic)
+template<typename T>
+__device__ inline T wave_sum(T v)
+{
+	for (int off = 32; off > 0; off >>= 1)
+		v += __shfl_xor(v, off, WAVE);
+	return v;
+}
+__device__ inline bool wave_any(bool p) { return __ballot(p) != 0; }
+__device__ inline bool wave_all(bool p) { return __ballot(!p) == 0; }
+
+template<typename T> struct Real;
+template<> struct Real<float>
+{
+	static constexpr bool ordered = false;
+	__device__ static float sin(float x) { return ::sinf(x); }
+	__device__ static float cos(float x) { return ::cosf(x); }
+	__device__ static float acos(float x) { return ::acosf(x); }
+	__device__ static float sqrt(float x) { return ::sqrtf(x); }
+	__device__ static float abs(float x) { return ::fabsf(x); }
+	__device__ static float ceil(float x) { return ::ceilf(x); }
+	__device__ static float eps() { return FLT_EPSILON; }
+	__device__ static float max() { return FLT_MAX; }
+};
+template<> struct Real<double>
+{
+	static constexpr bool ordered = true;
+	__device__ static double sin(double x) { return pmath::sin(x); }
+	__device__ static double cos(double x) { return pmath::cos(x); }
+	__device__ static double acos(double x) { return pmath::acos(x); }
+	__device__ static double sqrt(double x) { return __builtin_sqrt(x); }
+	__device__ static double abs(double x) { return __builtin_fabs(x); }
+	__device__ static double ceil(double x) { return __builtin_ceil(x); }
+	__device__ static double eps() { return DBL_EPSILON; }
+	__device__ static double max() { return DBL_MAX; }
+};
+
+template<typename T>
+struct V3
+{
+	T x, y, z;
+};
+template<typename T> __device__ inline V3<T> v3(T x, T y, T z) { return {x, y, z}; }
+template<typename T> __device__ inline V3<T> operator+(V3<T> a, V3<T> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template<typename T> __device__ inline V3<T> operator-(V3<T> a, V3<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template<typename T> __device__ inline V3<T> operator*(T s, V3<T> a) { return {s * a.x, s * a.y, s * a.z}; }
+template<typename T> __device__ inline T dot(V3<T> a, V3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template<typename T> __device__ inline T sqr(T x) { return x * x; }
+template<typename T> __device__ inline T norm_sqr(V3<T> a) { return sqr(a.x) + sqr(a.y) + sqr(a.z); }
+template<typename T> __device__ inline V3<T> cross(V3<T> a, V3<T> b)
+{
+	return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template<typename T> __device__ inline V3<T> ld3(const T* p) { return {p[0], p[1], p[2]}; }
+template<typename T> __device__ inline void st3(T* p, V3<T> v)
+{
+	p[0] = v.x;
+	p[1] = v.y;
+	p[2] = v.z;
+}
+
+// column-major 3x3 (m[i + 3j] = M(i, j)); M * v in smina's order
+template<typename T> __device__ inline V3<T> mat_vec(const T* m, V3<T> v)
+{
+	return {m[0] * v.x + m[3] * v.y + m[6] * v.z, m[1] * v.x + m[4] * v.y + m[7] * v.z,
+			m[2] * v.x + m[5] * v.y + m[8] * v.z};
+}
+
+template<typename T>
+struct Q4
+{
+	T a, b, c, d;
+};
+// Hamilton product in boost::math::quaternion's operand order
+template<typename T> __device__ inline Q4<T> qmul(Q4<T> l, Q4<T> r)
+{
+	return {l.a * r.a - l.b * r.b - l.c * r.c - l.d * r.d, l.a * r.b + l.b * r.a + l.c * r.d - l.d * r.c,
+			l.a * r.c - l.b * r.d + l.c * r.a + l.d * r.b, l.a * r.d + l.b * r.c - l.c * r.b + l.d * r.a};
+}
+
+// normalize_angle (src/lib/common.h:245-270)
+template<typename T> __device__ inline void normalize_angle(T& x)
+{
+	const T pi = T(3.1415926535897931);
+	for (;;)
+	{
+		if (x > 3 * pi)
+		{
+			T n = (x - pi) / (2 * pi);
+			x -= 2 * pi * Real<T>::ceil(n);
+		}
+		else if (x < -3 * pi)
+		{
+			T n = (-x - pi) / (2 * pi);
+			x += 2 * pi * Real<T>::ceil(n);
+		}
+		else
+		{
+			if (x > pi)
+				x -= 2 * pi;
+			else if (x < -pi)
+				x += 2 * pi;
+			return;
+		}
+	}
+}
+template<typename T> __device__ inline T normalized_angle(T x)
+{
+	normalize_angle(x);
+	return x;
+}
+
+// angle_to_quaternion (src/lib/quaternion.cpp:42-62)
+template<typename T> __device__ inline Q4<T> axis_angle_q(V3<T> axis, T angle)
+{
+	normalize_angle(angle);
+	const T c = Real<T>::cos(angle / 2);
+	const T s = Real<T>::sin(angle / 2);
+	return {c, s * axis.x, s * axis.y, s * axis.z};
+}
+template<typename T> __device__ inline Q4<T> rotation_q(V3<T> rot)
+{
+	const T angle = Real<T>::sqrt(norm_sqr(rot));
+	if (angle > Real<T>::eps())
+	{
+		V3<T> axis = (T(1) / angle) * rot;
+		return axis_angle_q(axis, angle);
+	}
+	return {T(1), T(0), T(0), T(0)};
+}
+// quaternion_normalize_approx (src/lib/quaternion.h:69-80)
+template<typename T> __device__ inline void q_normalize_approx(Q4<T>& q)
+{
+	const T s = sqr(q.a) + sqr(q.b) + sqr(q.c) + sqr(q.d);
+	if (Real<T>::abs(s - 1) < T(1e-6))
+		return;
+	const T inv = T(1) / Real<T>::sqrt(s);
+	q.a *= inv;
+	q.b *= inv;
+	q.c *= inv;
+	q.d *= inv;
+}
+// quaternion_increment (src/lib/quaternion.cpp:123-127)
+template<typename T> __device__ inline Q4<T> q_increment(Q4<T> q, V3<T> rot)
+{
+	Q4<T> r = qmul(rotation_q(rot), q);
+	q_normalize_approx(r);
+	return r;
+}
+// quaternion_to_angle (src/lib/quaternion.cpp:63-79)
+template<typename T> __device__ inline V3<T> q_to_angle(Q4<T> q)
+{
+	const T c = q.a;
+	const T pi = T(3.1415926535897931);
+	if (c > -1 && c < 1)
+	{
+		T angle = 2 * Real<T>::acos(c);
+		if (angle > pi)
+			angle -= 2 * pi;
+		V3<T> axis = v3(q.b, q.c, q.d);
+		const T s = Real<T>::sin(angle / 2);
+		if (Real<T>::abs(s) < Real<T>::eps())
+			return v3(T(0), T(0), T(0));
+		const T k = angle / s;
+		return v3(axis.x * k, axis.y * k, axis.z * k);
+	}
+	return v3(T(0), T(0), T(0));
+}
+// quaternion_to_r3 (src/lib/quaternion.cpp:81-107)
+template<typename T> __device__ inline void q_to_mat(Q4<T> q, T* m)
+{
+	const T a = q.a, b = q.b, c = q.c, d = q.d;
+	const T aa = a * a, ab = a * b, ac = a * c, ad = a * d;
+	const T bb = b * b, bc = b * c, bd = b * d;
+	const T cc = c * c, cd = c * d, dd = d * d;
+	m[0] = (aa + bb - cc - dd);
+	m[3] = 2 * (-ad + bc);
+	m[6] = 2 * (ac + bd);
+	m[1] = 2 * (ad + bc);
+	m[4] = (aa - bb + cc - dd);
+	m[7] = 2 * (-ab + cd);
+	m[2] = 2 * (-ac + bd);
+	m[5] = 2 * (ab + cd);
+	m[8] = (aa - bb - cc + dd);
+}
+
+// curl (src/lib/curl.h:28-43); every cap the search passes is below max/10
+template<typename T> __device__ inline void curl(T& e, V3<T>& d, T v)
+{
+	if (e > 0)
+	{
+		T tmp = (v < Real<T>::eps()) ? T(0) : (v / (v + e));
+		e *= tmp;
+		const T t2 = sqr(tmp);
+		d.x *= t2;
+		d.y *= t2;
+		d.z *= t2;
+	}
+}
+template<typename T> __device__ inline void curl(T& e, T v)
+{
+	if (e > 0)
+	{
+		T tmp = (v < Real<T>::eps()) ? T(0) : (v / (v + e));
+		e *= tmp;
+	}
+}
+
+} // namespace sgpu
+
+#endif

@@ -1,0 +1,994 @@
+/*
+ * Host side of the docking path: scoring tables, model flattening, chain
+ * seeding, the GpuDocking driver and its C wrappers (include/sgpu_host.h).
+ * Compiled with -ffp-contract=off: table values, local coordinates and
+ * initial confs must be the exact doubles smina computes.
+ */
+#include "gpu_docking.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+#include "../csrc/atom_types.h"
+#include "../csrc/rng.h"
+#include "mol.h"
+
+namespace sgpu_host
+{
+
+using namespace sgpu;
+
+namespace
+{
+const double kPi = 3.1415926535897931;
+const double kEps = std::numeric_limits<double>::epsilon();
+const double kMax = std::numeric_limits<double>::max();
+
+inline double sq(double x) { return x * x; }
+
+// ---- terms (src/lib/everything.h:156-541), charge independent
+double optimal_distance(int t1, int t2) { return atom_type_info(t1).xs_radius + atom_type_info(t2).xs_radius; }
+double slope_step(double bad, double good, double x)
+{
+	if (bad < good)
+	{
+		if (x <= bad)
+			return 0;
+		if (x >= good)
+			return 1;
+	}
+	else
+	{
+		if (x >= bad)
+			return 0;
+		if (x <= good)
+			return 1;
+	}
+	return (x - bad) / (good - bad);
+}
+bool h_bond_possible(int t1, int t2)
+{
+	const AtomTypeInfo a = atom_type_info(t1), b = atom_type_info(t2);
+	return (a.donor && b.acceptor) || (b.donor && a.acceptor);
+}
+bool anti_h_bond(int t1, int t2)
+{
+	const AtomTypeInfo a = atom_type_info(t1), b = atom_type_info(t2);
+	if (a.donor && !a.acceptor)
+		return b.donor && !b.acceptor;
+	if (!a.donor && a.acceptor)
+		return !b.donor && b.acceptor;
+	return false;
+}
+double ipow(double x, int n)
+{
+	double r = 1;
+	for (int i = 0; i < n; i++)
+		r *= x;
+	return r;
+}
+double quadratic_below(double r, double d0)
+{
+	const double d = r - d0;
+	return d > 0 ? 0.0 : d * d;
+}
+double lj(double r, double d0, double depth, int i, int j, double cap)
+{
+	const double ci = ipow(d0, i) * depth * j / (double(i) - double(j));
+	const double cj = ipow(d0, j) * depth * i / (double(j) - double(i));
+	const double ri = ipow(r, i), rj = ipow(r, j);
+	if (ri > kEps && rj > kEps)
+		return std::min(cap, ci / ri + cj / rj);
+	return cap;
+}
+double term(const sgpu_term& t, int t1, int t2, double r)
+{
+	const AtomTypeInfo a = atom_type_info(t1), b = atom_type_info(t2);
+	switch (t.kind)
+	{
+	case SGPU_TERM_GAUSS:
+	{
+		const double x = r - (optimal_distance(t1, t2) + t.p[0]);
+		return std::exp(-sq(x / t.p[1]));
+	}
+	case SGPU_TERM_REPULSION:
+		return quadratic_below(r, optimal_distance(t1, t2) + t.p[0]);
+	case SGPU_TERM_HYDROPHOBIC:
+		return (a.hydrophobe && b.hydrophobe) ? slope_step(t.p[1], t.p[0], r - optimal_distance(t1, t2)) : 0.0;
+	case SGPU_TERM_NON_HYDROPHOBIC:
+		return (!a.hydrophobe && !b.hydrophobe) ? slope_step(t.p[1], t.p[0], r - optimal_distance(t1, t2)) : 0.0;
+	case SGPU_TERM_NON_DIR_H_BOND:
+		return h_bond_possible(t1, t2) ? slope_step(t.p[1], t.p[0], r - optimal_distance(t1, t2)) : 0.0;
+	case SGPU_TERM_NON_DIR_ANTI_H_BOND_QUADRATIC:
+		return anti_h_bond(t1, t2) ? quadratic_below(r, optimal_distance(t1, t2) + t.p[0]) : 0.0;
+	case SGPU_TERM_DONOR_DONOR_QUADRATIC:
+		return (a.donor && b.donor) ? quadratic_below(r, optimal_distance(t1, t2) + t.p[0]) : 0.0;
+	case SGPU_TERM_ACCEPTOR_ACCEPTOR_QUADRATIC:
+		return (a.acceptor && b.acceptor) ? quadratic_below(r, optimal_distance(t1, t2) + t.p[0]) : 0.0;
+	case SGPU_TERM_VDW:
+	{
+		const double d0 = optimal_distance(t1, t2), s = t.p[2];
+		double rr = r;
+		if (rr > d0 + s)
+			rr -= s;
+		else if (rr < d0 - s)
+			rr += s;
+		else
+			rr = d0;
+		return lj(rr, d0, 1, int(t.p[0]), int(t.p[1]), t.p[3]);
+	}
+	case SGPU_TERM_NON_DIR_H_BOND_LJ:
+		return h_bond_possible(t1, t2) ? lj(r, optimal_distance(t1, t2) + t.p[0], 5, 10, 12, t.p[1]) : 0.0;
+	}
+	throw std::runtime_error("unknown scoring term");
+}
+
+// conf_independent with num_tors_div (src/lib/everything.h:808-830)
+double conf_independent(const sgpu_scoring& sf, double num_tors, double x)
+{
+	if (!sf.has_num_tors_div)
+		return x;
+	const double w = 0.1 * (sf.num_tors_div_weight + 1);
+	const double y = 1 + w * num_tors / 5.0;
+	if (std::abs(x) < kEps)
+		return 0;
+	if (std::abs(y) < kEps)
+		return (x * y > 0) ? kMax : -kMax;
+	return x / y;
+}
+
+// ---- chain initialisation randoms, on the product's MT19937
+struct Gen
+{
+	MTState s;
+	explicit Gen(uint32_t seed) { mt_seed(s, seed); }
+	uint32_t next() { return mt_next(s); }
+};
+// random_normal: Marsaglia's polar method, one fresh pair per call
+double normal(Gen& g)
+{
+	for (;;)
+	{
+		const double u = random_fl(-1.0, 1.0, g);
+		const double v = random_fl(-1.0, 1.0, g);
+		const double s = u * u + v * v;
+		if (s < 1 && s > 0)
+			return 0 + 1 * (u * std::sqrt(-2 * std::log(s) / s));
+	}
+}
+} // namespace
+
+Tables build_tables(const sgpu_scoring& sf, double factor)
+{
+	Tables t;
+	t.factor = factor;
+	const double cutoff = 8;
+	t.cutoff_sqr = cutoff * cutoff;
+	t.n = int(size_t(factor * t.cutoff_sqr) + 3);
+	const int n = t.n;
+	std::vector<double> rs(n + 2);
+	for (int i = 0; i < n + 2; i++)
+		rs[i] = std::sqrt(i / factor);
+	t.smooth.assign(size_t(SGPU_NUM_TYPE_PAIRS) * n * 2, 0);
+	t.fast.assign(size_t(SGPU_NUM_TYPE_PAIRS) * n, 0);
+	std::vector<double> e(n);
+	for (int t1 = 0; t1 < NUM_ATOM_TYPES; t1++)
+		for (int t2 = t1; t2 < NUM_ATOM_TYPES; t2++)
+		{
+			for (int i = 0; i < n; i++)
+			{
+				double acc = 0;
+				for (int k = 0; k < sf.nterms; k++)
+					acc += sf.terms[k].weight * term(sf.terms[k], t1, t2, rs[i]);
+				e[i] = acc;
+			}
+			const size_t base = size_t(tri_index(t1, t2)) * n;
+			for (int i = 0; i < n; i++)
+			{
+				double dor = 0;
+				if (!(i == 0 || i == n - 1))
+				{
+					const double delta = rs[i + 1] - rs[i - 1];
+					const double r = rs[i];
+					dor = (e[i + 1] - e[i - 1]) / (delta * r);
+				}
+				t.smooth[2 * (base + i)] = e[i];
+				t.smooth[2 * (base + i) + 1] = dor;
+				const double f1 = e[i];
+				const double f2 = (i + 1 >= n) ? 0 : e[i + 1];
+				t.fast[base + i] = (f2 + f1) / 2;
+			}
+		}
+	return t;
+}
+
+void FlatModel::publish()
+{
+	view.atom_type = atom_type.data();
+	view.atom_frame = atom_frame.data();
+	view.atom_local = atom_local.data();
+	view.atom_init = atom_init.data();
+	view.frame_parent = frame_parent.data();
+	view.frame_kind = frame_kind.data();
+	view.frame_begin = frame_begin.data();
+	view.frame_end = frame_end.data();
+	view.frame_tree = frame_tree.data();
+	view.frame_conf = frame_conf.data();
+	view.frame_change = frame_change.data();
+	view.frame_origin = frame_origin.data();
+	view.frame_axis = frame_axis.data();
+	view.pair_a = pair_a.data();
+	view.pair_b = pair_b.data();
+	view.pair_tri = pair_tri.data();
+}
+
+FlatModel flatten(const sgpu_model_desc& d)
+{
+	if (d.nmovable <= 0 || d.nmovable > d.natoms || d.nframes <= 0 || d.ntrees <= 0)
+		throw std::runtime_error("invalid model description");
+	FlatModel m;
+	const int N = d.nmovable, NA = d.natoms;
+	m.atom_type.assign(d.type, d.type + NA);
+	for (int t : m.atom_type)
+		if (t < 0 || t >= NUM_ATOM_TYPES)
+			throw std::runtime_error("atom type out of range");
+	m.atom_init.assign(d.xyz, d.xyz + 3 * size_t(NA));
+	auto X = [&](int a, int k) { return d.xyz[3 * a + k]; };
+	std::vector<int> tree_of(d.nframes, -1);
+	for (int t = 0; t < d.ntrees; t++)
+	{
+		const int last = t + 1 < d.ntrees ? d.tree_frame[t + 1] : d.nframes;
+		for (int f = d.tree_frame[t]; f < last; f++)
+			tree_of[f] = t;
+	}
+	m.atom_frame.assign(N, -1);
+	m.frame_origin.assign(3 * size_t(d.nframes), 0);
+	m.frame_axis.assign(3 * size_t(d.nframes), 0);
+	std::vector<double> abs_origin(3 * size_t(d.nframes));
+	int conf_off = 0, change_off = 0;
+	std::vector<int> torsion_count(d.ntrees, 0);
+	for (int t = 0; t < d.ntrees; t++)
+	{
+		const bool lig = t < d.nligands;
+		const int first = d.tree_frame[t];
+		const int last = t + 1 < d.ntrees ? d.tree_frame[t + 1] : d.nframes;
+		const int ntors = (last - first) - (lig ? 1 : 0);
+		m.tree_ligand.push_back(lig ? 1 : 0);
+		m.tree_ntors.push_back(ntors);
+		m.tree_conf.push_back(conf_off);
+		int k = 0;
+		for (int f = first; f < last; f++)
+		{
+			const int parent = d.frame_parent[f];
+			int kind = parent < 0 ? (lig ? SGPU_FRAME_RIGID_ROOT : SGPU_FRAME_FIRST_SEGMENT) : SGPU_FRAME_SEGMENT;
+			m.frame_parent.push_back(parent);
+			m.frame_kind.push_back(kind);
+			m.frame_begin.push_back(d.frame_begin[f]);
+			m.frame_end.push_back(d.frame_end[f]);
+			m.frame_tree.push_back(t);
+			for (int a = d.frame_begin[f]; a < d.frame_end[f]; a++)
+				m.atom_frame[a] = f;
+			const int oa = d.frame_origin_atom[f];
+			for (int c = 0; c < 3; c++)
+				abs_origin[3 * f + c] = X(oa, c);
+			if (kind == SGPU_FRAME_RIGID_ROOT)
+			{
+				m.frame_conf.push_back(conf_off);
+				m.frame_change.push_back(change_off);
+				for (int c = 0; c < 3; c++)
+					m.frame_origin[3 * f + c] = X(oa, c);
+			}
+			else
+			{
+				const int idx = lig ? k - 1 : k;
+				m.frame_conf.push_back(conf_off + (lig ? 7 : 0) + idx);
+				m.frame_change.push_back(change_off + (lig ? 6 : 0) + idx);
+				// axis_frame (src/lib/tree.h:116-125): unit vector from the axis
+				// atom to the origin
+				const int aa = d.frame_axis_atom[f];
+				if (aa < 0 || aa >= NA)
+					throw std::runtime_error("rotatable frame without an axis atom");
+				const double diff[3] = {X(oa, 0) - X(aa, 0), X(oa, 1) - X(aa, 1), X(oa, 2) - X(aa, 2)};
+				const double nrm = std::sqrt(sq(diff[0]) + sq(diff[1]) + sq(diff[2]));
+				if (!(nrm >= kEps))
+					throw std::runtime_error("zero-length rotatable bond");
+				const double inv = 1 / nrm;
+				for (int c = 0; c < 3; c++)
+					m.frame_axis[3 * f + c] = inv * diff[c];
+				if (kind == SGPU_FRAME_SEGMENT)
+				{ // segment (src/lib/tree.h:139-144): relative to the parent's origin
+					for (int c = 0; c < 3; c++)
+						m.frame_origin[3 * f + c] = X(oa, c) - abs_origin[3 * parent + c];
+				}
+				else
+					for (int c = 0; c < 3; c++)
+						m.frame_origin[3 * f + c] = X(oa, c);
+			}
+			++k;
+		}
+		conf_off += lig ? 7 + ntors : ntors;
+		change_off += lig ? 6 + ntors : ntors;
+	}
+	for (int a = 0; a < N; a++)
+		if (m.atom_frame[a] < 0)
+			throw std::runtime_error("movable atom outside every frame");
+	m.atom_local.resize(3 * size_t(N));
+	for (int a = 0; a < N; a++)
+		for (int c = 0; c < 3; c++)
+			m.atom_local[3 * a + c] = X(a, c) - abs_origin[3 * m.atom_frame[a] + c];
+	// bonds and mobility (src/lib/parse_pdbqt.cpp:279-336)
+	std::vector<std::vector<int>> bonds(NA);
+	for (int k = 0; k < d.nbonds; k++)
+	{
+		const int a = d.bonds[2 * k], b = d.bonds[2 * k + 1];
+		if (a < 0 || b < 0 || a >= NA || b >= NA)
+			throw std::runtime_error("bond atom out of range");
+		bonds[a].push_back(b);
+		bonds[b].push_back(a);
+	}
+	auto fixed = [&](int i, int j) -> bool {
+		if (i >= N && j >= N)
+			return true;
+		if (i >= N || j >= N)
+		{
+			const int mv = i < N ? i : j, o = i < N ? j : i;
+			const int f = m.atom_frame[mv];
+			return m.frame_kind[f] == SGPU_FRAME_FIRST_SEGMENT && d.frame_axis_atom[f] == o;
+		}
+		const int fi = m.atom_frame[i], fj = m.atom_frame[j];
+		if (fi == fj)
+			return true;
+		if (m.frame_kind[fj] != SGPU_FRAME_RIGID_ROOT && (d.frame_axis_atom[fj] == i || d.frame_origin_atom[fj] == i))
+			return true;
+		if (m.frame_kind[fi] != SGPU_FRAME_RIGID_ROOT && (d.frame_axis_atom[fi] == j || d.frame_origin_atom[fi] == j))
+			return true;
+		return false;
+	};
+	std::vector<int> lig_begin(d.nligands, N), lig_end(d.nligands, 0);
+	for (int f = 0; f < d.nframes; f++)
+	{
+		const int t = tree_of[f];
+		if (t < d.nligands)
+		{
+			lig_begin[t] = std::min(lig_begin[t], d.frame_begin[f]);
+			lig_end[t] = std::max(lig_end[t], d.frame_end[f]);
+		}
+	}
+	auto find_ligand = [&](int a) {
+		for (int l = 0; l < d.nligands; l++)
+			if (a >= lig_begin[l] && a < lig_end[l])
+				return l;
+		return d.nligands;
+	};
+	// bonded_to (src/lib/model.cpp:576-589): depth first, an atom already
+	// listed is not expanded again
+	std::function<void(int, int, std::vector<int>&)> bonded_to = [&](int a, int n, std::vector<int>& out) {
+		if (std::find(out.begin(), out.end(), a) != out.end())
+			return;
+		out.push_back(a);
+		if (n > 0)
+			for (int b : bonds[a])
+				bonded_to(b, n - 1, out);
+	};
+	// initialize_pairs (src/lib/model.cpp:598-624)
+	std::vector<std::pair<int, int>> other;
+	std::vector<std::vector<std::pair<int, int>>> ligp(d.nligands);
+	for (int i = 0; i < NA; i++)
+	{
+		const int il = find_ligand(i);
+		std::vector<int> near;
+		bonded_to(i, 3, near);
+		for (int j = i + 1; j < NA; j++)
+		{
+			if (i >= N && j >= N)
+				continue;
+			if (fixed(i, j) || std::find(near.begin(), near.end(), j) != near.end())
+				continue;
+			const int t1 = m.atom_type[i], t2 = m.atom_type[j];
+			if (is_hydrogen_type(t1) || is_hydrogen_type(t2))
+				continue;
+			if (il < d.nligands && find_ligand(j) == il)
+				ligp[il].push_back({i, j});
+			else
+				other.push_back({i, j});
+		}
+	}
+	auto push_pair = [&](std::pair<int, int> p) {
+		m.pair_a.push_back(p.first);
+		m.pair_b.push_back(p.second);
+		m.pair_tri.push_back(tri_index_permissive(m.atom_type[p.first], m.atom_type[p.second]));
+	};
+	for (auto& p : other)
+		push_pair(p);
+	for (auto& lp : ligp)
+		for (auto& p : lp)
+			push_pair(p);
+	// num_tors (src/lib/terms.cpp:39-110), rotatable hydrogens fixed
+	auto heavy_nbrs = [&](int a) {
+		int k = 0;
+		for (int b : bonds[a])
+			if (!is_hydrogen_type(m.atom_type[b]))
+				++k;
+		return k;
+	};
+	auto rotatable = [&](int a, int b) {
+		for (int f = 0; f < d.nframes; f++)
+			if (m.frame_kind[f] == SGPU_FRAME_SEGMENT &&
+					((d.frame_axis_atom[f] == a && d.frame_origin_atom[f] == b) ||
+							(d.frame_axis_atom[f] == b && d.frame_origin_atom[f] == a)))
+				return true;
+		return false;
+	};
+	m.num_tors = 0;
+	for (int l = 0; l < d.nligands; l++)
+		for (int i = lig_begin[l]; i < lig_end[l]; i++)
+		{
+			if (is_hydrogen_type(m.atom_type[i]))
+				continue;
+			unsigned ar = 0;
+			for (int b : bonds[i])
+				if (rotatable(i, b) && !is_hydrogen_type(m.atom_type[b]) && heavy_nbrs(b) > 1 && heavy_nbrs(i) > 1)
+					++ar;
+			m.num_tors += 0.5 * ar;
+		}
+	m.nheavy = 0;
+	for (int a = 0; a < N; a++)
+		if (!is_hydrogen_type(m.atom_type[a]))
+			++m.nheavy;
+	sgpu_flat_model& v = m.view;
+	v.natoms = N;
+	v.ninflex = NA - N;
+	v.nframes = d.nframes;
+	v.nligands = d.nligands;
+	v.ntrees = d.ntrees;
+	v.npairs = int(m.pair_a.size());
+	v.nother = int(other.size());
+	v.conf_size = conf_off;
+	v.change_size = change_off;
+	v.num_tors = int(m.num_tors);
+	m.publish();
+	return m;
+}
+
+std::vector<double> initial_conf(const FlatModel& m)
+{ // model::get_initial_conf (src/lib/model.cpp:650-659)
+	std::vector<double> c(m.view.conf_size, 0.0);
+	for (int f = 0; f < m.view.nframes; f++)
+		if (m.frame_kind[f] == SGPU_FRAME_RIGID_ROOT)
+		{
+			const int ci = m.frame_conf[f];
+			for (int k = 0; k < 3; k++)
+				c[ci + k] = m.frame_origin[3 * f + k];
+			c[ci + 3] = 1;
+		}
+	return c;
+}
+
+void make_chain(const FlatModel& m, uint32_t seed, const sgpu_grid_dims& gd, double* conf, uint32_t* rng_state)
+{
+	Gen g(seed);
+	std::fill(conf, conf + m.view.conf_size, 0.0);
+	// conf::randomize (src/lib/conf.h:362-367): ligands, then side chains
+	for (size_t t = 0; t < m.tree_ligand.size(); t++)
+	{
+		if (!m.tree_ligand[t])
+			continue;
+		double* p = conf + m.tree_conf[t];
+		for (int k = 0; k < 3; k++)
+			p[k] = random_fl(gd.begin[k], gd.end[k], g);
+		// random_orientation (src/lib/quaternion.cpp:129-141) with
+		// boost::math::abs's scaled norm
+		for (;;)
+		{
+			const double x0 = normal(g), x1 = normal(g), x2 = normal(g), x3 = normal(g);
+			const double mx = std::max(std::max(std::abs(x0), std::abs(x1)), std::max(std::abs(x2), std::abs(x3)));
+			double nrm = 0;
+			if (mx != 0)
+			{
+				const double k = 1 / mx;
+				const double a = x0 * k, b = x1 * k, c = x2 * k, d = x3 * k;
+				nrm = mx * std::sqrt(a * a + b * b + c * c + d * d);
+			}
+			if (nrm > kEps)
+			{
+				p[3] = x0 / nrm;
+				p[4] = x1 / nrm;
+				p[5] = x2 / nrm;
+				p[6] = x3 / nrm;
+				break;
+			}
+		}
+		for (int k = 0; k < m.tree_ntors[t]; k++)
+			p[7 + k] = random_fl(-kPi, kPi, g);
+	}
+	for (size_t t = 0; t < m.tree_ligand.size(); t++)
+		if (!m.tree_ligand[t])
+			for (int k = 0; k < m.tree_ntors[t]; k++)
+				conf[m.tree_conf[t] + k] = random_fl(-kPi, kPi, g);
+	if (rng_state)
+	{
+		std::memcpy(rng_state, g.s.x, sizeof(g.s.x));
+		rng_state[624] = g.s.i;
+	}
+}
+
+SearchParams setup_search(const FlatModel& m, int exhaustiveness)
+{
+	SearchParams p;
+	int dof = 0;
+	for (size_t t = 0; t < m.tree_ligand.size(); t++)
+		dof += m.tree_ntors[t] + (m.tree_ligand[t] ? 6 : 0);
+	const int heuristic = m.view.natoms + 10 * dof;
+	p.num_steps = int(unsigned(70 * 3 * (50 + heuristic) / 2));
+	p.maxiters = (25 + m.view.natoms) / 3;
+	p.num_tasks = exhaustiveness;
+	return p;
+}
+
+// add_to_output_container (src/lib/coords.cpp:34-58); the container stays
+// sorted by energy, ties in insertion order
+void add_to_output_container(std::vector<Pose>& out, const Pose& t, double min_rmsd, size_t max_size)
+{
+	size_t best_i = out.size();
+	double best = kMax;
+	for (size_t i = 0; i < out.size(); i++)
+	{
+		double acc = 0;
+		const size_t na = t.coords.size() / 3;
+		for (size_t a = 0; a < na; a++)
+			acc += sq(t.coords[3 * a] - out[i].coords[3 * a]) + sq(t.coords[3 * a + 1] - out[i].coords[3 * a + 1]) +
+					sq(t.coords[3 * a + 2] - out[i].coords[3 * a + 2]);
+		const double res = na > 0 ? std::sqrt(acc / na) : 0;
+		if (i == 0 || res < best)
+		{
+			best_i = i;
+			best = res;
+		}
+	}
+	if (best_i < out.size() && best < min_rmsd)
+	{
+		if (t.e < out[best_i].e)
+			out[best_i] = t;
+	}
+	else
+	{
+		if (out.size() < max_size)
+			out.push_back(t);
+		else if (!out.empty() && t.e < out.back().e)
+			out.back() = t;
+	}
+	std::stable_sort(out.begin(), out.end(), [](const Pose& a, const Pose& b) { return a.e < b.e; });
+}
+
+namespace
+{
+void check(sgpu_ctx* c, sgpu_status s)
+{
+	if (s != SGPU_OK)
+		throw std::runtime_error(std::string("sgpu: ") + sgpu_last_error(c));
+}
+} // namespace
+
+GpuDocking::GpuDocking(const GpuSettings& s, const sgpu_scoring& sf, const sgpu_receptor& rec,
+		const sgpu_grid_dims& gd) :
+		s_(s), gd_(gd)
+{
+	terms_.assign(sf.terms, sf.terms + sf.nterms);
+	sf_ = sf;
+	sf_.terms = terms_.data();
+	sgpu_status st = sgpu_create(s.device, &ctx_);
+	if (st != SGPU_OK)
+		throw std::runtime_error("sgpu_create failed: no HIP device");
+	Tables t = build_tables(sf_, 32);
+	sgpu_tables tt{t.n, t.factor, t.cutoff_sqr, t.smooth.data(), t.fast.data()};
+	check(ctx_, sgpu_upload_tables(ctx_, &tt));
+	check(ctx_, sgpu_upload_receptor(ctx_, &rec, &receptor_));
+}
+
+GpuDocking::~GpuDocking() { sgpu_destroy(ctx_); }
+
+std::string GpuDocking::description() const
+{
+	char name[256] = {0};
+	int cus = 0;
+	sgpu_device_info(ctx_, name, sizeof(name), &cus);
+	return std::string(name) + ", " + std::to_string(cus) + " CUs, " + (s_.fp64 ? "FP64" : "FP32");
+}
+
+void GpuDocking::ensure_grids(const std::vector<int>& types)
+{
+	bool have = grid_ >= 0;
+	for (int t : types)
+		have = have && std::find(grid_types_.begin(), grid_types_.end(), t) != grid_types_.end();
+	if (have)
+		return;
+	std::vector<int> all = grid_types_;
+	for (int t : types)
+		if (std::find(all.begin(), all.end(), t) == all.end())
+			all.push_back(t);
+	std::sort(all.begin(), all.end());
+	if (grid_ >= 0)
+		sgpu_release_grid(ctx_, grid_);
+	check(ctx_, sgpu_build_grids(ctx_, receptor_, &gd_, all.data(), int(all.size()), s_.fp64 ? SGPU_FP64 : SGPU_FP32,
+						&grid_));
+	grid_types_ = all;
+}
+
+void GpuDocking::dock_batch(std::vector<DockJob*>& jobs)
+{
+	const size_t per = s_.ligands_per_launch > 0 ? size_t(s_.ligands_per_launch) : jobs.size();
+	for (size_t b = 0; b < jobs.size(); b += per)
+		dock_sub(jobs.data() + b, std::min(per, jobs.size() - b));
+}
+
+void GpuDocking::dock_sub(DockJob* const* jobs, size_t nj)
+{
+	if (nj == 0)
+		return;
+	const auto t0 = std::chrono::steady_clock::now();
+	const int prec = s_.fp64 ? SGPU_FP64 : SGPU_FP32;
+	std::vector<FlatModel> fms;
+	std::vector<int> types;
+	for (size_t j = 0; j < nj; j++)
+	{
+		fms.push_back(flatten(*jobs[j]->m));
+		for (int a = 0; a < fms.back().view.natoms; a++)
+		{
+			const int t = fms.back().atom_type[a];
+			if (!is_hydrogen_type(t) && std::find(types.begin(), types.end(), t) == types.end())
+				types.push_back(t);
+		}
+	}
+	ensure_grids(types);
+	std::vector<sgpu_flat_model> views;
+	for (auto& f : fms)
+		views.push_back(f.view);
+	int batch = -1;
+	check(ctx_, sgpu_upload_models(ctx_, views.data(), int(views.size()), prec, &batch));
+	// chains: per ligand rng(seed), tasks seeded in order (parallel_mc)
+	int cstride = 1, hstride = 1;
+	for (auto& f : fms)
+	{
+		cstride = std::max(cstride, f.view.conf_size);
+		hstride = std::max(hstride, 3 * f.nheavy);
+	}
+	std::vector<int> cmodel, csteps, citers;
+	std::vector<double> cconf;
+	std::vector<uint32_t> crng;
+	std::vector<size_t> first(nj + 1, 0);
+	std::vector<SearchParams> sp;
+	for (size_t j = 0; j < nj; j++)
+	{
+		sp.push_back(setup_search(fms[j], s_.exhaustiveness));
+		Gen g(uint32_t(s_.seed));
+		first[j] = cmodel.size();
+		for (int t = 0; t < s_.exhaustiveness; t++)
+		{
+			const int seed = random_int(0, 1000000, g);
+			std::vector<double> conf(cstride, 0.0);
+			std::vector<uint32_t> st(625);
+			make_chain(fms[j], uint32_t(seed), gd_, conf.data(), st.data());
+			cmodel.push_back(int(j));
+			csteps.push_back(sp[j].num_steps);
+			citers.push_back(sp[j].maxiters);
+			cconf.insert(cconf.end(), conf.begin(), conf.end());
+			crng.insert(crng.end(), st.begin(), st.end());
+		}
+	}
+	first[nj] = cmodel.size();
+	const int nchains = int(cmodel.size());
+	sgpu_chains ch{nchains, cmodel.data(), cconf.data(), cstride, crng.data(), csteps.data(), citers.data()};
+	sgpu_mc_params mp{};
+	mp.temperature = 1.2;
+	mp.mutation_amplitude = 2;
+	mp.min_rmsd = 1.0;
+	mp.num_saved_mins = 20;
+	for (int k = 0; k < 3; k++)
+	{
+		mp.hunt_cap[k] = 10;
+		mp.authentic_v[k] = 1000;
+	}
+	mp.cache_slope = 1e6;
+	const int maxp = 20;
+	std::vector<int> cnt(nchains);
+	std::vector<double> oe(size_t(nchains) * maxp), oc(size_t(nchains) * maxp * cstride),
+			ox(size_t(nchains) * maxp * hstride);
+	std::vector<int64_t> evals(nchains, 0);
+	sgpu_chain_results res{maxp, cstride, hstride, cnt.data(), oe.data(), oc.data(), ox.data(), evals.data()};
+	check(ctx_, sgpu_run_mc(ctx_, batch, grid_, &mp, &ch, &res));
+	// merge_output_containers with min_rmsd 2 (src/gpu/gpu_docking.cpp:576-598)
+	std::vector<int> pose_model, pose_iters;
+	std::vector<double> pose_conf;
+	std::vector<std::pair<size_t, size_t>> pose_ref;
+	for (size_t j = 0; j < nj; j++)
+	{
+		auto& out = jobs[j]->out;
+		out.clear();
+		jobs[j]->evals = 0;
+		for (size_t c = first[j]; c < first[j + 1]; c++)
+		{
+			jobs[j]->evals += evals[c];
+			for (int k = 0; k < cnt[c]; k++)
+			{
+				Pose p;
+				const size_t slot = c * maxp + k;
+				p.e = oe[slot];
+				p.c.assign(oc.begin() + slot * cstride, oc.begin() + slot * cstride + fms[j].view.conf_size);
+				p.coords.assign(ox.begin() + slot * hstride, ox.begin() + slot * hstride + 3 * fms[j].nheavy);
+				add_to_output_container(out, p, 2, size_t(sp[j].num_saved_mins));
+			}
+		}
+		for (size_t k = 0; k < out.size(); k++)
+		{
+			pose_model.push_back(int(j));
+			pose_iters.push_back(sp[j].maxiters);
+			std::vector<double> c(cstride, 0.0);
+			std::copy(out[k].c.begin(), out[k].c.end(), c.begin());
+			pose_conf.insert(pose_conf.end(), c.begin(), c.end());
+			pose_ref.push_back({j, k});
+		}
+	}
+	// refine_structure of every pose (do_search, src/main/main.cpp:478-482)
+	const int np = int(pose_model.size());
+	if (np > 0)
+	{
+		std::vector<double> pe(np), px(size_t(np) * hstride);
+		sgpu_refine_request rq{};
+		rq.batch_id = batch;
+		rq.receptor_id = receptor_;
+		rq.gd = &gd_;
+		for (int k = 0; k < 3; k++)
+			rq.v[k] = 1000;
+		rq.nposes = np;
+		rq.model = pose_model.data();
+		rq.conf = pose_conf.data();
+		rq.conf_stride = cstride;
+		rq.maxiters = pose_iters.data();
+		rq.e = pe.data();
+		rq.coords = px.data();
+		rq.coord_stride = hstride;
+		check(ctx_, sgpu_run_refine(ctx_, &rq));
+		for (int i = 0; i < np; i++)
+		{
+			Pose& p = jobs[pose_ref[i].first]->out[pose_ref[i].second];
+			const FlatModel& f = fms[pose_ref[i].first];
+			p.c.assign(pose_conf.begin() + size_t(i) * cstride, pose_conf.begin() + size_t(i) * cstride + f.view.conf_size);
+			p.e = pe[i];
+			p.coords.assign(px.begin() + size_t(i) * hstride, px.begin() + size_t(i) * hstride + 3 * f.nheavy);
+		}
+	}
+	// finish_docking (src/main/main.cpp:256-324): FP64 rescoring on the device
+	std::vector<int> best_model;
+	std::vector<double> best_conf;
+	std::vector<size_t> with_poses;
+	for (size_t j = 0; j < nj; j++)
+	{
+		auto& out = jobs[j]->out;
+		std::stable_sort(out.begin(), out.end(), [](const Pose& a, const Pose& b) { return a.e < b.e; });
+		if (out.empty())
+			continue;
+		with_poses.push_back(j);
+		best_model.push_back(int(j));
+		std::vector<double> c(cstride, 0.0);
+		std::copy(out[0].c.begin(), out[0].c.end(), c.begin());
+		best_conf.insert(best_conf.end(), c.begin(), c.end());
+	}
+	if (!with_poses.empty())
+	{
+		const double v[3] = {1000, 1000, 1000};
+		std::vector<double> intra(with_poses.size());
+		sgpu_score_request sq{};
+		sq.batch_id = batch;
+		sq.receptor_id = receptor_;
+		sq.gd = &gd_;
+		sq.slope = 1e6;
+		for (int k = 0; k < 3; k++)
+			sq.v[k] = v[k];
+		sq.mode = 1;
+		sq.nconfs = int(with_poses.size());
+		sq.model = best_model.data();
+		sq.conf = best_conf.data();
+		sq.conf_stride = cstride;
+		sq.e = intra.data();
+		check(ctx_, sgpu_score(ctx_, &sq));
+		std::vector<int> smodel;
+		std::vector<double> sconf;
+		std::vector<std::pair<size_t, size_t>> sref;
+		for (size_t j : with_poses)
+			for (size_t k = 0; k < jobs[j]->out.size(); k++)
+				if (jobs[j]->out[k].e < 0.1 * kMax)
+				{
+					smodel.push_back(int(j));
+					std::vector<double> c(cstride, 0.0);
+					std::copy(jobs[j]->out[k].c.begin(), jobs[j]->out[k].c.end(), c.begin());
+					sconf.insert(sconf.end(), c.begin(), c.end());
+					sref.push_back({j, k});
+				}
+		if (!smodel.empty())
+		{
+			std::vector<double> se(smodel.size());
+			sq.mode = 0;
+			sq.nconfs = int(smodel.size());
+			sq.model = smodel.data();
+			sq.conf = sconf.data();
+			sq.e = se.data();
+			check(ctx_, sgpu_score(ctx_, &sq));
+			std::vector<double> intra_of(nj, 0);
+			for (size_t q = 0; q < with_poses.size(); q++)
+				intra_of[with_poses[q]] = intra[q];
+			for (size_t i = 0; i < smodel.size(); i++)
+			{
+				const size_t j = sref[i].first;
+				jobs[j]->out[sref[i].second].e = conf_independent(sf_, fms[j].num_tors, se[i] - intra_of[j]);
+			}
+		}
+	}
+	for (size_t j = 0; j < nj; j++)
+	{
+		auto& out = jobs[j]->out;
+		std::stable_sort(out.begin(), out.end(), [](const Pose& a, const Pose& b) { return a.e < b.e; });
+		// remove_redundant with out_min_rmsd 1 (src/main/main.cpp:204-210)
+		std::vector<Pose> tmp;
+		for (const Pose& p : out)
+			add_to_output_container(tmp, p, 1.0, out.size());
+		// the reported modes: num_modes within energy_range of the best
+		std::vector<Pose> fin;
+		for (size_t i = 0; i < tmp.size(); i++)
+		{
+			if (int(i) >= s_.num_modes || !(tmp[i].e < 0.1 * kMax) || tmp[i].e > tmp[0].e + s_.energy_range)
+				break;
+			fin.push_back(tmp[i]);
+		}
+		out.swap(fin);
+	}
+	sgpu_release_models(ctx_, batch);
+	const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+	for (size_t j = 0; j < nj; j++)
+		jobs[j]->seconds = secs / double(nj);
+}
+
+} // namespace sgpu_host
+
+// ---------------------------------------------------------------- C wrappers
+using namespace sgpu_host;
+
+struct sgpu_flat
+{
+	FlatModel m;
+};
+
+struct sgpu_docking
+{
+	std::unique_ptr<GpuDocking> d;
+	std::string err;
+	std::vector<sgpu_term> terms;
+};
+
+extern "C" {
+
+void sgpu_vina_scoring(sgpu_term* t, sgpu_scoring* out, double hydrophobic_scale)
+{ // src/main/main.cpp:1347-1353
+	t[0] = {SGPU_TERM_GAUSS, {0, 0.5, 8, 0, 0}, -0.035579};
+	t[1] = {SGPU_TERM_GAUSS, {3, 2, 8, 0, 0}, -0.005156};
+	t[2] = {SGPU_TERM_REPULSION, {0, 8, 0, 0, 0}, 0.840245};
+	t[3] = {SGPU_TERM_HYDROPHOBIC, {0.5, 1.5, 8, 0, 0}, -0.035069 * hydrophobic_scale};
+	t[4] = {SGPU_TERM_NON_DIR_H_BOND, {-0.7, 0, 8, 0, 0}, -0.587439};
+	out->nterms = 5;
+	out->terms = t;
+	out->has_num_tors_div = 1;
+	out->num_tors_div_weight = 5 * 0.05846 / 0.1 - 1;
+}
+
+int sgpu_build_tables(const sgpu_scoring* sf, double factor, double* smooth, double* fast)
+{
+	try
+	{
+		Tables t = build_tables(*sf, factor);
+		if (smooth)
+			std::copy(t.smooth.begin(), t.smooth.end(), smooth);
+		if (fast)
+			std::copy(t.fast.begin(), t.fast.end(), fast);
+		return t.n;
+	}
+	catch (...)
+	{
+		return -1;
+	}
+}
+
+sgpu_flat* sgpu_flatten(const sgpu_model_desc* d)
+{
+	try
+	{
+		sgpu_flat* f = new sgpu_flat;
+		f->m = flatten(*d);
+		f->m.publish();
+		return f;
+	}
+	catch (...)
+	{
+		return nullptr;
+	}
+}
+const sgpu_flat_model* sgpu_flat_get(const sgpu_flat* f) { return &f->m.view; }
+void sgpu_flat_free(sgpu_flat* f) { delete f; }
+void sgpu_initial_conf(const sgpu_flat* f, double* conf)
+{
+	std::vector<double> c = initial_conf(f->m);
+	std::copy(c.begin(), c.end(), conf);
+}
+void sgpu_make_chain(const sgpu_flat* f, uint32_t seed, const sgpu_grid_dims* gd, double* conf, uint32_t* rng_state)
+{
+	make_chain(f->m, seed, *gd, conf, rng_state);
+}
+
+sgpu_docking* sgpu_docking_create(const sgpu_dock_settings* s, const sgpu_mol* receptor, const sgpu_grid_dims* gd)
+{
+	sgpu_docking* d = new sgpu_docking;
+	try
+	{
+		GpuSettings gs;
+		gs.device = s->device;
+		gs.fp64 = s->fp64 != 0;
+		gs.exhaustiveness = s->exhaustiveness;
+		gs.seed = s->seed;
+		gs.num_modes = s->num_modes;
+		gs.energy_range = s->energy_range;
+		gs.ligands_per_launch = s->ligands_per_launch;
+		d->terms.resize(5);
+		sgpu_scoring sf;
+		sgpu_vina_scoring(d->terms.data(), &sf, s->hydrophobic_scale);
+		const sgpu_model_desc* rd = sgpu_mol_desc(receptor);
+		sgpu_receptor rec{rd->natoms, rd->xyz, rd->type};
+		d->d.reset(new GpuDocking(gs, sf, rec, *gd));
+	}
+	catch (std::exception& e)
+	{
+		d->err = e.what();
+	}
+	return d;
+}
+void sgpu_docking_free(sgpu_docking* d) { delete d; }
+const char* sgpu_docking_error(const sgpu_docking* d) { return d->err.c_str(); }
+sgpu_ctx* sgpu_docking_ctx(sgpu_docking* d) { return d->d ? d->d->ctx() : nullptr; }
+int sgpu_docking_receptor_id(sgpu_docking* d) { return d->d ? d->d->receptor_id() : -1; }
+
+int sgpu_dock(sgpu_docking* d, const sgpu_mol* const* ligands, int n, sgpu_dock_result* out)
+{
+	if (!d->d)
+		return -1;
+	try
+	{
+		std::vector<GpuDocking::DockJob> jobs(n);
+		std::vector<GpuDocking::DockJob*> jp;
+		for (int i = 0; i < n; i++)
+		{
+			jobs[i].m = sgpu_mol_desc(ligands[i]);
+			jp.push_back(&jobs[i]);
+		}
+		d->d->dock_batch(jp);
+		for (int i = 0; i < n; i++)
+		{
+			out[i].nposes = int(std::min<size_t>(32, jobs[i].out.size()));
+			for (int k = 0; k < out[i].nposes; k++)
+				out[i].e[k] = jobs[i].out[k].e;
+			out[i].evals = jobs[i].evals;
+			out[i].seconds = jobs[i].seconds;
+		}
+		return 0;
+	}
+	catch (std::exception& e)
+	{
+		d->err = e.what();
+		return -1;
+	}
+}
+
+} // extern "C"

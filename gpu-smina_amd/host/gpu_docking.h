@@ -1,0 +1,127 @@
+/*
+ * GpuDocking: the C++ facade of the docking path, mirroring the reference's
+ * class of the same name (src/gpu/gpu_docking.h:51-114) over the sgpu C ABI.
+ *
+ *   parallel_mc::operator()  -> GpuDocking::dock_batch (search + merge)
+ *   refine_structure         -> refine step of dock_batch
+ *   finish_docking rescoring -> FP64 device rescoring in dock_batch
+ *
+ * A smina build keeps its option names: --gpu (FP32 served search),
+ * --gpu_fp64 (FP64, bit-identical to the CPU search), --exhaustiveness,
+ * --seed, --num_modes, --energy_range, --gpu_batch (ligands per launch).
+ */
+#ifndef SGPU_GPU_DOCKING_H
+#define SGPU_GPU_DOCKING_H
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/sgpu.h"
+#include "../../include/sgpu_host.h"
+
+namespace sgpu_host
+{
+
+// a conf with its energy and heavy-atom coordinates (output_type,
+// src/lib/conf.h:400-406)
+struct Pose
+{
+	std::vector<double> c;
+	double e = 0;
+	std::vector<double> coords;
+};
+
+// smina's model after flattening (FlatModel, src/gpu/gpu_flatten_data.h)
+struct FlatModel
+{
+	sgpu_flat_model view{};
+	std::vector<int> atom_type, atom_frame, frame_parent, frame_kind, frame_begin, frame_end, frame_tree,
+			frame_conf, frame_change, pair_a, pair_b, pair_tri;
+	std::vector<double> atom_local, atom_init, frame_origin, frame_axis;
+	std::vector<int> tree_ligand, tree_ntors, tree_conf;
+	int nheavy = 0;
+	double num_tors = 0;
+	void publish();
+};
+
+FlatModel flatten(const sgpu_model_desc& d);
+std::vector<double> initial_conf(const FlatModel& m);
+// make_chain (src/gpu/gpu_search.cpp:19-31): rng(seed), conf::randomize in
+// the box; the generator state (624 words + position) after the draws
+void make_chain(const FlatModel& m, uint32_t seed, const sgpu_grid_dims& gd, double* conf, uint32_t* rng_state);
+
+// search parameters of one ligand (setup_search, src/main/main.cpp:513-529)
+struct SearchParams
+{
+	int num_steps = 0;
+	int maxiters = 0;
+	int num_saved_mins = 20;
+	double min_rmsd = 1.0;
+	int num_tasks = 8;
+};
+SearchParams setup_search(const FlatModel& m, int exhaustiveness);
+
+// scoring tables: precalculate_linear(sf, factor) (src/lib/precalculate.h:206-237)
+struct Tables
+{
+	int n = 0;
+	double factor = 32;
+	double cutoff_sqr = 64;
+	std::vector<double> smooth, fast;
+};
+Tables build_tables(const sgpu_scoring& sf, double factor);
+
+void add_to_output_container(std::vector<Pose>& out, const Pose& t, double min_rmsd, size_t max_size);
+
+struct GpuSettings
+{
+	int device = 0;
+	bool fp64 = false;
+	int exhaustiveness = 8;
+	int seed = 1;
+	int num_modes = 9;
+	double energy_range = 3.0;
+	int ligands_per_launch = 0; // 0: the whole batch in one launch
+};
+
+class GpuDocking
+{
+public:
+	GpuDocking(const GpuSettings& s, const sgpu_scoring& sf, const sgpu_receptor& rec, const sgpu_grid_dims& gd);
+	~GpuDocking();
+	GpuDocking(const GpuDocking&) = delete;
+	GpuDocking& operator=(const GpuDocking&) = delete;
+
+	struct DockJob
+	{
+		const sgpu_model_desc* m;
+		std::vector<Pose> out; // final, rescored poses (finish_docking order)
+		int64_t evals = 0;
+		double seconds = 0;
+	};
+	// for each job: parallel_mc seeded with rng(seed) (do_search), refine of
+	// every pose, FP64 rescoring, clustering (src/gpu/gpu_docking.cpp:756-831,
+	// src/main/main.cpp:256-324)
+	void dock_batch(std::vector<DockJob*>& jobs);
+
+	sgpu_ctx* ctx() { return ctx_; }
+	int receptor_id() const { return receptor_; }
+	std::string description() const;
+
+private:
+	void ensure_grids(const std::vector<int>& types);
+	void dock_sub(DockJob* const* jobs, size_t n);
+	GpuSettings s_;
+	sgpu_ctx* ctx_ = nullptr;
+	sgpu_scoring sf_;
+	std::vector<sgpu_term> terms_;
+	sgpu_grid_dims gd_;
+	int receptor_ = -1;
+	int grid_ = -1;
+	std::vector<int> grid_types_;
+};
+
+} // namespace sgpu_host
+
+#endif

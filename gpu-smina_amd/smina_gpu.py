@@ -1,0 +1,469 @@
+"""Python bindings (ctypes) of libsgpu.so: the C ABI of include/sgpu.h and the
+host entry points of include/sgpu_host.h.
+
+The library is built in-tree (``make -C gpu-smina_amd``); importing this
+module fails loudly when it is missing, so nothing here can silently run
+without the HIP kernels.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsgpu.so")
+
+FP32, FP64 = 0, 1
+OK = 0
+STATUS = {0: "OK", 1: "INVALID", 2: "HIP", 3: "OOM", 4: "ATOM_OVERLAP", 5: "NO_DEVICE", 6: "NOT_FOUND"}
+
+I32P = C.POINTER(C.c_int)
+F64P = C.POINTER(C.c_double)
+U32P = C.POINTER(C.c_uint32)
+I64P = C.POINTER(C.c_int64)
+
+
+class Tables(C.Structure):
+    _fields_ = [("n", C.c_int), ("factor", C.c_double), ("cutoff_sqr", C.c_double),
+                ("smooth", F64P), ("fast", F64P)]
+
+
+class Receptor(C.Structure):
+    _fields_ = [("natoms", C.c_int), ("xyz", F64P), ("type", I32P)]
+
+
+class GridDims(C.Structure):
+    _fields_ = [("begin", C.c_double * 3), ("end", C.c_double * 3), ("n", C.c_int * 3)]
+
+
+class FlatModel(C.Structure):
+    _fields_ = [("natoms", C.c_int), ("ninflex", C.c_int), ("nframes", C.c_int), ("nligands", C.c_int),
+                ("ntrees", C.c_int), ("npairs", C.c_int), ("nother", C.c_int), ("conf_size", C.c_int),
+                ("change_size", C.c_int), ("num_tors", C.c_int),
+                ("atom_type", I32P), ("atom_frame", I32P), ("atom_local", F64P), ("atom_init", F64P),
+                ("frame_parent", I32P), ("frame_kind", I32P), ("frame_begin", I32P), ("frame_end", I32P),
+                ("frame_tree", I32P), ("frame_conf", I32P), ("frame_change", I32P),
+                ("frame_origin", F64P), ("frame_axis", F64P),
+                ("pair_a", I32P), ("pair_b", I32P), ("pair_tri", I32P)]
+
+
+class McParams(C.Structure):
+    _fields_ = [("temperature", C.c_double), ("mutation_amplitude", C.c_double), ("min_rmsd", C.c_double),
+                ("num_saved_mins", C.c_int), ("hunt_cap", C.c_double * 3), ("authentic_v", C.c_double * 3),
+                ("cache_slope", C.c_double), ("accurate_line_search", C.c_int), ("early_term", C.c_int)]
+
+
+class Chains(C.Structure):
+    _fields_ = [("nchains", C.c_int), ("model", I32P), ("conf", F64P), ("conf_stride", C.c_int),
+                ("rng", U32P), ("num_steps", I32P), ("maxiters", I32P)]
+
+
+class ChainResults(C.Structure):
+    _fields_ = [("max_poses", C.c_int), ("conf_stride", C.c_int), ("coord_stride", C.c_int),
+                ("count", I32P), ("e", F64P), ("conf", F64P), ("coords", F64P), ("evals", I64P)]
+
+
+class EvalRequest(C.Structure):
+    _fields_ = [("batch_id", C.c_int), ("grid_id", C.c_int), ("receptor_id", C.c_int),
+                ("gd", C.POINTER(GridDims)), ("slope", C.c_double), ("v", C.c_double * 3), ("nconfs", C.c_int),
+                ("model", I32P), ("conf", F64P), ("conf_stride", C.c_int), ("e", F64P), ("g", F64P),
+                ("change_stride", C.c_int), ("coords", F64P), ("coord_stride", C.c_int)]
+
+
+class RefineRequest(C.Structure):
+    _fields_ = [("batch_id", C.c_int), ("receptor_id", C.c_int), ("gd", C.POINTER(GridDims)),
+                ("v", C.c_double * 3), ("accurate_line_search", C.c_int), ("early_term", C.c_int),
+                ("nposes", C.c_int), ("model", I32P), ("conf", F64P), ("conf_stride", C.c_int),
+                ("maxiters", I32P), ("e", F64P), ("coords", F64P), ("coord_stride", C.c_int)]
+
+
+class ScoreRequest(C.Structure):
+    _fields_ = [("batch_id", C.c_int), ("receptor_id", C.c_int), ("gd", C.POINTER(GridDims)),
+                ("slope", C.c_double), ("v", C.c_double * 3), ("mode", C.c_int), ("nconfs", C.c_int),
+                ("model", I32P), ("conf", F64P), ("conf_stride", C.c_int), ("e", F64P)]
+
+
+class ModelDesc(C.Structure):
+    _fields_ = [("natoms", C.c_int), ("nmovable", C.c_int), ("type", I32P), ("xyz", F64P),
+                ("nbonds", C.c_int), ("bonds", I32P), ("nframes", C.c_int),
+                ("frame_parent", I32P), ("frame_begin", I32P), ("frame_end", I32P),
+                ("frame_origin_atom", I32P), ("frame_axis_atom", I32P),
+                ("nligands", C.c_int), ("ntrees", C.c_int), ("tree_frame", I32P)]
+
+
+class Term(C.Structure):
+    _fields_ = [("kind", C.c_int), ("p", C.c_double * 5), ("weight", C.c_double)]
+
+
+class Scoring(C.Structure):
+    _fields_ = [("nterms", C.c_int), ("terms", C.POINTER(Term)), ("has_num_tors_div", C.c_int),
+                ("num_tors_div_weight", C.c_double)]
+
+
+class DockSettings(C.Structure):
+    _fields_ = [("device", C.c_int), ("fp64", C.c_int), ("exhaustiveness", C.c_int), ("seed", C.c_int),
+                ("num_modes", C.c_int), ("energy_range", C.c_double), ("hydrophobic_scale", C.c_double),
+                ("ligands_per_launch", C.c_int)]
+
+
+class DockResult(C.Structure):
+    _fields_ = [("nposes", C.c_int), ("e", C.c_double * 32), ("evals", C.c_int64), ("seconds", C.c_double)]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libsgpu.so is not built (run `make -C gpu-smina_amd` or __graft_entry__.build())")
+    lib = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    sig = {
+        "sgpu_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
+        "sgpu_destroy": (None, [vp]),
+        "sgpu_last_error": (C.c_char_p, [vp]),
+        "sgpu_device_info": (C.c_int, [vp, C.c_char_p, C.c_int, I32P]),
+        "sgpu_upload_tables": (C.c_int, [vp, C.POINTER(Tables)]),
+        "sgpu_upload_receptor": (C.c_int, [vp, C.POINTER(Receptor), I32P]),
+        "sgpu_build_grids": (C.c_int, [vp, C.c_int, C.POINTER(GridDims), I32P, C.c_int, C.c_int, I32P]),
+        "sgpu_download_grid": (C.c_int, [vp, C.c_int, C.c_int, F64P]),
+        "sgpu_release_grid": (C.c_int, [vp, C.c_int]),
+        "sgpu_upload_models": (C.c_int, [vp, C.POINTER(FlatModel), C.c_int, C.c_int, I32P]),
+        "sgpu_release_models": (C.c_int, [vp, C.c_int]),
+        "sgpu_eval_confs": (C.c_int, [vp, C.POINTER(EvalRequest)]),
+        "sgpu_run_mc": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(McParams), C.POINTER(Chains),
+                                  C.POINTER(ChainResults)]),
+        "sgpu_run_refine": (C.c_int, [vp, C.POINTER(RefineRequest)]),
+        "sgpu_score": (C.c_int, [vp, C.POINTER(ScoreRequest)]),
+        "sgpu_last_kernel_ms": (C.c_int, [vp, C.c_char_p, F64P]),
+        "sgpu_mol_free": (None, [vp]),
+        "sgpu_mol_desc": (C.POINTER(ModelDesc), [vp]),
+        "sgpu_synth_receptor": (vp, [C.c_uint32, C.c_int, C.c_double]),
+        "sgpu_synth_ligand": (vp, [C.c_uint32, C.c_int, C.c_int, F64P]),
+        "sgpu_vina_scoring": (None, [C.POINTER(Term), C.POINTER(Scoring), C.c_double]),
+        "sgpu_build_tables": (C.c_int, [C.POINTER(Scoring), C.c_double, F64P, F64P]),
+        "sgpu_flatten": (vp, [C.POINTER(ModelDesc)]),
+        "sgpu_flat_get": (C.POINTER(FlatModel), [vp]),
+        "sgpu_flat_free": (None, [vp]),
+        "sgpu_initial_conf": (None, [vp, F64P]),
+        "sgpu_make_chain": (None, [vp, C.c_uint32, C.POINTER(GridDims), F64P, U32P]),
+        "sgpu_docking_create": (vp, [C.POINTER(DockSettings), vp, C.POINTER(GridDims)]),
+        "sgpu_docking_free": (None, [vp]),
+        "sgpu_docking_error": (C.c_char_p, [vp]),
+        "sgpu_dock": (C.c_int, [vp, C.POINTER(vp), C.c_int, C.POINTER(DockResult)]),
+        "sgpu_docking_ctx": (vp, [vp]),
+        "sgpu_docking_receptor_id": (C.c_int, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+lib = _load()
+
+EXPORTED = ["sgpu_create", "sgpu_destroy", "sgpu_last_error", "sgpu_device_info", "sgpu_upload_tables",
+            "sgpu_upload_receptor", "sgpu_build_grids", "sgpu_download_grid", "sgpu_release_grid",
+            "sgpu_upload_models", "sgpu_release_models", "sgpu_eval_confs", "sgpu_run_mc", "sgpu_run_refine",
+            "sgpu_score", "sgpu_last_kernel_ms"]
+
+
+def ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+def i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# ------------------------------------------------------------------ molecules
+class Mol:
+    """A library-owned molecule (sgpu_mol) with numpy views of its description."""
+
+    def __init__(self, handle):
+        if not handle:
+            raise RuntimeError("molecule generation failed")
+        self.h = C.c_void_p(handle)
+        self.desc_ptr = lib.sgpu_mol_desc(self.h)
+        d = self.desc_ptr.contents
+        self.natoms = d.natoms
+        self.nmovable = d.nmovable
+        self.type = np.ctypeslib.as_array(d.type, (d.natoms,)).copy()
+        self.xyz = np.ctypeslib.as_array(d.xyz, (d.natoms * 3,)).copy().reshape(-1, 3)
+        self.nframes = d.nframes
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.sgpu_mol_free(self.h)
+            self.h = None
+
+    @staticmethod
+    def receptor(seed, nheavy=1800, pocket=8.0):
+        return Mol(lib.sgpu_synth_receptor(seed, nheavy, pocket))
+
+    @staticmethod
+    def ligand(seed, nheavy=24, ntors=6, center=(0.0, 0.0, 0.0)):
+        c = f64(center)
+        return Mol(lib.sgpu_synth_ligand(seed, nheavy, ntors, ptr(c, F64P)))
+
+
+def vina_scoring(hydrophobic_scale=1.0):
+    terms = (Term * 5)()
+    sf = Scoring()
+    lib.sgpu_vina_scoring(terms, C.byref(sf), hydrophobic_scale)
+    sf._terms = terms  # keep alive
+    return sf
+
+
+def build_tables(sf, factor=32.0):
+    n = lib.sgpu_build_tables(C.byref(sf), factor, None, None)
+    smooth = np.zeros(378 * n * 2)
+    fast = np.zeros(378 * n)
+    lib.sgpu_build_tables(C.byref(sf), factor, ptr(smooth, F64P), ptr(fast, F64P))
+    return n, smooth, fast
+
+
+def grid_dims(center, size, spacing=0.375):
+    """smina's box: n = ceil(size / spacing) intervals, the span widened to
+    n * spacing around the center (src/main/main.cpp:1370-1382)."""
+    gd = GridDims()
+    for i in range(3):
+        n = int(np.ceil(size[i] / spacing))
+        real = spacing * n
+        gd.begin[i] = center[i] - real / 2
+        gd.end[i] = gd.begin[i] + real
+        gd.n[i] = n
+    return gd
+
+
+class Flat:
+    """A flattened model (sgpu_flat) for the device layer."""
+
+    def __init__(self, mol):
+        self.h = C.c_void_p(lib.sgpu_flatten(mol.desc_ptr))
+        if not self.h:
+            raise RuntimeError("flatten failed")
+        self.mol = mol
+        self.view = lib.sgpu_flat_get(self.h).contents
+        v = self.view
+        self.conf_size = v.conf_size
+        self.change_size = v.change_size
+        self.natoms = v.natoms
+        self.npairs = v.npairs
+        self.nother = v.nother
+        self.pair_a = np.ctypeslib.as_array(v.pair_a, (v.npairs,)).copy() if v.npairs else np.zeros(0, np.int32)
+        self.pair_b = np.ctypeslib.as_array(v.pair_b, (v.npairs,)).copy() if v.npairs else np.zeros(0, np.int32)
+        t = np.ctypeslib.as_array(v.atom_type, (v.natoms,))
+        self.nheavy = int(np.sum((t != 0) & (t != 1)))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.sgpu_flat_free(self.h)
+            self.h = None
+
+    def initial_conf(self):
+        c = np.zeros(self.conf_size)
+        lib.sgpu_initial_conf(self.h, ptr(c, F64P))
+        return c
+
+    def make_chain(self, seed, gd):
+        c = np.zeros(self.conf_size)
+        s = np.zeros(625, np.uint32)
+        lib.sgpu_make_chain(self.h, seed, C.byref(gd), ptr(c, F64P), ptr(s, U32P))
+        return c, s
+
+
+# ------------------------------------------------------------------ device
+class SgpuError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"sgpu {STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class Context:
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        st = lib.sgpu_create(device, C.byref(h))
+        if st != OK:
+            raise SgpuError(st, "sgpu_create failed")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib.sgpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def check(self, st):
+        if st != OK:
+            raise SgpuError(st, lib.sgpu_last_error(self.h).decode())
+
+    def device_info(self):
+        name = C.create_string_buffer(256)
+        cus = C.c_int()
+        self.check(lib.sgpu_device_info(self.h, name, 256, C.byref(cus)))
+        return name.value.decode(), cus.value
+
+    def upload_tables(self, n, smooth, fast, factor=32.0, cutoff_sqr=64.0):
+        self._tables = (f64(smooth), f64(fast))
+        t = Tables(n, factor, cutoff_sqr, ptr(self._tables[0], F64P), ptr(self._tables[1], F64P))
+        self.check(lib.sgpu_upload_tables(self.h, C.byref(t)))
+
+    def upload_receptor(self, xyz, types):
+        xyz = f64(xyz).reshape(-1)
+        types = i32(types)
+        r = Receptor(len(types), ptr(xyz, F64P), ptr(types, I32P))
+        rid = C.c_int()
+        self.check(lib.sgpu_upload_receptor(self.h, C.byref(r), C.byref(rid)))
+        return rid.value
+
+    def build_grids(self, rid, gd, types, precision):
+        t = i32(types)
+        gid = C.c_int()
+        self.check(lib.sgpu_build_grids(self.h, rid, C.byref(gd), ptr(t, I32P), len(t), precision, C.byref(gid)))
+        return gid.value
+
+    def download_grid(self, gid, t, gd):
+        out = np.zeros((gd.n[0] + 1) * (gd.n[1] + 1) * (gd.n[2] + 1))
+        self.check(lib.sgpu_download_grid(self.h, gid, t, ptr(out, F64P)))
+        return out
+
+    def upload_models(self, flats, precision):
+        arr = (FlatModel * len(flats))(*[f.view for f in flats])
+        bid = C.c_int()
+        self.check(lib.sgpu_upload_models(self.h, arr, len(flats), precision, C.byref(bid)))
+        return bid.value
+
+    def release_models(self, bid):
+        self.check(lib.sgpu_release_models(self.h, bid))
+
+    def eval_confs(self, bid, models, confs, change_size, natoms, v, grid_id=-1, receptor_id=0, gd=None, slope=1e6):
+        models = i32(models)
+        confs = f64(confs)
+        n = len(models)
+        e = np.zeros(n)
+        g = np.zeros((n, change_size))
+        x = np.zeros((n, 3 * natoms))
+        q = EvalRequest()
+        q.batch_id, q.grid_id, q.receptor_id = bid, grid_id, receptor_id
+        q.gd = C.pointer(gd) if gd is not None else None
+        q.slope = slope
+        for k in range(3):
+            q.v[k] = v[k]
+        q.nconfs = n
+        q.model, q.conf, q.conf_stride = ptr(models, I32P), ptr(confs, F64P), confs.shape[1]
+        q.e, q.g, q.change_stride = ptr(e, F64P), ptr(g, F64P), change_size
+        q.coords, q.coord_stride = ptr(x, F64P), 3 * natoms
+        self.check(lib.sgpu_eval_confs(self.h, C.byref(q)))
+        return e, g, x
+
+    def run_mc(self, bid, gid, params, models, confs, rngs, steps, maxiters, max_poses=20, coord_stride=None):
+        models = i32(models)
+        confs = f64(confs)
+        rngs = np.ascontiguousarray(rngs, dtype=np.uint32)
+        steps = i32(steps)
+        maxiters = i32(maxiters)
+        n = len(models)
+        cs = confs.shape[1]
+        xs = coord_stride or 3 * 256
+        ch = Chains(n, ptr(models, I32P), ptr(confs, F64P), cs, ptr(rngs, U32P), ptr(steps, I32P),
+                    ptr(maxiters, I32P))
+        cnt = np.zeros(n, np.int32)
+        e = np.zeros((n, max_poses))
+        c = np.zeros((n, max_poses, cs))
+        x = np.zeros((n, max_poses, xs))
+        ev = np.zeros(n, np.int64)
+        res = ChainResults(max_poses, cs, xs, ptr(cnt, I32P), ptr(e, F64P), ptr(c, F64P), ptr(x, F64P),
+                           ptr(ev, I64P))
+        self.check(lib.sgpu_run_mc(self.h, bid, gid, C.byref(params), C.byref(ch), C.byref(res)))
+        return cnt, e, c, x, ev
+
+    def refine(self, bid, rid, gd, models, confs, maxiters, v=(1000.0, 1000.0, 1000.0), coord_stride=3 * 256):
+        models = i32(models)
+        confs = f64(confs).copy()
+        maxiters = i32(maxiters)
+        n = len(models)
+        e = np.zeros(n)
+        x = np.zeros((n, coord_stride))
+        q = RefineRequest()
+        q.batch_id, q.receptor_id, q.gd = bid, rid, C.pointer(gd)
+        for k in range(3):
+            q.v[k] = v[k]
+        q.nposes = n
+        q.model, q.conf, q.conf_stride = ptr(models, I32P), ptr(confs, F64P), confs.shape[1]
+        q.maxiters, q.e, q.coords, q.coord_stride = ptr(maxiters, I32P), ptr(e, F64P), ptr(x, F64P), coord_stride
+        self.check(lib.sgpu_run_refine(self.h, C.byref(q)))
+        return e, confs, x
+
+    def score(self, bid, rid, gd, models, confs, mode, v=(1000.0, 1000.0, 1000.0), slope=1e6):
+        models = i32(models)
+        confs = f64(confs)
+        n = len(models)
+        e = np.zeros(n)
+        q = ScoreRequest()
+        q.batch_id, q.receptor_id, q.gd, q.slope = bid, rid, C.pointer(gd), slope
+        for k in range(3):
+            q.v[k] = v[k]
+        q.mode, q.nconfs = mode, n
+        q.model, q.conf, q.conf_stride, q.e = ptr(models, I32P), ptr(confs, F64P), confs.shape[1], ptr(e, F64P)
+        self.check(lib.sgpu_score(self.h, C.byref(q)))
+        return e
+
+    def last_ms(self, kind):
+        ms = C.c_double()
+        self.check(lib.sgpu_last_kernel_ms(self.h, kind.encode(), C.byref(ms)))
+        return ms.value
+
+
+def mc_params(hunt=10.0, authentic=1000.0, num_saved=20):
+    p = McParams()
+    p.temperature = 1.2
+    p.mutation_amplitude = 2.0
+    p.min_rmsd = 1.0
+    p.num_saved_mins = num_saved
+    for k in range(3):
+        p.hunt_cap[k] = hunt
+        p.authentic_v[k] = authentic
+    p.cache_slope = 1e6
+    p.accurate_line_search = 0
+    p.early_term = 0
+    return p
+
+
+def search_params(natoms, ntors_total, nligands=1):
+    """setup_search (src/main/main.cpp:513-529): steps and BFGS iterations."""
+    heuristic = natoms + 10 * (ntors_total + 6 * nligands)
+    return int(70 * 3 * (50 + heuristic) // 2), (25 + natoms) // 3
+
+
+class Docking:
+    """GpuDocking driver: search, merge, refine, FP64 rescoring."""
+
+    def __init__(self, receptor, gd, fp64=False, exhaustiveness=8, seed=1, num_modes=9, energy_range=3.0,
+                 hydrophobic_scale=1.0, ligands_per_launch=0, device=0):
+        s = DockSettings(device, int(fp64), exhaustiveness, seed, num_modes, energy_range, hydrophobic_scale,
+                         ligands_per_launch)
+        self.gd = gd
+        self.h = C.c_void_p(lib.sgpu_docking_create(C.byref(s), receptor.h, C.byref(gd)))
+        err = lib.sgpu_docking_error(self.h)
+        if err:
+            msg = err.decode()
+            lib.sgpu_docking_free(self.h)
+            self.h = None
+            raise RuntimeError(msg)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.sgpu_docking_free(self.h)
+            self.h = None
+
+    def dock(self, ligands):
+        arr = (C.c_void_p * len(ligands))(*[m.h.value for m in ligands])
+        out = (DockResult * len(ligands))()
+        if lib.sgpu_dock(self.h, arr, len(ligands), out) != 0:
+            raise RuntimeError(lib.sgpu_docking_error(self.h).decode())
+        return [dict(e=list(r.e[:r.nposes]), evals=r.evals, seconds=r.seconds) for r in out]

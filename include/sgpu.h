@@ -1,0 +1,276 @@
+/*
+ * sgpu: MI355X (gfx950) docking-search kernels behind a C ABI.
+ *
+ * This is the device layer under the host GpuDocking facade
+ * (gpu-smina_amd/host/gpu_docking.h).  It replaces, one entry point each,
+ * the reference's kernel-level interface src/gpu/gpu_kernels.h:22-126
+ * (eval_confs, run_monte_carlo, run_bfgs, run_refine, residency) and the
+ * device-memory owners of src/gpu/gpu_device.h:59-119 (DeviceTables,
+ * DeviceReceptor, DeviceGrids::build, DeviceModels::upload).  Templates on
+ * float|double become a precision argument.
+ *
+ * Conventions: every call returns an sgpu_status; sgpu_last_error() gives the
+ * message of the last failure on a context.  Host arrays belong to the caller
+ * and are copied; device objects are owned by the context and named by ids.
+ * Calls on one context are serialised by the context.
+ */
+#ifndef SGPU_H
+#define SGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum sgpu_status
+{
+	SGPU_OK = 0,
+	SGPU_E_INVALID = 1,      /* bad argument or inconsistent sizes */
+	SGPU_E_HIP = 2,          /* a HIP runtime call failed */
+	SGPU_E_OOM = 3,          /* device allocation failed */
+	SGPU_E_ATOM_OVERLAP = 4, /* a ligand atom exactly on a receptor atom
+	                            (src/lib/non_cache.cpp:170-172) */
+	SGPU_E_NO_DEVICE = 5,
+	SGPU_E_NOT_FOUND = 6     /* unknown id */
+} sgpu_status;
+
+typedef enum sgpu_precision
+{
+	SGPU_FP32 = 0, /* served search: FP32 geometry, energies and forces */
+	SGPU_FP64 = 1  /* oracle-parity search, ordered like smina's CPU code */
+} sgpu_precision;
+
+enum
+{
+	SGPU_NUM_TYPES = 27,      /* smina atom types, src/lib/atom_constants.h:92-121 */
+	SGPU_NUM_TYPE_PAIRS = 378 /* triangular pairs of them */
+};
+
+/* ---- scoring tables (precalculate_linear, src/lib/precalculate.h:206-237) */
+typedef struct sgpu_tables
+{
+	int n;               /* control points per type pair (2051 at factor 32, cutoff 8) */
+	double factor;       /* points per A^2 */
+	double cutoff_sqr;
+	const double* smooth; /* [378][n][2]: (e, dor) */
+	const double* fast;   /* [378][n] */
+} sgpu_tables;
+
+/* ---- receptor (heavy, typed atoms; model::grid_atoms) */
+typedef struct sgpu_receptor
+{
+	int natoms;
+	const double* xyz; /* [3*natoms] */
+	const int* type;   /* [natoms] smina types */
+} sgpu_receptor;
+
+/* ---- a grid box, grid_dims (src/lib/grid_dim.h): n intervals per axis */
+typedef struct sgpu_grid_dims
+{
+	double begin[3];
+	double end[3];
+	int n[3];
+} sgpu_grid_dims;
+
+/* ---- flat model (FlatModel, src/gpu/gpu_flatten_data.h:16-46), built by the
+ * host facade from a model description.
+ * Atoms 0..natoms-1 move (ligand atoms, then flexible side-chain atoms); atoms
+ * natoms..natoms+ninflex-1 are the fixed atoms of flexible residues.
+ * Frames are in preorder within each tree; trees are the ligands, then the
+ * flexible side chains.  Conf layout per ligand: position[3], orientation
+ * quaternion[4], torsions; per side chain: torsions.  Change layout per
+ * ligand: force-like position[3], torque-like orientation[3], torsions. */
+enum
+{
+	SGPU_FRAME_RIGID_ROOT = 0, /* rigid_body: a ligand root */
+	SGPU_FRAME_FIRST_SEGMENT = 1, /* first_segment: a side-chain root */
+	SGPU_FRAME_SEGMENT = 2     /* segment: one rotatable bond */
+};
+typedef struct sgpu_flat_model
+{
+	int natoms;
+	int ninflex;
+	int nframes;
+	int nligands;
+	int ntrees;
+	int npairs;  /* other_pairs (list 2) first, then ligand-internal pairs (list 0) */
+	int nother;
+	int conf_size;
+	int change_size;
+	int num_tors; /* conf_independent_inputs::num_tors of the ligand(s) */
+	const int* atom_type;     /* [natoms+ninflex] */
+	const int* atom_frame;    /* [natoms] */
+	const double* atom_local; /* [3*natoms] position relative to the frame origin */
+	const double* atom_init;  /* [3*(natoms+ninflex)] model coordinates before the first set */
+	const int* frame_parent;  /* [nframes] -1 for a tree root */
+	const int* frame_kind;    /* [nframes] SGPU_FRAME_* */
+	const int* frame_begin;   /* [nframes] atom range */
+	const int* frame_end;
+	const int* frame_tree;    /* [nframes] */
+	const int* frame_conf;    /* [nframes] conf index: position (root) or torsion */
+	const int* frame_change;  /* [nframes] change index: position (root) or torsion */
+	const double* frame_origin; /* [3*nframes] root: initial origin; first segment:
+	                               origin; segment: origin relative to the parent */
+	const double* frame_axis;   /* [3*nframes] first segment: axis; segment: axis
+	                               relative to the parent; root: unused */
+	const int* pair_a;        /* [npairs] a < b; b may be a fixed atom */
+	const int* pair_b;
+	const int* pair_tri;      /* [npairs] triangular index of (min type, max type) */
+} sgpu_flat_model;
+
+/* ---- search parameters (MCParams, src/gpu/gpu_mc_structs.h:21-34) */
+typedef struct sgpu_mc_params
+{
+	double temperature;        /* 1.2 */
+	double mutation_amplitude; /* 2 */
+	double min_rmsd;           /* 1 */
+	int num_saved_mins;        /* 20 (at most SGPU_MAX_POSES) */
+	double hunt_cap[3];        /* (10, 10, 10) */
+	double authentic_v[3];     /* (1000, 1000, 1000) */
+	double cache_slope;        /* 1e6 */
+	int accurate_line_search;  /* 0: fast_line_search */
+	int early_term;
+} sgpu_mc_params;
+
+enum { SGPU_MAX_POSES = 64 };
+
+/* ---- chains (ChainInput, src/gpu/gpu_kernels.h:34-43) */
+typedef struct sgpu_chains
+{
+	int nchains;
+	const int* model;      /* [nchains] index into the model batch */
+	const double* conf;    /* [nchains*conf_stride] initial conf */
+	int conf_stride;
+	const uint32_t* rng;   /* [nchains*625]: MT words then position */
+	const int* num_steps;  /* [nchains] */
+	const int* maxiters;   /* [nchains] BFGS iterations */
+} sgpu_chains;
+
+/* ---- chain results (ChainResult, src/gpu/gpu_kernels.h:59-64): each
+ * chain's output container, sorted by energy */
+typedef struct sgpu_chain_results
+{
+	int max_poses;     /* poses per chain the arrays hold */
+	int conf_stride;
+	int coord_stride;  /* >= 3 * heavy movable atoms */
+	int* count;        /* [nchains] */
+	double* e;         /* [nchains*max_poses] */
+	double* conf;      /* [nchains*max_poses*conf_stride] */
+	double* coords;    /* [nchains*max_poses*coord_stride] heavy movable atoms */
+	int64_t* evals;    /* [nchains] eval_deriv calls (may be NULL) */
+} sgpu_chain_results;
+
+typedef struct sgpu_ctx sgpu_ctx;
+
+/* context: one per GPU; device = HIP ordinal */
+sgpu_status sgpu_create(int device, sgpu_ctx** out);
+void sgpu_destroy(sgpu_ctx* ctx);
+const char* sgpu_last_error(const sgpu_ctx* ctx);
+/* name and compute-unit count of the context's device */
+sgpu_status sgpu_device_info(sgpu_ctx* ctx, char* name, int name_len, int* cus);
+
+/* DeviceTables::upload (src/gpu/gpu_device.cu:71-90) */
+sgpu_status sgpu_upload_tables(sgpu_ctx* ctx, const sgpu_tables* t);
+
+/* DeviceReceptor::upload (src/gpu/gpu_device.cu:92-110) plus the 3 A cell
+ * lists of flatten_receptor (src/gpu/gpu_flatten.cpp:405-445), built on the
+ * device; cutoff_sqr as the tables' */
+sgpu_status sgpu_upload_receptor(sgpu_ctx* ctx, const sgpu_receptor* r, int* receptor_id);
+
+/* DeviceGrids::build (src/gpu/gpu_device.cu:199-250) = cache::populate
+ * (src/lib/cache.cpp:116-210): one map per listed ligand type, sums in FP64,
+ * stored in `precision`.  slope is the cache slope (1e6). */
+sgpu_status sgpu_build_grids(sgpu_ctx* ctx, int receptor_id, const sgpu_grid_dims* gd,
+		const int* types, int ntypes, int precision, int* grid_id);
+/* copy one type's map back, point layout x fastest, (n0+1)(n1+1)(n2+1) values */
+sgpu_status sgpu_download_grid(sgpu_ctx* ctx, int grid_id, int type, double* out);
+sgpu_status sgpu_release_grid(sgpu_ctx* ctx, int grid_id);
+
+/* DeviceModels::upload (src/gpu/gpu_device.cu:307-423) */
+sgpu_status sgpu_upload_models(sgpu_ctx* ctx, const sgpu_flat_model* models, int nmodels,
+		int precision, int* batch_id);
+sgpu_status sgpu_release_models(sgpu_ctx* ctx, int batch_id);
+
+/* eval_confs (src/gpu/gpu_kernels.h:22-31): model::eval_deriv for each conf.
+ * grid_id >= 0: cache back end (the docking search); grid_id < 0: direct
+ * receptor (non_cache) back end in box gd with `slope`.  v: curl caps.
+ * Outputs e[nconfs], g[nconfs*change_stride], coords[nconfs*coord_stride]
+ * (all movable atoms; may be NULL). */
+typedef struct sgpu_eval_request
+{
+	int batch_id;
+	int grid_id;
+	int receptor_id;
+	const sgpu_grid_dims* gd; /* non_cache box */
+	double slope;
+	double v[3];
+	int nconfs;
+	const int* model;
+	const double* conf;
+	int conf_stride;
+	double* e;
+	double* g;
+	int change_stride;
+	double* coords;
+	int coord_stride;
+} sgpu_eval_request;
+sgpu_status sgpu_eval_confs(sgpu_ctx* ctx, const sgpu_eval_request* req);
+
+/* run_monte_carlo (src/gpu/gpu_kernels.h:66-72): one chain per wavefront,
+ * monte_carlo::operator() (src/lib/monte_carlo.cpp:95-130) for each chain */
+sgpu_status sgpu_run_mc(sgpu_ctx* ctx, int batch_id, int grid_id, const sgpu_mc_params* p,
+		const sgpu_chains* chains, sgpu_chain_results* out);
+
+/* run_refine (src/gpu/gpu_kernels.h:100-110): refine_structure
+ * (src/main/main.cpp:165-191) of every pose on the non_cache back end with
+ * box gd; conf is updated in place; e = max double if the pose stays out of
+ * the box; coords of heavy movable atoms. */
+typedef struct sgpu_refine_request
+{
+	int batch_id;
+	int receptor_id;
+	const sgpu_grid_dims* gd;
+	double v[3];
+	int accurate_line_search;
+	int early_term;
+	int nposes;
+	const int* model;
+	double* conf;
+	int conf_stride;
+	const int* maxiters;
+	double* e;
+	double* coords;
+	int coord_stride;
+} sgpu_refine_request;
+sgpu_status sgpu_run_refine(sgpu_ctx* ctx, const sgpu_refine_request* req);
+
+/* final rescoring of finish_docking (src/main/main.cpp:256-324) in FP64:
+ * mode 0: model::eval (src/lib/model.cpp:931-953) with the non_cache back
+ *         end (slope) and fast tables;
+ * mode 1: model::eval_intramolecular (src/lib/model.cpp:1028-1083). */
+typedef struct sgpu_score_request
+{
+	int batch_id;
+	int receptor_id;
+	const sgpu_grid_dims* gd;
+	double slope;
+	double v[3];
+	int mode;
+	int nconfs;
+	const int* model;
+	const double* conf;
+	int conf_stride;
+	double* e;
+} sgpu_score_request;
+sgpu_status sgpu_score(sgpu_ctx* ctx, const sgpu_score_request* req);
+
+/* timing of the last launch of each kind, for measurement: milliseconds of
+ * the kernel measured with HIP events on the launch stream */
+sgpu_status sgpu_last_kernel_ms(sgpu_ctx* ctx, const char* kind, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,101 @@
+/*
+ * C entry points of the host side of the docking path (the C++ facade in
+ * gpu-smina_amd/host): seeded synthetic inputs, scoring tables, model
+ * flattening and the docking driver.  The C++ interface a smina build
+ * calls is the GpuDocking class (gpu-smina_amd/host/gpu_docking.h), which
+ * mirrors src/gpu/gpu_docking.h:51-114; these functions wrap it for bindings
+ * (Python ctypes in tests and the benchmark).
+ */
+#ifndef SGPU_HOST_H
+#define SGPU_HOST_H
+
+#include <stdint.h>
+#include "sgpu.h"
+#include "sgpu_model.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- molecules (owned by the library, freed with sgpu_mol_free) */
+typedef struct sgpu_mol sgpu_mol;
+void sgpu_mol_free(sgpu_mol* m);
+/* the molecule as a model description (pointers valid until freed) */
+const sgpu_model_desc* sgpu_mol_desc(const sgpu_mol* m);
+
+/* Receptor: nheavy heavy atoms as self-avoiding random-walk chains (1.5 A
+ * steps, >= 1.3 A apart) filling a shell around a spherical pocket of
+ * radius pocket_radius at the origin; element mix C 63 / N 17 / O 19 / S 1 %
+ * with polar hydrogens on donors.  Atoms: heavy and polar H, typed. */
+sgpu_mol* sgpu_synth_receptor(uint32_t seed, int nheavy, double pocket_radius);
+/* Ligand: nheavy heavy atoms as a tree with ntors rotatable bonds, C 70 /
+ * N 15 / O 15 %, polar hydrogens on donors, placed at a random pose near
+ * `center`. */
+sgpu_mol* sgpu_synth_ligand(uint32_t seed, int nheavy, int ntors, const double* center);
+/* Flexible side chains: nres residues of the receptor near the pocket made
+ * movable, with 2-4 chi torsions each; returns a model holding `lig`'s atoms
+ * followed by the side-chain atoms (and their fixed atoms), and replaces
+ * the receptor's atoms by the rest (receptor_out, freed by the caller). */
+sgpu_mol* sgpu_synth_flex_model(uint32_t seed, const sgpu_mol* receptor, const sgpu_mol* lig, int nres,
+		double max_dist, sgpu_mol** receptor_out);
+
+/* ---- scoring */
+/* the default Vina terms (src/main/main.cpp:1347-1353); hydrophobic_scale
+ * multiplies the hydrophobic weight (1 for Vina) */
+void sgpu_vina_scoring(sgpu_term* terms5, sgpu_scoring* out, double hydrophobic_scale);
+/* precalculate_linear(sf, factor) tables, FP64: smooth [378][n][2], fast
+ * [378][n]; returns n (call with NULL buffers to get n) */
+int sgpu_build_tables(const sgpu_scoring* sf, double factor, double* smooth, double* fast);
+
+/* ---- docking (GpuDocking::dock_batch, src/gpu/gpu_docking.cpp:756-831) */
+typedef struct sgpu_docking sgpu_docking;
+typedef struct sgpu_dock_settings
+{
+	int device;
+	int fp64;            /* --gpu_fp64 */
+	int exhaustiveness;  /* 8 */
+	int seed;            /* --seed */
+	int num_modes;       /* 9 */
+	double energy_range; /* 3 */
+	double hydrophobic_scale; /* custom scoring: Vina with this factor on hydrophobic */
+	int ligands_per_launch; /* batching: 0 = all jobs in one launch */
+} sgpu_dock_settings;
+
+sgpu_docking* sgpu_docking_create(const sgpu_dock_settings* s, const sgpu_mol* receptor,
+		const sgpu_grid_dims* gd);
+void sgpu_docking_free(sgpu_docking* d);
+const char* sgpu_docking_error(const sgpu_docking* d);
+
+typedef struct sgpu_dock_result
+{
+	int nposes;
+	double e[32];
+	int64_t evals;   /* eval_deriv calls in search and refinement */
+	double seconds;  /* wall time of the job on the device side */
+} sgpu_dock_result;
+
+/* dock each ligand model (search, merge, refine, final rescoring) */
+int sgpu_dock(sgpu_docking* d, const sgpu_mol* const* ligands, int n, sgpu_dock_result* out);
+
+/* low-level access for tests: the context and the device ids of the
+ * receptor, grids and tables this docking object built */
+sgpu_ctx* sgpu_docking_ctx(sgpu_docking* d);
+int sgpu_docking_receptor_id(sgpu_docking* d);
+
+/* flatten a model description for the device layer; the returned object
+ * owns the arrays sgpu_flat_model points to */
+typedef struct sgpu_flat sgpu_flat;
+sgpu_flat* sgpu_flatten(const sgpu_model_desc* d);
+const sgpu_flat_model* sgpu_flat_get(const sgpu_flat* f);
+void sgpu_flat_free(sgpu_flat* f);
+/* initial conf (get_initial_conf) and chain seeding (make_chain): conf and
+ * the generator state after conf::randomize with rng(seed) */
+void sgpu_initial_conf(const sgpu_flat* f, double* conf);
+void sgpu_make_chain(const sgpu_flat* f, uint32_t seed, const sgpu_grid_dims* gd, double* conf,
+		uint32_t* rng_state);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

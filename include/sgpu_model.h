@@ -1,0 +1,85 @@
+/*
+ * Inputs of the docking path, as plain arrays: a typed, tree-structured
+ * molecule (what smina's PDBQT parser builds into `model`, src/lib/model.h
+ * and src/lib/parse_pdbqt.cpp:255-330), a receptor and a scoring function
+ * (src/lib/custom_terms.cpp, the default Vina terms of
+ * src/main/main.cpp:1347-1353).
+ *
+ * The host facade (gpu-smina_amd/host) flattens a description into an
+ * sgpu_flat_model; the CPU oracle builds its own smina-like model from the
+ * same description.
+ */
+#ifndef SGPU_MODEL_H
+#define SGPU_MODEL_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* A model: ligand(s) plus optional flexible side chains.
+ * Atoms 0..nmovable-1 move: each tree's atoms are contiguous, frames of a
+ * tree are in preorder and own contiguous atom ranges (smina's atom order).
+ * Atoms nmovable..natoms-1 are fixed atoms of flexible residues ("inflex").
+ * Frame origins are input positions of an atom: the first root atom for a
+ * ligand root; the first atom of the branch (the moving end of the bond) for
+ * a segment.  Each segment and side-chain root rotates about the bond from
+ * axis_atom (on the parent side) to its origin atom. */
+typedef struct sgpu_model_desc
+{
+	int natoms;
+	int nmovable;
+	const int* type;       /* [natoms] smina atom types */
+	const double* xyz;     /* [3*natoms] */
+	int nbonds;
+	const int* bonds;      /* [2*nbonds] covalent bonds (atom indices) */
+	int nframes;
+	const int* frame_parent;      /* [nframes] -1 for a tree root */
+	const int* frame_begin;       /* [nframes] */
+	const int* frame_end;         /* [nframes] */
+	const int* frame_origin_atom; /* [nframes] */
+	const int* frame_axis_atom;   /* [nframes] -1 for a ligand root */
+	int nligands;                 /* the first nligands trees are ligands */
+	int ntrees;
+	const int* tree_frame;        /* [ntrees] first frame of each tree */
+} sgpu_model_desc;
+
+/* Scoring terms (src/lib/everything.h).  Parameters p[] in the order of the
+ * term's name, e.g. gauss(o, w, c), repulsion(o, c), hydrophobic(g, b, c),
+ * non_hydrophobic(g, b, c), non_dir_h_bond(g, b, c),
+ * non_dir_anti_h_bond_quadratic(o, c), donor_donor_quadratic(o, c),
+ * acceptor_acceptor_quadratic(o, c), vdw(i, j, s, cap, c),
+ * non_dir_h_bond_lj(o, cap, c). */
+typedef enum sgpu_term_kind
+{
+	SGPU_TERM_GAUSS = 0,
+	SGPU_TERM_REPULSION = 1,
+	SGPU_TERM_HYDROPHOBIC = 2,
+	SGPU_TERM_NON_HYDROPHOBIC = 3,
+	SGPU_TERM_NON_DIR_H_BOND = 4,
+	SGPU_TERM_NON_DIR_ANTI_H_BOND_QUADRATIC = 5,
+	SGPU_TERM_DONOR_DONOR_QUADRATIC = 6,
+	SGPU_TERM_ACCEPTOR_ACCEPTOR_QUADRATIC = 7,
+	SGPU_TERM_VDW = 8,
+	SGPU_TERM_NON_DIR_H_BOND_LJ = 9
+} sgpu_term_kind;
+
+typedef struct sgpu_term
+{
+	int kind;
+	double p[5];
+	double weight;
+} sgpu_term;
+
+typedef struct sgpu_scoring
+{
+	int nterms;
+	const sgpu_term* terms; /* charge-independent terms, in weight order */
+	int has_num_tors_div;   /* conf-independent num_tors_div term */
+	double num_tors_div_weight;
+} sgpu_scoring;
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

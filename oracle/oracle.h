@@ -1,0 +1,100 @@
+/*
+ * TEST INFRASTRUCTURE ONLY.  The CPU oracle: a plain FP64 C++ restatement of
+ * smina's docking search (the CPU code the reference's --gpu path replaces),
+ * used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * as the checker.  The product path never links or calls it.
+ *
+ * Each function cites the reference file:line it restates.  Built with
+ * -ffp-contract=off and the project's portable sin/cos/exp/acos so that the
+ * FP64 device search can be compared with it bit for bit.
+ *
+ * Parity pinning (tests/test_oracle_*.py): MT19937 against libstdc++'s
+ * std::mt19937 and the C++ standard's known answer; uniform draws against
+ * their algorithm's range/rejection invariants; portable math against glibc
+ * within 1 ulp; terms and tables against closed forms; trees against the
+ * zero-conf identity and rigid-motion invariants; eval_deriv gradients
+ * against central finite differences; grids against brute-force summation;
+ * BFGS against the exact minimum of a convex quadratic; MC against the
+ * steps = 1 identity.
+ * Parity unpinned: the normal sampler used for initial orientations
+ * (Boost's normal_distribution cannot be run here; this oracle and the
+ * product share one documented polar-method sampler instead).
+ */
+#ifndef SGPU_ORACLE_H
+#define SGPU_ORACLE_H
+
+#include <stdint.h>
+#include "../include/sgpu.h"
+#include "../include/sgpu_model.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_ctx orc_ctx;
+
+/* scoring + receptor + box; tables at factor 32 (precalculate_linear(wt, 32),
+ * src/main/main.cpp:1411-1421) */
+orc_ctx* orc_create(const sgpu_scoring* sf, const sgpu_receptor* rec, const sgpu_grid_dims* gd);
+void orc_destroy(orc_ctx* c);
+const char* orc_error(orc_ctx* c);
+
+/* table access: smooth[(tri*n + i)*2 + {0,1}], fast[tri*n + i] */
+int orc_table_n(orc_ctx* c);
+void orc_tables(orc_ctx* c, double* smooth, double* fast);
+/* one charge-independent weighted sum of terms (weighted_terms::eval_fast) */
+double orc_eval_terms(orc_ctx* c, int t1, int t2, double r);
+
+/* cache::populate for the listed types; maps in point layout (x fastest) */
+int orc_build_grids(orc_ctx* c, const int* types, int ntypes);
+int orc_grid_values(orc_ctx* c, int type, double* out);
+
+/* models */
+int orc_add_model(orc_ctx* c, const sgpu_model_desc* d);
+/* info[0..9]: movable atoms, all atoms, frames, pairs, other pairs, conf size,
+ * change size, heavy movable atoms, ligands, trees */
+void orc_model_info(orc_ctx* c, int model, int* info);
+void orc_initial_conf(orc_ctx* c, int model, double* conf);
+/* the pairs in evaluation order: a, b, list (0 ligand, 2 other) */
+void orc_model_pairs(orc_ctx* c, int model, int* a, int* b, int* list);
+double orc_num_tors(orc_ctx* c, int model);
+
+/* model::eval_deriv (src/lib/model.cpp:969-983); backend 0 = cache (grids),
+ * 1 = non_cache with `slope`.  coords may be NULL.  Returns e. */
+double orc_eval_deriv(orc_ctx* c, int model, int backend, double slope, const double* v,
+		const double* conf, double* g, double* coords);
+/* model::eval (non_cache, fast tables) and eval_intramolecular */
+double orc_eval(orc_ctx* c, int model, double slope, const double* v, const double* conf);
+double orc_eval_intramolecular(orc_ctx* c, int model, const double* v, const double* conf);
+
+/* bfgs (src/lib/bfgs.h:201-282) with the given back end; conf in/out;
+ * returns f; *evals counts eval_deriv calls */
+double orc_bfgs(orc_ctx* c, int model, int backend, double slope, const double* v,
+		int maxiters, int accurate, int early_term, double* conf, int64_t* evals);
+
+/* chain initialisation (make_chain / conf::randomize): seeds rng(seed),
+ * randomizes a conf in the box, returns the generator state (625 words) */
+void orc_randomize(orc_ctx* c, int model, uint32_t seed, double* conf, uint32_t* rng_state);
+
+/* monte_carlo::operator() (src/lib/monte_carlo.cpp:95-130) on the cache back
+ * end from a given start conf and generator state.  Output container sorted:
+ * count, e[], conf[] (stride conf_size), coords[] (stride 3*heavy). */
+int orc_mc_chain(orc_ctx* c, int model, const sgpu_mc_params* p, int num_steps, int maxiters,
+		const double* conf0, const uint32_t* rng_state, int max_out, double* e, double* conf,
+		double* coords, int64_t* evals);
+
+/* refine_structure (src/main/main.cpp:165-191); conf in/out; returns e */
+double orc_refine(orc_ctx* c, int model, const double* v, int maxiters, double* conf, double* coords);
+
+/* the whole docking of one ligand: parallel_mc (src/lib/parallel_mc.cpp:80-98)
+ * + refine + finish_docking (src/main/main.cpp:256-324), single-threaded or on
+ * `threads` host threads.  Results sorted: count, energies, confs. */
+int orc_dock(orc_ctx* c, int model, const sgpu_mc_params* p, int seed, int exhaustiveness,
+		int num_modes, double energy_range, int threads, int max_out, double* e, double* conf,
+		int64_t* evals);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

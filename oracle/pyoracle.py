@@ -1,0 +1,209 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes binding of the CPU oracle (liboracle.so).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it,
+as the checker.  Model descriptions, scoring and boxes are passed as the
+same C structs the product takes (include/sgpu.h, include/sgpu_model.h).
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+F64P = C.POINTER(C.c_double)
+I32P = C.POINTER(C.c_int)
+U32P = C.POINTER(C.c_uint32)
+I64P = C.POINTER(C.c_int64)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    sig = {
+        "orc_create": (vp, [vp, vp, vp]),
+        "orc_destroy": (None, [vp]),
+        "orc_error": (C.c_char_p, [vp]),
+        "orc_table_n": (C.c_int, [vp]),
+        "orc_tables": (None, [vp, F64P, F64P]),
+        "orc_eval_terms": (C.c_double, [vp, C.c_int, C.c_int, C.c_double]),
+        "orc_build_grids": (C.c_int, [vp, I32P, C.c_int]),
+        "orc_grid_values": (C.c_int, [vp, C.c_int, F64P]),
+        "orc_add_model": (C.c_int, [vp, vp]),
+        "orc_model_info": (None, [vp, C.c_int, I32P]),
+        "orc_initial_conf": (None, [vp, C.c_int, F64P]),
+        "orc_model_pairs": (None, [vp, C.c_int, I32P, I32P, I32P]),
+        "orc_num_tors": (C.c_double, [vp, C.c_int]),
+        "orc_eval_deriv": (C.c_double, [vp, C.c_int, C.c_int, C.c_double, F64P, F64P, F64P, F64P]),
+        "orc_eval": (C.c_double, [vp, C.c_int, C.c_double, F64P, F64P]),
+        "orc_eval_intramolecular": (C.c_double, [vp, C.c_int, F64P, F64P]),
+        "orc_bfgs": (C.c_double, [vp, C.c_int, C.c_int, C.c_double, F64P, C.c_int, C.c_int, C.c_int, F64P, I64P]),
+        "orc_randomize": (None, [vp, C.c_int, C.c_uint32, F64P, U32P]),
+        "orc_mc_chain": (C.c_int, [vp, C.c_int, vp, C.c_int, C.c_int, F64P, U32P, C.c_int, F64P, F64P, F64P,
+                                   I64P]),
+        "orc_refine": (C.c_double, [vp, C.c_int, F64P, C.c_int, F64P, F64P]),
+        "orc_dock": (C.c_int, [vp, C.c_int, vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, F64P,
+                               F64P, I64P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+class Oracle:
+    """One scoring function + receptor + box, with models added to it."""
+
+    def __init__(self, scoring, receptor_xyz, receptor_types, gd):
+        # scoring, gd: the product binding's ctypes structs (same C layout)
+        self._xyz = np.ascontiguousarray(receptor_xyz, dtype=np.float64).reshape(-1)
+        self._types = np.ascontiguousarray(receptor_types, dtype=np.int32)
+
+        class Rec(C.Structure):
+            _fields_ = [("natoms", C.c_int), ("xyz", F64P), ("type", I32P)]
+
+        self._rec = Rec(len(self._types), _p(self._xyz, F64P), _p(self._types, I32P))
+        self._sf = scoring
+        self._gd = gd
+        self.h = C.c_void_p(lib.orc_create(C.addressof(scoring), C.addressof(self._rec), C.addressof(gd)))
+        self.n = lib.orc_table_n(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.orc_destroy(self.h)
+            self.h = None
+
+    def error(self):
+        return lib.orc_error(self.h).decode()
+
+    def tables(self):
+        smooth = np.zeros(378 * self.n * 2)
+        fast = np.zeros(378 * self.n)
+        lib.orc_tables(self.h, _p(smooth, F64P), _p(fast, F64P))
+        return smooth, fast
+
+    def eval_terms(self, t1, t2, r):
+        return lib.orc_eval_terms(self.h, t1, t2, r)
+
+    def build_grids(self, types):
+        t = np.ascontiguousarray(types, dtype=np.int32)
+        if lib.orc_build_grids(self.h, _p(t, I32P), len(t)) != 0:
+            raise RuntimeError(self.error())
+
+    def grid(self, t):
+        gd = self._gd
+        out = np.zeros((gd.n[0] + 1) * (gd.n[1] + 1) * (gd.n[2] + 1))
+        if lib.orc_grid_values(self.h, t, _p(out, F64P)) != 0:
+            raise RuntimeError("no grid for type %d" % t)
+        return out
+
+    def add_model(self, desc_ptr):
+        m = lib.orc_add_model(self.h, C.cast(desc_ptr, C.c_void_p))
+        if m < 0:
+            raise RuntimeError(self.error())
+        return m
+
+    def info(self, m):
+        a = np.zeros(10, np.int32)
+        lib.orc_model_info(self.h, m, _p(a, I32P))
+        keys = ["natoms", "all_atoms", "nframes", "npairs", "nother", "conf_size", "change_size", "nheavy",
+                "nligands", "ntrees"]
+        return dict(zip(keys, a.tolist()))
+
+    def pairs(self, m):
+        n = self.info(m)["npairs"]
+        a, b, lst = (np.zeros(n, np.int32) for _ in range(3))
+        lib.orc_model_pairs(self.h, m, _p(a, I32P), _p(b, I32P), _p(lst, I32P))
+        return a, b, lst
+
+    def num_tors(self, m):
+        return lib.orc_num_tors(self.h, m)
+
+    def initial_conf(self, m):
+        c = np.zeros(self.info(m)["conf_size"])
+        lib.orc_initial_conf(self.h, m, _p(c, F64P))
+        return c
+
+    def eval_deriv(self, m, conf, v, backend=0, slope=1e6):
+        info = self.info(m)
+        conf = np.ascontiguousarray(conf, dtype=np.float64)
+        g = np.zeros(info["change_size"])
+        x = np.zeros(3 * info["natoms"])
+        vv = np.ascontiguousarray(v, dtype=np.float64)
+        e = lib.orc_eval_deriv(self.h, m, backend, slope, _p(vv, F64P), _p(conf, F64P), _p(g, F64P), _p(x, F64P))
+        if np.isnan(e):
+            raise RuntimeError(self.error())
+        return e, g, x
+
+    def eval(self, m, conf, v=(1000.0, 1000.0, 1000.0), slope=1e6):
+        conf = np.ascontiguousarray(conf, dtype=np.float64)
+        vv = np.ascontiguousarray(v, dtype=np.float64)
+        return lib.orc_eval(self.h, m, slope, _p(vv, F64P), _p(conf, F64P))
+
+    def eval_intramolecular(self, m, conf, v=(1000.0, 1000.0, 1000.0)):
+        conf = np.ascontiguousarray(conf, dtype=np.float64)
+        vv = np.ascontiguousarray(v, dtype=np.float64)
+        return lib.orc_eval_intramolecular(self.h, m, _p(vv, F64P), _p(conf, F64P))
+
+    def bfgs(self, m, conf, v, maxiters, backend=0, slope=1e6, accurate=False):
+        conf = np.ascontiguousarray(conf, dtype=np.float64).copy()
+        vv = np.ascontiguousarray(v, dtype=np.float64)
+        ev = np.zeros(1, np.int64)
+        f = lib.orc_bfgs(self.h, m, backend, slope, _p(vv, F64P), maxiters, int(accurate), 0, _p(conf, F64P),
+                         _p(ev, I64P))
+        return f, conf, int(ev[0])
+
+    def randomize(self, m, seed):
+        c = np.zeros(self.info(m)["conf_size"])
+        s = np.zeros(625, np.uint32)
+        lib.orc_randomize(self.h, m, seed, _p(c, F64P), _p(s, U32P))
+        return c, s
+
+    def mc_chain(self, m, params, steps, maxiters, conf0, rng_state, max_out=20):
+        info = self.info(m)
+        conf0 = np.ascontiguousarray(conf0, dtype=np.float64)
+        rng_state = np.ascontiguousarray(rng_state, dtype=np.uint32)
+        e = np.zeros(max_out)
+        c = np.zeros((max_out, info["conf_size"]))
+        x = np.zeros((max_out, 3 * info["nheavy"]))
+        ev = np.zeros(1, np.int64)
+        k = lib.orc_mc_chain(self.h, m, C.addressof(params), steps, maxiters, _p(conf0, F64P), _p(rng_state, U32P),
+                             max_out, _p(e, F64P), _p(c, F64P), _p(x, F64P), _p(ev, I64P))
+        if k < 0:
+            raise RuntimeError(self.error())
+        return e[:k], c[:k], x[:k], int(ev[0])
+
+    def refine(self, m, conf, maxiters, v=(1000.0, 1000.0, 1000.0)):
+        info = self.info(m)
+        conf = np.ascontiguousarray(conf, dtype=np.float64).copy()
+        vv = np.ascontiguousarray(v, dtype=np.float64)
+        x = np.zeros(3 * info["nheavy"])
+        e = lib.orc_refine(self.h, m, _p(vv, F64P), maxiters, _p(conf, F64P), _p(x, F64P))
+        return e, conf, x
+
+    def dock(self, m, params, seed=1, exhaustiveness=8, num_modes=9, energy_range=3.0, threads=1, max_out=32):
+        info = self.info(m)
+        e = np.zeros(max_out)
+        c = np.zeros((max_out, info["conf_size"]))
+        ev = np.zeros(1, np.int64)
+        k = lib.orc_dock(self.h, m, C.addressof(params), seed, exhaustiveness, num_modes, energy_range, threads,
+                         max_out, _p(e, F64P), _p(c, F64P), _p(ev, I64P))
+        if k < 0:
+            raise RuntimeError(self.error())
+        return e[:k], c[:k], int(ev[0])
