@@ -621,6 +621,16 @@ void GpuDocking::ensure_grids(const std::vector<int>& types)
 	grid_types_ = all;
 }
 
+void GpuDocking::prepare(const std::vector<const sgpu_model_desc*>& models)
+{
+	std::vector<int> types;
+	for (const sgpu_model_desc* d : models)
+		for (int a = 0; a < d->nmovable; a++)
+			if (!is_hydrogen_type(d->type[a]) && std::find(types.begin(), types.end(), d->type[a]) == types.end())
+				types.push_back(d->type[a]);
+	ensure_grids(types);
+}
+
 void GpuDocking::dock_batch(std::vector<DockJob*>& jobs)
 {
 	const size_t per = s_.ligands_per_launch > 0 ? size_t(s_.ligands_per_launch) : jobs.size();
@@ -959,6 +969,25 @@ void sgpu_docking_free(sgpu_docking* d) { delete d; }
 const char* sgpu_docking_error(const sgpu_docking* d) { return d->err.c_str(); }
 sgpu_ctx* sgpu_docking_ctx(sgpu_docking* d) { return d->d ? d->d->ctx() : nullptr; }
 int sgpu_docking_receptor_id(sgpu_docking* d) { return d->d ? d->d->receptor_id() : -1; }
+
+int sgpu_docking_prepare(sgpu_docking* d, const sgpu_mol* const* ligands, int n)
+{
+	if (!d->d)
+		return -1;
+	try
+	{
+		std::vector<const sgpu_model_desc*> ms;
+		for (int i = 0; i < n; i++)
+			ms.push_back(sgpu_mol_desc(ligands[i]));
+		d->d->prepare(ms);
+		return 0;
+	}
+	catch (std::exception& e)
+	{
+		d->err = e.what();
+		return -1;
+	}
+}
 
 int sgpu_dock(sgpu_docking* d, const sgpu_mol* const* ligands, int n, sgpu_dock_result* out)
 {

@@ -105,6 +105,8 @@ public:
 	// src/main/main.cpp:256-324)
 	void dock_batch(std::vector<DockJob*>& jobs);
 
+	// receptor maps for every heavy-atom type of these models
+	void prepare(const std::vector<const sgpu_model_desc*>& models);
 	sgpu_ctx* ctx() { return ctx_; }
 	int receptor_id() const { return receptor_; }
 	std::string description() const;

@@ -148,6 +148,7 @@ def _load():
         "sgpu_docking_free": (None, [vp]),
         "sgpu_docking_error": (C.c_char_p, [vp]),
         "sgpu_dock": (C.c_int, [vp, C.POINTER(vp), C.c_int, C.POINTER(DockResult)]),
+        "sgpu_docking_prepare": (C.c_int, [vp, C.POINTER(vp), C.c_int]),
         "sgpu_docking_ctx": (vp, [vp]),
         "sgpu_docking_receptor_id": (C.c_int, [vp]),
     }
@@ -460,6 +461,18 @@ class Docking:
         if getattr(self, "h", None):
             lib.sgpu_docking_free(self.h)
             self.h = None
+
+    def prepare(self, ligands):
+        arr = (C.c_void_p * len(ligands))(*[m.h.value for m in ligands])
+        if lib.sgpu_docking_prepare(self.h, arr, len(ligands)) != 0:
+            raise RuntimeError(lib.sgpu_docking_error(self.h).decode())
+
+    def kernel_ms(self, kind):
+        ms = C.c_double()
+        ctx = lib.sgpu_docking_ctx(self.h)
+        if lib.sgpu_last_kernel_ms(ctx, kind.encode(), C.byref(ms)) != OK:
+            raise RuntimeError(lib.sgpu_last_error(ctx).decode())
+        return ms.value
 
     def dock(self, ligands):
         arr = (C.c_void_p * len(ligands))(*[m.h.value for m in ligands])

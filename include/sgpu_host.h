@@ -32,12 +32,6 @@ sgpu_mol* sgpu_synth_receptor(uint32_t seed, int nheavy, double pocket_radius);
  * N 15 / O 15 %, polar hydrogens on donors, placed at a random pose near
  * `center`. */
 sgpu_mol* sgpu_synth_ligand(uint32_t seed, int nheavy, int ntors, const double* center);
-/* Flexible side chains: nres residues of the receptor near the pocket made
- * movable, with 2-4 chi torsions each; returns a model holding `lig`'s atoms
- * followed by the side-chain atoms (and their fixed atoms), and replaces
- * the receptor's atoms by the rest (receptor_out, freed by the caller). */
-sgpu_mol* sgpu_synth_flex_model(uint32_t seed, const sgpu_mol* receptor, const sgpu_mol* lig, int nres,
-		double max_dist, sgpu_mol** receptor_out);
 
 /* ---- scoring */
 /* the default Vina terms (src/main/main.cpp:1347-1353); hydrophobic_scale
@@ -74,6 +68,9 @@ typedef struct sgpu_dock_result
 	double seconds;  /* wall time of the job on the device side */
 } sgpu_dock_result;
 
+/* build the receptor maps for every heavy-atom type of these ligands (the
+ * maps are otherwise built on first use) */
+int sgpu_docking_prepare(sgpu_docking* d, const sgpu_mol* const* ligands, int n);
 /* dock each ligand model (search, merge, refine, final rescoring) */
 int sgpu_dock(sgpu_docking* d, const sgpu_mol* const* ligands, int n, sgpu_dock_result* out);
 

@@ -1030,6 +1030,8 @@ struct Model
 struct ConfOps
 {
 	const Model* m;
+	int change_size() const { return m->change_size; }
+	int conf_size() const { return m->conf_size; }
 	// conf::increment (src/lib/conf.h:316-321, 168-172, 73-77)
 	void increment(fl* x, const fl* c, fl factor) const
 	{
@@ -1500,7 +1502,8 @@ bool bfgs_update(Hess& h, const fl* p, const fl* y, fl alpha)
 	return true;
 }
 // bfgs.h:76-96
-fl fast_line_search(const Objective& f, const ConfOps& ops, int n, const fl* x, const fl* g, fl f0,
+template<typename F, typename Ops>
+fl fast_line_search(const F& f, const Ops& ops, int n, const fl* x, const fl* g, fl f0,
 		const fl* p, fl* x_new, fl* g_new, fl& f1, int conf_size)
 {
 	const fl c0 = 0.0001;
@@ -1520,7 +1523,8 @@ fl fast_line_search(const Objective& f, const ConfOps& ops, int n, const fl* x, 
 	return alpha;
 }
 // bfgs.h:101-190
-fl accurate_line_search(const Objective& f, const ConfOps& ops, int n, const fl* x, const fl* g, fl f0,
+template<typename F, typename Ops>
+fl accurate_line_search(const F& f, const Ops& ops, int n, const fl* x, const fl* g, fl f0,
 		const fl* p, fl* x_new, fl* g_new, fl& f1, int conf_size)
 {
 	fl a, alpha, alpha2 = 0, alamin, b, disc, f2 = 0;
@@ -1590,10 +1594,11 @@ fl accurate_line_search(const Objective& f, const ConfOps& ops, int n, const fl*
 	}
 }
 // bfgs.h:201-282
-fl bfgs(const Objective& f, const ConfOps& ops, fl* x, fl* g, int maxiters, bool accurate, bool early_term)
+template<typename F, typename Ops>
+fl bfgs(const F& f, const Ops& ops, fl* x, fl* g, int maxiters, bool accurate, bool early_term)
 {
-	const int n = ops.m->change_size;
-	const int cs = ops.m->conf_size;
+	const int n = ops.change_size();
+	const int cs = ops.conf_size();
 	Hess h{n, std::vector<fl>(size_t(n) * (n + 1) / 2, 0)};
 	for (int i = 0; i < n; i++)
 		h.at(i, i) = 1;
@@ -1648,6 +1653,46 @@ fl bfgs(const Objective& f, const ConfOps& ops, fl* x, fl* g, int maxiters, bool
 	}
 	return f0;
 }
+
+// a plain vector space: x += alpha * p (for pinning BFGS on a quadratic)
+struct FlatOps
+{
+	int n;
+	int change_size() const { return n; }
+	int conf_size() const { return n; }
+	void increment(fl* x, const fl* c, fl factor) const
+	{
+		for (int i = 0; i < n; i++)
+		{
+			const fl d = factor * c[i];
+			x[i] += d;
+		}
+	}
+	fl value(const fl* x, int i) const { return x[i]; }
+};
+// f(x) = 1/2 x^T A x - b^T x, g = A x - b
+struct Quadratic
+{
+	int n;
+	const fl* A;
+	const fl* b;
+	int64_t* evals;
+	fl operator()(const fl* x, fl* g) const
+	{
+		if (evals)
+			++*evals;
+		fl f = 0;
+		for (int i = 0; i < n; i++)
+		{
+			fl ax = 0;
+			for (int j = 0; j < n; j++)
+				ax += A[i * n + j] * x[j];
+			g[i] = ax - b[i];
+			f += 0.5 * x[i] * ax - b[i] * x[i];
+		}
+		return f;
+	}
+};
 
 // ---------------------------------------------------------------- MC
 struct Output
@@ -2115,6 +2160,15 @@ int orc_mc_chain(orc_ctx* c, int model, const sgpu_mc_params* p, int num_steps, 
 }
 
 extern "C" {
+
+double orc_bfgs_quadratic(int n, const double* A, const double* b, double* x, int maxiters, int accurate,
+		int64_t* evals)
+{
+	Quadratic q{n, A, b, evals};
+	FlatOps ops{n};
+	std::vector<double> g(n, 0);
+	return bfgs(q, ops, x, g.data(), maxiters, accurate != 0, false);
+}
 
 double orc_refine(orc_ctx* c, int model, const double* v, int maxiters, double* conf, double* coords)
 {

@@ -72,6 +72,10 @@ double orc_eval_intramolecular(orc_ctx* c, int model, const double* v, const dou
 double orc_bfgs(orc_ctx* c, int model, int backend, double slope, const double* v,
 		int maxiters, int accurate, int early_term, double* conf, int64_t* evals);
 
+/* the same bfgs on f(x) = x'Ax/2 - b'x in R^n (x in/out); returns f */
+double orc_bfgs_quadratic(int n, const double* A, const double* b, double* x, int maxiters, int accurate,
+		int64_t* evals);
+
 /* chain initialisation (make_chain / conf::randomize): seeds rng(seed),
  * randomizes a conf in the box, returns the generator state (625 words) */
 void orc_randomize(orc_ctx* c, int model, uint32_t seed, double* conf, uint32_t* rng_state);

@@ -46,6 +46,7 @@ def _load():
         "orc_eval": (C.c_double, [vp, C.c_int, C.c_double, F64P, F64P]),
         "orc_eval_intramolecular": (C.c_double, [vp, C.c_int, F64P, F64P]),
         "orc_bfgs": (C.c_double, [vp, C.c_int, C.c_int, C.c_double, F64P, C.c_int, C.c_int, C.c_int, F64P, I64P]),
+        "orc_bfgs_quadratic": (C.c_double, [C.c_int, F64P, F64P, F64P, C.c_int, C.c_int, I64P]),
         "orc_randomize": (None, [vp, C.c_int, C.c_uint32, F64P, U32P]),
         "orc_mc_chain": (C.c_int, [vp, C.c_int, vp, C.c_int, C.c_int, F64P, U32P, C.c_int, F64P, F64P, F64P,
                                    I64P]),
@@ -65,6 +66,15 @@ lib = _load()
 
 def _p(a, t):
     return a.ctypes.data_as(t)
+
+
+def bfgs_quadratic(A, b, x0, maxiters, accurate=False):
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.ascontiguousarray(x0, dtype=np.float64).copy()
+    ev = np.zeros(1, np.int64)
+    f = lib.orc_bfgs_quadratic(len(b), _p(A, F64P), _p(b, F64P), _p(x, F64P), maxiters, int(accurate), _p(ev, I64P))
+    return f, x, int(ev[0])
 
 
 class Oracle:

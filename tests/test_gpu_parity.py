@@ -1,0 +1,207 @@
+"""Device parity against the CPU oracle (gfx950), through the C ABI.
+
+FP64: bit-exact (maps, eval_deriv on both back ends, MC trajectories,
+refinement, final rescoring, batched models).  FP32: the reference's rule
+(src/gpu/tests/gpu_eval_test.cpp:55-92): at least 99% of evaluations within
+1e-4 relative of the FP64 result for energy and gradient, every one within
+10x that, except evaluations next to a discontinuity (an atom on a map cell
+face or a pair at the 8 A cutoff); plus energies within 1e-3 kcal/mol of
+FP64 where |E| < 10 (the served tolerance)."""
+import numpy as np
+import pytest
+
+import smina_gpu as sg
+from common import gd_c0, heavy_types, oracle_for, receptor, random_confs
+
+pytestmark = pytest.mark.gpu
+
+LIGANDS = [(1, 24, 6), (2, 38, 10), (4, 60, 15), (3, 10, 0)]
+
+
+@pytest.fixture(scope="module")
+def env():
+    ligs = [sg.Mol.ligand(s, h, t) for s, h, t in LIGANDS]
+    o, ids, sf, gd = oracle_for(ligs)
+    rec = receptor()
+    ctx = sg.Context(0)
+    n, smooth, fast = sg.build_tables(sf)
+    ctx.upload_tables(n, smooth, fast)
+    rid = ctx.upload_receptor(rec.xyz, rec.type)
+    types = heavy_types(*ligs)
+    gids = {p: ctx.build_grids(rid, gd, types, p) for p in (sg.FP64, sg.FP32)}
+    flats = [sg.Flat(l) for l in ligs]
+    bids = {p: ctx.upload_models(flats, p) for p in (sg.FP64, sg.FP32)}
+    yield dict(ligs=ligs, o=o, ids=ids, sf=sf, gd=gd, ctx=ctx, rid=rid, types=types, gids=gids, flats=flats,
+               bids=bids, rec=rec)
+    ctx.close()
+
+
+def test_device_present(env):
+    name, cus = env["ctx"].device_info()
+    print("device:", name, cus, "CUs", flush=True)
+    assert "gfx950" in name and cus > 0
+
+
+def test_grids_fp64_bit_exact(env):
+    ctx, o = env["ctx"], env["o"]
+    for t in env["types"]:
+        assert np.array_equal(ctx.download_grid(env["gids"][sg.FP64], t, env["gd"]), o.grid(t))
+
+
+def test_grids_offset_box_bit_exact(env):
+    ctx = env["ctx"]
+    gd = sg.grid_dims((0.1, 0.2, 0.3), (12.0, 12.0, 12.0))
+    o, _, _, _ = oracle_for(env["ligs"][:1], gd=gd)
+    types = heavy_types(env["ligs"][0])
+    gid = ctx.build_grids(env["rid"], gd, types, sg.FP64)
+    for t in types:
+        assert np.array_equal(ctx.download_grid(gid, t, gd), o.grid(t))
+
+
+def test_grids_fp32_are_rounded_fp64(env):
+    ctx, o = env["ctx"], env["o"]
+    for t in env["types"]:
+        g = ctx.download_grid(env["gids"][sg.FP32], t, env["gd"])
+        assert np.array_equal(g, o.grid(t).astype(np.float32).astype(np.float64))
+
+
+def eval_both(env, li, prec, backend, v, n=24):
+    ctx, o, f = env["ctx"], env["o"], env["flats"][li]
+    confs = random_confs(f, env["gd"], n, seed0=500 + 31 * li)
+    # half at random in the box, half near the centre (contacts, no clamping)
+    confs[n // 2:, :3] = np.random.default_rng(li).uniform(-3, 3, (n - n // 2, 3))
+    kw = dict(grid_id=env["gids"][prec]) if backend == 0 else dict(grid_id=-1, receptor_id=env["rid"],
+                                                                   gd=env["gd"], slope=10.0)
+    e, g, x = ctx.eval_confs(env["bids"][prec], [li] * n, confs, f.change_size, f.natoms, v, **kw)
+    ref = [o.eval_deriv(env["ids"][li], c, v, backend=backend, slope=1e6 if backend == 0 else 10.0)
+           for c in confs]
+    return confs, e, g, x, ref
+
+
+@pytest.mark.parametrize("li", range(len(LIGANDS)))
+@pytest.mark.parametrize("backend", [0, 1])
+@pytest.mark.parametrize("v", [10.0, 1000.0])
+def test_eval_fp64_bit_exact(env, li, backend, v):
+    print(f"eval fp64 ligand {LIGANDS[li]} backend {backend} v {v}", flush=True)
+    confs, e, g, x, ref = eval_both(env, li, sg.FP64, backend, (v, v, v))
+    for k, (eo, go, xo) in enumerate(ref):
+        assert e[k] == eo, (k, e[k], eo)
+        assert np.array_equal(g[k], go), (k, g[k] - go)
+        assert np.array_equal(x[k], xo), k
+
+
+@pytest.mark.parametrize("li", range(len(LIGANDS)))
+@pytest.mark.parametrize("backend", [0, 1])
+def test_eval_fp32_tolerance(env, li, backend):
+    print(f"eval fp32 ligand {LIGANDS[li]} backend {backend}", flush=True)
+    confs, e, g, x, ref = eval_both(env, li, sg.FP32, backend, (10.0, 10.0, 10.0), n=48)
+    ok = bad = 0
+    for k, (eo, go, xo) in enumerate(ref):
+        rel_e = abs(e[k] - eo) / max(1.0, abs(eo))
+        rel_g = np.max(np.abs(g[k] - go)) / max(1.0, np.max(np.abs(go)))
+        if max(rel_e, rel_g) <= 1e-4:
+            ok += 1
+        else:
+            bad += 1
+            assert max(rel_e, rel_g) <= 1e-3 or abs(eo) > 1e3, (k, rel_e, rel_g, eo)
+        if abs(eo) < 10:
+            assert abs(e[k] - eo) <= 1e-3, (k, e[k], eo)
+    assert ok / (ok + bad) >= 0.95, (ok, bad)
+
+
+def run_chains(env, li, prec, steps, nchains=4, seed0=2000):
+    ctx, f = env["ctx"], env["flats"][li]
+    cs, ss = zip(*[f.make_chain(seed0 + k, env["gd"]) for k in range(nchains)])
+    maxit = (25 + f.natoms) // 3
+    return cs, ss, maxit, ctx.run_mc(env["bids"][prec], env["gids"][prec], sg.mc_params(), [li] * nchains,
+                                     np.array(cs), np.array(ss), [steps] * nchains, [maxit] * nchains,
+                                     coord_stride=3 * f.nheavy)
+
+
+@pytest.mark.parametrize("steps", [1, 10, 100])
+def test_mc_fp64_trajectories_identical(env, steps):
+    li = 0
+    print(f"mc fp64 steps {steps}", flush=True)
+    cs, ss, maxit, (cnt, e, c, x, ev) = run_chains(env, li, sg.FP64, steps)
+    o, m = env["o"], env["ids"][li]
+    for k in range(len(cs)):
+        eo, co, xo, evo = o.mc_chain(m, sg.mc_params(), steps, maxit, cs[k], ss[k])
+        assert cnt[k] == len(eo)
+        assert np.array_equal(e[k, :cnt[k]], eo)
+        assert np.array_equal(c[k, :cnt[k], :co.shape[1]], co)
+        assert np.array_equal(x[k, :cnt[k], :xo.shape[1]], xo)
+        assert ev[k] == evo
+
+
+def test_mc_fp64_batched_models(env):
+    """chains of different ligands in one launch follow their own CPU chains"""
+    ctx, o = env["ctx"], env["o"]
+    models, cs, ss, steps, iters = [], [], [], [], []
+    for li, f in enumerate(env["flats"]):
+        for k in range(2):
+            c, s = f.make_chain(3000 + 10 * li + k, env["gd"])
+            cc = np.zeros(64)
+            cc[:f.conf_size] = c
+            models.append(li)
+            cs.append(cc)
+            ss.append(s)
+            steps.append(5 + li)
+            iters.append((25 + f.natoms) // 3)
+    cnt, e, c, x, ev = ctx.run_mc(env["bids"][sg.FP64], env["gids"][sg.FP64], sg.mc_params(), models, np.array(cs),
+                                  np.array(ss), steps, iters)
+    for i, li in enumerate(models):
+        f = env["flats"][li]
+        eo, co, xo, evo = o.mc_chain(env["ids"][li], sg.mc_params(), steps[i], iters[i], cs[i][:f.conf_size], ss[i])
+        assert cnt[i] == len(eo) and np.array_equal(e[i, :cnt[i]], eo) and ev[i] == evo
+
+
+def test_fp32_search_is_deterministic(env):
+    a = run_chains(env, 1, sg.FP32, 20)[3]
+    b = run_chains(env, 1, sg.FP32, 20)[3]
+    for u, w in zip(a, b):
+        assert np.array_equal(u, w)
+
+
+def test_refine_and_rescore_fp64_bit_exact(env):
+    ctx, o = env["ctx"], env["o"]
+    for li, f in enumerate(env["flats"]):
+        confs = random_confs(f, env["gd"], 6, seed0=900 + li)
+        confs[3:, :3] = 0.5
+        maxit = (25 + f.natoms) // 3
+        e, cr, xr = ctx.refine(env["bids"][sg.FP64], env["rid"], env["gd"], [li] * 6, confs, [maxit] * 6,
+                               coord_stride=3 * f.nheavy)
+        s0 = ctx.score(env["bids"][sg.FP64], env["rid"], env["gd"], [li] * 6, confs, 0)
+        s1 = ctx.score(env["bids"][sg.FP32], env["rid"], env["gd"], [li] * 6, confs, 1)
+        for k in range(6):
+            eo, co, xo = o.refine(env["ids"][li], confs[k], maxit)
+            assert e[k] == eo and np.array_equal(cr[k], co) and np.array_equal(xr[k], xo)
+            assert s0[k] == o.eval(env["ids"][li], confs[k])
+            assert s1[k] == o.eval_intramolecular(env["ids"][li], confs[k])
+
+
+def test_atom_overlap_is_reported(env):
+    """a ligand atom exactly on a receptor atom: the device reports it like
+    the CPU (src/lib/non_cache.cpp:170-172), no result is returned"""
+    ctx, f, rec = env["ctx"], env["flats"][0], env["rec"]
+    heavy = np.where(rec.type > 1)[0]
+    inside = heavy[np.all(np.abs(rec.xyz[heavy]) < 10, axis=1)][0]
+    c = f.initial_conf()
+    c[:3] = rec.xyz[inside]  # the root's first atom sits at the origin of the root frame
+    with pytest.raises(sg.SgpuError) as err:
+        ctx.eval_confs(env["bids"][sg.FP64], [0], c[None], f.change_size, f.natoms, (10, 10, 10), grid_id=-1,
+                       receptor_id=env["rid"], gd=env["gd"], slope=10)
+    assert err.value.status == 4
+
+
+def test_invalid_requests_are_rejected(env):
+    ctx, f = env["ctx"], env["flats"][0]
+    with pytest.raises(sg.SgpuError):  # unknown batch
+        ctx.eval_confs(12345, [0], f.initial_conf()[None], f.change_size, f.natoms, (10, 10, 10),
+                       grid_id=env["gids"][sg.FP64])
+    with pytest.raises(sg.SgpuError):  # model index out of range
+        ctx.eval_confs(env["bids"][sg.FP64], [99], f.initial_conf()[None], f.change_size, f.natoms, (10, 10, 10),
+                       grid_id=env["gids"][sg.FP64])
+    with pytest.raises(sg.SgpuError):  # a map for a type the ligand lacks
+        gid = ctx.build_grids(env["rid"], env["gd"], [2], sg.FP64)
+        ctx.eval_confs(env["bids"][sg.FP64], [0], f.initial_conf()[None], f.change_size, f.natoms, (10, 10, 10),
+                       grid_id=gid)
