@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -123,11 +124,8 @@ struct Grid
 struct HostBatch
 {
 	std::vector<ModelHdr> hdr;
-	std::vector<int> atom_type, atom_frame, nbr_start, nbr_code, frame_parent, frame_kind, frame_begin, frame_end,
-			frame_conf, frame_change, child_start, child_count, child_list, level_start, level_frames, pair_a, pair_b,
-			pair_tri, heavy, tree_root, tree_conf, tree_change, tree_ligand, tree_ntors, tree_atom_begin,
-			tree_atom_end, tree_pair_begin, tree_pair_end, maps;
-	std::vector<double> atom_xyz0, atom_local, frame_origin, frame_axis;
+	std::vector<int> iblob, pair_a, pair_b, pair_tri;
+	std::vector<double> tblob;
 	WavePlan plan{};
 };
 
@@ -136,10 +134,7 @@ struct Batch
 	int precision = 0;
 	int n = 0;
 	HostBatch h;
-	DBuf hdr, atom_type, atom_frame, nbr_start, nbr_code, frame_parent, frame_kind, frame_begin, frame_end,
-			frame_conf, frame_change, child_start, child_count, child_list, level_start, level_frames, pair_a,
-			pair_b, pair_tri, heavy, tree_root, tree_conf, tree_change, tree_ligand, tree_ntors, tree_atom_begin,
-			tree_atom_end, tree_pair_begin, tree_pair_end, maps, atom_xyz0, atom_local, frame_origin, frame_axis;
+	DBuf hdr, iblob, tblob, pair_a, pair_b, pair_tri;
 	// FP64 copy for the final rescoring when the batch searches in FP32
 	std::unique_ptr<Batch> fp64;
 };
@@ -167,16 +162,21 @@ struct sgpu_ctx
 namespace
 {
 
+// flat models -> per-model int and real blobs with packed neighbour codes
 void pack_batch(const sgpu_flat_model* ms, int n, HostBatch& B)
 {
-	int nmax = 0, fmax = 0, cmax = 0, gmax = 0, pmax = 0;
+	int nmax = 0, fmax = 0, gmax = 0, pmax = 0, imax = 0, tmax = 0, amax = 0;
 	for (int mi = 0; mi < n; mi++)
 	{
 		const sgpu_flat_model& m = ms[mi];
-		require(m.natoms > 0 && m.natoms <= MAX_ATOMS, "model atoms out of range (1.." + std::to_string(MAX_ATOMS) + ")");
+		const int na = m.natoms + m.ninflex;
+		require(m.natoms > 0 && na <= MAX_ATOMS, "model atoms out of range (1.." + std::to_string(MAX_ATOMS) + ")");
 		require(m.nframes > 0 && m.nframes <= MAX_FRAMES, "model frames out of range");
 		require(m.conf_size <= MAX_CONF && m.change_size <= MAX_CHANGE, "model conf/change size over 64");
-		require(m.ntrees > 0 && m.ntrees <= MAX_TREES, "model trees out of range");
+		require(m.ntrees > 0 && m.ntrees <= MAX_TREES && m.nligands <= m.ntrees, "model trees out of range");
+		require(m.npairs >= 0 && m.npairs <= MAX_PAIRS && m.nother >= 0 && m.nother <= m.npairs, "model pairs out of range");
+		for (int i = 0; i < na; i++)
+			require(m.atom_type[i] >= 0 && m.atom_type[i] < NUM_ATOM_TYPES, "atom type out of range");
 		ModelHdr h{};
 		h.natoms = m.natoms;
 		h.ninflex = m.ninflex;
@@ -188,54 +188,47 @@ void pack_batch(const sgpu_flat_model* ms, int n, HostBatch& B)
 		h.conf_size = m.conf_size;
 		h.change_size = m.change_size;
 		h.num_tors = m.num_tors;
-		const int na = m.natoms + m.ninflex;
-		h.atom_off = int(B.atom_type.size());
-		for (int i = 0; i < na; i++)
-		{
-			B.atom_type.push_back(m.atom_type[i]);
-			for (int d = 0; d < 3; d++)
-				B.atom_xyz0.push_back(m.atom_init[3 * i + d]);
-		}
-		h.mov_off = int(B.atom_frame.size());
+		std::vector<int> I;
+		std::vector<double> R;
+		auto mark = [&](int& off) { off = int(I.size()); };
+		// atoms
+		mark(h.o_type);
+		I.insert(I.end(), m.atom_type, m.atom_type + na);
+		mark(h.o_afr);
 		for (int i = 0; i < m.natoms; i++)
 		{
 			require(m.atom_frame[i] >= 0 && m.atom_frame[i] < m.nframes, "atom frame out of range");
-			B.atom_frame.push_back(m.atom_frame[i]);
-			for (int d = 0; d < 3; d++)
-				B.atom_local.push_back(m.atom_local[3 * i + d]);
+			I.push_back(m.atom_frame[i]);
 		}
-		// pair lists per movable atom, in evaluation (pair) order
+		// neighbour lists in evaluation (pair) order
 		std::vector<std::vector<int>> lists(m.natoms);
 		for (int p = 0; p < m.npairs; p++)
 		{
 			const int a = m.pair_a[p], b = m.pair_b[p];
 			require(a >= 0 && a < m.natoms && b > a && b < na, "pair atoms out of range");
-			require(m.pair_tri[p] >= 0 && m.pair_tri[p] < SGPU_NUM_TYPE_PAIRS, "pair type index out of range");
-			lists[a].push_back(p << 1);
+			const int tri = m.pair_tri[p];
+			require(tri >= 0 && tri < SGPU_NUM_TYPE_PAIRS, "pair type index out of range");
+			require(tri == tri_index_permissive(m.atom_type[a], m.atom_type[b]), "pair table index disagrees with types");
+			const bool other = p < m.nother;
+			lists[a].push_back(make_code(b, tri, false, other, p));
 			if (b < m.natoms)
-				lists[b].push_back((p << 1) | 1);
+				lists[b].push_back(make_code(a, tri, true, other, p));
 		}
-		h.nbr_off = int(B.nbr_start.size());
-		h.code_off = int(B.nbr_code.size());
+		mark(h.o_nbr);
 		int k = 0;
 		for (int i = 0; i < m.natoms; i++)
 		{
-			B.nbr_start.push_back(k);
-			for (int c : lists[i])
-				B.nbr_code.push_back(c);
+			I.push_back(k);
 			k += int(lists[i].size());
 		}
-		B.nbr_start.push_back(k);
-		h.pair_off = int(B.pair_a.size());
+		I.push_back(k);
+		mark(h.o_code);
+		for (int i = 0; i < m.natoms; i++)
+			I.insert(I.end(), lists[i].begin(), lists[i].end());
+		mark(h.o_pcode);
 		for (int p = 0; p < m.npairs; p++)
-		{
-			B.pair_a.push_back(m.pair_a[p]);
-			B.pair_b.push_back(m.pair_b[p]);
-			B.pair_tri.push_back(m.pair_tri[p]);
-		}
+			I.push_back(make_pcode(m.pair_a[p], m.pair_b[p], m.pair_tri[p], p < m.nother));
 		// frames, children, levels
-		h.frame_off = int(B.frame_parent.size());
-		h.child_off = int(B.child_list.size());
 		std::vector<int> depth(m.nframes, 0);
 		int nlevels = 0;
 		for (int f = 0; f < m.nframes; f++)
@@ -246,118 +239,124 @@ void pack_batch(const sgpu_flat_model* ms, int n, HostBatch& B)
 			nlevels = std::max(nlevels, depth[f] + 1);
 			require(m.frame_begin[f] >= 0 && m.frame_end[f] <= m.natoms && m.frame_begin[f] <= m.frame_end[f],
 					"frame atom range out of range");
+			require(m.frame_kind[f] >= SGPU_FRAME_RIGID_ROOT && m.frame_kind[f] <= SGPU_FRAME_SEGMENT, "frame kind");
+			require((m.frame_kind[f] == SGPU_FRAME_SEGMENT) == (p >= 0), "only segments have parents");
 		}
+		h.nlevels = nlevels;
+		mark(h.o_fparent);
+		I.insert(I.end(), m.frame_parent, m.frame_parent + m.nframes);
+		mark(h.o_fkind);
+		I.insert(I.end(), m.frame_kind, m.frame_kind + m.nframes);
+		mark(h.o_fbegin);
+		I.insert(I.end(), m.frame_begin, m.frame_begin + m.nframes);
+		mark(h.o_fend);
+		I.insert(I.end(), m.frame_end, m.frame_end + m.nframes);
+		mark(h.o_fconf);
+		I.insert(I.end(), m.frame_conf, m.frame_conf + m.nframes);
+		mark(h.o_fchange);
+		I.insert(I.end(), m.frame_change, m.frame_change + m.nframes);
+		std::vector<int> cstart, ccount, clist;
 		for (int f = 0; f < m.nframes; f++)
 		{
-			B.frame_parent.push_back(m.frame_parent[f]);
-			B.frame_kind.push_back(m.frame_kind[f]);
-			B.frame_begin.push_back(m.frame_begin[f]);
-			B.frame_end.push_back(m.frame_end[f]);
-			B.frame_conf.push_back(m.frame_conf[f]);
-			B.frame_change.push_back(m.frame_change[f]);
-			B.child_start.push_back(int(B.child_list.size()) - h.child_off);
+			cstart.push_back(int(clist.size()));
 			int cnt = 0;
 			for (int c = f + 1; c < m.nframes; c++)
 				if (m.frame_parent[c] == f)
 				{
-					B.child_list.push_back(c);
+					clist.push_back(c);
 					++cnt;
 				}
-			B.child_count.push_back(cnt);
-			for (int d = 0; d < 3; d++)
-			{
-				B.frame_origin.push_back(m.frame_origin[3 * f + d]);
-				B.frame_axis.push_back(m.frame_axis[3 * f + d]);
-			}
+			ccount.push_back(cnt);
 		}
-		h.nlevels = nlevels;
-		h.level_off = int(B.level_start.size());
-		h.lvf_off = int(B.level_frames.size());
-		int cntl = 0;
+		mark(h.o_cstart);
+		I.insert(I.end(), cstart.begin(), cstart.end());
+		mark(h.o_ccount);
+		I.insert(I.end(), ccount.begin(), ccount.end());
+		mark(h.o_clist);
+		I.insert(I.end(), clist.begin(), clist.end());
+		mark(h.o_lstart);
+		std::vector<int> lframes;
 		for (int L = 0; L < nlevels; L++)
 		{
-			B.level_start.push_back(cntl);
+			I.push_back(int(lframes.size()));
 			for (int f = 0; f < m.nframes; f++)
 				if (depth[f] == L)
-				{
-					B.level_frames.push_back(f);
-					++cntl;
-				}
+					lframes.push_back(f);
 		}
-		B.level_start.push_back(cntl);
+		I.push_back(int(lframes.size()));
+		mark(h.o_lframes);
+		I.insert(I.end(), lframes.begin(), lframes.end());
 		// heavy atoms
-		h.heavy_off = int(B.heavy.size());
+		mark(h.o_heavy);
 		int nh = 0;
 		for (int i = 0; i < m.natoms; i++)
 			if (!is_hydrogen_type(m.atom_type[i]))
 			{
-				B.heavy.push_back(i);
+				I.push_back(i);
 				++nh;
 			}
 		h.nheavy = nh;
 		// trees
-		h.tree_off = int(B.tree_root.size());
-		std::vector<int> troot(m.ntrees, -1);
-		for (int f = 0; f < m.nframes; f++)
-			if (m.frame_parent[f] < 0)
-			{
-				require(m.frame_tree[f] >= 0 && m.frame_tree[f] < m.ntrees, "frame tree out of range");
-				troot[m.frame_tree[f]] = f;
-			}
-		std::vector<int> tab(m.ntrees, m.natoms), tae(m.ntrees, 0), tn(m.ntrees, 0);
+		std::vector<int> troot(m.ntrees, -1), tab(m.ntrees, m.natoms), tae(m.ntrees, 0), tn(m.ntrees, 0);
 		for (int f = 0; f < m.nframes; f++)
 		{
 			const int t = m.frame_tree[f];
+			require(t >= 0 && t < m.ntrees, "frame tree out of range");
+			if (m.frame_parent[f] < 0)
+				troot[t] = f;
 			tab[t] = std::min(tab[t], m.frame_begin[f]);
 			tae[t] = std::max(tae[t], m.frame_end[f]);
 			if (m.frame_kind[f] != SGPU_FRAME_RIGID_ROOT)
 				++tn[t];
 		}
-		std::vector<int> tconf(m.ntrees), tchg(m.ntrees);
+		std::vector<int> tconf(m.ntrees), tchg(m.ntrees), tlig(m.ntrees), tpb(m.ntrees), tpe(m.ntrees);
 		for (int t = 0; t < m.ntrees; t++)
 		{
 			require(troot[t] >= 0, "tree without a root frame");
-			const int r = troot[t];
-			tconf[t] = m.frame_conf[r];
-			tchg[t] = m.frame_change[r];
-		}
-		for (int t = 0; t < m.ntrees; t++)
-		{
-			const bool lig = t < m.nligands;
-			B.tree_root.push_back(troot[t]);
-			B.tree_conf.push_back(tconf[t]);
-			B.tree_change.push_back(tchg[t]);
-			B.tree_ligand.push_back(lig ? 1 : 0);
-			B.tree_ntors.push_back(tn[t]);
-			B.tree_atom_begin.push_back(tab[t]);
-			B.tree_atom_end.push_back(tae[t]);
-			// the ligand's internal pairs: contiguous after the other pairs
-			int pb = m.npairs, pe = m.npairs;
-			if (lig)
-			{
-				pb = m.npairs;
-				pe = m.nother;
+			tconf[t] = m.frame_conf[troot[t]];
+			tchg[t] = m.frame_change[troot[t]];
+			tlig[t] = t < m.nligands ? 1 : 0;
+			// a ligand's internal pairs: contiguous after the other pairs
+			int pb = m.npairs, pe = m.nother;
+			if (tlig[t])
 				for (int p = m.nother; p < m.npairs; p++)
 					if (m.pair_a[p] >= tab[t] && m.pair_a[p] < tae[t])
 					{
 						pb = std::min(pb, p);
 						pe = std::max(pe, p + 1);
 					}
-				if (pb > pe)
-					pb = pe;
-			}
-			B.tree_pair_begin.push_back(pb);
-			B.tree_pair_end.push_back(pe);
+			if (pb > pe)
+				pb = pe;
+			tpb[t] = pb;
+			tpe[t] = pe;
 		}
+		mark(h.o_troot);
+		I.insert(I.end(), troot.begin(), troot.end());
+		mark(h.o_tconf);
+		I.insert(I.end(), tconf.begin(), tconf.end());
+		mark(h.o_tchange);
+		I.insert(I.end(), tchg.begin(), tchg.end());
+		mark(h.o_tlig);
+		I.insert(I.end(), tlig.begin(), tlig.end());
+		mark(h.o_tntors);
+		I.insert(I.end(), tn.begin(), tn.end());
+		mark(h.o_tab);
+		I.insert(I.end(), tab.begin(), tab.end());
+		mark(h.o_tae);
+		I.insert(I.end(), tae.begin(), tae.end());
+		mark(h.o_tpb);
+		I.insert(I.end(), tpb.begin(), tpb.end());
+		mark(h.o_tpe);
+		I.insert(I.end(), tpe.begin(), tpe.end());
 		// conf / change maps
-		h.map_off = int(B.maps.size());
 		std::vector<int> ck(m.conf_size, -1), cc(m.conf_size, 0), gk(m.change_size, -1), gc(m.change_size, 0);
 		for (int f = 0; f < m.nframes; f++)
 		{
 			const int ci = m.frame_conf[f], gi = m.frame_change[f];
 			if (m.frame_kind[f] == SGPU_FRAME_RIGID_ROOT)
 			{
-				require(ci + 7 <= m.conf_size && gi + 6 <= m.change_size, "root conf index out of range");
+				require(ci >= 0 && ci + 7 <= m.conf_size && gi >= 0 && gi + 6 <= m.change_size,
+						"root conf index out of range");
 				for (int d = 0; d < 3; d++)
 				{
 					ck[ci + d] = CK_POSITION;
@@ -378,7 +377,7 @@ void pack_batch(const sgpu_flat_model* ms, int n, HostBatch& B)
 			}
 			else
 			{
-				require(ci < m.conf_size && gi < m.change_size, "torsion index out of range");
+				require(ci >= 0 && ci < m.conf_size && gi >= 0 && gi < m.change_size, "torsion index out of range");
 				ck[ci] = CK_TORSION;
 				cc[ci] = gi;
 				gk[gi] = GK_TORSION;
@@ -389,16 +388,39 @@ void pack_batch(const sgpu_flat_model* ms, int n, HostBatch& B)
 			require(v >= 0, "conf element without a frame");
 		for (int v : gk)
 			require(v >= 0, "change element without a frame");
-		B.maps.insert(B.maps.end(), ck.begin(), ck.end());
-		B.maps.insert(B.maps.end(), cc.begin(), cc.end());
-		B.maps.insert(B.maps.end(), gk.begin(), gk.end());
-		B.maps.insert(B.maps.end(), gc.begin(), gc.end());
+		mark(h.o_maps);
+		I.insert(I.end(), ck.begin(), ck.end());
+		I.insert(I.end(), cc.begin(), cc.end());
+		I.insert(I.end(), gk.begin(), gk.end());
+		I.insert(I.end(), gc.begin(), gc.end());
+		// reals
+		h.o_xyz0 = int(R.size());
+		R.insert(R.end(), m.atom_init, m.atom_init + 3 * na);
+		h.o_local = int(R.size());
+		R.insert(R.end(), m.atom_local, m.atom_local + 3 * m.natoms);
+		h.o_forig = int(R.size());
+		R.insert(R.end(), m.frame_origin, m.frame_origin + 3 * m.nframes);
+		h.o_faxis = int(R.size());
+		R.insert(R.end(), m.frame_axis, m.frame_axis + 3 * m.nframes);
+		// append
+		h.iblob_off = int(B.iblob.size());
+		h.iblob_len = int(I.size());
+		h.tblob_off = int(B.tblob.size());
+		h.tblob_len = int(R.size());
+		h.pair_off = int(B.pair_a.size());
+		B.iblob.insert(B.iblob.end(), I.begin(), I.end());
+		B.tblob.insert(B.tblob.end(), R.begin(), R.end());
+		B.pair_a.insert(B.pair_a.end(), m.pair_a, m.pair_a + m.npairs);
+		B.pair_b.insert(B.pair_b.end(), m.pair_b, m.pair_b + m.npairs);
+		B.pair_tri.insert(B.pair_tri.end(), m.pair_tri, m.pair_tri + m.npairs);
 		B.hdr.push_back(h);
 		nmax = std::max(nmax, m.natoms);
+		amax = std::max(amax, na);
 		fmax = std::max(fmax, m.nframes);
-		cmax = std::max(cmax, m.conf_size);
 		gmax = std::max(gmax, m.change_size);
 		pmax = std::max(pmax, m.npairs);
+		imax = std::max(imax, h.iblob_len);
+		tmax = std::max(tmax, h.tblob_len);
 	}
 	WavePlan& p = B.plan;
 	p.nmax = std::max(WAVE, (nmax + WAVE - 1) / WAVE * WAVE);
@@ -406,7 +428,10 @@ void pack_batch(const sgpu_flat_model* ms, int n, HostBatch& B)
 	p.cmax = WAVE;
 	p.gmax = WAVE;
 	p.pmax = std::max(1, pmax);
-	p.hmax = gmax * (gmax + 1) / 2;
+	p.hmax = std::max(1, gmax * gmax);
+	p.imax = imax;
+	p.tmax = tmax;
+	p.amax = amax;
 	p.bytes = 0;
 }
 
@@ -421,49 +446,14 @@ void upload_batch(Batch& b, hipStream_t st)
 	const HostBatch& h = b.h;
 	b.hdr.alloc(h.hdr.size() * sizeof(ModelHdr));
 	hip_check(hipMemcpyAsync(b.hdr.p, h.hdr.data(), h.hdr.size() * sizeof(ModelHdr), hipMemcpyHostToDevice, st), "hdr");
-	upload(b.atom_type, h.atom_type, st);
-	upload(b.atom_frame, h.atom_frame, st);
-	upload(b.nbr_start, h.nbr_start, st);
-	upload(b.nbr_code, h.nbr_code, st);
-	upload(b.frame_parent, h.frame_parent, st);
-	upload(b.frame_kind, h.frame_kind, st);
-	upload(b.frame_begin, h.frame_begin, st);
-	upload(b.frame_end, h.frame_end, st);
-	upload(b.frame_conf, h.frame_conf, st);
-	upload(b.frame_change, h.frame_change, st);
-	upload(b.child_start, h.child_start, st);
-	upload(b.child_count, h.child_count, st);
-	upload(b.child_list, h.child_list, st);
-	upload(b.level_start, h.level_start, st);
-	upload(b.level_frames, h.level_frames, st);
+	upload(b.iblob, h.iblob, st);
 	upload(b.pair_a, h.pair_a, st);
 	upload(b.pair_b, h.pair_b, st);
 	upload(b.pair_tri, h.pair_tri, st);
-	upload(b.heavy, h.heavy, st);
-	upload(b.tree_root, h.tree_root, st);
-	upload(b.tree_conf, h.tree_conf, st);
-	upload(b.tree_change, h.tree_change, st);
-	upload(b.tree_ligand, h.tree_ligand, st);
-	upload(b.tree_ntors, h.tree_ntors, st);
-	upload(b.tree_atom_begin, h.tree_atom_begin, st);
-	upload(b.tree_atom_end, h.tree_atom_end, st);
-	upload(b.tree_pair_begin, h.tree_pair_begin, st);
-	upload(b.tree_pair_end, h.tree_pair_end, st);
-	upload(b.maps, h.maps, st);
 	if (b.precision == SGPU_FP64)
-	{
-		upload(b.atom_xyz0, h.atom_xyz0, st);
-		upload(b.atom_local, h.atom_local, st);
-		upload(b.frame_origin, h.frame_origin, st);
-		upload(b.frame_axis, h.frame_axis, st);
-	}
+		upload(b.tblob, h.tblob, st);
 	else
-	{
-		upload(b.atom_xyz0, cast<float>(h.atom_xyz0), st);
-		upload(b.atom_local, cast<float>(h.atom_local), st);
-		upload(b.frame_origin, cast<float>(h.frame_origin), st);
-		upload(b.frame_axis, cast<float>(h.frame_axis), st);
-	}
+		upload(b.tblob, cast<float>(h.tblob), st);
 	hip_check(hipStreamSynchronize(st), "upload models");
 }
 
@@ -472,47 +462,19 @@ BatchPtrs<T> ptrs(const Batch& b)
 {
 	BatchPtrs<T> p;
 	p.hdr = b.hdr.as<ModelHdr>();
-	p.atom_type = b.atom_type.as<int>();
-	p.atom_xyz0 = b.atom_xyz0.as<T>();
-	p.atom_frame = b.atom_frame.as<int>();
-	p.atom_local = b.atom_local.as<T>();
-	p.nbr_start = b.nbr_start.as<int>();
-	p.nbr_code = b.nbr_code.as<int>();
-	p.frame_parent = b.frame_parent.as<int>();
-	p.frame_kind = b.frame_kind.as<int>();
-	p.frame_begin = b.frame_begin.as<int>();
-	p.frame_end = b.frame_end.as<int>();
-	p.frame_conf = b.frame_conf.as<int>();
-	p.frame_change = b.frame_change.as<int>();
-	p.child_start = b.child_start.as<int>();
-	p.child_count = b.child_count.as<int>();
-	p.child_list = b.child_list.as<int>();
-	p.frame_origin = b.frame_origin.as<T>();
-	p.frame_axis = b.frame_axis.as<T>();
-	p.level_start = b.level_start.as<int>();
-	p.level_frames = b.level_frames.as<int>();
+	p.iblob = b.iblob.as<int>();
+	p.tblob = b.tblob.as<T>();
 	p.pair_a = b.pair_a.as<int>();
 	p.pair_b = b.pair_b.as<int>();
 	p.pair_tri = b.pair_tri.as<int>();
-	p.heavy = b.heavy.as<int>();
-	p.tree_root = b.tree_root.as<int>();
-	p.tree_conf = b.tree_conf.as<int>();
-	p.tree_change = b.tree_change.as<int>();
-	p.tree_ligand = b.tree_ligand.as<int>();
-	p.tree_ntors = b.tree_ntors.as<int>();
-	p.tree_atom_begin = b.tree_atom_begin.as<int>();
-	p.tree_atom_end = b.tree_atom_end.as<int>();
-	p.tree_pair_begin = b.tree_pair_begin.as<int>();
-	p.tree_pair_end = b.tree_pair_end.as<int>();
-	p.maps = b.maps.as<int>();
 	return p;
 }
 
-int plan_bytes(const WavePlan& p, int prec)
+int plan_bytes(const WavePlan& p, int prec, int waves)
 {
-	const int elems = prec == SGPU_FP64 ? wave_lds_elems<double>(p) : wave_lds_elems<float>(p);
-	return elems * (prec == SGPU_FP64 ? 8 : 4);
+	return prec == SGPU_FP64 ? block_lds_bytes<double>(p, waves) : block_lds_bytes<float>(p, waves);
 }
+constexpr int LDS_LIMIT = 160 * 1024;
 
 template<typename T>
 TablePtrs<T> table_ptrs(const sgpu_ctx* c)
@@ -715,6 +677,20 @@ void ensure_scratch(DBuf& b, size_t bytes)
 		b.alloc(bytes);
 }
 
+// every heavy movable atom of the listed models must have a map
+void check_maps(const Batch& b, const Grid& g, const int* models, int n)
+{
+	for (int i = 0; i < n; i++)
+	{
+		const ModelHdr& h = b.h.hdr[models[i]];
+		for (int a = 0; a < h.natoms; a++)
+		{
+			const int t = b.h.iblob[h.iblob_off + h.o_type + a];
+			require(is_hydrogen_type(t) || g.slot_of[t] >= 0, "a ligand atom type has no map");
+		}
+	}
+}
+
 } // namespace
 
 extern "C" {
@@ -761,9 +737,7 @@ sgpu_status sgpu_device_info(sgpu_ctx* c, char* name, int name_len, int* cus)
 		hipDeviceProp_t p;
 		hip_check(hipGetDeviceProperties(&p, c->device), "hipGetDeviceProperties");
 		if (name && name_len > 0)
-		{
 			std::snprintf(name, size_t(name_len), "%s (%s)", p.name, p.gcnArchName);
-		}
 		if (cus)
 			*cus = p.multiProcessorCount;
 	});
@@ -945,8 +919,8 @@ sgpu_status sgpu_upload_models(sgpu_ctx* c, const sgpu_flat_model* models, int n
 		b->precision = precision;
 		b->n = n;
 		pack_batch(models, n, b->h);
-		b->h.plan.bytes = plan_bytes(b->h.plan, precision);
-		require(b->h.plan.bytes * WAVES_PER_BLOCK <= 160 * 1024, "model too large for LDS");
+		b->h.plan.bytes = plan_bytes(b->h.plan, precision, 1);
+		require(plan_bytes(b->h.plan, SGPU_FP64, 1) <= LDS_LIMIT, "model too large for one workgroup's LDS");
 		upload_batch(*b, c->stream);
 		if (precision == SGPU_FP32)
 		{
@@ -954,7 +928,7 @@ sgpu_status sgpu_upload_models(sgpu_ctx* c, const sgpu_flat_model* models, int n
 			b->fp64->precision = SGPU_FP64;
 			b->fp64->n = n;
 			b->fp64->h = b->h;
-			b->fp64->h.plan.bytes = plan_bytes(b->h.plan, SGPU_FP64);
+			b->fp64->h.plan.bytes = plan_bytes(b->h.plan, SGPU_FP64, 1);
 			upload_batch(*b->fp64, c->stream);
 		}
 		*batch_id = c->next_id++;
@@ -1002,19 +976,10 @@ void run_eval(sgpu_ctx* c, const sgpu_eval_request* q)
 	{
 		Grid& g = find_grid(c, q->grid_id);
 		require(g.precision == b.precision, "grid and model precision differ");
+		check_maps(b, g, q->model, n);
 		ec.backend = 0;
 		ec.grid = grid_ptrs<T>(g, q->slope);
 		ec.slope = T(q->slope);
-		for (int i = 0; i < n; i++)
-		{
-			const int mi = q->model[i];
-			const ModelHdr& h = b.h.hdr[mi];
-			for (int a = 0; a < h.natoms; a++)
-			{
-				const int t = b.h.atom_type[h.atom_off + a];
-				require(is_hydrogen_type(t) || g.slot_of[t] >= 0, "a ligand atom type has no map");
-			}
-		}
 	}
 	else
 	{
@@ -1084,12 +1049,8 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 		require(out->coord_stride >= 3 * h.nheavy, "coord stride");
 		require(ch->num_steps[i] >= 0 && ch->maxiters[i] >= 0, "steps / maxiters");
 		require(ch->rng[size_t(i) * 625 + 624] <= 624, "rng position");
-		for (int a = 0; a < h.natoms; a++)
-		{
-			const int t = b.h.atom_type[h.atom_off + a];
-			require(is_hydrogen_type(t) || g.slot_of[t] >= 0, "a ligand atom type has no map");
-		}
 	}
+	check_maps(b, g, ch->model, n);
 	ChainLayout L{};
 	L.conf_stride = b.h.plan.cmax;
 	int hmax = 1;
@@ -1115,7 +1076,7 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 	L.off_outx = take(size_t(MAX_POSES) * L.coord_stride * sizeof(T));
 	L.off_mc = take(L.atom_stride * sizeof(T));
 	L.off_ro = take(MAX_TREES * 3 * sizeof(T));
-	L.off_pbuf = take(size_t(L.pair_stride) * sizeof(T));
+	L.off_pbuf = take((sizeof(T) == 8 ? size_t(L.pair_stride) : 1) * sizeof(T));
 	L.off_ev = take(sizeof(int64_t));
 	L.bytes = (off + 63) / 64 * 64;
 	// initial chain state
@@ -1135,9 +1096,54 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 		scal[0] = 0;
 		scal[1] = Real_host_max<T>();
 	}
-	DBuf dchains, dmodel, dsteps, dmax;
+	// workgroups of `waves` chains of one ligand: chains grouped by model in
+	// input order, longest runs (steps x atoms) launched first
+	int maxper = 1;
+	{
+		std::map<int, int> per;
+		for (int i = 0; i < n; i++)
+			maxper = std::max(maxper, ++per[ch->model[i]]);
+	}
+	int waves = 8;
+	while (waves > 1 && waves / 2 >= maxper)
+		waves /= 2;
+	while (waves > 1 && plan_bytes(b.h.plan, b.precision, waves) > LDS_LIMIT)
+		waves /= 2;
+	require(plan_bytes(b.h.plan, b.precision, waves) <= LDS_LIMIT, "chain state exceeds LDS");
+	std::map<int, std::vector<int>> by_model;
+	for (int i = 0; i < n; i++)
+		by_model[ch->model[i]].push_back(i);
+	struct Blk
+	{
+		double cost;
+		int model;
+		std::vector<int> chains;
+	};
+	std::vector<Blk> blks;
+	for (auto& kv : by_model)
+		for (size_t s = 0; s < kv.second.size(); s += size_t(waves))
+		{
+			Blk bk{0, kv.first, {}};
+			for (size_t k = s; k < std::min(kv.second.size(), s + size_t(waves)); k++)
+			{
+				bk.chains.push_back(kv.second[k]);
+				bk.cost = std::max(bk.cost, double(ch->num_steps[kv.second[k]]) *
+								double(b.h.hdr[kv.first].npairs + 4 * b.h.hdr[kv.first].natoms));
+			}
+			blks.push_back(bk);
+		}
+	std::stable_sort(blks.begin(), blks.end(), [](const Blk& a, const Blk& b2) { return a.cost > b2.cost; });
+	std::vector<int> block_model, slot_chain;
+	for (const Blk& bk : blks)
+	{
+		block_model.push_back(bk.model);
+		for (int k = 0; k < waves; k++)
+			slot_chain.push_back(k < int(bk.chains.size()) ? bk.chains[k] : -1);
+	}
+	DBuf dchains, dbm, dsc, dsteps, dmax;
 	upload(dchains, init, c->stream);
-	upload(dmodel, std::vector<int>(ch->model, ch->model + n), c->stream);
+	upload(dbm, block_model, c->stream);
+	upload(dsc, slot_chain, c->stream);
 	upload(dsteps, std::vector<int>(ch->num_steps, ch->num_steps + n), c->stream);
 	upload(dmax, std::vector<int>(ch->maxiters, ch->maxiters + n), c->stream);
 	EvalCtx<T> ec{};
@@ -1161,8 +1167,8 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 	for (int i = 0; i < n; i++)
 		max_steps = std::max(max_steps, ch->num_steps[i]);
 	Timer tm(c, "mc");
-	hip_check(launch_mc<T>(ptrs<T>(b), ec, b.h.plan, P, L, dchains.as<char>(), n, dmodel.as<int>(), dsteps.as<int>(),
-					  dmax.as<int>(), 0, max_steps, c->stream),
+	hip_check(launch_mc<T>(ptrs<T>(b), ec, b.h.plan, P, L, dchains.as<char>(), int(blks.size()), waves, dbm.as<int>(),
+					  dsc.as<int>(), dsteps.as<int>(), dmax.as<int>(), 0, max_steps, c->stream),
 			"mc kernel");
 	tm.stop();
 	hip_check(hipMemcpyAsync(init.data(), dchains.p, init.size(), hipMemcpyDeviceToHost, c->stream), "download");
@@ -1204,6 +1210,7 @@ void run_refine(sgpu_ctx* c, const sgpu_refine_request* q)
 	{
 		require(q->model[i] >= 0 && q->model[i] < b.n, "pose model out of range");
 		require(q->conf_stride >= b.h.hdr[q->model[i]].conf_size, "conf stride");
+		require(q->maxiters[i] >= 0, "maxiters");
 		if (q->coords)
 			require(q->coord_stride >= 3 * b.h.hdr[q->model[i]].nheavy, "coord stride");
 	}
@@ -1252,7 +1259,7 @@ void run_refine(sgpu_ctx* c, const sgpu_refine_request* q)
 		const ModelHdr& h = b.h.hdr[q->model[i]];
 		for (int k = 0; k < h.conf_size; k++)
 			q->conf[size_t(i) * q->conf_stride + k] = double(conf[size_t(i) * cs + k]);
-		q->e[i] = e[i] >= Real_host_max<T>() ? 1.7976931348623157e308 : double(e[i]);
+		q->e[i] = e[i] >= Real_host_max<T>() ? DBL_MAX : double(e[i]);
 		if (q->coords)
 			for (int k = 0; k < 3 * h.nheavy; k++)
 				q->coords[size_t(i) * q->coord_stride + k] = double(x[size_t(i) * xs + k]);

@@ -5,8 +5,8 @@
  * statement of the same semantics is src/gpu/gpu_mc.cuh).
  *
  * Vectors of the change and conf spaces live one element per lane; the
- * packed Hessian (h(i, j) at i + j(j+1)/2, i <= j) lives in LDS.  Scalars
- * are wave-uniform.  With T = double every dot product is a serial sum in
+ * Hessian lives in LDS as the full symmetric matrix.  Scalars are
+ * wave-uniform.  With T = double every dot product is a serial sum in
  * index order, as on the CPU.
  */
 #ifndef SGPU_CHAIN_H
@@ -32,26 +32,47 @@ __device__ inline T dot_n(T a, T b, int n)
 	return wave_sum(prod);
 }
 
-__device__ inline int hidx(int i, int j) { return i < j ? i + j * (j + 1) / 2 : j + i * (i + 1) / 2; }
+// The Hessian is held in LDS as the full symmetric n x n matrix, h(i, j) at
+// i * n + j, both halves written with the same value, so reads by column
+// are conflict-free.  An n x n sweep gives each lane one entry per pass:
+// HessMap holds the lane's (row offset, column) and the rows per pass.
+struct HessMap
+{
+	int rows; // rows per pass (64 / n)
+	int q;    // this lane's row within a pass
+	int c;    // this lane's column
+};
+__device__ inline HessMap hess_map(int n)
+{
+	const int lane = lane_id();
+	HessMap hm;
+	hm.rows = WAVE / n;
+	hm.q = lane / n;
+	hm.c = lane - hm.q * n;
+	return hm;
+}
 
 // minus_mat_vec_product (src/lib/bfgs.h:31-42)
 template<typename T>
 __device__ inline T minus_mat_vec(const T* h, T in, int n)
 {
 	const int lane = lane_id();
+	const T* col = h + (lane < n ? lane : 0);
 	T sum = 0;
+#pragma unroll 4
 	for (int j = 0; j < n; j++)
 	{
 		const T inj = readlane(in, j);
-		if (lane < n)
-			sum += h[hidx(lane, j)] * inj;
+		sum += col[j * n] * inj;
 	}
-	return -sum;
+	return lane < n ? -sum : -T(0);
 }
 
-// bfgs_update (src/lib/bfgs.h:52-72)
+// bfgs_update (src/lib/bfgs.h:52-72): entry (i, j) of the upper triangle
+// and its mirror take the same value, each computed by its own lane with
+// the CPU's expression on (min, max) of the two indices
 template<typename T>
-__device__ inline void bfgs_update(T* h, T p, T y, T alpha, int n)
+__device__ inline void bfgs_update(T* h, T* sp, T* smhy, T p, T y, T alpha, int n, const HessMap& hm)
 {
 	const T yp = dot_n(y, p, n);
 	if (alpha * yp < Real<T>::eps())
@@ -60,17 +81,23 @@ __device__ inline void bfgs_update(T* h, T p, T y, T alpha, int n)
 	const T yhy = -dot_n(y, mhy, n);
 	const T r = 1 / (alpha * yp);
 	const int lane = lane_id();
-	for (int j = 0; j < n; j++)
+	if (lane < n)
 	{
-		const T pj = readlane(p, j);
-		const T mhyj = readlane(mhy, j);
-		if (lane <= j)
+		sp[lane] = p;
+		smhy[lane] = mhy;
+	}
+	lsync();
+	for (int r0 = 0; r0 < n; r0 += hm.rows)
+	{
+		const int row = r0 + hm.q;
+		if (hm.q < hm.rows && row < n)
 		{
-			T& hij = h[lane + j * (j + 1) / 2];
-			hij += alpha * r * (mhy * pj + mhyj * p) + +alpha * alpha * (r * r * yhy + r) * p * pj;
+			const int i = row < hm.c ? row : hm.c, j = row < hm.c ? hm.c : row;
+			const T pi = sp[i], pj = sp[j], mi = smhy[i], mj = smhy[j];
+			h[row * n + hm.c] += alpha * r * (mi * pj + mj * pi) + +alpha * alpha * (r * r * yhy + r) * pi * pj;
 		}
 	}
-	wsync();
+	lsync();
 }
 
 // conf::increment (src/lib/conf.h:316-321) of the staged conf by
@@ -128,10 +155,17 @@ __device__ inline void stage(WaveMem<T>& W, T x, T p, int c, int n)
 		W.xstage[lane] = x;
 	if (lane < n)
 		W.pstage[lane] = p;
-	wsync();
+	lsync();
 }
 
-template<typename T>
+// bfgs (src/lib/bfgs.h:201-282) with fast_line_search (:76-96) or
+// accurate_line_search (:101-190), written as one loop around a single
+// model evaluation: the kernels inline the evaluation once.  Each pass of
+// the loop evaluates the conf the state machine asks for next: the start
+// point, then the line search's trials; a finished line search runs the
+// BFGS step body and starts the next line search.  The order of every
+// evaluation and every arithmetic operation is the CPU's.
+template<typename T, int BK>
 struct Searcher
 {
 	const MV<T>& m;
@@ -140,167 +174,197 @@ struct Searcher
 	T* pbuf;
 	int64_t evals;
 
-	__device__ T f(T x, T& g)
+	// the staged conf x plus alpha * p, this lane's element
+	__device__ __forceinline__ T trial_conf(T x, T p, T alpha)
 	{
-		++evals;
-		return eval_deriv(m, ec, W, pbuf, x, g);
+		stage(W, x, p, m.h.conf_size, m.h.change_size);
+		const T xn = conf_increment(m, W, alpha);
+		lsync();
+		return xn;
 	}
 
-	// fast_line_search (src/lib/bfgs.h:76-96)
-	__device__ T fast_line_search(T x, T g, T f0, T p, T& x_new, T& g_new, T& f1)
+	// x, g: this lane's elements; returns the final energy
+	__device__ __forceinline__ T bfgs(T& x, T& g, int maxiters, bool accurate, bool early_term)
 	{
 		const int n = m.h.change_size, c = m.h.conf_size;
-		const T c0 = T(0.0001);
-		const T multiplier = T(0.5);
-		T alpha = 1;
-		const T pg = dot_n(p, g, n);
-		for (int trial = 0; trial < 10; trial++)
+		const int lane = lane_id();
+		// identity
+		const HessMap hm = hess_map(n > 0 ? n : 1);
+		for (int r0 = 0; r0 < n; r0 += hm.rows)
 		{
-			stage(W, x, p, c, n);
-			x_new = conf_increment(m, W, alpha);
-			wsync();
-			f1 = f(x_new, g_new);
-			if (f1 - f0 < c0 * alpha * pg)
-				break;
-			alpha *= multiplier;
+			const int row = r0 + hm.q;
+			if (hm.q < hm.rows && row < n)
+				W.hess[row * n + hm.c] = row == hm.c ? T(1) : T(0);
 		}
-		return alpha;
-	}
-
-	// accurate_line_search (src/lib/bfgs.h:101-190)
-	__device__ T accurate_line_search(T x, T g, T f0, T p, T& x_new, T& g_new, T& f1)
-	{
-		const int n = m.h.change_size, c = m.h.conf_size;
-		const T ALF = T(1.0e-4);
-		const T FIRST = T(1.0);
-		T a, alpha, alpha2 = 0, alamin, b, disc, f2 = 0;
-		T rhs1, rhs2, slope, test, tmplam = 0;
-		T sum = dot_n(p, p, n);
-		sum = Real<T>::sqrt(sum);
-		(void)sum;
-		slope = dot_n(g, p, n);
-		if (slope >= 0)
+		lsync();
+		T f0 = 0, f_orig = 0, g_orig = 0, x_orig = x;
+		T p = 0, x_new = x, g_new = 0, f1 = 0;
+		T alpha = 1, slope = 0;   // slope: p.g (fast) or g.p (accurate)
+		T alamin = 0, alpha2 = 0, f2 = 0;
+		int trial = 0, step = 0;
+		bool initial = true;
+		T xe = x; // the conf evaluated next
+		for (;;)
 		{
-			x_new = x;
-			g_new = 0;
-			return 0;
-		}
-		stage(W, x, p, c, n);
-		const T xv = conf_value(m, W);
-		T temp = lane_id() < n ? Real<T>::abs(p) / fmax(Real<T>::abs(xv), T(1.0)) : T(0);
-		test = 0;
-		for (int i = 0; i < n; i++)
-		{
-			const T ti = readlane(temp, i);
-			if (ti > test)
-				test = ti;
-		}
-		alamin = Real<T>::eps() / test;
-		alpha = FIRST;
-		// every backtrack at least halves alpha, so a finite search ends
-		// within ~1100 trials; the cap only stops a NaN search
-		for (int trial = 0; trial < 2048; trial++)
-		{
-			stage(W, x, p, c, n);
-			x_new = conf_increment(m, W, alpha);
-			wsync();
-			f1 = f(x_new, g_new);
-			if (alpha < alamin)
+			T ge;
+			++evals;
+			const T fe = eval_deriv<T, BK>(m, ec, W, pbuf, xe, ge);
+			bool begin = false;
+			if (initial)
 			{
+				initial = false;
+				f0 = fe;
+				g = ge;
+				f_orig = f0;
+				g_orig = g;
+				x_orig = x;
 				x_new = x;
-				g_new = 0;
-				return 0;
+				g_new = g;
+				if (maxiters <= 0)
+					break;
+				begin = true;
 			}
-			else if (f1 <= f0 + ALF * alpha * slope)
-				return alpha;
 			else
 			{
-				if (alpha == FIRST)
-					tmplam = -slope / (T(2.0) * (f1 - f0 - slope));
-				else
+				// a line-search trial at alpha
+				f1 = fe;
+				g_new = ge;
+				x_new = xe;
+				bool done;
+				if (!accurate)
 				{
-					rhs1 = f1 - f0 - alpha * slope;
-					rhs2 = f2 - f0 - alpha2 * slope;
-					a = (rhs1 / (alpha * alpha) - rhs2 / (alpha2 * alpha2)) / (alpha - alpha2);
-					b = (-alpha2 * rhs1 / (alpha * alpha) + alpha * rhs2 / (alpha2 * alpha2)) / (alpha - alpha2);
-					if (a == T(0.0))
-						tmplam = -slope / (T(2.0) * b);
+					if (f1 - f0 < T(0.0001) * alpha * slope)
+						done = true;
 					else
 					{
-						disc = b * b - T(3.0) * a * slope;
-						if (disc < 0)
-							tmplam = T(0.5) * alpha;
-						else if (b <= 0)
-							tmplam = (-b + Real<T>::sqrt(disc)) / (T(3.0) * a);
-						else
-							tmplam = -slope / (b + Real<T>::sqrt(disc));
+						alpha *= T(0.5);
+						done = ++trial >= 10;
 					}
-					if (tmplam > T(.5) * alpha)
-						tmplam = T(.5) * alpha;
 				}
-			}
-			alpha2 = alpha;
-			f2 = f1;
-			alpha = fmax(tmplam, T(0.1) * alpha);
-		}
-		x_new = x;
-		g_new = 0;
-		return 0;
-	}
-
-	// bfgs (src/lib/bfgs.h:201-282); x, g: this lane's elements
-	__device__ T bfgs(T& x, T& g, int maxiters, bool accurate, bool early_term)
-	{
-		const int n = m.h.change_size;
-		const int lane = lane_id();
-		const int hn = n * (n + 1) / 2;
-		for (int k = lane; k < hn; k += WAVE)
-			W.hess[k] = 0;
-		wsync();
-		if (lane < n)
-			W.hess[lane + lane * (lane + 1) / 2] = 1;
-		wsync();
-		T f0 = f(x, g);
-		const T f_orig = f0;
-		const T g_orig = g, x_orig = x;
-		T x_new = x, g_new = g, p = g;
-		for (int step = 0; step < maxiters; step++)
-		{
-			p = minus_mat_vec(W.hess, g, n);
-			T f1 = 0;
-			T alpha;
-			if (accurate)
-				alpha = accurate_line_search(x, g, f0, p, x_new, g_new, f1);
-			else
-				alpha = fast_line_search(x, g, f0, p, x_new, g_new, f1);
-			if (alpha == 0)
-				break;
-			const T y = g_new - g;
-			const T prevf0 = f0;
-			f0 = f1;
-			x = x_new;
-			if (early_term)
-			{
-				const T diff = prevf0 - f0;
-				if (Real<T>::abs(diff) < T(1e-5))
+				else
+				{
+					const T ALF = T(1.0e-4), FIRST = T(1.0);
+					if (alpha < alamin)
+					{
+						x_new = x;
+						g_new = 0;
+						alpha = 0;
+						done = true;
+					}
+					else if (f1 <= f0 + ALF * alpha * slope)
+						done = true;
+					else
+					{
+						T tmplam;
+						if (alpha == FIRST)
+							tmplam = -slope / (T(2.0) * (f1 - f0 - slope));
+						else
+						{
+							const T rhs1 = f1 - f0 - alpha * slope;
+							const T rhs2 = f2 - f0 - alpha2 * slope;
+							const T a = (rhs1 / (alpha * alpha) - rhs2 / (alpha2 * alpha2)) / (alpha - alpha2);
+							const T b = (-alpha2 * rhs1 / (alpha * alpha) + alpha * rhs2 / (alpha2 * alpha2)) /
+									(alpha - alpha2);
+							if (a == T(0.0))
+								tmplam = -slope / (T(2.0) * b);
+							else
+							{
+								const T disc = b * b - T(3.0) * a * slope;
+								if (disc < 0)
+									tmplam = T(0.5) * alpha;
+								else if (b <= 0)
+									tmplam = (-b + Real<T>::sqrt(disc)) / (T(3.0) * a);
+								else
+									tmplam = -slope / (b + Real<T>::sqrt(disc));
+							}
+							if (tmplam > T(.5) * alpha)
+								tmplam = T(.5) * alpha;
+						}
+						alpha2 = alpha;
+						f2 = f1;
+						alpha = fmax(tmplam, T(0.1) * alpha);
+						// every backtrack at least halves alpha, so a finite
+						// search ends within ~1100 trials; the cap only stops
+						// a NaN search
+						if (++trial >= 2048)
+						{
+							x_new = x;
+							g_new = 0;
+							alpha = 0;
+							done = true;
+						}
+						else
+							done = false;
+					}
+				}
+				if (!done)
+				{
+					xe = trial_conf(x, p, alpha);
+					continue;
+				}
+				// the step body after the line search
+				if (alpha == 0)
+					break;
+				const T y = g_new - g;
+				const T prevf0 = f0;
+				f0 = f1;
+				x = x_new;
+				if (early_term)
+				{
+					const T diff = prevf0 - f0;
+					if (Real<T>::abs(diff) < T(1e-5))
+						break;
+				}
+				g = g_new;
+				const T gradnormsq = dot_n(g, g, n);
+				if (!(gradnormsq >= T(1e-4)))
+					break;
+				if (step == 0)
+				{
+					const T yy = dot_n(y, y, n);
+					if (Real<T>::abs(yy) > Real<T>::eps())
+					{
+						const T diag = alpha * dot_n(y, p, n) / yy;
+						if (lane < n)
+							W.hess[lane * n + lane] = diag;
+						lsync();
+					}
+				}
+				bfgs_update(W.hess, W.pstage, W.gbuf, p, y, alpha, n, hm);
+				begin = ++step < maxiters;
+				if (!begin)
 					break;
 			}
-			g = g_new;
-			const T gradnormsq = dot_n(g, g, n);
-			if (!(gradnormsq >= T(1e-4)))
-				break;
-			if (step == 0)
+			if (begin)
 			{
-				const T yy = dot_n(y, y, n);
-				if (Real<T>::abs(yy) > Real<T>::eps())
+				// the next step's direction and the start of its line search
+				p = minus_mat_vec(W.hess, g, n);
+				trial = 0;
+				alpha = 1;
+				if (!accurate)
+					slope = dot_n(p, g, n);
+				else
 				{
-					const T diag = alpha * dot_n(y, p, n) / yy;
-					if (lane < n)
-						W.hess[lane + lane * (lane + 1) / 2] = diag;
-					wsync();
+					(void)dot_n(p, p, n); // the unused norm of p
+					slope = dot_n(g, p, n);
+					if (slope >= 0)
+						break; // alpha 0
+					stage(W, x, p, c, n);
+					const T xv = conf_value(m, W);
+					const T temp = lane < n ? Real<T>::abs(p) / fmax(Real<T>::abs(xv), T(1.0)) : T(0);
+					T test = 0;
+					for (int i = 0; i < n; i++)
+					{
+						const T ti = readlane(temp, i);
+						if (ti > test)
+							test = ti;
+					}
+					alamin = Real<T>::eps() / test;
+					alpha2 = 0;
+					f2 = 0;
 				}
+				xe = trial_conf(x, p, alpha);
 			}
-			bfgs_update(W.hess, p, y, alpha, n);
 		}
 		if (!(f0 <= f_orig))
 		{
@@ -313,56 +377,61 @@ struct Searcher
 };
 
 // ------------------------------------------------------------------ RNG
+// MT19937's regeneration of the 624 words in place, lane-parallel.  Out of
+// line: it runs once per 624 draws and would otherwise be inlined at every
+// draw site.  new[k] uses old k, k+1 and (k+397 mod 624), which is already
+// new for k >= 227: three dependent ranges, chunks of 64 in order.
+static __device__ __noinline__ void mt_regenerate_wave(uint32_t* x)
+{
+	const int lane = lane_id();
+	for (int base = 0; base < 227; base += WAVE)
+	{
+		const int k = base + lane;
+		uint32_t v = 0;
+		if (k < 227)
+			v = mt_mix(x[k], x[k + 1], x[k + MT_M]);
+		gsync();
+		if (k < 227)
+			x[k] = v;
+		gsync();
+	}
+	for (int base = 227; base < 454; base += WAVE)
+	{
+		const int k = base + lane;
+		uint32_t v = 0;
+		if (k < 454)
+			v = mt_mix(x[k], x[k + 1], x[k - 227]);
+		gsync();
+		if (k < 454)
+			x[k] = v;
+		gsync();
+	}
+	for (int base = 454; base < MT_N; base += WAVE)
+	{
+		const int k = base + lane;
+		uint32_t v = 0;
+		if (k < MT_N)
+			v = mt_mix(x[k], x[(k + 1) % MT_N], x[k - 227]);
+		gsync();
+		if (k < MT_N)
+			x[k] = v;
+		gsync();
+	}
+}
+
 // the chain's MT19937 in global memory; every lane draws the same numbers
 struct WaveMT
 {
 	uint32_t* x; // 624 words
 	int i;
 
-	__device__ void regenerate()
-	{
-		const int lane = lane_id();
-		// new[k] uses old k, k+1 and (k+397 mod 624), which is already new for
-		// k >= 227: three dependent ranges, chunks of 64 in order
-		for (int base = 0; base < 227; base += WAVE)
-		{
-			const int k = base + lane;
-			uint32_t v = 0;
-			if (k < 227)
-				v = mt_mix(x[k], x[k + 1], x[k + MT_M]);
-			wsync();
-			if (k < 227)
-				x[k] = v;
-			wsync();
-		}
-		for (int base = 227; base < 454; base += WAVE)
-		{
-			const int k = base + lane;
-			uint32_t v = 0;
-			if (k < 454)
-				v = mt_mix(x[k], x[k + 1], x[k - 227]);
-			wsync();
-			if (k < 454)
-				x[k] = v;
-			wsync();
-		}
-		for (int base = 454; base < MT_N; base += WAVE)
-		{
-			const int k = base + lane;
-			uint32_t v = 0;
-			if (k < MT_N)
-				v = mt_mix(x[k], x[(k + 1) % MT_N], x[k - 227]);
-			wsync();
-			if (k < MT_N)
-				x[k] = v;
-			wsync();
-		}
-		i = 0;
-	}
-	__device__ uint32_t next()
+	__device__ __forceinline__ uint32_t next()
 	{
 		if (i >= MT_N)
-			regenerate();
+		{
+			mt_regenerate_wave(x);
+			i = 0;
+		}
 		const uint32_t v = x[i];
 		++i;
 		return mt_temper(v);
@@ -610,7 +679,7 @@ __device__ inline void add_to_output(const MV<T>& m, const WaveMem<T>& W, ChainS
 		}
 		if (lane == 0)
 			s.out_e[slot] = e;
-		wsync();
+		gsync();
 		if (lane == 0)
 		{ // stable insertion sort of the order by energy
 			for (int a = 1; a < count; a++)
@@ -627,7 +696,7 @@ __device__ inline void add_to_output(const MV<T>& m, const WaveMem<T>& W, ChainS
 			}
 			s.ints[1] = count;
 		}
-		wsync();
+		gsync();
 	}
 }
 

@@ -9,8 +9,6 @@
 namespace sgpu
 {
 
-constexpr int WAVES_PER_BLOCK = 1;
-
 struct WavePlan;
 template<typename T> struct EvalCtx;
 struct McParams;
@@ -20,10 +18,11 @@ template<typename T>
 hipError_t launch_eval(const BatchPtrs<T>& B, const EvalCtx<T>& ec, const WavePlan& plan, int n, const int* model,
 		const T* conf, int conf_stride, T* e, T* g, int g_stride, T* x, int x_stride, T* scratch, int pstride,
 		hipStream_t st);
+// nblocks workgroups of `waves` (1, 2, 4 or 8) chains of one ligand each
 template<typename T>
 hipError_t launch_mc(const BatchPtrs<T>& B, const EvalCtx<T>& ec, const WavePlan& plan, const McParams& P,
-		const ChainLayout& L, char* chains, int nchains, const int* chain_model, const int* chain_steps,
-		const int* chain_maxiters, int step_lo, int step_hi, hipStream_t st);
+		const ChainLayout& L, char* chains, int nblocks, int waves, const int* block_model, const int* slot_chain,
+		const int* chain_steps, const int* chain_maxiters, int step_lo, int step_hi, hipStream_t st);
 template<typename T>
 hipError_t launch_refine(const BatchPtrs<T>& B, const EvalCtx<T>& ec, const WavePlan& plan, int n, const int* model,
 		T* conf, int conf_stride, const int* maxiters, int accurate, int early_term, T* e, T* x, int x_stride,

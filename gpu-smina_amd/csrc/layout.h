@@ -1,22 +1,27 @@
 /*
  * Device-memory layout of the docking search (HBM, gfx950).
  *
- * A batch of models is packed into flat arrays; ModelHdr gives each model's
- * counts and offsets (the reference's ModelDesc / DevModels,
- * src/gpu/gpu_defs.h:98-115, src/gpu/gpu_dev_structs.h:137).  All offsets are
- * element offsets into the batch arrays.  Per model:
- *   atoms      type (movable then fixed), frame, local coordinates, initial
- *              coordinates (movable then fixed), per-atom pair lists (CSR:
- *              code = pair << 1 | (atom is the pair's b));
- *   frames     parent, kind, atom range, conf / change index, children list,
- *              origin and axis (segment: relative to the parent);
- *   levels     frames grouped by tree depth (set_conf top-down, derivative
- *              bottom-up);
- *   pairs      a, b, type-pair table index; other pairs first, then the
- *              ligand-internal pairs;
- *   change map kind and conf index of every change (gradient) element;
- *   trees      root frame, conf / change offsets, ligand flag, torsions,
- *              atom range.
+ * A batch of models is packed per model into one contiguous int blob and
+ * one contiguous real blob (the reference's FlatModel / DevModels,
+ * src/gpu/gpu_flatten_data.h:16-46, src/gpu/gpu_dev_structs.h:137).  A
+ * workgroup copies its model's two blobs into LDS with coalesced loads once,
+ * then every wave of the workgroup (one Monte Carlo chain each, all of the
+ * same ligand) reads the model from LDS only.  ModelHdr holds the counts
+ * and each array's offset inside the blobs.
+ *
+ * int blob:  atom type (movable then fixed atoms), atom frame, per-atom
+ *            neighbour lists (start, packed codes), frames (parent, kind,
+ *            atom range, conf / change index, children), tree levels, heavy
+ *            atoms, trees, conf / change maps;
+ * real blob: initial coordinates (movable then fixed), local coordinates,
+ *            frame origins and axes (segment: relative to the parent).
+ *
+ * A neighbour code is one 32-bit word: the partner atom (bits 0-7), the
+ * type-pair table index (8-16), "this atom is the pair's b" (17), "the pair
+ * is an other pair" (18) and the pair index (19-31), so the pair loop reads
+ * one LDS word per neighbour.  The pair-parallel loop of the FP32 search
+ * reads the pairs themselves, one word each: a (bits 0-7), b (8-15), the
+ * table index (16-24) and "other pair" (25).
  *
  * Receptor maps are stored per ligand type as corner-packed cells: for each
  * grid cell (x fastest), its 8 corner values in the order f000 f100 f010
@@ -26,16 +31,18 @@
 #ifndef SGPU_LAYOUT_H
 #define SGPU_LAYOUT_H
 
+#include <hip/hip_runtime.h>
+
 namespace sgpu
 {
 
 constexpr int WAVE = 64;
-constexpr int MAX_ATOMS = 256;    // movable atoms per model
-constexpr int MAX_FRAMES = 64;    // frames per model (one per lane)
-constexpr int MAX_CHANGE = 64;    // change (gradient) size (one per lane)
-constexpr int MAX_CONF = 64;      // conf size (one per lane)
+constexpr int MAX_ATOMS = 255;    // movable + fixed atoms per model (8-bit partner field)
+constexpr int MAX_PAIRS = 8191;   // 13-bit pair field
+constexpr int MAX_FRAMES = 64;
+constexpr int MAX_CHANGE = 64;
+constexpr int MAX_CONF = 64;
 constexpr int MAX_TREES = 16;
-constexpr int MAX_LEVELS = 64;
 constexpr int MAX_POSES = 64;
 
 // conf element kinds: position component, quaternion component a..d, torsion
@@ -53,61 +60,50 @@ enum ChangeKind
 	GK_TORSION = 4
 };
 
+__host__ __device__ inline int code_partner(int c) { return c & 255; }
+__host__ __device__ inline int code_tri(int c) { return (c >> 8) & 511; }
+__host__ __device__ inline bool code_is_b(int c) { return (c >> 17) & 1; }
+__host__ __device__ inline bool code_other(int c) { return (c >> 18) & 1; }
+__host__ __device__ inline int code_pair(int c) { return int(unsigned(c) >> 19); }
+__host__ __device__ inline int make_code(int partner, int tri, bool is_b, bool other, int pair)
+{
+	return int(unsigned(partner) | (unsigned(tri) << 8) | (unsigned(is_b) << 17) | (unsigned(other) << 18) |
+			(unsigned(pair) << 19));
+}
+
+__host__ __device__ inline int pcode_a(int c) { return c & 255; }
+__host__ __device__ inline int pcode_b(int c) { return (c >> 8) & 255; }
+__host__ __device__ inline int pcode_tri(int c) { return (c >> 16) & 511; }
+__host__ __device__ inline bool pcode_other(int c) { return (c >> 25) & 1; }
+__host__ __device__ inline int make_pcode(int a, int b, int tri, bool other)
+{
+	return int(unsigned(a) | (unsigned(b) << 8) | (unsigned(tri) << 16) | (unsigned(other) << 25));
+}
+
 struct ModelHdr
 {
 	int natoms, ninflex, nframes, ntrees, nligands, npairs, nother;
 	int conf_size, change_size, nheavy, nlevels, num_tors;
-	int atom_off;   // atom_type (natoms+ninflex), atom_xyz0 (3*(natoms+ninflex))
-	int mov_off;    // atom_frame, atom_local (3x), per movable atom
-	int nbr_off;    // nbr_start: natoms+1 entries
-	int code_off;   // nbr_code
-	int frame_off;  // frame arrays
-	int child_off;  // child_list
-	int level_off;  // level_start: nlevels+1 entries
-	int lvf_off;    // level_frames: nframes entries
-	int pair_off;
-	int heavy_off;  // heavy movable atom indices
-	int tree_off;
-	int map_off;    // conf_kind, conf_chg (conf_size each), change_kind, change_conf (change_size each)
+	int iblob_off, iblob_len; // in the batch int blob
+	int tblob_off, tblob_len; // in the batch real blob
+	int pair_off;             // pair arrays (a, b, table index), global
+	// offsets inside the int blob
+	int o_type, o_afr, o_nbr, o_code, o_pcode, o_fparent, o_fkind, o_fbegin, o_fend, o_fconf, o_fchange, o_cstart,
+			o_ccount, o_clist, o_lstart, o_lframes, o_heavy, o_troot, o_tconf, o_tchange, o_tlig, o_tntors,
+			o_tab, o_tae, o_tpb, o_tpe, o_maps;
+	// offsets inside the real blob
+	int o_xyz0, o_local, o_forig, o_faxis;
 };
 
 template<typename T>
 struct BatchPtrs
 {
 	const ModelHdr* hdr;
-	const int* atom_type;
-	const T* atom_xyz0;
-	const int* atom_frame;
-	const T* atom_local;
-	const int* nbr_start;
-	const int* nbr_code;
-	const int* frame_parent;
-	const int* frame_kind;
-	const int* frame_begin;
-	const int* frame_end;
-	const int* frame_conf;
-	const int* frame_change;
-	const int* child_start;
-	const int* child_count;
-	const int* child_list;
-	const T* frame_origin;
-	const T* frame_axis;
-	const int* level_start;
-	const int* level_frames;
+	const int* iblob;
+	const T* tblob;
 	const int* pair_a;
 	const int* pair_b;
 	const int* pair_tri;
-	const int* heavy;
-	const int* tree_root;
-	const int* tree_conf;
-	const int* tree_change;
-	const int* tree_ligand;
-	const int* tree_ntors;
-	const int* tree_atom_begin;
-	const int* tree_atom_end;
-	const int* tree_pair_begin;
-	const int* tree_pair_end;
-	const int* maps;
 };
 
 // scoring tables on the device: smooth as (e, dor) pairs per point; fast

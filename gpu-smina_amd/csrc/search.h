@@ -8,13 +8,16 @@
  *   atoms: atom i in lane i mod 64, coordinates and forces in LDS;
  *   frames: one per lane within a tree level (set_conf top-down, derivative
  *     bottom-up, level-serial as the tree requires);
- *   intramolecular pairs: atom-centric; lane i walks atom i's pair list in
- *     evaluation order and recomputes each pair force, so every atom's force
- *     accumulates in exactly smina's order with no atomics and no pair
- *     buffer; the first atom of a pair accounts for its energy;
+ *   intramolecular pairs: atom-centric; lane i walks atom i's neighbour
+ *     list (packed codes in LDS) in evaluation order and recomputes each
+ *     pair force, so every atom's force accumulates in exactly smina's order
+ *     with no atomics and no pair buffer; the pair's first atom accounts for
+ *     its energy;
  *   receptor maps: one corner-packed cell gather per heavy atom.
- * Control (BFGS, line search, Metropolis, RNG) is wave-uniform: every lane
- * computes the same scalars.
+ * The model itself (types, frames, neighbour lists) is staged in LDS once
+ * per workgroup; the hot loops read global memory only for the receptor map
+ * cells and the pair tables.  Control (BFGS, line search, Metropolis, RNG)
+ * is wave-uniform: every lane computes the same scalars.
  *
  * With T = double the search is "ordered": every sum runs in smina's CPU
  * order (energies serially over atoms and pairs in list order, dot products
@@ -32,16 +35,20 @@
 namespace sgpu
 {
 
-// sizes of a wave's LDS region (batch maxima)
+// LDS of a workgroup: the staged model (int blob, real blob, map slots)
+// then each wave's working state
 struct WavePlan
 {
-	int nmax;   // movable atoms
+	int nmax;   // movable atoms (multiple of 64)
 	int fmax;   // frames
 	int cmax;   // conf
 	int gmax;   // change
 	int pmax;   // pairs (FP64 energy scratch, global)
-	int hmax;   // packed Hessian entries
-	int bytes;  // LDS bytes per wave
+	int hmax;   // Hessian entries (change size squared)
+	int imax;   // model int blob
+	int tmax;   // model real blob
+	int amax;   // all atoms (movable + fixed)
+	int bytes;  // LDS bytes per workgroup
 };
 
 template<typename T>
@@ -66,6 +73,18 @@ template<typename T>
 __host__ __device__ inline int wave_lds_elems(const WavePlan& p)
 {
 	return 3 * p.nmax * 2 + p.fmax * (3 + 9 + 4 + 3 + 3 + 3) + p.gmax + p.nmax + p.cmax + p.gmax + p.hmax;
+}
+// model region in bytes, rounded to 16
+template<typename T>
+__host__ __device__ inline int model_lds_bytes(const WavePlan& p)
+{
+	const int ints = (p.imax + p.amax + 3) / 4 * 4;
+	return ints * 4 + (p.tmax * int(sizeof(T)) + 15) / 16 * 16;
+}
+template<typename T>
+__host__ __device__ inline int block_lds_bytes(const WavePlan& p, int waves)
+{
+	return model_lds_bytes<T>(p) + waves * wave_lds_elems<T>(p) * int(sizeof(T));
 }
 
 template<typename T>
@@ -101,6 +120,17 @@ __device__ inline WaveMem<T> carve(T* base, const WavePlan& p)
 	return w;
 }
 
+// receptor back ends: precomputed maps (smina's cache) or the receptor
+// atoms themselves (non_cache)
+enum Backend
+{
+	BK_MAPS = 0,
+	BK_DIRECT = 1
+};
+
+// neighbours per batch of the intramolecular pair loop
+constexpr int PAIR_BATCH = 4;
+
 // the evaluation back end
 template<typename T>
 struct EvalCtx
@@ -114,17 +144,20 @@ struct EvalCtx
 	int* error;        // set to 1 on an exact atom overlap (non_cache)
 };
 
-// the model a wave works on, with its per-model offsets applied
+// the model a wave works on: counts and arrays in LDS; the pair arrays
+// (used outside the hot loop) in global memory
 template<typename T>
 struct MV
 {
 	ModelHdr h;
 	const int* type;
+	const int* slot; // map slot of each movable atom's type
 	const T* xyz0;
 	const int* afr;
 	const T* local;
 	const int* nbr_start;
 	const int* code;
+	const int* pcode; // the pairs, packed (pair-parallel loop)
 	const int* fparent;
 	const int* fkind;
 	const int* fbegin;
@@ -157,45 +190,64 @@ struct MV
 	const int* gconf; // change element -> conf index (orientation: quaternion start)
 };
 
+// copy model m into the workgroup's LDS (every thread of the block takes
+// part; the caller synchronises the block afterwards) and view it there
 template<typename T>
-__device__ inline MV<T> model_view(const BatchPtrs<T>& B, int m)
+__device__ inline MV<T> stage_model(const BatchPtrs<T>& B, int m, unsigned char* lds, const WavePlan& plan,
+		const int* slot_of)
 {
+	const ModelHdr h = B.hdr[m];
+	int* li = reinterpret_cast<int*>(lds);
+	int* ls = li + plan.imax;
+	T* lt = reinterpret_cast<T*>(lds + ((plan.imax + plan.amax + 3) / 4 * 4) * 4);
+	const int tid = int(threadIdx.x), nt = int(blockDim.x);
+	for (int k = tid; k < h.iblob_len; k += nt)
+		li[k] = B.iblob[h.iblob_off + k];
+	for (int k = tid; k < h.tblob_len; k += nt)
+		lt[k] = B.tblob[h.tblob_off + k];
+	if (slot_of)
+		for (int k = tid; k < h.natoms; k += nt)
+		{
+			const int t = B.iblob[h.iblob_off + h.o_type + k];
+			ls[k] = is_hydrogen_type(t) ? 0 : slot_of[t];
+		}
 	MV<T> v;
-	v.h = B.hdr[m];
-	const ModelHdr& h = v.h;
-	v.type = B.atom_type + h.atom_off;
-	v.xyz0 = B.atom_xyz0 + 3 * h.atom_off;
-	v.afr = B.atom_frame + h.mov_off;
-	v.local = B.atom_local + 3 * h.mov_off;
-	v.nbr_start = B.nbr_start + h.nbr_off;
-	v.code = B.nbr_code + h.code_off;
-	v.fparent = B.frame_parent + h.frame_off;
-	v.fkind = B.frame_kind + h.frame_off;
-	v.fbegin = B.frame_begin + h.frame_off;
-	v.fend = B.frame_end + h.frame_off;
-	v.fconf = B.frame_conf + h.frame_off;
-	v.fchange = B.frame_change + h.frame_off;
-	v.cstart = B.child_start + h.frame_off;
-	v.ccount = B.child_count + h.frame_off;
-	v.clist = B.child_list + h.child_off;
-	v.forig = B.frame_origin + 3 * h.frame_off;
-	v.faxis = B.frame_axis + 3 * h.frame_off;
-	v.lstart = B.level_start + h.level_off;
-	v.lframes = B.level_frames + h.lvf_off;
+	v.h = h;
+	v.type = li + h.o_type;
+	v.slot = ls;
+	v.xyz0 = lt + h.o_xyz0;
+	v.afr = li + h.o_afr;
+	v.local = lt + h.o_local;
+	v.nbr_start = li + h.o_nbr;
+	v.code = li + h.o_code;
+	v.pcode = li + h.o_pcode;
+	v.fparent = li + h.o_fparent;
+	v.fkind = li + h.o_fkind;
+	v.fbegin = li + h.o_fbegin;
+	v.fend = li + h.o_fend;
+	v.fconf = li + h.o_fconf;
+	v.fchange = li + h.o_fchange;
+	v.cstart = li + h.o_cstart;
+	v.ccount = li + h.o_ccount;
+	v.clist = li + h.o_clist;
+	v.forig = lt + h.o_forig;
+	v.faxis = lt + h.o_faxis;
+	v.lstart = li + h.o_lstart;
+	v.lframes = li + h.o_lframes;
 	v.pa = B.pair_a + h.pair_off;
 	v.pb = B.pair_b + h.pair_off;
 	v.ptri = B.pair_tri + h.pair_off;
-	v.heavy = B.heavy + h.heavy_off;
-	v.troot = B.tree_root + h.tree_off;
-	v.tconf = B.tree_conf + h.tree_off;
-	v.tchange = B.tree_change + h.tree_off;
-	v.tlig = B.tree_ligand + h.tree_off;
-	v.tntors = B.tree_ntors + h.tree_off;
-	v.tab_ = B.tree_atom_begin + h.tree_off;
-	v.tae = B.tree_atom_end + h.tree_off;
-	v.tpb = B.tree_pair_begin + h.tree_off;
-	v.tpe = B.tree_pair_end + h.tree_off;
-	v.ckind = B.maps + h.map_off;
+	v.heavy = li + h.o_heavy;
+	v.troot = li + h.o_troot;
+	v.tconf = li + h.o_tconf;
+	v.tchange = li + h.o_tchange;
+	v.tlig = li + h.o_tlig;
+	v.tntors = li + h.o_tntors;
+	v.tab_ = li + h.o_tab;
+	v.tae = li + h.o_tae;
+	v.tpb = li + h.o_tpb;
+	v.tpe = li + h.o_tpe;
+	v.ckind = li + h.o_maps;
 	v.cchg = v.ckind + h.conf_size;
 	v.gkind = v.cchg + h.conf_size;
 	v.gconf = v.gkind + h.change_size;
@@ -207,12 +259,12 @@ __device__ inline MV<T> model_view(const BatchPtrs<T>& B, int m)
 // (rigid_body::set_conf, first_segment::set_conf, segment::set_conf,
 // src/lib/tree.h:111-195), then every atom x = origin + R * local
 template<typename T>
-__device__ void set_conf(const MV<T>& m, WaveMem<T>& W, T x_lane)
+__device__ __forceinline__ void set_conf(const MV<T>& m, WaveMem<T>& W, T x_lane)
 {
 	const int lane = lane_id();
 	if (lane < m.h.conf_size)
 		W.xstage[lane] = x_lane;
-	wsync();
+	lsync();
 	for (int L = 0; L < m.h.nlevels; L++)
 	{
 		const int s = m.lstart[L], e = m.lstart[L + 1];
@@ -254,7 +306,7 @@ __device__ void set_conf(const MV<T>& m, WaveMem<T>& W, T x_lane)
 			W.fq[4 * f + 3] = q.d;
 			q_to_mat(q, W.fR + 9 * f);
 		}
-		wsync();
+		lsync();
 	}
 	for (int i = lane; i < m.h.natoms; i += WAVE)
 	{
@@ -262,7 +314,7 @@ __device__ void set_conf(const MV<T>& m, WaveMem<T>& W, T x_lane)
 		const V3<T> x = ld3(W.fo + 3 * f) + mat_vec(W.fR + 9 * f, ld3(m.local + 3 * i));
 		st3(W.coords + 3 * i, x);
 	}
-	wsync();
+	lsync();
 }
 
 template<typename T>
@@ -418,78 +470,185 @@ __device__ inline T direct_term(const EvalCtx<T>& ec, int t1, V3<T> ac, V3<T>& o
 
 // ------------------------------------------------------------------ eval_deriv
 // model::eval_deriv (src/lib/model.cpp:969-983).  x: conf element in this
-// lane; returns e (uniform) and the gradient element in g_lane.
-template<typename T>
-__device__ T eval_deriv(const MV<T>& m, const EvalCtx<T>& ec, WaveMem<T>& W, T* pbuf, T x_lane, T& g_lane)
+// lane; returns e (uniform) and the gradient element in g_lane.  pbuf: the
+// wave's pair-energy scratch in global memory (ordered mode only).  BK: the
+// receptor back end (BK_MAPS or BK_DIRECT), fixed per kernel.
+template<typename T, int BK>
+__device__ __forceinline__ T eval_deriv(const MV<T>& m, const EvalCtx<T>& ec, WaveMem<T>& W, T* pbuf, T x_lane,
+		T& g_lane)
 {
 	const int lane = lane_id();
 	set_conf(m, W, x_lane);
 	const int N = m.h.natoms;
+	const T cutoff_sqr = T(ec.tab.cutoff_sqr);
 	T e_part = 0;
-	// inter: receptor maps or direct receptor, one heavy atom per lane
-	for (int i = lane; i < N; i += WAVE)
+	if constexpr (Real<T>::ordered)
 	{
-		const int t = m.type[i];
-		V3<T> d = v3(T(0), T(0), T(0));
-		T e = 0;
-		if (t < NUM_ATOM_TYPES && !is_hydrogen_type(t))
+		// atom-centric: every force accumulates in smina's order
+		for (int i = lane; i < N; i += WAVE)
 		{
-			const V3<T> x = ld3(W.coords + 3 * i);
-			if (ec.backend == 0)
-				e = grid_term(ec.grid, ec.grid.slot_of[t], x, ec.v[1], d);
-			else
-				e = direct_term(ec, t, x, d);
-		}
-		W.ebuf[i] = e;
-		e_part += e;
-		// intramolecular pairs of atom i, in evaluation order
-		T fx = d.x, fy = d.y, fz = d.z;
-		const V3<T> xi = ld3(W.coords + 3 * i);
-		for (int k = m.nbr_start[i]; k < m.nbr_start[i + 1]; k++)
-		{
-			const int code = m.code[k];
-			const int p = code >> 1;
-			const bool is_b = (code & 1) != 0;
-			const int a = m.pa[p], b = m.pb[p];
-			const V3<T> xa = is_b ? atom_pos(m, W, a) : xi;
-			const V3<T> xb = is_b ? xi : atom_pos(m, W, b);
-			const V3<T> r = xb - xa;
-			const T r2 = norm_sqr(r);
-			T pe = 0;
-			if (r2 < T(ec.tab.cutoff_sqr))
+			const int t = m.type[i];
+			const V3<T> xi = ld3(W.coords + 3 * i);
+			V3<T> d = v3(T(0), T(0), T(0));
+			T e = 0;
+			if (!is_hydrogen_type(t))
 			{
-				T dor;
-				table_deriv(ec.tab, m.ptri[p], r2, pe, dor);
-				V3<T> f = dor * r;
-				curl(pe, f, p < m.h.nother ? ec.v[2] : ec.v[0]);
-				if (is_b)
-				{
-					fx += f.x;
-					fy += f.y;
-					fz += f.z;
-				}
+				if constexpr (BK == BK_MAPS)
+					e = grid_term(ec.grid, m.slot[i], xi, ec.v[1], d);
 				else
+					e = direct_term(ec, t, xi, d);
+			}
+			W.ebuf[i] = e;
+			// intramolecular pairs of atom i, in evaluation order, PB at a time:
+			// the batch's table rows are fetched together, then accumulated in
+			// list order
+			T fx = d.x, fy = d.y, fz = d.z;
+			const int k0 = m.nbr_start[i], k1 = m.nbr_start[i + 1];
+			for (int k = k0; k < k1; k += PAIR_BATCH)
+			{
+				int cd[PAIR_BATCH];
+				T rx[PAIR_BATCH], ry[PAIR_BATCH], rz[PAIR_BATCH], r2[PAIR_BATCH], rem[PAIR_BATCH];
+				T e1[PAIR_BATCH], d1[PAIR_BATCH], e2[PAIR_BATCH], d2[PAIR_BATCH];
+#pragma unroll
+				for (int u = 0; u < PAIR_BATCH; u++)
 				{
-					fx -= f.x;
-					fy -= f.y;
-					fz -= f.z;
+					const int kk = k + u < k1 ? k + u : k0;
+					const int code = m.code[kk];
+					cd[u] = code;
+					const int j = code_partner(code);
+					const bool is_b = code_is_b(code);
+					const V3<T> xj = atom_pos(m, W, j);
+					// r = x_b - x_a
+					rx[u] = is_b ? xi.x - xj.x : xj.x - xi.x;
+					ry[u] = is_b ? xi.y - xj.y : xj.y - xi.y;
+					rz[u] = is_b ? xi.z - xj.z : xj.z - xi.z;
+					r2[u] = sqr(rx[u]) + sqr(ry[u]) + sqr(rz[u]);
+					// precalculate_linear_element::eval_deriv (src/lib/precalculate.h:111-151)
+					const T r2f = T(ec.tab.factor) * r2[u];
+					const int i1 = r2[u] < cutoff_sqr ? int(r2f) : 0;
+					rem[u] = r2f - T(i1);
+					const T* tp = ec.tab.smooth + (size_t(code_tri(code)) * ec.tab.n + i1) * 2;
+					e1[u] = tp[0];
+					d1[u] = tp[1];
+					e2[u] = tp[2];
+					d2[u] = tp[3];
+				}
+#pragma unroll
+				for (int u = 0; u < PAIR_BATCH; u++)
+				{
+					if (k + u >= k1)
+						break;
+					const int code = cd[u];
+					const bool is_b = code_is_b(code);
+					T pe = 0;
+					if (r2[u] < cutoff_sqr)
+					{
+						pe = e1[u] + rem[u] * (e2[u] - e1[u]);
+						const T dor = d1[u] + rem[u] * (d2[u] - d1[u]);
+						V3<T> f = v3(dor * rx[u], dor * ry[u], dor * rz[u]);
+						curl(pe, f, code_other(code) ? ec.v[2] : ec.v[0]);
+						if (is_b)
+						{
+							fx += f.x;
+							fy += f.y;
+							fz += f.z;
+						}
+						else
+						{
+							fx -= f.x;
+							fy -= f.y;
+							fz -= f.z;
+						}
+					}
+					if (!is_b)
+						pbuf[code_pair(code)] = pe;
 				}
 			}
-			if (!is_b)
-			{
-				if (Real<T>::ordered)
-					pbuf[p] = pe;
-				else
-					e_part += pe;
-			}
+			st3(W.forces + 3 * i, v3(fx, fy, fz));
 		}
-		st3(W.forces + 3 * i, v3(fx, fy, fz));
 	}
-	wsync();
-	// energy
+	else
+	{
+		// receptor term of every movable atom, its derivative seeding the force
+		for (int i = lane; i < N; i += WAVE)
+		{
+			const int t = m.type[i];
+			V3<T> d = v3(T(0), T(0), T(0));
+			if (!is_hydrogen_type(t))
+			{
+				const V3<T> xi = ld3(W.coords + 3 * i);
+				if constexpr (BK == BK_MAPS)
+					e_part += grid_term(ec.grid, m.slot[i], xi, ec.v[1], d);
+				else
+					e_part += direct_term(ec, t, xi, d);
+			}
+			st3(W.forces + 3 * i, d);
+		}
+		lsync();
+		// intramolecular pairs, pair-parallel: lane l takes the contiguous
+		// run [l Q, (l + 1) Q) so that one instruction's lanes touch
+		// different atoms; forces accumulate with LDS atomics (one wave owns
+		// the force array, so the order is fixed by the program)
+		const int P = m.h.npairs;
+		const int Q = (P + WAVE - 1) / WAVE;
+		for (int t0 = 0; t0 < Q; t0 += PAIR_BATCH)
+		{
+			int cd[PAIR_BATCH];
+			T rx[PAIR_BATCH], ry[PAIR_BATCH], rz[PAIR_BATCH], r2[PAIR_BATCH], rem[PAIR_BATCH];
+			T e1[PAIR_BATCH], d1[PAIR_BATCH], e2[PAIR_BATCH], d2[PAIR_BATCH];
+			bool ok[PAIR_BATCH];
+#pragma unroll
+			for (int u = 0; u < PAIR_BATCH; u++)
+			{
+				const int p = lane * Q + t0 + u;
+				ok[u] = t0 + u < Q && p < P;
+				const int code = m.pcode[ok[u] ? p : 0];
+				cd[u] = code;
+				const V3<T> xa = ld3(W.coords + 3 * pcode_a(code));
+				const V3<T> xb = atom_pos(m, W, pcode_b(code));
+				rx[u] = xb.x - xa.x;
+				ry[u] = xb.y - xa.y;
+				rz[u] = xb.z - xa.z;
+				r2[u] = sqr(rx[u]) + sqr(ry[u]) + sqr(rz[u]);
+				ok[u] = ok[u] && r2[u] < cutoff_sqr;
+				// precalculate_linear_element::eval_deriv (src/lib/precalculate.h:111-151)
+				const T r2f = T(ec.tab.factor) * r2[u];
+				const int i1 = ok[u] ? int(r2f) : 0;
+				rem[u] = r2f - T(i1);
+				const T* tp = ec.tab.smooth + (size_t(pcode_tri(code)) * ec.tab.n + i1) * 2;
+				e1[u] = tp[0];
+				d1[u] = tp[1];
+				e2[u] = tp[2];
+				d2[u] = tp[3];
+			}
+#pragma unroll
+			for (int u = 0; u < PAIR_BATCH; u++)
+			{
+				if (!ok[u])
+					continue;
+				const int code = cd[u];
+				T pe = e1[u] + rem[u] * (e2[u] - e1[u]);
+				const T dor = d1[u] + rem[u] * (d2[u] - d1[u]);
+				V3<T> f = v3(dor * rx[u], dor * ry[u], dor * rz[u]);
+				curl(pe, f, pcode_other(code) ? ec.v[2] : ec.v[0]);
+				e_part += pe;
+				const int a = pcode_a(code), b = pcode_b(code);
+				atomicAdd(W.forces + 3 * a, -f.x);
+				atomicAdd(W.forces + 3 * a + 1, -f.y);
+				atomicAdd(W.forces + 3 * a + 2, -f.z);
+				if (b < N)
+				{
+					atomicAdd(W.forces + 3 * b, f.x);
+					atomicAdd(W.forces + 3 * b + 1, f.y);
+					atomicAdd(W.forces + 3 * b + 2, f.z);
+				}
+			}
+		}
+	}
 	T e;
 	if (Real<T>::ordered)
 	{
+		gsync(); // the pair energies went through global memory
 		e = 0;
 		for (int i = 0; i < N; i++)
 			e += W.ebuf[i];
@@ -508,14 +667,19 @@ __device__ T eval_deriv(const MV<T>& m, const EvalCtx<T>& ec, WaveMem<T>& W, T* 
 		}
 	}
 	else
+	{
+		lsync();
 		e = wave_sum(e_part);
+	}
 	// derivative: per-frame sums, then the tree bottom-up
 	const int F = m.h.nframes;
 	for (int f = lane; f < F; f += WAVE)
 	{
 		const V3<T> o = ld3(W.fo + 3 * f);
 		V3<T> fo = v3(T(0), T(0), T(0)), to = v3(T(0), T(0), T(0));
-		for (int i = m.fbegin[f]; i < m.fend[f]; i++)
+		const int b = m.fbegin[f], en = m.fend[f];
+#pragma unroll 4
+		for (int i = b; i < en; i++)
 		{
 			const V3<T> fi = ld3(W.forces + 3 * i);
 			fo = fo + fi;
@@ -524,7 +688,7 @@ __device__ T eval_deriv(const MV<T>& m, const EvalCtx<T>& ec, WaveMem<T>& W, T* 
 		st3(W.ff + 3 * f, fo);
 		st3(W.ft + 3 * f, to);
 	}
-	wsync();
+	lsync();
 	for (int L = m.h.nlevels - 1; L >= 0; L--)
 	{
 		const int s = m.lstart[L], en = m.lstart[L + 1];
@@ -533,9 +697,10 @@ __device__ T eval_deriv(const MV<T>& m, const EvalCtx<T>& ec, WaveMem<T>& W, T* 
 			const int f = m.lframes[j];
 			V3<T> fo = ld3(W.ff + 3 * f), to = ld3(W.ft + 3 * f);
 			const V3<T> o = ld3(W.fo + 3 * f);
-			for (int k = 0; k < m.ccount[f]; k++)
+			const int cs = m.cstart[f], cc = m.ccount[f];
+			for (int k = 0; k < cc; k++)
 			{
-				const int c = m.clist[m.cstart[f] + k];
+				const int c = m.clist[cs + k];
 				const V3<T> cf = ld3(W.ff + 3 * c);
 				fo = fo + cf;
 				const V3<T> r = ld3(W.fo + 3 * c) - o;
@@ -556,10 +721,10 @@ __device__ T eval_deriv(const MV<T>& m, const EvalCtx<T>& ec, WaveMem<T>& W, T* 
 			else
 				W.gbuf[gi] = dot(to, ld3(W.fax + 3 * f));
 		}
-		wsync();
+		lsync();
 	}
 	g_lane = lane < m.h.change_size ? W.gbuf[lane] : T(0);
-	wsync();
+	lsync();
 	return e;
 }
 

@@ -21,9 +21,19 @@ namespace sgpu
 __device__ inline int lane_id() { return int(threadIdx.x) & (WAVE - 1); }
 __device__ inline int wave_in_block() { return int(threadIdx.x) / WAVE; }
 
-// order LDS and global accesses of one wavefront (all of its lanes) before
-// the accesses that follow; waves never wait for each other
-__device__ inline void wsync()
+// Syncs inside one wavefront; waves never wait for each other.
+// lsync: LDS written by some lanes is read by others next.  A wavefront's
+// LDS instructions execute in order, so this only has to stop the compiler
+// from reordering them (a wavefront-scope fence emits no wait).
+__device__ inline void lsync()
+{
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// gsync: global memory written by some lanes is read by others next: wait
+// for the stores (workgroup scope: the same CU's L1 serves the reads)
+__device__ inline void gsync()
 {
 	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
 	__builtin_amdgcn_wave_barrier();
