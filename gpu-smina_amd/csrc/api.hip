@@ -920,7 +920,8 @@ sgpu_status sgpu_upload_models(sgpu_ctx* c, const sgpu_flat_model* models, int n
 		b->n = n;
 		pack_batch(models, n, b->h);
 		b->h.plan.bytes = plan_bytes(b->h.plan, precision, 1);
-		require(plan_bytes(b->h.plan, SGPU_FP64, 1) <= LDS_LIMIT, "model too large for one workgroup's LDS");
+		require(plan_bytes(b->h.plan, SGPU_FP64, 1) <= LDS_LIMIT && plan_bytes(b->h.plan, SGPU_FP32, 1) <= LDS_LIMIT,
+				"model too large for one workgroup's LDS");
 		upload_batch(*b, c->stream);
 		if (precision == SGPU_FP32)
 		{

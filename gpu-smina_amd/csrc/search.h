@@ -35,6 +35,17 @@
 namespace sgpu
 {
 
+// neighbours per batch of the intramolecular pair loop
+constexpr int PAIR_BATCH = 4;
+// the unordered search evaluates each pair once, pair-parallel, and sums
+// each atom's pair forces from LDS; the ordered one walks each atom's
+// neighbour list in smina's order, evaluating every pair from both ends
+template<typename T>
+constexpr bool pair_parallel()
+{
+	return !Real<T>::ordered;
+}
+
 // LDS of a workgroup: the staged model (int blob, real blob, map slots)
 // then each wave's working state
 struct WavePlan
@@ -43,7 +54,7 @@ struct WavePlan
 	int fmax;   // frames
 	int cmax;   // conf
 	int gmax;   // change
-	int pmax;   // pairs (FP64 energy scratch, global)
+	int pmax;   // pairs (FP64: energy scratch in global memory; FP32: pair forces in LDS)
 	int hmax;   // Hessian entries (change size squared)
 	int imax;   // model int blob
 	int tmax;   // model real blob
@@ -67,12 +78,14 @@ struct WaveMem
 	T* xstage;
 	T* pstage;
 	T* hess;
+	T* pforce; // pair forces (pair-parallel search only)
 };
 
 template<typename T>
 __host__ __device__ inline int wave_lds_elems(const WavePlan& p)
 {
-	return 3 * p.nmax * 2 + p.fmax * (3 + 9 + 4 + 3 + 3 + 3) + p.gmax + p.nmax + p.cmax + p.gmax + p.hmax;
+	return 3 * p.nmax * 2 + p.fmax * (3 + 9 + 4 + 3 + 3 + 3) + p.gmax + p.nmax + p.cmax + p.gmax + p.hmax +
+			(pair_parallel<T>() ? 3 * p.pmax : 0);
 }
 // model region in bytes, rounded to 16
 template<typename T>
@@ -117,6 +130,8 @@ __device__ inline WaveMem<T> carve(T* base, const WavePlan& p)
 	w.pstage = q;
 	q += p.gmax;
 	w.hess = q;
+	q += p.hmax;
+	w.pforce = q;
 	return w;
 }
 
@@ -127,9 +142,6 @@ enum Backend
 	BK_MAPS = 0,
 	BK_DIRECT = 1
 };
-
-// neighbours per batch of the intramolecular pair loop
-constexpr int PAIR_BATCH = 4;
 
 // the evaluation back end
 template<typename T>
@@ -482,7 +494,7 @@ __device__ __forceinline__ T eval_deriv(const MV<T>& m, const EvalCtx<T>& ec, Wa
 	const int N = m.h.natoms;
 	const T cutoff_sqr = T(ec.tab.cutoff_sqr);
 	T e_part = 0;
-	if constexpr (Real<T>::ordered)
+	if constexpr (!pair_parallel<T>())
 	{
 		// atom-centric: every force accumulates in smina's order
 		for (int i = lane; i < N; i += WAVE)
@@ -498,7 +510,10 @@ __device__ __forceinline__ T eval_deriv(const MV<T>& m, const EvalCtx<T>& ec, Wa
 				else
 					e = direct_term(ec, t, xi, d);
 			}
-			W.ebuf[i] = e;
+			if constexpr (Real<T>::ordered)
+				W.ebuf[i] = e;
+			else
+				e_part += e;
 			// intramolecular pairs of atom i, in evaluation order, PB at a time:
 			// the batch's table rows are fetched together, then accumulated in
 			// list order
@@ -561,7 +576,12 @@ __device__ __forceinline__ T eval_deriv(const MV<T>& m, const EvalCtx<T>& ec, Wa
 						}
 					}
 					if (!is_b)
-						pbuf[code_pair(code)] = pe;
+					{
+						if constexpr (Real<T>::ordered)
+							pbuf[code_pair(code)] = pe;
+						else
+							e_part += pe;
+					}
 				}
 			}
 			st3(W.forces + 3 * i, v3(fx, fy, fz));
@@ -569,40 +589,21 @@ __device__ __forceinline__ T eval_deriv(const MV<T>& m, const EvalCtx<T>& ec, Wa
 	}
 	else
 	{
-		// receptor term of every movable atom, its derivative seeding the force
-		for (int i = lane; i < N; i += WAVE)
-		{
-			const int t = m.type[i];
-			V3<T> d = v3(T(0), T(0), T(0));
-			if (!is_hydrogen_type(t))
-			{
-				const V3<T> xi = ld3(W.coords + 3 * i);
-				if constexpr (BK == BK_MAPS)
-					e_part += grid_term(ec.grid, m.slot[i], xi, ec.v[1], d);
-				else
-					e_part += direct_term(ec, t, xi, d);
-			}
-			st3(W.forces + 3 * i, d);
-		}
-		lsync();
-		// intramolecular pairs, pair-parallel: lane l takes the contiguous
-		// run [l Q, (l + 1) Q) so that one instruction's lanes touch
-		// different atoms; forces accumulate with LDS atomics (one wave owns
-		// the force array, so the order is fixed by the program)
+		// intramolecular pairs, pair-parallel (pair t * 64 + lane), each
+		// pair's force into its own LDS slot
 		const int P = m.h.npairs;
-		const int Q = (P + WAVE - 1) / WAVE;
-		for (int t0 = 0; t0 < Q; t0 += PAIR_BATCH)
+		for (int t0 = 0; t0 * WAVE < P; t0 += PAIR_BATCH)
 		{
 			int cd[PAIR_BATCH];
 			T rx[PAIR_BATCH], ry[PAIR_BATCH], rz[PAIR_BATCH], r2[PAIR_BATCH], rem[PAIR_BATCH];
 			T e1[PAIR_BATCH], d1[PAIR_BATCH], e2[PAIR_BATCH], d2[PAIR_BATCH];
-			bool ok[PAIR_BATCH];
+			bool in[PAIR_BATCH], ok[PAIR_BATCH];
 #pragma unroll
 			for (int u = 0; u < PAIR_BATCH; u++)
 			{
-				const int p = lane * Q + t0 + u;
-				ok[u] = t0 + u < Q && p < P;
-				const int code = m.pcode[ok[u] ? p : 0];
+				const int p = (t0 + u) * WAVE + lane;
+				in[u] = p < P;
+				const int code = m.pcode[in[u] ? p : 0];
 				cd[u] = code;
 				const V3<T> xa = ld3(W.coords + 3 * pcode_a(code));
 				const V3<T> xb = atom_pos(m, W, pcode_b(code));
@@ -610,7 +611,7 @@ __device__ __forceinline__ T eval_deriv(const MV<T>& m, const EvalCtx<T>& ec, Wa
 				ry[u] = xb.y - xa.y;
 				rz[u] = xb.z - xa.z;
 				r2[u] = sqr(rx[u]) + sqr(ry[u]) + sqr(rz[u]);
-				ok[u] = ok[u] && r2[u] < cutoff_sqr;
+				ok[u] = in[u] && r2[u] < cutoff_sqr;
 				// precalculate_linear_element::eval_deriv (src/lib/precalculate.h:111-151)
 				const T r2f = T(ec.tab.factor) * r2[u];
 				const int i1 = ok[u] ? int(r2f) : 0;
@@ -624,25 +625,45 @@ __device__ __forceinline__ T eval_deriv(const MV<T>& m, const EvalCtx<T>& ec, Wa
 #pragma unroll
 			for (int u = 0; u < PAIR_BATCH; u++)
 			{
-				if (!ok[u])
-					continue;
-				const int code = cd[u];
-				T pe = e1[u] + rem[u] * (e2[u] - e1[u]);
-				const T dor = d1[u] + rem[u] * (d2[u] - d1[u]);
-				V3<T> f = v3(dor * rx[u], dor * ry[u], dor * rz[u]);
-				curl(pe, f, pcode_other(code) ? ec.v[2] : ec.v[0]);
-				e_part += pe;
-				const int a = pcode_a(code), b = pcode_b(code);
-				atomicAdd(W.forces + 3 * a, -f.x);
-				atomicAdd(W.forces + 3 * a + 1, -f.y);
-				atomicAdd(W.forces + 3 * a + 2, -f.z);
-				if (b < N)
+				V3<T> f = v3(T(0), T(0), T(0));
+				if (ok[u])
 				{
-					atomicAdd(W.forces + 3 * b, f.x);
-					atomicAdd(W.forces + 3 * b + 1, f.y);
-					atomicAdd(W.forces + 3 * b + 2, f.z);
+					T pe = e1[u] + rem[u] * (e2[u] - e1[u]);
+					const T dor = d1[u] + rem[u] * (d2[u] - d1[u]);
+					f = v3(dor * rx[u], dor * ry[u], dor * rz[u]);
+					curl(pe, f, pcode_other(cd[u]) ? ec.v[2] : ec.v[0]);
+					e_part += pe;
 				}
+				if (in[u])
+					st3(W.pforce + 3 * ((t0 + u) * WAVE + lane), f);
 			}
+		}
+		lsync();
+		// per movable atom: receptor term, then its pair forces in list order
+		for (int i = lane; i < N; i += WAVE)
+		{
+			const int t = m.type[i];
+			V3<T> d = v3(T(0), T(0), T(0));
+			if (!is_hydrogen_type(t))
+			{
+				const V3<T> xi = ld3(W.coords + 3 * i);
+				if constexpr (BK == BK_MAPS)
+					e_part += grid_term(ec.grid, m.slot[i], xi, ec.v[1], d);
+				else
+					e_part += direct_term(ec, t, xi, d);
+			}
+			const int k1 = m.nbr_start[i + 1];
+#pragma unroll 4
+			for (int k = m.nbr_start[i]; k < k1; k++)
+			{
+				const int code = m.code[k];
+				const V3<T> f = ld3(W.pforce + 3 * code_pair(code));
+				if (code_is_b(code))
+					d = d + f;
+				else
+					d = d - f;
+			}
+			st3(W.forces + 3 * i, d);
 		}
 	}
 	T e;

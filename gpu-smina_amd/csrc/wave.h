@@ -76,6 +76,8 @@ template<> struct Real<float>
 	__device__ static float ceil(float x) { return ::ceilf(x); }
 	__device__ static float eps() { return FLT_EPSILON; }
 	__device__ static float max() { return FLT_MAX; }
+	// a / b through the hardware reciprocal (1 ulp), for the served search
+	__device__ static float fdiv(float a, float b) { return __fdividef(a, b); }
 };
 template<> struct Real<double>
 {
@@ -88,6 +90,7 @@ template<> struct Real<double>
 	__device__ static double ceil(double x) { return __builtin_ceil(x); }
 	__device__ static double eps() { return DBL_EPSILON; }
 	__device__ static double max() { return DBL_MAX; }
+	__device__ static double fdiv(double a, double b) { return a / b; }
 };
 
 template<typename T>
@@ -244,7 +247,7 @@ template<typename T> __device__ inline void curl(T& e, V3<T>& d, T v)
 {
 	if (e > 0)
 	{
-		T tmp = (v < Real<T>::eps()) ? T(0) : (v / (v + e));
+		T tmp = (v < Real<T>::eps()) ? T(0) : Real<T>::fdiv(v, v + e);
 		e *= tmp;
 		const T t2 = sqr(tmp);
 		d.x *= t2;
