@@ -146,6 +146,10 @@ struct sgpu_ctx
 	int device = 0;
 	hipStream_t stream = nullptr;
 	hipEvent_t ev0 = nullptr, ev1 = nullptr;
+	// size classes of one Monte Carlo launch run concurrently, one stream each
+	static constexpr int NCLASS = 4;
+	hipStream_t class_stream[NCLASS] = {};
+	hipEvent_t class_done[NCLASS] = {};
 	std::mutex mu;
 	std::string err;
 	Tables tab;
@@ -162,10 +166,38 @@ struct sgpu_ctx
 namespace
 {
 
+// the LDS plan that fits every listed model of a packed batch
+WavePlan plan_for(const HostBatch& B, const std::vector<int>& models)
+{
+	int nmax = 0, fmax = 0, gmax = 0, pmax = 0, imax = 0, tmax = 0, amax = 0;
+	for (int mi : models)
+	{
+		const ModelHdr& h = B.hdr[size_t(mi)];
+		nmax = std::max(nmax, h.natoms);
+		amax = std::max(amax, h.natoms + h.ninflex);
+		fmax = std::max(fmax, h.nframes);
+		gmax = std::max(gmax, h.change_size);
+		pmax = std::max(pmax, h.npairs);
+		imax = std::max(imax, h.iblob_len);
+		tmax = std::max(tmax, h.tblob_len);
+	}
+	WavePlan p{};
+	p.nmax = std::max(WAVE, (nmax + WAVE - 1) / WAVE * WAVE);
+	p.fmax = fmax;
+	p.cmax = WAVE;
+	p.gmax = WAVE;
+	p.pmax = std::max(1, pmax);
+	p.hmax = std::max(1, gmax * gmax);
+	p.imax = imax;
+	p.tmax = tmax;
+	p.amax = amax;
+	p.bytes = 0;
+	return p;
+}
+
 // flat models -> per-model int and real blobs with packed neighbour codes
 void pack_batch(const sgpu_flat_model* ms, int n, HostBatch& B)
 {
-	int nmax = 0, fmax = 0, gmax = 0, pmax = 0, imax = 0, tmax = 0, amax = 0;
 	for (int mi = 0; mi < n; mi++)
 	{
 		const sgpu_flat_model& m = ms[mi];
@@ -255,6 +287,16 @@ void pack_batch(const sgpu_flat_model* ms, int n, HostBatch& B)
 		I.insert(I.end(), m.frame_conf, m.frame_conf + m.nframes);
 		mark(h.o_fchange);
 		I.insert(I.end(), m.frame_change, m.frame_change + m.nframes);
+		// end of each frame's subtree atoms: a subtree is a run of frames in
+		// preorder whose atom ranges follow each other
+		std::vector<int> send(m.frame_end, m.frame_end + m.nframes);
+		for (int f = m.nframes - 1; f > 0; f--)
+			if (m.frame_parent[f] >= 0)
+				send[size_t(m.frame_parent[f])] = std::max(send[size_t(m.frame_parent[f])], send[size_t(f)]);
+		for (int f = 0; f + 1 < m.nframes; f++)
+			require(m.frame_end[f] == m.frame_begin[f + 1], "frame atom ranges must follow preorder");
+		mark(h.o_fsend);
+		I.insert(I.end(), send.begin(), send.end());
 		std::vector<int> cstart, ccount, clist;
 		for (int f = 0; f < m.nframes; f++)
 		{
@@ -414,25 +456,11 @@ void pack_batch(const sgpu_flat_model* ms, int n, HostBatch& B)
 		B.pair_b.insert(B.pair_b.end(), m.pair_b, m.pair_b + m.npairs);
 		B.pair_tri.insert(B.pair_tri.end(), m.pair_tri, m.pair_tri + m.npairs);
 		B.hdr.push_back(h);
-		nmax = std::max(nmax, m.natoms);
-		amax = std::max(amax, na);
-		fmax = std::max(fmax, m.nframes);
-		gmax = std::max(gmax, m.change_size);
-		pmax = std::max(pmax, m.npairs);
-		imax = std::max(imax, h.iblob_len);
-		tmax = std::max(tmax, h.tblob_len);
 	}
-	WavePlan& p = B.plan;
-	p.nmax = std::max(WAVE, (nmax + WAVE - 1) / WAVE * WAVE);
-	p.fmax = fmax;
-	p.cmax = WAVE;
-	p.gmax = WAVE;
-	p.pmax = std::max(1, pmax);
-	p.hmax = std::max(1, gmax * gmax);
-	p.imax = imax;
-	p.tmax = tmax;
-	p.amax = amax;
-	p.bytes = 0;
+	std::vector<int> all(static_cast<size_t>(n));
+	for (int mi = 0; mi < n; mi++)
+		all[size_t(mi)] = mi;
+	B.plan = plan_for(B, all);
 }
 
 template<typename T>
@@ -710,6 +738,10 @@ sgpu_status sgpu_create(int device, sgpu_ctx** out)
 	if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
 			hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
 		return SGPU_E_HIP;
+	for (int k = 0; k < sgpu_ctx::NCLASS; k++)
+		if (hipStreamCreateWithFlags(&c->class_stream[k], hipStreamNonBlocking) != hipSuccess ||
+				hipEventCreateWithFlags(&c->class_done[k], hipEventDisableTiming) != hipSuccess)
+			return SGPU_E_HIP;
 	*out = c.release();
 	return SGPU_OK;
 }
@@ -725,6 +757,12 @@ void sgpu_destroy(sgpu_ctx* c)
 	c->receptors.clear();
 	(void)hipEventDestroy(c->ev0);
 	(void)hipEventDestroy(c->ev1);
+	for (int k = 0; k < sgpu_ctx::NCLASS; k++)
+	{
+		(void)hipStreamSynchronize(c->class_stream[k]);
+		(void)hipEventDestroy(c->class_done[k]);
+		(void)hipStreamDestroy(c->class_stream[k]);
+	}
 	(void)hipStreamDestroy(c->stream);
 	delete c;
 }
@@ -1097,49 +1135,74 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 		scal[0] = 0;
 		scal[1] = Real_host_max<T>();
 	}
-	// workgroups of `waves` chains of one ligand: chains grouped by model in
-	// input order, longest runs (steps x atoms) launched first
-	int maxper = 1;
-	{
-		std::map<int, int> per;
-		for (int i = 0; i < n; i++)
-			maxper = std::max(maxper, ++per[ch->model[i]]);
-	}
-	int waves = 8;
-	while (waves > 1 && waves / 2 >= maxper)
-		waves /= 2;
-	while (waves > 1 && plan_bytes(b.h.plan, b.precision, waves) > LDS_LIMIT)
-		waves /= 2;
-	require(plan_bytes(b.h.plan, b.precision, waves) <= LDS_LIMIT, "chain state exceeds LDS");
+	// Launch plan.  Ligands fall into size classes by pair count; each class
+	// gets its own LDS plan (sized for its largest member) and runs on its own
+	// stream, concurrently with the others.  Within a class, workgroups hold
+	// `waves` chains of one ligand (the largest power of two up to 8 that the
+	// chains per ligand and 160 KB of LDS allow), costliest first.
 	std::map<int, std::vector<int>> by_model;
 	for (int i = 0; i < n; i++)
 		by_model[ch->model[i]].push_back(i);
+	auto class_of = [&](int mi) {
+		const int np = b.h.hdr[mi].npairs;
+		return np <= 256 ? 0 : np <= 512 ? 1 : np <= 1024 ? 2 : 3;
+	};
 	struct Blk
 	{
 		double cost;
 		int model;
 		std::vector<int> chains;
 	};
-	std::vector<Blk> blks;
-	for (auto& kv : by_model)
-		for (size_t s = 0; s < kv.second.size(); s += size_t(waves))
-		{
-			Blk bk{0, kv.first, {}};
-			for (size_t k = s; k < std::min(kv.second.size(), s + size_t(waves)); k++)
-			{
-				bk.chains.push_back(kv.second[k]);
-				bk.cost = std::max(bk.cost, double(ch->num_steps[kv.second[k]]) *
-								double(b.h.hdr[kv.first].npairs + 4 * b.h.hdr[kv.first].natoms));
-			}
-			blks.push_back(bk);
-		}
-	std::stable_sort(blks.begin(), blks.end(), [](const Blk& a, const Blk& b2) { return a.cost > b2.cost; });
-	std::vector<int> block_model, slot_chain;
-	for (const Blk& bk : blks)
+	struct Cls
 	{
-		block_model.push_back(bk.model);
-		for (int k = 0; k < waves; k++)
-			slot_chain.push_back(k < int(bk.chains.size()) ? bk.chains[k] : -1);
+		std::vector<int> models;
+		WavePlan plan;
+		int waves = 1;
+		std::vector<Blk> blks;
+		size_t block_off = 0, slot_off = 0;
+	};
+	Cls cls[sgpu_ctx::NCLASS];
+	for (auto& kv : by_model)
+		cls[class_of(kv.first)].models.push_back(kv.first);
+	std::vector<int> block_model, slot_chain;
+	for (Cls& k : cls)
+	{
+		if (k.models.empty())
+			continue;
+		k.plan = plan_for(b.h, k.models);
+		int maxper = 1;
+		for (int mi : k.models)
+			maxper = std::max(maxper, int(by_model[mi].size()));
+		k.waves = 8;
+		while (k.waves > 1 && k.waves / 2 >= maxper)
+			k.waves /= 2;
+		while (k.waves > 1 && plan_bytes(k.plan, b.precision, k.waves) > LDS_LIMIT)
+			k.waves /= 2;
+		require(plan_bytes(k.plan, b.precision, k.waves) <= LDS_LIMIT, "chain state exceeds LDS");
+		for (int mi : k.models)
+		{
+			const std::vector<int>& cs = by_model[mi];
+			for (size_t s = 0; s < cs.size(); s += size_t(k.waves))
+			{
+				Blk bk{0, mi, {}};
+				for (size_t j = s; j < std::min(cs.size(), s + size_t(k.waves)); j++)
+				{
+					bk.chains.push_back(cs[j]);
+					bk.cost = std::max(bk.cost, double(ch->num_steps[cs[j]]) *
+									double(b.h.hdr[mi].npairs + 4 * b.h.hdr[mi].natoms));
+				}
+				k.blks.push_back(bk);
+			}
+		}
+		std::stable_sort(k.blks.begin(), k.blks.end(), [](const Blk& x, const Blk& y) { return x.cost > y.cost; });
+		k.block_off = block_model.size();
+		k.slot_off = slot_chain.size();
+		for (const Blk& bk : k.blks)
+		{
+			block_model.push_back(bk.model);
+			for (int j = 0; j < k.waves; j++)
+				slot_chain.push_back(j < int(bk.chains.size()) ? bk.chains[j] : -1);
+		}
 	}
 	DBuf dchains, dbm, dsc, dsteps, dmax;
 	upload(dchains, init, c->stream);
@@ -1168,9 +1231,20 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 	for (int i = 0; i < n; i++)
 		max_steps = std::max(max_steps, ch->num_steps[i]);
 	Timer tm(c, "mc");
-	hip_check(launch_mc<T>(ptrs<T>(b), ec, b.h.plan, P, L, dchains.as<char>(), int(blks.size()), waves, dbm.as<int>(),
-					  dsc.as<int>(), dsteps.as<int>(), dmax.as<int>(), 0, max_steps, c->stream),
-			"mc kernel");
+	for (int k = 0; k < sgpu_ctx::NCLASS; k++)
+	{
+		const Cls& cl = cls[k];
+		if (cl.blks.empty())
+			continue;
+		hipStream_t st = c->class_stream[k];
+		hip_check(hipStreamWaitEvent(st, c->ev0, 0), "stream wait");
+		hip_check(launch_mc<T>(ptrs<T>(b), ec, cl.plan, P, L, dchains.as<char>(), int(cl.blks.size()), cl.waves,
+						  dbm.as<int>() + cl.block_off, dsc.as<int>() + cl.slot_off, dsteps.as<int>(), dmax.as<int>(),
+						  0, max_steps, st),
+				"mc kernel");
+		hip_check(hipEventRecord(c->class_done[k], st), "event");
+		hip_check(hipStreamWaitEvent(c->stream, c->class_done[k], 0), "stream wait");
+	}
 	tm.stop();
 	hip_check(hipMemcpyAsync(init.data(), dchains.p, init.size(), hipMemcpyDeviceToHost, c->stream), "download");
 	hip_check(hipStreamSynchronize(c->stream), "mc");

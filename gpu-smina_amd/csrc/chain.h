@@ -87,13 +87,19 @@ __device__ inline void bfgs_update(T* h, T* sp, T* smhy, T p, T y, T alpha, int 
 		smhy[lane] = mhy;
 	}
 	lsync();
+	// this lane's column values, then one row value pair per pass
+	const bool col_ok = hm.q < hm.rows && hm.c < n;
+	const T pc = col_ok ? sp[hm.c] : T(0), mc = col_ok ? smhy[hm.c] : T(0);
+#pragma unroll 2
 	for (int r0 = 0; r0 < n; r0 += hm.rows)
 	{
 		const int row = r0 + hm.q;
-		if (hm.q < hm.rows && row < n)
+		if (col_ok && row < n)
 		{
-			const int i = row < hm.c ? row : hm.c, j = row < hm.c ? hm.c : row;
-			const T pi = sp[i], pj = sp[j], mi = smhy[i], mj = smhy[j];
+			const T pr = sp[row], mr = smhy[row];
+			// (i, j) = (min, max) of (row, column)
+			const bool lower = row < hm.c;
+			const T pi = lower ? pr : pc, pj = lower ? pc : pr, mi = lower ? mr : mc, mj = lower ? mc : mr;
 			h[row * n + hm.c] += alpha * r * (mi * pj + mj * pi) + +alpha * alpha * (r * r * yhy + r) * pi * pj;
 		}
 	}
@@ -520,13 +526,29 @@ __device__ inline double gyration_radius(const MV<T>& m, const WaveMem<T>& W, in
 	const V3<T> o = ld3(W.fo + 3 * root);
 	T acc = 0;
 	unsigned counter = 0;
-	for (int i = m.tab_[tree]; i < m.tae[tree]; i++)
-		if (!is_hydrogen_type(m.type[i]))
-		{
-			const V3<T> x = ld3(W.coords + 3 * i);
-			acc += sqr(x.x - o.x) + sqr(x.y - o.y) + sqr(x.z - o.z);
-			++counter;
-		}
+	if constexpr (Real<T>::ordered)
+	{
+		for (int i = m.tab_[tree]; i < m.tae[tree]; i++)
+			if (!is_hydrogen_type(m.type[i]))
+			{
+				const V3<T> x = ld3(W.coords + 3 * i);
+				acc += sqr(x.x - o.x) + sqr(x.y - o.y) + sqr(x.z - o.z);
+				++counter;
+			}
+	}
+	else
+	{
+		T part = 0, cnt = 0;
+		for (int i = m.tab_[tree] + lane_id(); i < m.tae[tree]; i += WAVE)
+			if (!is_hydrogen_type(m.type[i]))
+			{
+				const V3<T> x = ld3(W.coords + 3 * i);
+				part += sqr(x.x - o.x) + sqr(x.y - o.y) + sqr(x.z - o.z);
+				cnt += 1;
+			}
+		acc = wave_sum(part);
+		counter = unsigned(wave_sum(cnt));
+	}
 	return counter > 0 ? double(Real<T>::sqrt(acc / T(counter))) : 0.0;
 }
 

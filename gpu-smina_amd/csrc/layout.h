@@ -88,7 +88,7 @@ struct ModelHdr
 	int tblob_off, tblob_len; // in the batch real blob
 	int pair_off;             // pair arrays (a, b, table index), global
 	// offsets inside the int blob
-	int o_type, o_afr, o_nbr, o_code, o_pcode, o_fparent, o_fkind, o_fbegin, o_fend, o_fconf, o_fchange, o_cstart,
+	int o_type, o_afr, o_nbr, o_code, o_pcode, o_fparent, o_fkind, o_fbegin, o_fend, o_fconf, o_fchange, o_fsend, o_cstart,
 			o_ccount, o_clist, o_lstart, o_lframes, o_heavy, o_troot, o_tconf, o_tchange, o_tlig, o_tntors,
 			o_tab, o_tae, o_tpb, o_tpe, o_maps;
 	// offsets inside the real blob

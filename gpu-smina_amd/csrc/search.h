@@ -37,6 +37,8 @@ namespace sgpu
 
 // neighbours per batch of the intramolecular pair loop
 constexpr int PAIR_BATCH = 4;
+// list entries per batch of the FP32 per-atom pair-force gather
+constexpr int GATHER_BATCH = 4;
 // the unordered search evaluates each pair once, pair-parallel, and sums
 // each atom's pair forces from LDS; the ordered one walks each atom's
 // neighbour list in smina's order, evaluating every pair from both ends
@@ -176,6 +178,7 @@ struct MV
 	const int* fend;
 	const int* fconf;
 	const int* fchange;
+	const int* fsend; // end of the frame's subtree atoms
 	const int* cstart;
 	const int* ccount;
 	const int* clist;
@@ -239,6 +242,7 @@ __device__ inline MV<T> stage_model(const BatchPtrs<T>& B, int m, unsigned char*
 	v.fend = li + h.o_fend;
 	v.fconf = li + h.o_fconf;
 	v.fchange = li + h.o_fchange;
+	v.fsend = li + h.o_fsend;
 	v.cstart = li + h.o_cstart;
 	v.ccount = li + h.o_ccount;
 	v.clist = li + h.o_clist;
@@ -336,48 +340,69 @@ __device__ inline V3<T> atom_pos(const MV<T>& m, const WaveMem<T>& W, int i)
 }
 
 // ------------------------------------------------------- receptor map term
-// grid::evaluate_aux (src/lib/grid.cpp:111-236) on a corner-packed cell
+// grid::evaluate_aux (src/lib/grid.cpp:111-236) on a corner-packed cell, in
+// two halves so that callers can overlap the gather with other work:
+// grid_fetch locates the cell and loads its 8 corners, grid_finish
+// interpolates, curls and adds the out-of-box penalty
 template<typename T>
-__device__ inline T grid_term(const GridPtrs<T>& g, int slot, V3<T> loc, T v, V3<T>& deriv)
+struct GridSample
 {
-	T s[3] = {(loc.x - T(g.init[0])) * T(g.factor[0]), (loc.y - T(g.init[1])) * T(g.factor[1]),
-			(loc.z - T(g.init[2])) * T(g.factor[2])};
-	T miss[3] = {0, 0, 0};
+	T c[8];     // f000 f100 f010 f110 f001 f101 f011 f111
+	T s[3];     // position inside the cell
+	T miss[3];  // distance outside the box along each axis (grid units)
 	int region[3];
+};
+
+template<typename T>
+__device__ inline GridSample<T> grid_fetch(const GridPtrs<T>& g, int slot, V3<T> loc)
+{
+	GridSample<T> r;
+	r.s[0] = (loc.x - T(g.init[0])) * T(g.factor[0]);
+	r.s[1] = (loc.y - T(g.init[1])) * T(g.factor[1]);
+	r.s[2] = (loc.z - T(g.init[2])) * T(g.factor[2]);
 	int a[3];
 	for (int i = 0; i < 3; i++)
 	{
 		const T dm1 = T(g.dim_fl_minus_1[i]);
-		if (s[i] < 0)
+		r.miss[i] = 0;
+		if (r.s[i] < 0)
 		{
-			miss[i] = -s[i];
-			region[i] = -1;
+			r.miss[i] = -r.s[i];
+			r.region[i] = -1;
 			a[i] = 0;
-			s[i] = 0;
+			r.s[i] = 0;
 		}
-		else if (s[i] >= dm1)
+		else if (r.s[i] >= dm1)
 		{
-			miss[i] = s[i] - dm1;
-			region[i] = 1;
+			r.miss[i] = r.s[i] - dm1;
+			r.region[i] = 1;
 			a[i] = g.dim[i] - 2;
-			s[i] = 1;
+			r.s[i] = 1;
 		}
 		else
 		{
-			region[i] = 0;
-			a[i] = int(s[i]);
-			s[i] -= T(a[i]);
+			r.region[i] = 0;
+			a[i] = int(r.s[i]);
+			r.s[i] -= T(a[i]);
 		}
 		// in range for every finite location; keeps a NaN location's gather in bounds
 		a[i] = a[i] < 0 ? 0 : (a[i] > g.dim[i] - 2 ? g.dim[i] - 2 : a[i]);
 	}
-	const T penalty = T(g.slope) * (miss[0] * T(g.factor_inv[0]) + miss[1] * T(g.factor_inv[1]) +
-			miss[2] * T(g.factor_inv[2]));
 	const size_t cell = size_t(a[0]) + size_t(g.dim[0] - 1) * (size_t(a[1]) + size_t(g.dim[1] - 1) * a[2]);
 	const T* c = g.cells + (size_t(slot) * g.ncells + cell) * 8;
-	const T f000 = c[0], f100 = c[1], f010 = c[2], f110 = c[3];
-	const T f001 = c[4], f101 = c[5], f011 = c[6], f111 = c[7];
-	const T x = s[0], y = s[1], z = s[2];
+	for (int k = 0; k < 8; k++)
+		r.c[k] = c[k];
+	return r;
+}
+
+template<typename T>
+__device__ inline T grid_finish(const GridPtrs<T>& g, const GridSample<T>& r, T v, V3<T>& deriv)
+{
+	const T penalty = T(g.slope) * (r.miss[0] * T(g.factor_inv[0]) + r.miss[1] * T(g.factor_inv[1]) +
+			r.miss[2] * T(g.factor_inv[2]));
+	const T f000 = r.c[0], f100 = r.c[1], f010 = r.c[2], f110 = r.c[3];
+	const T f001 = r.c[4], f101 = r.c[5], f011 = r.c[6], f111 = r.c[7];
+	const T x = r.s[0], y = r.s[1], z = r.s[2];
 	const T mx = 1 - x, my = 1 - y, mz = 1 - z;
 	T f = f000 * mx * my * mz + f100 * x * my * mz + f010 * mx * y * mz + f110 * x * y * mz +
 			f001 * mx * my * z + f101 * x * my * z + f011 * mx * y * z + f111 * x * y * z;
@@ -394,11 +419,17 @@ __device__ inline T grid_term(const GridPtrs<T>& g, int slot, V3<T> loc, T v, V3
 	T d[3];
 	for (int i = 0; i < 3; i++)
 	{
-		const T ge = region[i] == 0 ? gr[i] : T(0);
-		d[i] = T(g.factor[i]) * ge + T(g.slope) * T(region[i]);
+		const T ge = r.region[i] == 0 ? gr[i] : T(0);
+		d[i] = T(g.factor[i]) * ge + T(g.slope) * T(r.region[i]);
 	}
 	deriv = v3(d[0], d[1], d[2]);
 	return f + penalty;
+}
+
+template<typename T>
+__device__ inline T grid_term(const GridPtrs<T>& g, int slot, V3<T> loc, T v, V3<T>& deriv)
+{
+	return grid_finish(g, grid_fetch(g, slot, loc), v, deriv);
 }
 
 // precalculate_linear_element::eval_deriv, one component
@@ -594,29 +625,44 @@ __device__ __forceinline__ T eval_deriv(const MV<T>& m, const EvalCtx<T>& ec, Wa
 		const int P = m.h.npairs;
 		for (int t0 = 0; t0 * WAVE < P; t0 += PAIR_BATCH)
 		{
+			// phases across the batch: codes, coordinates, table rows, then
+			// the arithmetic, so each phase's loads are in flight together
 			int cd[PAIR_BATCH];
-			T rx[PAIR_BATCH], ry[PAIR_BATCH], rz[PAIR_BATCH], r2[PAIR_BATCH], rem[PAIR_BATCH];
-			T e1[PAIR_BATCH], d1[PAIR_BATCH], e2[PAIR_BATCH], d2[PAIR_BATCH];
-			bool in[PAIR_BATCH], ok[PAIR_BATCH];
+			bool in[PAIR_BATCH];
 #pragma unroll
 			for (int u = 0; u < PAIR_BATCH; u++)
 			{
 				const int p = (t0 + u) * WAVE + lane;
 				in[u] = p < P;
-				const int code = m.pcode[in[u] ? p : 0];
-				cd[u] = code;
-				const V3<T> xa = ld3(W.coords + 3 * pcode_a(code));
-				const V3<T> xb = atom_pos(m, W, pcode_b(code));
-				rx[u] = xb.x - xa.x;
-				ry[u] = xb.y - xa.y;
-				rz[u] = xb.z - xa.z;
+				cd[u] = m.pcode[in[u] ? p : 0];
+			}
+			V3<T> xa[PAIR_BATCH], xb[PAIR_BATCH];
+#pragma unroll
+			for (int u = 0; u < PAIR_BATCH; u++)
+			{
+				xa[u] = ld3(W.coords + 3 * pcode_a(cd[u]));
+				xb[u] = atom_pos(m, W, pcode_b(cd[u]));
+			}
+			T rx[PAIR_BATCH], ry[PAIR_BATCH], rz[PAIR_BATCH], r2[PAIR_BATCH];
+#pragma unroll
+			for (int u = 0; u < PAIR_BATCH; u++)
+			{
+				rx[u] = xb[u].x - xa[u].x;
+				ry[u] = xb[u].y - xa[u].y;
+				rz[u] = xb[u].z - xa[u].z;
 				r2[u] = sqr(rx[u]) + sqr(ry[u]) + sqr(rz[u]);
-				ok[u] = in[u] && r2[u] < cutoff_sqr;
+			}
+			T rem[PAIR_BATCH], e1[PAIR_BATCH], d1[PAIR_BATCH], e2[PAIR_BATCH], d2[PAIR_BATCH];
+			bool ok[PAIR_BATCH];
+#pragma unroll
+			for (int u = 0; u < PAIR_BATCH; u++)
+			{
 				// precalculate_linear_element::eval_deriv (src/lib/precalculate.h:111-151)
+				ok[u] = in[u] && r2[u] < cutoff_sqr;
 				const T r2f = T(ec.tab.factor) * r2[u];
 				const int i1 = ok[u] ? int(r2f) : 0;
 				rem[u] = r2f - T(i1);
-				const T* tp = ec.tab.smooth + (size_t(pcode_tri(code)) * ec.tab.n + i1) * 2;
+				const T* tp = ec.tab.smooth + (size_t(pcode_tri(cd[u])) * ec.tab.n + i1) * 2;
 				e1[u] = tp[0];
 				d1[u] = tp[1];
 				e2[u] = tp[2];
@@ -625,45 +671,59 @@ __device__ __forceinline__ T eval_deriv(const MV<T>& m, const EvalCtx<T>& ec, Wa
 #pragma unroll
 			for (int u = 0; u < PAIR_BATCH; u++)
 			{
-				V3<T> f = v3(T(0), T(0), T(0));
-				if (ok[u])
-				{
-					T pe = e1[u] + rem[u] * (e2[u] - e1[u]);
-					const T dor = d1[u] + rem[u] * (d2[u] - d1[u]);
-					f = v3(dor * rx[u], dor * ry[u], dor * rz[u]);
-					curl(pe, f, pcode_other(cd[u]) ? ec.v[2] : ec.v[0]);
-					e_part += pe;
-				}
+				// interpolation and curl without branches (src/lib/curl.h:28-43)
+				T pe = e1[u] + rem[u] * (e2[u] - e1[u]);
+				const T dor = d1[u] + rem[u] * (d2[u] - d1[u]);
+				const T v = pcode_other(cd[u]) ? ec.v[2] : ec.v[0];
+				const T tmp = v < Real<T>::eps() ? T(0) : Real<T>::fdiv(v, v + pe);
+				const T k = pe > 0 ? tmp : T(1);
+				pe *= k;
+				const T fk = ok[u] ? dor * (k * k) : T(0);
+				e_part += ok[u] ? pe : T(0);
 				if (in[u])
-					st3(W.pforce + 3 * ((t0 + u) * WAVE + lane), f);
+					st3(W.pforce + 3 * ((t0 + u) * WAVE + lane), v3(fk * rx[u], fk * ry[u], fk * rz[u]));
 			}
 		}
 		lsync();
-		// per movable atom: receptor term, then its pair forces in list order
+		// per movable atom: receptor term, then its pair forces in list order;
+		// the map gather is issued first and consumed last, so its latency
+		// overlaps the list walk
 		for (int i = lane; i < N; i += WAVE)
 		{
 			const int t = m.type[i];
-			V3<T> d = v3(T(0), T(0), T(0));
-			if (!is_hydrogen_type(t))
+			const bool heavy = !is_hydrogen_type(t);
+			const V3<T> xi = ld3(W.coords + 3 * i);
+			GridSample<T> gs;
+			if constexpr (BK == BK_MAPS)
+				gs = grid_fetch(ec.grid, heavy ? m.slot[i] : 0, xi);
+			V3<T> fp = v3(T(0), T(0), T(0));
+			const int k0 = m.nbr_start[i], k1 = m.nbr_start[i + 1];
+			for (int k = k0; k < k1; k += GATHER_BATCH)
 			{
-				const V3<T> xi = ld3(W.coords + 3 * i);
+				int cd[GATHER_BATCH];
+#pragma unroll
+				for (int u = 0; u < GATHER_BATCH; u++)
+					cd[u] = m.code[k + u < k1 ? k + u : k0];
+				V3<T> f[GATHER_BATCH];
+#pragma unroll
+				for (int u = 0; u < GATHER_BATCH; u++)
+					f[u] = ld3(W.pforce + 3 * code_pair(cd[u]));
+#pragma unroll
+				for (int u = 0; u < GATHER_BATCH; u++)
+				{
+					const T sg = k + u >= k1 ? T(0) : (code_is_b(cd[u]) ? T(1) : T(-1));
+					fp = v3(fp.x + sg * f[u].x, fp.y + sg * f[u].y, fp.z + sg * f[u].z);
+				}
+			}
+			V3<T> d = v3(T(0), T(0), T(0));
+			if (heavy)
+			{
 				if constexpr (BK == BK_MAPS)
-					e_part += grid_term(ec.grid, m.slot[i], xi, ec.v[1], d);
+					e_part += grid_finish(ec.grid, gs, ec.v[1], d);
 				else
 					e_part += direct_term(ec, t, xi, d);
 			}
-			const int k1 = m.nbr_start[i + 1];
-#pragma unroll 4
-			for (int k = m.nbr_start[i]; k < k1; k++)
-			{
-				const int code = m.code[k];
-				const V3<T> f = ld3(W.pforce + 3 * code_pair(code));
-				if (code_is_b(code))
-					d = d + f;
-				else
-					d = d - f;
-			}
-			st3(W.forces + 3 * i, d);
+			st3(W.forces + 3 * i, d + fp);
 		}
 	}
 	T e;
@@ -692,57 +752,112 @@ __device__ __forceinline__ T eval_deriv(const MV<T>& m, const EvalCtx<T>& ec, Wa
 		lsync();
 		e = wave_sum(e_part);
 	}
-	// derivative: per-frame sums, then the tree bottom-up
-	const int F = m.h.nframes;
-	for (int f = lane; f < F; f += WAVE)
+	if constexpr (pair_parallel<T>())
 	{
-		const V3<T> o = ld3(W.fo + 3 * f);
+		// derivative, direct: each frame's force and torque summed over its
+		// subtree's atoms (a contiguous range), equal to smina's bottom-up
+		// accumulation (src/lib/tree.h:88-97, :214-289) up to rounding; a
+		// group of G lanes per frame, reduced with DPP
+		const int F = m.h.nframes;
+		int G = 16;
+		while (G > 1 && G * F > WAVE)
+			G >>= 1;
+		const int f = lane / G, sub = lane - f * G;
 		V3<T> fo = v3(T(0), T(0), T(0)), to = v3(T(0), T(0), T(0));
-		const int b = m.fbegin[f], en = m.fend[f];
-#pragma unroll 4
-		for (int i = b; i < en; i++)
+		V3<T> o = fo;
+		if (f < F)
 		{
-			const V3<T> fi = ld3(W.forces + 3 * i);
-			fo = fo + fi;
-			to = to + cross(ld3(W.coords + 3 * i) - o, fi);
-		}
-		st3(W.ff + 3 * f, fo);
-		st3(W.ft + 3 * f, to);
-	}
-	lsync();
-	for (int L = m.h.nlevels - 1; L >= 0; L--)
-	{
-		const int s = m.lstart[L], en = m.lstart[L + 1];
-		for (int j = s + lane; j < en; j += WAVE)
-		{
-			const int f = m.lframes[j];
-			V3<T> fo = ld3(W.ff + 3 * f), to = ld3(W.ft + 3 * f);
-			const V3<T> o = ld3(W.fo + 3 * f);
-			const int cs = m.cstart[f], cc = m.ccount[f];
-			for (int k = 0; k < cc; k++)
+			o = ld3(W.fo + 3 * f);
+			const int en = m.fsend[f];
+#pragma unroll 2
+			for (int i = m.fbegin[f] + sub; i < en; i += G)
 			{
-				const int c = m.clist[cs + k];
-				const V3<T> cf = ld3(W.ff + 3 * c);
-				fo = fo + cf;
-				const V3<T> r = ld3(W.fo + 3 * c) - o;
-				to = to + (cross(r, cf) + ld3(W.ft + 3 * c));
+				const V3<T> fi = ld3(W.forces + 3 * i);
+				fo = fo + fi;
+				to = to + cross(ld3(W.coords + 3 * i) - o, fi);
 			}
-			st3(W.ff + 3 * f, fo);
-			st3(W.ft + 3 * f, to);
+		}
+		T acc[6] = {fo.x, fo.y, fo.z, to.x, to.y, to.z};
+#pragma unroll
+		for (int k = 0; k < 6; k++)
+		{
+			if (G >= 2)
+				acc[k] += dpp<DPP_QUAD_1032>(acc[k]);
+			if (G >= 4)
+				acc[k] += dpp<DPP_QUAD_2301>(acc[k]);
+			if (G >= 8)
+				acc[k] += dpp<DPP_ROW_HALF_MIRROR>(acc[k]);
+			if (G >= 16)
+				acc[k] += dpp<DPP_ROW_MIRROR>(acc[k]);
+		}
+		if (f < F && sub == 0)
+		{
 			const int gi = m.fchange[f];
 			if (m.fkind[f] == SGPU_FRAME_RIGID_ROOT)
 			{
-				W.gbuf[gi] = fo.x;
-				W.gbuf[gi + 1] = fo.y;
-				W.gbuf[gi + 2] = fo.z;
-				W.gbuf[gi + 3] = to.x;
-				W.gbuf[gi + 4] = to.y;
-				W.gbuf[gi + 5] = to.z;
+#pragma unroll
+				for (int k = 0; k < 6; k++)
+					W.gbuf[gi + k] = acc[k];
 			}
 			else
-				W.gbuf[gi] = dot(to, ld3(W.fax + 3 * f));
+				W.gbuf[gi] = dot(v3(acc[3], acc[4], acc[5]), ld3(W.fax + 3 * f));
 		}
 		lsync();
+	}
+	else
+	{
+		// derivative: per-frame sums, then the tree bottom-up
+		const int F = m.h.nframes;
+		for (int f = lane; f < F; f += WAVE)
+		{
+			const V3<T> o = ld3(W.fo + 3 * f);
+			V3<T> fo = v3(T(0), T(0), T(0)), to = v3(T(0), T(0), T(0));
+			const int b = m.fbegin[f], en = m.fend[f];
+	#pragma unroll 4
+			for (int i = b; i < en; i++)
+			{
+				const V3<T> fi = ld3(W.forces + 3 * i);
+				fo = fo + fi;
+				to = to + cross(ld3(W.coords + 3 * i) - o, fi);
+			}
+			st3(W.ff + 3 * f, fo);
+			st3(W.ft + 3 * f, to);
+		}
+		lsync();
+		for (int L = m.h.nlevels - 1; L >= 0; L--)
+		{
+			const int s = m.lstart[L], en = m.lstart[L + 1];
+			for (int j = s + lane; j < en; j += WAVE)
+			{
+				const int f = m.lframes[j];
+				V3<T> fo = ld3(W.ff + 3 * f), to = ld3(W.ft + 3 * f);
+				const V3<T> o = ld3(W.fo + 3 * f);
+				const int cs = m.cstart[f], cc = m.ccount[f];
+				for (int k = 0; k < cc; k++)
+				{
+					const int c = m.clist[cs + k];
+					const V3<T> cf = ld3(W.ff + 3 * c);
+					fo = fo + cf;
+					const V3<T> r = ld3(W.fo + 3 * c) - o;
+					to = to + (cross(r, cf) + ld3(W.ft + 3 * c));
+				}
+				st3(W.ff + 3 * f, fo);
+				st3(W.ft + 3 * f, to);
+				const int gi = m.fchange[f];
+				if (m.fkind[f] == SGPU_FRAME_RIGID_ROOT)
+				{
+					W.gbuf[gi] = fo.x;
+					W.gbuf[gi + 1] = fo.y;
+					W.gbuf[gi + 2] = fo.z;
+					W.gbuf[gi + 3] = to.x;
+					W.gbuf[gi + 4] = to.y;
+					W.gbuf[gi + 5] = to.z;
+				}
+				else
+					W.gbuf[gi] = dot(to, ld3(W.fax + 3 * f));
+			}
+			lsync();
+		}
 	}
 	g_lane = lane < m.h.change_size ? W.gbuf[lane] : T(0);
 	lsync();

@@ -53,9 +53,33 @@ __device__ inline double readlane(double v, int l)
 }
 __device__ inline int readlane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 
-// wave-wide sum in a fixed butterfly order (deterministic)
-template<typename T>
-__device__ inline T wave_sum(T v)
+// DPP lane exchange within rows of 16 (no LDS traffic)
+template<int CTRL>
+__device__ inline float dpp(float v)
+{
+	return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+constexpr int DPP_QUAD_1032 = 0xB1;    // quad_perm [1, 0, 3, 2]
+constexpr int DPP_QUAD_2301 = 0x4E;    // quad_perm [2, 3, 0, 1]
+constexpr int DPP_ROW_HALF_MIRROR = 0x141;
+constexpr int DPP_ROW_MIRROR = 0x140;
+
+// wave-wide sum in a fixed order (deterministic), every lane gets it: DPP
+// within each row of 16 lanes, then the four row sums.  The whole wave must
+// be active.
+__device__ inline float wave_sum(float v)
+{
+	v += dpp<DPP_QUAD_1032>(v);
+	v += dpp<DPP_QUAD_2301>(v);
+	v += dpp<DPP_ROW_HALF_MIRROR>(v);
+	v += dpp<DPP_ROW_MIRROR>(v);
+	return (__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 0)) +
+				   __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 16))) +
+			(__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 32)) +
+					__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 48)));
+}
+// double: butterfly through lane permutes (the ordered search never uses it)
+__device__ inline double wave_sum(double v)
 {
 	for (int off = 32; off > 0; off >>= 1)
 		v += __shfl_xor(v, off, WAVE);
@@ -68,8 +92,10 @@ template<typename T> struct Real;
 template<> struct Real<float>
 {
 	static constexpr bool ordered = false;
-	__device__ static float sin(float x) { return ::sinf(x); }
-	__device__ static float cos(float x) { return ::cosf(x); }
+	// hardware sine/cosine: every argument the search passes is a normalised
+	// half angle, |x| <= pi / 2
+	__device__ static float sin(float x) { return __sinf(x); }
+	__device__ static float cos(float x) { return __cosf(x); }
 	__device__ static float acos(float x) { return ::acosf(x); }
 	__device__ static float sqrt(float x) { return ::sqrtf(x); }
 	__device__ static float abs(float x) { return ::fabsf(x); }
@@ -77,7 +103,7 @@ template<> struct Real<float>
 	__device__ static float eps() { return FLT_EPSILON; }
 	__device__ static float max() { return FLT_MAX; }
 	// a / b through the hardware reciprocal (1 ulp), for the served search
-	__device__ static float fdiv(float a, float b) { return __fdividef(a, b); }
+	__device__ static float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 };
 template<> struct Real<double>
 {

@@ -686,7 +686,8 @@ void GpuDocking::dock_sub(DockJob* const* jobs, size_t nj)
 			std::vector<uint32_t> st(625);
 			make_chain(fms[j], uint32_t(seed), gd_, conf.data(), st.data());
 			cmodel.push_back(int(j));
-			csteps.push_back(sp[j].num_steps);
+			csteps.push_back(s_.step_fraction >= 1 ? sp[j].num_steps
+			                                        : int(std::ceil(sp[j].num_steps * s_.step_fraction)));
 			citers.push_back(sp[j].maxiters);
 			cconf.insert(cconf.end(), conf.begin(), conf.end());
 			crng.insert(crng.end(), st.begin(), st.end());
@@ -952,6 +953,9 @@ sgpu_docking* sgpu_docking_create(const sgpu_dock_settings* s, const sgpu_mol* r
 		gs.num_modes = s->num_modes;
 		gs.energy_range = s->energy_range;
 		gs.ligands_per_launch = s->ligands_per_launch;
+		if (!(s->step_fraction > 0) || s->step_fraction > 1)
+			throw std::invalid_argument("step_fraction must be in (0, 1]");
+		gs.step_fraction = s->step_fraction;
 		d->terms.resize(5);
 		sgpu_scoring sf;
 		sgpu_vina_scoring(d->terms.data(), &sf, s->hydrophobic_scale);

@@ -83,6 +83,7 @@ struct GpuSettings
 	int num_modes = 9;
 	double energy_range = 3.0;
 	int ligands_per_launch = 0; // 0: the whole batch in one launch
+	double step_fraction = 1;   // fraction of smina's Monte Carlo steps (1: smina's search)
 };
 
 class GpuDocking

@@ -103,7 +103,7 @@ class Scoring(C.Structure):
 class DockSettings(C.Structure):
     _fields_ = [("device", C.c_int), ("fp64", C.c_int), ("exhaustiveness", C.c_int), ("seed", C.c_int),
                 ("num_modes", C.c_int), ("energy_range", C.c_double), ("hydrophobic_scale", C.c_double),
-                ("ligands_per_launch", C.c_int)]
+                ("ligands_per_launch", C.c_int), ("step_fraction", C.c_double)]
 
 
 class DockResult(C.Structure):
@@ -254,6 +254,8 @@ class Flat:
         self.natoms = v.natoms
         self.npairs = v.npairs
         self.nother = v.nother
+        # torsions of the search's degrees of freedom (ligand and flexible)
+        self.ntors_total = v.change_size - 6 * v.nligands
         self.pair_a = np.ctypeslib.as_array(v.pair_a, (v.npairs,)).copy() if v.npairs else np.zeros(0, np.int32)
         self.pair_b = np.ctypeslib.as_array(v.pair_b, (v.npairs,)).copy() if v.npairs else np.zeros(0, np.int32)
         t = np.ctypeslib.as_array(v.atom_type, (v.natoms,))
@@ -445,9 +447,9 @@ class Docking:
     """GpuDocking driver: search, merge, refine, FP64 rescoring."""
 
     def __init__(self, receptor, gd, fp64=False, exhaustiveness=8, seed=1, num_modes=9, energy_range=3.0,
-                 hydrophobic_scale=1.0, ligands_per_launch=0, device=0):
+                 hydrophobic_scale=1.0, ligands_per_launch=0, device=0, step_fraction=1.0):
         s = DockSettings(device, int(fp64), exhaustiveness, seed, num_modes, energy_range, hydrophobic_scale,
-                         ligands_per_launch)
+                         ligands_per_launch, step_fraction)
         self.gd = gd
         self.h = C.c_void_p(lib.sgpu_docking_create(C.byref(s), receptor.h, C.byref(gd)))
         err = lib.sgpu_docking_error(self.h)

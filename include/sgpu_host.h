@@ -53,6 +53,8 @@ typedef struct sgpu_dock_settings
 	double energy_range; /* 3 */
 	double hydrophobic_scale; /* custom scoring: Vina with this factor on hydrophobic */
 	int ligands_per_launch; /* batching: 0 = all jobs in one launch */
+	double step_fraction;   /* fraction of smina's Monte Carlo steps to run (1: smina's
+	                         * search; GpuSettings::step_fraction, src/gpu/gpu_docking.h:29-31) */
 } sgpu_dock_settings;
 
 sgpu_docking* sgpu_docking_create(const sgpu_dock_settings* s, const sgpu_mol* receptor,
