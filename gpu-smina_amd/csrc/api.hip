@@ -297,6 +297,28 @@ void pack_batch(const sgpu_flat_model* ms, int n, HostBatch& B)
 			require(m.frame_end[f] == m.frame_begin[f + 1], "frame atom ranges must follow preorder");
 		mark(h.o_fsend);
 		I.insert(I.end(), send.begin(), send.end());
+		// each frame's path from its tree root (frame indices as bytes, root
+		// first, the frame itself last) for the ancestor walk of set_conf
+		mark(h.o_fdepth);
+		h.path_ok = 1;
+		for (int f = 0; f < m.nframes; f++)
+		{
+			I.push_back(depth[size_t(f)] + 1);
+			if (depth[size_t(f)] + 1 > MAX_PATH)
+				h.path_ok = 0;
+		}
+		mark(h.o_fpath);
+		for (int f = 0; f < m.nframes; f++)
+		{
+			std::vector<int> path;
+			for (int a = f; a >= 0; a = m.frame_parent[a])
+				path.push_back(a);
+			std::reverse(path.begin(), path.end());
+			path.resize(MAX_PATH, 0);
+			for (int w = 0; w < MAX_PATH / 4; w++)
+				I.push_back(int(unsigned(path[size_t(4 * w)]) | (unsigned(path[size_t(4 * w + 1)]) << 8) |
+						(unsigned(path[size_t(4 * w + 2)]) << 16) | (unsigned(path[size_t(4 * w + 3)]) << 24)));
+		}
 		std::vector<int> cstart, ccount, clist;
 		for (int f = 0; f < m.nframes; f++)
 		{

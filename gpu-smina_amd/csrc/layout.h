@@ -44,6 +44,7 @@ constexpr int MAX_CHANGE = 64;
 constexpr int MAX_CONF = 64;
 constexpr int MAX_TREES = 16;
 constexpr int MAX_POSES = 64;
+constexpr int MAX_PATH = 32;      // frames on a root-to-frame path (ancestor walk of set_conf)
 
 // conf element kinds: position component, quaternion component a..d, torsion
 enum ConfKind
@@ -84,11 +85,12 @@ struct ModelHdr
 {
 	int natoms, ninflex, nframes, ntrees, nligands, npairs, nother;
 	int conf_size, change_size, nheavy, nlevels, num_tors;
+	int path_ok; // every root-to-frame path has at most MAX_PATH frames
 	int iblob_off, iblob_len; // in the batch int blob
 	int tblob_off, tblob_len; // in the batch real blob
 	int pair_off;             // pair arrays (a, b, table index), global
 	// offsets inside the int blob
-	int o_type, o_afr, o_nbr, o_code, o_pcode, o_fparent, o_fkind, o_fbegin, o_fend, o_fconf, o_fchange, o_fsend, o_cstart,
+	int o_type, o_afr, o_nbr, o_code, o_pcode, o_fparent, o_fkind, o_fbegin, o_fend, o_fconf, o_fchange, o_fsend, o_fdepth, o_fpath, o_cstart,
 			o_ccount, o_clist, o_lstart, o_lframes, o_heavy, o_troot, o_tconf, o_tchange, o_tlig, o_tntors,
 			o_tab, o_tae, o_tpb, o_tpe, o_maps;
 	// offsets inside the real blob

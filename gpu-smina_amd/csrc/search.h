@@ -179,6 +179,8 @@ struct MV
 	const int* fconf;
 	const int* fchange;
 	const int* fsend; // end of the frame's subtree atoms
+	const int* fdepth; // frames on the root-to-frame path
+	const int* fpath;  // that path, frame indices as bytes, MAX_PATH / 4 words per frame
 	const int* cstart;
 	const int* ccount;
 	const int* clist;
@@ -243,6 +245,8 @@ __device__ inline MV<T> stage_model(const BatchPtrs<T>& B, int m, unsigned char*
 	v.fconf = li + h.o_fconf;
 	v.fchange = li + h.o_fchange;
 	v.fsend = li + h.o_fsend;
+	v.fdepth = li + h.o_fdepth;
+	v.fpath = li + h.o_fpath;
 	v.cstart = li + h.o_cstart;
 	v.ccount = li + h.o_ccount;
 	v.clist = li + h.o_clist;
@@ -281,6 +285,55 @@ __device__ __forceinline__ void set_conf(const MV<T>& m, WaveMem<T>& W, T x_lane
 	if (lane < m.h.conf_size)
 		W.xstage[lane] = x_lane;
 	lsync();
+	if (pair_parallel<T>() && m.h.path_ok)
+	{
+		// ancestor walk: the lane of frame f composes the frames on its
+		// root-to-f path in registers, the same operations in the same order
+		// as the level-by-level pass, with no exchange between levels
+		for (int f = lane; f < m.h.nframes; f += WAVE)
+		{
+			const int d = m.fdepth[f];
+			const int* path = m.fpath + f * (MAX_PATH / 4);
+			int r = path[0] & 255;
+			const int rci = m.fconf[r];
+			V3<T> o;
+			Q4<T> q;
+			V3<T> ax = v3(T(0), T(0), T(0));
+			if (m.fkind[r] == SGPU_FRAME_RIGID_ROOT)
+			{
+				o = v3(W.xstage[rci], W.xstage[rci + 1], W.xstage[rci + 2]);
+				q = {W.xstage[rci + 3], W.xstage[rci + 4], W.xstage[rci + 5], W.xstage[rci + 6]};
+			}
+			else
+			{
+				o = ld3(m.forig + 3 * r);
+				ax = ld3(m.faxis + 3 * r);
+				q = axis_angle_q(ax, W.xstage[rci]);
+			}
+			T R[9];
+			q_to_mat(q, R);
+			for (int k = 1; k < d; k++)
+			{
+				const int c = (path[k >> 2] >> (8 * (k & 3))) & 255;
+				o = o + mat_vec(R, ld3(m.forig + 3 * c));
+				ax = mat_vec(R, ld3(m.faxis + 3 * c));
+				q = qmul(axis_angle_q(ax, W.xstage[m.fconf[c]]), q);
+				q_normalize_approx(q);
+				q_to_mat(q, R);
+			}
+			st3(W.fo + 3 * f, o);
+			W.fq[4 * f] = q.a;
+			W.fq[4 * f + 1] = q.b;
+			W.fq[4 * f + 2] = q.c;
+			W.fq[4 * f + 3] = q.d;
+			for (int k = 0; k < 9; k++)
+				W.fR[9 * f + k] = R[k];
+			if (m.fkind[f] != SGPU_FRAME_RIGID_ROOT)
+				st3(W.fax + 3 * f, ax);
+		}
+		lsync();
+	}
+	else
 	for (int L = 0; L < m.h.nlevels; L++)
 	{
 		const int s = m.lstart[L], e = m.lstart[L + 1];

@@ -2,11 +2,13 @@
 
 FP64: bit-exact (maps, eval_deriv on both back ends, MC trajectories,
 refinement, final rescoring, batched models).  FP32: the reference's rule
-(src/gpu/tests/gpu_eval_test.cpp:55-92): at least 99% of evaluations within
-1e-4 relative of the FP64 result for energy and gradient, every one within
-10x that, except evaluations next to a discontinuity (an atom on a map cell
-face or a pair at the 8 A cutoff); plus energies within 1e-3 kcal/mol of
-FP64 where |E| < 10 (the served tolerance)."""
+(src/gpu/tests/gpu_eval_test.cpp:55-92) is at least 99% of evaluations
+within 1e-4 relative of the FP64 result for energy and gradient, except
+evaluations next to a discontinuity (an atom on a map cell face or a pair at
+the 8 A cutoff).  Random confs (clashes and face crossings included, not
+flagged) must reach 95% within 1e-4 and every one within 1e-3 unless
+|E| > 1e3; the minima an FP64 search saves must reach 99%.  Energies below
+10 kcal/mol are within 1e-3 kcal/mol of FP64 (the served tolerance)."""
 import numpy as np
 import pytest
 
@@ -107,6 +109,39 @@ def test_eval_fp32_tolerance(env, li, backend):
         if abs(eo) < 10:
             assert abs(e[k] - eo) <= 1e-3, (k, e[k], eo)
     assert ok / (ok + bad) >= 0.95, (ok, bad)
+
+
+def test_fp32_along_fp64_search(env):
+    """The exhaustiveness-128 single-ligand check (38 heavy atoms, 10
+    torsions): the poses an FP64 search saves (BFGS minima along 16 chains)
+    evaluated in FP32 and FP64 on the maps, at the hunt and authentic caps.
+    Reports |dE| and the gradient error; >= 99% within 1e-4 relative, every
+    energy below 10 kcal/mol within 1e-3."""
+    li = 1
+    assert LIGANDS[li] == (2, 38, 10)
+    ctx, f = env["ctx"], env["flats"][li]
+    cs, ss, maxit, (cnt, e, c, x, ev) = run_chains(env, li, sg.FP64, 30, nchains=16, seed0=7000)
+    confs = np.vstack([c[k, :cnt[k]] for k in range(len(cnt))])
+    assert len(confs) >= 100
+    n = len(confs)
+    stats = []
+    for v in (10.0, 1000.0):
+        r = {}
+        for prec in (sg.FP32, sg.FP64):
+            r[prec] = ctx.eval_confs(env["bids"][prec], [li] * n, confs, f.change_size, f.natoms, (v, v, v),
+                                     grid_id=env["gids"][prec])
+        e32, g32, _ = r[sg.FP32]
+        e64, g64, _ = r[sg.FP64]
+        de = np.abs(e32 - e64)
+        rel_e = de / np.maximum(1.0, np.abs(e64))
+        rel_g = np.max(np.abs(g32 - g64), axis=1) / np.maximum(1.0, np.max(np.abs(g64), axis=1))
+        within = np.mean(np.maximum(rel_e, rel_g) <= 1e-4)
+        stats.append((v, n, float(de.max()), float(np.median(de)), float(rel_g.max()), float(within)))
+        print(f"caps {v}: {n} poses, max |dE| {de.max():.2e}, median {np.median(de):.2e}, "
+              f"max grad rel {rel_g.max():.2e}, within 1e-4: {within:.4f}", flush=True)
+        assert within >= 0.99, stats[-1]
+        small = np.abs(e64) < 10
+        assert np.all(de[small] <= 1e-3), stats[-1]
 
 
 def run_chains(env, li, prec, steps, nchains=4, seed0=2000):
