@@ -1311,6 +1311,18 @@ void run_refine(sgpu_ctx* c, const sgpu_refine_request* q)
 		if (q->coords)
 			require(q->coord_stride >= 3 * b.h.hdr[q->model[i]].nheavy, "coord stride");
 	}
+	DBuf dboxes;
+	if (q->boxes)
+	{
+		for (int i = 0; i < n; i++)
+			for (int d = 0; d < 3; d++)
+			{
+				const double lo = q->boxes[6 * size_t(i) + d], hi = q->boxes[6 * size_t(i) + 3 + d];
+				require(lo < hi, "pose box end must exceed begin");
+				require(lo >= q->gd->begin[d] && hi <= q->gd->end[d], "pose box outside the cell box");
+			}
+		upload(dboxes, std::vector<double>(q->boxes, q->boxes + 6 * size_t(n)), c->stream);
+	}
 	Cells& cl = receptor_cells(c, r, *q->gd);
 	EvalCtx<T> ec{};
 	ec.tab = table_ptrs<T>(c);
@@ -1341,7 +1353,7 @@ void run_refine(sgpu_ctx* c, const sgpu_refine_request* q)
 	Timer tm(c, "refine");
 	hip_check(launch_refine<T>(ptrs<T>(b), ec, b.h.plan, n, dm.as<int>(), dc.as<T>(), cs, dmi.as<int>(),
 					  q->accurate_line_search, q->early_term, de.as<T>(), dx.as<T>(), xs, c->scratch.as<T>(), ps,
-					  dev.as<int64_t>(), c->stream),
+					  dev.as<int64_t>(), q->boxes ? dboxes.as<double>() : nullptr, c->stream),
 			"refine kernel");
 	tm.stop();
 	std::vector<T> e(n), x(size_t(n) * xs);

@@ -26,7 +26,7 @@ hipError_t launch_mc(const BatchPtrs<T>& B, const EvalCtx<T>& ec, const WavePlan
 template<typename T>
 hipError_t launch_refine(const BatchPtrs<T>& B, const EvalCtx<T>& ec, const WavePlan& plan, int n, const int* model,
 		T* conf, int conf_stride, const int* maxiters, int accurate, int early_term, T* e, T* x, int x_stride,
-		T* scratch, int pstride, int64_t* evals, hipStream_t st);
+		T* scratch, int pstride, int64_t* evals, const double* boxes, hipStream_t st);
 template<typename T>
 hipError_t launch_score(const BatchPtrs<T>& B, const EvalCtx<T>& ec, const WavePlan& plan, int mode, int n,
 		const int* model, const T* conf, int conf_stride, T* e, T* scratch, int pstride, hipStream_t st);

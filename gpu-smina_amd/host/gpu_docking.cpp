@@ -638,6 +638,139 @@ void GpuDocking::dock_batch(std::vector<DockJob*>& jobs)
 		dock_sub(jobs.data() + b, std::min(per, jobs.size() - b));
 }
 
+sgpu_grid_dims movable_atoms_box(const FlatModel& m, double add, double granularity)
+{
+	double c1[3] = {0, 0, 0}, c2[3] = {0, 0, 0};
+	for (int i = 0; i < m.view.natoms; i++)
+		for (int j = 0; j < 3; j++)
+		{
+			const double v = m.atom_init[size_t(3 * i + j)];
+			if (i == 0 || v < c1[j])
+				c1[j] = v;
+			if (i == 0 || v > c2[j])
+				c2[j] = v;
+		}
+	sgpu_grid_dims gd{};
+	for (int j = 0; j < 3; j++)
+	{
+		c1[j] -= add / 2;
+		c2[j] += add / 2;
+	}
+	for (int j = 0; j < 3; j++)
+	{
+		const double center = 0.5 * (c2[j] + c1[j]);
+		gd.n[j] = int(std::ceil((c2[j] - c1[j]) / granularity));
+		const double span = granularity * gd.n[j];
+		gd.begin[j] = center - span / 2;
+		gd.end[j] = gd.begin[j] + span;
+	}
+	return gd;
+}
+
+void GpuDocking::minimize_batch(std::vector<MinimizeJob*>& jobs)
+{
+	if (jobs.empty())
+		return;
+	std::vector<FlatModel> fms;
+	for (MinimizeJob* j : jobs)
+		fms.push_back(flatten(*j->m));
+	std::vector<sgpu_flat_model> views;
+	for (auto& f : fms)
+		views.push_back(f.view);
+	int batch = -1;
+	check(ctx_, sgpu_upload_models(ctx_, views.data(), int(views.size()), SGPU_FP64, &batch));
+	int cstride = 1, hstride = 1;
+	for (auto& f : fms)
+	{
+		cstride = std::max(cstride, f.view.conf_size);
+		hstride = std::max(hstride, 3 * f.nheavy);
+	}
+	// one refine request per line-search kind; cells over the union of boxes
+	for (int acc = 0; acc < 2; acc++)
+	{
+		std::vector<size_t> idx;
+		for (size_t k = 0; k < jobs.size(); k++)
+			if (jobs[k]->accurate == (acc == 1))
+				idx.push_back(k);
+		if (idx.empty())
+			continue;
+		const int n = int(idx.size());
+		std::vector<int> model(n), iters(n);
+		std::vector<double> conf(size_t(n) * cstride, 0.0), boxes(size_t(n) * 6), e(n), x(size_t(n) * hstride);
+		std::vector<std::vector<double>> orig(n);
+		sgpu_grid_dims un{};
+		for (int i = 0; i < n; i++)
+		{
+			MinimizeJob& j = *jobs[idx[size_t(i)]];
+			const FlatModel& f = fms[idx[size_t(i)]];
+			if (j.gd.n[0] == 0)
+				j.gd = movable_atoms_box(f, 4, 0.375);
+			if (j.conf.empty())
+				j.conf = initial_conf(f);
+			if (int(j.conf.size()) != f.view.conf_size)
+				throw std::invalid_argument("minimize: conf size does not match the model");
+			model[size_t(i)] = int(idx[size_t(i)]);
+			iters[size_t(i)] = j.maxiters > 0 ? j.maxiters : 10000;
+			std::copy(j.conf.begin(), j.conf.end(), conf.begin() + size_t(i) * cstride);
+			for (int d = 0; d < 3; d++)
+			{
+				boxes[6 * size_t(i) + d] = j.gd.begin[d];
+				boxes[6 * size_t(i) + 3 + d] = j.gd.end[d];
+				un.begin[d] = i == 0 ? j.gd.begin[d] : std::min(un.begin[d], j.gd.begin[d]);
+				un.end[d] = i == 0 ? j.gd.end[d] : std::max(un.end[d], j.gd.end[d]);
+			}
+			for (int a = 0; a < f.view.natoms; a++)
+				if (!is_hydrogen_type(f.atom_type[size_t(a)]))
+					for (int d = 0; d < 3; d++)
+						orig[size_t(i)].push_back(f.atom_init[size_t(3 * a + d)]);
+		}
+		for (int d = 0; d < 3; d++)
+			un.n[d] = std::max(1, int(std::ceil((un.end[d] - un.begin[d]) / 0.375)));
+		sgpu_refine_request rq{};
+		rq.batch_id = batch;
+		rq.receptor_id = receptor_;
+		rq.gd = &un;
+		for (int k = 0; k < 3; k++)
+			rq.v[k] = 1000;
+		rq.accurate_line_search = acc;
+		rq.nposes = n;
+		rq.model = model.data();
+		rq.conf = conf.data();
+		rq.conf_stride = cstride;
+		rq.maxiters = iters.data();
+		rq.e = e.data();
+		rq.coords = x.data();
+		rq.coord_stride = hstride;
+		rq.boxes = boxes.data();
+		const sgpu_status st = sgpu_run_refine(ctx_, &rq);
+		if (st != SGPU_OK)
+		{
+			const std::string msg = sgpu_last_error(ctx_);
+			(void)sgpu_release_models(ctx_, batch);
+			throw std::runtime_error(msg);
+		}
+		for (int i = 0; i < n; i++)
+		{
+			MinimizeJob& j = *jobs[idx[size_t(i)]];
+			const FlatModel& f = fms[idx[size_t(i)]];
+			j.conf.assign(conf.begin() + size_t(i) * cstride, conf.begin() + size_t(i) * cstride + f.view.conf_size);
+			j.e = e[size_t(i)];
+			j.coords.assign(x.begin() + size_t(i) * hstride, x.begin() + size_t(i) * hstride + 3 * f.nheavy);
+			double acc2 = 0;
+			for (int a = 0; a < f.nheavy; a++)
+			{
+				double d2 = 0;
+				for (int d = 0; d < 3; d++)
+					d2 += (j.coords[size_t(3 * a + d)] - orig[size_t(i)][size_t(3 * a + d)]) *
+							(j.coords[size_t(3 * a + d)] - orig[size_t(i)][size_t(3 * a + d)]);
+				acc2 += d2;
+			}
+			j.rmsd = f.nheavy > 0 ? std::sqrt(acc2 / f.nheavy) : 0;
+		}
+	}
+	check(ctx_, sgpu_release_models(ctx_, batch));
+}
+
 void GpuDocking::dock_sub(DockJob* const* jobs, size_t nj)
 {
 	if (nj == 0)
@@ -1014,6 +1147,42 @@ int sgpu_dock(sgpu_docking* d, const sgpu_mol* const* ligands, int n, sgpu_dock_
 				out[i].e[k] = jobs[i].out[k].e;
 			out[i].evals = jobs[i].evals;
 			out[i].seconds = jobs[i].seconds;
+		}
+		return 0;
+	}
+	catch (std::exception& e)
+	{
+		d->err = e.what();
+		return -1;
+	}
+}
+
+int sgpu_minimize(sgpu_docking* d, const sgpu_mol* const* ligands, int n, int maxiters, int accurate,
+		sgpu_minimize_result* out)
+{
+	if (!d->d)
+		return -1;
+	try
+	{
+		std::vector<GpuDocking::MinimizeJob> jobs(n);
+		std::vector<GpuDocking::MinimizeJob*> jp;
+		for (int i = 0; i < n; i++)
+		{
+			jobs[i].m = sgpu_mol_desc(ligands[i]);
+			jobs[i].maxiters = maxiters;
+			jobs[i].accurate = accurate != 0;
+			jp.push_back(&jobs[i]);
+		}
+		d->d->minimize_batch(jp);
+		for (int i = 0; i < n; i++)
+		{
+			out[i].e = jobs[i].e;
+			out[i].rmsd = jobs[i].rmsd;
+			for (int k = 0; k < 3; k++)
+			{
+				out[i].box_begin[k] = jobs[i].gd.begin[k];
+				out[i].box_end[k] = jobs[i].gd.end[k];
+			}
 		}
 		return 0;
 	}

@@ -46,6 +46,8 @@ struct FlatModel
 };
 
 FlatModel flatten(const sgpu_model_desc& d);
+// model::movable_atoms_box (src/lib/model.cpp:677-707) on the parsed pose
+sgpu_grid_dims movable_atoms_box(const FlatModel& m, double add_to_each_dimension, double granularity);
 std::vector<double> initial_conf(const FlatModel& m);
 // make_chain (src/gpu/gpu_search.cpp:19-31): rng(seed), conf::randomize in
 // the box; the generator state (624 words + position) after the draws
@@ -105,6 +107,24 @@ public:
 	// every pose, FP64 rescoring, clustering (src/gpu/gpu_docking.cpp:756-831,
 	// src/main/main.cpp:256-324)
 	void dock_batch(std::vector<DockJob*>& jobs);
+
+	// local optimization (--minimize / --local_only), always in FP64:
+	// refine_structure of each job's start conf in the job's own box
+	// (GpuDocking::minimize_batch, src/gpu/gpu_docking.h:96-109 and
+	// src/gpu/gpu_docking.cpp:690-742; box and iterations as main.cpp sets
+	// them, src/main/main.cpp:1216-1233, :1688-1700)
+	struct MinimizeJob
+	{
+		const sgpu_model_desc* m;
+		sgpu_grid_dims gd{};        // the job's box; n[0] == 0: movable_atoms_box(4, 0.375)
+		int maxiters = 10000;       // --minimize_iters (0 in smina's options means 10000)
+		bool accurate = true;       // --minimize uses the accurate line search
+		std::vector<double> conf;   // in: start conf (empty: the parsed pose); out: refined
+		double e = 0;               // energy of the refined conf (maxed if outside the box)
+		std::vector<double> coords; // heavy movable atoms after refinement
+		double rmsd = 0;            // heavy-atom RMSD from the parsed pose
+	};
+	void minimize_batch(std::vector<MinimizeJob*>& jobs);
 
 	// receptor maps for every heavy-atom type of these models
 	void prepare(const std::vector<const sgpu_model_desc*>& models);

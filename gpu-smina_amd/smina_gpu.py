@@ -74,7 +74,8 @@ class RefineRequest(C.Structure):
     _fields_ = [("batch_id", C.c_int), ("receptor_id", C.c_int), ("gd", C.POINTER(GridDims)),
                 ("v", C.c_double * 3), ("accurate_line_search", C.c_int), ("early_term", C.c_int),
                 ("nposes", C.c_int), ("model", I32P), ("conf", F64P), ("conf_stride", C.c_int),
-                ("maxiters", I32P), ("e", F64P), ("coords", F64P), ("coord_stride", C.c_int)]
+                ("maxiters", I32P), ("e", F64P), ("coords", F64P), ("coord_stride", C.c_int),
+                ("boxes", F64P)]
 
 
 class ScoreRequest(C.Structure):
@@ -104,6 +105,10 @@ class DockSettings(C.Structure):
     _fields_ = [("device", C.c_int), ("fp64", C.c_int), ("exhaustiveness", C.c_int), ("seed", C.c_int),
                 ("num_modes", C.c_int), ("energy_range", C.c_double), ("hydrophobic_scale", C.c_double),
                 ("ligands_per_launch", C.c_int), ("step_fraction", C.c_double)]
+
+
+class MinimizeResult(C.Structure):
+    _fields_ = [("e", C.c_double), ("rmsd", C.c_double), ("box_begin", C.c_double * 3), ("box_end", C.c_double * 3)]
 
 
 class DockResult(C.Structure):
@@ -148,6 +153,7 @@ def _load():
         "sgpu_docking_free": (None, [vp]),
         "sgpu_docking_error": (C.c_char_p, [vp]),
         "sgpu_dock": (C.c_int, [vp, C.POINTER(vp), C.c_int, C.POINTER(DockResult)]),
+        "sgpu_minimize": (C.c_int, [vp, C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.POINTER(MinimizeResult)]),
         "sgpu_docking_prepare": (C.c_int, [vp, C.POINTER(vp), C.c_int]),
         "sgpu_docking_ctx": (vp, [vp]),
         "sgpu_docking_receptor_id": (C.c_int, [vp]),
@@ -385,10 +391,14 @@ class Context:
         self.check(lib.sgpu_run_mc(self.h, bid, gid, C.byref(params), C.byref(ch), C.byref(res)))
         return cnt, e, c, x, ev
 
-    def refine(self, bid, rid, gd, models, confs, maxiters, v=(1000.0, 1000.0, 1000.0), coord_stride=3 * 256):
+    def refine(self, bid, rid, gd, models, confs, maxiters, v=(1000.0, 1000.0, 1000.0), coord_stride=3 * 256,
+               boxes=None, accurate=False):
+        """refine_structure of each pose; boxes: optional [n, 6] per-pose
+        (begin, end) inside gd"""
         models = i32(models)
         confs = f64(confs).copy()
         maxiters = i32(maxiters)
+        boxes = None if boxes is None else f64(boxes)
         n = len(models)
         e = np.zeros(n)
         x = np.zeros((n, coord_stride))
@@ -399,6 +409,8 @@ class Context:
         q.nposes = n
         q.model, q.conf, q.conf_stride = ptr(models, I32P), ptr(confs, F64P), confs.shape[1]
         q.maxiters, q.e, q.coords, q.coord_stride = ptr(maxiters, I32P), ptr(e, F64P), ptr(x, F64P), coord_stride
+        q.accurate_line_search = int(accurate)
+        q.boxes = ptr(boxes, F64P) if boxes is not None else None
         self.check(lib.sgpu_run_refine(self.h, C.byref(q)))
         return e, confs, x
 
@@ -482,3 +494,11 @@ class Docking:
         if lib.sgpu_dock(self.h, arr, len(ligands), out) != 0:
             raise RuntimeError(lib.sgpu_docking_error(self.h).decode())
         return [dict(e=list(r.e[:r.nposes]), evals=r.evals, seconds=r.seconds) for r in out]
+
+    def minimize(self, ligands, maxiters=0, accurate=True):
+        """--minimize: FP64 refine_structure of each parsed pose in its own box"""
+        arr = (C.c_void_p * len(ligands))(*[m.h.value for m in ligands])
+        out = (MinimizeResult * len(ligands))()
+        if lib.sgpu_minimize(self.h, arr, len(ligands), maxiters, int(accurate), out) != 0:
+            raise RuntimeError(lib.sgpu_docking_error(self.h).decode())
+        return [dict(e=r.e, rmsd=r.rmsd, box=(tuple(r.box_begin), tuple(r.box_end))) for r in out]

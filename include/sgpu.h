@@ -242,6 +242,11 @@ typedef struct sgpu_refine_request
 	double* e;
 	double* coords;
 	int coord_stride;
+	/* optional per-pose boxes (GpuDocking::MinimizeJob::gd,
+	 * src/gpu/gpu_docking.h:101-108): [begin x y z, end x y z] per pose; NULL
+	 * = gd for every pose.  Each box must lie inside gd, which spans the
+	 * receptor cells. */
+	const double* boxes;
 } sgpu_refine_request;
 sgpu_status sgpu_run_refine(sgpu_ctx* ctx, const sgpu_refine_request* req);
 

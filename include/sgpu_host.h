@@ -76,6 +76,19 @@ int sgpu_docking_prepare(sgpu_docking* d, const sgpu_mol* const* ligands, int n)
 /* dock each ligand model (search, merge, refine, final rescoring) */
 int sgpu_dock(sgpu_docking* d, const sgpu_mol* const* ligands, int n, sgpu_dock_result* out);
 
+/* --minimize / --local_only in FP64 (GpuDocking::minimize_batch,
+ * src/gpu/gpu_docking.h:96-109): refine_structure of each ligand's parsed
+ * pose in its own movable_atoms_box(4, 0.375); maxiters 0 = 10000 */
+typedef struct sgpu_minimize_result
+{
+	double e;           /* energy of the refined pose (DBL_MAX if it left the box) */
+	double rmsd;        /* heavy-atom RMSD from the parsed pose */
+	double box_begin[3];
+	double box_end[3];
+} sgpu_minimize_result;
+int sgpu_minimize(sgpu_docking* d, const sgpu_mol* const* ligands, int n, int maxiters, int accurate,
+		sgpu_minimize_result* out);
+
 /* low-level access for tests: the context and the device ids of the
  * receptor, grids and tables this docking object built */
 sgpu_ctx* sgpu_docking_ctx(sgpu_docking* d);

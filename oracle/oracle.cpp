@@ -2172,11 +2172,17 @@ double orc_bfgs_quadratic(int n, const double* A, const double* b, double* x, in
 
 double orc_refine(orc_ctx* c, int model, const double* v, int maxiters, double* conf, double* coords)
 {
+	return orc_refine_opts(c, model, v, maxiters, 0, conf, coords);
+}
+
+double orc_refine_opts(orc_ctx* c, int model, const double* v, int maxiters, int accurate, double* conf,
+		double* coords)
+{
 	Model& m = *c->c.models[model];
 	Output o{std::vector<double>(conf, conf + m.conf_size), 0, {}};
 	try
 	{
-		refine_structure(c->c, m, o, v, maxiters, false, false, nullptr);
+		refine_structure(c->c, m, o, v, maxiters, accurate != 0, false, nullptr);
 	}
 	catch (std::exception& ex)
 	{

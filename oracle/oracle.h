@@ -89,6 +89,9 @@ int orc_mc_chain(orc_ctx* c, int model, const sgpu_mc_params* p, int num_steps, 
 
 /* refine_structure (src/main/main.cpp:165-191); conf in/out; returns e */
 double orc_refine(orc_ctx* c, int model, const double* v, int maxiters, double* conf, double* coords);
+/* refine_structure with the accurate line search when accurate != 0 */
+double orc_refine_opts(orc_ctx* c, int model, const double* v, int maxiters, int accurate, double* conf,
+		double* coords);
 
 /* the whole docking of one ligand: parallel_mc (src/lib/parallel_mc.cpp:80-98)
  * + refine + finish_docking (src/main/main.cpp:256-324), single-threaded or on

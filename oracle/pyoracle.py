@@ -51,6 +51,7 @@ def _load():
         "orc_mc_chain": (C.c_int, [vp, C.c_int, vp, C.c_int, C.c_int, F64P, U32P, C.c_int, F64P, F64P, F64P,
                                    I64P]),
         "orc_refine": (C.c_double, [vp, C.c_int, F64P, C.c_int, F64P, F64P]),
+        "orc_refine_opts": (C.c_double, [vp, C.c_int, F64P, C.c_int, C.c_int, F64P, F64P]),
         "orc_dock": (C.c_int, [vp, C.c_int, vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, F64P,
                                F64P, I64P]),
     }
@@ -199,12 +200,12 @@ class Oracle:
             raise RuntimeError(self.error())
         return e[:k], c[:k], x[:k], int(ev[0])
 
-    def refine(self, m, conf, maxiters, v=(1000.0, 1000.0, 1000.0)):
+    def refine(self, m, conf, maxiters, v=(1000.0, 1000.0, 1000.0), accurate=False):
         info = self.info(m)
         conf = np.ascontiguousarray(conf, dtype=np.float64).copy()
         vv = np.ascontiguousarray(v, dtype=np.float64)
         x = np.zeros(3 * info["nheavy"])
-        e = lib.orc_refine(self.h, m, _p(vv, F64P), maxiters, _p(conf, F64P), _p(x, F64P))
+        e = lib.orc_refine_opts(self.h, m, _p(vv, F64P), maxiters, int(accurate), _p(conf, F64P), _p(x, F64P))
         return e, conf, x
 
     def dock(self, m, params, seed=1, exhaustiveness=8, num_modes=9, energy_range=3.0, threads=1, max_out=32):

@@ -240,3 +240,26 @@ def test_invalid_requests_are_rejected(env):
         gid = ctx.build_grids(env["rid"], env["gd"], [2], sg.FP64)
         ctx.eval_confs(env["bids"][sg.FP64], [0], f.initial_conf()[None], f.change_size, f.natoms, (10, 10, 10),
                        grid_id=gid)
+
+
+def test_minimize_fp64_matches_oracle():
+    """--minimize (GpuDocking::minimize_batch): each parsed pose refined in
+    FP64 with the accurate line search inside its own movable_atoms_box,
+    several boxes in one launch; bit-exact against the oracle's
+    refine_structure over that box"""
+    rec = receptor()
+    ligs = [sg.Mol.ligand(40 + k, 16 + 6 * k, 2 + k, center=(0.5 * k, -0.5 * k, 0.25 * k)) for k in range(4)]
+    d = sg.Docking(rec, gd_c0(), fp64=False)
+    res = d.minimize(ligs, maxiters=0, accurate=True)
+    sf = sg.vina_scoring()
+    import pyoracle as po
+    for lig, r in zip(ligs, res):
+        gd = sg.GridDims()
+        for i in range(3):
+            gd.begin[i], gd.end[i] = r["box"][0][i], r["box"][1][i]
+            gd.n[i] = int(round((gd.end[i] - gd.begin[i]) / 0.375))
+        o = po.Oracle(sf, rec.xyz, rec.type, gd)
+        m = o.add_model(lig.desc_ptr)
+        e, _, _ = o.refine(m, o.initial_conf(m), 10000, accurate=True)
+        print(f"minimize: e {r['e']:.6f} rmsd {r['rmsd']:.3f}", flush=True)
+        assert r["e"] == e, (r["e"], e)
