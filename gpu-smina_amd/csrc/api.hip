@@ -146,10 +146,11 @@ struct sgpu_ctx
 	int device = 0;
 	hipStream_t stream = nullptr;
 	hipEvent_t ev0 = nullptr, ev1 = nullptr;
-	// size classes of one Monte Carlo launch run concurrently, one stream each
+	// size classes of one Monte Carlo launch run concurrently: the heaviest
+	// on `stream`, the others on these (one hardware queue per stream)
 	static constexpr int NCLASS = 4;
-	hipStream_t class_stream[NCLASS] = {};
-	hipEvent_t class_done[NCLASS] = {};
+	hipStream_t class_stream[NCLASS - 1] = {};
+	hipEvent_t class_done[NCLASS - 1] = {};
 	std::mutex mu;
 	std::string err;
 	Tables tab;
@@ -760,7 +761,7 @@ sgpu_status sgpu_create(int device, sgpu_ctx** out)
 	if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
 			hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
 		return SGPU_E_HIP;
-	for (int k = 0; k < sgpu_ctx::NCLASS; k++)
+	for (int k = 0; k < sgpu_ctx::NCLASS - 1; k++)
 		if (hipStreamCreateWithFlags(&c->class_stream[k], hipStreamNonBlocking) != hipSuccess ||
 				hipEventCreateWithFlags(&c->class_done[k], hipEventDisableTiming) != hipSuccess)
 			return SGPU_E_HIP;
@@ -779,7 +780,7 @@ void sgpu_destroy(sgpu_ctx* c)
 	c->receptors.clear();
 	(void)hipEventDestroy(c->ev0);
 	(void)hipEventDestroy(c->ev1);
-	for (int k = 0; k < sgpu_ctx::NCLASS; k++)
+	for (int k = 0; k < sgpu_ctx::NCLASS - 1; k++)
 	{
 		(void)hipStreamSynchronize(c->class_stream[k]);
 		(void)hipEventDestroy(c->class_done[k]);
@@ -1252,20 +1253,37 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 	int max_steps = 0;
 	for (int i = 0; i < n; i++)
 		max_steps = std::max(max_steps, ch->num_steps[i]);
+	// Heaviest class first, on the context's stream; each further class on a
+	// stream of its own (four streams at most: one hardware queue each, so
+	// no launch queues behind another stream's work).  The waits that join
+	// the streams are enqueued after every launch.
 	Timer tm(c, "mc");
-	for (int k = 0; k < sgpu_ctx::NCLASS; k++)
+	std::vector<hipStream_t> side;
+	for (int k = sgpu_ctx::NCLASS - 1; k >= 0; k--)
 	{
 		const Cls& cl = cls[k];
 		if (cl.blks.empty())
 			continue;
-		hipStream_t st = c->class_stream[k];
-		hip_check(hipStreamWaitEvent(st, c->ev0, 0), "stream wait");
+		hipStream_t st = c->stream;
+		bool first = true;
+		for (int j = k + 1; j < sgpu_ctx::NCLASS; j++)
+			if (!cls[j].blks.empty())
+				first = false;
+		if (!first)
+		{
+			st = c->class_stream[side.size()];
+			side.push_back(st);
+			hip_check(hipStreamWaitEvent(st, c->ev0, 0), "stream wait");
+		}
 		hip_check(launch_mc<T>(ptrs<T>(b), ec, cl.plan, P, L, dchains.as<char>(), int(cl.blks.size()), cl.waves,
 						  dbm.as<int>() + cl.block_off, dsc.as<int>() + cl.slot_off, dsteps.as<int>(), dmax.as<int>(),
 						  0, max_steps, st),
 				"mc kernel");
-		hip_check(hipEventRecord(c->class_done[k], st), "event");
-		hip_check(hipStreamWaitEvent(c->stream, c->class_done[k], 0), "stream wait");
+	}
+	for (size_t j = 0; j < side.size(); j++)
+	{
+		hip_check(hipEventRecord(c->class_done[j], side[j]), "event");
+		hip_check(hipStreamWaitEvent(c->stream, c->class_done[j], 0), "stream wait");
 	}
 	tm.stop();
 	hip_check(hipMemcpyAsync(init.data(), dchains.p, init.size(), hipMemcpyDeviceToHost, c->stream), "download");
