@@ -170,11 +170,12 @@ namespace
 // the LDS plan that fits every listed model of a packed batch
 WavePlan plan_for(const HostBatch& B, const std::vector<int>& models)
 {
-	int nmax = 0, fmax = 0, gmax = 0, pmax = 0, imax = 0, tmax = 0, amax = 0;
+	int nmax = 0, fmax = 0, cmax = 0, gmax = 0, pmax = 0, imax = 0, tmax = 0, amax = 0;
 	for (int mi : models)
 	{
 		const ModelHdr& h = B.hdr[size_t(mi)];
 		nmax = std::max(nmax, h.natoms);
+		cmax = std::max(cmax, h.conf_size);
 		amax = std::max(amax, h.natoms + h.ninflex);
 		fmax = std::max(fmax, h.nframes);
 		gmax = std::max(gmax, h.change_size);
@@ -182,11 +183,12 @@ WavePlan plan_for(const HostBatch& B, const std::vector<int>& models)
 		imax = std::max(imax, h.iblob_len);
 		tmax = std::max(tmax, h.tblob_len);
 	}
+	// tight sizes (multiples of 4): LDS decides how many chains share a CU
 	WavePlan p{};
-	p.nmax = std::max(WAVE, (nmax + WAVE - 1) / WAVE * WAVE);
+	p.nmax = std::max(4, (nmax + 3) / 4 * 4);
 	p.fmax = fmax;
-	p.cmax = WAVE;
-	p.gmax = WAVE;
+	p.cmax = std::max(8, (cmax + 3) / 4 * 4);
+	p.gmax = std::max(8, (gmax + 3) / 4 * 4);
 	p.pmax = std::max(1, pmax);
 	p.hmax = std::max(1, gmax * gmax);
 	p.imax = imax;
@@ -1196,12 +1198,23 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 		int maxper = 1;
 		for (int mi : k.models)
 			maxper = std::max(maxper, int(by_model[mi].size()));
-		k.waves = 8;
-		while (k.waves > 1 && k.waves / 2 >= maxper)
-			k.waves /= 2;
-		while (k.waves > 1 && plan_bytes(k.plan, b.precision, k.waves) > LDS_LIMIT)
-			k.waves /= 2;
-		require(plan_bytes(k.plan, b.precision, k.waves) <= LDS_LIMIT, "chain state exceeds LDS");
+		// chains per workgroup: the most resident waves per CU (8 at two
+		// waves per SIMD, the register limit), then the larger workgroup
+		k.waves = 0;
+		int best = 0;
+		for (int w = 1; w <= std::min(8, maxper); w++)
+		{
+			const int bytes = plan_bytes(k.plan, b.precision, w);
+			if (bytes > LDS_LIMIT)
+				break;
+			const int resident = w * std::min(LDS_LIMIT / bytes, 8 / w);
+			if (resident >= best)
+			{
+				best = resident;
+				k.waves = w;
+			}
+		}
+		require(k.waves > 0, "chain state exceeds LDS");
 		for (int mi : k.models)
 		{
 			const std::vector<int>& cs = by_model[mi];

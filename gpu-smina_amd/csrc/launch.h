@@ -18,7 +18,7 @@ template<typename T>
 hipError_t launch_eval(const BatchPtrs<T>& B, const EvalCtx<T>& ec, const WavePlan& plan, int n, const int* model,
 		const T* conf, int conf_stride, T* e, T* g, int g_stride, T* x, int x_stride, T* scratch, int pstride,
 		hipStream_t st);
-// nblocks workgroups of `waves` (1, 2, 4 or 8) chains of one ligand each
+// nblocks workgroups of `waves` (1..8) chains of one ligand each
 template<typename T>
 hipError_t launch_mc(const BatchPtrs<T>& B, const EvalCtx<T>& ec, const WavePlan& plan, const McParams& P,
 		const ChainLayout& L, char* chains, int nblocks, int waves, const int* block_model, const int* slot_chain,

@@ -44,7 +44,8 @@ constexpr int MAX_CHANGE = 64;
 constexpr int MAX_CONF = 64;
 constexpr int MAX_TREES = 16;
 constexpr int MAX_POSES = 64;
-constexpr int MAX_PATH = 32;      // frames on a root-to-frame path (ancestor walk of set_conf)
+constexpr int MAX_PATH = 32;
+constexpr int MAX_CHAINS_PER_GROUP = 8; // waves of one Monte Carlo workgroup      // frames on a root-to-frame path (ancestor walk of set_conf)
 
 // conf element kinds: position component, quaternion component a..d, torsion
 enum ConfKind

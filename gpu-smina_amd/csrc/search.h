@@ -52,7 +52,7 @@ constexpr bool pair_parallel()
 // then each wave's working state
 struct WavePlan
 {
-	int nmax;   // movable atoms (multiple of 64)
+	int nmax;   // movable atoms
 	int fmax;   // frames
 	int cmax;   // conf
 	int gmax;   // change
