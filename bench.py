@@ -26,7 +26,21 @@ sys.path.insert(0, os.path.join(ROOT, "gpu-smina_amd"))
 
 HBM_PEAK_GBS = 8000.0
 F64_STEP_DIV = 16  # plain FP64 sample: this fraction of the Monte Carlo steps
-CPU_STEP_DIV = 2   # CPU baseline sample: this fraction of one chain's steps
+CPU_STEP_DIV = 1   # CPU baseline sample: this fraction of one chain's steps
+TRAFFIC_FILE = "profiles/mc_kernel_screen1024_traffic.md"
+
+
+def measured_traffic(nligands, exhaustiveness):
+    """HBM bytes per Monte Carlo launch measured by rocprofv3 --pmc on the
+    1,024-ligand, exhaustiveness-8 screen (the workload is seeded, so the
+    launch is the same one); None for any other workload"""
+    path = os.path.join(ROOT, TRAFFIC_FILE)
+    if nligands != 1024 or exhaustiveness != 8 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
 
 
 def screen_ligands(sg, n, seed=2024):
@@ -91,6 +105,7 @@ def main():
     alg_bytes = float(np.sum(evals * bytes_per_eval))
     achieved = alg_bytes / (mc_ms / 1e3) / 1e9
     top = np.array([r["e"][0] if r["e"] else np.nan for r in results])
+    traffic = measured_traffic(len(ligs), args.exhaustiveness)
     log(f"screen: {value:.1f} ligands/s, MC kernel {mc_ms / 1e3:.2f}s, {evals.sum():.3e} evals")
 
     # plain per-ligand FP64 path (one ligand per launch) on a bounded sample:
@@ -175,9 +190,13 @@ def main():
         "latency_ms": {"p50": step_s * 1e3, "p99": step_s * 1e3,
                        "note": "one launch: every ligand of the screen completes with it"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "mc_kernel<float>", "kernel_ms": mc_ms,
-                     "algorithmic_bytes": alg_bytes},
+                     "algorithmic_bytes": alg_bytes,
+                     "note": "one launch = the four size-class dispatches of one Monte Carlo launch, run "
+                             "concurrently; kernel_ms spans them (HIP events on the launching stream); traffic "
+                             "(calibrated FETCH_SIZE + WRITE_SIZE bytes per launch) from " + TRAFFIC_FILE +
+                             " when this is the profiled workload, else null"},
         "cpu_baseline": cpu,
         "plain_fp64": plain,
         "speedup_vs_plain_fp64": (value / plain["value"]) if plain else None,
