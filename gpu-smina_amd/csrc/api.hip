@@ -260,9 +260,14 @@ void pack_batch(const sgpu_flat_model* ms, int n, HostBatch& B)
 		mark(h.o_code);
 		for (int i = 0; i < m.natoms; i++)
 			I.insert(I.end(), lists[i].begin(), lists[i].end());
-		mark(h.o_pcode);
+		if (I.size() % 2)
+			I.push_back(0); // pair records are read as 8-byte words
+		mark(h.o_prec);
 		for (int p = 0; p < m.npairs; p++)
-			I.push_back(make_pcode(m.pair_a[p], m.pair_b[p], m.pair_tri[p], p < m.nother));
+		{
+			I.push_back(int(unsigned(3 * m.pair_a[p]) | (unsigned(3 * m.pair_b[p]) << 16)));
+			I.push_back(int(unsigned(m.pair_tri[p]) | (unsigned(p < m.nother) << 16)));
+		}
 		// frames, children, levels
 		std::vector<int> depth(m.nframes, 0);
 		int nlevels = 0;

@@ -79,7 +79,7 @@ __device__ inline void bfgs_update(T* h, T* sp, T* smhy, T p, T y, T alpha, int 
 		return;
 	const T mhy = minus_mat_vec(h, y, n);
 	const T yhy = -dot_n(y, mhy, n);
-	const T r = 1 / (alpha * yp);
+	const T r = Real<T>::rcp(alpha * yp);
 	const int lane = lane_id();
 	if (lane < n)
 	{

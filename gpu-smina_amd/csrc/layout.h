@@ -20,8 +20,8 @@
  * type-pair table index (8-16), "this atom is the pair's b" (17), "the pair
  * is an other pair" (18) and the pair index (19-31), so the pair loop reads
  * one LDS word per neighbour.  The pair-parallel loop of the FP32 search
- * reads the pairs themselves, one word each: a (bits 0-7), b (8-15), the
- * table index (16-24) and "other pair" (25).
+ * reads the pairs themselves, two words each (coordinate offsets of a and
+ * b; table index and "other pair").
  *
  * Receptor maps are stored per ligand type as corner-packed cells: for each
  * grid cell (x fastest), its 8 corner values in the order f000 f100 f010
@@ -73,14 +73,13 @@ __host__ __device__ inline int make_code(int partner, int tri, bool is_b, bool o
 			(unsigned(pair) << 19));
 }
 
-__host__ __device__ inline int pcode_a(int c) { return c & 255; }
-__host__ __device__ inline int pcode_b(int c) { return (c >> 8) & 255; }
-__host__ __device__ inline int pcode_tri(int c) { return (c >> 16) & 511; }
-__host__ __device__ inline bool pcode_other(int c) { return (c >> 25) & 1; }
-__host__ __device__ inline int make_pcode(int a, int b, int tri, bool other)
-{
-	return int(unsigned(a) | (unsigned(b) << 8) | (unsigned(tri) << 16) | (unsigned(other) << 25));
-}
+// pair record of the pair-parallel loop: two words, the coordinate offsets
+// 3a (bits 0-15) and 3b (16-31), then the table index (0-15) and "other
+// pair" (16)
+__host__ __device__ inline int prec_xa(int w0) { return w0 & 0xffff; }
+__host__ __device__ inline int prec_xb(int w0) { return int(unsigned(w0) >> 16); }
+__host__ __device__ inline int prec_tri(int w1) { return w1 & 0xffff; }
+__host__ __device__ inline bool prec_other(int w1) { return (w1 >> 16) & 1; }
 
 struct ModelHdr
 {
@@ -91,7 +90,7 @@ struct ModelHdr
 	int tblob_off, tblob_len; // in the batch real blob
 	int pair_off;             // pair arrays (a, b, table index), global
 	// offsets inside the int blob
-	int o_type, o_afr, o_nbr, o_code, o_pcode, o_fparent, o_fkind, o_fbegin, o_fend, o_fconf, o_fchange, o_fsend, o_fdepth, o_fpath, o_cstart,
+	int o_type, o_afr, o_nbr, o_code, o_prec, o_fparent, o_fkind, o_fbegin, o_fend, o_fconf, o_fchange, o_fsend, o_fdepth, o_fpath, o_cstart,
 			o_ccount, o_clist, o_lstart, o_lframes, o_heavy, o_troot, o_tconf, o_tchange, o_tlig, o_tntors,
 			o_tab, o_tae, o_tpb, o_tpe, o_maps;
 	// offsets inside the real blob

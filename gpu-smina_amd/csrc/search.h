@@ -86,8 +86,8 @@ struct WaveMem
 template<typename T>
 __host__ __device__ inline int wave_lds_elems(const WavePlan& p)
 {
-	return 3 * p.nmax * 2 + p.fmax * (3 + 9 + 4 + 3 + 3 + 3) + p.gmax + p.nmax + p.cmax + p.gmax + p.hmax +
-			(pair_parallel<T>() ? 3 * p.pmax : 0);
+	return 3 * p.amax + 3 * p.nmax + p.fmax * (3 + 9 + 4 + 3 + 3 + 3) + p.gmax + p.nmax + p.cmax + p.gmax +
+			p.hmax + (pair_parallel<T>() ? 3 * p.pmax : 0);
 }
 // model region in bytes, rounded to 16
 template<typename T>
@@ -107,8 +107,8 @@ __device__ inline WaveMem<T> carve(T* base, const WavePlan& p)
 {
 	WaveMem<T> w;
 	T* q = base;
-	w.coords = q;
-	q += 3 * p.nmax;
+	w.coords = q; // movable atoms, then the fixed atoms (copied once)
+	q += 3 * p.amax;
 	w.forces = q;
 	q += 3 * p.nmax;
 	w.fo = q;
@@ -171,7 +171,7 @@ struct MV
 	const T* local;
 	const int* nbr_start;
 	const int* code;
-	const int* pcode; // the pairs, packed (pair-parallel loop)
+	const int2* prec; // the pairs as two-word records (pair-parallel loop)
 	const int* fparent;
 	const int* fkind;
 	const int* fbegin;
@@ -237,7 +237,7 @@ __device__ inline MV<T> stage_model(const BatchPtrs<T>& B, int m, unsigned char*
 	v.local = lt + h.o_local;
 	v.nbr_start = li + h.o_nbr;
 	v.code = li + h.o_code;
-	v.pcode = li + h.o_pcode;
+	v.prec = reinterpret_cast<const int2*>(li + h.o_prec);
 	v.fparent = li + h.o_fparent;
 	v.fkind = li + h.o_fkind;
 	v.fbegin = li + h.o_fbegin;
@@ -383,6 +383,16 @@ __device__ __forceinline__ void set_conf(const MV<T>& m, WaveMem<T>& W, T x_lane
 		const V3<T> x = ld3(W.fo + 3 * f) + mat_vec(W.fR + 9 * f, ld3(m.local + 3 * i));
 		st3(W.coords + 3 * i, x);
 	}
+	lsync();
+}
+
+// the fixed atoms (flexible-residue atoms that do not move) after the
+// movable ones in the wave's coordinate array, once per kernel
+template<typename T>
+__device__ inline void init_fixed_atoms(const MV<T>& m, WaveMem<T>& W)
+{
+	for (int k = lane_id(); k < 3 * m.h.ninflex; k += WAVE)
+		W.coords[3 * m.h.natoms + k] = m.xyz0[3 * m.h.natoms + k];
 	lsync();
 }
 
@@ -676,27 +686,33 @@ __device__ __forceinline__ T eval_deriv(const MV<T>& m, const EvalCtx<T>& ec, Wa
 		// intramolecular pairs, pair-parallel (pair t * 64 + lane), each
 		// pair's force into its own LDS slot
 		const int P = m.h.npairs;
+		// the curl caps of the two pair kinds; a cap below epsilon zeroes the
+		// term, as v < epsilon does in curl
+		const T v_lig = ec.v[0] < Real<T>::eps() ? T(0) : ec.v[0];
+		const T v_oth = ec.v[2] < Real<T>::eps() ? T(0) : ec.v[2];
+		const T factor = T(ec.tab.factor);
+		const int last_knot = ec.tab.n - 2;
+		const int row_stride = 2 * ec.tab.n;
 		for (int t0 = 0; t0 * WAVE < P; t0 += PAIR_BATCH)
 		{
-			// phases across the batch: codes, coordinates, table rows, then
+			// phases across the batch: records, coordinates, table rows, then
 			// the arithmetic, so each phase's loads are in flight together
-			int cd[PAIR_BATCH];
-			bool in[PAIR_BATCH];
+			int2 rc[PAIR_BATCH];
 #pragma unroll
 			for (int u = 0; u < PAIR_BATCH; u++)
 			{
 				const int p = (t0 + u) * WAVE + lane;
-				in[u] = p < P;
-				cd[u] = m.pcode[in[u] ? p : 0];
+				rc[u] = m.prec[p < P ? p : 0];
 			}
 			V3<T> xa[PAIR_BATCH], xb[PAIR_BATCH];
 #pragma unroll
 			for (int u = 0; u < PAIR_BATCH; u++)
 			{
-				xa[u] = ld3(W.coords + 3 * pcode_a(cd[u]));
-				xb[u] = atom_pos(m, W, pcode_b(cd[u]));
+				xa[u] = ld3(W.coords + prec_xa(rc[u].x));
+				xb[u] = ld3(W.coords + prec_xb(rc[u].x));
 			}
-			T rx[PAIR_BATCH], ry[PAIR_BATCH], rz[PAIR_BATCH], r2[PAIR_BATCH];
+			T rx[PAIR_BATCH], ry[PAIR_BATCH], rz[PAIR_BATCH], r2[PAIR_BATCH], rem[PAIR_BATCH];
+			T e1[PAIR_BATCH], d1[PAIR_BATCH], e2[PAIR_BATCH], d2[PAIR_BATCH];
 #pragma unroll
 			for (int u = 0; u < PAIR_BATCH; u++)
 			{
@@ -704,18 +720,12 @@ __device__ __forceinline__ T eval_deriv(const MV<T>& m, const EvalCtx<T>& ec, Wa
 				ry[u] = xb[u].y - xa[u].y;
 				rz[u] = xb[u].z - xa[u].z;
 				r2[u] = sqr(rx[u]) + sqr(ry[u]) + sqr(rz[u]);
-			}
-			T rem[PAIR_BATCH], e1[PAIR_BATCH], d1[PAIR_BATCH], e2[PAIR_BATCH], d2[PAIR_BATCH];
-			bool ok[PAIR_BATCH];
-#pragma unroll
-			for (int u = 0; u < PAIR_BATCH; u++)
-			{
-				// precalculate_linear_element::eval_deriv (src/lib/precalculate.h:111-151)
-				ok[u] = in[u] && r2[u] < cutoff_sqr;
-				const T r2f = T(ec.tab.factor) * r2[u];
-				const int i1 = ok[u] ? int(r2f) : 0;
+				// precalculate_linear_element::eval_deriv (src/lib/precalculate.h:111-151);
+				// the knot is clamped so that the row read stays in the table
+				const T r2f = factor * r2[u];
+				const int i1 = min(int(r2f), last_knot);
 				rem[u] = r2f - T(i1);
-				const T* tp = ec.tab.smooth + (size_t(pcode_tri(cd[u])) * ec.tab.n + i1) * 2;
+				const T* tp = ec.tab.smooth + int(__umul24(unsigned(prec_tri(rc[u].y)), unsigned(row_stride)) + 2 * i1);
 				e1[u] = tp[0];
 				d1[u] = tp[1];
 				e2[u] = tp[2];
@@ -724,17 +734,17 @@ __device__ __forceinline__ T eval_deriv(const MV<T>& m, const EvalCtx<T>& ec, Wa
 #pragma unroll
 			for (int u = 0; u < PAIR_BATCH; u++)
 			{
-				// interpolation and curl without branches (src/lib/curl.h:28-43)
+				const int p = (t0 + u) * WAVE + lane;
+				const bool ok = p < P && r2[u] < cutoff_sqr;
+				// interpolation and curl (src/lib/curl.h:28-43) without branches
 				T pe = e1[u] + rem[u] * (e2[u] - e1[u]);
 				const T dor = d1[u] + rem[u] * (d2[u] - d1[u]);
-				const T v = pcode_other(cd[u]) ? ec.v[2] : ec.v[0];
-				const T tmp = v < Real<T>::eps() ? T(0) : Real<T>::fdiv(v, v + pe);
-				const T k = pe > 0 ? tmp : T(1);
-				pe *= k;
-				const T fk = ok[u] ? dor * (k * k) : T(0);
-				e_part += ok[u] ? pe : T(0);
-				if (in[u])
-					st3(W.pforce + 3 * ((t0 + u) * WAVE + lane), v3(fk * rx[u], fk * ry[u], fk * rz[u]));
+				const T v = prec_other(rc[u].y) ? v_oth : v_lig;
+				const T k = pe > 0 ? Real<T>::fdiv(v, v + pe) : T(1);
+				const T fk = ok ? dor * (k * k) : T(0);
+				e_part += ok ? pe * k : T(0);
+				if (p < P)
+					st3(W.pforce + 3 * p, v3(fk * rx[u], fk * ry[u], fk * rz[u]));
 			}
 		}
 		lsync();

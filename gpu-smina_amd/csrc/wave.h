@@ -102,8 +102,11 @@ template<> struct Real<float>
 	__device__ static float ceil(float x) { return ::ceilf(x); }
 	__device__ static float eps() { return FLT_EPSILON; }
 	__device__ static float max() { return FLT_MAX; }
-	// a / b through the hardware reciprocal (1 ulp), for the served search
+	// a / b, 1 / x and 1 / sqrt(x) through the hardware reciprocal and
+	// reciprocal square root (1 ulp), for the served search
 	__device__ static float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+	__device__ static float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+	__device__ static float rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
 };
 template<> struct Real<double>
 {
@@ -117,6 +120,8 @@ template<> struct Real<double>
 	__device__ static double eps() { return DBL_EPSILON; }
 	__device__ static double max() { return DBL_MAX; }
 	__device__ static double fdiv(double a, double b) { return a / b; }
+	__device__ static double rcp(double x) { return 1 / x; }
+	__device__ static double rsqrt(double x) { return 1 / __builtin_sqrt(x); }
 };
 
 template<typename T>
@@ -207,7 +212,7 @@ template<typename T> __device__ inline Q4<T> rotation_q(V3<T> rot)
 	const T angle = Real<T>::sqrt(norm_sqr(rot));
 	if (angle > Real<T>::eps())
 	{
-		V3<T> axis = (T(1) / angle) * rot;
+		V3<T> axis = Real<T>::rcp(angle) * rot;
 		return axis_angle_q(axis, angle);
 	}
 	return {T(1), T(0), T(0), T(0)};
@@ -218,7 +223,7 @@ template<typename T> __device__ inline void q_normalize_approx(Q4<T>& q)
 	const T s = sqr(q.a) + sqr(q.b) + sqr(q.c) + sqr(q.d);
 	if (Real<T>::abs(s - 1) < T(1e-6))
 		return;
-	const T inv = T(1) / Real<T>::sqrt(s);
+	const T inv = Real<T>::rsqrt(s);
 	q.a *= inv;
 	q.b *= inv;
 	q.c *= inv;
