@@ -592,7 +592,41 @@ GpuDocking::GpuDocking(const GpuSettings& s, const sgpu_scoring& sf, const sgpu_
 	check(ctx_, sgpu_upload_receptor(ctx_, &rec, &receptor_));
 }
 
-GpuDocking::~GpuDocking() { sgpu_destroy(ctx_); }
+GpuDocking::GpuDocking(const GpuSettings& s, const sgpu_scoring& sf, const sgpu_receptor& rec,
+		const sgpu_grid_dims& gd, sgpu_ctx* shared) :
+		s_(s), gd_(gd)
+{
+	if (!shared)
+		throw std::invalid_argument("no shared context");
+	ctx_ = shared;
+	owns_ctx_ = false;
+	terms_.assign(sf.terms, sf.terms + sf.nterms);
+	sf_ = sf;
+	sf_.terms = terms_.data();
+	Tables t = build_tables(sf_, 32);
+	sgpu_tables tt{t.n, t.factor, t.cutoff_sqr, t.smooth.data(), t.fast.data()};
+	check(ctx_, sgpu_upload_tables(ctx_, &tt));
+	check(ctx_, sgpu_upload_receptor(ctx_, &rec, &receptor_));
+}
+
+GpuDocking::~GpuDocking()
+{
+	if (owns_ctx_)
+		sgpu_destroy(ctx_);
+	else if (grid_ >= 0)
+		sgpu_release_grid(ctx_, grid_);
+}
+
+size_t GpuDocking::map_bytes() const
+{
+	if (grid_ < 0)
+		return 0;
+	size_t cells = 1;
+	for (int d = 0; d < 3; d++)
+		cells *= size_t(gd_.n[d]);
+	const size_t points = size_t(gd_.n[0] + 1) * (gd_.n[1] + 1) * (gd_.n[2] + 1);
+	return grid_types_.size() * (cells * 8 * (s_.fp64 ? 8 : 4) + points * 8);
+}
 
 std::string GpuDocking::description() const
 {
@@ -809,10 +843,11 @@ void GpuDocking::dock_sub(DockJob* const* jobs, size_t nj)
 	std::vector<SearchParams> sp;
 	for (size_t j = 0; j < nj; j++)
 	{
-		sp.push_back(setup_search(fms[j], s_.exhaustiveness));
+		const int exh = jobs[j]->exhaustiveness > 0 ? jobs[j]->exhaustiveness : s_.exhaustiveness;
+		sp.push_back(setup_search(fms[j], exh));
 		Gen g(uint32_t(s_.seed));
 		first[j] = cmodel.size();
-		for (int t = 0; t < s_.exhaustiveness; t++)
+		for (int t = 0; t < exh; t++)
 		{
 			const int seed = random_int(0, 1000000, g);
 			std::vector<double> conf(cstride, 0.0);

@@ -92,6 +92,10 @@ class GpuDocking
 {
 public:
 	GpuDocking(const GpuSettings& s, const sgpu_scoring& sf, const sgpu_receptor& rec, const sgpu_grid_dims& gd);
+	// the same on a device context that the caller owns and shares (serving:
+	// several resident receptors per context)
+	GpuDocking(const GpuSettings& s, const sgpu_scoring& sf, const sgpu_receptor& rec, const sgpu_grid_dims& gd,
+			sgpu_ctx* shared);
 	~GpuDocking();
 	GpuDocking(const GpuDocking&) = delete;
 	GpuDocking& operator=(const GpuDocking&) = delete;
@@ -99,6 +103,7 @@ public:
 	struct DockJob
 	{
 		const sgpu_model_desc* m;
+		int exhaustiveness = 0; // 0: the settings' value
 		std::vector<Pose> out; // final, rescored poses (finish_docking order)
 		int64_t evals = 0;
 		double seconds = 0;
@@ -128,6 +133,10 @@ public:
 
 	// receptor maps for every heavy-atom type of these models
 	void prepare(const std::vector<const sgpu_model_desc*>& models);
+	// receptor maps for these ligand atom types
+	void prepare_types(const std::vector<int>& types) { ensure_grids(types); }
+	// device bytes of the maps built so far
+	size_t map_bytes() const;
 	sgpu_ctx* ctx() { return ctx_; }
 	int receptor_id() const { return receptor_; }
 	std::string description() const;
@@ -137,6 +146,7 @@ private:
 	void dock_sub(DockJob* const* jobs, size_t n);
 	GpuSettings s_;
 	sgpu_ctx* ctx_ = nullptr;
+	bool owns_ctx_ = true;
 	sgpu_scoring sf_;
 	std::vector<sgpu_term> terms_;
 	sgpu_grid_dims gd_;

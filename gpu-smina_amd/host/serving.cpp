@@ -1,0 +1,315 @@
+#include "serving.h"
+
+#include <algorithm>
+#include <chrono>
+#include <stdexcept>
+
+namespace sgpu_host
+{
+
+DockingServer::DockingServer(const ServerSettings& s, const sgpu_scoring& sf) : s_(s)
+{
+	if (s_.workers < 1 || s_.max_batch < 1 || !(s_.budget_bytes > 0))
+		throw std::invalid_argument("server: workers, max_batch and budget must be positive");
+	terms_.assign(sf.terms, sf.terms + sf.nterms);
+	sf_ = sf;
+	sf_.terms = terms_.data();
+	t0_ns_ = std::chrono::duration_cast<std::chrono::nanoseconds>(
+			std::chrono::steady_clock::now().time_since_epoch()).count();
+	for (int k = 0; k < s_.workers; k++)
+	{
+		std::unique_ptr<Worker> w(new Worker);
+		if (sgpu_create(s_.device, &w->ctx) != SGPU_OK)
+			throw std::runtime_error("server: no HIP device");
+		workers_.push_back(std::move(w));
+	}
+	for (auto& w : workers_)
+	{
+		Worker* wp = w.get();
+		w->thread = std::thread([this, wp]() { run(wp); });
+	}
+}
+
+DockingServer::~DockingServer()
+{
+	{
+		std::lock_guard<std::mutex> lock(mu_);
+		stop_ = true;
+	}
+	cv_work_.notify_all();
+	for (auto& w : workers_)
+		if (w->thread.joinable())
+			w->thread.join();
+	for (auto& w : workers_)
+	{
+		w->receptors.clear();
+		sgpu_destroy(w->ctx);
+	}
+}
+
+double DockingServer::now() const
+{
+	const int64_t t = std::chrono::duration_cast<std::chrono::nanoseconds>(
+			std::chrono::steady_clock::now().time_since_epoch()).count();
+	return double(t - t0_ns_) * 1e-9;
+}
+
+int DockingServer::add_receptor(const sgpu_receptor& rec, const sgpu_grid_dims& gd, const std::vector<int>& types)
+{
+	std::lock_guard<std::mutex> lock(mu_);
+	if (next_id_ > 0)
+		throw std::logic_error("server: receptors are added before the first job");
+	Rec r;
+	r.xyz.assign(rec.xyz, rec.xyz + 3 * size_t(rec.natoms));
+	r.type.assign(rec.type, rec.type + rec.natoms);
+	r.rec = sgpu_receptor{rec.natoms, r.xyz.data(), r.type.data()};
+	r.gd = gd;
+	r.types = types;
+	recs_.push_back(std::move(r));
+	const Rec& rr = recs_.back();
+	for (auto& w : workers_)
+	{
+		std::unique_ptr<GpuDocking> d(new GpuDocking(s_.dock, sf_, sgpu_receptor{rr.rec.natoms, rr.xyz.data(),
+				rr.type.data()}, rr.gd, w->ctx));
+		d->prepare_types(rr.types);
+		w->receptors.push_back(std::move(d));
+	}
+	return int(recs_.size()) - 1;
+}
+
+int DockingServer::submit(int receptor, const sgpu_model_desc* ligand, int exhaustiveness)
+{
+	std::lock_guard<std::mutex> lock(mu_);
+	if (receptor < 0 || receptor >= int(recs_.size()) || !ligand || exhaustiveness < 1)
+		throw std::invalid_argument("server: invalid job");
+	std::unique_ptr<Job> j(new Job);
+	j->id = next_id_++;
+	j->receptor = receptor;
+	j->ligand = ligand;
+	j->exhaustiveness = exhaustiveness;
+	j->t_submit = now();
+	queue_.push_back(j.get());
+	const int id = j->id;
+	jobs_[id] = std::move(j);
+	cv_work_.notify_one();
+	return id;
+}
+
+size_t DockingServer::job_bytes(const Job& j) const
+{
+	// chain state (MT words, output container, coordinates) and the model
+	const int n = j.ligand->natoms;
+	const size_t chain = 2600 + 64 * size_t(8 * 64 + 4 * 3 * n) + 16 * size_t(n);
+	return size_t(j.exhaustiveness) * chain + 64 * size_t(n) * size_t(n);
+}
+
+void DockingServer::run(Worker* w)
+{
+	for (;;)
+	{
+		std::vector<Job*> batch;
+		int receptor = -1;
+		{
+			std::unique_lock<std::mutex> lock(mu_);
+			cv_work_.wait(lock, [this]() { return stop_ || !queue_.empty(); });
+			if (stop_ && queue_.empty())
+				return;
+			// the receptor of the job that has waited longest, then its queue in
+			// arrival order, within the batch size and the memory budget
+			receptor = queue_.front()->receptor;
+			const size_t maps = w->receptors[size_t(receptor)]->map_bytes();
+			size_t bytes = maps;
+			for (auto it = queue_.begin(); it != queue_.end() && int(batch.size()) < s_.max_batch;)
+			{
+				Job* j = *it;
+				if (j->receptor != receptor)
+				{
+					++it;
+					continue;
+				}
+				const size_t b = job_bytes(*j);
+				if (!batch.empty() && double(bytes + b) > s_.budget_bytes)
+					break;
+				bytes += b;
+				batch.push_back(j);
+				it = queue_.erase(it);
+			}
+			const double t = now();
+			for (Job* j : batch)
+				j->t_admit = t;
+		}
+		std::vector<GpuDocking::DockJob> dj(batch.size());
+		std::vector<GpuDocking::DockJob*> dp;
+		for (size_t k = 0; k < batch.size(); k++)
+		{
+			dj[k].m = batch[k]->ligand;
+			dj[k].exhaustiveness = batch[k]->exhaustiveness;
+			dp.push_back(&dj[k]);
+		}
+		std::string err;
+		try
+		{
+			w->receptors[size_t(receptor)]->dock_batch(dp);
+		}
+		catch (std::exception& e)
+		{
+			err = e.what();
+		}
+		{
+			std::lock_guard<std::mutex> lock(mu_);
+			const double t = now();
+			if (!err.empty())
+				err_ = err;
+			for (size_t k = 0; k < batch.size(); k++)
+			{
+				Job* j = batch[k];
+				j->t_done = t;
+				j->result.poses = std::move(dj[k].out);
+				j->result.evals = dj[k].evals;
+				j->result.latency_s = j->t_done - j->t_submit;
+				j->result.queued_s = j->t_admit - j->t_submit;
+				j->result.batch_size = int(batch.size());
+				j->done = true;
+			}
+		}
+		cv_done_.notify_all();
+	}
+}
+
+DockingServer::Result DockingServer::wait(int id)
+{
+	std::unique_lock<std::mutex> lock(mu_);
+	auto it = jobs_.find(id);
+	if (it == jobs_.end())
+		throw std::invalid_argument("server: unknown job");
+	Job* j = it->second.get();
+	cv_done_.wait(lock, [j]() { return j->done; });
+	return j->result;
+}
+
+void DockingServer::drain()
+{
+	std::unique_lock<std::mutex> lock(mu_);
+	cv_done_.wait(lock, [this]() {
+		for (auto& kv : jobs_)
+			if (!kv.second->done)
+				return false;
+		return true;
+	});
+}
+
+std::string DockingServer::error() const
+{
+	std::lock_guard<std::mutex> lock(mu_);
+	return err_;
+}
+
+} // namespace sgpu_host
+
+// ---------------------------------------------------------------- C wrappers
+using namespace sgpu_host;
+
+struct sgpu_server
+{
+	std::unique_ptr<DockingServer> s;
+	std::vector<sgpu_term> terms;
+	std::string err;
+};
+
+extern "C" {
+
+sgpu_server* sgpu_server_create(const sgpu_server_settings* s)
+{
+	sgpu_server* sv = new sgpu_server;
+	try
+	{
+		ServerSettings ss;
+		ss.device = s->device;
+		ss.workers = s->workers;
+		ss.max_batch = s->max_batch;
+		ss.budget_bytes = s->budget_gb * 1e9;
+		ss.dock.device = s->device;
+		ss.dock.fp64 = s->fp64 != 0;
+		ss.dock.seed = s->seed;
+		ss.dock.num_modes = s->num_modes;
+		ss.dock.energy_range = s->energy_range;
+		sv->terms.resize(5);
+		sgpu_scoring sf;
+		sgpu_vina_scoring(sv->terms.data(), &sf, s->hydrophobic_scale);
+		sv->s.reset(new DockingServer(ss, sf));
+	}
+	catch (std::exception& e)
+	{
+		sv->err = e.what();
+	}
+	return sv;
+}
+
+void sgpu_server_free(sgpu_server* sv) { delete sv; }
+
+const char* sgpu_server_error(const sgpu_server* sv)
+{
+	if (!sv->err.empty() || !sv->s)
+		return sv->err.c_str();
+	static thread_local std::string e;
+	e = sv->s->error();
+	return e.c_str();
+}
+
+int sgpu_server_add_receptor(sgpu_server* sv, const sgpu_mol* receptor, const sgpu_grid_dims* gd, const int* types,
+		int ntypes)
+{
+	if (!sv->s)
+		return -1;
+	try
+	{
+		const sgpu_model_desc* rd = sgpu_mol_desc(receptor);
+		return sv->s->add_receptor(sgpu_receptor{rd->natoms, rd->xyz, rd->type}, *gd,
+				std::vector<int>(types, types + ntypes));
+	}
+	catch (std::exception& e)
+	{
+		sv->err = e.what();
+		return -1;
+	}
+}
+
+int sgpu_server_submit(sgpu_server* sv, int receptor, const sgpu_mol* ligand, int exhaustiveness)
+{
+	if (!sv->s)
+		return -1;
+	try
+	{
+		return sv->s->submit(receptor, sgpu_mol_desc(ligand), exhaustiveness);
+	}
+	catch (std::exception& e)
+	{
+		sv->err = e.what();
+		return -1;
+	}
+}
+
+int sgpu_server_wait(sgpu_server* sv, int job, sgpu_serve_result* out)
+{
+	if (!sv->s)
+		return -1;
+	try
+	{
+		DockingServer::Result r = sv->s->wait(job);
+		out->nposes = int(std::min<size_t>(32, r.poses.size()));
+		for (int k = 0; k < out->nposes; k++)
+			out->e[k] = r.poses[size_t(k)].e;
+		out->evals = r.evals;
+		out->latency_s = r.latency_s;
+		out->queued_s = r.queued_s;
+		out->batch_size = r.batch_size;
+		return 0;
+	}
+	catch (std::exception& e)
+	{
+		sv->err = e.what();
+		return -1;
+	}
+}
+
+} // extern "C"

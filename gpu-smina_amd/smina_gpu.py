@@ -107,6 +107,17 @@ class DockSettings(C.Structure):
                 ("ligands_per_launch", C.c_int), ("step_fraction", C.c_double)]
 
 
+class ServerSettings(C.Structure):
+    _fields_ = [("device", C.c_int), ("workers", C.c_int), ("max_batch", C.c_int), ("budget_gb", C.c_double),
+                ("fp64", C.c_int), ("seed", C.c_int), ("num_modes", C.c_int), ("energy_range", C.c_double),
+                ("hydrophobic_scale", C.c_double)]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [("nposes", C.c_int), ("e", C.c_double * 32), ("evals", C.c_int64), ("latency_s", C.c_double),
+                ("queued_s", C.c_double), ("batch_size", C.c_int)]
+
+
 class MinimizeResult(C.Structure):
     _fields_ = [("e", C.c_double), ("rmsd", C.c_double), ("box_begin", C.c_double * 3), ("box_end", C.c_double * 3)]
 
@@ -154,6 +165,12 @@ def _load():
         "sgpu_docking_error": (C.c_char_p, [vp]),
         "sgpu_dock": (C.c_int, [vp, C.POINTER(vp), C.c_int, C.POINTER(DockResult)]),
         "sgpu_minimize": (C.c_int, [vp, C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.POINTER(MinimizeResult)]),
+        "sgpu_server_create": (vp, [C.POINTER(ServerSettings)]),
+        "sgpu_server_free": (None, [vp]),
+        "sgpu_server_error": (C.c_char_p, [vp]),
+        "sgpu_server_add_receptor": (C.c_int, [vp, vp, C.POINTER(GridDims), I32P, C.c_int]),
+        "sgpu_server_submit": (C.c_int, [vp, C.c_int, vp, C.c_int]),
+        "sgpu_server_wait": (C.c_int, [vp, C.c_int, C.POINTER(ServeResult)]),
         "sgpu_docking_prepare": (C.c_int, [vp, C.POINTER(vp), C.c_int]),
         "sgpu_docking_ctx": (vp, [vp]),
         "sgpu_docking_receptor_id": (C.c_int, [vp]),
@@ -502,3 +519,49 @@ class Docking:
         if lib.sgpu_minimize(self.h, arr, len(ligands), maxiters, int(accurate), out) != 0:
             raise RuntimeError(lib.sgpu_docking_error(self.h).decode())
         return [dict(e=r.e, rmsd=r.rmsd, box=(tuple(r.box_begin), tuple(r.box_end))) for r in out]
+
+
+class Server:
+    """The serving path: receptors resident on the device, docking jobs
+    admitted as they arrive and batched continuously over `workers`
+    concurrent launches (sgpu_server_*)."""
+
+    def __init__(self, workers=2, max_batch=64, budget_gb=64.0, fp64=False, seed=1, num_modes=9,
+                 energy_range=3.0, hydrophobic_scale=1.0, device=0):
+        s = ServerSettings(device, workers, max_batch, budget_gb, int(fp64), seed, num_modes, energy_range,
+                           hydrophobic_scale)
+        self.h = C.c_void_p(lib.sgpu_server_create(C.byref(s)))
+        err = lib.sgpu_server_error(self.h)
+        if err:
+            msg = err.decode()
+            lib.sgpu_server_free(self.h)
+            self.h = None
+            raise RuntimeError(msg)
+        self._keep = {}
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.sgpu_server_free(self.h)
+            self.h = None
+
+    def _check(self, v):
+        if v < 0:
+            raise RuntimeError(lib.sgpu_server_error(self.h).decode())
+        return v
+
+    def add_receptor(self, receptor, gd, types):
+        t = i32(types)
+        self._keep[("rec", id(receptor))] = (receptor, gd)
+        return self._check(lib.sgpu_server_add_receptor(self.h, receptor.h, C.byref(gd), ptr(t, I32P), len(t)))
+
+    def submit(self, receptor_index, ligand, exhaustiveness=8):
+        job = self._check(lib.sgpu_server_submit(self.h, receptor_index, ligand.h, exhaustiveness))
+        self._keep[job] = ligand
+        return job
+
+    def wait(self, job):
+        r = ServeResult()
+        self._check(lib.sgpu_server_wait(self.h, job, C.byref(r)))
+        self._keep.pop(job, None)
+        return dict(e=list(r.e[:r.nposes]), evals=r.evals, latency_s=r.latency_s, queued_s=r.queued_s,
+                    batch_size=r.batch_size)

@@ -91,6 +91,42 @@ int sgpu_minimize(sgpu_docking* d, const sgpu_mol* const* ligands, int n, int ma
 
 /* low-level access for tests: the context and the device ids of the
  * receptor, grids and tables this docking object built */
+/* ---- serving (the served, batched FP32 path): receptors resident on the
+ * device, jobs admitted as they arrive and batched continuously over
+ * `workers` concurrent launches.  Receptors are added before the first job. */
+typedef struct sgpu_server sgpu_server;
+typedef struct sgpu_server_settings
+{
+	int device;
+	int workers;           /* concurrent launches (each its own device context) */
+	int max_batch;         /* ligands per launch */
+	double budget_gb;      /* device memory per worker for maps and jobs */
+	int fp64;
+	int seed;              /* --seed of every job */
+	int num_modes;
+	double energy_range;
+	double hydrophobic_scale;
+} sgpu_server_settings;
+typedef struct sgpu_serve_result
+{
+	int nposes;
+	double e[32];
+	int64_t evals;
+	double latency_s;      /* submit to result */
+	double queued_s;       /* submit to admission */
+	int batch_size;        /* jobs in its launch */
+} sgpu_serve_result;
+sgpu_server* sgpu_server_create(const sgpu_server_settings* s);
+void sgpu_server_free(sgpu_server* sv);
+const char* sgpu_server_error(const sgpu_server* sv);
+/* returns the receptor's index (< 0 on error); its maps cover `types` */
+int sgpu_server_add_receptor(sgpu_server* sv, const sgpu_mol* receptor, const sgpu_grid_dims* gd,
+		const int* types, int ntypes);
+/* returns the job id (< 0 on error); the ligand must outlive the job */
+int sgpu_server_submit(sgpu_server* sv, int receptor, const sgpu_mol* ligand, int exhaustiveness);
+/* blocks until the job completes */
+int sgpu_server_wait(sgpu_server* sv, int job, sgpu_serve_result* out);
+
 sgpu_ctx* sgpu_docking_ctx(sgpu_docking* d);
 int sgpu_docking_receptor_id(sgpu_docking* d);
 

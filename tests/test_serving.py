@@ -1,0 +1,48 @@
+"""The serving path (sgpu_server_*): receptors resident on the device, jobs
+admitted as they arrive and batched continuously over concurrent launches.
+Each job's result must be the same alone and in the mix: its chains are
+seeded by the job alone and never interact with other jobs' chains."""
+import numpy as np
+import pytest
+
+import smina_gpu as sg
+
+pytestmark = pytest.mark.gpu
+
+
+def test_results_same_alone_and_in_the_mix():
+    recs = [sg.Mol.receptor(11 + k, 1200 + 400 * k, 8.0) for k in range(2)]
+    gds = [sg.grid_dims((0.0, 0.0, 0.0), (20.0, 20.0, 20.0)) for _ in recs]
+    types = list(range(2, 14))  # the carbon, nitrogen and oxygen kinds (smina types 2-13)
+    ligs = [sg.Mol.ligand(300 + k, 10 + 3 * k, min(k, 5)) for k in range(8)]
+    exh = [2, 4, 2, 8, 4, 2, 4, 2]
+    used = sorted({int(t) for l in ligs for t in l.type[:l.nmovable] if t > 1})
+    assert set(used) <= set(types), used
+    srv = sg.Server(workers=2, max_batch=3)
+    ri = [srv.add_receptor(r, g, types) for r, g in zip(recs, gds)]
+    # the mix: every job submitted at once, alternating receptors
+    jobs = [srv.submit(ri[k % 2], l, exh[k]) for k, l in enumerate(ligs)]
+    mixed = [srv.wait(j) for j in jobs]
+    assert max(r["batch_size"] for r in mixed) > 1
+    # alone: one job at a time
+    for k, l in enumerate(ligs):
+        alone = srv.wait(srv.submit(ri[k % 2], l, exh[k]))
+        print(f"job {k}: batch {mixed[k]['batch_size']} e {mixed[k]['e'][:1]} latency {mixed[k]['latency_s']:.2f}s",
+              flush=True)
+        assert alone["batch_size"] == 1
+        assert alone["e"] == mixed[k]["e"] and alone["evals"] == mixed[k]["evals"], k
+    del srv
+
+
+def test_server_matches_docking_facade():
+    """a served job equals GpuDocking::dock_batch of the same ligand"""
+    rec = sg.Mol.receptor(11, 1200, 8.0)
+    gd = sg.grid_dims((0.0, 0.0, 0.0), (20.0, 20.0, 20.0))
+    lig = sg.Mol.ligand(77, 18, 3)
+    types = sorted({int(t) for t in lig.type[:lig.nmovable] if t > 1})
+    srv = sg.Server(workers=1)
+    r = srv.wait(srv.submit(srv.add_receptor(rec, gd, types), lig, 4))
+    d = sg.Docking(rec, gd, exhaustiveness=4)
+    d.prepare([lig])
+    ref = d.dock([lig])[0]
+    assert r["e"] == ref["e"] and r["evals"] == ref["evals"]
