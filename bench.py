@@ -17,6 +17,7 @@ import concurrent.futures as cf
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -73,13 +74,25 @@ def serve(sg, args, log):
     ligs = [sg.Mol.ligand(5000 + i, int(h), int(tt)) for i, (h, tt) in enumerate(zip(heavy, tors))]
     gaps = rng.exponential(1.0 / args.rate, args.jobs)
     t0 = time.perf_counter()
-    ids = []
+    res = [None] * args.jobs
+
+    def waiter(i, job):
+        res[i] = srv.wait(job)
+        log(f"job {i} done: {int(heavy[i])} heavy atoms, exhaustiveness {int(exh[i])}, "
+            f"latency {res[i]['latency_s']:.1f}s, batch {res[i]['batch_size']}")
+
+    waiters = []
     for i in range(args.jobs):
         due = t0 + float(np.sum(gaps[:i + 1]))
         while time.perf_counter() < due:
             time.sleep(min(0.01, due - time.perf_counter()))
-        ids.append(srv.submit(ri[int(target[i])], ligs[i], int(exh[i])))
-    res = [srv.wait(j) for j in ids]
+        th = threading.Thread(target=waiter, args=(i, srv.submit(ri[int(target[i])], ligs[i], int(exh[i]))))
+        th.start()
+        waiters.append(th)
+        if (i + 1) % 8 == 0:
+            log(f"submitted {i + 1}/{args.jobs}")
+    for th in waiters:
+        th.join()
     wall = time.perf_counter() - t0
     lat = np.array([r["latency_s"] for r in res])
     line = {
@@ -111,7 +124,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=0, help="ligands for the CPU baseline (0: one per core)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve", action="store_true", help="the serving workload instead of the screen")
-    ap.add_argument("--jobs", type=int, default=64, help="serving: jobs")
+    ap.add_argument("--jobs", type=int, default=32, help="serving: jobs")
     ap.add_argument("--rate", type=float, default=1.0, help="serving: Poisson arrival rate, jobs/s")
     ap.add_argument("--workers", type=int, default=3, help="serving: concurrent launches")
     args = ap.parse_args()
