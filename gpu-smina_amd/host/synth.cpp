@@ -543,4 +543,154 @@ sgpu_mol* sgpu_synth_ligand(uint32_t seed, int nheavy, int ntors, const double* 
 	return m;
 }
 
+// Flexible side chains cut from a synthetic receptor, in the layout smina's
+// flexible-residue parser builds (src/lib/parse_pdbqt.cpp:360-380,
+// postprocess_residue): per residue, the root atom stays fixed ("inflex"),
+// each rotatable bond below it is a branch, the first one a first_segment
+// rotating about the bond from the root atom.  A side chain with c torsions
+// is the last c + 2 heavy atoms of a receptor chain that ends in the pocket:
+// anchor, a1 .. a(c+1); frame k (1..c) holds a(k) with its hydrogens, the
+// last frame also a(c+1); frame k turns about the bond a(k-1) -> a(k)
+// (a0 = anchor).  The model is the ligand's trees, then one tree per side
+// chain, then the anchors; *rigid receives the receptor without the side
+// chains' atoms (anchors included).
+sgpu_mol* sgpu_synth_flex(const sgpu_mol* receptor, const sgpu_mol* ligand, uint32_t seed, int nflex, int min_chi,
+		int max_chi, const double* center, sgpu_mol** rigid)
+{
+	if (!receptor || !ligand || !rigid || nflex < 0 || min_chi < 1 || max_chi < min_chi ||
+			ligand->nmovable != ligand->natoms())
+		return nullptr;
+	*rigid = nullptr;
+	Gen g(seed * 2246822519u + 77u);
+	const int na = receptor->natoms();
+	std::vector<std::vector<int>> heavy_nb(na), hyd(na);
+	for (size_t k = 0; k + 1 < receptor->bonds.size(); k += 2)
+	{
+		const int a = receptor->bonds[k], b = receptor->bonds[k + 1];
+		const bool ha = receptor->type[a] == T_PolarHydrogen, hb = receptor->type[b] == T_PolarHydrogen;
+		if (!ha && !hb)
+		{
+			heavy_nb[a].push_back(b);
+			heavy_nb[b].push_back(a);
+		}
+		else if (ha != hb)
+			hyd[ha ? b : a].push_back(ha ? a : b);
+	}
+	const V3 c0{center ? center[0] : 0, center ? center[1] : 0, center ? center[2] : 0};
+	auto at = [&](int a) { return V3{receptor->xyz[3 * a], receptor->xyz[3 * a + 1], receptor->xyz[3 * a + 2]}; };
+	// chain ends, nearest the centre first
+	std::vector<int> ends;
+	for (int a = 0; a < na; a++)
+		if (receptor->type[a] != T_PolarHydrogen && heavy_nb[a].size() == 1)
+			ends.push_back(a);
+	std::stable_sort(ends.begin(), ends.end(), [&](int a, int b) { return norm(at(a) - c0) < norm(at(b) - c0); });
+	std::vector<char> taken(na, 0);
+	std::vector<std::vector<int>> chains; // [a(c+1), ..., a1, anchor]
+	for (int e : ends)
+	{
+		if (int(chains.size()) == nflex)
+			break;
+		const int c = g.i(min_chi, max_chi);
+		std::vector<int> path{e};
+		while (int(path.size()) < c + 2)
+		{
+			const int cur = path.back(), prev = path.size() > 1 ? path[path.size() - 2] : -1;
+			int next = -1;
+			for (int q : heavy_nb[cur])
+				if (q != prev)
+					next = q;
+			// a movable atom bonds to no heavy atom beyond the path
+			if (next < 0 || (path.size() > 1 && heavy_nb[cur].size() != 2))
+				break;
+			path.push_back(next);
+		}
+		if (int(path.size()) < c + 2)
+			continue;
+		bool free = true;
+		for (int a : path)
+			free = free && !taken[a];
+		if (!free)
+			continue;
+		for (int a : path)
+			taken[a] = 1;
+		chains.push_back(path);
+	}
+	if (int(chains.size()) < nflex)
+		return nullptr;
+	sgpu_mol* m = new sgpu_mol(*ligand);
+	std::vector<int> new_index(na, -1);
+	auto add_atom = [&](int a) {
+		new_index[a] = m->natoms();
+		m->type.push_back(receptor->type[a]);
+		V3 p = at(a);
+		m->xyz.insert(m->xyz.end(), {p.x, p.y, p.z});
+	};
+	struct Fr
+	{
+		int parent, begin, end, origin, axis;
+	};
+	std::vector<Fr> frames;
+	for (const auto& path : chains)
+	{
+		const int c = int(path.size()) - 2;
+		m->tree_frame.push_back(int(m->frame_parent.size()));
+		for (int k = 1; k <= c; k++)
+		{
+			const int ak = path[c + 1 - k];
+			const int first = m->natoms();
+			add_atom(ak);
+			for (int h : hyd[ak])
+				add_atom(h);
+			if (k == c)
+			{
+				add_atom(path[0]);
+				for (int h : hyd[path[0]])
+					add_atom(h);
+			}
+			const int f = int(m->frame_parent.size());
+			m->frame_parent.push_back(k == 1 ? -1 : f - 1);
+			m->frame_begin.push_back(first);
+			m->frame_end.push_back(m->natoms());
+			m->frame_origin_atom.push_back(first);
+			m->frame_axis_atom.push_back(-1 - path[c + 2 - k]); // receptor index, resolved below
+		}
+	}
+	m->nmovable = m->natoms();
+	for (const auto& path : chains)
+	{
+		const int anchor = path.back();
+		add_atom(anchor);
+		for (int h : hyd[anchor])
+			add_atom(h);
+	}
+	for (size_t f = ligand->frame_parent.size(); f < m->frame_axis_atom.size(); f++)
+		m->frame_axis_atom[f] = new_index[-1 - m->frame_axis_atom[f]];
+	for (size_t k = 0; k + 1 < receptor->bonds.size(); k += 2)
+	{
+		const int a = receptor->bonds[k], b = receptor->bonds[k + 1];
+		if (new_index[a] >= 0 && new_index[b] >= 0)
+			m->bonds.insert(m->bonds.end(), {new_index[a], new_index[b]});
+	}
+	m->nligands = ligand->nligands;
+	m->publish();
+	sgpu_mol* r = new sgpu_mol;
+	std::vector<int> rigid_index(na, -1);
+	for (int a = 0; a < na; a++)
+		if (new_index[a] < 0)
+		{
+			rigid_index[a] = r->natoms();
+			r->type.push_back(receptor->type[a]);
+			r->xyz.insert(r->xyz.end(), {receptor->xyz[3 * a], receptor->xyz[3 * a + 1], receptor->xyz[3 * a + 2]});
+		}
+	for (size_t k = 0; k + 1 < receptor->bonds.size(); k += 2)
+	{
+		const int a = rigid_index[receptor->bonds[k]], b = rigid_index[receptor->bonds[k + 1]];
+		if (a >= 0 && b >= 0)
+			r->bonds.insert(r->bonds.end(), {a, b});
+	}
+	r->publish();
+	*rigid = r;
+	return m;
+}
+
 } // extern "C"

@@ -153,6 +153,7 @@ def _load():
         "sgpu_mol_desc": (C.POINTER(ModelDesc), [vp]),
         "sgpu_synth_receptor": (vp, [C.c_uint32, C.c_int, C.c_double]),
         "sgpu_synth_ligand": (vp, [C.c_uint32, C.c_int, C.c_int, F64P]),
+        "sgpu_synth_flex": (vp, [vp, vp, C.c_uint32, C.c_int, C.c_int, C.c_int, F64P, C.POINTER(vp)]),
         "sgpu_vina_scoring": (None, [C.POINTER(Term), C.POINTER(Scoring), C.c_double]),
         "sgpu_build_tables": (C.c_int, [C.POINTER(Scoring), C.c_double, F64P, F64P]),
         "sgpu_flatten": (vp, [C.POINTER(ModelDesc)]),
@@ -231,6 +232,17 @@ class Mol:
     def ligand(seed, nheavy=24, ntors=6, center=(0.0, 0.0, 0.0)):
         c = f64(center)
         return Mol(lib.sgpu_synth_ligand(seed, nheavy, ntors, ptr(c, F64P)))
+
+    @staticmethod
+    def flex(receptor, ligand, seed, nflex=4, min_chi=2, max_chi=4, center=(0.0, 0.0, 0.0)):
+        """(model, rigid): `ligand` plus `nflex` flexible side chains cut from
+        `receptor`, and the receptor without them (sgpu_synth_flex)."""
+        c = f64(center)
+        rigid = C.c_void_p()
+        h = lib.sgpu_synth_flex(receptor.h, ligand.h, seed, nflex, min_chi, max_chi, ptr(c, F64P), C.byref(rigid))
+        if not h:
+            raise RuntimeError("receptor has too few chain ends for the side chains")
+        return Mol(h), Mol(rigid.value)
 
 
 def vina_scoring(hydrophobic_scale=1.0):

@@ -32,6 +32,16 @@ sgpu_mol* sgpu_synth_receptor(uint32_t seed, int nheavy, double pocket_radius);
  * N 15 / O 15 %, polar hydrogens on donors, placed at a random pose near
  * `center`. */
 sgpu_mol* sgpu_synth_ligand(uint32_t seed, int nheavy, int ntors, const double* center);
+/* A ligand plus nflex flexible side chains cut from `receptor`, in the atom
+ * and frame order of smina's flexible-residue parser
+ * (src/lib/parse_pdbqt.cpp:360-380): the ligand's trees; one tree per side
+ * chain, with min_chi..max_chi torsions, whose root is a first segment
+ * turning about the bond from its fixed anchor atom; then the anchors (the
+ * fixed "inflex" atoms).  The side chains are receptor chain ends nearest
+ * `center`.  *rigid receives the receptor without the side chains' atoms.
+ * Returns NULL if the receptor has too few chain ends. */
+sgpu_mol* sgpu_synth_flex(const sgpu_mol* receptor, const sgpu_mol* ligand, uint32_t seed, int nflex, int min_chi,
+		int max_chi, const double* center, sgpu_mol** rigid);
 
 /* ---- scoring */
 /* the default Vina terms (src/main/main.cpp:1347-1353); hydrophobic_scale

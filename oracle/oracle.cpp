@@ -2198,7 +2198,8 @@ double orc_refine_opts(orc_ctx* c, int model, const double* v, int maxiters, int
 }
 
 int orc_dock(orc_ctx* c, int model, const sgpu_mc_params* p, int seed, int exhaustiveness, int num_modes,
-		double energy_range, int threads, int max_out, double* e, double* conf, int64_t* evals)
+		double energy_range, int threads, double step_fraction, int max_out, double* e, double* conf,
+		int64_t* evals)
 {
 	const Model& proto = *c->c.models[model];
 	// setup_search (src/main/main.cpp:513-529)
@@ -2207,7 +2208,10 @@ int orc_dock(orc_ctx* c, int model, const sgpu_mc_params* p, int seed, int exhau
 	for (const Tree& tr : proto.trees)
 		dof += tr.ntors;
 	const int heuristic = N + 10 * dof;
-	const int num_steps = int(unsigned(70 * 3 * (50 + heuristic) / 2));
+	const int full_steps = int(unsigned(70 * 3 * (50 + heuristic) / 2));
+	// a shortened search (the device facade's step_fraction): the first
+	// ceil(fraction x steps) steps of every chain
+	const int num_steps = step_fraction >= 1 ? full_steps : int(std::ceil(full_steps * step_fraction));
 	const int maxiters = (25 + N) / 3;
 	// parallel_mc::operator() (src/lib/parallel_mc.cpp:80-98)
 	rng generator{uint32_t(seed)};

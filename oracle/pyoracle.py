@@ -52,8 +52,8 @@ def _load():
                                    I64P]),
         "orc_refine": (C.c_double, [vp, C.c_int, F64P, C.c_int, F64P, F64P]),
         "orc_refine_opts": (C.c_double, [vp, C.c_int, F64P, C.c_int, C.c_int, F64P, F64P]),
-        "orc_dock": (C.c_int, [vp, C.c_int, vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, F64P,
-                               F64P, I64P]),
+        "orc_dock": (C.c_int, [vp, C.c_int, vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double,
+                               C.c_int, F64P, F64P, I64P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -208,13 +208,14 @@ class Oracle:
         e = lib.orc_refine_opts(self.h, m, _p(vv, F64P), maxiters, int(accurate), _p(conf, F64P), _p(x, F64P))
         return e, conf, x
 
-    def dock(self, m, params, seed=1, exhaustiveness=8, num_modes=9, energy_range=3.0, threads=1, max_out=32):
+    def dock(self, m, params, seed=1, exhaustiveness=8, num_modes=9, energy_range=3.0, threads=1, max_out=32,
+             step_fraction=1.0):
         info = self.info(m)
         e = np.zeros(max_out)
         c = np.zeros((max_out, info["conf_size"]))
         ev = np.zeros(1, np.int64)
         k = lib.orc_dock(self.h, m, C.addressof(params), seed, exhaustiveness, num_modes, energy_range, threads,
-                         max_out, _p(e, F64P), _p(c, F64P), _p(ev, I64P))
+                         step_fraction, max_out, _p(e, F64P), _p(c, F64P), _p(ev, I64P))
         if k < 0:
             raise RuntimeError(self.error())
         return e[:k], c[:k], int(ev[0])

@@ -22,6 +22,13 @@ def ligand(seed=1, nheavy=24, ntors=6):
     return sg.Mol.ligand(seed, nheavy, ntors)
 
 
+@functools.lru_cache(maxsize=None)
+def flex_model(seed=7, nflex=4, min_chi=2, max_chi=4):
+    """(model, rigid receptor) of the flexible-receptor configuration: the
+    24-atom ligand plus four side chains with 2-4 chi torsions each"""
+    return sg.Mol.flex(receptor(), ligand(), seed, nflex, min_chi, max_chi)
+
+
 def heavy_types(*mols):
     ts = set()
     for m in mols:

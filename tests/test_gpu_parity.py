@@ -263,3 +263,20 @@ def test_minimize_fp64_matches_oracle():
         e, _, _ = o.refine(m, o.initial_conf(m), 10000, accurate=True)
         print(f"minimize: e {r['e']:.6f} rmsd {r['rmsd']:.3f}", flush=True)
         assert r["e"] == e, (r["e"], e)
+
+
+@pytest.mark.parametrize("lig,exh,frac", [((3, 10, 0), 8, 1.0), ((1, 24, 6), 4, 0.1)])
+def test_dock_fp64_matches_oracle(lig, exh, frac):
+    """the single-ligand docking end to end through the host facade
+    (GpuDocking::dock_batch: seeded chains, merge at 2 A, refinement,
+    finish_docking's FP64 rescoring, mode selection) in FP64, against the
+    oracle's dock (parallel_mc + refine_structure + finish_docking,
+    src/lib/parallel_mc.cpp:80-98, src/main/main.cpp:256-324); the second
+    case cuts every chain to a tenth of its steps on both sides"""
+    rec, l = receptor(), sg.Mol.ligand(*lig)
+    d = sg.Docking(rec, gd_c0(), fp64=True, exhaustiveness=exh, seed=5, step_fraction=frac)
+    r = d.dock([l])[0]
+    o, ids, _, _ = oracle_for([l])
+    e, c, ev = o.dock(ids[0], sg.mc_params(), seed=5, exhaustiveness=exh, threads=exh, step_fraction=frac)
+    print(f"dock {lig}: {r['e']}", flush=True)
+    assert len(r["e"]) > 0 and r["e"] == list(e)
