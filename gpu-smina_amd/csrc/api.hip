@@ -530,8 +530,12 @@ BatchPtrs<T> ptrs(const Batch& b)
 
 int plan_bytes(const WavePlan& p, int prec, int waves)
 {
-	return prec == SGPU_FP64 ? block_lds_bytes<double>(p, waves) : block_lds_bytes<float>(p, waves);
+	return prec == SGPU_FP64 ? block_lds_bytes<double>(p, waves)
+	       : prec == SGPU_MIXED ? block_lds_bytes<float, double>(p, waves)
+	                            : block_lds_bytes<float>(p, waves);
 }
+// the precision of a batch's model evaluation (and so of its maps)
+int eval_precision(int prec) { return prec == SGPU_FP64 ? SGPU_FP64 : SGPU_FP32; }
 constexpr int LDS_LIMIT = 160 * 1024;
 
 template<typename T>
@@ -982,16 +986,16 @@ sgpu_status sgpu_upload_models(sgpu_ctx* c, const sgpu_flat_model* models, int n
 {
 	return guarded(c, [&]() {
 		require(models && n > 0 && batch_id, "no models");
-		require(precision == SGPU_FP32 || precision == SGPU_FP64, "invalid precision");
+		require(precision == SGPU_FP32 || precision == SGPU_FP64 || precision == SGPU_MIXED, "invalid precision");
 		std::unique_ptr<Batch> b(new Batch);
 		b->precision = precision;
 		b->n = n;
 		pack_batch(models, n, b->h);
 		b->h.plan.bytes = plan_bytes(b->h.plan, precision, 1);
-		require(plan_bytes(b->h.plan, SGPU_FP64, 1) <= LDS_LIMIT && plan_bytes(b->h.plan, SGPU_FP32, 1) <= LDS_LIMIT,
+		require(plan_bytes(b->h.plan, SGPU_FP64, 1) <= LDS_LIMIT && b->h.plan.bytes <= LDS_LIMIT,
 				"model too large for one workgroup's LDS");
 		upload_batch(*b, c->stream);
-		if (precision == SGPU_FP32)
+		if (precision != SGPU_FP64)
 		{
 			b->fp64.reset(new Batch);
 			b->fp64->precision = SGPU_FP64;
@@ -1044,7 +1048,7 @@ void run_eval(sgpu_ctx* c, const sgpu_eval_request* q)
 	if (q->grid_id >= 0)
 	{
 		Grid& g = find_grid(c, q->grid_id);
-		require(g.precision == b.precision, "grid and model precision differ");
+		require(g.precision == eval_precision(b.precision), "grid and model precision differ");
 		check_maps(b, g, q->model, n);
 		ec.backend = 0;
 		ec.grid = grid_ptrs<T>(g, q->slope);
@@ -1099,13 +1103,13 @@ void run_eval(sgpu_ctx* c, const sgpu_eval_request* q)
 		throw Fail{SGPU_E_ATOM_OVERLAP, "Ligand atom exactly overlaps receptor atom.  I can't deal with this."};
 }
 
-template<typename T>
+template<typename T, typename S>
 void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, const sgpu_chains* ch,
 		sgpu_chain_results* out)
 {
 	Batch& b = find_batch(c, batch_id);
 	Grid& g = find_grid(c, grid_id);
-	require(g.precision == b.precision, "grid and model precision differ");
+	require(g.precision == eval_precision(b.precision), "grid and model precision differ");
 	require(p && ch && out && ch->nchains > 0, "mc request incomplete");
 	require(p->num_saved_mins >= 1 && p->num_saved_mins <= MAX_POSES, "num_saved_mins out of range");
 	require(out->max_poses >= p->num_saved_mins, "result arrays hold fewer poses than num_saved_mins");
@@ -1138,10 +1142,10 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 	};
 	L.off_mt = take(MT_N * sizeof(uint32_t));
 	L.off_ints = take((3 + MAX_POSES) * sizeof(int));
-	L.off_tmp = take(L.conf_stride * sizeof(T));
-	L.off_scal = take(2 * sizeof(T));
-	L.off_oute = take(MAX_POSES * sizeof(T));
-	L.off_outc = take(size_t(MAX_POSES) * L.conf_stride * sizeof(T));
+	L.off_tmp = take(L.conf_stride * sizeof(S));
+	L.off_scal = take(2 * sizeof(S));
+	L.off_oute = take(MAX_POSES * sizeof(S));
+	L.off_outc = take(size_t(MAX_POSES) * L.conf_stride * sizeof(S));
 	L.off_outx = take(size_t(MAX_POSES) * L.coord_stride * sizeof(T));
 	L.off_mc = take(L.atom_stride * sizeof(T));
 	L.off_ro = take(MAX_TREES * 3 * sizeof(T));
@@ -1158,12 +1162,12 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 		ints[0] = int(ch->rng[size_t(i) * 625 + 624]);
 		ints[1] = 0;
 		ints[2] = 0;
-		T* tmp = reinterpret_cast<T*>(base + L.off_tmp);
+		S* tmp = reinterpret_cast<S*>(base + L.off_tmp);
 		for (int k = 0; k < b.h.hdr[ch->model[i]].conf_size; k++)
-			tmp[k] = T(ch->conf[size_t(i) * ch->conf_stride + k]);
-		T* scal = reinterpret_cast<T*>(base + L.off_scal);
+			tmp[k] = S(ch->conf[size_t(i) * ch->conf_stride + k]);
+		S* scal = reinterpret_cast<S*>(base + L.off_scal);
 		scal[0] = 0;
-		scal[1] = Real_host_max<T>();
+		scal[1] = Real_host_max<S>();
 	}
 	// Launch plan.  Ligands fall into size classes by pair count; each class
 	// gets its own LDS plan (sized for its largest member) and runs on its own
@@ -1293,7 +1297,7 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 			side.push_back(st);
 			hip_check(hipStreamWaitEvent(st, c->ev0, 0), "stream wait");
 		}
-		hip_check(launch_mc<T>(ptrs<T>(b), ec, cl.plan, P, L, dchains.as<char>(), int(cl.blks.size()), cl.waves,
+		hip_check(launch_mc<T, S>(ptrs<T>(b), ec, cl.plan, P, L, dchains.as<char>(), int(cl.blks.size()), cl.waves,
 						  dbm.as<int>() + cl.block_off, dsc.as<int>() + cl.slot_off, dsteps.as<int>(), dmax.as<int>(),
 						  0, max_steps, st),
 				"mc kernel");
@@ -1312,8 +1316,8 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 		const ModelHdr& h = b.h.hdr[ch->model[i]];
 		const int* ints = reinterpret_cast<const int*>(base + L.off_ints);
 		const int cnt = std::min(ints[1], out->max_poses);
-		const T* oe = reinterpret_cast<const T*>(base + L.off_oute);
-		const T* oc = reinterpret_cast<const T*>(base + L.off_outc);
+		const S* oe = reinterpret_cast<const S*>(base + L.off_oute);
+		const S* oc = reinterpret_cast<const S*>(base + L.off_outc);
 		const T* ox = reinterpret_cast<const T*>(base + L.off_outx);
 		out->count[i] = cnt;
 		for (int k = 0; k < cnt; k++)
@@ -1331,7 +1335,7 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 	}
 }
 
-template<typename T>
+template<typename T, typename S>
 void run_refine(sgpu_ctx* c, const sgpu_refine_request* q)
 {
 	Batch& b = find_batch(c, q->batch_id);
@@ -1374,28 +1378,29 @@ void run_refine(sgpu_ctx* c, const sgpu_refine_request* q)
 	for (const ModelHdr& h : b.h.hdr)
 		hmax = std::max(hmax, h.nheavy);
 	const int xs = 3 * hmax;
-	std::vector<T> conf(size_t(n) * cs, T(0));
+	std::vector<S> conf(size_t(n) * cs, S(0));
 	for (int i = 0; i < n; i++)
 		for (int k = 0; k < b.h.hdr[q->model[i]].conf_size; k++)
-			conf[size_t(i) * cs + k] = T(q->conf[size_t(i) * q->conf_stride + k]);
+			conf[size_t(i) * cs + k] = S(q->conf[size_t(i) * q->conf_stride + k]);
 	DBuf dm, dc, dmi, de, dx, dev;
 	upload(dm, std::vector<int>(q->model, q->model + n), c->stream);
 	upload(dc, conf, c->stream);
 	upload(dmi, std::vector<int>(q->maxiters, q->maxiters + n), c->stream);
-	de.alloc(size_t(n) * sizeof(T));
+	de.alloc(size_t(n) * sizeof(S));
 	dx.alloc(size_t(n) * xs * sizeof(T));
 	dev.alloc(size_t(n) * sizeof(int64_t));
 	ensure_scratch(c->scratch, size_t(n) * ps * sizeof(T));
 	Timer tm(c, "refine");
-	hip_check(launch_refine<T>(ptrs<T>(b), ec, b.h.plan, n, dm.as<int>(), dc.as<T>(), cs, dmi.as<int>(),
-					  q->accurate_line_search, q->early_term, de.as<T>(), dx.as<T>(), xs, c->scratch.as<T>(), ps,
+	hip_check(launch_refine<T, S>(ptrs<T>(b), ec, b.h.plan, n, dm.as<int>(), dc.as<S>(), cs, dmi.as<int>(),
+					  q->accurate_line_search, q->early_term, de.as<S>(), dx.as<T>(), xs, c->scratch.as<T>(), ps,
 					  dev.as<int64_t>(), q->boxes ? dboxes.as<double>() : nullptr, c->stream),
 			"refine kernel");
 	tm.stop();
-	std::vector<T> e(n), x(size_t(n) * xs);
+	std::vector<S> e(n);
+	std::vector<T> x(size_t(n) * xs);
 	int flag = 0;
-	hip_check(hipMemcpyAsync(conf.data(), dc.p, conf.size() * sizeof(T), hipMemcpyDeviceToHost, c->stream), "download");
-	hip_check(hipMemcpyAsync(e.data(), de.p, e.size() * sizeof(T), hipMemcpyDeviceToHost, c->stream), "download");
+	hip_check(hipMemcpyAsync(conf.data(), dc.p, conf.size() * sizeof(S), hipMemcpyDeviceToHost, c->stream), "download");
+	hip_check(hipMemcpyAsync(e.data(), de.p, e.size() * sizeof(S), hipMemcpyDeviceToHost, c->stream), "download");
 	hip_check(hipMemcpyAsync(x.data(), dx.p, x.size() * sizeof(T), hipMemcpyDeviceToHost, c->stream), "download");
 	hip_check(hipMemcpyAsync(&flag, c->err_flag.p, sizeof(int), hipMemcpyDeviceToHost, c->stream), "download");
 	hip_check(hipStreamSynchronize(c->stream), "refine");
@@ -1404,7 +1409,7 @@ void run_refine(sgpu_ctx* c, const sgpu_refine_request* q)
 		const ModelHdr& h = b.h.hdr[q->model[i]];
 		for (int k = 0; k < h.conf_size; k++)
 			q->conf[size_t(i) * q->conf_stride + k] = double(conf[size_t(i) * cs + k]);
-		q->e[i] = e[i] >= Real_host_max<T>() ? DBL_MAX : double(e[i]);
+		q->e[i] = e[i] >= Real_host_max<S>() ? DBL_MAX : double(e[i]);
 		if (q->coords)
 			for (int k = 0; k < 3 * h.nheavy; k++)
 				q->coords[size_t(i) * q->coord_stride + k] = double(x[size_t(i) * xs + k]);
@@ -1477,9 +1482,11 @@ sgpu_status sgpu_run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_pa
 		require(c->tab.ok, "tables not uploaded");
 		Batch& b = find_batch(c, batch_id);
 		if (b.precision == SGPU_FP64)
-			run_mc<double>(c, batch_id, grid_id, p, ch, out);
+			run_mc<double, double>(c, batch_id, grid_id, p, ch, out);
+		else if (b.precision == SGPU_MIXED)
+			run_mc<float, double>(c, batch_id, grid_id, p, ch, out);
 		else
-			run_mc<float>(c, batch_id, grid_id, p, ch, out);
+			run_mc<float, float>(c, batch_id, grid_id, p, ch, out);
 	});
 }
 
@@ -1490,9 +1497,11 @@ sgpu_status sgpu_run_refine(sgpu_ctx* c, const sgpu_refine_request* q)
 		require(c->tab.ok, "tables not uploaded");
 		Batch& b = find_batch(c, q->batch_id);
 		if (b.precision == SGPU_FP64)
-			run_refine<double>(c, q);
+			run_refine<double, double>(c, q);
+		else if (b.precision == SGPU_MIXED)
+			run_refine<float, double>(c, q);
 		else
-			run_refine<float>(c, q);
+			run_refine<float, float>(c, q);
 	});
 }
 

@@ -8,6 +8,12 @@
  * Hessian lives in LDS as the full symmetric matrix.  Scalars are
  * wave-uniform.  With T = double every dot product is a serial sum in
  * index order, as on the CPU.
+ *
+ * Two real types: T for the model evaluation (coordinates, maps, pair
+ * tables, forces) and S for the search state (conf, gradient and direction
+ * vectors, Hessian, line search, energies the chain compares).  The served
+ * search uses T = S = float, the oracle-parity search T = S = double, and
+ * the mixed search T = float with S = double.
  */
 #ifndef SGPU_CHAIN_H
 #define SGPU_CHAIN_H
@@ -108,53 +114,53 @@ __device__ inline void bfgs_update(T* h, T* sp, T* smhy, T p, T y, T alpha, int 
 
 // conf::increment (src/lib/conf.h:316-321) of the staged conf by
 // alpha * staged change; returns this lane's new conf element
-template<typename T>
-__device__ inline T conf_increment(const MV<T>& m, const WaveMem<T>& W, T alpha)
+template<typename T, typename S>
+__device__ inline S conf_increment(const MV<T>& m, const SearchMem<S>& W, S alpha)
 {
 	const int k = lane_id();
 	if (k >= m.h.conf_size)
-		return T(0);
+		return S(0);
 	const int kind = m.ckind[k];
 	const int gi = m.cchg[k];
-	const T x = W.xstage[k];
+	const S x = W.xstage[k];
 	if (kind == CK_POSITION)
 	{
-		const T d = alpha * W.pstage[gi];
+		const S d = alpha * W.pstage[gi];
 		return x + d;
 	}
 	if (kind == CK_TORSION)
 	{
-		T t = x + normalized_angle(alpha * W.pstage[gi]);
+		S t = x + normalized_angle(alpha * W.pstage[gi]);
 		normalize_angle(t);
 		return t;
 	}
 	const int comp = kind - CK_QUAT0;
 	const int q0 = k - comp;
-	const V3<T> rot = v3(alpha * W.pstage[gi], alpha * W.pstage[gi + 1], alpha * W.pstage[gi + 2]);
-	const Q4<T> q = {W.xstage[q0], W.xstage[q0 + 1], W.xstage[q0 + 2], W.xstage[q0 + 3]};
-	const Q4<T> r = q_increment(q, rot);
+	const V3<S> rot = v3(alpha * W.pstage[gi], alpha * W.pstage[gi + 1], alpha * W.pstage[gi + 2]);
+	const Q4<S> q = {W.xstage[q0], W.xstage[q0 + 1], W.xstage[q0 + 2], W.xstage[q0 + 3]};
+	const Q4<S> r = q_increment(q, rot);
 	return comp == 0 ? r.a : comp == 1 ? r.b : comp == 2 ? r.c : r.d;
 }
 
 // conf::operator()(index) (src/lib/conf.h:376-397) for the staged conf
-template<typename T>
-__device__ inline T conf_value(const MV<T>& m, const WaveMem<T>& W)
+template<typename T, typename S>
+__device__ inline S conf_value(const MV<T>& m, const SearchMem<S>& W)
 {
 	const int i = lane_id();
 	if (i >= m.h.change_size)
-		return T(0);
+		return S(0);
 	const int kind = m.gkind[i];
 	const int ci = m.gconf[i];
 	if (kind == GK_POSITION || kind == GK_TORSION)
 		return W.xstage[ci];
-	const Q4<T> q = {W.xstage[ci], W.xstage[ci + 1], W.xstage[ci + 2], W.xstage[ci + 3]};
-	const V3<T> a = q_to_angle(q);
+	const Q4<S> q = {W.xstage[ci], W.xstage[ci + 1], W.xstage[ci + 2], W.xstage[ci + 3]};
+	const V3<S> a = q_to_angle(q);
 	const int comp = kind - GK_ORIENT0;
 	return comp == 0 ? a.x : comp == 1 ? a.y : a.z;
 }
 
-template<typename T>
-__device__ inline void stage(WaveMem<T>& W, T x, T p, int c, int n)
+template<typename S>
+__device__ inline void stage(SearchMem<S>& W, S x, S p, int c, int n)
 {
 	const int lane = lane_id();
 	if (lane < c)
@@ -171,26 +177,27 @@ __device__ inline void stage(WaveMem<T>& W, T x, T p, int c, int n)
 // point, then the line search's trials; a finished line search runs the
 // BFGS step body and starts the next line search.  The order of every
 // evaluation and every arithmetic operation is the CPU's.
-template<typename T, int BK>
+template<typename T, typename S, int BK>
 struct Searcher
 {
 	const MV<T>& m;
 	const EvalCtx<T>& ec;
 	WaveMem<T>& W;
+	SearchMem<S>& SM;
 	T* pbuf;
 	int64_t evals;
 
 	// the staged conf x plus alpha * p, this lane's element
-	__device__ __forceinline__ T trial_conf(T x, T p, T alpha)
+	__device__ __forceinline__ S trial_conf(S x, S p, S alpha)
 	{
-		stage(W, x, p, m.h.conf_size, m.h.change_size);
-		const T xn = conf_increment(m, W, alpha);
+		stage(SM, x, p, m.h.conf_size, m.h.change_size);
+		const S xn = conf_increment(m, SM, alpha);
 		lsync();
 		return xn;
 	}
 
 	// x, g: this lane's elements; returns the final energy
-	__device__ __forceinline__ T bfgs(T& x, T& g, int maxiters, bool accurate, bool early_term)
+	__device__ __forceinline__ S bfgs(S& x, S& g, int maxiters, bool accurate, bool early_term)
 	{
 		const int n = m.h.change_size, c = m.h.conf_size;
 		const int lane = lane_id();
@@ -200,21 +207,22 @@ struct Searcher
 		{
 			const int row = r0 + hm.q;
 			if (hm.q < hm.rows && row < n)
-				W.hess[row * n + hm.c] = row == hm.c ? T(1) : T(0);
+				SM.hess[row * n + hm.c] = row == hm.c ? S(1) : S(0);
 		}
 		lsync();
-		T f0 = 0, f_orig = 0, g_orig = 0, x_orig = x;
-		T p = 0, x_new = x, g_new = 0, f1 = 0;
-		T alpha = 1, slope = 0;   // slope: p.g (fast) or g.p (accurate)
-		T alamin = 0, alpha2 = 0, f2 = 0;
+		S f0 = 0, f_orig = 0, g_orig = 0, x_orig = x;
+		S p = 0, x_new = x, g_new = 0, f1 = 0;
+		S alpha = 1, slope = 0;   // slope: p.g (fast) or g.p (accurate)
+		S alamin = 0, alpha2 = 0, f2 = 0;
 		int trial = 0, step = 0;
 		bool initial = true;
-		T xe = x; // the conf evaluated next
+		S xe = x; // the conf evaluated next
 		for (;;)
 		{
-			T ge;
 			++evals;
-			const T fe = eval_deriv<T, BK>(m, ec, W, pbuf, xe, ge);
+			T ge_t;
+			const S fe = S(eval_deriv<T, BK>(m, ec, W, pbuf, T(xe), ge_t));
+			const S ge = S(ge_t);
 			bool begin = false;
 			if (initial)
 			{
@@ -239,17 +247,17 @@ struct Searcher
 				bool done;
 				if (!accurate)
 				{
-					if (f1 - f0 < T(0.0001) * alpha * slope)
+					if (f1 - f0 < S(0.0001) * alpha * slope)
 						done = true;
 					else
 					{
-						alpha *= T(0.5);
+						alpha *= S(0.5);
 						done = ++trial >= 10;
 					}
 				}
 				else
 				{
-					const T ALF = T(1.0e-4), FIRST = T(1.0);
+					const S ALF = S(1.0e-4), FIRST = S(1.0);
 					if (alpha < alamin)
 					{
 						x_new = x;
@@ -261,34 +269,34 @@ struct Searcher
 						done = true;
 					else
 					{
-						T tmplam;
+						S tmplam;
 						if (alpha == FIRST)
-							tmplam = -slope / (T(2.0) * (f1 - f0 - slope));
+							tmplam = -slope / (S(2.0) * (f1 - f0 - slope));
 						else
 						{
-							const T rhs1 = f1 - f0 - alpha * slope;
-							const T rhs2 = f2 - f0 - alpha2 * slope;
-							const T a = (rhs1 / (alpha * alpha) - rhs2 / (alpha2 * alpha2)) / (alpha - alpha2);
-							const T b = (-alpha2 * rhs1 / (alpha * alpha) + alpha * rhs2 / (alpha2 * alpha2)) /
+							const S rhs1 = f1 - f0 - alpha * slope;
+							const S rhs2 = f2 - f0 - alpha2 * slope;
+							const S a = (rhs1 / (alpha * alpha) - rhs2 / (alpha2 * alpha2)) / (alpha - alpha2);
+							const S b = (-alpha2 * rhs1 / (alpha * alpha) + alpha * rhs2 / (alpha2 * alpha2)) /
 									(alpha - alpha2);
-							if (a == T(0.0))
-								tmplam = -slope / (T(2.0) * b);
+							if (a == S(0.0))
+								tmplam = -slope / (S(2.0) * b);
 							else
 							{
-								const T disc = b * b - T(3.0) * a * slope;
+								const S disc = b * b - S(3.0) * a * slope;
 								if (disc < 0)
-									tmplam = T(0.5) * alpha;
+									tmplam = S(0.5) * alpha;
 								else if (b <= 0)
-									tmplam = (-b + Real<T>::sqrt(disc)) / (T(3.0) * a);
+									tmplam = (-b + Real<S>::sqrt(disc)) / (S(3.0) * a);
 								else
-									tmplam = -slope / (b + Real<T>::sqrt(disc));
+									tmplam = -slope / (b + Real<S>::sqrt(disc));
 							}
-							if (tmplam > T(.5) * alpha)
-								tmplam = T(.5) * alpha;
+							if (tmplam > S(.5) * alpha)
+								tmplam = S(.5) * alpha;
 						}
 						alpha2 = alpha;
 						f2 = f1;
-						alpha = fmax(tmplam, T(0.1) * alpha);
+						alpha = fmax(tmplam, S(0.1) * alpha);
 						// every backtrack at least halves alpha, so a finite
 						// search ends within ~1100 trials; the cap only stops
 						// a NaN search
@@ -311,32 +319,32 @@ struct Searcher
 				// the step body after the line search
 				if (alpha == 0)
 					break;
-				const T y = g_new - g;
-				const T prevf0 = f0;
+				const S y = g_new - g;
+				const S prevf0 = f0;
 				f0 = f1;
 				x = x_new;
 				if (early_term)
 				{
-					const T diff = prevf0 - f0;
-					if (Real<T>::abs(diff) < T(1e-5))
+					const S diff = prevf0 - f0;
+					if (Real<S>::abs(diff) < S(1e-5))
 						break;
 				}
 				g = g_new;
-				const T gradnormsq = dot_n(g, g, n);
-				if (!(gradnormsq >= T(1e-4)))
+				const S gradnormsq = dot_n(g, g, n);
+				if (!(gradnormsq >= S(1e-4)))
 					break;
 				if (step == 0)
 				{
-					const T yy = dot_n(y, y, n);
-					if (Real<T>::abs(yy) > Real<T>::eps())
+					const S yy = dot_n(y, y, n);
+					if (Real<S>::abs(yy) > Real<S>::eps())
 					{
-						const T diag = alpha * dot_n(y, p, n) / yy;
+						const S diag = alpha * dot_n(y, p, n) / yy;
 						if (lane < n)
-							W.hess[lane * n + lane] = diag;
+							SM.hess[lane * n + lane] = diag;
 						lsync();
 					}
 				}
-				bfgs_update(W.hess, W.pstage, W.gbuf, p, y, alpha, n, hm);
+				bfgs_update(SM.hess, SM.pstage, SM.smhy, p, y, alpha, n, hm);
 				begin = ++step < maxiters;
 				if (!begin)
 					break;
@@ -344,7 +352,7 @@ struct Searcher
 			if (begin)
 			{
 				// the next step's direction and the start of its line search
-				p = minus_mat_vec(W.hess, g, n);
+				p = minus_mat_vec(SM.hess, g, n);
 				trial = 0;
 				alpha = 1;
 				if (!accurate)
@@ -355,17 +363,17 @@ struct Searcher
 					slope = dot_n(g, p, n);
 					if (slope >= 0)
 						break; // alpha 0
-					stage(W, x, p, c, n);
-					const T xv = conf_value(m, W);
-					const T temp = lane < n ? Real<T>::abs(p) / fmax(Real<T>::abs(xv), T(1.0)) : T(0);
-					T test = 0;
+					stage(SM, x, p, c, n);
+					const S xv = conf_value(m, SM);
+					const S temp = lane < n ? Real<S>::abs(p) / fmax(Real<S>::abs(xv), S(1.0)) : S(0);
+					S test = 0;
 					for (int i = 0; i < n; i++)
 					{
-						const T ti = readlane(temp, i);
+						const S ti = readlane(temp, i);
 						if (ti > test)
 							test = ti;
 					}
-					alamin = Real<T>::eps() / test;
+					alamin = Real<S>::eps() / test;
 					alpha2 = 0;
 					f2 = 0;
 				}
@@ -458,16 +466,17 @@ __device__ inline V3<double> random_inside_sphere(G& g)
 	}
 }
 
-// per-chain state in global memory
-template<typename T>
+// per-chain state in global memory: confs and energies in the search
+// precision S, coordinates in the evaluation precision T
+template<typename T, typename S>
 struct ChainState
 {
 	uint32_t* mt;     // 624 words
 	int* ints;        // [0] mt position, [1] pose count, [2] started, [3..3+MAX_POSES) order
-	T* tmp;           // current conf (conf stride)
-	T* scal;          // [0] current e, [1] best e
-	T* out_e;         // [MAX_POSES]
-	T* out_conf;      // [MAX_POSES][conf stride]
+	S* tmp;           // current conf (conf stride)
+	S* scal;          // [0] current e, [1] best e
+	S* out_e;         // [MAX_POSES]
+	S* out_conf;      // [MAX_POSES][conf stride]
 	T* out_coords;    // [MAX_POSES][coord stride]
 	T* model_coords;  // the model's coordinates (3 * movable) between launches
 	T* root_origin;   // [MAX_TREES * 3]
@@ -486,17 +495,17 @@ struct ChainLayout
 	size_t off_mt, off_ints, off_tmp, off_scal, off_oute, off_outc, off_outx, off_mc, off_ro, off_pbuf, off_ev;
 };
 
-template<typename T>
-__device__ inline ChainState<T> chain_state(char* base, const ChainLayout& L, int c)
+template<typename T, typename S>
+__device__ inline ChainState<T, S> chain_state(char* base, const ChainLayout& L, int c)
 {
 	char* p = base + size_t(c) * L.bytes;
-	ChainState<T> s;
+	ChainState<T, S> s;
 	s.mt = reinterpret_cast<uint32_t*>(p + L.off_mt);
 	s.ints = reinterpret_cast<int*>(p + L.off_ints);
-	s.tmp = reinterpret_cast<T*>(p + L.off_tmp);
-	s.scal = reinterpret_cast<T*>(p + L.off_scal);
-	s.out_e = reinterpret_cast<T*>(p + L.off_oute);
-	s.out_conf = reinterpret_cast<T*>(p + L.off_outc);
+	s.tmp = reinterpret_cast<S*>(p + L.off_tmp);
+	s.scal = reinterpret_cast<S*>(p + L.off_scal);
+	s.out_e = reinterpret_cast<S*>(p + L.off_oute);
+	s.out_conf = reinterpret_cast<S*>(p + L.off_outc);
 	s.out_coords = reinterpret_cast<T*>(p + L.off_outx);
 	s.model_coords = reinterpret_cast<T*>(p + L.off_mc);
 	s.root_origin = reinterpret_cast<T*>(p + L.off_ro);
@@ -554,8 +563,8 @@ __device__ inline double gyration_radius(const MV<T>& m, const WaveMem<T>& W, in
 
 // mutate_conf (src/lib/mutate.cpp:35-63): one of position, orientation or a
 // torsion; returns this lane's conf element
-template<typename T, typename G>
-__device__ inline T mutate(const MV<T>& m, const WaveMem<T>& W, T x, double amplitude, G& g)
+template<typename T, typename S, typename G>
+__device__ inline S mutate(const MV<T>& m, const WaveMem<T>& W, S x, double amplitude, G& g)
 {
 	const int lane = lane_id();
 	int count = 0;
@@ -574,11 +583,11 @@ __device__ inline T mutate(const MV<T>& m, const WaveMem<T>& W, T x, double ampl
 			const V3<double> s = random_inside_sphere(g);
 			const V3<double> d = amplitude * s;
 			if (lane == cc)
-				x += T(d.x);
+				x += S(d.x);
 			if (lane == cc + 1)
-				x += T(d.y);
+				x += S(d.y);
 			if (lane == cc + 2)
-				x += T(d.z);
+				x += S(d.z);
 			return x;
 		}
 		--which;
@@ -589,9 +598,9 @@ __device__ inline T mutate(const MV<T>& m, const WaveMem<T>& W, T x, double ampl
 			{
 				const V3<double> s = random_inside_sphere(g);
 				const double k = amplitude / gr;
-				const V3<T> rot = v3(T(k * s.x), T(k * s.y), T(k * s.z));
-				const Q4<T> q = {readlane(x, cc + 3), readlane(x, cc + 4), readlane(x, cc + 5), readlane(x, cc + 6)};
-				const Q4<T> r = q_increment(q, rot);
+				const V3<S> rot = v3(S(k * s.x), S(k * s.y), S(k * s.z));
+				const Q4<S> q = {readlane(x, cc + 3), readlane(x, cc + 4), readlane(x, cc + 5), readlane(x, cc + 6)};
+				const Q4<S> r = q_increment(q, rot);
 				if (lane == cc + 3)
 					x = r.a;
 				if (lane == cc + 4)
@@ -608,7 +617,7 @@ __device__ inline T mutate(const MV<T>& m, const WaveMem<T>& W, T x, double ampl
 		{
 			const double v = random_fl(-3.1415926535897931, 3.1415926535897931, g);
 			if (lane == cc + 7 + which)
-				x = T(v);
+				x = S(v);
 			return x;
 		}
 		which -= m.tntors[t];
@@ -621,7 +630,7 @@ __device__ inline T mutate(const MV<T>& m, const WaveMem<T>& W, T x, double ampl
 		{
 			const double v = random_fl(-3.1415926535897931, 3.1415926535897931, g);
 			if (lane == m.tconf[t] + which)
-				x = T(v);
+				x = S(v);
 			return x;
 		}
 		which -= m.tntors[t];
@@ -641,9 +650,9 @@ __device__ inline bool metropolis_accept(double old_f, double new_f, double temp
 
 // add_to_output_container (src/lib/coords.cpp:34-58) with find_closest's
 // rmsd_upper_bound over the heavy movable atoms, then a stable sort
-template<typename T>
-__device__ inline void add_to_output(const MV<T>& m, const WaveMem<T>& W, ChainState<T>& s, const ChainLayout& L,
-		T x, T e, double min_rmsd, int max_size, int& count)
+template<typename T, typename S>
+__device__ inline void add_to_output(const MV<T>& m, const WaveMem<T>& W, ChainState<T, S>& s, const ChainLayout& L,
+		S x, S e, double min_rmsd, int max_size, int& count)
 {
 	const int lane = lane_id();
 	const int H = m.h.nheavy;
@@ -707,7 +716,7 @@ __device__ inline void add_to_output(const MV<T>& m, const WaveMem<T>& W, ChainS
 			for (int a = 1; a < count; a++)
 			{
 				const int v = order[a];
-				const T ev = s.out_e[v];
+				const S ev = s.out_e[v];
 				int b = a - 1;
 				while (b >= 0 && s.out_e[order[b]] > ev)
 				{

@@ -1,5 +1,6 @@
 // Host-side declarations of the kernel launches (one definition per
-// precision in kernels_f32.hip / kernels_f64.hip; grid build in grid.hip).
+// precision in kernels_f32.hip / kernels_f64.hip / kernels_mixed.hip; grid
+// build in grid.hip).
 #ifndef SGPU_LAUNCH_H
 #define SGPU_LAUNCH_H
 
@@ -18,14 +19,16 @@ template<typename T>
 hipError_t launch_eval(const BatchPtrs<T>& B, const EvalCtx<T>& ec, const WavePlan& plan, int n, const int* model,
 		const T* conf, int conf_stride, T* e, T* g, int g_stride, T* x, int x_stride, T* scratch, int pstride,
 		hipStream_t st);
-// nblocks workgroups of `waves` (1..8) chains of one ligand each
-template<typename T>
+// nblocks workgroups of `waves` (1..8) chains of one ligand each; T is the
+// evaluation's real type, S the search state's (kernels_mixed.hip: float,
+// double)
+template<typename T, typename S>
 hipError_t launch_mc(const BatchPtrs<T>& B, const EvalCtx<T>& ec, const WavePlan& plan, const McParams& P,
 		const ChainLayout& L, char* chains, int nblocks, int waves, const int* block_model, const int* slot_chain,
 		const int* chain_steps, const int* chain_maxiters, int step_lo, int step_hi, hipStream_t st);
-template<typename T>
+template<typename T, typename S>
 hipError_t launch_refine(const BatchPtrs<T>& B, const EvalCtx<T>& ec, const WavePlan& plan, int n, const int* model,
-		T* conf, int conf_stride, const int* maxiters, int accurate, int early_term, T* e, T* x, int x_stride,
+		S* conf, int conf_stride, const int* maxiters, int accurate, int early_term, S* e, T* x, int x_stride,
 		T* scratch, int pstride, int64_t* evals, const double* boxes, hipStream_t st);
 template<typename T>
 hipError_t launch_score(const BatchPtrs<T>& B, const EvalCtx<T>& ec, const WavePlan& plan, int mode, int n,

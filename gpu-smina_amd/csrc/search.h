@@ -78,16 +78,39 @@ struct WaveMem
 	T* gbuf;
 	T* ebuf;
 	T* xstage;
-	T* pstage;
-	T* hess;
 	T* pforce; // pair forces (pair-parallel search only)
 };
 
-template<typename T>
-__host__ __device__ inline int wave_lds_elems(const WavePlan& p)
+// The search's own vectors in the search precision S: the staged conf and
+// direction of a line-search trial, the Hessian and the BFGS update's
+// scratch.  With S = T the conf stage and the scratch share the evaluation's
+// buffers (xstage, gbuf).
+template<typename S>
+struct SearchMem
 {
-	return 3 * p.amax + 3 * p.nmax + p.fmax * (3 + 9 + 4 + 3 + 3 + 3) + p.gmax + p.nmax + p.cmax + p.gmax +
-			p.hmax + (pair_parallel<T>() ? 3 * p.pmax : 0);
+	S* xstage;
+	S* pstage;
+	S* hess;
+	S* smhy;
+};
+
+// Per-wave LDS: the evaluation's region in T, then the search's in S
+template<typename T>
+__host__ __device__ inline int eval_lds_elems(const WavePlan& p)
+{
+	return 3 * p.amax + 3 * p.nmax + p.fmax * (3 + 9 + 4 + 3 + 3 + 3) + p.gmax + p.nmax + p.cmax +
+			(pair_parallel<T>() ? 3 * p.pmax : 0);
+}
+template<typename T, typename S>
+__host__ __device__ inline int search_lds_elems(const WavePlan& p)
+{
+	return p.gmax + p.hmax + (sizeof(S) != sizeof(T) ? p.cmax + p.gmax : 0);
+}
+template<typename T, typename S>
+__host__ __device__ inline int wave_lds_bytes(const WavePlan& p)
+{
+	const int eb = (eval_lds_elems<T>(p) * int(sizeof(T)) + 15) / 16 * 16;
+	return eb + (search_lds_elems<T, S>(p) * int(sizeof(S)) + 15) / 16 * 16;
 }
 // model region in bytes, rounded to 16
 template<typename T>
@@ -96,17 +119,17 @@ __host__ __device__ inline int model_lds_bytes(const WavePlan& p)
 	const int ints = (p.imax + p.amax + 3) / 4 * 4;
 	return ints * 4 + (p.tmax * int(sizeof(T)) + 15) / 16 * 16;
 }
-template<typename T>
+template<typename T, typename S = T>
 __host__ __device__ inline int block_lds_bytes(const WavePlan& p, int waves)
 {
-	return model_lds_bytes<T>(p) + waves * wave_lds_elems<T>(p) * int(sizeof(T));
+	return model_lds_bytes<T>(p) + waves * wave_lds_bytes<T, S>(p);
 }
 
-template<typename T>
-__device__ inline WaveMem<T> carve(T* base, const WavePlan& p)
+// one wave's LDS (16-byte aligned) as its evaluation and search buffers
+template<typename T, typename S>
+__device__ inline void carve(unsigned char* base, const WavePlan& p, WaveMem<T>& w, SearchMem<S>& sm)
 {
-	WaveMem<T> w;
-	T* q = base;
+	T* q = reinterpret_cast<T*>(base);
 	w.coords = q; // movable atoms, then the fixed atoms (copied once)
 	q += 3 * p.amax;
 	w.forces = q;
@@ -129,12 +152,23 @@ __device__ inline WaveMem<T> carve(T* base, const WavePlan& p)
 	q += p.nmax;
 	w.xstage = q;
 	q += p.cmax;
-	w.pstage = q;
-	q += p.gmax;
-	w.hess = q;
-	q += p.hmax;
 	w.pforce = q;
-	return w;
+	S* r = reinterpret_cast<S*>(base + (eval_lds_elems<T>(p) * int(sizeof(T)) + 15) / 16 * 16);
+	sm.pstage = r;
+	r += p.gmax;
+	sm.hess = r;
+	r += p.hmax;
+	if constexpr (sizeof(S) == sizeof(T))
+	{
+		sm.xstage = reinterpret_cast<S*>(w.xstage);
+		sm.smhy = reinterpret_cast<S*>(w.gbuf);
+	}
+	else
+	{
+		sm.xstage = r;
+		r += p.cmax;
+		sm.smhy = r;
+	}
 }
 
 // receptor back ends: precomputed maps (smina's cache) or the receptor

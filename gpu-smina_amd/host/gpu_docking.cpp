@@ -625,7 +625,7 @@ size_t GpuDocking::map_bytes() const
 	for (int d = 0; d < 3; d++)
 		cells *= size_t(gd_.n[d]);
 	const size_t points = size_t(gd_.n[0] + 1) * (gd_.n[1] + 1) * (gd_.n[2] + 1);
-	return grid_types_.size() * (cells * 8 * (s_.fp64 ? 8 : 4) + points * 8);
+	return grid_types_.size() * (cells * 8 * (s_.precision == SGPU_FP64 ? 8 : 4) + points * 8);
 }
 
 std::string GpuDocking::description() const
@@ -633,7 +633,7 @@ std::string GpuDocking::description() const
 	char name[256] = {0};
 	int cus = 0;
 	sgpu_device_info(ctx_, name, sizeof(name), &cus);
-	return std::string(name) + ", " + std::to_string(cus) + " CUs, " + (s_.fp64 ? "FP64" : "FP32");
+	return std::string(name) + ", " + std::to_string(cus) + " CUs, " + (s_.precision == SGPU_FP64 ? "FP64" : s_.precision == SGPU_MIXED ? "FP32 evaluation, FP64 search" : "FP32");
 }
 
 void GpuDocking::ensure_grids(const std::vector<int>& types)
@@ -650,7 +650,7 @@ void GpuDocking::ensure_grids(const std::vector<int>& types)
 	std::sort(all.begin(), all.end());
 	if (grid_ >= 0)
 		sgpu_release_grid(ctx_, grid_);
-	check(ctx_, sgpu_build_grids(ctx_, receptor_, &gd_, all.data(), int(all.size()), s_.fp64 ? SGPU_FP64 : SGPU_FP32,
+	check(ctx_, sgpu_build_grids(ctx_, receptor_, &gd_, all.data(), int(all.size()), s_.precision == SGPU_FP64 ? SGPU_FP64 : SGPU_FP32,
 						&grid_));
 	grid_types_ = all;
 }
@@ -810,7 +810,7 @@ void GpuDocking::dock_sub(DockJob* const* jobs, size_t nj)
 	if (nj == 0)
 		return;
 	const auto t0 = std::chrono::steady_clock::now();
-	const int prec = s_.fp64 ? SGPU_FP64 : SGPU_FP32;
+	const int prec = s_.precision;
 	std::vector<FlatModel> fms;
 	std::vector<int> types;
 	for (size_t j = 0; j < nj; j++)
@@ -1115,7 +1115,9 @@ sgpu_docking* sgpu_docking_create(const sgpu_dock_settings* s, const sgpu_mol* r
 	{
 		GpuSettings gs;
 		gs.device = s->device;
-		gs.fp64 = s->fp64 != 0;
+		if (s->precision != SGPU_FP32 && s->precision != SGPU_FP64 && s->precision != SGPU_MIXED)
+			throw std::invalid_argument("precision must be SGPU_FP32, SGPU_FP64 or SGPU_MIXED");
+		gs.precision = s->precision;
 		gs.exhaustiveness = s->exhaustiveness;
 		gs.seed = s->seed;
 		gs.num_modes = s->num_modes;

@@ -79,7 +79,7 @@ void add_to_output_container(std::vector<Pose>& out, const Pose& t, double min_r
 struct GpuSettings
 {
 	int device = 0;
-	bool fp64 = false;
+	int precision = SGPU_FP32; // SGPU_FP32, SGPU_FP64 (--gpu_fp64) or SGPU_MIXED
 	int exhaustiveness = 8;
 	int seed = 1;
 	int num_modes = 9;

@@ -229,7 +229,9 @@ sgpu_server* sgpu_server_create(const sgpu_server_settings* s)
 		ss.max_batch = s->max_batch;
 		ss.budget_bytes = s->budget_gb * 1e9;
 		ss.dock.device = s->device;
-		ss.dock.fp64 = s->fp64 != 0;
+		if (s->precision != SGPU_FP32 && s->precision != SGPU_FP64 && s->precision != SGPU_MIXED)
+			throw std::invalid_argument("precision must be SGPU_FP32, SGPU_FP64 or SGPU_MIXED");
+		ss.dock.precision = s->precision;
 		ss.dock.seed = s->seed;
 		ss.dock.num_modes = s->num_modes;
 		ss.dock.energy_range = s->energy_range;

@@ -13,7 +13,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsgpu.so")
 
-FP32, FP64 = 0, 1
+FP32, FP64, MIXED = 0, 1, 2  # sgpu_precision
 OK = 0
 STATUS = {0: "OK", 1: "INVALID", 2: "HIP", 3: "OOM", 4: "ATOM_OVERLAP", 5: "NO_DEVICE", 6: "NOT_FOUND"}
 
@@ -102,14 +102,14 @@ class Scoring(C.Structure):
 
 
 class DockSettings(C.Structure):
-    _fields_ = [("device", C.c_int), ("fp64", C.c_int), ("exhaustiveness", C.c_int), ("seed", C.c_int),
+    _fields_ = [("device", C.c_int), ("precision", C.c_int), ("exhaustiveness", C.c_int), ("seed", C.c_int),
                 ("num_modes", C.c_int), ("energy_range", C.c_double), ("hydrophobic_scale", C.c_double),
                 ("ligands_per_launch", C.c_int), ("step_fraction", C.c_double)]
 
 
 class ServerSettings(C.Structure):
     _fields_ = [("device", C.c_int), ("workers", C.c_int), ("max_batch", C.c_int), ("budget_gb", C.c_double),
-                ("fp64", C.c_int), ("seed", C.c_int), ("num_modes", C.c_int), ("energy_range", C.c_double),
+                ("precision", C.c_int), ("seed", C.c_int), ("num_modes", C.c_int), ("energy_range", C.c_double),
                 ("hydrophobic_scale", C.c_double)]
 
 
@@ -488,8 +488,10 @@ class Docking:
     """GpuDocking driver: search, merge, refine, FP64 rescoring."""
 
     def __init__(self, receptor, gd, fp64=False, exhaustiveness=8, seed=1, num_modes=9, energy_range=3.0,
-                 hydrophobic_scale=1.0, ligands_per_launch=0, device=0, step_fraction=1.0):
-        s = DockSettings(device, int(fp64), exhaustiveness, seed, num_modes, energy_range, hydrophobic_scale,
+                 hydrophobic_scale=1.0, ligands_per_launch=0, device=0, step_fraction=1.0, precision=None):
+        """precision: FP32, FP64 or MIXED (default: FP64 if fp64 else FP32)"""
+        prec = precision if precision is not None else (FP64 if fp64 else FP32)
+        s = DockSettings(device, prec, exhaustiveness, seed, num_modes, energy_range, hydrophobic_scale,
                          ligands_per_launch, step_fraction)
         self.gd = gd
         self.h = C.c_void_p(lib.sgpu_docking_create(C.byref(s), receptor.h, C.byref(gd)))
@@ -539,8 +541,9 @@ class Server:
     concurrent launches (sgpu_server_*)."""
 
     def __init__(self, workers=2, max_batch=64, budget_gb=64.0, fp64=False, seed=1, num_modes=9,
-                 energy_range=3.0, hydrophobic_scale=1.0, device=0):
-        s = ServerSettings(device, workers, max_batch, budget_gb, int(fp64), seed, num_modes, energy_range,
+                 energy_range=3.0, hydrophobic_scale=1.0, device=0, precision=None):
+        prec = precision if precision is not None else (FP64 if fp64 else FP32)
+        s = ServerSettings(device, workers, max_batch, budget_gb, prec, seed, num_modes, energy_range,
                            hydrophobic_scale)
         self.h = C.c_void_p(lib.sgpu_server_create(C.byref(s)))
         err = lib.sgpu_server_error(self.h)

@@ -38,7 +38,11 @@ typedef enum sgpu_status
 typedef enum sgpu_precision
 {
 	SGPU_FP32 = 0, /* served search: FP32 geometry, energies and forces */
-	SGPU_FP64 = 1  /* oracle-parity search, ordered like smina's CPU code */
+	SGPU_FP64 = 1, /* oracle-parity search, ordered like smina's CPU code */
+	SGPU_MIXED = 2 /* FP32 evaluation (maps, pair tables, forces) with the
+	                  search state in FP64: conf and root accumulation,
+	                  BFGS Hessian update, line search and Armijo test,
+	                  Metropolis energies.  Models only: its maps are FP32. */
 } sgpu_precision;
 
 enum

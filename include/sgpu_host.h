@@ -56,7 +56,7 @@ typedef struct sgpu_docking sgpu_docking;
 typedef struct sgpu_dock_settings
 {
 	int device;
-	int fp64;            /* --gpu_fp64 */
+	int precision;       /* SGPU_FP32 (served), SGPU_FP64 (--gpu_fp64) or SGPU_MIXED */
 	int exhaustiveness;  /* 8 */
 	int seed;            /* --seed */
 	int num_modes;       /* 9 */
@@ -111,7 +111,7 @@ typedef struct sgpu_server_settings
 	int workers;           /* concurrent launches (each its own device context) */
 	int max_batch;         /* ligands per launch */
 	double budget_gb;      /* device memory per worker for maps and jobs */
-	int fp64;
+	int precision;         /* SGPU_FP32, SGPU_FP64 or SGPU_MIXED */
 	int seed;              /* --seed of every job */
 	int num_modes;
 	double energy_range;

@@ -280,3 +280,41 @@ def test_dock_fp64_matches_oracle(lig, exh, frac):
     e, c, ev = o.dock(ids[0], sg.mc_params(), seed=5, exhaustiveness=exh, threads=exh, step_fraction=frac)
     print(f"dock {lig}: {r['e']}", flush=True)
     assert len(r["e"]) > 0 and r["e"] == list(e)
+
+
+@pytest.mark.parametrize("prec", [sg.MIXED, sg.FP32])
+def test_search_energies_match_fp64_oracle(env, prec):
+    """the served searches: FP32 throughout, and mixed (FP32 evaluation,
+    FP64 search state).  Every pose a Monte Carlo chain saves, and every
+    refined pose, carries the energy the FP64 oracle gives its conf, within
+    the served tolerance (1e-3 kcal/mol, 1e-4 relative above 10 kcal/mol)"""
+    ctx, o = env["ctx"], env["o"]
+    li = 1
+    f = env["flats"][li]
+    bid = ctx.upload_models(env["flats"], prec)
+    cs, ss = zip(*[f.make_chain(6000 + k, env["gd"]) for k in range(4)])
+    maxit = (25 + f.natoms) // 3
+    cnt, e, c, x, ev = ctx.run_mc(bid, env["gids"][sg.FP32], sg.mc_params(), [li] * 4, np.array(cs), np.array(ss),
+                                  [40] * 4, [maxit] * 4, coord_stride=3 * f.nheavy)
+    worst = 0.0
+    poses = []
+    for k in range(4):
+        assert cnt[k] > 0
+        for j in range(cnt[k]):
+            conf = c[k, j, :f.conf_size]
+            eo = o.eval_deriv(env["ids"][li], conf, (1000.0,) * 3, backend=0, slope=1e6)[0]
+            err = abs(e[k, j] - eo)
+            assert err <= 1e-3 * max(1.0, abs(eo) / 10), (k, j, e[k, j], eo)
+            worst = max(worst, err)
+            poses.append(conf)
+    confs = np.array(poses[:8])
+    er, cr, xr = ctx.refine(bid, env["rid"], env["gd"], [li] * len(confs), confs, [maxit] * len(confs),
+                            coord_stride=3 * f.nheavy)
+    for k in range(len(confs)):
+        if er[k] > 1e300:
+            continue  # left the box at every slope
+        # a refined pose lies inside the box, where the slope adds nothing
+        eo = o.eval_deriv(env["ids"][li], cr[k], (1000.0,) * 3, backend=1, slope=10.0)[0]
+        assert abs(er[k] - eo) <= 1e-3 * max(1.0, abs(eo) / 10), (k, er[k], eo)
+    print(f"precision {prec}: worst saved-pose |dE| vs FP64 oracle {worst:.2e}", flush=True)
+    ctx.release_models(bid)
