@@ -11,6 +11,12 @@ descriptions are host inputs (their upload is inside it).
 Beside it: the plain per-ligand FP64 path (one ligand per launch, FP64) on
 a bounded sample, and the CPU oracle (smina's search restated in C++) on
 a bounded sample of the same ligands, one single-threaded ligand per core.
+
+--flex: the flexible-receptor batch instead: 256 ligands (20-50 heavy
+atoms, 3-12 torsions), each docked with four flexible side chains (2-4 chi
+torsions) cut from a 4,000-heavy-atom receptor, 24 A box, Vina scoring with
+the hydrophobic weight x 1.1, mixed precision (FP32 evaluation, FP64 search
+state).  --serve: the serving workload (see serve()).
 """
 import argparse
 import concurrent.futures as cf
@@ -127,7 +133,12 @@ def main():
     ap.add_argument("--jobs", type=int, default=32, help="serving: jobs")
     ap.add_argument("--rate", type=float, default=1.0, help="serving: Poisson arrival rate, jobs/s")
     ap.add_argument("--workers", type=int, default=3, help="serving: concurrent launches")
+    ap.add_argument("--flex", action="store_true",
+                    help="the flexible-receptor batch (256 ligands, 4 side chains, mixed precision)")
+    ap.add_argument("--precision", choices=["fp32", "mixed", "fp64"], default=None,
+                    help="search precision (default: fp32 for the screen, mixed for --flex)")
     args = ap.parse_args()
+    args.ligands_set = any(a.startswith("--ligands") for a in sys.argv[1:])
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the docking path runs on one "
               "GPU; ligand jobs would shard across GPUs with no collective on the critical path).", file=sys.stderr)
@@ -146,14 +157,37 @@ def main():
         serve(sg, args, log)
         return
 
-    t0 = time.time()
-    rec = sg.Mol.receptor(1, 1800, 8.0)
-    ligs, heavy, tors = screen_ligands(sg, args.ligands)
-    gd = sg.grid_dims((0.0, 0.0, 0.0), (26.0, 26.0, 26.0))
+    if args.flex:
+        rec0 = sg.Mol.receptor(7, 4000, 8.0)
+        rng = np.random.default_rng(77)
+        n = args.ligands if args.ligands_set else 256
+        heavy = rng.integers(20, 51, n)
+        tors = np.minimum(rng.integers(3, 13, n), heavy - 3)
+        ligs, rigid = [], None
+        for i, (h, t) in enumerate(zip(heavy, tors)):
+            m, r = sg.Mol.flex(rec0, sg.Mol.ligand(9000 + i, int(h), int(t)), 7, 4, 2, 4)
+            ligs.append(m)
+            rigid = rigid or r
+        rec, gd, hyd = rigid, sg.grid_dims((0.0, 0.0, 0.0), (24.0, 24.0, 24.0)), 1.1
+        prec = {"fp32": sg.FP32, "mixed": sg.MIXED, "fp64": sg.FP64}[args.precision or "mixed"]
+        workload = (f"flexible-receptor batch: {n} ligands (20-50 heavy atoms, 3-12 torsions) with 4 flexible side "
+                    f"chains (2-4 chi torsions each) cut from a 4,000-atom receptor, 24 A box at 0.375 A, Vina "
+                    f"scoring with hydrophobic x 1.1, exhaustiveness {args.exhaustiveness}, "
+                    f"{args.precision or 'mixed'} precision, one launch")
+        metric = "docked ligands/sec/GPU @ exh 8, flexible receptor"
+    else:
+        rec = sg.Mol.receptor(1, 1800, 8.0)
+        ligs, heavy, tors = screen_ligands(sg, args.ligands)
+        gd, hyd = sg.grid_dims((0.0, 0.0, 0.0), (26.0, 26.0, 26.0)), 1.0
+        prec = {"fp32": sg.FP32, "mixed": sg.MIXED, "fp64": sg.FP64}[args.precision or "fp32"]
+        workload = (f"batched screen: {len(ligs)} ligands (10-60 heavy atoms, Poisson(5) torsions <= 15) into a "
+                    f"1,800-atom receptor, 26 A box at 0.375 A, exhaustiveness {args.exhaustiveness}, "
+                    f"{args.precision or 'fp32'}, one launch")
+        metric = "docked ligands/sec/GPU @ exh 8 (FP32 served vs FP64 plain); p50/p99 latency"
     flats = [sg.Flat(l) for l in ligs]
-    log(f"inputs: {len(ligs)} ligands, {sum(f.npairs for f in flats)} pairs, setup {time.time() - t0:.1f}s")
+    log(f"inputs: {len(ligs)} models, {sum(f.npairs for f in flats)} pairs")
 
-    dock = sg.Docking(rec, gd, fp64=False, exhaustiveness=args.exhaustiveness, seed=1)
+    dock = sg.Docking(rec, gd, precision=prec, exhaustiveness=args.exhaustiveness, seed=1, hydrophobic_scale=hyd)
     # build the maps for every ligand type before timing (receptor setup)
     dock.prepare(ligs)
     for w in range(args.warmup):
@@ -172,14 +206,17 @@ def main():
     evals = np.array([r["evals"] for r in results], dtype=np.float64)
     nheavy = np.array([f.nheavy for f in flats], dtype=np.float64)
     npairs = np.array([f.npairs for f in flats], dtype=np.float64)
-    # algorithmic bytes per evaluation, FP32: 8 corner values per heavy atom
-    # (one corner-packed cell) + 2 table points of (e, dor) per pair
-    bytes_per_eval = 8 * 4 * nheavy + 4 * 4 * npairs
+    # algorithmic bytes per evaluation: 8 corner values per heavy atom (one
+    # corner-packed cell) + 2 table points of (e, dor) per pair, in the
+    # evaluation's precision
+    real = 8 if prec == sg.FP64 else 4
+    bytes_per_eval = 8 * real * nheavy + 4 * real * npairs
     alg_bytes = float(np.sum(evals * bytes_per_eval))
     achieved = alg_bytes / (mc_ms / 1e3) / 1e9
     top = np.array([r["e"][0] if r["e"] else np.nan for r in results])
-    traffic = measured_traffic(len(ligs), args.exhaustiveness)
-    log(f"screen: {value:.1f} ligands/s, MC kernel {mc_ms / 1e3:.2f}s, {evals.sum():.3e} evals")
+    traffic = None if args.flex or prec != sg.FP32 else measured_traffic(len(ligs), args.exhaustiveness)
+    log(f"{'flex' if args.flex else 'screen'}: {value:.2f} ligands/s, MC kernel {mc_ms / 1e3:.2f}s, "
+        f"{evals.sum():.3e} evals")
 
     # plain per-ligand FP64 path (one ligand per launch) on a bounded sample:
     # 1/F64_STEP_DIV of the Monte Carlo steps, the MC kernel time scaled back up
@@ -187,7 +224,7 @@ def main():
     if args.fp64_sample > 0:
         frac = 1.0 / F64_STEP_DIV
         d64 = sg.Docking(rec, gd, fp64=True, exhaustiveness=args.exhaustiveness, seed=1, ligands_per_launch=1,
-                         step_fraction=frac)
+                         step_fraction=frac, hydrophobic_scale=hyd)
         sample = ligs[:args.fp64_sample]
         d64.prepare(sample)
         per_lig = []
@@ -198,7 +235,7 @@ def main():
             mc = d64.kernel_ms("mc") / 1e3
             per_lig.append(dt - mc + mc / frac)
         plain = {"value": len(sample) / float(np.sum(per_lig)), "unit": "ligands/s",
-                 "sample": f"first {len(sample)} screen ligand(s), FP64, one ligand per launch, "
+                 "sample": f"first {len(sample)} ligand(s) of the batch, FP64, one ligand per launch, "
                            f"1/{F64_STEP_DIV} of the MC steps with the MC kernel time scaled by {F64_STEP_DIV}",
                  "seconds_per_ligand": per_lig}
         log(f"plain FP64: {plain['value']:.4f} ligands/s (extrapolated)")
@@ -206,43 +243,10 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import pyoracle as po
-        cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-        cores = min(cores, 16)
-        nsample = min(args.cpu_sample or cores, len(ligs))
-        sf = sg.vina_scoring()
-        params = sg.mc_params()
-
-        def one(i):
-            # one ligand per core: maps, then one of its chains at 1/CPU_STEP_DIV
-            # of the steps; the ligand's time is maps + exhaustiveness chains
-            f = flats[i]
-            o = po.Oracle(sf, rec.xyz, rec.type, gd)
-            m = o.add_model(ligs[i].desc_ptr)
-            t0 = time.perf_counter()
-            o.build_grids(sorted(set(int(t) for t in ligs[i].type[:ligs[i].nmovable] if t > 1)))
-            t1 = time.perf_counter()
-            steps, maxit = sg.search_params(f.natoms, f.ntors_total)
-            c0, s0 = f.make_chain(1000 + i, gd)
-            _, _, _, ev = o.mc_chain(m, params, max(1, steps // CPU_STEP_DIV), maxit, c0, s0)
-            t2 = time.perf_counter()
-            return (t1 - t0) + args.exhaustiveness * CPU_STEP_DIV * (t2 - t1), ev, t2 - t1
-
-        t = time.perf_counter()
-        with cf.ThreadPoolExecutor(cores) as ex:
-            out = list(ex.map(one, range(nsample)))
-        dt = time.perf_counter() - t
-        est = [o[0] for o in out]
-        cpu = {"value": cores / float(np.mean(est)), "unit": "ligands/s", "cores": cores, "kind": "port",
-               "sample": f"first {nsample} screen ligands, one per core on {cores} threads: receptor maps plus "
-                         f"one chain at 1/{CPU_STEP_DIV} of its Monte Carlo steps, per-ligand time = maps + "
-                         f"{args.exhaustiveness} chains x {CPU_STEP_DIV} ({dt:.1f}s wall)",
-               "evals_per_s_per_core": float(np.mean([o[1] / o[2] for o in out]))}
-        log(f"CPU oracle: {cpu['value']:.3f} ligands/s on {cores} cores (extrapolated)")
+        cpu = cpu_baseline(sg, args, log, rec, ligs, flats, gd, hyd)
 
     line = {
-        "metric": "docked ligands/sec/GPU @ exh 8 (FP32 served vs FP64 plain); p50/p99 latency",
+        "metric": metric,
         "value": value,
         "unit": "ligands/s",
         "n_gpus": 1,
@@ -252,19 +256,19 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": {sg.FP32: "fp32", sg.MIXED: "fp32 eval + fp64 search", sg.FP64: "fp64"}[prec],
         "data": "synthetic (seeded receptor and ligands)",
-        "config": {"workload": f"batched screen: {len(ligs)} ligands (10-60 heavy atoms, Poisson(5) torsions "
-                               f"<= 15) into a 1,800-atom receptor, 26 A box at 0.375 A, exhaustiveness "
-                               f"{args.exhaustiveness}, FP32, one launch",
-                   "ligands": len(ligs), "exhaustiveness": args.exhaustiveness, "chains": len(ligs) *
-                   args.exhaustiveness, "box_points": int((gd.n[0] + 1) * (gd.n[1] + 1) * (gd.n[2] + 1))},
+        "config": {"workload": workload, "ligands": len(ligs), "exhaustiveness": args.exhaustiveness,
+                   "chains": len(ligs) * args.exhaustiveness,
+                   "box_points": int((gd.n[0] + 1) * (gd.n[1] + 1) * (gd.n[2] + 1))},
         "evals_per_s": float(evals.sum() / (mc_ms / 1e3)),
         "latency_ms": {"p50": step_s * 1e3, "p99": step_s * 1e3,
-                       "note": "one launch: every ligand of the screen completes with it"},
+                       "note": "one launch: every ligand of the batch completes with it"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "mc_kernel<float>", "kernel_ms": mc_ms,
+                     "kernel": "mc_kernel<float>" if prec == sg.FP32 else
+                     "mc_kernel<float, double>" if prec == sg.MIXED else "mc_kernel<double>",
+                     "kernel_ms": mc_ms,
                      "algorithmic_bytes": alg_bytes,
                      "note": "one launch = the four size-class dispatches of one Monte Carlo launch, run "
                              "concurrently; kernel_ms spans them (HIP events on the launching stream); traffic "
@@ -276,6 +280,46 @@ def main():
         "top_score_mean": float(np.nanmean(top)),
     }
     print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(sg, args, log, rec, ligs, flats, gd, hyd):
+    """the FP64 oracle on the host cores: one ligand per core, its receptor
+    maps plus one chain at 1/step_div of its Monte Carlo steps; the ligand's
+    time is maps + exhaustiveness x step_div x the chain"""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cores = min(cores, 16)
+    nsample = min(args.cpu_sample or cores, len(ligs))
+    step_div = CPU_STEP_DIV * (16 if args.flex else 1)
+    sf = sg.vina_scoring(hyd)
+    params = sg.mc_params()
+
+    def one(i):
+        f = flats[i]
+        o = po.Oracle(sf, rec.xyz, rec.type, gd)
+        m = o.add_model(ligs[i].desc_ptr)
+        t0 = time.perf_counter()
+        o.build_grids(sorted(set(int(t) for t in ligs[i].type[:ligs[i].nmovable] if t > 1)))
+        t1 = time.perf_counter()
+        steps, maxit = sg.search_params(f.natoms, f.ntors_total)
+        c0, s0 = f.make_chain(1000 + i, gd)
+        _, _, _, ev = o.mc_chain(m, params, max(1, steps // step_div), maxit, c0, s0)
+        t2 = time.perf_counter()
+        return (t1 - t0) + args.exhaustiveness * step_div * (t2 - t1), ev, t2 - t1
+
+    t = time.perf_counter()
+    with cf.ThreadPoolExecutor(cores) as ex:
+        out = list(ex.map(one, range(nsample)))
+    dt = time.perf_counter() - t
+    est = [o[0] for o in out]
+    cpu = {"value": cores / float(np.mean(est)), "unit": "ligands/s", "cores": cores, "kind": "port",
+           "sample": f"first {nsample} ligands of the batch, one per core on {cores} threads: receptor maps plus "
+                     f"one chain at 1/{step_div} of its Monte Carlo steps, per-ligand time = maps + "
+                     f"{args.exhaustiveness} chains x {step_div} ({dt:.1f}s wall)",
+           "evals_per_s_per_core": float(np.mean([o[1] / o[2] for o in out]))}
+    log(f"CPU oracle: {cpu['value']:.3f} ligands/s on {cores} cores (extrapolated)")
+    return cpu
 
 
 if __name__ == "__main__":

@@ -578,18 +578,16 @@ sgpu_mol* sgpu_synth_flex(const sgpu_mol* receptor, const sgpu_mol* ligand, uint
 	}
 	const V3 c0{center ? center[0] : 0, center ? center[1] : 0, center ? center[2] : 0};
 	auto at = [&](int a) { return V3{receptor->xyz[3 * a], receptor->xyz[3 * a + 1], receptor->xyz[3 * a + 2]}; };
-	// chain ends, nearest the centre first
+	// candidate side chains: every chain end with its path; the chosen ones
+	// are those whose farthest atom is nearest the centre
 	std::vector<int> ends;
 	for (int a = 0; a < na; a++)
 		if (receptor->type[a] != T_PolarHydrogen && heavy_nb[a].size() == 1)
 			ends.push_back(a);
-	std::stable_sort(ends.begin(), ends.end(), [&](int a, int b) { return norm(at(a) - c0) < norm(at(b) - c0); });
-	std::vector<char> taken(na, 0);
-	std::vector<std::vector<int>> chains; // [a(c+1), ..., a1, anchor]
+	std::vector<std::vector<int>> cand; // [a(c+1), ..., a1, anchor]
+	std::vector<double> reach;
 	for (int e : ends)
 	{
-		if (int(chains.size()) == nflex)
-			break;
 		const int c = g.i(min_chi, max_chi);
 		std::vector<int> path{e};
 		while (int(path.size()) < c + 2)
@@ -606,14 +604,29 @@ sgpu_mol* sgpu_synth_flex(const sgpu_mol* receptor, const sgpu_mol* ligand, uint
 		}
 		if (int(path.size()) < c + 2)
 			continue;
-		bool free = true;
+		double r = 0;
 		for (int a : path)
+			r = std::max(r, norm(at(a) - c0));
+		cand.push_back(path);
+		reach.push_back(r);
+	}
+	std::vector<int> order(cand.size());
+	std::iota(order.begin(), order.end(), 0);
+	std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return reach[a] < reach[b]; });
+	std::vector<char> taken(na, 0);
+	std::vector<std::vector<int>> chains;
+	for (int k : order)
+	{
+		if (int(chains.size()) == nflex)
+			break;
+		bool free = true;
+		for (int a : cand[k])
 			free = free && !taken[a];
 		if (!free)
 			continue;
-		for (int a : path)
+		for (int a : cand[k])
 			taken[a] = 1;
-		chains.push_back(path);
+		chains.push_back(cand[k]);
 	}
 	if (int(chains.size()) < nflex)
 		return nullptr;

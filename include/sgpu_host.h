@@ -37,8 +37,9 @@ sgpu_mol* sgpu_synth_ligand(uint32_t seed, int nheavy, int ntors, const double* 
  * (src/lib/parse_pdbqt.cpp:360-380): the ligand's trees; one tree per side
  * chain, with min_chi..max_chi torsions, whose root is a first segment
  * turning about the bond from its fixed anchor atom; then the anchors (the
- * fixed "inflex" atoms).  The side chains are receptor chain ends nearest
- * `center`.  *rigid receives the receptor without the side chains' atoms.
+ * fixed "inflex" atoms).  The side chains are receptor chain ends, those
+ * whose farthest atom is nearest `center`.  *rigid receives the receptor
+ * without the side chains' atoms.
  * Returns NULL if the receptor has too few chain ends. */
 sgpu_mol* sgpu_synth_flex(const sgpu_mol* receptor, const sgpu_mol* ligand, uint32_t seed, int nflex, int min_chi,
 		int max_chi, const double* center, sgpu_mol** rigid);
