@@ -187,7 +187,12 @@ def main():
     flats = [sg.Flat(l) for l in ligs]
     log(f"inputs: {len(ligs)} models, {sum(f.npairs for f in flats)} pairs")
 
-    dock = sg.Docking(rec, gd, precision=prec, exhaustiveness=args.exhaustiveness, seed=1, hydrophobic_scale=hyd)
+    # a long search reports its progress (smina's progress display): the
+    # flexible batch runs its Monte Carlo steps as 8 consecutive launches
+    chunks = 8 if args.flex else 1
+    dock = sg.Docking(rec, gd, precision=prec, exhaustiveness=args.exhaustiveness, seed=1, hydrophobic_scale=hyd,
+                      progress=lambda f: log(f"Monte Carlo {100 * f:.0f}% of steps") if chunks > 1 else None,
+                      progress_chunks=chunks)
     # build the maps for every ligand type before timing (receptor setup)
     dock.prepare(ligs)
     for w in range(args.warmup):

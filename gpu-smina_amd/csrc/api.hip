@@ -162,6 +162,9 @@ struct sgpu_ctx
 	DBuf scratch; // per-call scratch
 	DBuf scratch2;
 	DBuf err_flag;
+	int progress_chunks = 1;
+	sgpu_progress_fn progress_fn = nullptr;
+	void* progress_user = nullptr;
 };
 
 namespace
@@ -1278,36 +1281,49 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 	// Heaviest class first, on the context's stream; each further class on a
 	// stream of its own (four streams at most: one hardware queue each, so
 	// no launch queues behind another stream's work).  The waits that join
-	// the streams are enqueued after every launch.
+	// the streams are enqueued after every launch.  With progress reporting
+	// the steps run as consecutive chunks, joined and reported one by one.
+	const int chunks = std::max(1, std::min(c->progress_chunks, max_steps));
 	Timer tm(c, "mc");
-	std::vector<hipStream_t> side;
-	for (int k = sgpu_ctx::NCLASS - 1; k >= 0; k--)
+	for (int ck = 0; ck < chunks; ck++)
 	{
-		const Cls& cl = cls[k];
-		if (cl.blks.empty())
-			continue;
-		hipStream_t st = c->stream;
-		bool first = true;
-		for (int j = k + 1; j < sgpu_ctx::NCLASS; j++)
-			if (!cls[j].blks.empty())
-				first = false;
-		if (!first)
+		const int lo = int(int64_t(max_steps) * ck / chunks), hi = int(int64_t(max_steps) * (ck + 1) / chunks);
+		std::vector<hipStream_t> side;
+		for (int k = sgpu_ctx::NCLASS - 1; k >= 0; k--)
 		{
-			st = c->class_stream[side.size()];
-			side.push_back(st);
-			hip_check(hipStreamWaitEvent(st, c->ev0, 0), "stream wait");
+			const Cls& cl = cls[k];
+			if (cl.blks.empty())
+				continue;
+			hipStream_t st = c->stream;
+			bool first = true;
+			for (int j = k + 1; j < sgpu_ctx::NCLASS; j++)
+				if (!cls[j].blks.empty())
+					first = false;
+			if (!first)
+			{
+				st = c->class_stream[side.size()];
+				side.push_back(st);
+				hip_check(hipStreamWaitEvent(st, c->ev0, 0), "stream wait");
+			}
+			hip_check(launch_mc<T, S>(ptrs<T>(b), ec, cl.plan, P, L, dchains.as<char>(), int(cl.blks.size()),
+							  cl.waves, dbm.as<int>() + cl.block_off, dsc.as<int>() + cl.slot_off, dsteps.as<int>(),
+							  dmax.as<int>(), lo, hi, st),
+					"mc kernel");
 		}
-		hip_check(launch_mc<T, S>(ptrs<T>(b), ec, cl.plan, P, L, dchains.as<char>(), int(cl.blks.size()), cl.waves,
-						  dbm.as<int>() + cl.block_off, dsc.as<int>() + cl.slot_off, dsteps.as<int>(), dmax.as<int>(),
-						  0, max_steps, st),
-				"mc kernel");
-	}
-	for (size_t j = 0; j < side.size(); j++)
-	{
-		hip_check(hipEventRecord(c->class_done[j], side[j]), "event");
-		hip_check(hipStreamWaitEvent(c->stream, c->class_done[j], 0), "stream wait");
+		for (size_t j = 0; j < side.size(); j++)
+		{
+			hip_check(hipEventRecord(c->class_done[j], side[j]), "event");
+			hip_check(hipStreamWaitEvent(c->stream, c->class_done[j], 0), "stream wait");
+		}
+		if (c->progress_fn && ck + 1 < chunks)
+		{
+			hip_check(hipStreamSynchronize(c->stream), "mc chunk");
+			c->progress_fn(c->progress_user, double(hi) / double(max_steps));
+		}
 	}
 	tm.stop();
+	if (c->progress_fn)
+		c->progress_fn(c->progress_user, 1.0);
 	hip_check(hipMemcpyAsync(init.data(), dchains.p, init.size(), hipMemcpyDeviceToHost, c->stream), "download");
 	hip_check(hipStreamSynchronize(c->stream), "mc");
 	for (int i = 0; i < n; i++)
@@ -1511,6 +1527,16 @@ sgpu_status sgpu_score(sgpu_ctx* c, const sgpu_score_request* q)
 		require(q != nullptr, "no request");
 		require(c->tab.ok, "tables not uploaded");
 		run_score(c, q);
+	});
+}
+
+sgpu_status sgpu_set_progress(sgpu_ctx* c, int chunks, sgpu_progress_fn fn, void* user)
+{
+	return guarded(c, [&]() {
+		require(chunks >= 1 && chunks <= 1000, "1..1000 progress chunks");
+		c->progress_chunks = chunks;
+		c->progress_fn = fn;
+		c->progress_user = user;
 	});
 }
 

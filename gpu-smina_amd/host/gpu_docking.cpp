@@ -590,6 +590,7 @@ GpuDocking::GpuDocking(const GpuSettings& s, const sgpu_scoring& sf, const sgpu_
 	sgpu_tables tt{t.n, t.factor, t.cutoff_sqr, t.smooth.data(), t.fast.data()};
 	check(ctx_, sgpu_upload_tables(ctx_, &tt));
 	check(ctx_, sgpu_upload_receptor(ctx_, &rec, &receptor_));
+	check(ctx_, sgpu_set_progress(ctx_, s.progress_chunks, s.progress, s.progress_user));
 }
 
 GpuDocking::GpuDocking(const GpuSettings& s, const sgpu_scoring& sf, const sgpu_receptor& rec,
@@ -1126,6 +1127,9 @@ sgpu_docking* sgpu_docking_create(const sgpu_dock_settings* s, const sgpu_mol* r
 		if (!(s->step_fraction > 0) || s->step_fraction > 1)
 			throw std::invalid_argument("step_fraction must be in (0, 1]");
 		gs.step_fraction = s->step_fraction;
+		gs.progress_chunks = s->progress_chunks > 0 ? s->progress_chunks : 1;
+		gs.progress = s->progress;
+		gs.progress_user = s->progress_user;
 		d->terms.resize(5);
 		sgpu_scoring sf;
 		sgpu_vina_scoring(d->terms.data(), &sf, s->hydrophobic_scale);

@@ -86,6 +86,11 @@ struct GpuSettings
 	double energy_range = 3.0;
 	int ligands_per_launch = 0; // 0: the whole batch in one launch
 	double step_fraction = 1;   // fraction of smina's Monte Carlo steps (1: smina's search)
+	// progress of the Monte Carlo search (sgpu_set_progress): its steps in
+	// this many launches, progress(progress_user, fraction) after each
+	int progress_chunks = 1;
+	sgpu_progress_fn progress = nullptr;
+	void* progress_user = nullptr;
 };
 
 class GpuDocking

@@ -101,10 +101,14 @@ class Scoring(C.Structure):
                 ("num_tors_div_weight", C.c_double)]
 
 
+PROGRESS_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_double)
+
+
 class DockSettings(C.Structure):
     _fields_ = [("device", C.c_int), ("precision", C.c_int), ("exhaustiveness", C.c_int), ("seed", C.c_int),
                 ("num_modes", C.c_int), ("energy_range", C.c_double), ("hydrophobic_scale", C.c_double),
-                ("ligands_per_launch", C.c_int), ("step_fraction", C.c_double)]
+                ("ligands_per_launch", C.c_int), ("step_fraction", C.c_double), ("progress_chunks", C.c_int),
+                ("progress", PROGRESS_FN), ("progress_user", C.c_void_p)]
 
 
 class ServerSettings(C.Structure):
@@ -149,6 +153,7 @@ def _load():
         "sgpu_run_refine": (C.c_int, [vp, C.POINTER(RefineRequest)]),
         "sgpu_score": (C.c_int, [vp, C.POINTER(ScoreRequest)]),
         "sgpu_last_kernel_ms": (C.c_int, [vp, C.c_char_p, F64P]),
+        "sgpu_set_progress": (C.c_int, [vp, C.c_int, PROGRESS_FN, vp]),
         "sgpu_mol_free": (None, [vp]),
         "sgpu_mol_desc": (C.POINTER(ModelDesc), [vp]),
         "sgpu_synth_receptor": (vp, [C.c_uint32, C.c_int, C.c_double]),
@@ -188,7 +193,7 @@ lib = _load()
 EXPORTED = ["sgpu_create", "sgpu_destroy", "sgpu_last_error", "sgpu_device_info", "sgpu_upload_tables",
             "sgpu_upload_receptor", "sgpu_build_grids", "sgpu_download_grid", "sgpu_release_grid",
             "sgpu_upload_models", "sgpu_release_models", "sgpu_eval_confs", "sgpu_run_mc", "sgpu_run_refine",
-            "sgpu_score", "sgpu_last_kernel_ms"]
+            "sgpu_score", "sgpu_last_kernel_ms", "sgpu_set_progress"]
 
 
 def ptr(a, t):
@@ -462,6 +467,11 @@ class Context:
         self.check(lib.sgpu_last_kernel_ms(self.h, kind.encode(), C.byref(ms)))
         return ms.value
 
+    def set_progress(self, chunks, fn=None):
+        """run_mc in `chunks` launches, fn(fraction) after each"""
+        self._progress = PROGRESS_FN(lambda user, frac: fn(frac)) if fn else PROGRESS_FN()
+        self.check(lib.sgpu_set_progress(self.h, chunks, self._progress, None))
+
 
 def mc_params(hunt=10.0, authentic=1000.0, num_saved=20):
     p = McParams()
@@ -488,11 +498,16 @@ class Docking:
     """GpuDocking driver: search, merge, refine, FP64 rescoring."""
 
     def __init__(self, receptor, gd, fp64=False, exhaustiveness=8, seed=1, num_modes=9, energy_range=3.0,
-                 hydrophobic_scale=1.0, ligands_per_launch=0, device=0, step_fraction=1.0, precision=None):
-        """precision: FP32, FP64 or MIXED (default: FP64 if fp64 else FP32)"""
+                 hydrophobic_scale=1.0, ligands_per_launch=0, device=0, step_fraction=1.0, precision=None,
+                 progress=None, progress_chunks=1):
+        """precision: FP32, FP64 or MIXED (default: FP64 if fp64 else FP32);
+        progress(fraction): called after each of progress_chunks launches
+        of the Monte Carlo search"""
         prec = precision if precision is not None else (FP64 if fp64 else FP32)
+        # keep the callback alive as long as the docking object
+        self._progress = PROGRESS_FN(lambda user, frac: progress(frac)) if progress else PROGRESS_FN()
         s = DockSettings(device, prec, exhaustiveness, seed, num_modes, energy_range, hydrophobic_scale,
-                         ligands_per_launch, step_fraction)
+                         ligands_per_launch, step_fraction, progress_chunks, self._progress, None)
         self.gd = gd
         self.h = C.c_void_p(lib.sgpu_docking_create(C.byref(s), receptor.h, C.byref(gd)))
         err = lib.sgpu_docking_error(self.h)

@@ -278,6 +278,14 @@ sgpu_status sgpu_score(sgpu_ctx* ctx, const sgpu_score_request* req);
  * the kernel measured with HIP events on the launch stream */
 sgpu_status sgpu_last_kernel_ms(sgpu_ctx* ctx, const char* kind, double* ms);
 
+/* Progress of the Monte Carlo search (smina's progress display,
+ * src/lib/parallel_progress.h): sgpu_run_mc runs the chains' steps as
+ * `chunks` consecutive launches (the chains resume from their device state)
+ * and calls fn(user, fraction of steps done) after each, on the calling
+ * thread.  chunks 1 (the default) runs one launch; fn may be NULL. */
+typedef void (*sgpu_progress_fn)(void* user, double fraction);
+sgpu_status sgpu_set_progress(sgpu_ctx* ctx, int chunks, sgpu_progress_fn fn, void* user);
+
 #ifdef __cplusplus
 }
 #endif

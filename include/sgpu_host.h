@@ -66,6 +66,9 @@ typedef struct sgpu_dock_settings
 	int ligands_per_launch; /* batching: 0 = all jobs in one launch */
 	double step_fraction;   /* fraction of smina's Monte Carlo steps to run (1: smina's
 	                         * search; GpuSettings::step_fraction, src/gpu/gpu_docking.h:29-31) */
+	int progress_chunks;    /* Monte Carlo search in this many launches (0 or 1: one) */
+	sgpu_progress_fn progress; /* called after each with the fraction done (sgpu_set_progress) */
+	void* progress_user;
 } sgpu_dock_settings;
 
 sgpu_docking* sgpu_docking_create(const sgpu_dock_settings* s, const sgpu_mol* receptor,

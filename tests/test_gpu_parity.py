@@ -168,6 +168,23 @@ def test_mc_fp64_trajectories_identical(env, steps):
         assert ev[k] == evo
 
 
+def test_mc_fp64_chunked_launches(env):
+    """the search run as several launches (progress reporting): chains
+    resume from their device state and still follow the oracle exactly"""
+    ctx, o, li = env["ctx"], env["o"], 2
+    seen = []
+    ctx.set_progress(3, seen.append)
+    try:
+        cs, ss, maxit, (cnt, e, c, x, ev) = run_chains(env, li, sg.FP64, 25, nchains=3, seed0=2500)
+    finally:
+        ctx.set_progress(1)
+    assert len(seen) == 3 and seen[-1] == 1.0 and seen == sorted(seen)
+    for k in range(len(cs)):
+        eo, co, xo, evo = o.mc_chain(env["ids"][li], sg.mc_params(), 25, maxit, cs[k], ss[k])
+        assert cnt[k] == len(eo) and np.array_equal(e[k, :cnt[k]], eo)
+        assert np.array_equal(c[k, :cnt[k], :co.shape[1]], co) and ev[k] == evo
+
+
 def test_mc_fp64_batched_models(env):
     """chains of different ligands in one launch follow their own CPU chains"""
     ctx, o = env["ctx"], env["o"]
