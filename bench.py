@@ -135,6 +135,8 @@ def main():
     ap.add_argument("--workers", type=int, default=3, help="serving: concurrent launches")
     ap.add_argument("--flex", action="store_true",
                     help="the flexible-receptor batch (256 ligands, 4 side chains, mixed precision)")
+    ap.add_argument("--accuracy", type=int, default=0,
+                    help="top-pose accuracy on this many ligands: served vs FP64, mixed vs FP64, FP64 seed 2 vs 1")
     ap.add_argument("--precision", choices=["fp32", "mixed", "fp64"], default=None,
                     help="search precision (default: fp32 for the screen, mixed for --flex)")
     args = ap.parse_args()
@@ -250,6 +252,10 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(sg, args, log, rec, ligs, flats, gd, hyd)
 
+    accuracy = None
+    if args.accuracy > 0:
+        accuracy = top_pose_accuracy(sg, args, log, rec, ligs[:args.accuracy], gd, hyd, results[:args.accuracy])
+
     line = {
         "metric": metric,
         "value": value,
@@ -284,7 +290,37 @@ def main():
         "speedup_vs_plain_fp64": (value / plain["value"]) if plain else None,
         "top_score_mean": float(np.nanmean(top)),
     }
+    if accuracy:
+        line["top_pose_accuracy"] = accuracy
     print(json.dumps(line), flush=True)
+
+
+def top_pose_accuracy(sg, args, log, rec, sample, gd, hyd, served):
+    """per-ligand top score of the served search minus FP64's (the same
+    seed, full search, one launch), beside FP64 at another seed minus FP64
+    (the search's own noise floor), as the reference reports it"""
+    def run(prec, seed):
+        d = sg.Docking(rec, gd, precision=prec, exhaustiveness=args.exhaustiveness, seed=seed, hydrophobic_scale=hyd,
+                       progress=lambda f: log(f"accuracy {prec_name[prec]} seed {seed}: {100 * f:.0f}%"),
+                       progress_chunks=8)
+        out = d.dock(sample)
+        return np.array([r["e"][0] if r["e"] else np.nan for r in out])
+    prec_name = {sg.FP32: "fp32", sg.MIXED: "mixed", sg.FP64: "fp64"}
+    e64 = run(sg.FP64, 1)
+    e64b = run(sg.FP64, 2)
+    emix = run(sg.MIXED, 1)
+    eserved = np.array([r["e"][0] if r["e"] else np.nan for r in served])
+
+    def stats(d):
+        d = d[np.isfinite(d)]
+        return {"median": float(np.median(d)), "sd": float(np.std(d, ddof=1)), "max_abs": float(np.max(np.abs(d)))}
+    acc = {"sample": f"first {len(sample)} ligands of the batch, exhaustiveness {args.exhaustiveness}, top score "
+                     f"after refinement and FP64 rescoring, kcal/mol",
+           "served_minus_fp64": stats(eserved - e64),
+           "mixed_minus_fp64": stats(emix - e64),
+           "fp64_seed2_minus_fp64": stats(e64b - e64)}
+    log(f"top-pose accuracy: {json.dumps(acc)}")
+    return acc
 
 
 def cpu_baseline(sg, args, log, rec, ligs, flats, gd, hyd):
