@@ -79,6 +79,7 @@ struct WaveMem
 	T* ebuf;
 	T* xstage;
 	T* pforce; // pair forces (pair-parallel search only)
+	T* tsum;   // prefix sums of the atoms' torque terms (pair-parallel search only)
 };
 
 // The search's own vectors in the search precision S: the staged conf and
@@ -99,7 +100,7 @@ template<typename T>
 __host__ __device__ inline int eval_lds_elems(const WavePlan& p)
 {
 	return 3 * p.amax + 3 * p.nmax + p.fmax * (3 + 9 + 4 + 3 + 3 + 3) + p.gmax + p.nmax + p.cmax +
-			(pair_parallel<T>() ? 3 * p.pmax : 0);
+			(pair_parallel<T>() ? 3 * p.pmax + 3 * p.nmax : 0);
 }
 template<typename T, typename S>
 __host__ __device__ inline int search_lds_elems(const WavePlan& p)
@@ -153,6 +154,8 @@ __device__ inline void carve(unsigned char* base, const WavePlan& p, WaveMem<T>&
 	w.xstage = q;
 	q += p.cmax;
 	w.pforce = q;
+	q += pair_parallel<T>() ? 3 * p.pmax : 0;
+	w.tsum = q;
 	S* r = reinterpret_cast<S*>(base + (eval_lds_elems<T>(p) * int(sizeof(T)) + 15) / 16 * 16);
 	sm.pstage = r;
 	r += p.gmax;
@@ -321,47 +324,60 @@ __device__ __forceinline__ void set_conf(const MV<T>& m, WaveMem<T>& W, T x_lane
 	lsync();
 	if (pair_parallel<T>() && m.h.path_ok)
 	{
-		// ancestor walk: the lane of frame f composes the frames on its
-		// root-to-f path in registers, the same operations in the same order
-		// as the level-by-level pass, with no exchange between levels
-		for (int f = lane; f < m.h.nframes; f += WAVE)
+		// Ancestor walk, one lane per frame (at most 64).  A segment's
+		// rotation q(R_p l, theta) * q_p equals q_p * q(l, theta), l its axis
+		// in the input coordinates (every frame starts unrotated), so each
+		// frame's own factor is made once by its lane and the walk composes
+		// the path's factors and rotates its relative origins.
+		const int F = m.h.nframes;
+		const int f = lane;
+		if (f < F)
+		{
+			const int ci = m.fconf[f];
+			Q4<T> lq;
+			if (m.fkind[f] == SGPU_FRAME_RIGID_ROOT)
+				lq = {W.xstage[ci + 3], W.xstage[ci + 4], W.xstage[ci + 5], W.xstage[ci + 6]};
+			else
+				lq = axis_angle_q(ld3(m.faxis + 3 * f), W.xstage[ci]);
+			W.fq[4 * f] = lq.a;
+			W.fq[4 * f + 1] = lq.b;
+			W.fq[4 * f + 2] = lq.c;
+			W.fq[4 * f + 3] = lq.d;
+		}
+		lsync();
+		V3<T> o = v3(T(0), T(0), T(0)), ax = o;
+		Q4<T> q = {T(1), T(0), T(0), T(0)};
+		if (f < F)
 		{
 			const int d = m.fdepth[f];
 			const int* path = m.fpath + f * (MAX_PATH / 4);
-			int r = path[0] & 255;
+			const int r = path[0] & 255;
 			const int rci = m.fconf[r];
-			V3<T> o;
-			Q4<T> q;
-			V3<T> ax = v3(T(0), T(0), T(0));
-			if (m.fkind[r] == SGPU_FRAME_RIGID_ROOT)
-			{
-				o = v3(W.xstage[rci], W.xstage[rci + 1], W.xstage[rci + 2]);
-				q = {W.xstage[rci + 3], W.xstage[rci + 4], W.xstage[rci + 5], W.xstage[rci + 6]};
-			}
-			else
-			{
-				o = ld3(m.forig + 3 * r);
-				ax = ld3(m.faxis + 3 * r);
-				q = axis_angle_q(ax, W.xstage[rci]);
-			}
-			T R[9];
-			q_to_mat(q, R);
+			o = m.fkind[r] == SGPU_FRAME_RIGID_ROOT ? v3(W.xstage[rci], W.xstage[rci + 1], W.xstage[rci + 2])
+			                                        : ld3(m.forig + 3 * r);
+			q = {W.fq[4 * r], W.fq[4 * r + 1], W.fq[4 * r + 2], W.fq[4 * r + 3]};
+			Q4<T> qp = {T(1), T(0), T(0), T(0)};
 			for (int k = 1; k < d; k++)
 			{
 				const int c = (path[k >> 2] >> (8 * (k & 3))) & 255;
-				o = o + mat_vec(R, ld3(m.forig + 3 * c));
-				ax = mat_vec(R, ld3(m.faxis + 3 * c));
-				q = qmul(axis_angle_q(ax, W.xstage[m.fconf[c]]), q);
-				q_normalize_approx(q);
-				q_to_mat(q, R);
+				o = o + q_rotate(q, ld3(m.forig + 3 * c));
+				qp = q;
+				q = qmul(q, Q4<T>{W.fq[4 * c], W.fq[4 * c + 1], W.fq[4 * c + 2], W.fq[4 * c + 3]});
 			}
+			// the world axis: the frame's own axis under its parent's rotation
+			// (a first segment's parent rotation is the identity)
+			if (m.fkind[f] != SGPU_FRAME_RIGID_ROOT)
+				ax = q_rotate(qp, ld3(m.faxis + 3 * f));
+		}
+		lsync(); // every lane has read the path's factors
+		if (f < F)
+		{
 			st3(W.fo + 3 * f, o);
 			W.fq[4 * f] = q.a;
 			W.fq[4 * f + 1] = q.b;
 			W.fq[4 * f + 2] = q.c;
 			W.fq[4 * f + 3] = q.d;
-			for (int k = 0; k < 9; k++)
-				W.fR[9 * f + k] = R[k];
+			q_to_mat(q, W.fR + 9 * f);
 			if (m.fkind[f] != SGPU_FRAME_RIGID_ROOT)
 				st3(W.fax + 3 * f, ax);
 		}
@@ -782,45 +798,67 @@ __device__ __forceinline__ T eval_deriv(const MV<T>& m, const EvalCtx<T>& ec, Wa
 			}
 		}
 		lsync();
-		// per movable atom: receptor term, then its pair forces in list order;
+		// Per movable atom: receptor term, then its pair forces in list order;
 		// the map gather is issued first and consumed last, so its latency
-		// overlaps the list walk
-		for (int i = lane; i < N; i += WAVE)
+		// overlaps the list walk.  Then inclusive prefix sums over the atoms
+		// of the force and of (x - o0) x force (o0: frame 0's origin) go to
+		// LDS: every frame's subtree is a contiguous atom range, so the
+		// derivative reads its sums as two differences.
+		const V3<T> o0 = ld3(W.fo);
+		V3<T> carry_f = v3(T(0), T(0), T(0)), carry_t = carry_f;
+		for (int base = 0; base < N; base += WAVE)
 		{
-			const int t = m.type[i];
-			const bool heavy = !is_hydrogen_type(t);
-			const V3<T> xi = ld3(W.coords + 3 * i);
-			GridSample<T> gs;
-			if constexpr (BK == BK_MAPS)
-				gs = grid_fetch(ec.grid, heavy ? m.slot[i] : 0, xi);
-			V3<T> fp = v3(T(0), T(0), T(0));
-			const int k0 = m.nbr_start[i], k1 = m.nbr_start[i + 1];
-			for (int k = k0; k < k1; k += GATHER_BATCH)
+			const int i = base + lane;
+			V3<T> force = v3(T(0), T(0), T(0)), xi = force;
+			if (i < N)
 			{
-				int cd[GATHER_BATCH];
-#pragma unroll
-				for (int u = 0; u < GATHER_BATCH; u++)
-					cd[u] = m.code[k + u < k1 ? k + u : k0];
-				V3<T> f[GATHER_BATCH];
-#pragma unroll
-				for (int u = 0; u < GATHER_BATCH; u++)
-					f[u] = ld3(W.pforce + 3 * code_pair(cd[u]));
-#pragma unroll
-				for (int u = 0; u < GATHER_BATCH; u++)
-				{
-					const T sg = k + u >= k1 ? T(0) : (code_is_b(cd[u]) ? T(1) : T(-1));
-					fp = v3(fp.x + sg * f[u].x, fp.y + sg * f[u].y, fp.z + sg * f[u].z);
-				}
-			}
-			V3<T> d = v3(T(0), T(0), T(0));
-			if (heavy)
-			{
+				const int t = m.type[i];
+				const bool heavy = !is_hydrogen_type(t);
+				xi = ld3(W.coords + 3 * i);
+				GridSample<T> gs;
 				if constexpr (BK == BK_MAPS)
-					e_part += grid_finish(ec.grid, gs, ec.v[1], d);
-				else
-					e_part += direct_term(ec, t, xi, d);
+					gs = grid_fetch(ec.grid, heavy ? m.slot[i] : 0, xi);
+				V3<T> fp = v3(T(0), T(0), T(0));
+				const int k0 = m.nbr_start[i], k1 = m.nbr_start[i + 1];
+				for (int k = k0; k < k1; k += GATHER_BATCH)
+				{
+					int cd[GATHER_BATCH];
+#pragma unroll
+					for (int u = 0; u < GATHER_BATCH; u++)
+						cd[u] = m.code[k + u < k1 ? k + u : k0];
+					V3<T> f[GATHER_BATCH];
+#pragma unroll
+					for (int u = 0; u < GATHER_BATCH; u++)
+						f[u] = ld3(W.pforce + 3 * code_pair(cd[u]));
+#pragma unroll
+					for (int u = 0; u < GATHER_BATCH; u++)
+					{
+						const T sg = k + u >= k1 ? T(0) : (code_is_b(cd[u]) ? T(1) : T(-1));
+						fp = v3(fp.x + sg * f[u].x, fp.y + sg * f[u].y, fp.z + sg * f[u].z);
+					}
+				}
+				V3<T> d = v3(T(0), T(0), T(0));
+				if (heavy)
+				{
+					if constexpr (BK == BK_MAPS)
+						e_part += grid_finish(ec.grid, gs, ec.v[1], d);
+					else
+						e_part += direct_term(ec, t, xi, d);
+				}
+				force = d + fp;
 			}
-			st3(W.forces + 3 * i, d + fp);
+			const V3<T> tq = cross(xi - o0, force);
+			const V3<T> sf = v3(wave_inclusive_scan(force.x), wave_inclusive_scan(force.y),
+							  wave_inclusive_scan(force.z)) + carry_f;
+			const V3<T> st = v3(wave_inclusive_scan(tq.x), wave_inclusive_scan(tq.y), wave_inclusive_scan(tq.z)) +
+					carry_t;
+			if (i < N)
+			{
+				st3(W.forces + 3 * i, sf);
+				st3(W.tsum + 3 * i, st);
+			}
+			carry_f = v3(readlane(sf.x, WAVE - 1), readlane(sf.y, WAVE - 1), readlane(sf.z, WAVE - 1));
+			carry_t = v3(readlane(st.x, WAVE - 1), readlane(st.y, WAVE - 1), readlane(st.z, WAVE - 1));
 		}
 	}
 	T e;
@@ -851,53 +889,35 @@ __device__ __forceinline__ T eval_deriv(const MV<T>& m, const EvalCtx<T>& ec, Wa
 	}
 	if constexpr (pair_parallel<T>())
 	{
-		// derivative, direct: each frame's force and torque summed over its
-		// subtree's atoms (a contiguous range), equal to smina's bottom-up
-		// accumulation (src/lib/tree.h:88-97, :214-289) up to rounding; a
-		// group of G lanes per frame, reduced with DPP
+		// derivative, direct: each frame's force and torque over its
+		// subtree's atoms (a contiguous range) from the prefix sums, equal to
+		// smina's bottom-up accumulation (src/lib/tree.h:88-97, :214-289) up
+		// to rounding: torque about the frame's origin o is
+		// sum (x - o0) x f - (o - o0) x sum f
 		const int F = m.h.nframes;
-		int G = 16;
-		while (G > 1 && G * F > WAVE)
-			G >>= 1;
-		const int f = lane / G, sub = lane - f * G;
-		V3<T> fo = v3(T(0), T(0), T(0)), to = v3(T(0), T(0), T(0));
-		V3<T> o = fo;
+		const int f = lane;
 		if (f < F)
 		{
-			o = ld3(W.fo + 3 * f);
-			const int en = m.fsend[f];
-#pragma unroll 2
-			for (int i = m.fbegin[f] + sub; i < en; i += G)
+			const int b = m.fbegin[f], en = m.fsend[f];
+			V3<T> fs = ld3(W.forces + 3 * (en - 1)), ts = ld3(W.tsum + 3 * (en - 1));
+			if (b > 0)
 			{
-				const V3<T> fi = ld3(W.forces + 3 * i);
-				fo = fo + fi;
-				to = to + cross(ld3(W.coords + 3 * i) - o, fi);
+				fs = fs - ld3(W.forces + 3 * (b - 1));
+				ts = ts - ld3(W.tsum + 3 * (b - 1));
 			}
-		}
-		T acc[6] = {fo.x, fo.y, fo.z, to.x, to.y, to.z};
-#pragma unroll
-		for (int k = 0; k < 6; k++)
-		{
-			if (G >= 2)
-				acc[k] += dpp<DPP_QUAD_1032>(acc[k]);
-			if (G >= 4)
-				acc[k] += dpp<DPP_QUAD_2301>(acc[k]);
-			if (G >= 8)
-				acc[k] += dpp<DPP_ROW_HALF_MIRROR>(acc[k]);
-			if (G >= 16)
-				acc[k] += dpp<DPP_ROW_MIRROR>(acc[k]);
-		}
-		if (f < F && sub == 0)
-		{
+			const V3<T> to = ts - cross(ld3(W.fo + 3 * f) - ld3(W.fo), fs);
 			const int gi = m.fchange[f];
 			if (m.fkind[f] == SGPU_FRAME_RIGID_ROOT)
 			{
-#pragma unroll
-				for (int k = 0; k < 6; k++)
-					W.gbuf[gi + k] = acc[k];
+				W.gbuf[gi] = fs.x;
+				W.gbuf[gi + 1] = fs.y;
+				W.gbuf[gi + 2] = fs.z;
+				W.gbuf[gi + 3] = to.x;
+				W.gbuf[gi + 4] = to.y;
+				W.gbuf[gi + 5] = to.z;
 			}
 			else
-				W.gbuf[gi] = dot(v3(acc[3], acc[4], acc[5]), ld3(W.fax + 3 * f));
+				W.gbuf[gi] = dot(to, ld3(W.fax + 3 * f));
 		}
 		lsync();
 	}

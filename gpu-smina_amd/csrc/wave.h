@@ -78,6 +78,32 @@ __device__ inline float wave_sum(float v)
 			(__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 32)) +
 					__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 48)));
 }
+// DPP with row and bank write masks; lanes masked off, and sources outside
+// the row when bound_ctrl is set, give 0
+template<int CTRL, int ROW_MASK, int BANK_MASK, bool BOUND_ZERO>
+__device__ inline float dpp_masked(float v)
+{
+	return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, ROW_MASK,
+											 BANK_MASK, BOUND_ZERO));
+}
+constexpr int DPP_ROW_SHR1 = 0x111, DPP_ROW_SHR2 = 0x112, DPP_ROW_SHR3 = 0x113, DPP_ROW_SHR4 = 0x114,
+			  DPP_ROW_SHR8 = 0x118, DPP_ROW_BCAST15 = 0x142, DPP_ROW_BCAST31 = 0x143;
+
+// inclusive prefix sum over the 64 lanes: sums of 4 within each row from
+// the input, widened to 8 and 16 with bank masks, then the row totals
+// carried into the next rows (row_bcast).  The whole wave must be active.
+__device__ inline float wave_inclusive_scan(float v)
+{
+	float s = v + dpp_masked<DPP_ROW_SHR1, 0xF, 0xF, true>(v);
+	s += dpp_masked<DPP_ROW_SHR2, 0xF, 0xF, true>(v);
+	s += dpp_masked<DPP_ROW_SHR3, 0xF, 0xF, true>(v);
+	s += dpp_masked<DPP_ROW_SHR4, 0xF, 0xE, false>(s);
+	s += dpp_masked<DPP_ROW_SHR8, 0xF, 0xC, false>(s);
+	s += dpp_masked<DPP_ROW_BCAST15, 0xA, 0xF, false>(s);
+	s += dpp_masked<DPP_ROW_BCAST31, 0xC, 0xF, false>(s);
+	return s;
+}
+
 // double: butterfly through lane permutes (the ordered search never uses it)
 __device__ inline double wave_sum(double v)
 {
@@ -216,6 +242,14 @@ template<typename T> __device__ inline Q4<T> rotation_q(V3<T> rot)
 		return axis_angle_q(axis, angle);
 	}
 	return {T(1), T(0), T(0), T(0)};
+}
+// v rotated by the unit quaternion q (the matrix of q_to_mat times v):
+// v + 2a (u x v) + 2 u x (u x v), u the vector part
+template<typename T> __device__ inline V3<T> q_rotate(Q4<T> q, V3<T> v)
+{
+	const T tx = 2 * (q.c * v.z - q.d * v.y), ty = 2 * (q.d * v.x - q.b * v.z), tz = 2 * (q.b * v.y - q.c * v.x);
+	return {v.x + q.a * tx + (q.c * tz - q.d * ty), v.y + q.a * ty + (q.d * tx - q.b * tz),
+			v.z + q.a * tz + (q.b * ty - q.c * tx)};
 }
 // quaternion_normalize_approx (src/lib/quaternion.h:69-80)
 template<typename T> __device__ inline void q_normalize_approx(Q4<T>& q)
