@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -1227,6 +1228,16 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 			}
 		}
 		require(k.waves > 0, "chain state exceeds LDS");
+		if (std::getenv("SGPU_LOG_PLAN"))
+		{
+			const int bytes = plan_bytes(k.plan, b.precision, k.waves);
+			std::fprintf(stderr,
+					"[sgpu] mc class %d: %zu models, %d waves/workgroup, %d LDS bytes/workgroup, %d resident "
+					"waves/CU (atoms %d, pairs %d, frames %d, change %d)\n",
+					int(&k - cls), k.models.size(), k.waves, bytes,
+					k.waves * std::min(LDS_LIMIT / bytes, 8 / k.waves), k.plan.nmax, k.plan.pmax, k.plan.fmax,
+					k.plan.gmax);
+		}
 		for (int mi : k.models)
 		{
 			const std::vector<int>& cs = by_model[mi];

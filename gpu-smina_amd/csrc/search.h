@@ -78,8 +78,7 @@ struct WaveMem
 	T* gbuf;
 	T* ebuf;
 	T* xstage;
-	T* pforce; // pair forces (pair-parallel search only)
-	T* tsum;   // prefix sums of the atoms' torque terms (pair-parallel search only)
+	T* pforce; // pair forces, then prefix sums of the atoms' torque terms (pair-parallel search only)
 };
 
 // The search's own vectors in the search precision S: the staged conf and
@@ -100,7 +99,7 @@ template<typename T>
 __host__ __device__ inline int eval_lds_elems(const WavePlan& p)
 {
 	return 3 * p.amax + 3 * p.nmax + p.fmax * (3 + 9 + 4 + 3 + 3 + 3) + p.gmax + p.nmax + p.cmax +
-			(pair_parallel<T>() ? 3 * p.pmax + 3 * p.nmax : 0);
+			(pair_parallel<T>() ? 3 * (p.pmax > p.nmax ? p.pmax : p.nmax) : 0);
 }
 template<typename T, typename S>
 __host__ __device__ inline int search_lds_elems(const WavePlan& p)
@@ -154,8 +153,6 @@ __device__ inline void carve(unsigned char* base, const WavePlan& p, WaveMem<T>&
 	w.xstage = q;
 	q += p.cmax;
 	w.pforce = q;
-	q += pair_parallel<T>() ? 3 * p.pmax : 0;
-	w.tsum = q;
 	S* r = reinterpret_cast<S*>(base + (eval_lds_elems<T>(p) * int(sizeof(T)) + 15) / 16 * 16);
 	sm.pstage = r;
 	r += p.gmax;
@@ -798,56 +795,62 @@ __device__ __forceinline__ T eval_deriv(const MV<T>& m, const EvalCtx<T>& ec, Wa
 			}
 		}
 		lsync();
-		// Per movable atom: receptor term, then its pair forces in list order;
+		// per movable atom: receptor term, then its pair forces in list order;
 		// the map gather is issued first and consumed last, so its latency
-		// overlaps the list walk.  Then inclusive prefix sums over the atoms
-		// of the force and of (x - o0) x force (o0: frame 0's origin) go to
-		// LDS: every frame's subtree is a contiguous atom range, so the
-		// derivative reads its sums as two differences.
+		// overlaps the list walk
+		for (int i = lane; i < N; i += WAVE)
+		{
+			const int t = m.type[i];
+			const bool heavy = !is_hydrogen_type(t);
+			const V3<T> xi = ld3(W.coords + 3 * i);
+			GridSample<T> gs;
+			if constexpr (BK == BK_MAPS)
+				gs = grid_fetch(ec.grid, heavy ? m.slot[i] : 0, xi);
+			V3<T> fp = v3(T(0), T(0), T(0));
+			const int k0 = m.nbr_start[i], k1 = m.nbr_start[i + 1];
+			for (int k = k0; k < k1; k += GATHER_BATCH)
+			{
+				int cd[GATHER_BATCH];
+#pragma unroll
+				for (int u = 0; u < GATHER_BATCH; u++)
+					cd[u] = m.code[k + u < k1 ? k + u : k0];
+				V3<T> f[GATHER_BATCH];
+#pragma unroll
+				for (int u = 0; u < GATHER_BATCH; u++)
+					f[u] = ld3(W.pforce + 3 * code_pair(cd[u]));
+#pragma unroll
+				for (int u = 0; u < GATHER_BATCH; u++)
+				{
+					const T sg = k + u >= k1 ? T(0) : (code_is_b(cd[u]) ? T(1) : T(-1));
+					fp = v3(fp.x + sg * f[u].x, fp.y + sg * f[u].y, fp.z + sg * f[u].z);
+				}
+			}
+			V3<T> d = v3(T(0), T(0), T(0));
+			if (heavy)
+			{
+				if constexpr (BK == BK_MAPS)
+					e_part += grid_finish(ec.grid, gs, ec.v[1], d);
+				else
+					e_part += direct_term(ec, t, xi, d);
+			}
+			st3(W.forces + 3 * i, d + fp);
+		}
+		lsync();
+		// Inclusive prefix sums over the atoms of the force (in place) and of
+		// (x - o0) x force (o0: frame 0's origin; into the pair-force
+		// buffer, free now): every frame's subtree is a contiguous atom range,
+		// so the derivative reads its sums as two differences
 		const V3<T> o0 = ld3(W.fo);
 		V3<T> carry_f = v3(T(0), T(0), T(0)), carry_t = carry_f;
 		for (int base = 0; base < N; base += WAVE)
 		{
 			const int i = base + lane;
-			V3<T> force = v3(T(0), T(0), T(0)), xi = force;
+			V3<T> force = v3(T(0), T(0), T(0)), tq = force;
 			if (i < N)
 			{
-				const int t = m.type[i];
-				const bool heavy = !is_hydrogen_type(t);
-				xi = ld3(W.coords + 3 * i);
-				GridSample<T> gs;
-				if constexpr (BK == BK_MAPS)
-					gs = grid_fetch(ec.grid, heavy ? m.slot[i] : 0, xi);
-				V3<T> fp = v3(T(0), T(0), T(0));
-				const int k0 = m.nbr_start[i], k1 = m.nbr_start[i + 1];
-				for (int k = k0; k < k1; k += GATHER_BATCH)
-				{
-					int cd[GATHER_BATCH];
-#pragma unroll
-					for (int u = 0; u < GATHER_BATCH; u++)
-						cd[u] = m.code[k + u < k1 ? k + u : k0];
-					V3<T> f[GATHER_BATCH];
-#pragma unroll
-					for (int u = 0; u < GATHER_BATCH; u++)
-						f[u] = ld3(W.pforce + 3 * code_pair(cd[u]));
-#pragma unroll
-					for (int u = 0; u < GATHER_BATCH; u++)
-					{
-						const T sg = k + u >= k1 ? T(0) : (code_is_b(cd[u]) ? T(1) : T(-1));
-						fp = v3(fp.x + sg * f[u].x, fp.y + sg * f[u].y, fp.z + sg * f[u].z);
-					}
-				}
-				V3<T> d = v3(T(0), T(0), T(0));
-				if (heavy)
-				{
-					if constexpr (BK == BK_MAPS)
-						e_part += grid_finish(ec.grid, gs, ec.v[1], d);
-					else
-						e_part += direct_term(ec, t, xi, d);
-				}
-				force = d + fp;
+				force = ld3(W.forces + 3 * i);
+				tq = cross(ld3(W.coords + 3 * i) - o0, force);
 			}
-			const V3<T> tq = cross(xi - o0, force);
 			const V3<T> sf = v3(wave_inclusive_scan(force.x), wave_inclusive_scan(force.y),
 							  wave_inclusive_scan(force.z)) + carry_f;
 			const V3<T> st = v3(wave_inclusive_scan(tq.x), wave_inclusive_scan(tq.y), wave_inclusive_scan(tq.z)) +
@@ -855,7 +858,7 @@ __device__ __forceinline__ T eval_deriv(const MV<T>& m, const EvalCtx<T>& ec, Wa
 			if (i < N)
 			{
 				st3(W.forces + 3 * i, sf);
-				st3(W.tsum + 3 * i, st);
+				st3(W.pforce + 3 * i, st);
 			}
 			carry_f = v3(readlane(sf.x, WAVE - 1), readlane(sf.y, WAVE - 1), readlane(sf.z, WAVE - 1));
 			carry_t = v3(readlane(st.x, WAVE - 1), readlane(st.y, WAVE - 1), readlane(st.z, WAVE - 1));
@@ -899,11 +902,11 @@ __device__ __forceinline__ T eval_deriv(const MV<T>& m, const EvalCtx<T>& ec, Wa
 		if (f < F)
 		{
 			const int b = m.fbegin[f], en = m.fsend[f];
-			V3<T> fs = ld3(W.forces + 3 * (en - 1)), ts = ld3(W.tsum + 3 * (en - 1));
+			V3<T> fs = ld3(W.forces + 3 * (en - 1)), ts = ld3(W.pforce + 3 * (en - 1));
 			if (b > 0)
 			{
 				fs = fs - ld3(W.forces + 3 * (b - 1));
-				ts = ts - ld3(W.tsum + 3 * (b - 1));
+				ts = ts - ld3(W.pforce + 3 * (b - 1));
 			}
 			const V3<T> to = ts - cross(ld3(W.fo + 3 * f) - ld3(W.fo), fs);
 			const int gi = m.fchange[f];
