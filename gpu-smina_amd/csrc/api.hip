@@ -1196,6 +1196,7 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 		std::vector<int> models;
 		WavePlan plan;
 		int waves = 1;
+		int resident = 0; // waves per CU
 		std::vector<Blk> blks;
 		size_t block_off = 0, slot_off = 0;
 	};
@@ -1211,22 +1212,24 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 		int maxper = 1;
 		for (int mi : k.models)
 			maxper = std::max(maxper, int(by_model[mi].size()));
-		// chains per workgroup: the most resident waves per CU (8 at two
-		// waves per SIMD, the register limit), then the larger workgroup
+		// chains per workgroup: the most resident waves per CU (registers
+		// and LDS, from the runtime's occupancy calculator), then the larger
+		// workgroup
 		k.waves = 0;
 		int best = 0;
-		for (int w = 1; w <= std::min(8, maxper); w++)
+		for (int w = 1; w <= std::min(MAX_CHAINS_PER_GROUP, maxper); w++)
 		{
 			const int bytes = plan_bytes(k.plan, b.precision, w);
 			if (bytes > LDS_LIMIT)
 				break;
-			const int resident = w * std::min(LDS_LIMIT / bytes, 8 / w);
-			if (resident >= best)
+			const int resident = w * mc_blocks_per_cu<T, S>(w, size_t(bytes));
+			if (resident >= best && resident > 0)
 			{
 				best = resident;
 				k.waves = w;
 			}
 		}
+		k.resident = best;
 		require(k.waves > 0, "chain state exceeds LDS");
 		if (std::getenv("SGPU_LOG_PLAN"))
 		{
@@ -1235,7 +1238,7 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 					"[sgpu] mc class %d: %zu models, %d waves/workgroup, %d LDS bytes/workgroup, %d resident "
 					"waves/CU (atoms %d, pairs %d, frames %d, change %d)\n",
 					int(&k - cls), k.models.size(), k.waves, bytes,
-					k.waves * std::min(LDS_LIMIT / bytes, 8 / k.waves), k.plan.nmax, k.plan.pmax, k.plan.fmax,
+					k.resident, k.plan.nmax, k.plan.pmax, k.plan.fmax,
 					k.plan.gmax);
 		}
 		for (int mi : k.models)

@@ -26,6 +26,10 @@ template<typename T, typename S>
 hipError_t launch_mc(const BatchPtrs<T>& B, const EvalCtx<T>& ec, const WavePlan& plan, const McParams& P,
 		const ChainLayout& L, char* chains, int nblocks, int waves, const int* block_model, const int* slot_chain,
 		const int* chain_steps, const int* chain_maxiters, int step_lo, int step_hi, hipStream_t st);
+// workgroups of `waves` chains with `lds` bytes of LDS that one CU holds at
+// once (the runtime's occupancy calculator: registers and LDS)
+template<typename T, typename S>
+int mc_blocks_per_cu(int waves, size_t lds);
 template<typename T, typename S>
 hipError_t launch_refine(const BatchPtrs<T>& B, const EvalCtx<T>& ec, const WavePlan& plan, int n, const int* model,
 		S* conf, int conf_stride, const int* maxiters, int accurate, int early_term, S* e, T* x, int x_stride,
