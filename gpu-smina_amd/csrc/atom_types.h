@@ -67,6 +67,51 @@ SGPU_HD AtomTypeInfo atom_type_info(int t)
 	}
 }
 
+// vinardo's own atom parameters (src/lib/builtinscoring.cpp:6-62, installed
+// in place of the default table when vinardo scoring is chosen): other
+// X-Score radii, and aromatic non-hydrophobe carbon and sulfur acceptor
+// count as hydrophobes
+SGPU_HD AtomTypeInfo atom_type_info_vinardo(int t)
+{
+	// xs_radius, xs hydrophobe, xs donor, xs acceptor, hydrogen, heteroatom
+	switch (t)
+	{
+	case T_Hydrogen: return {0.0, false, false, false, true, false};
+	case T_PolarHydrogen: return {0.0, false, false, false, true, false};
+	case T_CAliph_H: return {2.0, true, false, false, false, false};
+	case T_CAliph_P: return {2.0, false, false, false, false, false};
+	case T_CArom_H: return {1.9, true, false, false, false, false};
+	case T_CArom_P: return {1.9, true, false, false, false, false};
+	case T_N_P: return {1.7, false, false, false, false, true};
+	case T_N_D: return {1.7, false, true, false, false, true};
+	case T_N_DA: return {1.7, false, true, true, false, true};
+	case T_N_A: return {1.7, false, false, true, false, true};
+	case T_O_P: return {1.6, false, false, false, false, true};
+	case T_O_D: return {1.6, false, true, false, false, true};
+	case T_O_DA: return {1.6, false, true, true, false, true};
+	case T_O_A: return {1.6, false, false, true, false, true};
+	case T_S_P: return {2.0, false, false, false, false, true};
+	case T_S_A: return {2.0, true, false, false, false, true};
+	case T_P_P: return {2.1, false, false, false, false, true};
+	case T_F_H: return {1.5, true, false, false, false, true};
+	case T_Cl_H: return {1.8, true, false, false, false, true};
+	case T_Br_H: return {2.0, true, false, false, false, true};
+	case T_I_H: return {2.2, true, false, false, false, true};
+	default: return {1.2, false, true, false, false, true}; // metals
+	}
+}
+
+// the parameter set of a scoring function (sgpu_scoring::atom_params)
+enum AtomParams
+{
+	ATOM_PARAMS_DEFAULT = 0,
+	ATOM_PARAMS_VINARDO = 1
+};
+SGPU_HD AtomTypeInfo atom_type_info(int t, int params)
+{
+	return params == ATOM_PARAMS_VINARDO ? atom_type_info_vinardo(t) : atom_type_info(t);
+}
+
 SGPU_HD bool is_hydrogen_type(int t) { return t == T_Hydrogen || t == T_PolarHydrogen; }
 
 // triangular_matrix_index (src/lib/triangular_matrix_index.h) of i <= j

@@ -33,8 +33,12 @@ const double kMax = std::numeric_limits<double>::max();
 
 inline double sq(double x) { return x * x; }
 
-// ---- terms (src/lib/everything.h:156-541), charge independent
-double optimal_distance(int t1, int t2) { return atom_type_info(t1).xs_radius + atom_type_info(t2).xs_radius; }
+// ---- terms (src/lib/everything.h:156-541), charge independent, on the
+// scoring function's atom parameter set
+double optimal_distance(int t1, int t2, int ap)
+{
+	return atom_type_info(t1, ap).xs_radius + atom_type_info(t2, ap).xs_radius;
+}
 double slope_step(double bad, double good, double x)
 {
 	if (bad < good)
@@ -53,14 +57,14 @@ double slope_step(double bad, double good, double x)
 	}
 	return (x - bad) / (good - bad);
 }
-bool h_bond_possible(int t1, int t2)
+bool h_bond_possible(int t1, int t2, int ap)
 {
-	const AtomTypeInfo a = atom_type_info(t1), b = atom_type_info(t2);
+	const AtomTypeInfo a = atom_type_info(t1, ap), b = atom_type_info(t2, ap);
 	return (a.donor && b.acceptor) || (b.donor && a.acceptor);
 }
-bool anti_h_bond(int t1, int t2)
+bool anti_h_bond(int t1, int t2, int ap)
 {
-	const AtomTypeInfo a = atom_type_info(t1), b = atom_type_info(t2);
+	const AtomTypeInfo a = atom_type_info(t1, ap), b = atom_type_info(t2, ap);
 	if (a.donor && !a.acceptor)
 		return b.donor && !b.acceptor;
 	if (!a.donor && a.acceptor)
@@ -88,33 +92,33 @@ double lj(double r, double d0, double depth, int i, int j, double cap)
 		return std::min(cap, ci / ri + cj / rj);
 	return cap;
 }
-double term(const sgpu_term& t, int t1, int t2, double r)
+double term(const sgpu_term& t, int t1, int t2, double r, int ap)
 {
-	const AtomTypeInfo a = atom_type_info(t1), b = atom_type_info(t2);
+	const AtomTypeInfo a = atom_type_info(t1, ap), b = atom_type_info(t2, ap);
 	switch (t.kind)
 	{
 	case SGPU_TERM_GAUSS:
 	{
-		const double x = r - (optimal_distance(t1, t2) + t.p[0]);
+		const double x = r - (optimal_distance(t1, t2, ap) + t.p[0]);
 		return std::exp(-sq(x / t.p[1]));
 	}
 	case SGPU_TERM_REPULSION:
-		return quadratic_below(r, optimal_distance(t1, t2) + t.p[0]);
+		return quadratic_below(r, optimal_distance(t1, t2, ap) + t.p[0]);
 	case SGPU_TERM_HYDROPHOBIC:
-		return (a.hydrophobe && b.hydrophobe) ? slope_step(t.p[1], t.p[0], r - optimal_distance(t1, t2)) : 0.0;
+		return (a.hydrophobe && b.hydrophobe) ? slope_step(t.p[1], t.p[0], r - optimal_distance(t1, t2, ap)) : 0.0;
 	case SGPU_TERM_NON_HYDROPHOBIC:
-		return (!a.hydrophobe && !b.hydrophobe) ? slope_step(t.p[1], t.p[0], r - optimal_distance(t1, t2)) : 0.0;
+		return (!a.hydrophobe && !b.hydrophobe) ? slope_step(t.p[1], t.p[0], r - optimal_distance(t1, t2, ap)) : 0.0;
 	case SGPU_TERM_NON_DIR_H_BOND:
-		return h_bond_possible(t1, t2) ? slope_step(t.p[1], t.p[0], r - optimal_distance(t1, t2)) : 0.0;
+		return h_bond_possible(t1, t2, ap) ? slope_step(t.p[1], t.p[0], r - optimal_distance(t1, t2, ap)) : 0.0;
 	case SGPU_TERM_NON_DIR_ANTI_H_BOND_QUADRATIC:
-		return anti_h_bond(t1, t2) ? quadratic_below(r, optimal_distance(t1, t2) + t.p[0]) : 0.0;
+		return anti_h_bond(t1, t2, ap) ? quadratic_below(r, optimal_distance(t1, t2, ap) + t.p[0]) : 0.0;
 	case SGPU_TERM_DONOR_DONOR_QUADRATIC:
-		return (a.donor && b.donor) ? quadratic_below(r, optimal_distance(t1, t2) + t.p[0]) : 0.0;
+		return (a.donor && b.donor) ? quadratic_below(r, optimal_distance(t1, t2, ap) + t.p[0]) : 0.0;
 	case SGPU_TERM_ACCEPTOR_ACCEPTOR_QUADRATIC:
-		return (a.acceptor && b.acceptor) ? quadratic_below(r, optimal_distance(t1, t2) + t.p[0]) : 0.0;
+		return (a.acceptor && b.acceptor) ? quadratic_below(r, optimal_distance(t1, t2, ap) + t.p[0]) : 0.0;
 	case SGPU_TERM_VDW:
 	{
-		const double d0 = optimal_distance(t1, t2), s = t.p[2];
+		const double d0 = optimal_distance(t1, t2, ap), s = t.p[2];
 		double rr = r;
 		if (rr > d0 + s)
 			rr -= s;
@@ -125,23 +129,53 @@ double term(const sgpu_term& t, int t1, int t2, double r)
 		return lj(rr, d0, 1, int(t.p[0]), int(t.p[1]), t.p[3]);
 	}
 	case SGPU_TERM_NON_DIR_H_BOND_LJ:
-		return h_bond_possible(t1, t2) ? lj(r, optimal_distance(t1, t2) + t.p[0], 5, 10, 12, t.p[1]) : 0.0;
+		return h_bond_possible(t1, t2, ap) ? lj(r, optimal_distance(t1, t2, ap) + t.p[0], 5, 10, 12, t.p[1]) : 0.0;
 	}
 	throw std::runtime_error("unknown scoring term");
 }
 
-// conf_independent with num_tors_div (src/lib/everything.h:808-830)
-double conf_independent(const sgpu_scoring& sf, double num_tors, double x)
+// smooth_div (src/lib/everything.cpp)
+double smooth_div(double x, double y)
 {
-	if (!sf.has_num_tors_div)
-		return x;
-	const double w = 0.1 * (sf.num_tors_div_weight + 1);
-	const double y = 1 + w * num_tors / 5.0;
 	if (std::abs(x) < kEps)
 		return 0;
 	if (std::abs(y) < kEps)
 		return (x * y > 0) ? kMax : -kMax;
 	return x / y;
+}
+
+// weighted_terms::conf_independent (src/lib/weighted_terms.cpp:74-80): the
+// conf-independent terms in order (src/lib/everything.h:730-856, :980-1002)
+double conf_independent(const sgpu_scoring& sf, double num_tors, double x)
+{
+	for (int k = 0; k < sf.nci; k++)
+	{
+		const double w = sf.ci[k].weight;
+		switch (sf.ci[k].kind)
+		{
+		case SGPU_CI_NUM_TORS_DIV:
+			x = smooth_div(x, 1 + 0.1 * (w + 1) * num_tors / 5.0);
+			break;
+		case SGPU_CI_NUM_TORS_ADD:
+			x = x + w * num_tors;
+			break;
+		case SGPU_CI_NUM_TORS_SQR:
+			x = x + 0.1 * w * sq(num_tors) / 5;
+			break;
+		case SGPU_CI_NUM_TORS_SQRT:
+			x = x + 0.1 * w * std::sqrt(num_tors) / std::sqrt(5.0);
+			break;
+		case SGPU_CI_NUM_TORS_DIV_SIMPLE:
+			x = smooth_div(x, 1 + w * num_tors);
+			break;
+		case SGPU_CI_CONSTANT:
+			x = x + w;
+			break;
+		default:
+			throw std::runtime_error("unknown conf-independent term");
+		}
+	}
+	return x;
 }
 
 // ---- chain initialisation randoms, on the product's MT19937
@@ -186,7 +220,7 @@ Tables build_tables(const sgpu_scoring& sf, double factor)
 			{
 				double acc = 0;
 				for (int k = 0; k < sf.nterms; k++)
-					acc += sf.terms[k].weight * term(sf.terms[k], t1, t2, rs[i]);
+					acc += sf.terms[k].weight * term(sf.terms[k], t1, t2, rs[i], sf.atom_params);
 				e[i] = acc;
 			}
 			const size_t base = size_t(tri_index(t1, t2)) * n;
@@ -581,8 +615,10 @@ GpuDocking::GpuDocking(const GpuSettings& s, const sgpu_scoring& sf, const sgpu_
 		s_(s), gd_(gd)
 {
 	terms_.assign(sf.terms, sf.terms + sf.nterms);
+	ci_.assign(sf.ci, sf.ci + sf.nci);
 	sf_ = sf;
 	sf_.terms = terms_.data();
+	sf_.ci = ci_.data();
 	sgpu_status st = sgpu_create(s.device, &ctx_);
 	if (st != SGPU_OK)
 		throw std::runtime_error("sgpu_create failed: no HIP device");
@@ -602,8 +638,10 @@ GpuDocking::GpuDocking(const GpuSettings& s, const sgpu_scoring& sf, const sgpu_
 	ctx_ = shared;
 	owns_ctx_ = false;
 	terms_.assign(sf.terms, sf.terms + sf.nterms);
+	ci_.assign(sf.ci, sf.ci + sf.nci);
 	sf_ = sf;
 	sf_.terms = terms_.data();
+	sf_.ci = ci_.data();
 	Tables t = build_tables(sf_, 32);
 	sgpu_tables tt{t.n, t.factor, t.cutoff_sqr, t.smooth.data(), t.fast.data()};
 	check(ctx_, sgpu_upload_tables(ctx_, &tt));
@@ -1049,21 +1087,68 @@ struct sgpu_docking
 	std::unique_ptr<GpuDocking> d;
 	std::string err;
 	std::vector<sgpu_term> terms;
+	std::vector<sgpu_ci_term> ci;
 };
 
 extern "C" {
 
-void sgpu_vina_scoring(sgpu_term* t, sgpu_scoring* out, double hydrophobic_scale)
-{ // src/main/main.cpp:1347-1353
-	t[0] = {SGPU_TERM_GAUSS, {0, 0.5, 8, 0, 0}, -0.035579};
-	t[1] = {SGPU_TERM_GAUSS, {3, 2, 8, 0, 0}, -0.005156};
-	t[2] = {SGPU_TERM_REPULSION, {0, 8, 0, 0, 0}, 0.840245};
-	t[3] = {SGPU_TERM_HYDROPHOBIC, {0.5, 1.5, 8, 0, 0}, -0.035069 * hydrophobic_scale};
-	t[4] = {SGPU_TERM_NON_DIR_H_BOND, {-0.7, 0, 8, 0, 0}, -0.587439};
-	out->nterms = 5;
+int sgpu_builtin_scoring(const char* name, double hydrophobic_scale, sgpu_term* t, sgpu_ci_term* ci,
+		sgpu_scoring* out)
+{ // builtin_scoring (src/lib/builtinscoring.cpp:65-113), terms in its order
+	const std::string n = name ? name : "vina";
+	int k = 0, c = 0;
+	out->atom_params = SGPU_ATOM_PARAMS_DEFAULT;
+	if (n == "vina" || n == "default")
+	{ // src/main/main.cpp:1347-1353
+		t[k++] = {SGPU_TERM_GAUSS, {0, 0.5, 8, 0, 0}, -0.035579};
+		t[k++] = {SGPU_TERM_GAUSS, {3, 2, 8, 0, 0}, -0.005156};
+		t[k++] = {SGPU_TERM_REPULSION, {0, 8, 0, 0, 0}, 0.840245};
+		t[k++] = {SGPU_TERM_HYDROPHOBIC, {0.5, 1.5, 8, 0, 0}, -0.035069 * hydrophobic_scale};
+		t[k++] = {SGPU_TERM_NON_DIR_H_BOND, {-0.7, 0, 8, 0, 0}, -0.587439};
+		ci[c++] = {SGPU_CI_NUM_TORS_DIV, 5 * 0.05846 / 0.1 - 1};
+	}
+	else if (n == "vinardo")
+	{
+		t[k++] = {SGPU_TERM_GAUSS, {0, 0.8, 8, 0, 0}, -0.045};
+		t[k++] = {SGPU_TERM_REPULSION, {0, 8, 0, 0, 0}, 0.80};
+		t[k++] = {SGPU_TERM_HYDROPHOBIC, {0.0, 2.5, 8, 0, 0}, -0.035 * hydrophobic_scale};
+		t[k++] = {SGPU_TERM_NON_DIR_H_BOND, {-0.6, 0, 8, 0, 0}, -0.60};
+		ci[c++] = {SGPU_CI_NUM_TORS_DIV, 5 * 0.02 / 0.1 - 1};
+		out->atom_params = SGPU_ATOM_PARAMS_VINARDO;
+	}
+	else if (n == "dkoes_fast")
+	{
+		t[k++] = {SGPU_TERM_VDW, {4, 8, 0, 100, 8}, 0.008962};
+		t[k++] = {SGPU_TERM_NON_DIR_H_BOND, {-0.7, 0, 8, 0, 0}, 0.387739};
+		ci[c++] = {SGPU_CI_NUM_TORS_SQR, .285035};
+		ci[c++] = {SGPU_CI_CONSTANT, -2.467357};
+	}
+	else if (n == "dkoes_scoring_old")
+	{
+		t[k++] = {SGPU_TERM_VDW, {4, 8, 0, 100, 8}, 0.010607};
+		t[k++] = {SGPU_TERM_NON_DIR_H_BOND, {-0.7, 0, 8, 0, 0}, 0.197201};
+		ci[c++] = {SGPU_CI_NUM_TORS_SQR, .285035};
+		ci[c++] = {SGPU_CI_CONSTANT, -2.585651};
+	}
+	else if (n == "dkoes_scoring" || n == "ad4_scoring")
+		return -2;
+	else
+		return -1;
+	out->nterms = k;
 	out->terms = t;
-	out->has_num_tors_div = 1;
-	out->num_tors_div_weight = 5 * 0.05846 / 0.1 - 1;
+	out->nci = c;
+	out->ci = ci;
+	return 0;
+}
+
+double sgpu_conf_independent(const sgpu_scoring* sf, double num_tors, double e)
+{
+	return conf_independent(*sf, num_tors, e);
+}
+
+void sgpu_vina_scoring(sgpu_term* t, sgpu_ci_term* ci, sgpu_scoring* out, double hydrophobic_scale)
+{
+	sgpu_builtin_scoring("vina", hydrophobic_scale, t, ci, out);
 }
 
 int sgpu_build_tables(const sgpu_scoring* sf, double factor, double* smooth, double* fast)
@@ -1130,9 +1215,13 @@ sgpu_docking* sgpu_docking_create(const sgpu_dock_settings* s, const sgpu_mol* r
 		gs.progress_chunks = s->progress_chunks > 0 ? s->progress_chunks : 1;
 		gs.progress = s->progress;
 		gs.progress_user = s->progress_user;
-		d->terms.resize(5);
+		d->terms.resize(SGPU_MAX_TERMS);
+		d->ci.resize(SGPU_MAX_CI);
 		sgpu_scoring sf;
-		sgpu_vina_scoring(d->terms.data(), &sf, s->hydrophobic_scale);
+		const int rc = sgpu_builtin_scoring(s->scoring, s->hydrophobic_scale, d->terms.data(), d->ci.data(), &sf);
+		if (rc != 0)
+			throw std::invalid_argument(rc == -2 ? "scoring function needs atom charges, which this path does not carry"
+			                                     : "unknown scoring function");
 		const sgpu_model_desc* rd = sgpu_mol_desc(receptor);
 		sgpu_receptor rec{rd->natoms, rd->xyz, rd->type};
 		d->d.reset(new GpuDocking(gs, sf, rec, *gd));

@@ -154,6 +154,7 @@ private:
 	bool owns_ctx_ = true;
 	sgpu_scoring sf_;
 	std::vector<sgpu_term> terms_;
+	std::vector<sgpu_ci_term> ci_;
 	sgpu_grid_dims gd_;
 	int receptor_ = -1;
 	int grid_ = -1;

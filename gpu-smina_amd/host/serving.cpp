@@ -12,8 +12,10 @@ DockingServer::DockingServer(const ServerSettings& s, const sgpu_scoring& sf) : 
 	if (s_.workers < 1 || s_.max_batch < 1 || !(s_.budget_bytes > 0))
 		throw std::invalid_argument("server: workers, max_batch and budget must be positive");
 	terms_.assign(sf.terms, sf.terms + sf.nterms);
+	ci_.assign(sf.ci, sf.ci + sf.nci);
 	sf_ = sf;
 	sf_.terms = terms_.data();
+	sf_.ci = ci_.data();
 	t0_ns_ = std::chrono::duration_cast<std::chrono::nanoseconds>(
 			std::chrono::steady_clock::now().time_since_epoch()).count();
 	for (int k = 0; k < s_.workers; k++)
@@ -213,6 +215,7 @@ struct sgpu_server
 {
 	std::unique_ptr<DockingServer> s;
 	std::vector<sgpu_term> terms;
+	std::vector<sgpu_ci_term> ci;
 	std::string err;
 };
 
@@ -235,9 +238,13 @@ sgpu_server* sgpu_server_create(const sgpu_server_settings* s)
 		ss.dock.seed = s->seed;
 		ss.dock.num_modes = s->num_modes;
 		ss.dock.energy_range = s->energy_range;
-		sv->terms.resize(5);
+		sv->terms.resize(SGPU_MAX_TERMS);
+		sv->ci.resize(SGPU_MAX_CI);
 		sgpu_scoring sf;
-		sgpu_vina_scoring(sv->terms.data(), &sf, s->hydrophobic_scale);
+		const int rc = sgpu_builtin_scoring(s->scoring, s->hydrophobic_scale, sv->terms.data(), sv->ci.data(), &sf);
+		if (rc != 0)
+			throw std::invalid_argument(rc == -2 ? "scoring function needs atom charges, which this path does not carry"
+			                                     : "unknown scoring function");
 		sv->s.reset(new DockingServer(ss, sf));
 	}
 	catch (std::exception& e)

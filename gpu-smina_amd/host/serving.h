@@ -88,6 +88,7 @@ private:
 
 	ServerSettings s_;
 	std::vector<sgpu_term> terms_;
+	std::vector<sgpu_ci_term> ci_;
 	sgpu_scoring sf_;
 	struct Rec
 	{

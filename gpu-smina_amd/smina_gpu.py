@@ -96,9 +96,13 @@ class Term(C.Structure):
     _fields_ = [("kind", C.c_int), ("p", C.c_double * 5), ("weight", C.c_double)]
 
 
+class CITerm(C.Structure):
+    _fields_ = [("kind", C.c_int), ("weight", C.c_double)]
+
+
 class Scoring(C.Structure):
-    _fields_ = [("nterms", C.c_int), ("terms", C.POINTER(Term)), ("has_num_tors_div", C.c_int),
-                ("num_tors_div_weight", C.c_double)]
+    _fields_ = [("nterms", C.c_int), ("terms", C.POINTER(Term)), ("nci", C.c_int), ("ci", C.POINTER(CITerm)),
+                ("atom_params", C.c_int)]
 
 
 PROGRESS_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_double)
@@ -108,13 +112,13 @@ class DockSettings(C.Structure):
     _fields_ = [("device", C.c_int), ("precision", C.c_int), ("exhaustiveness", C.c_int), ("seed", C.c_int),
                 ("num_modes", C.c_int), ("energy_range", C.c_double), ("hydrophobic_scale", C.c_double),
                 ("ligands_per_launch", C.c_int), ("step_fraction", C.c_double), ("progress_chunks", C.c_int),
-                ("progress", PROGRESS_FN), ("progress_user", C.c_void_p)]
+                ("progress", PROGRESS_FN), ("progress_user", C.c_void_p), ("scoring", C.c_char_p)]
 
 
 class ServerSettings(C.Structure):
     _fields_ = [("device", C.c_int), ("workers", C.c_int), ("max_batch", C.c_int), ("budget_gb", C.c_double),
                 ("precision", C.c_int), ("seed", C.c_int), ("num_modes", C.c_int), ("energy_range", C.c_double),
-                ("hydrophobic_scale", C.c_double)]
+                ("hydrophobic_scale", C.c_double), ("scoring", C.c_char_p)]
 
 
 class ServeResult(C.Structure):
@@ -159,7 +163,10 @@ def _load():
         "sgpu_synth_receptor": (vp, [C.c_uint32, C.c_int, C.c_double]),
         "sgpu_synth_ligand": (vp, [C.c_uint32, C.c_int, C.c_int, F64P]),
         "sgpu_synth_flex": (vp, [vp, vp, C.c_uint32, C.c_int, C.c_int, C.c_int, F64P, C.POINTER(vp)]),
-        "sgpu_vina_scoring": (None, [C.POINTER(Term), C.POINTER(Scoring), C.c_double]),
+        "sgpu_vina_scoring": (None, [C.POINTER(Term), C.POINTER(CITerm), C.POINTER(Scoring), C.c_double]),
+        "sgpu_conf_independent": (C.c_double, [C.POINTER(Scoring), C.c_double, C.c_double]),
+        "sgpu_builtin_scoring": (C.c_int, [C.c_char_p, C.c_double, C.POINTER(Term), C.POINTER(CITerm),
+                                           C.POINTER(Scoring)]),
         "sgpu_build_tables": (C.c_int, [C.POINTER(Scoring), C.c_double, F64P, F64P]),
         "sgpu_flatten": (vp, [C.POINTER(ModelDesc)]),
         "sgpu_flat_get": (C.POINTER(FlatModel), [vp]),
@@ -250,12 +257,18 @@ class Mol:
         return Mol(h), Mol(rigid.value)
 
 
-def vina_scoring(hydrophobic_scale=1.0):
-    terms = (Term * 5)()
-    sf = Scoring()
-    lib.sgpu_vina_scoring(terms, C.byref(sf), hydrophobic_scale)
-    sf._terms = terms  # keep alive
+def builtin_scoring(name="vina", hydrophobic_scale=1.0):
+    """a built-in scoring function by smina's name (sgpu_builtin_scoring)"""
+    terms, ci, sf = (Term * 8)(), (CITerm * 4)(), Scoring()
+    rc = lib.sgpu_builtin_scoring(name.encode(), hydrophobic_scale, terms, ci, C.byref(sf))
+    if rc != 0:
+        raise ValueError(f"scoring {name!r}: " + ("needs atom charges" if rc == -2 else "unknown"))
+    sf._keep = (terms, ci)  # keep alive
     return sf
+
+
+def vina_scoring(hydrophobic_scale=1.0):
+    return builtin_scoring("vina", hydrophobic_scale)
 
 
 def build_tables(sf, factor=32.0):
@@ -499,7 +512,7 @@ class Docking:
 
     def __init__(self, receptor, gd, fp64=False, exhaustiveness=8, seed=1, num_modes=9, energy_range=3.0,
                  hydrophobic_scale=1.0, ligands_per_launch=0, device=0, step_fraction=1.0, precision=None,
-                 progress=None, progress_chunks=1):
+                 progress=None, progress_chunks=1, scoring="vina"):
         """precision: FP32, FP64 or MIXED (default: FP64 if fp64 else FP32);
         progress(fraction): called after each of progress_chunks launches
         of the Monte Carlo search"""
@@ -507,7 +520,8 @@ class Docking:
         # keep the callback alive as long as the docking object
         self._progress = PROGRESS_FN(lambda user, frac: progress(frac)) if progress else PROGRESS_FN()
         s = DockSettings(device, prec, exhaustiveness, seed, num_modes, energy_range, hydrophobic_scale,
-                         ligands_per_launch, step_fraction, progress_chunks, self._progress, None)
+                         ligands_per_launch, step_fraction, progress_chunks, self._progress, None,
+                         scoring.encode())
         self.gd = gd
         self.h = C.c_void_p(lib.sgpu_docking_create(C.byref(s), receptor.h, C.byref(gd)))
         err = lib.sgpu_docking_error(self.h)
@@ -556,10 +570,10 @@ class Server:
     concurrent launches (sgpu_server_*)."""
 
     def __init__(self, workers=2, max_batch=64, budget_gb=64.0, fp64=False, seed=1, num_modes=9,
-                 energy_range=3.0, hydrophobic_scale=1.0, device=0, precision=None):
+                 energy_range=3.0, hydrophobic_scale=1.0, device=0, precision=None, scoring="vina"):
         prec = precision if precision is not None else (FP64 if fp64 else FP32)
         s = ServerSettings(device, workers, max_batch, budget_gb, prec, seed, num_modes, energy_range,
-                           hydrophobic_scale)
+                           hydrophobic_scale, scoring.encode())
         self.h = C.c_void_p(lib.sgpu_server_create(C.byref(s)))
         err = lib.sgpu_server_error(self.h)
         if err:

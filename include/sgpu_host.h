@@ -45,9 +45,24 @@ sgpu_mol* sgpu_synth_flex(const sgpu_mol* receptor, const sgpu_mol* ligand, uint
 		int max_chi, const double* center, sgpu_mol** rigid);
 
 /* ---- scoring */
-/* the default Vina terms (src/main/main.cpp:1347-1353); hydrophobic_scale
- * multiplies the hydrophobic weight (1 for Vina) */
-void sgpu_vina_scoring(sgpu_term* terms5, sgpu_scoring* out, double hydrophobic_scale);
+/* A built-in scoring function by smina's name (builtin_scoring,
+ * src/lib/builtinscoring.cpp:65-113): "vina" (= "default"), "vinardo",
+ * "dkoes_fast", "dkoes_scoring_old".  hydrophobic_scale multiplies the
+ * weight of the hydrophobic term (1: the built-in weights).  terms and ci
+ * hold at least SGPU_MAX_TERMS and SGPU_MAX_CI entries and back `out`.
+ * Returns 0, or -1 for an unknown name and -2 for the charge-dependent
+ * functions (dkoes_scoring, ad4_scoring), which this path does not carry. */
+#define SGPU_MAX_TERMS 8
+#define SGPU_MAX_CI 4
+int sgpu_builtin_scoring(const char* name, double hydrophobic_scale, sgpu_term* terms, sgpu_ci_term* ci,
+		sgpu_scoring* out);
+/* the final score of a pose (weighted_terms::conf_independent,
+ * src/lib/weighted_terms.cpp:74-80): the conf-independent terms applied in
+ * order to its intermolecular energy e with the ligand's num_tors */
+double sgpu_conf_independent(const sgpu_scoring* sf, double num_tors, double e);
+/* the default Vina terms (src/main/main.cpp:1347-1353): sgpu_builtin_scoring
+ * of "vina" */
+void sgpu_vina_scoring(sgpu_term* terms, sgpu_ci_term* ci, sgpu_scoring* out, double hydrophobic_scale);
 /* precalculate_linear(sf, factor) tables, FP64: smooth [378][n][2], fast
  * [378][n]; returns n (call with NULL buffers to get n) */
 int sgpu_build_tables(const sgpu_scoring* sf, double factor, double* smooth, double* fast);
@@ -69,6 +84,7 @@ typedef struct sgpu_dock_settings
 	int progress_chunks;    /* Monte Carlo search in this many launches (0 or 1: one) */
 	sgpu_progress_fn progress; /* called after each with the fraction done (sgpu_set_progress) */
 	void* progress_user;
+	const char* scoring;    /* --scoring: a built-in function's name (NULL: vina) */
 } sgpu_dock_settings;
 
 sgpu_docking* sgpu_docking_create(const sgpu_dock_settings* s, const sgpu_mol* receptor,
@@ -120,6 +136,7 @@ typedef struct sgpu_server_settings
 	int num_modes;
 	double energy_range;
 	double hydrophobic_scale;
+	const char* scoring;   /* a built-in function's name (NULL: vina) */
 } sgpu_server_settings;
 typedef struct sgpu_serve_result
 {

@@ -70,12 +70,45 @@ typedef struct sgpu_term
 	double weight;
 } sgpu_term;
 
+/* Conf-independent terms (src/lib/everything.h:730-856, :980-1002): the
+ * final score x of a pose becomes f(x, num_tors) with weight w:
+ *   NUM_TORS_DIV         smooth_div(x, 1 + 0.1 (w + 1) num_tors / 5)
+ *   NUM_TORS_ADD         x + w num_tors
+ *   NUM_TORS_SQR         x + 0.1 w num_tors^2 / 5
+ *   NUM_TORS_SQRT        x + 0.1 w sqrt(num_tors) / sqrt(5)
+ *   NUM_TORS_DIV_SIMPLE  smooth_div(x, 1 + w num_tors)
+ *   CONSTANT             x + w */
+typedef enum sgpu_ci_kind
+{
+	SGPU_CI_NUM_TORS_DIV = 0,
+	SGPU_CI_NUM_TORS_ADD = 1,
+	SGPU_CI_NUM_TORS_SQR = 2,
+	SGPU_CI_NUM_TORS_SQRT = 3,
+	SGPU_CI_NUM_TORS_DIV_SIMPLE = 4,
+	SGPU_CI_CONSTANT = 5
+} sgpu_ci_kind;
+
+typedef struct sgpu_ci_term
+{
+	int kind;
+	double weight;
+} sgpu_ci_term;
+
+/* atom parameter sets: smina's default table, or vinardo's
+ * (src/lib/builtinscoring.cpp:6-62, :83) */
+typedef enum sgpu_atom_params
+{
+	SGPU_ATOM_PARAMS_DEFAULT = 0,
+	SGPU_ATOM_PARAMS_VINARDO = 1
+} sgpu_atom_params;
+
 typedef struct sgpu_scoring
 {
 	int nterms;
 	const sgpu_term* terms; /* charge-independent terms, in weight order */
-	int has_num_tors_div;   /* conf-independent num_tors_div term */
-	double num_tors_div_weight;
+	int nci;
+	const sgpu_ci_term* ci; /* conf-independent terms, applied in this order */
+	int atom_params;        /* sgpu_atom_params */
 } sgpu_scoring;
 
 #ifdef __cplusplus

@@ -305,7 +305,13 @@ qt random_orientation(rng& gen)
 // ---------------------------------------------------------------- scoring
 // terms of src/lib/everything.h:156-541, weighted sum of
 // weighted_terms::eval_fast (src/lib/weighted_terms.cpp:52-62)
-inline fl optimal_distance(int t1, int t2) { return atom_type_info(t1).xs_radius + atom_type_info(t2).xs_radius; }
+// atom properties from the scoring function's parameter set: smina's
+// default table, or vinardo's, which builtin_scoring installs in its place
+// (src/lib/builtinscoring.cpp:83)
+inline fl optimal_distance(int t1, int t2, int ap)
+{
+	return atom_type_info(t1, ap).xs_radius + atom_type_info(t2, ap).xs_radius;
+}
 inline fl slope_step(fl x_bad, fl x_good, fl x)
 {
 	if (x_bad < x_good)
@@ -324,14 +330,14 @@ inline fl slope_step(fl x_bad, fl x_good, fl x)
 	}
 	return (x - x_bad) / (x_good - x_bad);
 }
-inline bool xs_h_bond_possible(int t1, int t2)
+inline bool xs_h_bond_possible(int t1, int t2, int ap)
 {
-	return (atom_type_info(t1).donor && atom_type_info(t2).acceptor) ||
-			(atom_type_info(t2).donor && atom_type_info(t1).acceptor);
+	return (atom_type_info(t1, ap).donor && atom_type_info(t2, ap).acceptor) ||
+			(atom_type_info(t2, ap).donor && atom_type_info(t1, ap).acceptor);
 }
-inline bool xs_anti_h_bond(int t1, int t2)
+inline bool xs_anti_h_bond(int t1, int t2, int ap)
 {
-	const auto a = atom_type_info(t1), b = atom_type_info(t2);
+	const auto a = atom_type_info(t1, ap), b = atom_type_info(t2, ap);
 	if (a.donor && !a.acceptor)
 		return b.donor && !b.acceptor;
 	if (!a.donor && a.acceptor)
@@ -345,45 +351,45 @@ inline fl int_pow(fl x, int n)
 		r *= x;
 	return r;
 }
-fl term_eval(const sgpu_term& t, int t1, int t2, fl r)
+fl term_eval(const sgpu_term& t, int t1, int t2, fl r, int ap)
 {
-	const auto a = atom_type_info(t1), b = atom_type_info(t2);
+	const auto a = atom_type_info(t1, ap), b = atom_type_info(t2, ap);
 	switch (t.kind)
 	{
 	case SGPU_TERM_GAUSS: // gauss(o, w): exp(-((r - (d0 + o)) / w)^2) with std::exp
 	{
-		fl x = r - (optimal_distance(t1, t2) + t.p[0]);
+		fl x = r - (optimal_distance(t1, t2, ap) + t.p[0]);
 		return std::exp(-sqr(x / t.p[1]));
 	}
 	case SGPU_TERM_REPULSION:
 	{
-		fl d = r - (optimal_distance(t1, t2) + t.p[0]);
+		fl d = r - (optimal_distance(t1, t2, ap) + t.p[0]);
 		if (d > 0)
 			return 0;
 		return d * d;
 	}
 	case SGPU_TERM_HYDROPHOBIC:
 		if (a.hydrophobe && b.hydrophobe)
-			return slope_step(t.p[1], t.p[0], r - optimal_distance(t1, t2));
+			return slope_step(t.p[1], t.p[0], r - optimal_distance(t1, t2, ap));
 		return 0;
 	case SGPU_TERM_NON_HYDROPHOBIC:
 		if (!a.hydrophobe && !b.hydrophobe)
-			return slope_step(t.p[1], t.p[0], r - optimal_distance(t1, t2));
+			return slope_step(t.p[1], t.p[0], r - optimal_distance(t1, t2, ap));
 		return 0;
 	case SGPU_TERM_NON_DIR_H_BOND:
-		if (xs_h_bond_possible(t1, t2))
-			return slope_step(t.p[1], t.p[0], r - optimal_distance(t1, t2));
+		if (xs_h_bond_possible(t1, t2, ap))
+			return slope_step(t.p[1], t.p[0], r - optimal_distance(t1, t2, ap));
 		return 0;
 	case SGPU_TERM_NON_DIR_ANTI_H_BOND_QUADRATIC:
 	case SGPU_TERM_DONOR_DONOR_QUADRATIC:
 	case SGPU_TERM_ACCEPTOR_ACCEPTOR_QUADRATIC:
 	{
-		bool on = t.kind == SGPU_TERM_NON_DIR_ANTI_H_BOND_QUADRATIC ? xs_anti_h_bond(t1, t2)
+		bool on = t.kind == SGPU_TERM_NON_DIR_ANTI_H_BOND_QUADRATIC ? xs_anti_h_bond(t1, t2, ap)
 				: t.kind == SGPU_TERM_DONOR_DONOR_QUADRATIC ? (a.donor && b.donor)
 				: (a.acceptor && b.acceptor);
 		if (!on)
 			return 0;
-		fl d = r - (optimal_distance(t1, t2) + t.p[0]);
+		fl d = r - (optimal_distance(t1, t2, ap) + t.p[0]);
 		if (d > 0)
 			return 0;
 		return d * d;
@@ -392,7 +398,7 @@ fl term_eval(const sgpu_term& t, int t1, int t2, fl r)
 	{
 		int i = int(t.p[0]), j = int(t.p[1]);
 		fl smoothing = t.p[2], cap = t.p[3];
-		fl d0 = optimal_distance(t1, t2);
+		fl d0 = optimal_distance(t1, t2, ap);
 		fl depth = 1;
 		fl c_i = int_pow(d0, i) * depth * j / (fl(i) - fl(j));
 		fl c_j = int_pow(d0, j) * depth * i / (fl(j) - fl(i));
@@ -409,10 +415,10 @@ fl term_eval(const sgpu_term& t, int t1, int t2, fl r)
 	}
 	case SGPU_TERM_NON_DIR_H_BOND_LJ:
 	{
-		if (!xs_h_bond_possible(t1, t2))
+		if (!xs_h_bond_possible(t1, t2, ap))
 			return 0;
 		fl cap = t.p[1];
-		fl d0 = optimal_distance(t1, t2) + t.p[0];
+		fl d0 = optimal_distance(t1, t2, ap) + t.p[0];
 		fl depth = 5;
 		fl c_i = int_pow(d0, 10) * depth * 12 / (fl(10) - fl(12));
 		fl c_j = int_pow(d0, 12) * depth * 10 / (fl(12) - fl(10));
@@ -425,31 +431,53 @@ fl term_eval(const sgpu_term& t, int t1, int t2, fl r)
 	throw std::runtime_error("unknown term");
 }
 
+// smooth_div (src/lib/everything.cpp)
+inline fl smooth_div(fl x, fl y)
+{
+	if (std::abs(x) < epsilon_fl)
+		return 0;
+	if (std::abs(y) < epsilon_fl)
+		return (x * y > 0) ? max_fl : -max_fl;
+	return x / y;
+}
+
 struct Scoring
 {
 	std::vector<sgpu_term> terms;
-	bool has_num_tors_div = false;
-	fl num_tors_div_weight = 0;
+	std::vector<sgpu_ci_term> ci;
+	int atom_params = 0;
 	fl cutoff = 8;
 	fl eval_fast(int t1, int t2, fl r) const
 	{
 		fl acc = 0;
 		for (const auto& t : terms)
-			acc += t.weight * term_eval(t, t1, t2, r);
+			acc += t.weight * term_eval(t, t1, t2, r, atom_params);
 		return acc;
 	}
-	// weighted_terms::conf_independent with num_tors_div (src/lib/everything.h:808-830)
+	// weighted_terms::conf_independent (src/lib/weighted_terms.cpp:74-80):
+	// each conf-independent term's eval in order (src/lib/everything.h:730-856,
+	// :980-1002)
 	fl conf_independent(fl num_tors, fl x) const
 	{
-		if (!has_num_tors_div)
-			return x;
-		fl w = 0.1 * (num_tors_div_weight + 1);
-		fl y = 1 + w * num_tors / 5.0;
-		if (std::abs(x) < epsilon_fl)
-			return 0;
-		if (std::abs(y) < epsilon_fl)
-			return (x * y > 0) ? max_fl : -max_fl;
-		return x / y;
+		for (const sgpu_ci_term& c : ci)
+		{
+			const fl w = c.weight;
+			if (c.kind == SGPU_CI_NUM_TORS_DIV)
+				x = smooth_div(x, 1 + 0.1 * (w + 1) * num_tors / 5.0);
+			else if (c.kind == SGPU_CI_NUM_TORS_ADD)
+				x = x + w * num_tors;
+			else if (c.kind == SGPU_CI_NUM_TORS_SQR)
+				x = x + 0.1 * w * sqr(num_tors) / 5;
+			else if (c.kind == SGPU_CI_NUM_TORS_SQRT)
+				x = x + 0.1 * w * std::sqrt(num_tors) / std::sqrt(5.0);
+			else if (c.kind == SGPU_CI_NUM_TORS_DIV_SIMPLE)
+				x = smooth_div(x, 1 + w * num_tors);
+			else if (c.kind == SGPU_CI_CONSTANT)
+				x = x + w;
+			else
+				throw std::runtime_error("unknown conf-independent term");
+		}
+		return x;
 	}
 };
 
@@ -1921,8 +1949,8 @@ orc_ctx* orc_create(const sgpu_scoring* sf, const sgpu_receptor* rec, const sgpu
 	orc_ctx* o = new orc_ctx;
 	Ctx& c = o->c;
 	c.sf.terms.assign(sf->terms, sf->terms + sf->nterms);
-	c.sf.has_num_tors_div = sf->has_num_tors_div != 0;
-	c.sf.num_tors_div_weight = sf->num_tors_div_weight;
+	c.sf.ci.assign(sf->ci, sf->ci + sf->nci);
+	c.sf.atom_params = sf->atom_params;
 	c.tab.build(c.sf, 32);
 	for (int i = 0; i < 3; i++)
 	{
@@ -1958,6 +1986,7 @@ void orc_tables(orc_ctx* c, double* smooth, double* fast)
 	}
 }
 double orc_eval_terms(orc_ctx* c, int t1, int t2, double r) { return c->c.sf.eval_fast(t1, t2, r); }
+double orc_conf_independent(orc_ctx* c, double num_tors, double e) { return c->c.sf.conf_independent(num_tors, e); }
 int orc_build_grids(orc_ctx* c, const int* types, int ntypes)
 {
 	try

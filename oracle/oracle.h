@@ -44,6 +44,8 @@ int orc_table_n(orc_ctx* c);
 void orc_tables(orc_ctx* c, double* smooth, double* fast);
 /* one charge-independent weighted sum of terms (weighted_terms::eval_fast) */
 double orc_eval_terms(orc_ctx* c, int t1, int t2, double r);
+/* weighted_terms::conf_independent: the final score of intermolecular energy e */
+double orc_conf_independent(orc_ctx* c, double num_tors, double e);
 
 /* cache::populate for the listed types; maps in point layout (x fastest) */
 int orc_build_grids(orc_ctx* c, const int* types, int ntypes);

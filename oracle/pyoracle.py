@@ -35,6 +35,7 @@ def _load():
         "orc_table_n": (C.c_int, [vp]),
         "orc_tables": (None, [vp, F64P, F64P]),
         "orc_eval_terms": (C.c_double, [vp, C.c_int, C.c_int, C.c_double]),
+        "orc_conf_independent": (C.c_double, [vp, C.c_double, C.c_double]),
         "orc_build_grids": (C.c_int, [vp, I32P, C.c_int]),
         "orc_grid_values": (C.c_int, [vp, C.c_int, F64P]),
         "orc_add_model": (C.c_int, [vp, vp]),
@@ -108,6 +109,9 @@ class Oracle:
         fast = np.zeros(378 * self.n)
         lib.orc_tables(self.h, _p(smooth, F64P), _p(fast, F64P))
         return smooth, fast
+
+    def conf_independent(self, num_tors, e):
+        return lib.orc_conf_independent(self.h, num_tors, e)
 
     def eval_terms(self, t1, t2, r):
         return lib.orc_eval_terms(self.h, t1, t2, r)

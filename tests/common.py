@@ -40,10 +40,10 @@ def gd_c0():
     return sg.grid_dims(*BOX_C0)
 
 
-def oracle_for(ligands, gd=None, hydrophobic_scale=1.0, rec=None):
+def oracle_for(ligands, gd=None, hydrophobic_scale=1.0, rec=None, scoring="vina"):
     rec = rec or receptor()
     gd = gd or gd_c0()
-    sf = sg.vina_scoring(hydrophobic_scale)
+    sf = sg.builtin_scoring(scoring, hydrophobic_scale)
     o = po.Oracle(sf, rec.xyz, rec.type, gd)
     ids = [o.add_model(l.desc_ptr) for l in ligands]
     o.build_grids(heavy_types(*ligands))
