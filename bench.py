@@ -69,7 +69,10 @@ def serve(sg, args, log):
     recs = [sg.Mol.receptor(100 + k, int(rng.integers(2000, 6001)), 8.0) for k in range(8)]
     gd = sg.grid_dims((0.0, 0.0, 0.0), (30.0, 30.0, 30.0))
     types = list(range(2, 14))
-    srv = sg.Server(workers=args.workers, max_batch=64)
+    # each launch's search runs in 8 parts, and every part reports (a long
+    # launch otherwise prints nothing until its jobs complete)
+    srv = sg.Server(workers=args.workers, max_batch=64, progress_chunks=8,
+                    progress=lambda f: log(f"a serving launch: Monte Carlo {100 * f:.0f}% of steps"))
     t = time.perf_counter()
     ri = [srv.add_receptor(r, gd, types) for r in recs]
     log(f"serving: {len(recs)} receptors resident ({time.perf_counter() - t:.1f}s to build maps)")

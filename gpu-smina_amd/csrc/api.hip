@@ -51,26 +51,35 @@ template<> float Real_host_max<float>() { return FLT_MAX; }
 template<> double Real_host_max<double>() { return DBL_MAX; }
 
 // device buffer owned by the context
+// The stream of the API call in progress on this thread.  Device buffers
+// are allocated and freed in its order: stream-ordered allocation does not
+// synchronize the device (hipFree does), so a context's calls never wait for
+// another context's kernels.
+thread_local hipStream_t t_stream = nullptr;
+
 struct DBuf
 {
 	void* p = nullptr;
 	size_t n = 0;
+	hipStream_t st = nullptr; // the stream its allocation and free are ordered on
 	DBuf() = default;
 	DBuf(const DBuf&) = delete;
 	DBuf& operator=(const DBuf&) = delete;
-	~DBuf()
+	~DBuf() { reset(); }
+	void reset()
 	{
 		if (p)
-			(void)hipFree(p);
+			(void)hipFreeAsync(p, st);
+		p = nullptr;
+		n = 0;
 	}
 	void alloc(size_t bytes)
 	{
-		if (p)
-			(void)hipFree(p);
-		p = nullptr;
+		reset();
+		st = t_stream;
 		n = bytes;
 		if (bytes)
-			hip_check(hipMalloc(&p, bytes), "hipMalloc");
+			hip_check(hipMallocAsync(&p, bytes, st), "hipMallocAsync");
 	}
 	template<typename X> X* as() const { return static_cast<X*>(p); }
 };
@@ -682,6 +691,12 @@ sgpu_status guarded(sgpu_ctx* c, F&& f)
 	if (!c)
 		return SGPU_E_INVALID;
 	std::lock_guard<std::mutex> lock(c->mu);
+	struct StreamScope
+	{
+		hipStream_t saved;
+		explicit StreamScope(hipStream_t s) : saved(t_stream) { t_stream = s; }
+		~StreamScope() { t_stream = saved; }
+	} scope(c->stream);
 	try
 	{
 		hip_check(hipSetDevice(c->device), "hipSetDevice");
@@ -790,9 +805,18 @@ void sgpu_destroy(sgpu_ctx* c)
 		return;
 	(void)hipSetDevice(c->device);
 	(void)hipStreamSynchronize(c->stream);
+	// every buffer's free is ordered on the context's stream: release them
+	// all before the stream goes
 	c->batches.clear();
 	c->grids.clear();
 	c->receptors.clear();
+	c->tab.smooth32.reset();
+	c->tab.smooth64.reset();
+	c->tab.fast.reset();
+	c->scratch.reset();
+	c->scratch2.reset();
+	c->err_flag.reset();
+	(void)hipStreamSynchronize(c->stream);
 	(void)hipEventDestroy(c->ev0);
 	(void)hipEventDestroy(c->ev1);
 	for (int k = 0; k < sgpu_ctx::NCLASS - 1; k++)

@@ -21,7 +21,8 @@ DockingServer::DockingServer(const ServerSettings& s, const sgpu_scoring& sf) : 
 	for (int k = 0; k < s_.workers; k++)
 	{
 		std::unique_ptr<Worker> w(new Worker);
-		if (sgpu_create(s_.device, &w->ctx) != SGPU_OK)
+		if (sgpu_create(s_.device, &w->ctx) != SGPU_OK ||
+				sgpu_set_progress(w->ctx, s_.dock.progress_chunks, s_.dock.progress, s_.dock.progress_user) != SGPU_OK)
 			throw std::runtime_error("server: no HIP device");
 		workers_.push_back(std::move(w));
 	}
@@ -238,6 +239,9 @@ sgpu_server* sgpu_server_create(const sgpu_server_settings* s)
 		ss.dock.seed = s->seed;
 		ss.dock.num_modes = s->num_modes;
 		ss.dock.energy_range = s->energy_range;
+		ss.dock.progress_chunks = s->progress_chunks > 0 ? s->progress_chunks : 1;
+		ss.dock.progress = s->progress;
+		ss.dock.progress_user = s->progress_user;
 		sv->terms.resize(SGPU_MAX_TERMS);
 		sv->ci.resize(SGPU_MAX_CI);
 		sgpu_scoring sf;

@@ -137,6 +137,9 @@ typedef struct sgpu_server_settings
 	double energy_range;
 	double hydrophobic_scale;
 	const char* scoring;   /* a built-in function's name (NULL: vina) */
+	int progress_chunks;   /* each launch's Monte Carlo search in this many parts (0 or 1: one) */
+	sgpu_progress_fn progress; /* called after each part, on the worker's thread */
+	void* progress_user;
 } sgpu_server_settings;
 typedef struct sgpu_serve_result
 {
