@@ -151,8 +151,10 @@ def main():
 
     if args.serve:
         # every worker context runs its size classes on four streams: give the
-        # process a hardware queue per stream (set before the runtime starts)
-        os.environ.setdefault("GPU_MAX_HW_QUEUES", str(min(32, 4 * args.workers)))
+        # process a hardware queue per stream (set before the runtime starts;
+        # streams that share a hardware queue run their kernels in FIFO order,
+        # so a short job would wait behind another worker's long launch)
+        os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, 4 * args.workers))
     import smina_gpu as sg
 
     def log(msg):
