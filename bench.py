@@ -135,7 +135,7 @@ def main():
     ap.add_argument("--serve", action="store_true", help="the serving workload instead of the screen")
     ap.add_argument("--jobs", type=int, default=32, help="serving: jobs")
     ap.add_argument("--rate", type=float, default=1.0, help="serving: Poisson arrival rate, jobs/s")
-    ap.add_argument("--workers", type=int, default=3, help="serving: concurrent launches")
+    ap.add_argument("--workers", type=int, default=8, help="serving: concurrent launches")
     ap.add_argument("--flex", action="store_true",
                     help="the flexible-receptor batch (256 ligands, 4 side chains, mixed precision)")
     ap.add_argument("--accuracy", type=int, default=0,

@@ -11,6 +11,11 @@
 // while others run start as soon as a worker frees up, and their work
 // overlaps the running launches.  A job's chains are seeded by the job
 // alone, so its result does not depend on what it was batched with.
+//
+// Each worker's context drives up to four streams.  Streams that share a
+// hardware queue run their kernels in FIFO order, so the process needs
+// GPU_MAX_HW_QUEUES >= 4 x workers (set before the HIP runtime starts) for
+// a short job never to wait behind another worker's long launch.
 #ifndef SGPU_SERVING_H
 #define SGPU_SERVING_H
 
