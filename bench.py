@@ -122,6 +122,73 @@ def serve(sg, args, log):
     print(json.dumps(line), flush=True)
 
 
+def single(sg, args, log):
+    """One ligand (38 heavy atoms, 10 torsions) at exhaustiveness 128 into
+    the 1,800-atom receptor, 22.5 A box, docked in FP64, FP32 and mixed
+    precision.  Reports each run's time, the top-pose score drift and
+    heavy-atom RMSD against FP64, and the per-evaluation error of FP32
+    against FP64 on the device (the FP64 evaluation is bit-exact with the
+    oracle) over chain-start confs and confs near the pocket."""
+    rec = sg.Mol.receptor(1, 1800, 8.0)
+    lig = sg.Mol.ligand(2, 38, 10)
+    gd = sg.grid_dims((0.0, 0.0, 0.0), (22.5, 22.5, 22.5))
+    exh = 128 if args.exhaustiveness == 8 else args.exhaustiveness
+    runs = {}
+    for name, prec in (("fp64", sg.FP64), ("fp32", sg.FP32), ("mixed", sg.MIXED)):
+        d = sg.Docking(rec, gd, precision=prec, exhaustiveness=exh, seed=1, progress_chunks=8,
+                       progress=lambda f, name=name: log(f"{name}: Monte Carlo {100 * f:.0f}% of steps"))
+        d.prepare([lig])
+        t = time.perf_counter()
+        r = d.dock([lig], top_coords=True)[0]
+        runs[name] = (r, time.perf_counter() - t)
+        log(f"{name}: {runs[name][1]:.1f}s, top {r['e'][0]:.4f} kcal/mol")
+        del d
+    r64 = runs["fp64"][0]
+
+    def drift(r):
+        x, y = r["coords"], r64["coords"]
+        return {"top_score_minus_fp64": r["e"][0] - r64["e"][0],
+                "top_pose_rmsd_vs_fp64": float(np.sqrt(np.mean(np.sum((x - y) ** 2, axis=1))))}
+    # per-evaluation error of the FP32 evaluation
+    ctx = sg.Context(0)
+    n, smooth, fast = sg.build_tables(sg.vina_scoring())
+    ctx.upload_tables(n, smooth, fast)
+    rid = ctx.upload_receptor(rec.xyz, rec.type)
+    types = sorted({int(t) for t in lig.type[:lig.nmovable] if t > 1})
+    f = sg.Flat(lig)
+    confs = np.array([f.make_chain(7000 + k, gd)[0] for k in range(512)])
+    confs[256:, :3] = np.random.default_rng(3).uniform(-3, 3, (256, 3))
+    out = {}
+    for prec in (sg.FP64, sg.FP32):
+        gid = ctx.build_grids(rid, gd, types, prec)
+        bid = ctx.upload_models([f], prec)
+        out[prec] = ctx.eval_confs(bid, [0] * len(confs), confs, f.change_size, f.natoms, (10.0, 10.0, 10.0),
+                                   grid_id=gid)
+    e64, g64, _ = out[sg.FP64]
+    e32, g32, _ = out[sg.FP32]
+    de = np.abs(e32 - e64)
+    rel_e = de / np.maximum(1.0, np.abs(e64))
+    rel_g = np.max(np.abs(g32 - g64), axis=1) / np.maximum(1.0, np.max(np.abs(g64), axis=1))
+    small = np.abs(e64) < 10
+    ctx.close()
+    t32 = runs["fp32"][1]
+    line = {
+        "metric": "single ligand at exhaustiveness 128: FP32 vs FP64 per-evaluation error and top-pose drift",
+        "value": 1.0 / t32, "unit": "ligands/s", "n_gpus": 1, "higher_is_better": True, "dtype": "fp32",
+        "data": "synthetic (seeded receptor and ligand)",
+        "config": {"workload": f"one ligand (38 heavy atoms, 10 torsions) into the 1,800-atom receptor, 22.5 A box, "
+                               f"exhaustiveness {exh}, one launch per precision", "exhaustiveness": exh},
+        "seconds": {k: v[1] for k, v in runs.items()},
+        "top_score": {k: v[0]["e"][0] for k, v in runs.items()},
+        "fp32": drift(runs["fp32"][0]), "mixed": drift(runs["mixed"][0]),
+        "per_evaluation": {"confs": len(confs), "max_abs_dE_below_10_kcal": float(de[small].max()),
+                           "median_abs_dE": float(np.median(de)),
+                           "within_1e-4_rel": float(np.mean(np.maximum(rel_e, rel_g) <= 1e-4)),
+                           "max_rel_grad": float(rel_g.max())},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,6 +203,8 @@ def main():
     ap.add_argument("--jobs", type=int, default=32, help="serving: jobs")
     ap.add_argument("--rate", type=float, default=1.0, help="serving: Poisson arrival rate, jobs/s")
     ap.add_argument("--workers", type=int, default=8, help="serving: concurrent launches")
+    ap.add_argument("--single", action="store_true",
+                    help="one ligand at exhaustiveness 128 in FP64, FP32 and mixed: error and top-pose drift")
     ap.add_argument("--flex", action="store_true",
                     help="the flexible-receptor batch (256 ligands, 4 side chains, mixed precision)")
     ap.add_argument("--accuracy", type=int, default=0,
@@ -162,6 +231,9 @@ def main():
 
     if args.serve:
         serve(sg, args, log)
+        return
+    if args.single:
+        single(sg, args, log)
         return
 
     if args.flex:

@@ -1277,6 +1277,13 @@ int sgpu_dock(sgpu_docking* d, const sgpu_mol* const* ligands, int n, sgpu_dock_
 				out[i].e[k] = jobs[i].out[k].e;
 			out[i].evals = jobs[i].evals;
 			out[i].seconds = jobs[i].seconds;
+			out[i].ncoords = 0;
+			if (out[i].coords && !jobs[i].out.empty())
+			{
+				const std::vector<double>& x = jobs[i].out[0].coords;
+				out[i].ncoords = int(std::min<size_t>(x.size(), size_t(std::max(0, out[i].coords_capacity))));
+				std::copy(x.begin(), x.begin() + out[i].ncoords, out[i].coords);
+			}
 		}
 		return 0;
 	}

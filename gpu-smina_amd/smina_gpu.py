@@ -132,7 +132,8 @@ class MinimizeResult(C.Structure):
 
 
 class DockResult(C.Structure):
-    _fields_ = [("nposes", C.c_int), ("e", C.c_double * 32), ("evals", C.c_int64), ("seconds", C.c_double)]
+    _fields_ = [("nposes", C.c_int), ("e", C.c_double * 32), ("evals", C.c_int64), ("seconds", C.c_double),
+                ("coords", F64P), ("coords_capacity", C.c_int), ("ncoords", C.c_int)]
 
 
 def _load():
@@ -549,12 +550,24 @@ class Docking:
             raise RuntimeError(lib.sgpu_last_error(ctx).decode())
         return ms.value
 
-    def dock(self, ligands):
+    def dock(self, ligands, top_coords=False):
+        """per ligand: energies of the modes, evaluations, seconds; with
+        top_coords, the top pose's heavy-atom coordinates ([H, 3])"""
         arr = (C.c_void_p * len(ligands))(*[m.h.value for m in ligands])
         out = (DockResult * len(ligands))()
+        bufs = []
+        if top_coords:
+            for r in out:
+                b = np.zeros(3 * 256)
+                bufs.append(b)
+                r.coords = ptr(b, F64P)
+                r.coords_capacity = b.size
         if lib.sgpu_dock(self.h, arr, len(ligands), out) != 0:
             raise RuntimeError(lib.sgpu_docking_error(self.h).decode())
-        return [dict(e=list(r.e[:r.nposes]), evals=r.evals, seconds=r.seconds) for r in out]
+        res = [dict(e=list(r.e[:r.nposes]), evals=r.evals, seconds=r.seconds) for r in out]
+        for k, b in enumerate(bufs):
+            res[k]["coords"] = b[:out[k].ncoords].reshape(-1, 3).copy()
+        return res
 
     def minimize(self, ligands, maxiters=0, accurate=True):
         """--minimize: FP64 refine_structure of each parsed pose in its own box"""

@@ -98,6 +98,9 @@ typedef struct sgpu_dock_result
 	double e[32];
 	int64_t evals;   /* eval_deriv calls in search and refinement */
 	double seconds;  /* wall time of the job on the device side */
+	double* coords;  /* in: the caller's array for the top pose's heavy-atom coordinates, or NULL */
+	int coords_capacity; /* in: its length in doubles */
+	int ncoords;     /* out: doubles written (3 per heavy atom) */
 } sgpu_dock_result;
 
 /* build the receptor maps for every heavy-atom type of these ligands (the
