@@ -193,6 +193,21 @@ def single(sg, args, log):
     print(json.dumps(line), flush=True)
 
 
+def copy_calibration():
+    """A plain device copy of known size (three copies of 1 GiB through
+    torch), the reference that FETCH_SIZE / WRITE_SIZE counts are scaled by
+    to give the Monte Carlo launch's HBM bytes."""
+    import torch
+    n = 256 * 1024 * 1024
+    a = torch.ones(n, device="cuda", dtype=torch.float32)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": 3 * n * 4}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,7 +230,12 @@ def main():
                     help="top-pose accuracy on this many ligands: served vs FP64, mixed vs FP64, FP64 seed 2 vs 1")
     ap.add_argument("--precision", choices=["fp32", "mixed", "fp64"], default=None,
                     help="search precision (default: fp32 for the screen, mixed for --flex)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only a 3 x 1 GiB device copy (the reference for HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     args.ligands_set = any(a.startswith("--ligands") for a in sys.argv[1:])
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the docking path runs on one "
