@@ -230,6 +230,9 @@ def main():
                     help="top-pose accuracy on this many ligands: served vs FP64, mixed vs FP64, FP64 seed 2 vs 1")
     ap.add_argument("--precision", choices=["fp32", "mixed", "fp64"], default=None,
                     help="search precision (default: fp32 for the screen, mixed for --flex)")
+    ap.add_argument("--fast", action="store_true",
+                    help="the screen with --fast's GPU-style search (automatic chain split and step fraction, early "
+                         "stopping, two-pass screening), compared with the default search on the same ligands")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only a 3 x 1 GiB device copy (the reference for HBM byte counters)")
     args = ap.parse_args()
@@ -295,7 +298,10 @@ def main():
     chunks = 8 if args.flex else 1
     dock = sg.Docking(rec, gd, precision=prec, exhaustiveness=args.exhaustiveness, seed=1, hydrophobic_scale=hyd,
                       progress=lambda f: log(f"Monte Carlo {100 * f:.0f}% of steps") if chunks > 1 else None,
-                      progress_chunks=chunks)
+                      progress_chunks=chunks, fast=args.fast)
+    if args.fast:
+        workload += ", --fast"
+        metric += ", --fast search (not smina's)"
     # build the maps for every ligand type before timing (receptor setup)
     dock.prepare(ligs)
     for w in range(args.warmup):
@@ -322,7 +328,11 @@ def main():
     alg_bytes = float(np.sum(evals * bytes_per_eval))
     achieved = alg_bytes / (mc_ms / 1e3) / 1e9
     top = np.array([r["e"][0] if r["e"] else np.nan for r in results])
-    traffic = None if args.flex or prec != sg.FP32 else measured_traffic(len(ligs), args.exhaustiveness)
+    traffic = None if args.flex or args.fast or prec != sg.FP32 else measured_traffic(len(ligs), args.exhaustiveness)
+    fast = None
+    if args.fast:
+        fast = fast_vs_full(sg, args, log, rec, ligs, gd, hyd, prec, chunks, top)
+        fast["full_searches"] = dock.full_searches()
     log(f"{'flex' if args.flex else 'screen'}: {value:.2f} ligands/s, MC kernel {mc_ms / 1e3:.2f}s, "
         f"{evals.sum():.3e} evals")
 
@@ -393,7 +403,39 @@ def main():
     }
     if accuracy:
         line["top_pose_accuracy"] = accuracy
+    if fast:
+        # the Monte Carlo launch timed last is the second pass only, while
+        # the evaluation counts span both passes: no roofline for this line
+        line["roofline"] = None
+        line["fast"] = fast
     print(json.dumps(line), flush=True)
+
+
+def fast_vs_full(sg, args, log, rec, ligs, gd, hyd, prec, chunks, top_fast):
+    """the --fast screen's top scores against the default search's on the
+    same ligands and seed (one untimed run): per-ligand differences, and how
+    many of the default search's best 10% the fast screen also ranks there"""
+    full = sg.Docking(rec, gd, precision=prec, exhaustiveness=args.exhaustiveness, seed=1, hydrophobic_scale=hyd)
+    full.prepare(ligs)
+    t = time.perf_counter()
+    out = full.dock(ligs)
+    secs = time.perf_counter() - t
+    top_full = np.array([r["e"][0] if r["e"] else np.nan for r in out])
+    ok = np.isfinite(top_full) & np.isfinite(top_fast)
+    d = top_fast[ok] - top_full[ok]
+    q = max(1, int(np.ceil(0.1 * ok.sum())))
+    best_full = set(np.argsort(np.where(ok, top_full, np.inf))[:q])
+    best_fast = set(np.argsort(np.where(ok, top_fast, np.inf))[:q])
+    log(f"default search: {len(ligs) / secs:.2f} ligands/s; fast minus default: median {np.median(d):.3f}, "
+        f"top-10% overlap {len(best_full & best_fast)}/{q}")
+    return {"default_ligands_per_s": len(ligs) / secs,
+            "fast_minus_default": {"median": float(np.median(d)), "mean": float(np.mean(d)),
+                                   "p90": float(np.percentile(d, 90)), "max": float(np.max(d)),
+                                   "within_0.5": float(np.mean(d <= 0.5))},
+            "top10_overlap": len(best_full & best_fast) / q,
+            "note": "per-ligand top score (kcal/mol, after refinement and FP64 rescoring) of the --fast screen "
+                    "minus the default search's, same seed; top10_overlap: share of the default search's best "
+                    "10% that the fast screen also ranks in its best 10%"}
 
 
 def top_pose_accuracy(sg, args, log, rec, sample, gd, hyd, served):

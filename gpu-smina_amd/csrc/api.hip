@@ -843,6 +843,21 @@ sgpu_status sgpu_device_info(sgpu_ctx* c, char* name, int name_len, int* cus)
 	});
 }
 
+sgpu_status sgpu_mc_capacity(sgpu_ctx* c, int precision, int* chains)
+{
+	return guarded(c, [&]() {
+		require(chains && (precision == SGPU_FP32 || precision == SGPU_FP64 || precision == SGPU_MIXED),
+				"capacity request");
+		hipDeviceProp_t p;
+		hip_check(hipGetDeviceProperties(&p, c->device), "hipGetDeviceProperties");
+		const int per_cu = precision == SGPU_FP64    ? mc_blocks_per_cu<double, double>(1, 0)
+		                   : precision == SGPU_MIXED ? mc_blocks_per_cu<float, double>(1, 0)
+		                                             : mc_blocks_per_cu<float, float>(1, 0);
+		require(per_cu > 0, "occupancy query failed");
+		*chains = per_cu * p.multiProcessorCount;
+	});
+}
+
 sgpu_status sgpu_upload_tables(sgpu_ctx* c, const sgpu_tables* t)
 {
 	return guarded(c, [&]() {
@@ -1131,6 +1146,64 @@ void run_eval(sgpu_ctx* c, const sgpu_eval_request* q)
 		throw Fail{SGPU_E_ATOM_OVERLAP, "Ligand atom exactly overlaps receptor atom.  I can't deal with this."};
 }
 
+// Early stopping between the checkpoint launches (stop_agreeing,
+// src/gpu/gpu_kernels_impl.cuh:607-657): each model's search units take the
+// best energy of their chains; a model stops once at least max(2, ceil(
+// fraction x units)) units are within eps of its best.  A stopped model's
+// chains get no further steps (step limit 0: every later part is empty).
+template<typename S>
+void stop_agreeing(sgpu_ctx* c, const sgpu_mc_params& p, const sgpu_chains* ch, const ChainLayout& L,
+		const DBuf& dchains, std::vector<int>& steps_now, std::vector<char>& stopped, DBuf& dsteps)
+{
+	const int n = ch->nchains;
+	std::vector<S> best(static_cast<size_t>(n));
+	hip_check(hipMemcpy2DAsync(best.data(), sizeof(S), dchains.as<char>() + L.off_scal + sizeof(S), L.bytes,
+					  sizeof(S), size_t(n), hipMemcpyDeviceToHost, c->stream),
+			"best energies");
+	hip_check(hipStreamSynchronize(c->stream), "best energies");
+	std::map<int, std::map<int, S>> unit_best;
+	std::map<int, int> chains_of_model;
+	for (int i = 0; i < n; i++)
+	{
+		const int m = ch->model[i];
+		if (stopped[size_t(m)])
+			continue;
+		const int u = ch->unit ? ch->unit[i] : chains_of_model[m]++;
+		auto it = unit_best[m].find(u);
+		if (it == unit_best[m].end())
+			unit_best[m][u] = best[size_t(i)];
+		else
+			it->second = std::min(it->second, best[size_t(i)]);
+	}
+	bool any = false;
+	for (auto& kv : unit_best)
+	{
+		const auto& ub = kv.second;
+		if (ub.size() < 2)
+			continue;
+		S lowest = ub.begin()->second;
+		for (auto& u : ub)
+			lowest = std::min(lowest, u.second);
+		int agree = 0;
+		for (auto& u : ub)
+			agree += (u.second - lowest <= S(p.early_stop_eps));
+		const int need = std::max(2, int(std::ceil(p.early_stop_fraction * double(ub.size()) - 1e-9)));
+		if (agree >= need)
+		{
+			stopped[size_t(kv.first)] = 1;
+			any = true;
+		}
+	}
+	if (!any)
+		return;
+	for (int i = 0; i < n; i++)
+		if (stopped[size_t(ch->model[i])])
+			steps_now[size_t(i)] = 0;
+	hip_check(hipMemcpyAsync(dsteps.p, steps_now.data(), steps_now.size() * sizeof(int), hipMemcpyHostToDevice,
+					  c->stream),
+			"step limits");
+}
+
 template<typename T, typename S>
 void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, const sgpu_chains* ch,
 		sgpu_chain_results* out)
@@ -1150,7 +1223,13 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 		require(out->coord_stride >= 3 * h.nheavy, "coord stride");
 		require(ch->num_steps[i] >= 0 && ch->maxiters[i] >= 0, "steps / maxiters");
 		require(ch->rng[size_t(i) * 625 + 624] <= 624, "rng position");
+		require(!ch->unit || ch->unit[i] >= 0, "search unit");
 	}
+	const bool early_stop = p->early_stop_checkpoints > 0;
+	if (early_stop)
+		require(p->early_stop_first >= 1 && p->early_stop_fraction > 0 && p->early_stop_fraction <= 1 &&
+						p->early_stop_eps >= 0,
+				"early stopping rule");
 	check_maps(b, g, ch->model, n);
 	ChainLayout L{};
 	L.conf_stride = b.h.plan.cmax;
@@ -1320,12 +1399,22 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 	// stream of its own (four streams at most: one hardware queue each, so
 	// no launch queues behind another stream's work).  The waits that join
 	// the streams are enqueued after every launch.  With progress reporting
-	// the steps run as consecutive chunks, joined and reported one by one.
-	const int chunks = std::max(1, std::min(c->progress_chunks, max_steps));
-	Timer tm(c, "mc");
-	for (int ck = 0; ck < chunks; ck++)
+	// the steps run as consecutive parts, joined and reported one by one;
+	// with early stopping (run_monte_carlo, src/gpu/gpu_kernels_impl.cuh:
+	// 742-808) the parts end at first/checkpoints, (first+1)/checkpoints, ..
+	// of every chain's steps, and the rule runs after each part but the last.
+	int parts = std::max(1, std::min(c->progress_chunks, max_steps)), part0 = 1;
+	if (early_stop)
 	{
-		const int lo = int(int64_t(max_steps) * ck / chunks), hi = int(int64_t(max_steps) * (ck + 1) / chunks);
+		parts = p->early_stop_checkpoints;
+		part0 = std::min(parts, p->early_stop_first);
+	}
+	std::vector<int> steps_now(ch->num_steps, ch->num_steps + n);
+	std::vector<char> stopped(size_t(b.n), 0);
+	Timer tm(c, "mc");
+	for (int part = part0; part <= parts; part++)
+	{
+		const int lo = part == part0 ? 0 : part - 1, hi = part;
 		std::vector<hipStream_t> side;
 		for (int k = sgpu_ctx::NCLASS - 1; k >= 0; k--)
 		{
@@ -1345,7 +1434,7 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 			}
 			hip_check(launch_mc<T, S>(ptrs<T>(b), ec, cl.plan, P, L, dchains.as<char>(), int(cl.blks.size()),
 							  cl.waves, dbm.as<int>() + cl.block_off, dsc.as<int>() + cl.slot_off, dsteps.as<int>(),
-							  dmax.as<int>(), lo, hi, st),
+							  dmax.as<int>(), lo, hi, parts, st),
 					"mc kernel");
 		}
 		for (size_t j = 0; j < side.size(); j++)
@@ -1353,10 +1442,13 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 			hip_check(hipEventRecord(c->class_done[j], side[j]), "event");
 			hip_check(hipStreamWaitEvent(c->stream, c->class_done[j], 0), "stream wait");
 		}
-		if (c->progress_fn && ck + 1 < chunks)
+		if ((c->progress_fn || early_stop) && part < parts)
 		{
-			hip_check(hipStreamSynchronize(c->stream), "mc chunk");
-			c->progress_fn(c->progress_user, double(hi) / double(max_steps));
+			hip_check(hipStreamSynchronize(c->stream), "mc part");
+			if (early_stop)
+				stop_agreeing<S>(c, *p, ch, L, dchains, steps_now, stopped, dsteps);
+			if (c->progress_fn)
+				c->progress_fn(c->progress_user, double(hi) / double(parts));
 		}
 	}
 	tm.stop();

@@ -19,13 +19,15 @@ template<typename T>
 hipError_t launch_eval(const BatchPtrs<T>& B, const EvalCtx<T>& ec, const WavePlan& plan, int n, const int* model,
 		const T* conf, int conf_stride, T* e, T* g, int g_stride, T* x, int x_stride, T* scratch, int pstride,
 		hipStream_t st);
-// nblocks workgroups of `waves` (1..8) chains of one ligand each; T is the
+// nblocks workgroups of `waves` (1..8) chains of one ligand each, running
+// the steps [ceil(steps x part_lo / parts), ceil(steps x part_hi / parts))
+// of every chain; T is the
 // evaluation's real type, S the search state's (kernels_mixed.hip: float,
 // double)
 template<typename T, typename S>
 hipError_t launch_mc(const BatchPtrs<T>& B, const EvalCtx<T>& ec, const WavePlan& plan, const McParams& P,
 		const ChainLayout& L, char* chains, int nblocks, int waves, const int* block_model, const int* slot_chain,
-		const int* chain_steps, const int* chain_maxiters, int step_lo, int step_hi, hipStream_t st);
+		const int* chain_steps, const int* chain_maxiters, int part_lo, int part_hi, int parts, hipStream_t st);
 // workgroups of `waves` chains with `lds` bytes of LDS that one CU holds at
 // once (the runtime's occupancy calculator: registers and LDS)
 template<typename T, typename S>

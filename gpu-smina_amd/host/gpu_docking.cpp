@@ -627,6 +627,7 @@ GpuDocking::GpuDocking(const GpuSettings& s, const sgpu_scoring& sf, const sgpu_
 	check(ctx_, sgpu_upload_tables(ctx_, &tt));
 	check(ctx_, sgpu_upload_receptor(ctx_, &rec, &receptor_));
 	check(ctx_, sgpu_set_progress(ctx_, s.progress_chunks, s.progress, s.progress_user));
+	check(ctx_, sgpu_mc_capacity(ctx_, s.precision, &capacity_));
 }
 
 GpuDocking::GpuDocking(const GpuSettings& s, const sgpu_scoring& sf, const sgpu_receptor& rec,
@@ -646,6 +647,7 @@ GpuDocking::GpuDocking(const GpuSettings& s, const sgpu_scoring& sf, const sgpu_
 	sgpu_tables tt{t.n, t.factor, t.cutoff_sqr, t.smooth.data(), t.fast.data()};
 	check(ctx_, sgpu_upload_tables(ctx_, &tt));
 	check(ctx_, sgpu_upload_receptor(ctx_, &rec, &receptor_));
+	check(ctx_, sgpu_mc_capacity(ctx_, s.precision, &capacity_));
 }
 
 GpuDocking::~GpuDocking()
@@ -704,11 +706,82 @@ void GpuDocking::prepare(const std::vector<const sgpu_model_desc*>& models)
 	ensure_grids(types);
 }
 
-void GpuDocking::dock_batch(std::vector<DockJob*>& jobs)
+void apply_fast(GpuSettings& s)
+{
+	s.chain_split = 0;
+	s.step_fraction = 0;
+	if (s.early_stop == GpuSettings::EARLY_STOP_OFF)
+		s.early_stop = GpuSettings::EARLY_STOP_FULL;
+	s.fast_top = 0.1;
+}
+
+// auto_split (src/gpu/gpu_docking.cpp:87-102): the largest power of two from
+// 4 to 32 that keeps tasks x split within 4 times the chains the device holds
+int GpuDocking::split_for(size_t tasks) const
+{
+	if (s_.chain_split > 0)
+		return s_.chain_split;
+	int split = 32;
+	while (split > 4 && tasks * size_t(split) > 4 * size_t(capacity_))
+		split /= 2;
+	return split;
+}
+
+// fraction_for and auto_fraction (src/gpu/gpu_docking.cpp:104-112, :167-171):
+// a first-pass budget, else the set fraction, else half the steps for a split
+// in [4, 32) and all of them otherwise
+double GpuDocking::fraction_for(int split, double budget) const
+{
+	if (budget > 0)
+		return budget;
+	if (s_.step_fraction > 0)
+		return s_.step_fraction;
+	return split >= 4 && split < 32 ? 0.5 : 1.0;
+}
+
+void GpuDocking::dock_pass(std::vector<DockJob*>& jobs, double budget)
 {
 	const size_t per = s_.ligands_per_launch > 0 ? size_t(s_.ligands_per_launch) : jobs.size();
 	for (size_t b = 0; b < jobs.size(); b += per)
-		dock_sub(jobs.data() + b, std::min(per, jobs.size() - b));
+		dock_sub(jobs.data() + b, std::min(per, jobs.size() - b), budget);
+}
+
+// the batch flush of src/main/main.cpp:1551-1597: one pass, or with fast_top
+// in (0, 1) and at least 20 ligands seen, a first pass at fast_first of the
+// steps whose top scores join the history, then the full search for the
+// ligands within fast_top_margin of the history's best fast_top
+void GpuDocking::dock_batch(std::vector<DockJob*>& jobs)
+{
+	full_searches_ = jobs.size();
+	if (!(s_.fast_top > 0 && s_.fast_top < 1 && top_history_.size() + jobs.size() >= 20))
+	{
+		dock_pass(jobs, 0);
+		return;
+	}
+	if (!(s_.fast_first > 0))
+		throw std::invalid_argument("fast_first must be positive");
+	dock_pass(jobs, s_.fast_first);
+	std::vector<double> score(jobs.size());
+	for (size_t k = 0; k < jobs.size(); k++)
+		score[k] = jobs[k]->out.empty() ? kMax : jobs[k]->out[0].e;
+	top_history_.insert(top_history_.end(), score.begin(), score.end());
+	std::vector<double> sorted(top_history_);
+	const size_t q = std::max<size_t>(1, size_t(std::ceil(s_.fast_top * double(sorted.size())))) - 1;
+	std::nth_element(sorted.begin(), sorted.begin() + ptrdiff_t(q), sorted.end());
+	const double cutoff = sorted[q] + s_.fast_top_margin;
+	std::vector<DockJob*> again;
+	std::vector<int64_t> first_evals;
+	for (size_t k = 0; k < jobs.size(); k++)
+		if (score[k] <= cutoff)
+		{
+			jobs[k]->out.clear();
+			again.push_back(jobs[k]);
+			first_evals.push_back(jobs[k]->evals);
+		}
+	full_searches_ = again.size();
+	dock_pass(again, 0);
+	for (size_t k = 0; k < again.size(); k++)
+		again[k]->evals += first_evals[k];
 }
 
 sgpu_grid_dims movable_atoms_box(const FlatModel& m, double add, double granularity)
@@ -844,7 +917,7 @@ void GpuDocking::minimize_batch(std::vector<MinimizeJob*>& jobs)
 	check(ctx_, sgpu_release_models(ctx_, batch));
 }
 
-void GpuDocking::dock_sub(DockJob* const* jobs, size_t nj)
+void GpuDocking::dock_sub(DockJob* const* jobs, size_t nj, double budget)
 {
 	if (nj == 0)
 		return;
@@ -875,34 +948,43 @@ void GpuDocking::dock_sub(DockJob* const* jobs, size_t nj)
 		cstride = std::max(cstride, f.view.conf_size);
 		hstride = std::max(hstride, 3 * f.nheavy);
 	}
-	std::vector<int> cmodel, csteps, citers;
+	// with split n, n chains per exhaustiveness unit, each with 1/n of the
+	// steps (add_chains and chain_steps, src/gpu/gpu_docking.cpp:200-220)
+	std::vector<int> cmodel, csteps, citers, cunit;
 	std::vector<double> cconf;
 	std::vector<uint32_t> crng;
 	std::vector<size_t> first(nj + 1, 0);
 	std::vector<SearchParams> sp;
+	size_t tasks = 0;
+	for (size_t j = 0; j < nj; j++)
+		tasks += size_t(jobs[j]->exhaustiveness > 0 ? jobs[j]->exhaustiveness : s_.exhaustiveness);
+	const int split = split_for(tasks);
+	const double fraction = fraction_for(split, budget);
 	for (size_t j = 0; j < nj; j++)
 	{
 		const int exh = jobs[j]->exhaustiveness > 0 ? jobs[j]->exhaustiveness : s_.exhaustiveness;
 		sp.push_back(setup_search(fms[j], exh));
 		Gen g(uint32_t(s_.seed));
 		first[j] = cmodel.size();
-		for (int t = 0; t < exh; t++)
+		const int steps = std::max(1, int(std::ceil(double(sp[j].num_steps) * fraction / double(split))));
+		for (int t = 0; t < exh * split; t++)
 		{
 			const int seed = random_int(0, 1000000, g);
 			std::vector<double> conf(cstride, 0.0);
 			std::vector<uint32_t> st(625);
 			make_chain(fms[j], uint32_t(seed), gd_, conf.data(), st.data());
 			cmodel.push_back(int(j));
-			csteps.push_back(s_.step_fraction >= 1 ? sp[j].num_steps
-			                                        : int(std::ceil(sp[j].num_steps * s_.step_fraction)));
+			csteps.push_back(steps);
 			citers.push_back(sp[j].maxiters);
+			cunit.push_back(t / split);
 			cconf.insert(cconf.end(), conf.begin(), conf.end());
 			crng.insert(crng.end(), st.begin(), st.end());
 		}
 	}
 	first[nj] = cmodel.size();
 	const int nchains = int(cmodel.size());
-	sgpu_chains ch{nchains, cmodel.data(), cconf.data(), cstride, crng.data(), csteps.data(), citers.data()};
+	sgpu_chains ch{nchains, cmodel.data(), cconf.data(), cstride, crng.data(), csteps.data(), citers.data(),
+			cunit.data()};
 	sgpu_mc_params mp{};
 	mp.temperature = 1.2;
 	mp.mutation_amplitude = 2;
@@ -914,6 +996,18 @@ void GpuDocking::dock_sub(DockJob* const* jobs, size_t nj)
 		mp.authentic_v[k] = 1000;
 	}
 	mp.cache_slope = 1e6;
+	// stops and stop_for (src/gpu/gpu_docking.cpp:172-189): not in a
+	// first pass; checkpoints after every tenth of smina's steps the chains
+	// run, the rule from the second on
+	const bool stops = !(budget > 0) && (s_.early_stop == GpuSettings::EARLY_STOP_ALWAYS ||
+	                                     (s_.early_stop == GpuSettings::EARLY_STOP_FULL && split < 32));
+	if (stops)
+	{
+		mp.early_stop_first = 2;
+		mp.early_stop_checkpoints = std::max(mp.early_stop_first + 1, int(std::lround(10 * fraction)));
+		mp.early_stop_fraction = 0.5;
+		mp.early_stop_eps = 0.05;
+	}
 	const int maxp = 20;
 	std::vector<int> cnt(nchains);
 	std::vector<double> oe(size_t(nchains) * maxp), oc(size_t(nchains) * maxp * cstride),
@@ -1212,6 +1306,19 @@ sgpu_docking* sgpu_docking_create(const sgpu_dock_settings* s, const sgpu_mol* r
 		if (!(s->step_fraction > 0) || s->step_fraction > 1)
 			throw std::invalid_argument("step_fraction must be in (0, 1]");
 		gs.step_fraction = s->step_fraction;
+		if (s->chain_split < 0)
+			throw std::invalid_argument("chain_split must be at least 1 (0: default)");
+		gs.chain_split = s->chain_split > 0 ? s->chain_split : 1;
+		gs.early_stop = s->early_stop ? GpuSettings::EARLY_STOP_ALWAYS : GpuSettings::EARLY_STOP_OFF;
+		if (s->fast)
+		{
+			// explicit chain_split / step_fraction take precedence
+			apply_fast(gs);
+			if (s->chain_split > 0)
+				gs.chain_split = s->chain_split;
+			if (s->step_fraction < 1)
+				gs.step_fraction = s->step_fraction;
+		}
 		gs.progress_chunks = s->progress_chunks > 0 ? s->progress_chunks : 1;
 		gs.progress = s->progress;
 		gs.progress_user = s->progress_user;
@@ -1233,6 +1340,7 @@ sgpu_docking* sgpu_docking_create(const sgpu_dock_settings* s, const sgpu_mol* r
 	return d;
 }
 void sgpu_docking_free(sgpu_docking* d) { delete d; }
+int sgpu_docking_full_searches(const sgpu_docking* d) { return d->d ? int(d->d->full_searches()) : 0; }
 const char* sgpu_docking_error(const sgpu_docking* d) { return d->err.c_str(); }
 sgpu_ctx* sgpu_docking_ctx(sgpu_docking* d) { return d->d ? d->d->ctx() : nullptr; }
 int sgpu_docking_receptor_id(sgpu_docking* d) { return d->d ? d->d->receptor_id() : -1; }

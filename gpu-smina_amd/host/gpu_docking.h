@@ -85,13 +85,35 @@ struct GpuSettings
 	int num_modes = 9;
 	double energy_range = 3.0;
 	int ligands_per_launch = 0; // 0: the whole batch in one launch
-	double step_fraction = 1;   // fraction of smina's Monte Carlo steps (1: smina's search)
+	// GPU-style search, not smina's (GpuSettings, src/gpu/gpu_docking.h:23-44).
+	// chain_split: every Monte Carlo chain becomes this many chains with
+	// 1/chain_split of its steps (0: automatic, auto_split); step_fraction:
+	// the fraction of smina's steps (1: smina's search; 0: automatic,
+	// auto_fraction); early_stop: stop a ligand once its exhaustiveness
+	// units agree on its best energy (EARLY_STOP_FULL: only while the device
+	// is full, chain split below 32)
+	int chain_split = 1;
+	double step_fraction = 1;
+	enum { EARLY_STOP_OFF, EARLY_STOP_FULL, EARLY_STOP_ALWAYS };
+	int early_stop = EARLY_STOP_OFF;
+	// two-pass screening (src/main/main.cpp:1563-1597): every ligand with
+	// fast_first of the steps, then the full search for those within
+	// fast_top_margin of the best fast_top of all ligands so far (0: one pass)
+	double fast_top = 0;
+	double fast_first = 0.0625;
+	double fast_top_margin = 0.3;
 	// progress of the Monte Carlo search (sgpu_set_progress): its steps in
 	// this many launches, progress(progress_user, fraction) after each
 	int progress_chunks = 1;
 	sgpu_progress_fn progress = nullptr;
 	void* progress_user = nullptr;
 };
+
+// the settings --fast selects (src/main/main.cpp:1451-1461, :1426-1431):
+// automatic chain split and step fraction, early stopping while the device is
+// full, two-pass screening of the best 10%.  QuickVina 2's history check,
+// which --fast also turns on, is not carried by this path.
+void apply_fast(GpuSettings& s);
 
 class GpuDocking
 {
@@ -117,6 +139,9 @@ public:
 	// every pose, FP64 rescoring, clustering (src/gpu/gpu_docking.cpp:756-831,
 	// src/main/main.cpp:256-324)
 	void dock_batch(std::vector<DockJob*>& jobs);
+	// the ligands of the last dock_batch that got the full search of a
+	// two-pass screen
+	size_t full_searches() const { return full_searches_; }
 
 	// local optimization (--minimize / --local_only), always in FP64:
 	// refine_structure of each job's start conf in the job's own box
@@ -148,8 +173,16 @@ public:
 
 private:
 	void ensure_grids(const std::vector<int>& types);
-	void dock_sub(DockJob* const* jobs, size_t n);
+	// budget > 0: that fraction of smina's steps, without early stopping
+	// (the first pass of a two-pass screen)
+	void dock_sub(DockJob* const* jobs, size_t n, double budget);
+	void dock_pass(std::vector<DockJob*>& jobs, double budget);
+	int split_for(size_t tasks) const;
+	double fraction_for(int split, double budget) const;
 	GpuSettings s_;
+	int capacity_ = 1;                // chains the device holds at once
+	std::vector<double> top_history_; // two-pass: first-pass top scores so far
+	size_t full_searches_ = 0;
 	sgpu_ctx* ctx_ = nullptr;
 	bool owns_ctx_ = true;
 	sgpu_scoring sf_;

@@ -50,12 +50,14 @@ class FlatModel(C.Structure):
 class McParams(C.Structure):
     _fields_ = [("temperature", C.c_double), ("mutation_amplitude", C.c_double), ("min_rmsd", C.c_double),
                 ("num_saved_mins", C.c_int), ("hunt_cap", C.c_double * 3), ("authentic_v", C.c_double * 3),
-                ("cache_slope", C.c_double), ("accurate_line_search", C.c_int), ("early_term", C.c_int)]
+                ("cache_slope", C.c_double), ("accurate_line_search", C.c_int), ("early_term", C.c_int),
+                ("early_stop_checkpoints", C.c_int), ("early_stop_first", C.c_int),
+                ("early_stop_fraction", C.c_double), ("early_stop_eps", C.c_double)]
 
 
 class Chains(C.Structure):
     _fields_ = [("nchains", C.c_int), ("model", I32P), ("conf", F64P), ("conf_stride", C.c_int),
-                ("rng", U32P), ("num_steps", I32P), ("maxiters", I32P)]
+                ("rng", U32P), ("num_steps", I32P), ("maxiters", I32P), ("unit", I32P)]
 
 
 class ChainResults(C.Structure):
@@ -112,7 +114,8 @@ class DockSettings(C.Structure):
     _fields_ = [("device", C.c_int), ("precision", C.c_int), ("exhaustiveness", C.c_int), ("seed", C.c_int),
                 ("num_modes", C.c_int), ("energy_range", C.c_double), ("hydrophobic_scale", C.c_double),
                 ("ligands_per_launch", C.c_int), ("step_fraction", C.c_double), ("progress_chunks", C.c_int),
-                ("progress", PROGRESS_FN), ("progress_user", C.c_void_p), ("scoring", C.c_char_p)]
+                ("progress", PROGRESS_FN), ("progress_user", C.c_void_p), ("scoring", C.c_char_p),
+                ("chain_split", C.c_int), ("early_stop", C.c_int), ("fast", C.c_int)]
 
 
 class ServerSettings(C.Structure):
@@ -146,6 +149,7 @@ def _load():
         "sgpu_destroy": (None, [vp]),
         "sgpu_last_error": (C.c_char_p, [vp]),
         "sgpu_device_info": (C.c_int, [vp, C.c_char_p, C.c_int, I32P]),
+        "sgpu_mc_capacity": (C.c_int, [vp, C.c_int, I32P]),
         "sgpu_upload_tables": (C.c_int, [vp, C.POINTER(Tables)]),
         "sgpu_upload_receptor": (C.c_int, [vp, C.POINTER(Receptor), I32P]),
         "sgpu_build_grids": (C.c_int, [vp, C.c_int, C.POINTER(GridDims), I32P, C.c_int, C.c_int, I32P]),
@@ -189,6 +193,7 @@ def _load():
         "sgpu_docking_prepare": (C.c_int, [vp, C.POINTER(vp), C.c_int]),
         "sgpu_docking_ctx": (vp, [vp]),
         "sgpu_docking_receptor_id": (C.c_int, [vp]),
+        "sgpu_docking_full_searches": (C.c_int, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -419,17 +424,20 @@ class Context:
         self.check(lib.sgpu_eval_confs(self.h, C.byref(q)))
         return e, g, x
 
-    def run_mc(self, bid, gid, params, models, confs, rngs, steps, maxiters, max_poses=20, coord_stride=None):
+    def run_mc(self, bid, gid, params, models, confs, rngs, steps, maxiters, max_poses=20, coord_stride=None,
+               units=None):
+        """units: each chain's search unit within its model (early stopping)"""
         models = i32(models)
         confs = f64(confs)
         rngs = np.ascontiguousarray(rngs, dtype=np.uint32)
         steps = i32(steps)
         maxiters = i32(maxiters)
+        units = None if units is None else i32(units)
         n = len(models)
         cs = confs.shape[1]
         xs = coord_stride or 3 * 256
         ch = Chains(n, ptr(models, I32P), ptr(confs, F64P), cs, ptr(rngs, U32P), ptr(steps, I32P),
-                    ptr(maxiters, I32P))
+                    ptr(maxiters, I32P), ptr(units, I32P) if units is not None else None)
         cnt = np.zeros(n, np.int32)
         e = np.zeros((n, max_poses))
         c = np.zeros((n, max_poses, cs))
@@ -514,16 +522,17 @@ class Docking:
 
     def __init__(self, receptor, gd, fp64=False, exhaustiveness=8, seed=1, num_modes=9, energy_range=3.0,
                  hydrophobic_scale=1.0, ligands_per_launch=0, device=0, step_fraction=1.0, precision=None,
-                 progress=None, progress_chunks=1, scoring="vina"):
+                 progress=None, progress_chunks=1, scoring="vina", chain_split=0, early_stop=False, fast=False):
         """precision: FP32, FP64 or MIXED (default: FP64 if fp64 else FP32);
         progress(fraction): called after each of progress_chunks launches
-        of the Monte Carlo search"""
+        of the Monte Carlo search; chain_split, early_stop, fast: the
+        GPU-style search options --gpu_split, --gpu_early_stop, --fast"""
         prec = precision if precision is not None else (FP64 if fp64 else FP32)
         # keep the callback alive as long as the docking object
         self._progress = PROGRESS_FN(lambda user, frac: progress(frac)) if progress else PROGRESS_FN()
         s = DockSettings(device, prec, exhaustiveness, seed, num_modes, energy_range, hydrophobic_scale,
                          ligands_per_launch, step_fraction, progress_chunks, self._progress, None,
-                         scoring.encode())
+                         scoring.encode(), chain_split, int(early_stop), int(fast))
         self.gd = gd
         self.h = C.c_void_p(lib.sgpu_docking_create(C.byref(s), receptor.h, C.byref(gd)))
         err = lib.sgpu_docking_error(self.h)
@@ -568,6 +577,10 @@ class Docking:
         for k, b in enumerate(bufs):
             res[k]["coords"] = b[:out[k].ncoords].reshape(-1, 3).copy()
         return res
+
+    def full_searches(self):
+        """ligands of the last dock() that got the full search"""
+        return lib.sgpu_docking_full_searches(self.h)
 
     def minimize(self, ligands, maxiters=0, accurate=True):
         """--minimize: FP64 refine_structure of each parsed pose in its own box"""

@@ -135,6 +135,17 @@ typedef struct sgpu_mc_params
 	double cache_slope;        /* 1e6 */
 	int accurate_line_search;  /* 0: fast_line_search */
 	int early_term;
+	/* Early stopping (not smina's search; EarlyStop and stop_agreeing,
+	 * src/gpu/gpu_kernels.h:51-56, src/gpu/gpu_kernels_impl.cuh:607-657):
+	 * with n = early_stop_checkpoints, the launches end at first/n,
+	 * (first+1)/n, .., n/n of every chain's steps (first =
+	 * early_stop_first); after each but the last, a model whose search units'
+	 * best energies agree (at least max(2, ceil(fraction x units)) within eps
+	 * of the best) stops.  0 checkpoints: off. */
+	int early_stop_checkpoints;
+	int early_stop_first;
+	double early_stop_fraction; /* 0.5 */
+	double early_stop_eps;      /* 0.05 kcal/mol */
 } sgpu_mc_params;
 
 enum { SGPU_MAX_POSES = 64 };
@@ -149,6 +160,7 @@ typedef struct sgpu_chains
 	const uint32_t* rng;   /* [nchains*625]: MT words then position */
 	const int* num_steps;  /* [nchains] */
 	const int* maxiters;   /* [nchains] BFGS iterations */
+	const int* unit;       /* [nchains] search unit within its model, for early stopping (NULL: one per chain) */
 } sgpu_chains;
 
 /* ---- chain results (ChainResult, src/gpu/gpu_kernels.h:59-64): each
@@ -173,6 +185,10 @@ void sgpu_destroy(sgpu_ctx* ctx);
 const char* sgpu_last_error(const sgpu_ctx* ctx);
 /* name and compute-unit count of the context's device */
 sgpu_status sgpu_device_info(sgpu_ctx* ctx, char* name, int name_len, int* cus);
+/* Monte Carlo chains of this precision the device holds at once (one per
+ * wave, as the occupancy calculator gives it from the search kernel's
+ * registers): EngineBase::capacity, src/gpu/gpu_docking.cpp:223-226 */
+sgpu_status sgpu_mc_capacity(sgpu_ctx* ctx, int precision, int* chains);
 
 /* DeviceTables::upload (src/gpu/gpu_device.cu:71-90) */
 sgpu_status sgpu_upload_tables(sgpu_ctx* ctx, const sgpu_tables* t);

@@ -85,6 +85,14 @@ typedef struct sgpu_dock_settings
 	sgpu_progress_fn progress; /* called after each with the fraction done (sgpu_set_progress) */
 	void* progress_user;
 	const char* scoring;    /* --scoring: a built-in function's name (NULL: vina) */
+	/* GPU-style search, not smina's (GpuSettings, src/gpu/gpu_docking.h:23-44;
+	 * options at src/main/main.cpp:1436-1461) */
+	int chain_split;        /* --gpu_split: chains per exhaustiveness unit, each with 1/split of the
+	                         * steps (0: 1, or automatic with fast) */
+	int early_stop;         /* --gpu_early_stop: stop a ligand once its units agree on its best energy */
+	int fast;               /* --fast: automatic split and step fraction (unless chain_split > 0 or
+	                         * step_fraction < 1), early stopping while the device is full, two-pass
+	                         * screening of the best 10% (QuickVina 2's history check not carried) */
 } sgpu_dock_settings;
 
 sgpu_docking* sgpu_docking_create(const sgpu_dock_settings* s, const sgpu_mol* receptor,
@@ -96,7 +104,7 @@ typedef struct sgpu_dock_result
 {
 	int nposes;
 	double e[32];
-	int64_t evals;   /* eval_deriv calls in search and refinement */
+	int64_t evals;   /* eval_deriv calls of the Monte Carlo search */
 	double seconds;  /* wall time of the job on the device side */
 	double* coords;  /* in: the caller's array for the top pose's heavy-atom coordinates, or NULL */
 	int coords_capacity; /* in: its length in doubles */
@@ -121,6 +129,9 @@ typedef struct sgpu_minimize_result
 } sgpu_minimize_result;
 int sgpu_minimize(sgpu_docking* d, const sgpu_mol* const* ligands, int n, int maxiters, int accurate,
 		sgpu_minimize_result* out);
+/* ligands of the last sgpu_dock that got the full search (all of them
+ * unless a two-pass screen ran) */
+int sgpu_docking_full_searches(const sgpu_docking* d);
 
 /* low-level access for tests: the context and the device ids of the
  * receptor, grids and tables this docking object built */

@@ -2227,9 +2227,14 @@ double orc_refine_opts(orc_ctx* c, int model, const double* v, int maxiters, int
 }
 
 int orc_dock(orc_ctx* c, int model, const sgpu_mc_params* p, int seed, int exhaustiveness, int num_modes,
-		double energy_range, int threads, double step_fraction, int max_out, double* e, double* conf,
+		double energy_range, int threads, double step_fraction, int split, int max_out, double* e, double* conf,
 		int64_t* evals)
 {
+	if (split < 1)
+	{
+		c->c.err = "split must be at least 1";
+		return -1;
+	}
 	const Model& proto = *c->c.models[model];
 	// setup_search (src/main/main.cpp:513-529)
 	const int N = proto.N;
@@ -2238,18 +2243,23 @@ int orc_dock(orc_ctx* c, int model, const sgpu_mc_params* p, int seed, int exhau
 		dof += tr.ntors;
 	const int heuristic = N + 10 * dof;
 	const int full_steps = int(unsigned(70 * 3 * (50 + heuristic) / 2));
-	// a shortened search (the device facade's step_fraction): the first
-	// ceil(fraction x steps) steps of every chain
-	const int num_steps = step_fraction >= 1 ? full_steps : int(std::ceil(full_steps * step_fraction));
+	// a shortened or split search (the GPU-style options; add_chains and
+	// chain_steps, src/gpu/gpu_docking.cpp:200-220): exhaustiveness x split
+	// chains seeded in order, each with max(1, ceil(steps x fraction /
+	// split)) steps
+	const int num_steps = step_fraction >= 1 && split == 1
+	                              ? full_steps
+	                              : std::max(1, int(std::ceil(double(full_steps) * step_fraction / double(split))));
 	const int maxiters = (25 + N) / 3;
 	// parallel_mc::operator() (src/lib/parallel_mc.cpp:80-98)
+	const int tasks = exhaustiveness * split;
 	rng generator{uint32_t(seed)};
-	std::vector<int> seeds(exhaustiveness);
-	for (int i = 0; i < exhaustiveness; i++)
+	std::vector<int> seeds(tasks);
+	for (int i = 0; i < tasks; i++)
 		seeds[i] = random_int(0, 1000000, generator);
-	std::vector<std::vector<Output>> task_out(exhaustiveness);
-	std::vector<int64_t> task_evals(exhaustiveness, 0);
-	std::vector<std::string> errs(exhaustiveness);
+	std::vector<std::vector<Output>> task_out(tasks);
+	std::vector<int64_t> task_evals(tasks, 0);
+	std::vector<std::string> errs(tasks);
 	vec c1(c->c.gd.begin[0], c->c.gd.begin[1], c->c.gd.begin[2]);
 	vec c2(c->c.gd.end[0], c->c.gd.end[1], c->c.gd.end[2]);
 	auto run_task = [&](int t) {
@@ -2268,7 +2278,7 @@ int orc_dock(orc_ctx* c, int model, const sgpu_mc_params* p, int seed, int exhau
 		}
 	};
 	if (threads <= 1)
-		for (int t = 0; t < exhaustiveness; t++)
+		for (int t = 0; t < tasks; t++)
 			run_task(t);
 	else
 	{
@@ -2276,7 +2286,7 @@ int orc_dock(orc_ctx* c, int model, const sgpu_mc_params* p, int seed, int exhau
 		std::atomic<int> next(0);
 		for (int w = 0; w < threads; w++)
 			pool.emplace_back([&]() {
-				for (int t = next++; t < exhaustiveness; t = next++)
+				for (int t = next++; t < tasks; t = next++)
 					run_task(t);
 			});
 		for (auto& th : pool)
@@ -2290,7 +2300,7 @@ int orc_dock(orc_ctx* c, int model, const sgpu_mc_params* p, int seed, int exhau
 		}
 	// merge_output_containers with min_rmsd 2 (src/lib/parallel_mc.cpp:71-78)
 	std::vector<Output> out;
-	for (int t = 0; t < exhaustiveness; t++)
+	for (int t = 0; t < tasks; t++)
 		for (const Output& o : task_out[t])
 			add_to_output_container(out, o, 2, size_t(p->num_saved_mins));
 	sort_container(out);

@@ -97,10 +97,12 @@ double orc_refine_opts(orc_ctx* c, int model, const double* v, int maxiters, int
 
 /* the whole docking of one ligand: parallel_mc (src/lib/parallel_mc.cpp:80-98)
  * + refine + finish_docking (src/main/main.cpp:256-324), single-threaded or on
- * `threads` host threads, each chain cut to ceil(step_fraction x steps) when
- * step_fraction < 1.  Results sorted: count, energies, confs. */
+ * `threads` host threads.  With step_fraction < 1 or split > 1 (the GPU-style
+ * search, src/gpu/gpu_docking.cpp:200-220): exhaustiveness x split chains,
+ * each cut to max(1, ceil(steps x step_fraction / split)) steps.  Results
+ * sorted: count, energies, confs. */
 int orc_dock(orc_ctx* c, int model, const sgpu_mc_params* p, int seed, int exhaustiveness,
-		int num_modes, double energy_range, int threads, double step_fraction, int max_out, double* e,
+		int num_modes, double energy_range, int threads, double step_fraction, int split, int max_out, double* e,
 		double* conf, int64_t* evals);
 
 #ifdef __cplusplus

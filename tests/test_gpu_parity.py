@@ -9,6 +9,8 @@ the 8 A cutoff).  Random confs (clashes and face crossings included, not
 flagged) must reach 95% within 1e-4 and every one within 1e-3 unless
 |E| > 1e3; the minima an FP64 search saves must reach 99%.  Energies below
 10 kcal/mol are within 1e-3 kcal/mol of FP64 (the served tolerance)."""
+import math
+
 import numpy as np
 import pytest
 
@@ -185,6 +187,76 @@ def test_mc_fp64_chunked_launches(env):
         assert np.array_equal(c[k, :cnt[k], :co.shape[1]], co) and ev[k] == evo
 
 
+def test_mc_fp64_early_stop(env):
+    """early stopping (EarlyStop, stop_agreeing, run_monte_carlo;
+    src/gpu/gpu_kernels.h:51-56, src/gpu/gpu_kernels_impl.cuh:607-657,
+    :742-808): the chains run to first/n, (first+1)/n, .. of their steps; after
+    each checkpoint but the last, a ligand whose units' best energies agree
+    stops.  The expected stops come from the oracle's chains cut at each
+    checkpoint (a chain's best energy is the lowest of its output
+    container); every FP64 chain then matches the oracle's chain cut where
+    its ligand stopped, poses and evaluation count alike."""
+    ctx, o = env["ctx"], env["o"]
+    n, first, fraction = 5, 2, 0.5
+    chains = []  # (ligand, unit, conf, rng state, steps, maxiters)
+    for li in (0, 2):
+        f = env["flats"][li]
+        for k in range(6):
+            c, s = f.make_chain(7000 + 10 * li + k, env["gd"])
+            chains.append((li, k // 2, c, s, 12 + 2 * li, (25 + f.natoms) // 3))
+    ckpt = lambda steps, k: -(-steps * k // n)
+    runs = {}  # (chain, checkpoint) -> oracle chain cut there
+    for i, (li, u, c, s, steps, maxit) in enumerate(chains):
+        for k in range(first, n + 1):
+            runs[i, k] = o.mc_chain(env["ids"][li], sg.mc_params(), ckpt(steps, k), maxit, c, s)
+
+    def expected_stop(eps):
+        stop = {}
+        for k in range(first, n):
+            for li in (0, 2):
+                if li in stop:
+                    continue
+                ub = {}
+                for i, ch in enumerate(chains):
+                    if ch[0] == li:
+                        eo = runs[i, k][0]
+                        ub[ch[1]] = min(ub.get(ch[1], np.inf), eo.min() if len(eo) else np.inf)
+                best = min(ub.values())
+                agree = sum(v - best <= eps for v in ub.values())
+                if agree >= max(2, math.ceil(fraction * len(ub) - 1e-9)):
+                    stop[li] = k
+        return stop
+
+    # the spread of the two best units at each checkpoint: an eps between
+    # spreads makes a ligand stop at a later checkpoint than the first
+    spreads = []
+    for k in range(first, n):
+        for li in (0, 2):
+            ub = sorted(min(runs[i, k][0].min() for i, ch in enumerate(chains) if ch[0] == li and ch[1] == u)
+                        for u in range(3))
+            spreads.append(ub[1] - ub[0])
+    cases = [0.0, 1e9] + sorted(set(spreads))[:2]
+    confs = np.zeros((len(chains), 64))
+    for i, ch in enumerate(chains):
+        confs[i, :len(ch[2])] = ch[2]
+    seen = set()
+    for eps in cases:
+        stop = expected_stop(eps)
+        seen.add(tuple(sorted(stop.items())))
+        print(f"early stop eps {eps:.6g}: stops {stop}", flush=True)
+        p = sg.mc_params()
+        p.early_stop_checkpoints, p.early_stop_first = n, first
+        p.early_stop_fraction, p.early_stop_eps = fraction, eps
+        cnt, e, c, x, ev = ctx.run_mc(env["bids"][sg.FP64], env["gids"][sg.FP64], p, [ch[0] for ch in chains],
+                                      confs, np.array([ch[3] for ch in chains]), [ch[4] for ch in chains],
+                                      [ch[5] for ch in chains], units=[ch[1] for ch in chains])
+        for i, ch in enumerate(chains):
+            eo, co, xo, evo = runs[i, stop.get(ch[0], n)]
+            assert cnt[i] == len(eo) and np.array_equal(e[i, :cnt[i]], eo), (eps, i)
+            assert np.array_equal(c[i, :cnt[i], :co.shape[1]], co) and ev[i] == evo, (eps, i)
+    assert len(seen) >= 2  # the cases stop differently
+
+
 def test_mc_fp64_batched_models(env):
     """chains of different ligands in one launch follow their own CPU chains"""
     ctx, o = env["ctx"], env["o"]
@@ -282,20 +354,24 @@ def test_minimize_fp64_matches_oracle():
         assert r["e"] == e, (r["e"], e)
 
 
-@pytest.mark.parametrize("lig,exh,frac", [((3, 10, 0), 8, 1.0), ((1, 24, 6), 4, 0.1)])
-def test_dock_fp64_matches_oracle(lig, exh, frac):
+@pytest.mark.parametrize("lig,exh,frac,split", [((3, 10, 0), 8, 1.0, 1), ((1, 24, 6), 4, 0.1, 1),
+                                                ((1, 24, 6), 2, 0.1, 4)])
+def test_dock_fp64_matches_oracle(lig, exh, frac, split):
     """the single-ligand docking end to end through the host facade
     (GpuDocking::dock_batch: seeded chains, merge at 2 A, refinement,
     finish_docking's FP64 rescoring, mode selection) in FP64, against the
     oracle's dock (parallel_mc + refine_structure + finish_docking,
     src/lib/parallel_mc.cpp:80-98, src/main/main.cpp:256-324); the second
-    case cuts every chain to a tenth of its steps on both sides"""
+    case cuts every chain to a tenth of its steps on both sides, the third
+    also splits every chain into 4 (--gpu_split: 4 x exhaustiveness chains
+    with a quarter of those steps each, src/gpu/gpu_docking.cpp:200-220)"""
     rec, l = receptor(), sg.Mol.ligand(*lig)
-    d = sg.Docking(rec, gd_c0(), fp64=True, exhaustiveness=exh, seed=5, step_fraction=frac)
+    d = sg.Docking(rec, gd_c0(), fp64=True, exhaustiveness=exh, seed=5, step_fraction=frac, chain_split=split)
     r = d.dock([l])[0]
     o, ids, _, _ = oracle_for([l])
-    e, c, ev = o.dock(ids[0], sg.mc_params(), seed=5, exhaustiveness=exh, threads=exh, step_fraction=frac)
-    print(f"dock {lig}: {r['e']}", flush=True)
+    e, c, ev = o.dock(ids[0], sg.mc_params(), seed=5, exhaustiveness=exh, threads=8, step_fraction=frac,
+                      split=split)
+    print(f"dock {lig} split {split}: {r['e']}", flush=True)
     assert len(r["e"]) > 0 and r["e"] == list(e)
 
 
