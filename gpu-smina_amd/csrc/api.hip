@@ -1258,8 +1258,16 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 	L.off_ro = take(MAX_TREES * 3 * sizeof(T));
 	L.off_pbuf = take((sizeof(T) == 8 ? size_t(L.pair_stride) : 1) * sizeof(T));
 	L.off_ev = take(sizeof(int64_t));
+	// QuickVina 2's history: 5n points of n coordinates, n up to the batch's
+	// largest change size and at most 64 (chain_layout,
+	// src/gpu/gpu_mc_structs.h:97-152)
+	L.vis_n = p->visited ? std::min(b.h.plan.gmax, 64) : 0;
+	L.off_vx = take(size_t(5) * L.vis_n * L.vis_n * sizeof(S));
+	L.off_vf = take(size_t(5) * L.vis_n * sizeof(S));
+	L.off_vb = take(size_t(10) * L.vis_n * sizeof(uint64_t));
+	L.off_vs = take(3 * sizeof(int));
 	L.bytes = (off + 63) / 64 * 64;
-	// initial chain state
+	// initial chain state (the history empty: zeros)
 	std::vector<char> init(L.bytes * n, 0);
 	for (int i = 0; i < n; i++)
 	{
@@ -1325,7 +1333,7 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 			const int bytes = plan_bytes(k.plan, b.precision, w);
 			if (bytes > LDS_LIMIT)
 				break;
-			const int resident = w * mc_blocks_per_cu<T, S>(w, size_t(bytes));
+			const int resident = w * mc_blocks_per_cu<T, S>(w, size_t(bytes), p->visited != 0);
 			if (resident >= best && resident > 0)
 			{
 				best = resident;
@@ -1392,6 +1400,7 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 	}
 	P.accurate = p->accurate_line_search;
 	P.early_term = p->early_term;
+	P.visited = p->visited;
 	int max_steps = 0;
 	for (int i = 0; i < n; i++)
 		max_steps = std::max(max_steps, ch->num_steps[i]);

@@ -170,6 +170,124 @@ __device__ inline void stage(SearchMem<S>& W, S x, S p, int c, int n)
 	lsync();
 }
 
+// QuickVina 2's history of visited points (not smina's search; the
+// GPU-style --fast search, src/gpu/gpu_mc.cuh:146-264; Alhossary et al.,
+// Bioinformatics 2015): a ring of the last 5n points a chain's local
+// optimizations visited (n = change size <= 64), each with its energy and
+// the masks of its zero and positive gradient components.  Points are stored
+// coordinate-major, x[i * stride + e]: the lanes of a wave read 64 entries of
+// one coordinate in one coalesced load.  The ring's state (size, next slot,
+// full) is uniform and lives in registers during a launch.
+template<typename S>
+struct Visited
+{
+	S* x;            // null: no history
+	S* f;
+	uint64_t* bits;  // [2e] zero mask, [2e + 1] positive mask
+	int stride;      // 5 x the largest change size the storage holds
+	int size, next, full;
+
+	__device__ __forceinline__ bool on(int n) const { return x != nullptr && 5 * n <= stride; }
+
+	// visited::interesting: until the ring is full every point is;
+	// otherwise the nearest stored point (by squared distance, then slot)
+	// that passes the first-order consistency check must rank below 4n.  A
+	// point passes unless a coordinate where both gradients have the same
+	// nonzero sign breaks (g > 0) == (now > x_e) xor (f > f_e).  now, g:
+	// this lane's change-indexed component.
+	__device__ __forceinline__ bool interesting(int n, S now, S fv, S g) const
+	{
+		if (!full)
+			return true;
+		const int cap = 5 * n, lane = lane_id();
+		S d[5];
+		S best = Real<S>::max();
+		int besti = cap;
+#pragma unroll
+		for (int k = 0; k < 5; k++)
+		{
+			const int e = lane + WAVE * k;
+			const bool valid = e < cap;
+			const bool newY = valid && fv - f[e] > 0;
+			const uint64_t zb = valid ? bits[2 * e] : 0, pb = valid ? bits[2 * e + 1] : 0;
+			S d2 = 0;
+			bool pass = true;
+			for (int i = 0; i < n; i++)
+			{
+				const S nowi = readlane(now, i), gi = readlane(g, i);
+				const S xe = valid ? x[size_t(i) * stride + e] : S(0);
+				const S dx = xe - nowi;
+				d2 += dx * dx;
+				if (((zb >> i) & 1) || gi == 0)
+					continue;
+				const bool nowPos = gi > 0;
+				if (nowPos != bool((pb >> i) & 1))
+					continue;
+				const bool newX = nowi - xe > 0;
+				if (nowPos ^ newX ^ newY)
+					pass = false;
+			}
+			d[k] = d2;
+			if (valid && pass && d2 < best)
+			{
+				best = d2;
+				besti = e;
+			}
+		}
+		for (int off = WAVE / 2; off >= 1; off >>= 1)
+		{
+			const S ob = __shfl_xor(best, off);
+			const int oi = __shfl_xor(besti, off);
+			if (ob < best || (ob == best && oi < besti))
+			{
+				best = ob;
+				besti = oi;
+			}
+		}
+		if (besti >= cap)
+			return false;
+		int ahead = 0;
+#pragma unroll
+		for (int k = 0; k < 5; k++)
+		{
+			const int e = lane + WAVE * k;
+			ahead += __popcll(__ballot(e < cap && (d[k] < best || (d[k] == best && e < besti))));
+		}
+		return ahead < 4 * n;
+	}
+
+	// visited_add: the point takes the next slot of the ring
+	__device__ __forceinline__ void add(int n, S xv, S fv, S g)
+	{
+		const int cap = 5 * n, lane = lane_id();
+		int slot;
+		if (!full)
+		{
+			slot = size++;
+			if (size >= cap)
+			{
+				full = 1;
+				next = 0;
+			}
+		}
+		else
+		{
+			slot = next;
+			next = next + 1 == cap ? 0 : next + 1;
+		}
+		const uint64_t zero = __ballot(lane < n && g == 0), pos = __ballot(lane < n && g > 0);
+		if (lane == 0)
+		{
+			f[slot] = fv;
+			bits[2 * slot] = zero;
+			bits[2 * slot + 1] = pos;
+		}
+		if (lane < n)
+			x[size_t(lane) * stride + slot] = xv;
+		gsync();
+	}
+};
+
 // bfgs (src/lib/bfgs.h:201-282) with fast_line_search (:76-96) or
 // accurate_line_search (:101-190), written as one loop around a single
 // model evaluation: the kernels inline the evaluation once.  Each pass of
@@ -177,7 +295,7 @@ __device__ inline void stage(SearchMem<S>& W, S x, S p, int c, int n)
 // point, then the line search's trials; a finished line search runs the
 // BFGS step body and starts the next line search.  The order of every
 // evaluation and every arithmetic operation is the CPU's.
-template<typename T, typename S, int BK>
+template<typename T, typename S, int BK, bool VIS = false>
 struct Searcher
 {
 	const MV<T>& m;
@@ -186,6 +304,18 @@ struct Searcher
 	SearchMem<S>& SM;
 	T* pbuf;
 	int64_t evals;
+	Visited<S> vis; // QuickVina 2's history (vis.x null: none)
+
+	// this lane's change-indexed component of the conf x
+	__device__ __forceinline__ S change_value(S x)
+	{
+		if (lane_id() < m.h.conf_size)
+			SM.xstage[lane_id()] = x;
+		lsync();
+		const S v = conf_value(m, SM);
+		lsync();
+		return v;
+	}
 
 	// the staged conf x plus alpha * p, this lane's element
 	__device__ __forceinline__ S trial_conf(S x, S p, S alpha)
@@ -196,11 +326,14 @@ struct Searcher
 		return xn;
 	}
 
-	// x, g: this lane's elements; returns the final energy
+	// x, g: this lane's elements; returns the final energy.  With a history
+	// (src/gpu/gpu_mc.cuh:440-515): an uninteresting start returns its
+	// energy unoptimized; the start and every completed step join the history.
 	__device__ __forceinline__ S bfgs(S& x, S& g, int maxiters, bool accurate, bool early_term)
 	{
 		const int n = m.h.change_size, c = m.h.conf_size;
 		const int lane = lane_id();
+		const bool history = VIS && vis.on(n);
 		// identity
 		const HessMap hm = hess_map(n > 0 ? n : 1);
 		for (int r0 = 0; r0 < n; r0 += hm.rows)
@@ -234,6 +367,13 @@ struct Searcher
 				x_orig = x;
 				x_new = x;
 				g_new = g;
+				if (history)
+				{
+					const S xv = change_value(x);
+					if (!vis.interesting(n, xv, f0, g))
+						break;
+					vis.add(n, xv, f0, g);
+				}
 				if (maxiters <= 0)
 					break;
 				begin = true;
@@ -345,6 +485,8 @@ struct Searcher
 					}
 				}
 				bfgs_update(SM.hess, SM.pstage, SM.smhy, p, y, alpha, n, hm);
+				if (history)
+					vis.add(n, change_value(x), f0, g);
 				begin = ++step < maxiters;
 				if (!begin)
 					break;
@@ -482,6 +624,10 @@ struct ChainState
 	T* root_origin;   // [MAX_TREES * 3]
 	T* pbuf;          // pair-energy scratch (ordered mode)
 	int64_t* evals;
+	S* vis_x;         // QuickVina 2's history (null: none)
+	S* vis_f;
+	uint64_t* vis_bits;
+	int* vis_state;   // size, next slot, full
 };
 
 struct ChainLayout
@@ -493,6 +639,8 @@ struct ChainLayout
 	int max_poses;
 	size_t bytes;      // per chain
 	size_t off_mt, off_ints, off_tmp, off_scal, off_oute, off_outc, off_outx, off_mc, off_ro, off_pbuf, off_ev;
+	int vis_n;         // QuickVina 2's history for change sizes up to this (0: none)
+	size_t off_vx, off_vf, off_vb, off_vs;
 };
 
 template<typename T, typename S>
@@ -511,6 +659,10 @@ __device__ inline ChainState<T, S> chain_state(char* base, const ChainLayout& L,
 	s.root_origin = reinterpret_cast<T*>(p + L.off_ro);
 	s.pbuf = reinterpret_cast<T*>(p + L.off_pbuf);
 	s.evals = reinterpret_cast<int64_t*>(p + L.off_ev);
+	s.vis_x = L.vis_n > 0 ? reinterpret_cast<S*>(p + L.off_vx) : nullptr;
+	s.vis_f = reinterpret_cast<S*>(p + L.off_vf);
+	s.vis_bits = reinterpret_cast<uint64_t*>(p + L.off_vb);
+	s.vis_state = reinterpret_cast<int*>(p + L.off_vs);
 	return s;
 }
 
@@ -524,6 +676,7 @@ struct McParams
 	double authentic[3];
 	int accurate;
 	int early_term;
+	int visited;  // QuickVina 2's history check
 };
 
 // model::gyration_radius (src/lib/model.cpp:868-883) on the model's current

@@ -710,6 +710,7 @@ void apply_fast(GpuSettings& s)
 {
 	s.chain_split = 0;
 	s.step_fraction = 0;
+	s.skip_visited = true;
 	if (s.early_stop == GpuSettings::EARLY_STOP_OFF)
 		s.early_stop = GpuSettings::EARLY_STOP_FULL;
 	s.fast_top = 0.1;
@@ -996,6 +997,7 @@ void GpuDocking::dock_sub(DockJob* const* jobs, size_t nj, double budget)
 		mp.authentic_v[k] = 1000;
 	}
 	mp.cache_slope = 1e6;
+	mp.visited = s_.skip_visited ? 1 : 0;
 	// stops and stop_for (src/gpu/gpu_docking.cpp:172-189): not in a
 	// first pass; checkpoints after every tenth of smina's steps the chains
 	// run, the rule from the second on
@@ -1310,6 +1312,7 @@ sgpu_docking* sgpu_docking_create(const sgpu_dock_settings* s, const sgpu_mol* r
 			throw std::invalid_argument("chain_split must be at least 1 (0: default)");
 		gs.chain_split = s->chain_split > 0 ? s->chain_split : 1;
 		gs.early_stop = s->early_stop ? GpuSettings::EARLY_STOP_ALWAYS : GpuSettings::EARLY_STOP_OFF;
+		gs.skip_visited = s->skip_visited != 0;
 		if (s->fast)
 		{
 			// explicit chain_split / step_fraction take precedence

@@ -94,6 +94,9 @@ struct GpuSettings
 	// is full, chain split below 32)
 	int chain_split = 1;
 	double step_fraction = 1;
+	// QuickVina 2's history check (--gpu_qvina): local optimizations that
+	// would revisit known minima are skipped
+	bool skip_visited = false;
 	enum { EARLY_STOP_OFF, EARLY_STOP_FULL, EARLY_STOP_ALWAYS };
 	int early_stop = EARLY_STOP_OFF;
 	// two-pass screening (src/main/main.cpp:1563-1597): every ligand with
@@ -110,9 +113,8 @@ struct GpuSettings
 };
 
 // the settings --fast selects (src/main/main.cpp:1451-1461, :1426-1431):
-// automatic chain split and step fraction, early stopping while the device is
-// full, two-pass screening of the best 10%.  QuickVina 2's history check,
-// which --fast also turns on, is not carried by this path.
+// automatic chain split and step fraction, QuickVina 2's history check, early
+// stopping while the device is full, two-pass screening of the best 10%.
 void apply_fast(GpuSettings& s);
 
 class GpuDocking

@@ -52,7 +52,7 @@ class McParams(C.Structure):
                 ("num_saved_mins", C.c_int), ("hunt_cap", C.c_double * 3), ("authentic_v", C.c_double * 3),
                 ("cache_slope", C.c_double), ("accurate_line_search", C.c_int), ("early_term", C.c_int),
                 ("early_stop_checkpoints", C.c_int), ("early_stop_first", C.c_int),
-                ("early_stop_fraction", C.c_double), ("early_stop_eps", C.c_double)]
+                ("early_stop_fraction", C.c_double), ("early_stop_eps", C.c_double), ("visited", C.c_int)]
 
 
 class Chains(C.Structure):
@@ -115,7 +115,7 @@ class DockSettings(C.Structure):
                 ("num_modes", C.c_int), ("energy_range", C.c_double), ("hydrophobic_scale", C.c_double),
                 ("ligands_per_launch", C.c_int), ("step_fraction", C.c_double), ("progress_chunks", C.c_int),
                 ("progress", PROGRESS_FN), ("progress_user", C.c_void_p), ("scoring", C.c_char_p),
-                ("chain_split", C.c_int), ("early_stop", C.c_int), ("fast", C.c_int)]
+                ("chain_split", C.c_int), ("early_stop", C.c_int), ("fast", C.c_int), ("skip_visited", C.c_int)]
 
 
 class ServerSettings(C.Structure):
@@ -496,8 +496,9 @@ class Context:
         self.check(lib.sgpu_set_progress(self.h, chunks, self._progress, None))
 
 
-def mc_params(hunt=10.0, authentic=1000.0, num_saved=20):
+def mc_params(hunt=10.0, authentic=1000.0, num_saved=20, visited=False):
     p = McParams()
+    p.visited = int(visited)
     p.temperature = 1.2
     p.mutation_amplitude = 2.0
     p.min_rmsd = 1.0
@@ -522,17 +523,19 @@ class Docking:
 
     def __init__(self, receptor, gd, fp64=False, exhaustiveness=8, seed=1, num_modes=9, energy_range=3.0,
                  hydrophobic_scale=1.0, ligands_per_launch=0, device=0, step_fraction=1.0, precision=None,
-                 progress=None, progress_chunks=1, scoring="vina", chain_split=0, early_stop=False, fast=False):
+                 progress=None, progress_chunks=1, scoring="vina", chain_split=0, early_stop=False, fast=False,
+                 skip_visited=False):
         """precision: FP32, FP64 or MIXED (default: FP64 if fp64 else FP32);
         progress(fraction): called after each of progress_chunks launches
-        of the Monte Carlo search; chain_split, early_stop, fast: the
-        GPU-style search options --gpu_split, --gpu_early_stop, --fast"""
+        of the Monte Carlo search; chain_split, early_stop, skip_visited,
+        fast: the GPU-style search options --gpu_split, --gpu_early_stop,
+        --gpu_qvina, --fast"""
         prec = precision if precision is not None else (FP64 if fp64 else FP32)
         # keep the callback alive as long as the docking object
         self._progress = PROGRESS_FN(lambda user, frac: progress(frac)) if progress else PROGRESS_FN()
         s = DockSettings(device, prec, exhaustiveness, seed, num_modes, energy_range, hydrophobic_scale,
                          ligands_per_launch, step_fraction, progress_chunks, self._progress, None,
-                         scoring.encode(), chain_split, int(early_stop), int(fast))
+                         scoring.encode(), chain_split, int(early_stop), int(fast), int(skip_visited))
         self.gd = gd
         self.h = C.c_void_p(lib.sgpu_docking_create(C.byref(s), receptor.h, C.byref(gd)))
         err = lib.sgpu_docking_error(self.h)

@@ -146,6 +146,12 @@ typedef struct sgpu_mc_params
 	int early_stop_first;
 	double early_stop_fraction; /* 0.5 */
 	double early_stop_eps;      /* 0.05 kcal/mol */
+	/* QuickVina 2's history check (not smina's search; MCParams::visited,
+	 * src/gpu/gpu_mc.cuh:146-264, --gpu_qvina): a local optimization whose
+	 * start is not "interesting" to the chain's last 5n visited points is
+	 * skipped (n = degrees of freedom, at most 64; larger models keep no
+	 * history).  0: off. */
+	int visited;
 } sgpu_mc_params;
 
 enum { SGPU_MAX_POSES = 64 };

@@ -91,8 +91,9 @@ typedef struct sgpu_dock_settings
 	                         * steps (0: 1, or automatic with fast) */
 	int early_stop;         /* --gpu_early_stop: stop a ligand once its units agree on its best energy */
 	int fast;               /* --fast: automatic split and step fraction (unless chain_split > 0 or
-	                         * step_fraction < 1), early stopping while the device is full, two-pass
-	                         * screening of the best 10% (QuickVina 2's history check not carried) */
+	                         * step_fraction < 1), QuickVina 2's history check, early stopping while the
+	                         * device is full, two-pass screening of the best 10% */
+	int skip_visited;       /* --gpu_qvina: QuickVina 2's history check (sgpu_mc_params.visited) */
 } sgpu_dock_settings;
 
 sgpu_docking* sgpu_docking_create(const sgpu_dock_settings* s, const sgpu_mol* receptor,

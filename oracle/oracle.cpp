@@ -1621,9 +1621,95 @@ fl accurate_line_search(const F& f, const Ops& ops, int n, const fl* x, const fl
 		alpha = std::max(tmplam, (fl)0.1 * alpha);
 	}
 }
-// bfgs.h:201-282
+// QuickVina 2's history of visited points (Alhossary et al., Bioinformatics
+// 2015, github.com/QVina/qvina branch qvina2: visited.h, visited.cpp; as the
+// GPU-style search restates it, src/gpu/gpu_mc.cuh:146-264).  Not smina's
+// search.  A chain keeps the last 5n points its local optimizations visited
+// (n degrees of freedom, conf in change indexing), each with its energy and
+// the signs of its gradient, in a ring of 5n slots filled in order.
+struct Visited
+{
+	int n, cap, size = 0, next = 0;
+	bool full = false;
+	std::vector<fl> x, f;          // x: [cap][n]
+	std::vector<signed char> sign; // [cap][n]: -1, 0 (zero gradient) or +1; NaN: -1
+	explicit Visited(int n_) : n(n_), cap(5 * n_), x(size_t(cap) * n_), f(cap), sign(size_t(cap) * n_) {}
+	void add(const fl* xv, fl fv, const fl* g)
+	{
+		int slot;
+		if (!full)
+		{
+			slot = size++;
+			if (size >= cap)
+			{
+				full = true;
+				next = 0;
+			}
+		}
+		else
+		{
+			slot = next;
+			next = (next + 1) % cap;
+		}
+		f[size_t(slot)] = fv;
+		for (int i = 0; i < n; i++)
+		{
+			x[size_t(slot) * n + i] = xv[i];
+			sign[size_t(slot) * n + i] = g[i] == 0 ? 0 : g[i] > 0 ? 1 : -1;
+		}
+	}
+	// visited::interesting in QuickVina 2's own form: until the ring is full
+	// every point is; otherwise the 4n stored points nearest to xv, in order
+	// of (squared distance, slot), are checked and the first that passes the
+	// first-order consistency check makes xv interesting.  The check: in
+	// every coordinate where both gradients have the same nonzero sign,
+	// (gradient > 0) must equal (xv > x_e) xor (fv > f_e).
+	bool interesting(const fl* xv, fl fv, const fl* g) const
+	{
+		if (!full)
+			return true;
+		std::vector<std::pair<fl, int>> order;
+		for (int e = 0; e < cap; e++)
+		{
+			fl d2 = 0;
+			for (int i = 0; i < n; i++)
+			{
+				const fl dx = x[size_t(e) * n + i] - xv[i];
+				d2 += dx * dx;
+			}
+			order.push_back({std::isnan(d2) ? max_fl : d2, e});
+		}
+		std::sort(order.begin(), order.end());
+		for (int r = 0; r < 4 * n; r++)
+		{
+			const int e = order[size_t(r)].second;
+			const bool newY = fv - f[size_t(e)] > 0;
+			bool pass = true;
+			for (int i = 0; i < n; i++)
+			{
+				const int se = sign[size_t(e) * n + i];
+				if (se == 0 || g[i] == 0)
+					continue;
+				const bool nowPos = g[i] > 0;
+				if (nowPos != (se > 0))
+					continue;
+				const bool newX = xv[i] - x[size_t(e) * n + i] > 0;
+				if (nowPos ^ newX ^ newY)
+					pass = false;
+			}
+			if (pass)
+				return true;
+		}
+		return false;
+	}
+};
+
+// bfgs.h:201-282; with a history (QuickVina 2, src/gpu/gpu_mc.cuh:440-515):
+// an uninteresting start returns its energy unoptimized, and the start and
+// every completed step join the history
 template<typename F, typename Ops>
-fl bfgs(const F& f, const Ops& ops, fl* x, fl* g, int maxiters, bool accurate, bool early_term)
+fl bfgs(const F& f, const Ops& ops, fl* x, fl* g, int maxiters, bool accurate, bool early_term,
+		Visited* vis = nullptr)
 {
 	const int n = ops.change_size();
 	const int cs = ops.conf_size();
@@ -1632,6 +1718,18 @@ fl bfgs(const F& f, const Ops& ops, fl* x, fl* g, int maxiters, bool accurate, b
 		h.at(i, i) = 1;
 	std::vector<fl> g_new(g, g + n), x_new(x, x + cs);
 	fl f0 = f(x, g);
+	std::vector<fl> xv(vis ? n : 0);
+	auto values = [&]() {
+		for (int i = 0; i < n; i++)
+			xv[size_t(i)] = ops.value(x, i);
+		return xv.data();
+	};
+	if (vis)
+	{
+		if (!vis->interesting(values(), f0, g))
+			return f0;
+		vis->add(xv.data(), f0, g);
+	}
 	fl f_orig = f0;
 	std::vector<fl> g_orig(g, g + n), x_orig(x, x + cs);
 	std::vector<fl> p(g, g + n), y(n);
@@ -1672,6 +1770,8 @@ fl bfgs(const F& f, const Ops& ops, fl* x, fl* g, int maxiters, bool accurate, b
 			}
 		}
 		bfgs_update(h, p.data(), y.data(), alpha);
+		if (vis)
+			vis->add(values(), f0, g);
 	}
 	if (!(f0 <= f_orig))
 	{
@@ -1873,6 +1973,9 @@ void mc_chain(const Ctx& ctx, Model& m, const sgpu_mc_params& p, int num_steps, 
 	std::vector<fl> g(n, 0);
 	Output tmp{start, 0, {}};
 	fl best_e = max_fl;
+	// QuickVina 2's history: models of at most 64 degrees of freedom
+	Visited history(n);
+	Visited* vis = p.visited && n <= 64 ? &history : nullptr;
 	Objective hunt{&ctx, &m, 0, p.cache_slope, {p.hunt_cap[0], p.hunt_cap[1], p.hunt_cap[2]}, evals};
 	Objective auth{&ctx, &m, 0, p.cache_slope, {authentic_v[0], authentic_v[1], authentic_v[2]}, evals};
 	for (int step = 0; step < num_steps; step++)
@@ -1880,14 +1983,15 @@ void mc_chain(const Ctx& ctx, Model& m, const sgpu_mc_params& p, int num_steps, 
 		Output candidate = tmp;
 		mutate_conf(candidate.c.data(), m, p.mutation_amplitude, gen);
 		candidate.e = bfgs(hunt, ops, candidate.c.data(), g.data(), maxiters, p.accurate_line_search,
-				p.early_term);
+				p.early_term, vis);
 		if (step == 0 || metropolis_accept(tmp.e, candidate.e, p.temperature, gen))
 		{
 			tmp = candidate;
 			m.set(tmp.c.data());
 			if (tmp.e < best_e || out.size() < size_t(p.num_saved_mins))
 			{
-				tmp.e = bfgs(auth, ops, tmp.c.data(), g.data(), maxiters, p.accurate_line_search, p.early_term);
+				tmp.e = bfgs(auth, ops, tmp.c.data(), g.data(), maxiters, p.accurate_line_search, p.early_term,
+						vis);
 				m.set(tmp.c.data());
 				tmp.coords = m.heavy_coords();
 				add_to_output_container(out, tmp, p.min_rmsd, size_t(p.num_saved_mins));
@@ -2197,6 +2301,15 @@ double orc_bfgs_quadratic(int n, const double* A, const double* b, double* x, in
 	FlatOps ops{n};
 	std::vector<double> g(n, 0);
 	return bfgs(q, ops, x, g.data(), maxiters, accurate != 0, false);
+}
+
+int orc_visited_check(int n, int count, const double* xs, const double* fs, const double* gs, const double* xv,
+		double fv, const double* g)
+{
+	Visited v(n);
+	for (int k = 0; k < count; k++)
+		v.add(xs + size_t(k) * n, fs[k], gs + size_t(k) * n);
+	return v.interesting(xv, fv, g) ? 1 : 0;
 }
 
 double orc_refine(orc_ctx* c, int model, const double* v, int maxiters, double* conf, double* coords)

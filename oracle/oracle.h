@@ -78,6 +78,13 @@ double orc_bfgs(orc_ctx* c, int model, int backend, double slope, const double* 
 double orc_bfgs_quadratic(int n, const double* A, const double* b, double* x, int maxiters, int accurate,
 		int64_t* evals);
 
+/* QuickVina 2's history check (src/gpu/gpu_mc.cuh:146-264): a history of n
+ * degrees of freedom fed `count` points in order (xs [count][n], fs, gs
+ * [count][n]; the ring keeps the last 5n), then whether (xv, fv, g) is
+ * interesting: 1 or 0 */
+int orc_visited_check(int n, int count, const double* xs, const double* fs, const double* gs, const double* xv,
+		double fv, const double* g);
+
 /* chain initialisation (make_chain / conf::randomize): seeds rng(seed),
  * randomizes a conf in the box, returns the generator state (625 words) */
 void orc_randomize(orc_ctx* c, int model, uint32_t seed, double* conf, uint32_t* rng_state);

@@ -49,6 +49,7 @@ def _load():
         "orc_bfgs": (C.c_double, [vp, C.c_int, C.c_int, C.c_double, F64P, C.c_int, C.c_int, C.c_int, F64P, I64P]),
         "orc_bfgs_quadratic": (C.c_double, [C.c_int, F64P, F64P, F64P, C.c_int, C.c_int, I64P]),
         "orc_randomize": (None, [vp, C.c_int, C.c_uint32, F64P, U32P]),
+        "orc_visited_check": (C.c_int, [C.c_int, C.c_int, F64P, F64P, F64P, F64P, C.c_double, F64P]),
         "orc_mc_chain": (C.c_int, [vp, C.c_int, vp, C.c_int, C.c_int, F64P, U32P, C.c_int, F64P, F64P, F64P,
                                    I64P]),
         "orc_refine": (C.c_double, [vp, C.c_int, F64P, C.c_int, F64P, F64P]),
@@ -68,6 +69,19 @@ lib = _load()
 
 def _p(a, t):
     return a.ctypes.data_as(t)
+
+
+def visited_check(points, energies, grads, x, f, g):
+    """QuickVina 2's history fed `points` in order, then whether (x, f, g)
+    is interesting"""
+    points = np.ascontiguousarray(points, dtype=np.float64)
+    energies = np.ascontiguousarray(energies, dtype=np.float64)
+    grads = np.ascontiguousarray(grads, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    g = np.ascontiguousarray(g, dtype=np.float64)
+    n = x.size
+    return bool(lib.orc_visited_check(n, len(energies), _p(points, F64P), _p(energies, F64P), _p(grads, F64P),
+                                      _p(x, F64P), float(f), _p(g, F64P)))
 
 
 def bfgs_quadratic(A, b, x0, maxiters, accurate=False):

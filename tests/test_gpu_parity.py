@@ -257,6 +257,33 @@ def test_mc_fp64_early_stop(env):
     assert len(seen) >= 2  # the cases stop differently
 
 
+@pytest.mark.parametrize("li", [0, 1, 2])
+def test_mc_fp64_visited_history(env, li):
+    """QuickVina 2's history check (--gpu_qvina, src/gpu/gpu_mc.cuh:146-264):
+    FP64 chains long enough for the ring of 5n points to fill and wrap
+    follow the oracle's chains (QuickVina 2's own nearest-first form of the
+    check) exactly, and skip local optimizations: fewer evaluations than
+    the same chains without the history"""
+    ctx, o, f = env["ctx"], env["o"], env["flats"][li]
+    steps, n = 40, 4
+    cs, ss = zip(*[f.make_chain(8100 + 10 * li + k, env["gd"]) for k in range(n)])
+    maxit = (25 + f.natoms) // 3
+    confs = np.zeros((n, 64))
+    for k in range(n):
+        confs[k, :len(cs[k])] = cs[k]
+    p = sg.mc_params(visited=True)
+    cnt, e, c, x, ev = ctx.run_mc(env["bids"][sg.FP64], env["gids"][sg.FP64], p, [li] * n, confs, np.array(ss),
+                                  [steps] * n, [maxit] * n, coord_stride=3 * f.nheavy)
+    plain = ctx.run_mc(env["bids"][sg.FP64], env["gids"][sg.FP64], sg.mc_params(), [li] * n, confs, np.array(ss),
+                       [steps] * n, [maxit] * n, coord_stride=3 * f.nheavy)[4]
+    print(f"history: ligand {li}, n {f.change_size}, evals {list(ev)} vs {list(plain)} without", flush=True)
+    for k in range(n):
+        eo, co, xo, evo = o.mc_chain(env["ids"][li], p, steps, maxit, cs[k], ss[k])
+        assert cnt[k] == len(eo) and np.array_equal(e[k, :cnt[k]], eo), k
+        assert np.array_equal(c[k, :cnt[k], :co.shape[1]], co) and ev[k] == evo, k
+    assert ev.sum() < plain.sum()
+
+
 def test_mc_fp64_batched_models(env):
     """chains of different ligands in one launch follow their own CPU chains"""
     ctx, o = env["ctx"], env["o"]
@@ -354,9 +381,9 @@ def test_minimize_fp64_matches_oracle():
         assert r["e"] == e, (r["e"], e)
 
 
-@pytest.mark.parametrize("lig,exh,frac,split", [((3, 10, 0), 8, 1.0, 1), ((1, 24, 6), 4, 0.1, 1),
-                                                ((1, 24, 6), 2, 0.1, 4)])
-def test_dock_fp64_matches_oracle(lig, exh, frac, split):
+@pytest.mark.parametrize("lig,exh,frac,split,qv", [((3, 10, 0), 8, 1.0, 1, False), ((1, 24, 6), 4, 0.1, 1, False),
+                                                   ((1, 24, 6), 2, 0.1, 4, False), ((1, 24, 6), 2, 0.1, 4, True)])
+def test_dock_fp64_matches_oracle(lig, exh, frac, split, qv):
     """the single-ligand docking end to end through the host facade
     (GpuDocking::dock_batch: seeded chains, merge at 2 A, refinement,
     finish_docking's FP64 rescoring, mode selection) in FP64, against the
@@ -364,14 +391,16 @@ def test_dock_fp64_matches_oracle(lig, exh, frac, split):
     src/lib/parallel_mc.cpp:80-98, src/main/main.cpp:256-324); the second
     case cuts every chain to a tenth of its steps on both sides, the third
     also splits every chain into 4 (--gpu_split: 4 x exhaustiveness chains
-    with a quarter of those steps each, src/gpu/gpu_docking.cpp:200-220)"""
+    with a quarter of those steps each, src/gpu/gpu_docking.cpp:200-220), the
+    fourth adds QuickVina 2's history check (--gpu_qvina)"""
     rec, l = receptor(), sg.Mol.ligand(*lig)
-    d = sg.Docking(rec, gd_c0(), fp64=True, exhaustiveness=exh, seed=5, step_fraction=frac, chain_split=split)
+    d = sg.Docking(rec, gd_c0(), fp64=True, exhaustiveness=exh, seed=5, step_fraction=frac, chain_split=split,
+                   skip_visited=qv)
     r = d.dock([l])[0]
     o, ids, _, _ = oracle_for([l])
-    e, c, ev = o.dock(ids[0], sg.mc_params(), seed=5, exhaustiveness=exh, threads=8, step_fraction=frac,
+    e, c, ev = o.dock(ids[0], sg.mc_params(visited=qv), seed=5, exhaustiveness=exh, threads=8, step_fraction=frac,
                       split=split)
-    print(f"dock {lig} split {split}: {r['e']}", flush=True)
+    print(f"dock {lig} split {split} history {qv}: {r['e']}", flush=True)
     assert len(r["e"]) > 0 and r["e"] == list(e)
 
 
