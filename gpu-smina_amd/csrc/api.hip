@@ -97,6 +97,11 @@ struct Tables
 	double factor = 0, cutoff_sqr = 0;
 	DBuf smooth32, smooth64, fast;
 	bool ok = false;
+	// precalculate_splines (kind 1)
+	int kind = 0;
+	DBuf spl, spl_cut;
+	double fraction = 0;
+	int nseg = 0;
 };
 
 struct Cells
@@ -560,7 +565,19 @@ TablePtrs<T> table_ptrs(const sgpu_ctx* c)
 	t.n = c->tab.n;
 	t.factor = c->tab.factor;
 	t.cutoff_sqr = c->tab.cutoff_sqr;
+	t.kind = c->tab.kind;
+	t.spl = c->tab.spl.as<double>();
+	t.spl_cut = c->tab.spl_cut.as<double>();
+	t.fraction = c->tab.fraction;
+	t.nseg = c->tab.nseg;
 	return t;
+}
+
+// the spline approximation runs on the FP64 direct receptor path only
+void require_linear(const sgpu_ctx* c, const char* what)
+{
+	require(c->tab.kind == 0, std::string(what) + " needs the linear tables (the spline approximation is carried "
+	                                              "by FP64 direct evaluation and refinement)");
 }
 
 template<typename T>
@@ -813,6 +830,8 @@ void sgpu_destroy(sgpu_ctx* c)
 	c->tab.smooth32.reset();
 	c->tab.smooth64.reset();
 	c->tab.fast.reset();
+	c->tab.spl.reset();
+	c->tab.spl_cut.reset();
 	c->scratch.reset();
 	c->scratch2.reset();
 	c->err_flag.reset();
@@ -858,6 +877,23 @@ sgpu_status sgpu_mc_capacity(sgpu_ctx* c, int precision, int* chains)
 	});
 }
 
+sgpu_status sgpu_upload_splines(sgpu_ctx* c, const sgpu_splines* s)
+{
+	return guarded(c, [&]() {
+		require(s && s->coef && s->cut && s->nseg >= 2 && s->nseg <= 100000 && s->fraction > 0 && s->cutoff_sqr > 0,
+				"invalid splines");
+		const size_t ncoef = size_t(SGPU_NUM_TYPE_PAIRS) * s->nseg * 5;
+		upload(c->tab.spl, std::vector<double>(s->coef, s->coef + ncoef), c->stream);
+		upload(c->tab.spl_cut, std::vector<double>(s->cut, s->cut + SGPU_NUM_TYPE_PAIRS), c->stream);
+		c->tab.fraction = s->fraction;
+		c->tab.nseg = s->nseg;
+		c->tab.cutoff_sqr = s->cutoff_sqr;
+		c->tab.kind = 1;
+		c->tab.ok = true;
+		hip_check(hipStreamSynchronize(c->stream), "upload splines");
+	});
+}
+
 sgpu_status sgpu_upload_tables(sgpu_ctx* c, const sgpu_tables* t)
 {
 	return guarded(c, [&]() {
@@ -871,6 +907,7 @@ sgpu_status sgpu_upload_tables(sgpu_ctx* c, const sgpu_tables* t)
 		c->tab.n = t->n;
 		c->tab.factor = t->factor;
 		c->tab.cutoff_sqr = t->cutoff_sqr;
+		c->tab.kind = 0;
 		c->tab.ok = true;
 		hip_check(hipStreamSynchronize(c->stream), "upload tables");
 	});
@@ -900,6 +937,7 @@ sgpu_status sgpu_build_grids(sgpu_ctx* c, int receptor_id, const sgpu_grid_dims*
 {
 	return guarded(c, [&]() {
 		require(c->tab.ok, "tables not uploaded");
+		require_linear(c, "the map build");
 		check_gd(gd);
 		require(grid_id && types && ntypes > 0 && ntypes <= 16, "1..16 map types");
 		require(precision == SGPU_FP32 || precision == SGPU_FP64, "invalid precision");
@@ -1623,6 +1661,8 @@ sgpu_status sgpu_eval_confs(sgpu_ctx* c, const sgpu_eval_request* q)
 		require(q != nullptr, "no request");
 		require(c->tab.ok, "tables not uploaded");
 		Batch& b = find_batch(c, q->batch_id);
+		if (b.precision != SGPU_FP64 || q->grid_id >= 0)
+			require_linear(c, "FP32 or map-based evaluation");
 		if (b.precision == SGPU_FP64)
 			run_eval<double>(c, q);
 		else
@@ -1635,6 +1675,7 @@ sgpu_status sgpu_run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_pa
 {
 	return guarded(c, [&]() {
 		require(c->tab.ok, "tables not uploaded");
+		require_linear(c, "the Monte Carlo search");
 		Batch& b = find_batch(c, batch_id);
 		if (b.precision == SGPU_FP64)
 			run_mc<double, double>(c, batch_id, grid_id, p, ch, out);
@@ -1651,6 +1692,8 @@ sgpu_status sgpu_run_refine(sgpu_ctx* c, const sgpu_refine_request* q)
 		require(q != nullptr, "no request");
 		require(c->tab.ok, "tables not uploaded");
 		Batch& b = find_batch(c, q->batch_id);
+		if (b.precision != SGPU_FP64)
+			require_linear(c, "FP32 refinement");
 		if (b.precision == SGPU_FP64)
 			run_refine<double, double>(c, q);
 		else if (b.precision == SGPU_MIXED)
@@ -1665,6 +1708,7 @@ sgpu_status sgpu_score(sgpu_ctx* c, const sgpu_score_request* q)
 	return guarded(c, [&]() {
 		require(q != nullptr, "no request");
 		require(c->tab.ok, "tables not uploaded");
+		require_linear(c, "the rescoring");
 		run_score(c, q);
 	});
 }

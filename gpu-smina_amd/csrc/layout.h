@@ -118,6 +118,12 @@ struct TablePtrs
 	int n;
 	double factor;
 	double cutoff_sqr;
+	// precalculate_splines (kind 1; FP64 direct evaluation only)
+	int kind;            // 0: linear tables, 1: splines
+	const double* spl;   // [378][nseg][5]: x, a, b, c, d
+	const double* spl_cut; // [378]: 0 for a zero spline
+	double fraction;     // segment length
+	int nseg;
 };
 
 // one receptor map set (cache back end)

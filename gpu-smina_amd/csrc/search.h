@@ -542,11 +542,38 @@ __device__ inline T grid_term(const GridPtrs<T>& g, int slot, V3<T> loc, T v, V3
 	return grid_finish(g, grid_fetch(g, slot, loc), v, deriv);
 }
 
+// precalculate_splines::eval_deriv, one component, no slow terms
+// (src/lib/precalculate.h:418-450) with Spline::eval_deriv
+// (src/lib/splines.h:103-121): the spline at r = sqrt(r2), its derivative
+// over r
+__device__ inline void spline_deriv(const TablePtrs<double>& tab, int tri, double r2, double& e, double& dor)
+{
+	const double r = __builtin_sqrt(r2);
+	double val = 0, der = 0;
+	if (r < tab.spl_cut[tri])
+	{
+		unsigned index = unsigned(r / tab.fraction);
+		index = index < unsigned(tab.nseg) ? index : unsigned(tab.nseg - 1);
+		const double* s = tab.spl + (size_t(tri) * tab.nseg + index) * 5;
+		const double lx = r - s[0];
+		val = ((s[1] * lx + s[2]) * lx + s[3]) * lx + s[4];
+		der = (3 * s[1] * lx + 2 * s[2]) * lx + s[3];
+	}
+	e = val;
+	dor = der / r;
+}
+
 // precalculate_linear_element::eval_deriv, one component
-// (src/lib/precalculate.h:111-151)
+// (src/lib/precalculate.h:111-151), or the spline's
 template<typename T>
 __device__ inline void table_deriv(const TablePtrs<T>& tab, int tri, T r2, T& e, T& dor)
 {
+	if constexpr (sizeof(T) == 8)
+		if (tab.kind == 1)
+		{
+			spline_deriv(tab, tri, r2, e, dor);
+			return;
+		}
 	const T r2f = T(tab.factor) * r2;
 	const int i1 = int(r2f);
 	const T rem = r2f - T(i1);
@@ -680,6 +707,19 @@ __device__ __forceinline__ T eval_deriv(const MV<T>& m, const EvalCtx<T>& ec, Wa
 					rz[u] = is_b ? xi.z - xj.z : xj.z - xi.z;
 					r2[u] = sqr(rx[u]) + sqr(ry[u]) + sqr(rz[u]);
 					// precalculate_linear_element::eval_deriv (src/lib/precalculate.h:111-151)
+					if (ec.tab.kind == 1)
+					{
+						// the spline's value and derivative over r, carried
+						// through the interpolation below unchanged (rem 0)
+						if (r2[u] < cutoff_sqr)
+							spline_deriv(ec.tab, code_tri(code), r2[u], e1[u], d1[u]);
+						else
+							e1[u] = d1[u] = 0;
+						rem[u] = 0;
+						e2[u] = e1[u];
+						d2[u] = d1[u];
+						continue;
+					}
 					const T r2f = T(ec.tab.factor) * r2[u];
 					const int i1 = r2[u] < cutoff_sqr ? int(r2f) : 0;
 					rem[u] = r2f - T(i1);

@@ -243,6 +243,92 @@ Tables build_tables(const sgpu_scoring& sf, double factor)
 	return t;
 }
 
+// The terms at nseg = factor x cutoff distances i x cutoff / nseg plus a
+// last point (cutoff, 0); per type pair the cubic spline through them with
+// zero slope at both ends (Spline::initialize, src/lib/splines.h:37-99).
+// Its second derivatives solve a tridiagonal system: row 0 is
+// 2h m0 + h m1 = 6 (y1 - y0) / h; interior row i is
+// hi m(i-1) + 2 (h + hi) mi + hi m(i+1) = 6 ((y(i+1) - yi) / hi - (yi - y(i-1)) / h)
+// with hi the last segment's length on the last interior row; the last row
+// is hl m(e-1) + 2 hl me = -6 (ye - y(e-1)) / hl.  The reference inverts the
+// dense matrix; elimination gives the same solution up to rounding.  Pairs
+// whose terms vanish at every point get no spline (cutoff 0).
+Splines build_splines(const sgpu_scoring& sf, double factor)
+{
+	if (!(factor > 0))
+		throw std::invalid_argument("spline factor must be positive");
+	Splines s;
+	const double cutoff = 8;
+	s.cutoff_sqr = cutoff * cutoff;
+	const unsigned np = unsigned(factor * cutoff);
+	if (np < 2)
+		throw std::invalid_argument("spline factor too small");
+	const int e = int(np);
+	s.nseg = e;
+	s.fraction = cutoff / double(np);
+	s.coef.assign(size_t(SGPU_NUM_TYPE_PAIRS) * e * 5, 0);
+	s.cut.assign(size_t(SGPU_NUM_TYPE_PAIRS), 0);
+	std::vector<double> x(e + 1), y(e + 1), lo(e + 1), di(e + 1), up(e + 1), rhs(e + 1), cp(e + 1), rp(e + 1),
+			m(e + 1);
+	for (int t1 = 0; t1 < NUM_ATOM_TYPES; t1++)
+		for (int t2 = t1; t2 < NUM_ATOM_TYPES; t2++)
+		{
+			bool nonzero = false;
+			for (int i = 0; i < e; i++)
+			{
+				x[i] = i * s.fraction;
+				double acc = 0;
+				for (int k = 0; k < sf.nterms; k++)
+					acc += sf.terms[k].weight * term(sf.terms[k], t1, t2, x[i], sf.atom_params);
+				y[i] = acc;
+				nonzero = nonzero || acc != 0;
+			}
+			x[e] = cutoff;
+			y[e] = 0;
+			if (!nonzero)
+				continue;
+			const double h = x[1] - x[0], hl = x[e] - x[e - 1];
+			di[0] = 2 * h;
+			up[0] = h;
+			rhs[0] = 6 * ((y[1] - y[0]) / h);
+			for (int i = 1; i < e; i++)
+			{
+				const double hi = i == e - 1 ? hl : h;
+				lo[i] = hi;
+				di[i] = 2 * (h + hi);
+				up[i] = hi;
+				rhs[i] = 6 * ((y[i + 1] - y[i]) / hi - (y[i] - y[i - 1]) / h);
+			}
+			lo[e] = hl;
+			di[e] = 2 * hl;
+			rhs[e] = 6 * (-(y[e] - y[e - 1]) / hl);
+			cp[0] = up[0] / di[0];
+			rp[0] = rhs[0] / di[0];
+			for (int i = 1; i <= e; i++)
+			{
+				const double piv = di[i] - lo[i] * cp[i - 1];
+				cp[i] = i < e ? up[i] / piv : 0;
+				rp[i] = (rhs[i] - lo[i] * rp[i - 1]) / piv;
+			}
+			m[e] = rp[e];
+			for (int i = e - 1; i >= 0; i--)
+				m[i] = rp[i] - cp[i] * m[i + 1];
+			const size_t tri = size_t(tri_index(t1, t2));
+			for (int i = 0; i < e; i++)
+			{
+				const double hi = i == e - 1 ? hl : h;
+				double* c = &s.coef[(tri * e + i) * 5];
+				c[0] = x[i];
+				c[1] = (m[i + 1] - m[i]) / (6 * hi);
+				c[2] = m[i] / 2;
+				c[3] = (y[i + 1] - y[i]) / hi - m[i + 1] * hi / 6 - m[i] * hi / 3;
+				c[4] = y[i];
+			}
+			s.cut[tri] = cutoff;
+		}
+	return s;
+}
+
 void FlatModel::publish()
 {
 	view.atom_type = atom_type.data();
@@ -619,12 +705,23 @@ GpuDocking::GpuDocking(const GpuSettings& s, const sgpu_scoring& sf, const sgpu_
 	sf_ = sf;
 	sf_.terms = terms_.data();
 	sf_.ci = ci_.data();
+	if (s.approximation != GpuSettings::LINEAR && s.approximation != GpuSettings::SPLINE)
+		throw std::invalid_argument("approximation must be LINEAR or SPLINE");
 	sgpu_status st = sgpu_create(s.device, &ctx_);
 	if (st != SGPU_OK)
 		throw std::runtime_error("sgpu_create failed: no HIP device");
-	Tables t = build_tables(sf_, 32);
-	sgpu_tables tt{t.n, t.factor, t.cutoff_sqr, t.smooth.data(), t.fast.data()};
-	check(ctx_, sgpu_upload_tables(ctx_, &tt));
+	if (s.approximation == GpuSettings::SPLINE)
+	{
+		Splines sp = build_splines(sf_, s.approx_factor);
+		sgpu_splines ss{sp.nseg, sp.fraction, sp.cutoff_sqr, sp.coef.data(), sp.cut.data()};
+		check(ctx_, sgpu_upload_splines(ctx_, &ss));
+	}
+	else
+	{
+		Tables t = build_tables(sf_, s.approx_factor);
+		sgpu_tables tt{t.n, t.factor, t.cutoff_sqr, t.smooth.data(), t.fast.data()};
+		check(ctx_, sgpu_upload_tables(ctx_, &tt));
+	}
 	check(ctx_, sgpu_upload_receptor(ctx_, &rec, &receptor_));
 	check(ctx_, sgpu_set_progress(ctx_, s.progress_chunks, s.progress, s.progress_user));
 	check(ctx_, sgpu_mc_capacity(ctx_, s.precision, &capacity_));
@@ -643,7 +740,9 @@ GpuDocking::GpuDocking(const GpuSettings& s, const sgpu_scoring& sf, const sgpu_
 	sf_ = sf;
 	sf_.terms = terms_.data();
 	sf_.ci = ci_.data();
-	Tables t = build_tables(sf_, 32);
+	if (s.approximation != GpuSettings::LINEAR)
+		throw std::invalid_argument("a shared context carries the linear tables");
+	Tables t = build_tables(sf_, s.approx_factor);
 	sgpu_tables tt{t.n, t.factor, t.cutoff_sqr, t.smooth.data(), t.fast.data()};
 	check(ctx_, sgpu_upload_tables(ctx_, &tt));
 	check(ctx_, sgpu_upload_receptor(ctx_, &rec, &receptor_));
@@ -1247,6 +1346,23 @@ void sgpu_vina_scoring(sgpu_term* t, sgpu_ci_term* ci, sgpu_scoring* out, double
 	sgpu_builtin_scoring("vina", hydrophobic_scale, t, ci, out);
 }
 
+int sgpu_build_splines(const sgpu_scoring* sf, double factor, double* coef, double* cut)
+{
+	try
+	{
+		Splines s = build_splines(*sf, factor);
+		if (coef)
+			std::copy(s.coef.begin(), s.coef.end(), coef);
+		if (cut)
+			std::copy(s.cut.begin(), s.cut.end(), cut);
+		return s.nseg;
+	}
+	catch (...)
+	{
+		return -1;
+	}
+}
+
 int sgpu_build_tables(const sgpu_scoring* sf, double factor, double* smooth, double* fast)
 {
 	try
@@ -1313,6 +1429,12 @@ sgpu_docking* sgpu_docking_create(const sgpu_dock_settings* s, const sgpu_mol* r
 		gs.chain_split = s->chain_split > 0 ? s->chain_split : 1;
 		gs.early_stop = s->early_stop ? GpuSettings::EARLY_STOP_ALWAYS : GpuSettings::EARLY_STOP_OFF;
 		gs.skip_visited = s->skip_visited != 0;
+		if (s->approximation != 0 && s->approximation != 1)
+			throw std::invalid_argument("approximation must be 0 (linear) or 1 (spline)");
+		gs.approximation = s->approximation == 1 ? GpuSettings::SPLINE : GpuSettings::LINEAR;
+		if (s->approx_factor < 0)
+			throw std::invalid_argument("approx_factor must be positive (0: default)");
+		gs.approx_factor = s->approx_factor > 0 ? s->approx_factor : (s->approximation == 1 ? 10 : 32);
 		if (s->fast)
 		{
 			// explicit chain_split / step_fraction take precedence

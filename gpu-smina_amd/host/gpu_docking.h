@@ -73,6 +73,15 @@ struct Tables
 	std::vector<double> smooth, fast;
 };
 Tables build_tables(const sgpu_scoring& sf, double factor);
+// precalculate_splines(sf, factor) (src/lib/precalculate.h:313-455)
+struct Splines
+{
+	int nseg = 0;
+	double fraction = 0;
+	double cutoff_sqr = 64;
+	std::vector<double> coef, cut; // [378][nseg][5], [378]
+};
+Splines build_splines(const sgpu_scoring& sf, double factor);
 
 void add_to_output_container(std::vector<Pose>& out, const Pose& t, double min_rmsd, size_t max_size);
 
@@ -85,6 +94,12 @@ struct GpuSettings
 	int num_modes = 9;
 	double energy_range = 3.0;
 	int ligands_per_launch = 0; // 0: the whole batch in one launch
+	// --approximation: LINEAR (the docking search) or SPLINE (the default of
+	// --minimize, src/main/main.cpp:1225-1228; FP64 direct evaluation and
+	// refinement only); factor: points per A^2 (linear) or per A (spline)
+	enum { LINEAR, SPLINE };
+	int approximation = LINEAR;
+	double approx_factor = 32;
 	// GPU-style search, not smina's (GpuSettings, src/gpu/gpu_docking.h:23-44).
 	// chain_split: every Monte Carlo chain becomes this many chains with
 	// 1/chain_split of its steps (0: automatic, auto_split); step_fraction:

@@ -28,6 +28,11 @@ class Tables(C.Structure):
                 ("smooth", F64P), ("fast", F64P)]
 
 
+class Splines(C.Structure):
+    _fields_ = [("nseg", C.c_int), ("fraction", C.c_double), ("cutoff_sqr", C.c_double),
+                ("coef", F64P), ("cut", F64P)]
+
+
 class Receptor(C.Structure):
     _fields_ = [("natoms", C.c_int), ("xyz", F64P), ("type", I32P)]
 
@@ -115,7 +120,8 @@ class DockSettings(C.Structure):
                 ("num_modes", C.c_int), ("energy_range", C.c_double), ("hydrophobic_scale", C.c_double),
                 ("ligands_per_launch", C.c_int), ("step_fraction", C.c_double), ("progress_chunks", C.c_int),
                 ("progress", PROGRESS_FN), ("progress_user", C.c_void_p), ("scoring", C.c_char_p),
-                ("chain_split", C.c_int), ("early_stop", C.c_int), ("fast", C.c_int), ("skip_visited", C.c_int)]
+                ("chain_split", C.c_int), ("early_stop", C.c_int), ("fast", C.c_int), ("skip_visited", C.c_int),
+                ("approximation", C.c_int), ("approx_factor", C.c_double)]
 
 
 class ServerSettings(C.Structure):
@@ -174,6 +180,8 @@ def _load():
         "sgpu_builtin_scoring": (C.c_int, [C.c_char_p, C.c_double, C.POINTER(Term), C.POINTER(CITerm),
                                            C.POINTER(Scoring)]),
         "sgpu_build_tables": (C.c_int, [C.POINTER(Scoring), C.c_double, F64P, F64P]),
+        "sgpu_build_splines": (C.c_int, [C.POINTER(Scoring), C.c_double, F64P, F64P]),
+        "sgpu_upload_splines": (C.c_int, [vp, C.POINTER(Splines)]),
         "sgpu_flatten": (vp, [C.POINTER(ModelDesc)]),
         "sgpu_flat_get": (C.POINTER(FlatModel), [vp]),
         "sgpu_flat_free": (None, [vp]),
@@ -286,6 +294,17 @@ def build_tables(sf, factor=32.0):
     return n, smooth, fast
 
 
+def build_splines(sf, factor=10.0):
+    """precalculate_splines(sf, factor): nseg, coef [378, nseg, 5], cut [378]"""
+    nseg = lib.sgpu_build_splines(C.byref(sf), factor, None, None)
+    if nseg < 0:
+        raise ValueError("bad spline factor")
+    coef = np.zeros((378, nseg, 5))
+    cut = np.zeros(378)
+    lib.sgpu_build_splines(C.byref(sf), factor, ptr(coef, F64P), ptr(cut, F64P))
+    return nseg, coef, cut
+
+
 def grid_dims(center, size, spacing=0.375):
     """smina's box: n = ceil(size / spacing) intervals, the span widened to
     n * spacing around the center (src/main/main.cpp:1370-1382)."""
@@ -375,6 +394,11 @@ class Context:
         self._tables = (f64(smooth), f64(fast))
         t = Tables(n, factor, cutoff_sqr, ptr(self._tables[0], F64P), ptr(self._tables[1], F64P))
         self.check(lib.sgpu_upload_tables(self.h, C.byref(t)))
+
+    def upload_splines(self, nseg, coef, cut, cutoff=8.0):
+        self._splines = (f64(coef).reshape(-1), f64(cut))
+        s = Splines(nseg, cutoff / nseg, cutoff * cutoff, ptr(self._splines[0], F64P), ptr(self._splines[1], F64P))
+        self.check(lib.sgpu_upload_splines(self.h, C.byref(s)))
 
     def upload_receptor(self, xyz, types):
         xyz = f64(xyz).reshape(-1)
@@ -524,18 +548,20 @@ class Docking:
     def __init__(self, receptor, gd, fp64=False, exhaustiveness=8, seed=1, num_modes=9, energy_range=3.0,
                  hydrophobic_scale=1.0, ligands_per_launch=0, device=0, step_fraction=1.0, precision=None,
                  progress=None, progress_chunks=1, scoring="vina", chain_split=0, early_stop=False, fast=False,
-                 skip_visited=False):
+                 skip_visited=False, approximation="linear", approx_factor=0.0):
         """precision: FP32, FP64 or MIXED (default: FP64 if fp64 else FP32);
         progress(fraction): called after each of progress_chunks launches
         of the Monte Carlo search; chain_split, early_stop, skip_visited,
         fast: the GPU-style search options --gpu_split, --gpu_early_stop,
-        --gpu_qvina, --fast"""
+        --gpu_qvina, --fast; approximation "linear" or "spline" (--minimize's
+        default, with factor 10 unless approx_factor is given)"""
         prec = precision if precision is not None else (FP64 if fp64 else FP32)
         # keep the callback alive as long as the docking object
         self._progress = PROGRESS_FN(lambda user, frac: progress(frac)) if progress else PROGRESS_FN()
         s = DockSettings(device, prec, exhaustiveness, seed, num_modes, energy_range, hydrophobic_scale,
                          ligands_per_launch, step_fraction, progress_chunks, self._progress, None,
-                         scoring.encode(), chain_split, int(early_stop), int(fast), int(skip_visited))
+                         scoring.encode(), chain_split, int(early_stop), int(fast), int(skip_visited),
+                         {"linear": 0, "spline": 1}[approximation], approx_factor)
         self.gd = gd
         self.h = C.c_void_p(lib.sgpu_docking_create(C.byref(s), receptor.h, C.byref(gd)))
         err = lib.sgpu_docking_error(self.h)

@@ -61,6 +61,17 @@ typedef struct sgpu_tables
 	const double* fast;   /* [378][n] */
 } sgpu_tables;
 
+/* ---- precalculate_splines (src/lib/precalculate.h:313-455): per type pair,
+ * the cubic spline of the terms over nseg segments of `fraction` A */
+typedef struct sgpu_splines
+{
+	int nseg;            /* factor x cutoff (80 at factor 10, cutoff 8) */
+	double fraction;     /* cutoff / nseg */
+	double cutoff_sqr;
+	const double* coef;  /* [378][nseg][5]: x, a, b, c, d */
+	const double* cut;   /* [378]: the spline's cutoff, 0 for a pair whose terms vanish */
+} sgpu_splines;
+
 /* ---- receptor (heavy, typed atoms; model::grid_atoms) */
 typedef struct sgpu_receptor
 {
@@ -198,6 +209,13 @@ sgpu_status sgpu_mc_capacity(sgpu_ctx* ctx, int precision, int* chains);
 
 /* DeviceTables::upload (src/gpu/gpu_device.cu:71-90) */
 sgpu_status sgpu_upload_tables(sgpu_ctx* ctx, const sgpu_tables* t);
+/* the spline approximation (--approximation spline, the default of
+ * --minimize; DeviceTables::upload's spline branch, src/gpu/gpu_flatten.cpp:
+ * 275-325): replaces the context's tables.  Carried by the FP64 direct
+ * receptor path (evaluation and refinement, i.e. --minimize); the map
+ * build, the Monte Carlo search and the rescoring need the linear tables
+ * and refuse splines. */
+sgpu_status sgpu_upload_splines(sgpu_ctx* ctx, const sgpu_splines* s);
 
 /* DeviceReceptor::upload (src/gpu/gpu_device.cu:92-110) plus the 3 A cell
  * lists of flatten_receptor (src/gpu/gpu_flatten.cpp:405-445), built on the

@@ -66,6 +66,10 @@ void sgpu_vina_scoring(sgpu_term* terms, sgpu_ci_term* ci, sgpu_scoring* out, do
 /* precalculate_linear(sf, factor) tables, FP64: smooth [378][n][2], fast
  * [378][n]; returns n (call with NULL buffers to get n) */
 int sgpu_build_tables(const sgpu_scoring* sf, double factor, double* smooth, double* fast);
+/* precalculate_splines(sf, factor) (src/lib/precalculate.h:313-455): coef
+ * [378][nseg][5] (x, a, b, c, d per segment), cut [378]; returns nseg =
+ * factor x 8 (call with NULL buffers to get it), -1 for a bad factor */
+int sgpu_build_splines(const sgpu_scoring* sf, double factor, double* coef, double* cut);
 
 /* ---- docking (GpuDocking::dock_batch, src/gpu/gpu_docking.cpp:756-831) */
 typedef struct sgpu_docking sgpu_docking;
@@ -94,6 +98,9 @@ typedef struct sgpu_dock_settings
 	                         * step_fraction < 1), QuickVina 2's history check, early stopping while the
 	                         * device is full, two-pass screening of the best 10% */
 	int skip_visited;       /* --gpu_qvina: QuickVina 2's history check (sgpu_mc_params.visited) */
+	int approximation;      /* --approximation: 0 linear (docking), 1 spline (the default of --minimize,
+	                         * src/main/main.cpp:1225-1228; sgpu_minimize only) */
+	double approx_factor;   /* --factor (0: 32 for linear, 10 for spline as --minimize sets it) */
 } sgpu_dock_settings;
 
 sgpu_docking* sgpu_docking_create(const sgpu_dock_settings* s, const sgpu_mol* receptor,

@@ -524,9 +524,126 @@ struct Tables
 				}
 			}
 	}
-	// precalculate_linear_element::eval_deriv with one component
+	// precalculate_splines (src/lib/precalculate.h:313-455) with Spline
+	// (src/lib/splines.h:37-121), one component.  Per type pair: the terms
+	// at n = factor x cutoff distances i x cutoff / n plus (cutoff, 0), and
+	// the cubic spline through them with zero first derivative at both
+	// ends.  Spline::initialize inverts the dense (n+1) x (n+1) matrix
+	// (Eigen); the system is tridiagonal and is solved here by elimination
+	// (Thomas algorithm), equal up to rounding.  A pair whose terms vanish at
+	// every point keeps an uninitialized spline (cutoff 0: value 0).
+	int kind = 0;      // 0: precalculate_linear, 1: precalculate_splines
+	int nseg = 0;      // spline segments per type pair
+	fl fraction = 0;   // segment length in A
+	std::vector<fl> spl;     // [tri][nseg][5]: x, a, b, c, d
+	std::vector<fl> spl_cut; // [tri]: the spline's cutoff, 0 for a zero spline
+	void build_splines(const Scoring& sf, fl factor_)
+	{
+		kind = 1;
+		const fl cutoff = sf.cutoff;
+		cutoff_sqr = sqr(cutoff);
+		const unsigned np = unsigned(factor_ * cutoff);
+		nseg = int(np);
+		fraction = cutoff / fl(np);
+		const int ntri = NUM_TYPES * (NUM_TYPES + 1) / 2;
+		spl.assign(size_t(ntri) * nseg * 5, 0);
+		spl_cut.assign(size_t(ntri), 0);
+		const int e = int(np); // points 0..e
+		std::vector<fl> xs(e + 1), ys(e + 1), ddy(e + 1), cp(e + 1), rp(e + 1);
+		for (int t1 = 0; t1 < NUM_TYPES; t1++)
+			for (int t2 = t1; t2 < NUM_TYPES; t2++)
+			{
+				const int tri = sgpu::tri_index(t1, t2);
+				bool nonzero = false;
+				for (int i = 0; i < e; i++)
+				{
+					xs[i] = i * fraction;
+					ys[i] = sf.eval_fast(t1, t2, xs[i]);
+					if (ys[i] != 0)
+						nonzero = true;
+				}
+				xs[e] = cutoff;
+				ys[e] = 0;
+				if (!nonzero)
+					continue;
+				const fl h = xs[1] - xs[0];
+				const fl hlast = xs[e] - xs[e - 1];
+				// the equations: row 0: 2h y0'' + h y1'' = 6 (y1 - y0) / h;
+				// row i: hi y''(i-1) + 2 (h + hi) y''i + hi y''(i+1) =
+				// 6 ((y(i+1) - yi) / hi - (yi - y(i-1)) / h), hi = hlast for
+				// the last interior row; row e: hlast y''(e-1) + 2 hlast y''e
+				// = -6 (ye - y(e-1)) / hlast
+				auto sub = [&](int i) { return i == e ? hlast : (i == e - 1 ? hlast : h); };
+				auto diag = [&](int i) {
+					if (i == 0)
+						return 2 * h;
+					if (i == e)
+						return 2 * hlast;
+					const fl hi = i == e - 1 ? hlast : h;
+					return 2 * (h + hi);
+				};
+				auto sup = [&](int i) { return i == 0 ? h : (i == e - 1 ? hlast : h); };
+				auto rhs = [&](int i) {
+					if (i == 0)
+						return 6 * ((ys[1] - ys[0]) / h);
+					if (i == e)
+						return 6 * (-(ys[e] - ys[e - 1]) / hlast);
+					const fl hi = i == e - 1 ? hlast : h;
+					return 6 * ((ys[i + 1] - ys[i]) / hi - (ys[i] - ys[i - 1]) / h);
+				};
+				cp[0] = sup(0) / diag(0);
+				rp[0] = rhs(0) / diag(0);
+				for (int i = 1; i <= e; i++)
+				{
+					const fl m = diag(i) - sub(i) * cp[i - 1];
+					cp[i] = i < e ? sup(i) / m : 0;
+					rp[i] = (rhs(i) - sub(i) * rp[i - 1]) / m;
+				}
+				ddy[e] = rp[e];
+				for (int i = e - 1; i >= 0; i--)
+					ddy[i] = rp[i] - cp[i] * ddy[i + 1];
+				for (int i = 0; i < e; i++)
+				{
+					const fl hi = i == e - 1 ? hlast : h;
+					fl* s = &spl[(size_t(tri) * nseg + i) * 5];
+					s[0] = xs[i];
+					s[1] = (ddy[i + 1] - ddy[i]) / (6 * hi);
+					s[2] = ddy[i] / 2;
+					s[3] = (ys[i + 1] - ys[i]) / hi - ddy[i + 1] * hi / 6 - ddy[i] * hi / 3;
+					s[4] = ys[i];
+				}
+				spl_cut[size_t(tri)] = cutoff;
+			}
+	}
+	// Spline::eval_deriv (src/lib/splines.h:103-121) at distance r
+	void spline_eval(int tri, fl r, fl& val, fl& der) const
+	{
+		if (r >= spl_cut[size_t(tri)])
+		{
+			val = 0;
+			der = 0;
+			return;
+		}
+		const unsigned index = unsigned(r / fraction);
+		const fl* s = &spl[(size_t(tri) * nseg + index) * 5];
+		const fl lx = r - s[0];
+		val = ((s[1] * lx + s[2]) * lx + s[3]) * lx + s[4];
+		der = (3 * s[1] * lx + 2 * s[2]) * lx + s[3];
+	}
+	// precalculate_linear_element::eval_deriv with one component, or
+	// precalculate_splines::eval_deriv (src/lib/precalculate.h:418-450, no
+	// slow terms): the spline's derivative over r
 	void eval_deriv(int t1, int t2, fl r2, fl& eo, fl& doro) const
 	{
+		if (kind == 1)
+		{
+			const fl r = std::sqrt(r2);
+			fl val, der;
+			spline_eval(sgpu::tri_index_permissive(t1, t2), r, val, der);
+			eo = val;
+			doro = der / r;
+			return;
+		}
 		const size_t base = size_t(sgpu::tri_index_permissive(t1, t2)) * n;
 		fl r2_factored = factor * r2;
 		size_t i1 = size_t(r2_factored);
@@ -547,6 +664,13 @@ struct Tables
 	}
 	fl fast_value(int t1, int t2, fl r2) const
 	{
+		if (kind == 1)
+		{
+			// precalculate_splines::eval_fast: the spline's value
+			fl val, der;
+			spline_eval(sgpu::tri_index_permissive(t1, t2), std::sqrt(r2), val, der);
+			return val;
+		}
 		const size_t base = size_t(sgpu::tri_index_permissive(t1, t2)) * n;
 		return fast[base + size_t(factor * r2)];
 	}
@@ -2074,6 +2198,23 @@ orc_ctx* orc_create(const sgpu_scoring* sf, const sgpu_receptor* rec, const sgpu
 void orc_destroy(orc_ctx* c) { delete c; }
 const char* orc_error(orc_ctx* c) { return c->c.err.c_str(); }
 int orc_table_n(orc_ctx* c) { return c->c.tab.n; }
+int orc_use_splines(orc_ctx* c, double factor, double* coef, double* cut)
+{
+	if (!(factor > 0))
+	{
+		c->c.err = "spline factor must be positive";
+		return -1;
+	}
+	Tables& t = c->c.tab;
+	t.build_splines(c->c.sf, factor);
+	if (coef)
+		std::copy(t.spl.begin(), t.spl.end(), coef);
+	if (cut)
+		std::copy(t.spl_cut.begin(), t.spl_cut.end(), cut);
+	// the maps built so far used the linear tables
+	std::fill(c->c.have_grid.begin(), c->c.have_grid.end(), false);
+	return t.nseg;
+}
 void orc_tables(orc_ctx* c, double* smooth, double* fast)
 {
 	const Tables& t = c->c.tab;

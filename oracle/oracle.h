@@ -41,6 +41,11 @@ const char* orc_error(orc_ctx* c);
 
 /* table access: smooth[(tri*n + i)*2 + {0,1}], fast[tri*n + i] */
 int orc_table_n(orc_ctx* c);
+/* switch the context to precalculate_splines(sf, factor)
+ * (src/lib/precalculate.h:313-455): every later evaluation, refinement and
+ * map uses the splines.  coef [378][nseg][5] (x, a, b, c, d) and cut [378]
+ * receive the tables when not NULL; returns nseg (factor x cutoff), or -1 */
+int orc_use_splines(orc_ctx* c, double factor, double* coef, double* cut);
 void orc_tables(orc_ctx* c, double* smooth, double* fast);
 /* one charge-independent weighted sum of terms (weighted_terms::eval_fast) */
 double orc_eval_terms(orc_ctx* c, int t1, int t2, double r);

@@ -49,6 +49,7 @@ def _load():
         "orc_bfgs": (C.c_double, [vp, C.c_int, C.c_int, C.c_double, F64P, C.c_int, C.c_int, C.c_int, F64P, I64P]),
         "orc_bfgs_quadratic": (C.c_double, [C.c_int, F64P, F64P, F64P, C.c_int, C.c_int, I64P]),
         "orc_randomize": (None, [vp, C.c_int, C.c_uint32, F64P, U32P]),
+        "orc_use_splines": (C.c_int, [vp, C.c_double, F64P, F64P]),
         "orc_visited_check": (C.c_int, [C.c_int, C.c_int, F64P, F64P, F64P, F64P, C.c_double, F64P]),
         "orc_mc_chain": (C.c_int, [vp, C.c_int, vp, C.c_int, C.c_int, F64P, U32P, C.c_int, F64P, F64P, F64P,
                                    I64P]),
@@ -114,6 +115,18 @@ class Oracle:
         if getattr(self, "h", None):
             lib.orc_destroy(self.h)
             self.h = None
+
+    def use_splines(self, factor):
+        """switch to precalculate_splines(sf, factor); returns the tables:
+        coef [378, nseg, 5] (x, a, b, c, d) and cut [378]"""
+        nseg = int(factor * 8.0)
+        coef = np.zeros((378, nseg, 5))
+        cut = np.zeros(378)
+        k = lib.orc_use_splines(self.h, factor, _p(coef, F64P), _p(cut, F64P))
+        if k < 0:
+            raise RuntimeError(self.error())
+        assert k == nseg
+        return coef, cut
 
     def error(self):
         return lib.orc_error(self.h).decode()

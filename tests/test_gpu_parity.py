@@ -358,14 +358,17 @@ def test_invalid_requests_are_rejected(env):
                        grid_id=gid)
 
 
-def test_minimize_fp64_matches_oracle():
+@pytest.mark.parametrize("approximation", ["linear", "spline"])
+def test_minimize_fp64_matches_oracle(approximation):
     """--minimize (GpuDocking::minimize_batch): each parsed pose refined in
     FP64 with the accurate line search inside its own movable_atoms_box,
     several boxes in one launch; bit-exact against the oracle's
-    refine_structure over that box"""
+    refine_structure over that box.  With the spline approximation, the
+    default of --minimize (src/main/main.cpp:1225-1228, factor 10), both
+    sides evaluate precalculate_splines"""
     rec = receptor()
     ligs = [sg.Mol.ligand(40 + k, 16 + 6 * k, 2 + k, center=(0.5 * k, -0.5 * k, 0.25 * k)) for k in range(4)]
-    d = sg.Docking(rec, gd_c0(), fp64=False)
+    d = sg.Docking(rec, gd_c0(), fp64=False, approximation=approximation)
     res = d.minimize(ligs, maxiters=0, accurate=True)
     sf = sg.vina_scoring()
     import pyoracle as po
@@ -375,10 +378,47 @@ def test_minimize_fp64_matches_oracle():
             gd.begin[i], gd.end[i] = r["box"][0][i], r["box"][1][i]
             gd.n[i] = int(round((gd.end[i] - gd.begin[i]) / 0.375))
         o = po.Oracle(sf, rec.xyz, rec.type, gd)
+        if approximation == "spline":
+            o.use_splines(10.0)
         m = o.add_model(lig.desc_ptr)
         e, _, _ = o.refine(m, o.initial_conf(m), 10000, accurate=True)
-        print(f"minimize: e {r['e']:.6f} rmsd {r['rmsd']:.3f}", flush=True)
+        print(f"minimize ({approximation}): e {r['e']:.6f} rmsd {r['rmsd']:.3f}", flush=True)
         assert r["e"] == e, (r["e"], e)
+
+
+def test_spline_eval_fp64_bit_exact():
+    """FP64 evaluation on the direct receptor term with the spline tables
+    (precalculate_splines::eval_deriv): energies and gradients equal the
+    oracle's; the paths that need the linear tables (maps, Monte Carlo,
+    rescoring, FP32) refuse the splines with an error"""
+    ligs = [sg.Mol.ligand(s, h, t) for s, h, t in LIGANDS[:3]]
+    o, ids, sf, gd = oracle_for(ligs)
+    o.use_splines(10.0)
+    rec = receptor()
+    ctx = sg.Context(0)
+    try:
+        nseg, coef, cut = sg.build_splines(sf, 10.0)
+        ctx.upload_splines(nseg, coef, cut)
+        rid = ctx.upload_receptor(rec.xyz, rec.type)
+        flats = [sg.Flat(l) for l in ligs]
+        bid = ctx.upload_models(flats, sg.FP64)
+        v = (1000.0, 1000.0, 1000.0)
+        kw = dict(grid_id=-1, receptor_id=rid, gd=gd, slope=10.0)
+        for li, f in enumerate(flats):
+            confs = random_confs(f, gd, 8, seed0=4100 + li)
+            confs[4:, :3] = np.random.default_rng(li).uniform(-3, 3, (4, 3))
+            e, g, x = ctx.eval_confs(bid, [li] * len(confs), confs, f.change_size, f.natoms, v, **kw)
+            for k in range(len(confs)):
+                eo, go, xo = o.eval_deriv(ids[li], confs[k], v, backend=1, slope=10.0)
+                assert e[k] == eo and np.array_equal(g[k], go), (li, k)
+        with pytest.raises(sg.SgpuError, match="linear tables"):
+            ctx.build_grids(rid, gd, heavy_types(*ligs), sg.FP64)
+        bid32 = ctx.upload_models(flats, sg.FP32)
+        f = flats[0]
+        with pytest.raises(sg.SgpuError, match="linear tables"):
+            ctx.eval_confs(bid32, [0], random_confs(f, gd, 1), f.change_size, f.natoms, v, **kw)
+    finally:
+        ctx.close()
 
 
 @pytest.mark.parametrize("lig,exh,frac,split,qv", [((3, 10, 0), 8, 1.0, 1, False), ((1, 24, 6), 4, 0.1, 1, False),

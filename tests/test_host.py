@@ -38,6 +38,18 @@ def test_tables_match_oracle(setup):
     assert np.array_equal(s2, a2) and np.array_equal(f2, b2) and not np.array_equal(s2, smooth)
 
 
+@pytest.mark.parametrize("factor", [10.0, 32.0])
+def test_splines_match_oracle(factor):
+    """the product's precalculate_splines tables (the --minimize default)
+    equal the oracle's bit for bit, for vina and vinardo"""
+    lig = sg.Mol.ligand(1, 24, 6)
+    for name in ("vina", "vinardo"):
+        o, _, sf, _ = oracle_for([lig], scoring=name)
+        nseg, coef, cut = sg.build_splines(sf, factor)
+        ocoef, ocut = o.use_splines(factor)
+        assert nseg == int(factor * 8) and np.array_equal(coef, ocoef) and np.array_equal(cut, ocut), name
+
+
 def test_synthetic_ligands_have_requested_shape(setup):
     ligs = setup[0]
     for (seed, heavy, tors), lig in zip(LIGANDS, ligs):

@@ -256,3 +256,75 @@ def test_mc_chain_is_deterministic_and_sorted(c0):
     assert len(e3) == 1 and ev3 > 2
     e0, _, _, ev0 = o.mc_chain(m, p, 0, (25 + f.natoms) // 3, c, s)
     assert len(e0) == 0 and ev0 == 0
+
+
+def spline_at(coef, cut, tri, r, fraction):
+    """Spline::eval_deriv (src/lib/splines.h:103-121) in numpy"""
+    if r >= cut[tri]:
+        return 0.0, 0.0
+    x, a, b, c, d = coef[tri, int(r / fraction)]
+    lx = r - x
+    return ((a * lx + b) * lx + c) * lx + d, (3 * a * lx + 2 * b) * lx + c
+
+
+def test_splines_interpolate_the_terms():
+    """precalculate_splines (src/lib/precalculate.h:313-455, factor 10): the
+    cubic spline through the terms at r = i/10 plus (8, 0) reproduces them
+    at the knots, is continuous with its first and second derivatives
+    there, has zero slope at both ends, and between the knots of a smooth
+    stretch (C_H pair beyond the hydrophobic ramp: gaussians only) stays
+    within 1e-6 of the terms and of their derivative.  (The kink at the end
+    of the hydrophobic ramp, r = 5.3, and the pinned zero at the cutoff bend
+    the spline away from the terms near them, by design.)"""
+    lig = ligand()
+    o, ids, sf, gd = oracle_for([lig])
+    coef, cut = o.use_splines(10.0)
+    h = 0.1
+    for t1, t2 in ((C_H, C_H), (N_D, O_A), (C_P, O_DA)):
+        tri = min(t1, t2) + max(t1, t2) * (max(t1, t2) + 1) // 2
+        assert cut[tri] == 8.0
+        s = coef[tri]
+        assert np.array_equal(s[:, 0], np.arange(80) * h)
+        for i in range(80):
+            assert s[i, 4] == o.eval_terms(t1, t2, i * h)  # the knots themselves
+        # the end of each segment meets the next one: value, slope, curvature
+        a, b, c, d = s[:-1, 1], s[:-1, 2], s[:-1, 3], s[:-1, 4]
+        assert np.allclose(((a * h + b) * h + c) * h + d, s[1:, 4], rtol=0, atol=1e-9 * max(1, np.abs(d).max()))
+        assert np.allclose((3 * a * h + 2 * b) * h + c, s[1:, 3], rtol=0, atol=1e-8 * max(1, np.abs(c).max()))
+        assert np.allclose(6 * a * h + 2 * b, 2 * s[1:, 2], rtol=0, atol=1e-7 * max(1, np.abs(b).max()))
+        # clamped ends: zero slope at 0 and at the cutoff, value 0 there
+        assert abs(s[0, 3]) < 1e-9 * max(1, np.abs(s[:, 3]).max())
+        x, a, b, c, d = s[-1]
+        assert abs(((a * h + b) * h + c) * h + d) < 1e-9 and abs((3 * a * h + 2 * b) * h + c) < 1e-8
+    tri = C_H + C_H * (C_H + 1) // 2
+    for r in np.arange(6.05, 7.0, 0.1):
+        v, dv = spline_at(coef, cut, tri, r, h)
+        exact = o.eval_terms(C_H, C_H, r)
+        slope = (o.eval_terms(C_H, C_H, r + 1e-6) - o.eval_terms(C_H, C_H, r - 1e-6)) / 2e-6
+        assert abs(v - exact) < 1e-6 and abs(dv - slope) < 1e-6, r
+
+
+def test_spline_eval_deriv_matches_the_tables():
+    """the oracle's direct-receptor evaluation with splines: the energy
+    moves off the linear tables' (the splines reach zero at the cutoff,
+    where the terms do not), and the gradient is the finite difference of
+    the energy (the splines are C2)"""
+    lig = ligand()
+    o, ids, sf, gd = oracle_for([lig])
+    # the parsed pose, moved a little: every atom inside the box, where the
+    # direct term is smooth (outside it, smina adds the clamped point's
+    # derivative to the penalty's)
+    conf = o.initial_conf(ids[0])
+    conf[:3] += (0.3, -0.2, 0.1)
+    e_lin, g_lin, _ = o.eval_deriv(ids[0], conf, (1000.0, 1000.0, 1000.0), backend=1)
+    o.use_splines(10.0)
+    e_spl, g_spl, _ = o.eval_deriv(ids[0], conf, (1000.0, 1000.0, 1000.0), backend=1)
+    assert e_spl != e_lin and abs(e_spl - e_lin) < 1.0
+    # finite differences along the position coordinates
+    for k in range(3):
+        cp, cm = conf.copy(), conf.copy()
+        cp[k] += 1e-6
+        cm[k] -= 1e-6
+        fd = (o.eval_deriv(ids[0], cp, (1000.0, 1000.0, 1000.0), backend=1)[0] -
+              o.eval_deriv(ids[0], cm, (1000.0, 1000.0, 1000.0), backend=1)[0]) / 2e-6
+        assert abs(fd - g_spl[k]) < 1e-4 * max(1.0, abs(fd)), k
