@@ -1510,6 +1510,16 @@ int sgpu_dock(sgpu_docking* d, const sgpu_mol* const* ligands, int n, sgpu_dock_
 				out[i].e[k] = jobs[i].out[k].e;
 			out[i].evals = jobs[i].evals;
 			out[i].seconds = jobs[i].seconds;
+			out[i].conf_size = 0;
+			if (out[i].confs && !jobs[i].out.empty())
+			{
+				const size_t cs = jobs[i].out[0].c.size();
+				out[i].conf_size = int(cs);
+				for (int k = 0; k < out[i].nposes; k++)
+					if (size_t(k + 1) * cs <= size_t(std::max(0, out[i].confs_capacity)))
+						std::copy(jobs[i].out[size_t(k)].c.begin(), jobs[i].out[size_t(k)].c.end(),
+								out[i].confs + size_t(k) * cs);
+			}
 			out[i].ncoords = 0;
 			if (out[i].coords && !jobs[i].out.empty())
 			{

@@ -2,6 +2,7 @@
 #ifndef SGPU_HOST_MOL_H
 #define SGPU_HOST_MOL_H
 
+#include <string>
 #include <vector>
 #include "../../include/sgpu_host.h"
 
@@ -15,6 +16,10 @@ struct sgpu_mol
 	int nmovable = 0;
 	int nligands = 0;
 	sgpu_model_desc desc{};
+	// a parsed PDBQT file: its lines, and for an atom record the index of
+	// its atom in the model (-1 for other lines), for writing poses back
+	std::vector<std::string> text;
+	std::vector<int> text_atom;
 
 	int natoms() const { return int(type.size()); }
 	// point desc at the vectors (after any change of them)

@@ -1,0 +1,843 @@
+// PDBQT input and output for the docking path, without OpenBabel.
+//
+// Reading follows smina's own PDBQT parser and model set-up:
+//   - the record grammar of parse_pdbqt_rigid / parse_pdbqt_ligand_stream
+//     (src/lib/parse_pdbqt.cpp:146-176, :179-268, :440-477): atom records,
+//     ROOT / ENDROOT, nested BRANCH a b / ENDBRANCH a b, one TORSDOF;
+//   - the atom order of postprocess_branch (src/lib/parse_pdbqt.cpp:
+//     296-336): a frame's atoms in file order, each followed by the first
+//     ("immobile") atoms of its child branches, which turn about the bond but
+//     do not move; then the child frames depth first;
+//   - branches whose moving atoms are all hydrogens merge into their parent
+//     (parsing_struct::mobile_hydrogens_only, src/lib/parsing.h:200-215; the
+//     default without --flex_hydrogens);
+//   - AutoDock type names to smina types (string_to_smina_type,
+//     src/lib/atom_constants.h:294-318);
+//   - covalent bonds by distance between atoms whose distance cannot change
+//     (model::assign_bonds, src/lib/model.cpp:448-527), then the X-Score
+//     types from the bonds (model::assign_types, adjust_smina_type,
+//     src/lib/model.cpp:552-566, src/lib/atom_constants.h:347-374).
+// Writing follows context::writePDBQT and result_info::write
+// (src/lib/model.cpp:710-750, src/lib/result_info.cpp:146-153): the file's
+// lines with each atom's coordinates at the pose, one MODEL per pose.
+
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <functional>
+#include <map>
+#include <cstring>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../csrc/atom_types.h"
+#include "gpu_docking.h"
+#include "mol.h"
+
+namespace sgpu_host
+{
+using namespace sgpu;
+namespace
+{
+
+// per smina type (atom_constants.h default_data): AutoDock name, smina
+// name, covalent radius, AutoDock heteroatom flag
+struct TypeRow
+{
+	const char* ad;
+	const char* name;
+	double covalent;
+	bool hetero;
+};
+const TypeRow kTypes[NUM_ATOM_TYPES] = {
+		{"H", "Hydrogen", 0.37, false},
+		{"HD", "PolarHydrogen", 0.37, false},
+		{"C", "AliphaticCarbonXSHydrophobe", 0.77, false},
+		{"C", "AliphaticCarbonXSNonHydrophobe", 0.77, false},
+		{"A", "AromaticCarbonXSHydrophobe", 0.77, false},
+		{"A", "AromaticCarbonXSNonHydrophobe", 0.77, false},
+		{"N", "Nitrogen", 0.75, true},
+		{"N", "NitrogenXSDonor", 0.75, true},
+		{"NA", "NitrogenXSDonorAcceptor", 0.75, true},
+		{"NA", "NitrogenXSAcceptor", 0.75, true},
+		{"O", "Oxygen", 0.73, true},
+		{"O", "OxygenXSDonor", 0.73, true},
+		{"OA", "OxygenXSDonorAcceptor", 0.73, true},
+		{"OA", "OxygenXSAcceptor", 0.73, true},
+		{"S", "Sulfur", 1.02, true},
+		{"SA", "SulfurAcceptor", 1.02, true},
+		{"P", "Phosphorus", 1.06, true},
+		{"F", "Fluorine", 0.71, true},
+		{"Cl", "Chlorine", 0.99, true},
+		{"Br", "Bromine", 1.14, true},
+		{"I", "Iodine", 1.33, true},
+		{"Mg", "Magnesium", 1.30, true},
+		{"Mn", "Manganese", 1.39, true},
+		{"Zn", "Zinc", 1.31, true},
+		{"Ca", "Calcium", 1.74, true},
+		{"Fe", "Iron", 1.25, true},
+		{"M", "GenericMetal", 1.75, true},
+};
+
+double max_covalent_radius()
+{
+	double m = 0;
+	for (const TypeRow& r : kTypes)
+		m = std::max(m, r.covalent);
+	return m;
+}
+
+struct ParseError : std::runtime_error
+{
+	ParseError(int line, const std::string& why) : std::runtime_error("line " + std::to_string(line) + ": " + why) {}
+};
+
+bool starts(const std::string& s, const char* p) { return s.compare(0, std::strlen(p), p) == 0; }
+
+std::string trim(const std::string& s)
+{
+	size_t a = 0, b = s.size();
+	while (a < b && std::isspace(static_cast<unsigned char>(s[a])))
+		++a;
+	while (b > a && std::isspace(static_cast<unsigned char>(s[b - 1])))
+		--b;
+	return s.substr(a, b - a);
+}
+
+// string_to_smina_type: a name of up to 2 characters is an AutoDock type
+// (the first table row with that name; Se reads as S; other metals as the
+// generic metal), a longer one a smina type name; -1 if unknown
+int type_of_name(const std::string& name)
+{
+	if (name.size() <= 2)
+	{
+		for (int t = 0; t < NUM_ATOM_TYPES; t++)
+			if (name == kTypes[t].ad)
+				return t;
+		if (name == "Se")
+			return type_of_name("S");
+		static const char* metals[] = {"Cu", "Fe", "Na", "K", "Hg", "Co", "U", "Cd", "Ni"};
+		for (const char* m : metals)
+			if (name == m)
+				return T_Met;
+		return -1;
+	}
+	for (int t = 0; t < NUM_ATOM_TYPES; t++)
+		if (name == kTypes[t].name)
+			return t;
+	return -1;
+}
+
+struct Atom
+{
+	int number = 0;
+	int type = 0;
+	double xyz[3] = {0, 0, 0};
+	int line = 0; // index into the file's lines
+};
+
+// columns (1-based, inclusive) of an atom record, blank-skipping like
+// checked_convert_substring
+double column_number(const std::string& s, size_t i, size_t j, int line, const char* what)
+{
+	if (j > s.size())
+		throw ParseError(line, "ATOM syntax incorrect: the line is too short");
+	while (i <= j && std::isspace(static_cast<unsigned char>(s[i - 1])))
+		++i;
+	const std::string sub = s.substr(i - 1, j - i + 1);
+	char* end = nullptr;
+	const double v = std::strtod(sub.c_str(), &end);
+	if (sub.empty() || end == sub.c_str() || !trim(std::string(end)).empty())
+		throw ParseError(line, std::string("ATOM syntax incorrect: \"") + sub + "\" is not a valid " + what);
+	return v;
+}
+
+// parse_pdbqt_atom_string (src/lib/parse_pdbqt.cpp:103-122); idx: the
+// record's 0-based line index
+Atom parse_atom(const std::string& s, int idx)
+{
+	const int line = idx + 1;
+	Atom a;
+	const double number = column_number(s, 7, 11, line, "atom number");
+	if (number < 0 || number != std::floor(number))
+		throw ParseError(line, "ATOM syntax incorrect: bad atom number");
+	a.number = int(number);
+	for (int k = 0; k < 3; k++)
+		a.xyz[k] = column_number(s, 31 + 8 * k, 38 + 8 * k, line, "coordinate");
+	if (s.size() < 78)
+		throw ParseError(line, "ATOM syntax incorrect: no atom type");
+	const std::string name = trim(s.substr(77));
+	a.type = type_of_name(name);
+	if (a.type < 0)
+		throw ParseError(line, "\"" + name + "\" is not a valid AutoDock type (types are case-sensitive)");
+	a.line = idx;
+	return a;
+}
+
+bool ignorable(const std::string& s)
+{
+	return s.empty() || starts(s, "WARNING") || starts(s, "REMARK") || starts(s, "USER") || starts(s, "TER");
+}
+
+// parsing_struct: one frame level of the file
+struct Level;
+struct Node
+{
+	Atom a;
+	std::vector<Level> children;
+};
+struct Level
+{
+	std::vector<Node> atoms;
+	int immobile = -1; // the branch's first atom (BRANCH a b: atom b)
+	bool essentially_empty() const
+	{
+		for (size_t i = 0; i < atoms.size(); i++)
+			if (int(i) != immobile || !atoms[i].children.empty())
+				return false;
+		return true;
+	}
+	bool mobile_hydrogens_only() const
+	{
+		for (size_t i = 0; i < atoms.size(); i++)
+		{
+			if (!atoms[i].children.empty())
+				return false;
+			if (int(i) != immobile && !(atoms[i].a.type == T_Hydrogen || atoms[i].a.type == T_PolarHydrogen))
+				return false;
+		}
+		return true;
+	}
+};
+
+struct Reader
+{
+	std::vector<std::string> lines;
+	size_t pos = 0;
+	bool next(std::string& s)
+	{
+		if (pos >= lines.size())
+			return false;
+		s = lines[pos++];
+		return true;
+	}
+	int line() const { return int(pos); } // 1-based number of the line just read
+};
+
+std::vector<std::string> split_lines(const char* text)
+{
+	std::vector<std::string> out;
+	std::string cur;
+	for (const char* p = text; *p; ++p)
+	{
+		if (*p == '\n')
+		{
+			if (!cur.empty() && cur.back() == '\r')
+				cur.pop_back();
+			out.push_back(cur);
+			cur.clear();
+		}
+		else
+			cur.push_back(*p);
+	}
+	if (!cur.empty())
+		out.push_back(cur);
+	return out;
+}
+
+void two_numbers(const std::string& s, const char* tag, int line, int& a, int& b)
+{
+	std::istringstream in(s.substr(std::strlen(tag)));
+	long x = -1, y = -1;
+	in >> x >> y;
+	if (!in || x < 0 || y < 0)
+		throw ParseError(line, "syntax error");
+	a = int(x);
+	b = int(y);
+}
+
+void parse_branch(Reader& r, Level& lv, int from, int to);
+
+// BRANCH a b inside level lv (parse_pdbqt_branch_aux): the branch hangs
+// from lv's atom numbered a, or merges into lv if only hydrogens move
+void branch_into(Reader& r, const std::string& s, Level& lv)
+{
+	int from, to;
+	two_numbers(s, "BRANCH", r.line(), from, to);
+	for (Node& nd : lv.atoms)
+		if (nd.a.number == from)
+		{
+			Level b;
+			parse_branch(r, b, from, to);
+			if (b.mobile_hydrogens_only())
+				for (const Node& m : b.atoms)
+					lv.atoms.push_back(Node{m.a, {}});
+			else
+				nd.children.push_back(std::move(b));
+			return;
+		}
+	throw ParseError(r.line(), "no atom number " + std::to_string(from) + " in this branch");
+}
+
+void parse_branch(Reader& r, Level& lv, int from, int to)
+{
+	std::string s;
+	while (r.next(s))
+	{
+		if (ignorable(s) && !starts(s, "TER"))
+			continue;
+		if (starts(s, "BRANCH"))
+			branch_into(r, s, lv);
+		else if (starts(s, "ENDBRANCH"))
+		{
+			int a, b;
+			two_numbers(s, "ENDBRANCH", r.line(), a, b);
+			if (a != from || b != to)
+				throw ParseError(r.line(), "inconsistent branch numbers");
+			if (lv.immobile < 0)
+				throw ParseError(r.line(), "atom " + std::to_string(to) + " has not been found in this branch");
+			return;
+		}
+		else if (starts(s, "ATOM  ") || starts(s, "HETATM"))
+		{
+			Atom a = parse_atom(s, int(r.pos) - 1);
+			if (a.number == to)
+				lv.immobile = int(lv.atoms.size());
+			lv.atoms.push_back(Node{a, {}});
+		}
+		else if (starts(s, "MODEL"))
+			throw ParseError(r.line(), "unexpected multi-MODEL input");
+		else
+			throw ParseError(r.line(), "unknown or inappropriate tag");
+	}
+	throw ParseError(r.line(), "missing ENDBRANCH");
+}
+
+// mobility of a pair (distance_type): can the distance change?
+enum Mobility : unsigned char
+{
+	VARIABLE = 0,
+	FIXED = 1,
+	ROTOR = 2
+};
+
+// the model under construction: atoms in smina's order, frames, mobility
+struct Builder
+{
+	std::vector<Atom> atoms;
+	std::vector<int> frame_parent, frame_begin, frame_end, frame_origin, frame_axis;
+	std::vector<std::vector<unsigned char>> mob;
+
+	int push(const Atom& a)
+	{
+		atoms.push_back(a);
+		return int(atoms.size()) - 1;
+	}
+	void grow()
+	{
+		const size_t n = atoms.size();
+		for (auto& row : mob)
+			row.resize(n, VARIABLE);
+		mob.resize(n, std::vector<unsigned char>(n, VARIABLE));
+	}
+	void set(int i, int j, Mobility m)
+	{
+		mob[size_t(i)][size_t(j)] = m;
+		mob[size_t(j)][size_t(i)] = m;
+	}
+
+	// postprocess_branch: this level's atoms (skipping its immobile atom,
+	// already placed in the parent frame), each followed by the immobile
+	// atoms of its children; axis_begin / axis_end are this frame's bond
+	// atoms in the parent frame (-1 for a root)
+	void frame(Level& lv, int parent, int axis_begin, int axis_end, int origin)
+	{
+		const int f = int(frame_parent.size());
+		frame_parent.push_back(parent);
+		frame_begin.push_back(int(atoms.size()));
+		frame_end.push_back(0);
+		frame_origin.push_back(origin);
+		frame_axis.push_back(axis_begin);
+		std::vector<std::vector<int>> child_axis(lv.atoms.size()); // per node: its children's immobile atoms
+		std::vector<int> node_index(lv.atoms.size(), -1);
+		for (size_t i = 0; i < lv.atoms.size(); i++)
+		{
+			Node& nd = lv.atoms[i];
+			if (int(i) != lv.immobile)
+				node_index[i] = push(nd.a);
+			for (Level& c : nd.children)
+				child_axis[i].push_back(c.atoms.empty() ? -1 : push(c.atoms[size_t(c.immobile)].a));
+		}
+		frame_end[size_t(f)] = int(atoms.size());
+		grow();
+		const int b = frame_begin[size_t(f)], e = frame_end[size_t(f)];
+		for (int i = b; i < e; i++)
+		{
+			if (axis_begin >= 0)
+				set(axis_begin, i, FIXED);
+			if (axis_end >= 0)
+				set(axis_end, i, FIXED);
+		}
+		if (axis_begin >= 0 && axis_end >= 0)
+			set(axis_begin, axis_end, ROTOR);
+		for (int i = b; i < e; i++)
+			for (int j = i + 1; j < e; j++)
+				set(i, j, FIXED);
+		for (size_t i = 0; i < lv.atoms.size(); i++)
+		{
+			Node& nd = lv.atoms[i];
+			// the parent atom of a child branch: this node's own index, or for
+			// the immobile atom (placed by the parent) the axis end
+			const int here = int(i) == lv.immobile ? axis_end : node_index[i];
+			for (size_t k = 0; k < nd.children.size(); k++)
+			{
+				Level& c = nd.children[k];
+				if (!c.essentially_empty())
+					frame(c, f, here, child_axis[i][k], child_axis[i][k]);
+			}
+		}
+	}
+};
+
+// model::assign_bonds over atoms whose mutual distance cannot change:
+// bonded if closer than 1.1 x the sum of covalent radii and no third atom,
+// immobile relative to both and within reach of the first, is closer to
+// both than they are to each other
+std::vector<int> assign_bonds(const std::vector<Atom>& atoms,
+		const std::function<bool(int, int)>& fixed_pair, const std::vector<std::vector<int>>* neighbors)
+{
+	const double allowance = 1.1, maxr = max_covalent_radius();
+	const int n = int(atoms.size());
+	auto d2 = [&](int i, int j) {
+		double s = 0;
+		for (int k = 0; k < 3; k++)
+			s += (atoms[size_t(i)].xyz[k] - atoms[size_t(j)].xyz[k]) * (atoms[size_t(i)].xyz[k] - atoms[size_t(j)].xyz[k]);
+		return s;
+	};
+	std::vector<int> bonds;
+	std::vector<int> relevant;
+	for (int i = 0; i < n; i++)
+	{
+		const double ri = kTypes[atoms[size_t(i)].type].covalent;
+		const double reach = allowance * (ri + maxr);
+		relevant.clear();
+		auto consider = [&](int j) {
+			if (j != i && fixed_pair(i, j) && d2(i, j) < reach * reach)
+				relevant.push_back(j);
+		};
+		if (neighbors)
+			for (int j : (*neighbors)[size_t(i)])
+				consider(j);
+		else
+			for (int j = 0; j < n; j++)
+				consider(j);
+		for (int j : relevant)
+		{
+			if (j <= i)
+				continue;
+			const double len = allowance * (ri + kTypes[atoms[size_t(j)].type].covalent);
+			const double r2 = d2(i, j);
+			if (!(r2 < len * len))
+				continue;
+			bool between = false;
+			for (int c : relevant)
+				if (c != j && fixed_pair(j, c) && d2(i, c) < r2 && d2(j, c) < r2)
+				{
+					between = true;
+					break;
+				}
+			if (!between)
+			{
+				bonds.push_back(i);
+				bonds.push_back(j);
+			}
+		}
+	}
+	return bonds;
+}
+
+// adjust_smina_type from the bonds: carbon is polar next to a heteroatom,
+// nitrogen and oxygen donate with a polar hydrogen
+void assign_types(std::vector<Atom>& atoms, const std::vector<int>& bonds)
+{
+	const size_t n = atoms.size();
+	std::vector<char> to_hd(n, 0), to_hetero(n, 0);
+	for (size_t k = 0; k + 1 < bonds.size(); k += 2)
+	{
+		const int a = bonds[k], b = bonds[k + 1];
+		for (int pass = 0; pass < 2; pass++)
+		{
+			const int x = pass ? b : a, y = pass ? a : b;
+			if (atoms[size_t(y)].type == T_PolarHydrogen)
+				to_hd[size_t(x)] = 1;
+			if (kTypes[atoms[size_t(y)].type].hetero)
+				to_hetero[size_t(x)] = 1;
+		}
+	}
+	for (size_t i = 0; i < n; i++)
+	{
+		int& t = atoms[i].type;
+		const bool hd = to_hd[i], het = to_hetero[i];
+		switch (t)
+		{
+		case T_CAliph_H:
+		case T_CAliph_P: t = het ? T_CAliph_P : T_CAliph_H; break;
+		case T_CArom_H:
+		case T_CArom_P: t = het ? T_CArom_P : T_CArom_H; break;
+		case T_N_D:
+		case T_N_P: t = hd ? T_N_D : T_N_P; break;
+		case T_N_DA:
+		case T_N_A: t = hd ? T_N_DA : T_N_A; break;
+		case T_O_D:
+		case T_O_P: t = hd ? T_O_D : T_O_P; break;
+		case T_O_DA:
+		case T_O_A: t = hd ? T_O_DA : T_O_A; break;
+		default: break;
+		}
+	}
+}
+
+sgpu_mol* finish(std::vector<Atom>& atoms, const std::vector<int>& bonds, std::vector<std::string>& lines)
+{
+	assign_types(atoms, bonds);
+	sgpu_mol* m = new sgpu_mol;
+	for (const Atom& a : atoms)
+	{
+		m->type.push_back(a.type);
+		m->xyz.insert(m->xyz.end(), a.xyz, a.xyz + 3);
+	}
+	m->bonds = bonds;
+	m->text = std::move(lines);
+	m->text_atom.assign(m->text.size(), -1);
+	for (size_t i = 0; i < atoms.size(); i++)
+		m->text_atom[size_t(atoms[i].line)] = int(i);
+	return m;
+}
+
+sgpu_mol* read_ligand(const char* text)
+{
+	Reader r;
+	r.lines = split_lines(text);
+	std::string s;
+	Level root;
+	bool in_root = false;
+	// parse_pdbqt_root and its _aux: up to ROOT, then its atoms
+	while (r.next(s))
+	{
+		if (ignorable(s))
+			continue;
+		if (starts(s, "ROOT"))
+		{
+			in_root = true;
+			break;
+		}
+		if (starts(s, "MODEL"))
+			throw ParseError(r.line(), "unexpected multi-MODEL input");
+		throw ParseError(r.line(), "unknown or inappropriate tag");
+	}
+	if (!in_root)
+		throw ParseError(r.line(), "no ROOT");
+	bool end_root = false;
+	while (r.next(s))
+	{
+		if (ignorable(s))
+			continue;
+		if (starts(s, "ATOM  ") || starts(s, "HETATM"))
+			root.atoms.push_back(Node{parse_atom(s, int(r.pos) - 1), {}});
+		else if (starts(s, "ENDROOT"))
+		{
+			end_root = true;
+			break;
+		}
+		else if (starts(s, "MODEL"))
+			throw ParseError(r.line(), "unexpected multi-MODEL input");
+		else
+			throw ParseError(r.line(), "unknown or inappropriate tag");
+	}
+	if (!end_root)
+		throw ParseError(r.line(), "missing ENDROOT");
+	// parse_pdbqt_aux: branches of the root and one TORSDOF
+	bool torsdof = false;
+	while (r.next(s))
+	{
+		if (ignorable(s))
+			continue;
+		if (starts(s, "BRANCH"))
+			branch_into(r, s, root);
+		else if (starts(s, "TORSDOF"))
+		{
+			if (torsdof)
+				throw ParseError(r.line(), "TORSDOF can occur only once");
+			std::istringstream in(s.substr(7));
+			long v = -1;
+			in >> v;
+			if (!in || v < 0)
+				throw ParseError(r.line(), "syntax error");
+			torsdof = true;
+		}
+		else if (starts(s, "MODEL"))
+			throw ParseError(r.line(), "unexpected multi-MODEL input");
+		else
+			throw ParseError(r.line(), "unknown or inappropriate tag");
+	}
+	if (root.atoms.empty())
+		throw ParseError(r.line(), "no atoms in the ligand");
+	if (!torsdof)
+		throw ParseError(r.line(), "missing TORSDOF");
+	// postprocess_ligand: the root frame's origin is its first atom
+	Builder b;
+	b.frame(root, -1, -1, -1, 0);
+	b.grow();
+	const auto fixed = [&](int i, int j) { return b.mob[size_t(i)][size_t(j)] != VARIABLE; };
+	std::vector<int> bonds = assign_bonds(b.atoms, fixed, nullptr);
+	sgpu_mol* m = finish(b.atoms, bonds, r.lines);
+	m->nmovable = m->natoms();
+	m->frame_parent = b.frame_parent;
+	m->frame_begin = b.frame_begin;
+	m->frame_end = b.frame_end;
+	m->frame_origin_atom = b.frame_origin;
+	m->frame_axis_atom = b.frame_axis;
+	m->tree_frame = {0};
+	m->nligands = 1;
+	m->publish();
+	return m;
+}
+
+sgpu_mol* read_receptor(const char* text)
+{
+	std::vector<std::string> lines = split_lines(text);
+	std::vector<Atom> atoms;
+	for (size_t i = 0; i < lines.size(); i++)
+	{
+		const std::string& s = lines[i];
+		if (starts(s, "ATOM  ") || starts(s, "HETATM"))
+			atoms.push_back(parse_atom(s, int(i)));
+		else if (starts(s, "MODEL"))
+			throw ParseError(int(i) + 1, "unexpected multi-MODEL input");
+	}
+	// every receptor pair is fixed; neighbours from 4 A cells (the bond
+	// reach is at most 1.1 x 2 x 1.75 A)
+	const int n = int(atoms.size());
+	std::vector<std::vector<int>> nb(static_cast<size_t>(n));
+	if (n > 0)
+	{
+		const double cell = 4.0;
+		double lo[3] = {atoms[0].xyz[0], atoms[0].xyz[1], atoms[0].xyz[2]};
+		for (const Atom& a : atoms)
+			for (int k = 0; k < 3; k++)
+				lo[k] = std::min(lo[k], a.xyz[k]);
+		std::vector<std::array<int, 3>> key(static_cast<size_t>(n));
+		std::map<std::array<int, 3>, std::vector<int>> grid;
+		for (int i = 0; i < n; i++)
+		{
+			for (int k = 0; k < 3; k++)
+				key[size_t(i)][k] = int(std::floor((atoms[size_t(i)].xyz[k] - lo[k]) / cell));
+			grid[key[size_t(i)]].push_back(i);
+		}
+		for (int i = 0; i < n; i++)
+			for (int dx = -1; dx <= 1; dx++)
+				for (int dy = -1; dy <= 1; dy++)
+					for (int dz = -1; dz <= 1; dz++)
+					{
+						const std::array<int, 3> c = {key[size_t(i)][0] + dx, key[size_t(i)][1] + dy, key[size_t(i)][2] + dz};
+						auto it = grid.find(c);
+						if (it != grid.end())
+							for (int j : it->second)
+								nb[size_t(i)].push_back(j);
+					}
+	}
+	std::vector<int> bonds = assign_bonds(atoms, [](int, int) { return true; }, &nb);
+	sgpu_mol* m = finish(atoms, bonds, lines);
+	m->nmovable = 0;
+	m->publish();
+	return m;
+}
+
+// set_conf (src/lib/tree.h:84-195) on the host: the movable atoms at conf
+std::vector<double> pose_coords(const FlatModel& f, const double* conf)
+{
+	const int nf = int(f.frame_parent.size());
+	std::vector<double> O(3 * size_t(nf)), R(9 * size_t(nf)), Q(4 * size_t(nf));
+	auto to_r3 = [](const double* q, double* r) {
+		const double a = q[0], b = q[1], c = q[2], d = q[3];
+		r[0] = a * a + b * b - c * c - d * d;
+		r[1] = 2 * (b * c - a * d);
+		r[2] = 2 * (a * c + b * d);
+		r[3] = 2 * (b * c + a * d);
+		r[4] = a * a - b * b + c * c - d * d;
+		r[5] = 2 * (c * d - a * b);
+		r[6] = 2 * (b * d - a * c);
+		r[7] = 2 * (a * b + c * d);
+		r[8] = a * a - b * b - c * c + d * d;
+	};
+	auto normalize = [](double* q) {
+		const double s = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+		for (int k = 0; k < 4; k++)
+			q[k] /= s;
+	};
+	std::vector<double> x(f.atom_local.size());
+	for (int fr = 0; fr < nf; fr++)
+	{
+		double* o = &O[3 * size_t(fr)];
+		double* q = &Q[4 * size_t(fr)];
+		const int ci = f.frame_conf[size_t(fr)];
+		const int kind = f.frame_kind[size_t(fr)];
+		if (kind == SGPU_FRAME_RIGID_ROOT)
+		{
+			for (int k = 0; k < 3; k++)
+				o[k] = conf[ci + k];
+			for (int k = 0; k < 4; k++)
+				q[k] = conf[ci + 3 + k];
+			normalize(q);
+		}
+		else
+		{
+			const int p = f.frame_parent[size_t(fr)];
+			double axis[3];
+			if (kind == SGPU_FRAME_SEGMENT)
+			{
+				const double* rp = &R[9 * size_t(p)];
+				const double* ro = &f.frame_origin[3 * size_t(fr)];
+				const double* ra = &f.frame_axis[3 * size_t(fr)];
+				for (int k = 0; k < 3; k++)
+				{
+					o[k] = O[3 * size_t(p) + k] + rp[3 * k] * ro[0] + rp[3 * k + 1] * ro[1] + rp[3 * k + 2] * ro[2];
+					axis[k] = rp[3 * k] * ra[0] + rp[3 * k + 1] * ra[1] + rp[3 * k + 2] * ra[2];
+				}
+			}
+			else
+				for (int k = 0; k < 3; k++)
+				{
+					o[k] = f.frame_origin[3 * size_t(fr) + k];
+					axis[k] = f.frame_axis[3 * size_t(fr) + k];
+				}
+			const double h = conf[ci] / 2, s = std::sin(h);
+			const double t[4] = {std::cos(h), s * axis[0], s * axis[1], s * axis[2]};
+			if (kind == SGPU_FRAME_SEGMENT)
+			{
+				const double* u = &Q[4 * size_t(p)];
+				q[0] = t[0] * u[0] - t[1] * u[1] - t[2] * u[2] - t[3] * u[3];
+				q[1] = t[0] * u[1] + t[1] * u[0] + t[2] * u[3] - t[3] * u[2];
+				q[2] = t[0] * u[2] - t[1] * u[3] + t[2] * u[0] + t[3] * u[1];
+				q[3] = t[0] * u[3] + t[1] * u[2] - t[2] * u[1] + t[3] * u[0];
+			}
+			else
+				for (int k = 0; k < 4; k++)
+					q[k] = t[k];
+			normalize(q);
+		}
+		double* r = &R[9 * size_t(fr)];
+		to_r3(q, r);
+		for (int a = f.frame_begin[size_t(fr)]; a < f.frame_end[size_t(fr)]; a++)
+		{
+			const double* l = &f.atom_local[3 * size_t(a)];
+			for (int k = 0; k < 3; k++)
+				x[3 * size_t(a) + k] = o[k] + r[3 * k] * l[0] + r[3 * k + 1] * l[1] + r[3 * k + 2] * l[2];
+		}
+	}
+	return x;
+}
+
+// string_write_coord: %8.3f at a 1-based column
+void write_coord(std::string& s, size_t col, double v)
+{
+	char buf[64];
+	std::snprintf(buf, sizeof buf, "%8.3f", v);
+	if (std::strlen(buf) != 8 || s.size() < col - 1 + 8)
+		throw std::runtime_error("coordinate does not fit the PDBQT columns");
+	for (int k = 0; k < 8; k++)
+		s[col - 1 + size_t(k)] = buf[k];
+}
+
+} // namespace
+
+std::string write_pdbqt_poses(const sgpu_mol& lig, int nposes, const double* conf, int conf_stride, const double* e,
+		const double* rmsd)
+{
+	if (lig.text.empty())
+		throw std::invalid_argument("the ligand was not read from PDBQT");
+	FlatModel f = flatten(lig.desc);
+	if (conf_stride < f.view.conf_size)
+		throw std::invalid_argument("conf stride below the conf size");
+	std::ostringstream out;
+	for (int p = 0; p < nposes; p++)
+	{
+		const std::vector<double> x = pose_coords(f, conf + size_t(p) * conf_stride);
+		out << "MODEL " << p + 1 << "\n";
+		char buf[64];
+		std::snprintf(buf, sizeof buf, "%.9g", double(float(e[p])));
+		out << "REMARK minimizedAffinity " << buf << "\n";
+		std::snprintf(buf, sizeof buf, "%.9g", double(float(rmsd ? rmsd[p] : -1.0)));
+		out << "REMARK minimizedRMSD " << buf << "\n";
+		for (size_t i = 0; i < lig.text.size(); i++)
+		{
+			std::string s = lig.text[i];
+			const int a = lig.text_atom[i];
+			if (a >= 0)
+				for (int k = 0; k < 3; k++)
+					write_coord(s, 31 + 8 * size_t(k), x[3 * size_t(a) + k]);
+			else if (starts(s, "BEGIN_RES") || starts(s, "END_RES"))
+				continue;
+			out << s << "\n";
+		}
+		out << "ENDMDL\n";
+	}
+	return out.str();
+}
+
+} // namespace sgpu_host
+
+using namespace sgpu_host;
+
+namespace
+{
+sgpu_mol* guarded_read(sgpu_mol* (*fn)(const char*), const char* text, char* err, int err_len)
+{
+	try
+	{
+		if (!text)
+			throw std::invalid_argument("no text");
+		return fn(text);
+	}
+	catch (std::exception& e)
+	{
+		if (err && err_len > 0)
+			std::snprintf(err, size_t(err_len), "%s", e.what());
+		return nullptr;
+	}
+}
+} // namespace
+
+extern "C" {
+
+sgpu_mol* sgpu_read_pdbqt_ligand(const char* text, char* err, int err_len)
+{
+	return guarded_read(read_ligand, text, err, err_len);
+}
+
+sgpu_mol* sgpu_read_pdbqt_receptor(const char* text, char* err, int err_len)
+{
+	return guarded_read(read_receptor, text, err, err_len);
+}
+
+int sgpu_write_pdbqt_poses(const sgpu_mol* lig, int nposes, const double* conf, int conf_stride, const double* e,
+		const double* rmsd, char* out, int out_len)
+{
+	try
+	{
+		if (!lig || nposes < 0 || (nposes > 0 && (!conf || !e)))
+			return -1;
+		const std::string s = write_pdbqt_poses(*lig, nposes, conf, conf_stride, e, rmsd);
+		if (out && out_len > 0)
+			std::snprintf(out, size_t(out_len), "%s", s.c_str());
+		return int(s.size());
+	}
+	catch (...)
+	{
+		return -1;
+	}
+}
+
+} // extern "C"

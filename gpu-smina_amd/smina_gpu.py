@@ -142,7 +142,8 @@ class MinimizeResult(C.Structure):
 
 class DockResult(C.Structure):
     _fields_ = [("nposes", C.c_int), ("e", C.c_double * 32), ("evals", C.c_int64), ("seconds", C.c_double),
-                ("coords", F64P), ("coords_capacity", C.c_int), ("ncoords", C.c_int)]
+                ("coords", F64P), ("coords_capacity", C.c_int), ("ncoords", C.c_int),
+                ("confs", F64P), ("confs_capacity", C.c_int), ("conf_size", C.c_int)]
 
 
 def _load():
@@ -174,6 +175,9 @@ def _load():
         "sgpu_mol_desc": (C.POINTER(ModelDesc), [vp]),
         "sgpu_synth_receptor": (vp, [C.c_uint32, C.c_int, C.c_double]),
         "sgpu_synth_ligand": (vp, [C.c_uint32, C.c_int, C.c_int, F64P]),
+        "sgpu_read_pdbqt_ligand": (vp, [C.c_char_p, C.c_char_p, C.c_int]),
+        "sgpu_read_pdbqt_receptor": (vp, [C.c_char_p, C.c_char_p, C.c_int]),
+        "sgpu_write_pdbqt_poses": (C.c_int, [vp, C.c_int, F64P, C.c_int, F64P, F64P, C.c_char_p, C.c_int]),
         "sgpu_synth_flex": (vp, [vp, vp, C.c_uint32, C.c_int, C.c_int, C.c_int, F64P, C.POINTER(vp)]),
         "sgpu_vina_scoring": (None, [C.POINTER(Term), C.POINTER(CITerm), C.POINTER(Scoring), C.c_double]),
         "sgpu_conf_independent": (C.c_double, [C.POINTER(Scoring), C.c_double, C.c_double]),
@@ -245,6 +249,10 @@ class Mol:
         self.type = np.ctypeslib.as_array(d.type, (d.natoms,)).copy()
         self.xyz = np.ctypeslib.as_array(d.xyz, (d.natoms * 3,)).copy().reshape(-1, 3)
         self.nframes = d.nframes
+        self.bonds = np.ctypeslib.as_array(d.bonds, (2 * d.nbonds,)).copy().reshape(-1, 2) if d.nbonds else \
+            np.zeros((0, 2), np.int32)
+        self.frames = [dict(parent=d.frame_parent[f], begin=d.frame_begin[f], end=d.frame_end[f],
+                            origin=d.frame_origin_atom[f], axis=d.frame_axis_atom[f]) for f in range(d.nframes)]
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -259,6 +267,38 @@ class Mol:
     def ligand(seed, nheavy=24, ntors=6, center=(0.0, 0.0, 0.0)):
         c = f64(center)
         return Mol(lib.sgpu_synth_ligand(seed, nheavy, ntors, ptr(c, F64P)))
+
+    @staticmethod
+    def pdbqt_ligand(text):
+        """a ligand read from PDBQT text (sgpu_read_pdbqt_ligand)"""
+        err = C.create_string_buffer(512)
+        h = lib.sgpu_read_pdbqt_ligand(text.encode(), err, 512)
+        if not h:
+            raise ValueError(err.value.decode())
+        return Mol(h)
+
+    @staticmethod
+    def pdbqt_receptor(text):
+        """a rigid receptor read from PDBQT text (sgpu_read_pdbqt_receptor)"""
+        err = C.create_string_buffer(512)
+        h = lib.sgpu_read_pdbqt_receptor(text.encode(), err, 512)
+        if not h:
+            raise ValueError(err.value.decode())
+        return Mol(h)
+
+    def write_pdbqt(self, confs, energies, rmsds=None):
+        """the poses as PDBQT MODEL blocks (sgpu_write_pdbqt_poses)"""
+        confs = np.atleast_2d(f64(confs))
+        e = f64(energies)
+        r = None if rmsds is None else f64(rmsds)
+        n = lib.sgpu_write_pdbqt_poses(self.h, len(e), ptr(confs, F64P), confs.shape[1], ptr(e, F64P),
+                                       None if r is None else ptr(r, F64P), None, 0)
+        if n < 0:
+            raise ValueError("cannot write these poses")
+        buf = C.create_string_buffer(n + 1)
+        lib.sgpu_write_pdbqt_poses(self.h, len(e), ptr(confs, F64P), confs.shape[1], ptr(e, F64P),
+                                   None if r is None else ptr(r, F64P), buf, n + 1)
+        return buf.value.decode()
 
     @staticmethod
     def flex(receptor, ligand, seed, nflex=4, min_chi=2, max_chi=4, center=(0.0, 0.0, 0.0)):
@@ -588,23 +628,33 @@ class Docking:
             raise RuntimeError(lib.sgpu_last_error(ctx).decode())
         return ms.value
 
-    def dock(self, ligands, top_coords=False):
+    def dock(self, ligands, top_coords=False, confs=False):
         """per ligand: energies of the modes, evaluations, seconds; with
-        top_coords, the top pose's heavy-atom coordinates ([H, 3])"""
+        top_coords, the top pose's heavy-atom coordinates ([H, 3]); with
+        confs, every mode's conf ([modes, conf_size])"""
         arr = (C.c_void_p * len(ligands))(*[m.h.value for m in ligands])
         out = (DockResult * len(ligands))()
-        bufs = []
+        bufs, cbufs = [], []
         if top_coords:
             for r in out:
                 b = np.zeros(3 * 256)
                 bufs.append(b)
                 r.coords = ptr(b, F64P)
                 r.coords_capacity = b.size
+        if confs:
+            for r in out:
+                b = np.zeros(32 * 512)
+                cbufs.append(b)
+                r.confs = ptr(b, F64P)
+                r.confs_capacity = b.size
         if lib.sgpu_dock(self.h, arr, len(ligands), out) != 0:
             raise RuntimeError(lib.sgpu_docking_error(self.h).decode())
         res = [dict(e=list(r.e[:r.nposes]), evals=r.evals, seconds=r.seconds) for r in out]
         for k, b in enumerate(bufs):
             res[k]["coords"] = b[:out[k].ncoords].reshape(-1, 3).copy()
+        for k, b in enumerate(cbufs):
+            cs = out[k].conf_size
+            res[k]["confs"] = b[:out[k].nposes * cs].reshape(-1, cs).copy() if cs else np.zeros((0, 0))
         return res
 
     def full_searches(self):

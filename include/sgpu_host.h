@@ -44,6 +44,29 @@ sgpu_mol* sgpu_synth_ligand(uint32_t seed, int nheavy, int ntors, const double* 
 sgpu_mol* sgpu_synth_flex(const sgpu_mol* receptor, const sgpu_mol* ligand, uint32_t seed, int nflex, int min_chi,
 		int max_chi, const double* center, sgpu_mol** rigid);
 
+/* ---- PDBQT input and output, without OpenBabel (smina's own parser and
+ * model set-up: src/lib/parse_pdbqt.cpp:146-336, src/lib/model.cpp:448-566;
+ * the writer of src/lib/model.cpp:710-750 and src/lib/result_info.cpp:
+ * 146-153).  The text is the whole file, NUL-terminated. */
+/* A ligand: ROOT / ENDROOT, nested BRANCH a b / ENDBRANCH a b, one TORSDOF.
+ * Atoms in smina's order (each frame's atoms in file order, each followed
+ * by the first atoms of its child branches); branches that move only
+ * hydrogens merge into their parent (the default without --flex_hydrogens);
+ * covalent bonds by distance between atoms whose distance cannot change;
+ * X-Score types from the bonds.  NULL on a syntax error, with the reason and
+ * line in err. */
+sgpu_mol* sgpu_read_pdbqt_ligand(const char* text, char* err, int err_len);
+/* A rigid receptor: its ATOM / HETATM records (other lines are ignored),
+ * typed from its bonds. */
+sgpu_mol* sgpu_read_pdbqt_receptor(const char* text, char* err, int err_len);
+/* Poses of a ligand read from PDBQT: per pose, "MODEL k", "REMARK
+ * minimizedAffinity e", "REMARK minimizedRMSD r" (-1 without rmsd), the
+ * file's lines with each atom at the pose's conf (conf [nposes][conf_stride]),
+ * "ENDMDL".  Writes at most out_len bytes (NUL included) and returns the
+ * full length, or -1. */
+int sgpu_write_pdbqt_poses(const sgpu_mol* ligand, int nposes, const double* conf, int conf_stride,
+		const double* e, const double* rmsd, char* out, int out_len);
+
 /* ---- scoring */
 /* A built-in scoring function by smina's name (builtin_scoring,
  * src/lib/builtinscoring.cpp:65-113): "vina" (= "default"), "vinardo",
@@ -117,6 +140,9 @@ typedef struct sgpu_dock_result
 	double* coords;  /* in: the caller's array for the top pose's heavy-atom coordinates, or NULL */
 	int coords_capacity; /* in: its length in doubles */
 	int ncoords;     /* out: doubles written (3 per heavy atom) */
+	double* confs;   /* in: the caller's array for the poses' confs (nposes x conf_size), or NULL */
+	int confs_capacity; /* in: its length in doubles */
+	int conf_size;   /* out: doubles per conf (poses that did not fit are not written) */
 } sgpu_dock_result;
 
 /* build the receptor maps for every heavy-atom type of these ligands (the
