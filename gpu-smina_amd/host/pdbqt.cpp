@@ -26,6 +26,7 @@
 #include <cstdio>
 #include <functional>
 #include <map>
+#include <set>
 #include <cstring>
 #include <sstream>
 #include <stdexcept>
@@ -655,6 +656,245 @@ sgpu_mol* read_receptor(const char* text)
 	return m;
 }
 
+// Flexible residues (parse_pdbqt_flex and postprocess_residue,
+// src/lib/parse_pdbqt.cpp:278-294, :413-437): BEGIN_RES ... END_RES blocks,
+// each a ROOT and its branches.  A residue's root atoms and the first atoms
+// of their branches are fixed ("inflex"); each branch of a root atom is a
+// side-chain tree whose first segment turns about the bond from the root
+// atom to the branch's first atom.  References: >= 0 a movable atom of the
+// side chains, <= -2 the fixed atom -2 - ref.
+struct FlexBuilder
+{
+	std::vector<Atom> mov, fixed;
+	std::vector<int> frame_parent, frame_begin, frame_end, frame_origin, frame_axis, tree_frame;
+	std::vector<std::pair<int, int>> tied; // reference pairs whose distance cannot change
+
+	int fixed_ref(const Atom& a)
+	{
+		fixed.push_back(a);
+		return -2 - (int(fixed.size()) - 1);
+	}
+	int push(const Atom& a)
+	{
+		mov.push_back(a);
+		return int(mov.size()) - 1;
+	}
+	// as Builder::frame, with references
+	void frame(Level& lv, int parent, int axis_begin, int axis_end)
+	{
+		const int f = int(frame_parent.size());
+		frame_parent.push_back(parent);
+		frame_begin.push_back(int(mov.size()));
+		frame_end.push_back(0);
+		frame_origin.push_back(axis_end);
+		frame_axis.push_back(axis_begin);
+		std::vector<std::vector<int>> child_axis(lv.atoms.size());
+		std::vector<int> node_index(lv.atoms.size(), -1);
+		for (size_t i = 0; i < lv.atoms.size(); i++)
+		{
+			Node& nd = lv.atoms[i];
+			if (int(i) != lv.immobile)
+				node_index[i] = push(nd.a);
+			for (Level& c : nd.children)
+				child_axis[i].push_back(push(c.atoms[size_t(c.immobile)].a));
+		}
+		frame_end[size_t(f)] = int(mov.size());
+		for (int i = frame_begin[size_t(f)]; i < frame_end[size_t(f)]; i++)
+		{
+			tied.push_back({axis_begin, i});
+			tied.push_back({axis_end, i});
+		}
+		tied.push_back({axis_begin, axis_end});
+		for (size_t i = 0; i < lv.atoms.size(); i++)
+		{
+			Node& nd = lv.atoms[i];
+			const int here = int(i) == lv.immobile ? axis_end : node_index[i];
+			for (size_t k = 0; k < nd.children.size(); k++)
+				if (!nd.children[k].essentially_empty())
+					frame(nd.children[k], f, here, child_axis[i][k]);
+		}
+	}
+	void residue(Level& root)
+	{
+		std::vector<int> node_ref(root.atoms.size());
+		std::vector<std::vector<int>> child_ref(root.atoms.size());
+		for (size_t i = 0; i < root.atoms.size(); i++)
+		{
+			node_ref[i] = fixed_ref(root.atoms[i].a);
+			for (Level& c : root.atoms[i].children)
+				child_ref[i].push_back(fixed_ref(c.atoms[size_t(c.immobile)].a));
+		}
+		for (size_t i = 0; i < root.atoms.size(); i++)
+			for (size_t k = 0; k < root.atoms[i].children.size(); k++)
+			{
+				Level& c = root.atoms[i].children[k];
+				if (c.essentially_empty())
+					continue;
+				tree_frame.push_back(int(frame_parent.size()));
+				frame(c, -1, node_ref[i], child_ref[i][k]);
+			}
+	}
+};
+
+// parse_pdbqt_aux for a residue: ROOT ... ENDROOT, then branches up to
+// END_RES (no TORSDOF)
+void parse_residue(Reader& r, Level& root)
+{
+	std::string s;
+	bool in_root = false;
+	while (r.next(s))
+	{
+		if (ignorable(s))
+			continue;
+		if (starts(s, "ROOT"))
+		{
+			in_root = true;
+			break;
+		}
+		throw ParseError(r.line(), "unknown or inappropriate tag");
+	}
+	if (!in_root)
+		throw ParseError(r.line(), "no ROOT in the residue");
+	bool end_root = false;
+	while (r.next(s))
+	{
+		if (ignorable(s))
+			continue;
+		if (starts(s, "ATOM  ") || starts(s, "HETATM"))
+			root.atoms.push_back(Node{parse_atom(s, int(r.pos) - 1), {}});
+		else if (starts(s, "ENDROOT"))
+		{
+			end_root = true;
+			break;
+		}
+		else
+			throw ParseError(r.line(), "unknown or inappropriate tag");
+	}
+	if (!end_root)
+		throw ParseError(r.line(), "missing ENDROOT");
+	while (r.next(s))
+	{
+		if (ignorable(s))
+			continue;
+		if (starts(s, "BRANCH"))
+			branch_into(r, s, root);
+		else if (starts(s, "END_RES"))
+			return;
+		else
+			throw ParseError(r.line(), "unknown or inappropriate tag");
+	}
+	throw ParseError(r.line(), "missing END_RES");
+}
+
+// the ligand plus the side chains of a flexible-residue file, in the layout
+// of the model description: the ligand's atoms and trees, the side chains'
+// movable atoms and trees, then the fixed atoms.  Bonds of the side chains
+// by distance (model::assign_bonds): within a frame, a frame and its axis
+// atoms, and among the fixed atoms and the rigid receptor's atoms near them
+// (the fixed atoms' types see the receptor).
+sgpu_mol* read_flex(const sgpu_mol& lig, const sgpu_mol& rigid, const char* text)
+{
+	Reader r;
+	r.lines = split_lines(text);
+	FlexBuilder fb;
+	std::string s;
+	while (r.next(s))
+	{
+		if (s.empty() || starts(s, "WARNING") || starts(s, "REMARK") || starts(s, "USER"))
+			continue;
+		if (starts(s, "BEGIN_RES"))
+		{
+			Level root;
+			parse_residue(r, root);
+			fb.residue(root);
+		}
+		else if (starts(s, "MODEL"))
+			throw ParseError(r.line(), "unexpected multi-MODEL input");
+		else
+			throw ParseError(r.line(), "unknown or inappropriate tag");
+	}
+	if (fb.tree_frame.empty())
+		throw ParseError(r.line(), "no flexible side chain");
+	const int M = int(fb.mov.size()), I = int(fb.fixed.size());
+	// receptor atoms within 4 A of a fixed atom
+	std::vector<Atom> atoms = fb.mov;
+	atoms.insert(atoms.end(), fb.fixed.begin(), fb.fixed.end());
+	const int nrig = rigid.natoms();
+	for (int a = 0; a < nrig; a++)
+	{
+		bool near = false;
+		for (const Atom& x : fb.fixed)
+		{
+			double d2 = 0;
+			for (int k = 0; k < 3; k++)
+				d2 += (rigid.xyz[3 * size_t(a) + size_t(k)] - x.xyz[k]) * (rigid.xyz[3 * size_t(a) + size_t(k)] - x.xyz[k]);
+			near = near || d2 < 16.0;
+		}
+		if (near)
+		{
+			Atom ra;
+			ra.type = rigid.type[size_t(a)];
+			for (int k = 0; k < 3; k++)
+				ra.xyz[k] = rigid.xyz[3 * size_t(a) + size_t(k)];
+			ra.line = -1;
+			atoms.push_back(ra);
+		}
+	}
+	auto local = [&](int ref) { return ref >= 0 ? ref : M + (-2 - ref); };
+	std::vector<int> frame_of(size_t(M), -1);
+	for (size_t f = 0; f < fb.frame_parent.size(); f++)
+		for (int a = fb.frame_begin[f]; a < fb.frame_end[f]; a++)
+			frame_of[size_t(a)] = int(f);
+	std::set<std::pair<int, int>> tied;
+	for (auto& t : fb.tied)
+	{
+		const int a = local(t.first), b = local(t.second);
+		tied.insert({std::min(a, b), std::max(a, b)});
+	}
+	const auto fixed_pair = [&](int i, int j) {
+		const bool mi = i < M, mj = j < M;
+		if (!mi && !mj)
+			return true; // fixed-fixed, fixed-receptor, receptor-receptor
+		if (mi && mj && frame_of[size_t(i)] == frame_of[size_t(j)])
+			return true;
+		if ((mi && j >= M + I) || (mj && i >= M + I))
+			return false;
+		return tied.count({std::min(i, j), std::max(i, j)}) > 0;
+	};
+	std::vector<int> all_bonds = assign_bonds(atoms, fixed_pair, nullptr);
+	assign_types(atoms, all_bonds);
+	sgpu_mol* m = new sgpu_mol(lig);
+	m->text.clear();
+	m->text_atom.clear();
+	const int L = lig.natoms();
+	auto final_index = [&](int loc) { return L + loc; }; // movable then fixed, in local order
+	for (int a = 0; a < M + I; a++)
+	{
+		m->type.push_back(atoms[size_t(a)].type);
+		m->xyz.insert(m->xyz.end(), atoms[size_t(a)].xyz, atoms[size_t(a)].xyz + 3);
+	}
+	for (size_t k = 0; k + 1 < all_bonds.size(); k += 2)
+		if (all_bonds[k] < M + I && all_bonds[k + 1] < M + I)
+		{
+			m->bonds.push_back(final_index(all_bonds[k]));
+			m->bonds.push_back(final_index(all_bonds[k + 1]));
+		}
+	const int f0 = int(lig.frame_parent.size());
+	for (size_t f = 0; f < fb.frame_parent.size(); f++)
+	{
+		m->frame_parent.push_back(fb.frame_parent[f] < 0 ? -1 : f0 + fb.frame_parent[f]);
+		m->frame_begin.push_back(L + fb.frame_begin[f]);
+		m->frame_end.push_back(L + fb.frame_end[f]);
+		m->frame_origin_atom.push_back(final_index(local(fb.frame_origin[f])));
+		m->frame_axis_atom.push_back(final_index(local(fb.frame_axis[f])));
+	}
+	for (int t : fb.tree_frame)
+		m->tree_frame.push_back(f0 + t);
+	m->nmovable = L + M;
+	m->publish();
+	return m;
+}
+
 // set_conf (src/lib/tree.h:84-195) on the host: the movable atoms at conf
 std::vector<double> pose_coords(const FlatModel& f, const double* conf)
 {
@@ -820,6 +1060,25 @@ sgpu_mol* sgpu_read_pdbqt_ligand(const char* text, char* err, int err_len)
 sgpu_mol* sgpu_read_pdbqt_receptor(const char* text, char* err, int err_len)
 {
 	return guarded_read(read_receptor, text, err, err_len);
+}
+
+sgpu_mol* sgpu_read_pdbqt_flex(const sgpu_mol* ligand, const sgpu_mol* rigid, const char* text, char* err,
+		int err_len)
+{
+	try
+	{
+		if (!ligand || !rigid || !text)
+			throw std::invalid_argument("ligand, rigid receptor and text are required");
+		if (ligand->nmovable != ligand->natoms() || ligand->nligands < 1)
+			throw std::invalid_argument("the ligand already carries side chains");
+		return read_flex(*ligand, *rigid, text);
+	}
+	catch (std::exception& e)
+	{
+		if (err && err_len > 0)
+			std::snprintf(err, size_t(err_len), "%s", e.what());
+		return nullptr;
+	}
 }
 
 int sgpu_write_pdbqt_poses(const sgpu_mol* lig, int nposes, const double* conf, int conf_stride, const double* e,

@@ -177,6 +177,7 @@ def _load():
         "sgpu_synth_ligand": (vp, [C.c_uint32, C.c_int, C.c_int, F64P]),
         "sgpu_read_pdbqt_ligand": (vp, [C.c_char_p, C.c_char_p, C.c_int]),
         "sgpu_read_pdbqt_receptor": (vp, [C.c_char_p, C.c_char_p, C.c_int]),
+        "sgpu_read_pdbqt_flex": (vp, [vp, vp, C.c_char_p, C.c_char_p, C.c_int]),
         "sgpu_write_pdbqt_poses": (C.c_int, [vp, C.c_int, F64P, C.c_int, F64P, F64P, C.c_char_p, C.c_int]),
         "sgpu_synth_flex": (vp, [vp, vp, C.c_uint32, C.c_int, C.c_int, C.c_int, F64P, C.POINTER(vp)]),
         "sgpu_vina_scoring": (None, [C.POINTER(Term), C.POINTER(CITerm), C.POINTER(Scoring), C.c_double]),
@@ -282,6 +283,16 @@ class Mol:
         """a rigid receptor read from PDBQT text (sgpu_read_pdbqt_receptor)"""
         err = C.create_string_buffer(512)
         h = lib.sgpu_read_pdbqt_receptor(text.encode(), err, 512)
+        if not h:
+            raise ValueError(err.value.decode())
+        return Mol(h)
+
+    @staticmethod
+    def pdbqt_flex(ligand, rigid, text):
+        """the ligand plus the side chains of a flexible-residue PDBQT file
+        (sgpu_read_pdbqt_flex)"""
+        err = C.create_string_buffer(512)
+        h = lib.sgpu_read_pdbqt_flex(ligand.h, rigid.h, text.encode(), err, 512)
         if not h:
             raise ValueError(err.value.decode())
         return Mol(h)

@@ -59,6 +59,15 @@ sgpu_mol* sgpu_read_pdbqt_ligand(const char* text, char* err, int err_len);
 /* A rigid receptor: its ATOM / HETATM records (other lines are ignored),
  * typed from its bonds. */
 sgpu_mol* sgpu_read_pdbqt_receptor(const char* text, char* err, int err_len);
+/* --flex: a flexible-residue file (BEGIN_RES ... END_RES blocks of ROOT and
+ * branches; parse_pdbqt_flex, src/lib/parse_pdbqt.cpp:278-294, :413-437)
+ * joined to a ligand read as above: the ligand's atoms and trees, then one
+ * tree per branch of a residue's root atoms (a first segment turning about
+ * the bond from the root atom), then the fixed atoms (the root atoms and
+ * the first atoms of their branches).  The fixed atoms are typed with the
+ * rigid receptor's atoms near them. */
+sgpu_mol* sgpu_read_pdbqt_flex(const sgpu_mol* ligand, const sgpu_mol* rigid, const char* text, char* err,
+		int err_len);
 /* Poses of a ligand read from PDBQT: per pose, "MODEL k", "REMARK
  * minimizedAffinity e", "REMARK minimizedRMSD r" (-1 without rmsd), the
  * file's lines with each atom at the pose's conf (conf [nposes][conf_stride]),

@@ -226,3 +226,84 @@ def test_docking_from_pdbqt_files():
                     for l in to_pdbqt(sg.Mol.ligand(1, 24, 6)).splitlines() if l.startswith("ATOM")])
     order = [int(np.argmin(np.abs(src - lig.xyz[i]).sum(axis=1))) for i in heavy]
     assert np.allclose(xyz[order], r["coords"], atol=1.5e-3)
+
+
+def flex_pdbqt(model, nlig_frames):
+    """the side chains of a model description (trees after the ligand's)
+    written as a flexible-residue file: per chain a residue whose ROOT holds
+    the fixed anchor atom and its hydrogens, and whose branches follow the
+    chain's frames (each opening with its origin atom)"""
+    fr = model.frames
+    children = {f: [c for c in range(len(fr)) if fr[c]["parent"] == f] for f in range(len(fr))}
+    bonded = {}
+    for a, b in model.bonds.tolist():
+        bonded.setdefault(a, []).append(b)
+        bonded.setdefault(b, []).append(a)
+    lines = []
+
+    def rec(a):
+        x, y, z = model.xyz[a]
+        lines.append(atom(a + 1, "X", x, y, z, AD_NAME[int(model.type[a])]))
+
+    def body(f):
+        imm = {fr[c]["origin"] for c in children[f]}
+        for a in range(fr[f]["begin"], fr[f]["end"]):
+            if a not in imm and a != fr[f]["origin"]:
+                rec(a)
+        for c in children[f]:
+            lines.append(f"BRANCH {fr[c]['axis'] + 1:3d} {fr[c]['origin'] + 1:3d}")
+            rec(fr[c]["origin"])
+            body(c)
+            lines.append(f"ENDBRANCH {fr[c]['axis'] + 1:3d} {fr[c]['origin'] + 1:3d}")
+
+    for f in range(nlig_frames, len(fr)):
+        if fr[f]["parent"] != -1:
+            continue
+        anchor = fr[f]["axis"]
+        lines.append("BEGIN_RES SCH A   1")
+        lines.append("ROOT")
+        rec(anchor)
+        for h in bonded.get(anchor, []):
+            if model.type[h] == HD and h >= model.nmovable:
+                rec(h)
+        lines.append("ENDROOT")
+        lines.append(f"BRANCH {anchor + 1:3d} {fr[f]['origin'] + 1:3d}")
+        rec(fr[f]["origin"])
+        body(f)
+        lines.append(f"ENDBRANCH {anchor + 1:3d} {fr[f]['origin'] + 1:3d}")
+        lines.append("END_RES SCH A   1")
+    return "\n".join(lines) + "\n"
+
+
+def test_flexible_residues_read_back():
+    """side chains cut from the synthetic receptor, written as a
+    flexible-residue file and read back beside the ligand: one tree per
+    chain with the same torsions, each chain's first atom now fixed (smina
+    keeps the first atom of a root branch in place), the same atoms.  The
+    two models move alike: over several confs their energies differ by one
+    constant, the pairs between fixed atoms that smina leaves out
+    (model::initialize_pairs) and the synthetic layout counts, up to the
+    file's rounding"""
+    rec = sg.Mol.receptor(1, 4000, 9.0)
+    lig = sg.Mol.ligand(7, 30, 6)
+    model, rigid = sg.Mol.flex(rec, lig, 3, nflex=4)
+    back_lig = sg.Mol.pdbqt_ligand(to_pdbqt(lig))
+    back = sg.Mol.pdbqt_flex(back_lig, rigid, flex_pdbqt(model, lig.nframes))
+    assert back.natoms == model.natoms and sorted(back.type) == sorted(model.type)
+    assert back.nframes == model.nframes and back.nmovable == model.nmovable - 4
+    roots = [f for f in back.frames[lig.nframes:] if f["parent"] == -1]
+    assert len(roots) == 4 and all(f["origin"] >= back.nmovable and f["axis"] >= back.nmovable for f in roots)
+    o, ids, _, _ = oracle_for([model, back], rec=rigid)
+    c0 = o.initial_conf(ids[0])
+    rng = np.random.default_rng(5)
+    diffs = []
+    for k in range(4):
+        c = c0.copy()
+        if k:
+            c[7:] += rng.uniform(-0.6, 0.6, c.size - 7)
+            c[:3] += rng.uniform(-0.5, 0.5, 3)
+        # box slope 10 (refinement's): an atom outside the box moves the
+        # penalty by the slope times the file's rounding
+        diffs.append(o.eval(ids[0], c, slope=10.0) - o.eval(ids[1], c, slope=10.0))
+    print("flex energy differences", diffs)
+    assert max(diffs) - min(diffs) < 0.1
