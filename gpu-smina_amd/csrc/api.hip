@@ -573,12 +573,6 @@ TablePtrs<T> table_ptrs(const sgpu_ctx* c)
 	return t;
 }
 
-// the spline approximation runs on the FP64 direct receptor path only
-void require_linear(const sgpu_ctx* c, const char* what)
-{
-	require(c->tab.kind == 0, std::string(what) + " needs the linear tables (the spline approximation is carried "
-	                                              "by FP64 direct evaluation and refinement)");
-}
 
 template<typename T>
 GridPtrs<T> grid_ptrs(const Grid& g, double slope)
@@ -937,7 +931,6 @@ sgpu_status sgpu_build_grids(sgpu_ctx* c, int receptor_id, const sgpu_grid_dims*
 {
 	return guarded(c, [&]() {
 		require(c->tab.ok, "tables not uploaded");
-		require_linear(c, "the map build");
 		check_gd(gd);
 		require(grid_id && types && ntypes > 0 && ntypes <= 16, "1..16 map types");
 		require(precision == SGPU_FP32 || precision == SGPU_FP64, "invalid precision");
@@ -964,6 +957,11 @@ sgpu_status sgpu_build_grids(sgpu_ctx* c, int receptor_id, const sgpu_grid_dims*
 		a.table_n = c->tab.n;
 		a.factor = c->tab.factor;
 		a.cutoff_sqr = c->tab.cutoff_sqr;
+		a.kind = c->tab.kind;
+		a.spl = c->tab.spl.as<double>();
+		a.spl_cut = c->tab.spl_cut.as<double>();
+		a.fraction = c->tab.fraction;
+		a.nseg = c->tab.nseg;
 		std::vector<double> lo, hi;
 		for (int d = 0; d < 3; d++)
 		{
@@ -1371,7 +1369,7 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 			const int bytes = plan_bytes(k.plan, b.precision, w);
 			if (bytes > LDS_LIMIT)
 				break;
-			const int resident = w * mc_blocks_per_cu<T, S>(w, size_t(bytes), p->visited != 0);
+			const int resident = w * mc_blocks_per_cu<T, S>(w, size_t(bytes), p->visited != 0 || c->tab.kind == 1);
 			if (resident >= best && resident > 0)
 			{
 				best = resident;
@@ -1661,8 +1659,6 @@ sgpu_status sgpu_eval_confs(sgpu_ctx* c, const sgpu_eval_request* q)
 		require(q != nullptr, "no request");
 		require(c->tab.ok, "tables not uploaded");
 		Batch& b = find_batch(c, q->batch_id);
-		if (b.precision != SGPU_FP64 || q->grid_id >= 0)
-			require_linear(c, "FP32 or map-based evaluation");
 		if (b.precision == SGPU_FP64)
 			run_eval<double>(c, q);
 		else
@@ -1675,7 +1671,6 @@ sgpu_status sgpu_run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_pa
 {
 	return guarded(c, [&]() {
 		require(c->tab.ok, "tables not uploaded");
-		require_linear(c, "the Monte Carlo search");
 		Batch& b = find_batch(c, batch_id);
 		if (b.precision == SGPU_FP64)
 			run_mc<double, double>(c, batch_id, grid_id, p, ch, out);
@@ -1692,8 +1687,6 @@ sgpu_status sgpu_run_refine(sgpu_ctx* c, const sgpu_refine_request* q)
 		require(q != nullptr, "no request");
 		require(c->tab.ok, "tables not uploaded");
 		Batch& b = find_batch(c, q->batch_id);
-		if (b.precision != SGPU_FP64)
-			require_linear(c, "FP32 refinement");
 		if (b.precision == SGPU_FP64)
 			run_refine<double, double>(c, q);
 		else if (b.precision == SGPU_MIXED)
@@ -1708,7 +1701,6 @@ sgpu_status sgpu_score(sgpu_ctx* c, const sgpu_score_request* q)
 	return guarded(c, [&]() {
 		require(q != nullptr, "no request");
 		require(c->tab.ok, "tables not uploaded");
-		require_linear(c, "the rescoring");
 		run_score(c, q);
 	});
 }

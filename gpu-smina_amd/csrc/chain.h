@@ -295,7 +295,7 @@ struct Visited
 // point, then the line search's trials; a finished line search runs the
 // BFGS step body and starts the next line search.  The order of every
 // evaluation and every arithmetic operation is the CPU's.
-template<typename T, typename S, int BK, bool VIS = false>
+template<typename T, typename S, int BK, bool VIS = false, bool SPL = true>
 struct Searcher
 {
 	const MV<T>& m;
@@ -354,7 +354,7 @@ struct Searcher
 		{
 			++evals;
 			T ge_t;
-			const S fe = S(eval_deriv<T, BK>(m, ec, W, pbuf, T(xe), ge_t));
+			const S fe = S(eval_deriv<T, BK, SPL>(m, ec, W, pbuf, T(xe), ge_t));
 			const S ge = S(ge_t);
 			bool begin = false;
 			if (initial)

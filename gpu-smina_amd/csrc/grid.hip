@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 #include "launch.h"
 #include "atom_types.h"
+#include "spline.h"
 
 namespace sgpu
 {
@@ -155,6 +156,19 @@ __global__ __launch_bounds__(256) void grid_points_kernel(GridBuildArgs a, doubl
 		if (r2 <= a.cutoff_sqr)
 		{
 			const int t1 = a.rec_type[i];
+			if (a.kind == 1)
+			{
+				// precalculate_splines::eval_fast: the spline's value
+				const double r = __builtin_sqrt(r2);
+				for (int j = 0; j < MAX_GRID_TYPES; j++)
+					if (j < a.ntypes)
+					{
+						double val, der;
+						spline_eval(a.spl, a.spl_cut, a.fraction, a.nseg, tri_index_permissive(t1, tt[j]), r, val, der);
+						aff[j] += val;
+					}
+				continue;
+			}
 			const size_t idx = size_t(a.factor * r2);
 			for (int j = 0; j < MAX_GRID_TYPES; j++)
 				if (j < a.ntypes)

@@ -31,7 +31,7 @@ hipError_t launch_mc(const BatchPtrs<T>& B, const EvalCtx<T>& ec, const WavePlan
 // workgroups of `waves` chains with `lds` bytes of LDS that one CU holds at
 // once (the runtime's occupancy calculator: registers and LDS)
 template<typename T, typename S>
-int mc_blocks_per_cu(int waves, size_t lds, bool visited = false);
+int mc_blocks_per_cu(int waves, size_t lds, bool extra = false);
 template<typename T, typename S>
 hipError_t launch_refine(const BatchPtrs<T>& B, const EvalCtx<T>& ec, const WavePlan& plan, int n, const int* model,
 		S* conf, int conf_stride, const int* maxiters, int accurate, int early_term, S* e, T* x, int x_stride,
@@ -51,6 +51,11 @@ struct GridBuildArgs
 	int table_n;
 	double factor;
 	double cutoff_sqr;
+	int kind;              // 0: linear tables (fast), 1: splines
+	const double* spl;
+	const double* spl_cut;
+	double fraction;
+	int nseg;
 	int cell_off[3];
 	int cell_dim[3];
 	const double* cell_lo; // [cell_dim[0] + cell_dim[1] + cell_dim[2]] per dimension

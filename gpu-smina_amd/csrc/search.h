@@ -30,6 +30,7 @@
 #include "wave.h"
 #include "rng.h"
 #include "atom_types.h"
+#include "spline.h"
 #include "../../include/sgpu.h"
 
 namespace sgpu
@@ -546,21 +547,15 @@ __device__ inline T grid_term(const GridPtrs<T>& g, int slot, V3<T> loc, T v, V3
 // (src/lib/precalculate.h:418-450) with Spline::eval_deriv
 // (src/lib/splines.h:103-121): the spline at r = sqrt(r2), its derivative
 // over r
-__device__ inline void spline_deriv(const TablePtrs<double>& tab, int tri, double r2, double& e, double& dor)
+// over r, in FP64 (the FP32 search rounds the results)
+template<typename T>
+__device__ inline void spline_deriv(const TablePtrs<T>& tab, int tri, T r2, T& e, T& dor)
 {
-	const double r = __builtin_sqrt(r2);
-	double val = 0, der = 0;
-	if (r < tab.spl_cut[tri])
-	{
-		unsigned index = unsigned(r / tab.fraction);
-		index = index < unsigned(tab.nseg) ? index : unsigned(tab.nseg - 1);
-		const double* s = tab.spl + (size_t(tri) * tab.nseg + index) * 5;
-		const double lx = r - s[0];
-		val = ((s[1] * lx + s[2]) * lx + s[3]) * lx + s[4];
-		der = (3 * s[1] * lx + 2 * s[2]) * lx + s[3];
-	}
-	e = val;
-	dor = der / r;
+	const double r = __builtin_sqrt(double(r2));
+	double val, der;
+	spline_eval(tab.spl, tab.spl_cut, tab.fraction, tab.nseg, tri, r, val, der);
+	e = T(val);
+	dor = T(der / r);
 }
 
 // precalculate_linear_element::eval_deriv, one component
@@ -568,12 +563,11 @@ __device__ inline void spline_deriv(const TablePtrs<double>& tab, int tri, doubl
 template<typename T>
 __device__ inline void table_deriv(const TablePtrs<T>& tab, int tri, T r2, T& e, T& dor)
 {
-	if constexpr (sizeof(T) == 8)
-		if (tab.kind == 1)
-		{
-			spline_deriv(tab, tri, r2, e, dor);
-			return;
-		}
+	if (tab.kind == 1)
+	{
+		spline_deriv(tab, tri, r2, e, dor);
+		return;
+	}
 	const T r2f = T(tab.factor) * r2;
 	const int i1 = int(r2f);
 	const T rem = r2f - T(i1);
@@ -652,8 +646,10 @@ __device__ inline T direct_term(const EvalCtx<T>& ec, int t1, V3<T> ac, V3<T>& o
 // model::eval_deriv (src/lib/model.cpp:969-983).  x: conf element in this
 // lane; returns e (uniform) and the gradient element in g_lane.  pbuf: the
 // wave's pair-energy scratch in global memory (ordered mode only).  BK: the
-// receptor back end (BK_MAPS or BK_DIRECT), fixed per kernel.
-template<typename T, int BK>
+// receptor back end (BK_MAPS or BK_DIRECT), fixed per kernel.  SPL: the
+// pair loops can read the spline tables (off in the search kernel of
+// smina's own search, which keeps its registers without them).
+template<typename T, int BK, bool SPL = true>
 __device__ __forceinline__ T eval_deriv(const MV<T>& m, const EvalCtx<T>& ec, WaveMem<T>& W, T* pbuf, T x_lane,
 		T& g_lane)
 {
@@ -707,7 +703,7 @@ __device__ __forceinline__ T eval_deriv(const MV<T>& m, const EvalCtx<T>& ec, Wa
 					rz[u] = is_b ? xi.z - xj.z : xj.z - xi.z;
 					r2[u] = sqr(rx[u]) + sqr(ry[u]) + sqr(rz[u]);
 					// precalculate_linear_element::eval_deriv (src/lib/precalculate.h:111-151)
-					if (ec.tab.kind == 1)
+					if (SPL && ec.tab.kind == 1)
 					{
 						// the spline's value and derivative over r, carried
 						// through the interpolation below unchanged (rem 0)
@@ -808,7 +804,20 @@ __device__ __forceinline__ T eval_deriv(const MV<T>& m, const EvalCtx<T>& ec, Wa
 				rz[u] = xb[u].z - xa[u].z;
 				r2[u] = sqr(rx[u]) + sqr(ry[u]) + sqr(rz[u]);
 				// precalculate_linear_element::eval_deriv (src/lib/precalculate.h:111-151);
-				// the knot is clamped so that the row read stays in the table
+				// the knot is clamped so that the row read stays in the table.
+				// With the spline tables, the spline's value and derivative
+				// over r pass through the interpolation below unchanged.
+				if (SPL && ec.tab.kind == 1)
+				{
+					if (r2[u] < cutoff_sqr)
+						spline_deriv(ec.tab, prec_tri(rc[u].y), r2[u], e1[u], d1[u]);
+					else
+						e1[u] = d1[u] = T(0);
+					rem[u] = T(0);
+					e2[u] = e1[u];
+					d2[u] = d1[u];
+					continue;
+				}
 				const T r2f = factor * r2[u];
 				const int i1 = min(int(r2f), last_knot);
 				rem[u] = r2f - T(i1);

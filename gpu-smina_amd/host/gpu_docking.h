@@ -94,9 +94,9 @@ struct GpuSettings
 	int num_modes = 9;
 	double energy_range = 3.0;
 	int ligands_per_launch = 0; // 0: the whole batch in one launch
-	// --approximation: LINEAR (the docking search) or SPLINE (the default of
-	// --minimize, src/main/main.cpp:1225-1228; FP64 direct evaluation and
-	// refinement only); factor: points per A^2 (linear) or per A (spline)
+	// --approximation: LINEAR or SPLINE (the default of --minimize,
+	// src/main/main.cpp:1225-1228); factor: points per A^2 (linear) or per
+	// A (spline)
 	enum { LINEAR, SPLINE };
 	int approximation = LINEAR;
 	double approx_factor = 32;

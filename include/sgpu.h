@@ -211,10 +211,9 @@ sgpu_status sgpu_mc_capacity(sgpu_ctx* ctx, int precision, int* chains);
 sgpu_status sgpu_upload_tables(sgpu_ctx* ctx, const sgpu_tables* t);
 /* the spline approximation (--approximation spline, the default of
  * --minimize; DeviceTables::upload's spline branch, src/gpu/gpu_flatten.cpp:
- * 275-325): replaces the context's tables.  Carried by the FP64 direct
- * receptor path (evaluation and refinement, i.e. --minimize); the map
- * build, the Monte Carlo search and the rescoring need the linear tables
- * and refuse splines. */
+ * 275-325): replaces the context's tables for every later call (maps,
+ * evaluation, search, refinement, rescoring).  The splines are evaluated in
+ * FP64; the FP32 search rounds the results. */
 sgpu_status sgpu_upload_splines(sgpu_ctx* ctx, const sgpu_splines* s);
 
 /* DeviceReceptor::upload (src/gpu/gpu_device.cu:92-110) plus the 3 A cell

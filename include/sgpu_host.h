@@ -130,8 +130,8 @@ typedef struct sgpu_dock_settings
 	                         * step_fraction < 1), QuickVina 2's history check, early stopping while the
 	                         * device is full, two-pass screening of the best 10% */
 	int skip_visited;       /* --gpu_qvina: QuickVina 2's history check (sgpu_mc_params.visited) */
-	int approximation;      /* --approximation: 0 linear (docking), 1 spline (the default of --minimize,
-	                         * src/main/main.cpp:1225-1228; sgpu_minimize only) */
+	int approximation;      /* --approximation: 0 linear, 1 spline (the default of --minimize,
+	                         * src/main/main.cpp:1225-1228) */
 	double approx_factor;   /* --factor (0: 32 for linear, 10 for spline as --minimize sets it) */
 } sgpu_dock_settings;
 
