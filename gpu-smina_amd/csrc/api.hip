@@ -1369,7 +1369,8 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 			const int bytes = plan_bytes(k.plan, b.precision, w);
 			if (bytes > LDS_LIMIT)
 				break;
-			const int resident = w * mc_blocks_per_cu<T, S>(w, size_t(bytes), p->visited != 0 || c->tab.kind == 1);
+			const int variant = c->tab.kind == 1 ? 2 : p->visited ? 1 : 0;
+			const int resident = w * mc_blocks_per_cu<T, S>(w, size_t(bytes), variant);
 			if (resident >= best && resident > 0)
 			{
 				best = resident;

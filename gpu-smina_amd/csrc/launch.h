@@ -31,7 +31,9 @@ hipError_t launch_mc(const BatchPtrs<T>& B, const EvalCtx<T>& ec, const WavePlan
 // workgroups of `waves` chains with `lds` bytes of LDS that one CU holds at
 // once (the runtime's occupancy calculator: registers and LDS)
 template<typename T, typename S>
-int mc_blocks_per_cu(int waves, size_t lds, bool extra = false);
+// variant: 0 smina's search, 1 with QuickVina 2's history, 2 with the
+// spline tables (and the history)
+int mc_blocks_per_cu(int waves, size_t lds, int variant = 0);
 template<typename T, typename S>
 hipError_t launch_refine(const BatchPtrs<T>& B, const EvalCtx<T>& ec, const WavePlan& plan, int n, const int* model,
 		S* conf, int conf_stride, const int* maxiters, int accurate, int early_term, S* e, T* x, int x_stride,
