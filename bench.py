@@ -71,7 +71,7 @@ def serve(sg, args, log):
     types = list(range(2, 14))
     # each launch's search runs in 8 parts, and every part reports (a long
     # launch otherwise prints nothing until its jobs complete)
-    srv = sg.Server(workers=args.workers, max_batch=64, progress_chunks=8,
+    srv = sg.Server(workers=args.workers, max_batch=64, progress_chunks=8, fast=args.fast,
                     progress=lambda f: log(f"a serving launch: Monte Carlo {100 * f:.0f}% of steps"))
     t = time.perf_counter()
     ri = [srv.add_receptor(r, gd, types) for r in recs]
@@ -105,13 +105,14 @@ def serve(sg, args, log):
     wall = time.perf_counter() - t0
     lat = np.array([r["latency_s"] for r in res])
     line = {
-        "metric": "served docking jobs: ligands/s and p50/p99 job latency",
+        "metric": "served docking jobs: ligands/s and p50/p99 job latency" +
+                  (", --fast search (not smina's)" if args.fast else ""),
         "value": args.jobs / wall, "unit": "ligands/s", "n_gpus": 1, "higher_is_better": True,
         "dtype": "fp32", "data": "synthetic (seeded receptors and ligands)",
         "config": {"workload": "serving: 8 resident receptors (2,000-6,000 heavy atoms, 30 A boxes at 0.375 A, "
                                "12 FP32 maps each), Poisson arrivals of ligands with 8-80 heavy atoms, 0-20 "
                                "torsions, exhaustiveness from {4, 8, 16, 32}",
-                   "jobs": args.jobs, "rate_per_s": args.rate, "workers": args.workers},
+                   "jobs": args.jobs, "rate_per_s": args.rate, "workers": args.workers, "fast": bool(args.fast)},
         "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                       "max": float(lat.max())},
         "queued_s_p50": float(np.median([r["queued_s"] for r in res])),

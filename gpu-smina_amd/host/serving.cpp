@@ -242,6 +242,15 @@ sgpu_server* sgpu_server_create(const sgpu_server_settings* s)
 		ss.dock.progress_chunks = s->progress_chunks > 0 ? s->progress_chunks : 1;
 		ss.dock.progress = s->progress;
 		ss.dock.progress_user = s->progress_user;
+		if (s->fast)
+		{
+			// auto_split's choice when the batch leaves the device idle
+			// (src/gpu/gpu_docking.cpp:96-102), fixed: a served job's
+			// search must not depend on its batch
+			apply_fast(ss.dock);
+			ss.dock.chain_split = 32;
+			ss.dock.fast_top = 0;
+		}
 		sv->terms.resize(SGPU_MAX_TERMS);
 		sv->ci.resize(SGPU_MAX_CI);
 		sgpu_scoring sf;

@@ -128,7 +128,7 @@ class ServerSettings(C.Structure):
     _fields_ = [("device", C.c_int), ("workers", C.c_int), ("max_batch", C.c_int), ("budget_gb", C.c_double),
                 ("precision", C.c_int), ("seed", C.c_int), ("num_modes", C.c_int), ("energy_range", C.c_double),
                 ("hydrophobic_scale", C.c_double), ("scoring", C.c_char_p), ("progress_chunks", C.c_int),
-                ("progress", PROGRESS_FN), ("progress_user", C.c_void_p)]
+                ("progress", PROGRESS_FN), ("progress_user", C.c_void_p), ("fast", C.c_int)]
 
 
 class ServeResult(C.Structure):
@@ -688,13 +688,13 @@ class Server:
 
     def __init__(self, workers=2, max_batch=64, budget_gb=64.0, fp64=False, seed=1, num_modes=9,
                  energy_range=3.0, hydrophobic_scale=1.0, device=0, precision=None, scoring="vina", progress=None,
-                 progress_chunks=1):
+                 progress_chunks=1, fast=False):
         """progress(fraction): after each of progress_chunks parts of every
         launch's Monte Carlo search, on the worker's thread"""
         prec = precision if precision is not None else (FP64 if fp64 else FP32)
         self._progress = PROGRESS_FN(lambda user, frac: progress(frac)) if progress else PROGRESS_FN()
         s = ServerSettings(device, workers, max_batch, budget_gb, prec, seed, num_modes, energy_range,
-                           hydrophobic_scale, scoring.encode(), progress_chunks, self._progress, None)
+                           hydrophobic_scale, scoring.encode(), progress_chunks, self._progress, None, int(fast))
         self.h = C.c_void_p(lib.sgpu_server_create(C.byref(s)))
         err = lib.sgpu_server_error(self.h)
         if err:

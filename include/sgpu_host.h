@@ -197,6 +197,11 @@ typedef struct sgpu_server_settings
 	int progress_chunks;   /* each launch's Monte Carlo search in this many parts (0 or 1: one) */
 	sgpu_progress_fn progress; /* called after each part, on the worker's thread */
 	void* progress_user;
+	int fast;              /* the --fast search per job, as auto_split and auto_fraction choose it for a
+	                        * device the batch does not fill: 32 chains per exhaustiveness unit, each
+	                        * with 1/32 of the steps, and QuickVina 2's history (no early stopping at
+	                        * that split).  The split is fixed so that a job's search does not depend
+	                        * on the jobs it is batched with; no two-pass screening. */
 } sgpu_server_settings;
 typedef struct sgpu_serve_result
 {

@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 import smina_gpu as sg
+from common import gd_c0, receptor
 
 pytestmark = pytest.mark.gpu
 
@@ -46,3 +47,23 @@ def test_server_matches_docking_facade():
     d.prepare([lig])
     ref = d.dock([lig])[0]
     assert r["e"] == ref["e"] and r["evals"] == ref["evals"]
+
+
+@pytest.mark.gpu
+def test_fast_serving_does_not_depend_on_the_batch():
+    """--fast per job in the server (history check, a fixed split of 32): a
+    job's modes are the same docked alone or beside others"""
+    rec = receptor()
+    gd = gd_c0()
+    ligs = [sg.Mol.ligand(60 + k, 14 + 4 * k, 1 + k) for k in range(3)]
+    types = list(range(2, 14))
+
+    def run(batch):
+        srv = sg.Server(workers=1, max_batch=8, fast=True)
+        r = srv.add_receptor(rec, gd, types)
+        jobs = [srv.submit(r, l, 4) for l in batch]
+        return [srv.wait(j)["e"] for j in jobs]
+    alone = run(ligs[:1])[0]
+    mixed = run(ligs)[0]
+    print("fast serving", alone[:3], mixed[:3], flush=True)
+    assert len(alone) > 0 and alone == mixed
