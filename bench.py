@@ -234,6 +234,8 @@ def main():
     ap.add_argument("--fast", action="store_true",
                     help="the screen with --fast's GPU-style search (automatic chain split and step fraction, early "
                          "stopping, two-pass screening), compared with the default search on the same ligands")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="with --fast: skip the default-search run that the fast screen is compared with")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only a 3 x 1 GiB device copy (the reference for HBM byte counters)")
     args = ap.parse_args()
@@ -332,7 +334,7 @@ def main():
     traffic = None if args.flex or args.fast or prec != sg.FP32 else measured_traffic(len(ligs), args.exhaustiveness)
     fast = None
     if args.fast:
-        fast = fast_vs_full(sg, args, log, rec, ligs, gd, hyd, prec, chunks, top)
+        fast = {} if args.no_compare else fast_vs_full(sg, args, log, rec, ligs, gd, hyd, prec, chunks, top)
         fast["full_searches"] = dock.full_searches()
     log(f"{'flex' if args.flex else 'screen'}: {value:.2f} ligands/s, MC kernel {mc_ms / 1e3:.2f}s, "
         f"{evals.sum():.3e} evals")
@@ -416,7 +418,10 @@ def fast_vs_full(sg, args, log, rec, ligs, gd, hyd, prec, chunks, top_fast):
     """the --fast screen's top scores against the default search's on the
     same ligands and seed (one untimed run): per-ligand differences, and how
     many of the default search's best 10% the fast screen also ranks there"""
-    full = sg.Docking(rec, gd, precision=prec, exhaustiveness=args.exhaustiveness, seed=1, hydrophobic_scale=hyd)
+    # a long default search reports its progress (8 parts)
+    full = sg.Docking(rec, gd, precision=prec, exhaustiveness=args.exhaustiveness, seed=1, hydrophobic_scale=hyd,
+                      progress=lambda f: log(f"default search: Monte Carlo {100 * f:.0f}% of steps"),
+                      progress_chunks=8)
     full.prepare(ligs)
     t = time.perf_counter()
     out = full.dock(ligs)
