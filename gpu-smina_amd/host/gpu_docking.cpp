@@ -817,26 +817,33 @@ void apply_fast(GpuSettings& s)
 
 // auto_split (src/gpu/gpu_docking.cpp:87-102): the largest power of two from
 // 4 to 32 that keeps tasks x split within 4 times the chains the device holds
-int GpuDocking::split_for(size_t tasks) const
+// with fraction_for and auto_fraction (src/gpu/gpu_docking.cpp:104-112,
+// :167-171): a first-pass budget, else the set fraction, else half the steps
+// for a split in [4, 32) and all of them otherwise; and stops / stop_for
+// (:172-189): no early stopping in a first pass, EARLY_STOP_FULL only below
+// split 32, checkpoints after every tenth of smina's steps the chains run
+// (at least first + 1), the rule from the second on
+SearchPlan plan_search(const GpuSettings& s, size_t tasks, int capacity, double budget)
 {
-	if (s_.chain_split > 0)
-		return s_.chain_split;
-	int split = 32;
-	while (split > 4 && tasks * size_t(split) > 4 * size_t(capacity_))
-		split /= 2;
-	return split;
-}
-
-// fraction_for and auto_fraction (src/gpu/gpu_docking.cpp:104-112, :167-171):
-// a first-pass budget, else the set fraction, else half the steps for a split
-// in [4, 32) and all of them otherwise
-double GpuDocking::fraction_for(int split, double budget) const
-{
-	if (budget > 0)
-		return budget;
-	if (s_.step_fraction > 0)
-		return s_.step_fraction;
-	return split >= 4 && split < 32 ? 0.5 : 1.0;
+	SearchPlan p;
+	if (s.chain_split > 0)
+		p.split = s.chain_split;
+	else
+	{
+		p.split = 32;
+		while (p.split > 4 && tasks * size_t(p.split) > 4 * size_t(std::max(capacity, 1)))
+			p.split /= 2;
+	}
+	p.fraction = budget > 0 ? budget : s.step_fraction > 0 ? s.step_fraction
+	                                                        : (p.split >= 4 && p.split < 32 ? 0.5 : 1.0);
+	const bool stops = !(budget > 0) && (s.early_stop == GpuSettings::EARLY_STOP_ALWAYS ||
+	                                     (s.early_stop == GpuSettings::EARLY_STOP_FULL && p.split < 32));
+	if (stops)
+	{
+		p.first = 2;
+		p.checkpoints = std::max(p.first + 1, int(std::lround(10 * p.fraction)));
+	}
+	return p;
 }
 
 void GpuDocking::dock_pass(std::vector<DockJob*>& jobs, double budget)
@@ -1058,8 +1065,9 @@ void GpuDocking::dock_sub(DockJob* const* jobs, size_t nj, double budget)
 	size_t tasks = 0;
 	for (size_t j = 0; j < nj; j++)
 		tasks += size_t(jobs[j]->exhaustiveness > 0 ? jobs[j]->exhaustiveness : s_.exhaustiveness);
-	const int split = split_for(tasks);
-	const double fraction = fraction_for(split, budget);
+	const SearchPlan plan = plan_search(s_, tasks, capacity_, budget);
+	const int split = plan.split;
+	const double fraction = plan.fraction;
 	for (size_t j = 0; j < nj; j++)
 	{
 		const int exh = jobs[j]->exhaustiveness > 0 ? jobs[j]->exhaustiveness : s_.exhaustiveness;
@@ -1097,15 +1105,10 @@ void GpuDocking::dock_sub(DockJob* const* jobs, size_t nj, double budget)
 	}
 	mp.cache_slope = 1e6;
 	mp.visited = s_.skip_visited ? 1 : 0;
-	// stops and stop_for (src/gpu/gpu_docking.cpp:172-189): not in a
-	// first pass; checkpoints after every tenth of smina's steps the chains
-	// run, the rule from the second on
-	const bool stops = !(budget > 0) && (s_.early_stop == GpuSettings::EARLY_STOP_ALWAYS ||
-	                                     (s_.early_stop == GpuSettings::EARLY_STOP_FULL && split < 32));
-	if (stops)
+	if (plan.checkpoints > 0)
 	{
-		mp.early_stop_first = 2;
-		mp.early_stop_checkpoints = std::max(mp.early_stop_first + 1, int(std::lround(10 * fraction)));
+		mp.early_stop_first = plan.first;
+		mp.early_stop_checkpoints = plan.checkpoints;
 		mp.early_stop_fraction = 0.5;
 		mp.early_stop_eps = 0.05;
 	}
@@ -1466,6 +1469,23 @@ sgpu_docking* sgpu_docking_create(const sgpu_dock_settings* s, const sgpu_mol* r
 }
 void sgpu_docking_free(sgpu_docking* d) { delete d; }
 int sgpu_docking_full_searches(const sgpu_docking* d) { return d->d ? int(d->d->full_searches()) : 0; }
+
+int sgpu_search_plan(int chain_split, double step_fraction, int early_stop, int tasks, int capacity, double budget,
+		int* split, double* fraction, int* checkpoints)
+{
+	if (chain_split < 0 || step_fraction < 0 || early_stop < 0 || early_stop > 2 || tasks < 0 || !split ||
+			!fraction || !checkpoints)
+		return -1;
+	GpuSettings s;
+	s.chain_split = chain_split;
+	s.step_fraction = step_fraction;
+	s.early_stop = early_stop;
+	const SearchPlan p = plan_search(s, size_t(tasks), capacity, budget);
+	*split = p.split;
+	*fraction = p.fraction;
+	*checkpoints = p.checkpoints;
+	return 0;
+}
 const char* sgpu_docking_error(const sgpu_docking* d) { return d->err.c_str(); }
 sgpu_ctx* sgpu_docking_ctx(sgpu_docking* d) { return d->d ? d->d->ctx() : nullptr; }
 int sgpu_docking_receptor_id(sgpu_docking* d) { return d->d ? d->d->receptor_id() : -1; }

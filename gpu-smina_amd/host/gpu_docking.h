@@ -132,6 +132,19 @@ struct GpuSettings
 // stopping while the device is full, two-pass screening of the best 10%.
 void apply_fast(GpuSettings& s);
 
+// the GPU-style search of a launch of `tasks` exhaustiveness units on a
+// device holding `capacity` chains: chains per unit, fraction of smina's
+// steps, early-stopping checkpoints (0: none; the rule from `first` on).
+// budget > 0: a first pass at that fraction, without early stopping.
+struct SearchPlan
+{
+	int split = 1;
+	double fraction = 1;
+	int checkpoints = 0;
+	int first = 2;
+};
+SearchPlan plan_search(const GpuSettings& s, size_t tasks, int capacity, double budget);
+
 class GpuDocking
 {
 public:
@@ -194,8 +207,6 @@ private:
 	// (the first pass of a two-pass screen)
 	void dock_sub(DockJob* const* jobs, size_t n, double budget);
 	void dock_pass(std::vector<DockJob*>& jobs, double budget);
-	int split_for(size_t tasks) const;
-	double fraction_for(int split, double budget) const;
 	GpuSettings s_;
 	int capacity_ = 1;                // chains the device holds at once
 	std::vector<double> top_history_; // two-pass: first-pass top scores so far

@@ -207,6 +207,8 @@ def _load():
         "sgpu_docking_ctx": (vp, [vp]),
         "sgpu_docking_receptor_id": (C.c_int, [vp]),
         "sgpu_docking_full_searches": (C.c_int, [vp]),
+        "sgpu_search_plan": (C.c_int, [C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_double, I32P, F64P,
+                                       I32P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -343,6 +345,16 @@ def build_tables(sf, factor=32.0):
     fast = np.zeros(378 * n)
     lib.sgpu_build_tables(C.byref(sf), factor, ptr(smooth, F64P), ptr(fast, F64P))
     return n, smooth, fast
+
+
+def search_plan(tasks, capacity, chain_split=0, step_fraction=0.0, early_stop=1, budget=0.0):
+    """the GPU-style search plan (sgpu_search_plan): (split, fraction,
+    early-stopping checkpoints)"""
+    split, frac, ck = C.c_int(), C.c_double(), C.c_int()
+    if lib.sgpu_search_plan(chain_split, step_fraction, early_stop, tasks, capacity, budget, C.byref(split),
+                            C.byref(frac), C.byref(ck)) != 0:
+        raise ValueError("bad search plan request")
+    return split.value, frac.value, ck.value
 
 
 def build_splines(sf, factor=10.0):

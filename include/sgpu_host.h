@@ -175,6 +175,15 @@ int sgpu_minimize(sgpu_docking* d, const sgpu_mol* const* ligands, int n, int ma
 /* ligands of the last sgpu_dock that got the full search (all of them
  * unless a two-pass screen ran) */
 int sgpu_docking_full_searches(const sgpu_docking* d);
+/* the GPU-style search plan of a launch (auto_split, fraction_for, stops
+ * and stop_for; src/gpu/gpu_docking.cpp:87-112, :167-193): chain_split 0 and
+ * step_fraction 0 are automatic; early_stop 0 off, 1 while the device is
+ * full (split below 32), 2 always; tasks = exhaustiveness units of the
+ * launch; capacity = chains the device holds (sgpu_mc_capacity); budget > 0
+ * for a two-pass first pass.  Out: chains per unit, step fraction,
+ * early-stopping checkpoints (0: none).  Returns 0, or -1 for bad input. */
+int sgpu_search_plan(int chain_split, double step_fraction, int early_stop, int tasks, int capacity, double budget,
+		int* split, double* fraction, int* checkpoints);
 
 /* low-level access for tests: the context and the device ids of the
  * receptor, grids and tables this docking object built */

@@ -133,3 +133,27 @@ def test_context_without_device_fails_cleanly():
     with pytest.raises(sg.SgpuError) as e:
         sg.Context(0)
     assert e.value.status == 5
+
+
+@pytest.mark.parametrize("tasks,kw,expect", [
+    # --fast (automatic split and fraction, early stopping while the device
+    # is full) on a device holding 2,048 chains: the largest power of two
+    # from 4 to 32 with tasks x split <= 4 x 2,048
+    (8, {}, (32, 1.0, 0)),          # one ligand: the device is idle, full steps, no early stop
+    (256, {}, (32, 1.0, 0)),        # 256 x 32 = 8,192: still 32
+    (257, {}, (16, 0.5, 5)),        # one more: 16, half the steps, checkpoints round(10 x 0.5)
+    (1024, {}, (8, 0.5, 5)),
+    (8192, {}, (4, 0.5, 5)),        # the 1,024-ligand screen: the floor of 4
+    (8192, dict(budget=0.0625), (4, 0.0625, 0)),  # a two-pass first pass: its budget, never stopped early
+    # explicit settings (--gpu_split, --gpu_budget, --gpu_early_stop)
+    (8, dict(chain_split=2), (2, 1.0, 10)),            # below 4: full steps; 10 checkpoints
+    (8, dict(chain_split=1, step_fraction=1.0, early_stop=0), (1, 1.0, 0)),  # smina's search
+    (8, dict(chain_split=1, step_fraction=0.3, early_stop=2), (1, 0.3, 3)),  # at least first + 1 = 3
+    (8, dict(chain_split=32, early_stop=1), (32, 1.0, 0)),  # while the device is full only
+    (8, dict(chain_split=32, early_stop=2), (32, 1.0, 10)),
+])
+def test_search_plan_known_answers(tasks, kw, expect):
+    """the GPU-style search plan against the rules of auto_split,
+    auto_fraction, stops and stop_for (src/gpu/gpu_docking.cpp:87-112,
+    :167-193), worked out by hand"""
+    assert sg.search_plan(tasks, 2048, **kw) == expect
