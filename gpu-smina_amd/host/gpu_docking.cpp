@@ -853,6 +853,17 @@ void GpuDocking::dock_pass(std::vector<DockJob*>& jobs, double budget)
 		dock_sub(jobs.data() + b, std::min(per, jobs.size() - b), budget);
 }
 
+// src/main/main.cpp:1578-1586: the score of the history's best fast_top
+// (the ceil(fast_top x n)-th lowest) plus the margin
+double full_search_cutoff(std::vector<double> history, double fast_top, double margin)
+{
+	if (history.empty())
+		throw std::invalid_argument("no first-pass scores");
+	const size_t q = std::max<size_t>(1, size_t(std::ceil(fast_top * double(history.size())))) - 1;
+	std::nth_element(history.begin(), history.begin() + ptrdiff_t(q), history.end());
+	return history[q] + margin;
+}
+
 // the batch flush of src/main/main.cpp:1551-1597: one pass, or with fast_top
 // in (0, 1) and at least 20 ligands seen, a first pass at fast_first of the
 // steps whose top scores join the history, then the full search for the
@@ -872,10 +883,7 @@ void GpuDocking::dock_batch(std::vector<DockJob*>& jobs)
 	for (size_t k = 0; k < jobs.size(); k++)
 		score[k] = jobs[k]->out.empty() ? kMax : jobs[k]->out[0].e;
 	top_history_.insert(top_history_.end(), score.begin(), score.end());
-	std::vector<double> sorted(top_history_);
-	const size_t q = std::max<size_t>(1, size_t(std::ceil(s_.fast_top * double(sorted.size())))) - 1;
-	std::nth_element(sorted.begin(), sorted.begin() + ptrdiff_t(q), sorted.end());
-	const double cutoff = sorted[q] + s_.fast_top_margin;
+	const double cutoff = full_search_cutoff(top_history_, s_.fast_top, s_.fast_top_margin);
 	std::vector<DockJob*> again;
 	std::vector<int64_t> first_evals;
 	for (size_t k = 0; k < jobs.size(); k++)
@@ -1469,6 +1477,13 @@ sgpu_docking* sgpu_docking_create(const sgpu_dock_settings* s, const sgpu_mol* r
 }
 void sgpu_docking_free(sgpu_docking* d) { delete d; }
 int sgpu_docking_full_searches(const sgpu_docking* d) { return d->d ? int(d->d->full_searches()) : 0; }
+
+double sgpu_full_search_cutoff(const double* history, int n, double fast_top, double margin)
+{
+	if (!history || n <= 0 || !(fast_top > 0) || fast_top > 1)
+		return std::numeric_limits<double>::quiet_NaN();
+	return full_search_cutoff(std::vector<double>(history, history + n), fast_top, margin);
+}
 
 int sgpu_search_plan(int chain_split, double step_fraction, int early_stop, int tasks, int capacity, double budget,
 		int* split, double* fraction, int* checkpoints)

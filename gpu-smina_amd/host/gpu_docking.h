@@ -144,6 +144,9 @@ struct SearchPlan
 	int first = 2;
 };
 SearchPlan plan_search(const GpuSettings& s, size_t tasks, int capacity, double budget);
+// two-pass screening: ligands whose first-pass score is at most this get the
+// full search (the best fast_top of all first-pass scores so far, plus margin)
+double full_search_cutoff(std::vector<double> history, double fast_top, double margin);
 
 class GpuDocking
 {

@@ -207,6 +207,7 @@ def _load():
         "sgpu_docking_ctx": (vp, [vp]),
         "sgpu_docking_receptor_id": (C.c_int, [vp]),
         "sgpu_docking_full_searches": (C.c_int, [vp]),
+        "sgpu_full_search_cutoff": (C.c_double, [F64P, C.c_int, C.c_double, C.c_double]),
         "sgpu_search_plan": (C.c_int, [C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_double, I32P, F64P,
                                        I32P]),
     }
@@ -355,6 +356,12 @@ def search_plan(tasks, capacity, chain_split=0, step_fraction=0.0, early_stop=1,
                             C.byref(frac), C.byref(ck)) != 0:
         raise ValueError("bad search plan request")
     return split.value, frac.value, ck.value
+
+
+def full_search_cutoff(history, fast_top=0.1, margin=0.3):
+    """two-pass screening's cutoff (sgpu_full_search_cutoff)"""
+    h = f64(history)
+    return lib.sgpu_full_search_cutoff(ptr(h, F64P), len(h), fast_top, margin)
 
 
 def build_splines(sf, factor=10.0):

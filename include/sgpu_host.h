@@ -184,6 +184,11 @@ int sgpu_docking_full_searches(const sgpu_docking* d);
  * early-stopping checkpoints (0: none).  Returns 0, or -1 for bad input. */
 int sgpu_search_plan(int chain_split, double step_fraction, int early_stop, int tasks, int capacity, double budget,
 		int* split, double* fraction, int* checkpoints);
+/* two-pass screening (src/main/main.cpp:1578-1586): the first-pass score at
+ * or below which a ligand gets the full search, from the n first-pass top
+ * scores so far: the ceil(fast_top x n)-th lowest plus margin (NaN for bad
+ * input) */
+double sgpu_full_search_cutoff(const double* history, int n, double fast_top, double margin);
 
 /* low-level access for tests: the context and the device ids of the
  * receptor, grids and tables this docking object built */

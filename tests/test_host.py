@@ -157,3 +157,19 @@ def test_search_plan_known_answers(tasks, kw, expect):
     auto_fraction, stops and stop_for (src/gpu/gpu_docking.cpp:87-112,
     :167-193), worked out by hand"""
     assert sg.search_plan(tasks, 2048, **kw) == expect
+
+
+def test_two_pass_cutoff_known_answers():
+    """two-pass screening (src/main/main.cpp:1578-1586): the full search goes
+    to ligands within the margin of the ceil(fast_top x n)-th lowest
+    first-pass score"""
+    scores = [-5.0 - 0.1 * k for k in range(30)]  # -5.0 .. -7.9
+    # n = 30, fast_top 0.1: the 3rd lowest is -7.7; + 0.3
+    assert sg.full_search_cutoff(scores) == pytest.approx(-7.4)
+    # n = 31 after one more: ceil(3.1) = 4th lowest, -7.6
+    assert sg.full_search_cutoff(scores + [-1.0]) == pytest.approx(-7.3)
+    # fast_top 1: everything (the highest plus the margin)
+    assert sg.full_search_cutoff(scores, 1.0, 0.0) == pytest.approx(-5.0)
+    # a single score: itself plus the margin
+    assert sg.full_search_cutoff([-6.0], 0.1, 0.3) == pytest.approx(-5.7)
+    assert np.isnan(sg.full_search_cutoff([], 0.1, 0.3))
