@@ -307,3 +307,18 @@ def test_flexible_residues_read_back():
         diffs.append(o.eval(ids[0], c, slope=10.0) - o.eval(ids[1], c, slope=10.0))
     print("flex energy differences", diffs)
     assert max(diffs) - min(diffs) < 0.1
+
+
+def test_receptor_and_writer_errors():
+    """a multi-MODEL receptor and an unknown type are refused with the line;
+    poses without rmsd say -1; a model that was not read from PDBQT cannot
+    be written back"""
+    with pytest.raises(ValueError, match="line 1: unexpected multi-MODEL"):
+        sg.Mol.pdbqt_receptor("MODEL 1\n" + atom(1, "N", 0, 0, 0, "N") + "\n")
+    with pytest.raises(ValueError, match="line 2:.*AutoDock type"):
+        sg.Mol.pdbqt_receptor(atom(1, "N", 0, 0, 0, "N") + "\n" + atom(2, "Q", 1, 0, 0, "Qq") + "\n")
+    m = sg.Mol.pdbqt_ligand(CHAIN)
+    c0 = sg.Flat(m).initial_conf()
+    assert "REMARK minimizedRMSD -1\n" in m.write_pdbqt(c0[None], [-5.0])
+    with pytest.raises(ValueError):
+        sg.Mol.ligand(1, 24, 6).write_pdbqt(c0[None], [-5.0])
