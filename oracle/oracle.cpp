@@ -2516,6 +2516,7 @@ int orc_dock(orc_ctx* c, int model, const sgpu_mc_params* p, int seed, int exhau
 	std::vector<std::string> errs(tasks);
 	vec c1(c->c.gd.begin[0], c->c.gd.begin[1], c->c.gd.begin[2]);
 	vec c2(c->c.gd.end[0], c->c.gd.end[1], c->c.gd.end[2]);
+	int steps_now = num_steps;
 	auto run_task = [&](int t) {
 		try
 		{
@@ -2524,34 +2525,76 @@ int orc_dock(orc_ctx* c, int model, const sgpu_mc_params* p, int seed, int exhau
 			rng gen(uint32_t(seeds[t]));
 			std::vector<double> start(m.conf_size, 0.0);
 			randomize_conf(m, start.data(), c1, c2, gen);
-			mc_chain(c->c, m, *p, num_steps, maxiters, start, gen, task_out[t], &task_evals[t]);
+			task_out[t].clear();
+			task_evals[t] = 0;
+			mc_chain(c->c, m, *p, steps_now, maxiters, start, gen, task_out[t], &task_evals[t]);
 		}
 		catch (std::exception& ex)
 		{
 			errs[t] = ex.what();
 		}
 	};
-	if (threads <= 1)
-		for (int t = 0; t < tasks; t++)
-			run_task(t);
-	else
-	{
-		std::vector<std::thread> pool;
-		std::atomic<int> next(0);
-		for (int w = 0; w < threads; w++)
-			pool.emplace_back([&]() {
-				for (int t = next++; t < tasks; t = next++)
-					run_task(t);
-			});
-		for (auto& th : pool)
-			th.join();
-	}
-	for (auto& s : errs)
-		if (!s.empty())
+	auto run_all = [&](int steps) {
+		steps_now = steps;
+		if (threads <= 1)
+			for (int t = 0; t < tasks; t++)
+				run_task(t);
+		else
 		{
-			c->c.err = s;
-			return -1;
+			std::vector<std::thread> pool;
+			std::atomic<int> next(0);
+			for (int w = 0; w < threads; w++)
+				pool.emplace_back([&]() {
+					for (int t = next++; t < tasks; t = next++)
+						run_task(t);
+				});
+			for (auto& th : pool)
+				th.join();
 		}
+		for (auto& s : errs)
+			if (!s.empty())
+				throw std::runtime_error(s);
+	};
+	try
+	{
+		// early stopping (the GPU-style --gpu_early_stop; EarlyStop and
+		// stop_agreeing, src/gpu/gpu_kernels_impl.cuh:607-657, :742-808):
+		// at checkpoint k of n every chain has run ceil(steps x k / n) steps
+		// (a chain cut there is the prefix of the longer one); the ligand
+		// stops at the first checkpoint from `first` on, before the last,
+		// where at least max(2, ceil(fraction x units)) of its units
+		// (`split` consecutive chains each) have best energies within eps of
+		// the lowest
+		int stop_at = 0;
+		const int n = p->early_stop_checkpoints;
+		if (n > 0)
+		{
+			const int first = std::min(n, std::max(1, p->early_stop_first));
+			for (int k = first; k < n && !stop_at; k++)
+			{
+				run_all(int((int64_t(num_steps) * k + n - 1) / n));
+				std::vector<fl> ub(size_t(exhaustiveness), max_fl);
+				for (int t = 0; t < tasks; t++)
+					for (const Output& o : task_out[t])
+						ub[size_t(t / split)] = std::min(ub[size_t(t / split)], o.e);
+				if (ub.size() < 2)
+					continue;
+				const fl best = *std::min_element(ub.begin(), ub.end());
+				int agree = 0;
+				for (fl u : ub)
+					agree += (u - best <= p->early_stop_eps);
+				if (agree >= std::max(2, int(std::ceil(p->early_stop_fraction * double(ub.size()) - 1e-9))))
+					stop_at = k;
+			}
+		}
+		if (!stop_at)
+			run_all(num_steps);
+	}
+	catch (std::exception& ex)
+	{
+		c->c.err = ex.what();
+		return -1;
+	}
 	// merge_output_containers with min_rmsd 2 (src/lib/parallel_mc.cpp:71-78)
 	std::vector<Output> out;
 	for (int t = 0; t < tasks; t++)

@@ -107,7 +107,8 @@ double orc_refine(orc_ctx* c, int model, const double* v, int maxiters, double* 
 double orc_refine_opts(orc_ctx* c, int model, const double* v, int maxiters, int accurate, double* conf,
 		double* coords);
 
-/* the whole docking of one ligand: parallel_mc (src/lib/parallel_mc.cpp:80-98)
+/* the whole docking of one ligand (with early stopping when
+ * p->early_stop_checkpoints > 0, units of `split` chains): parallel_mc (src/lib/parallel_mc.cpp:80-98)
  * + refine + finish_docking (src/main/main.cpp:256-324), single-threaded or on
  * `threads` host threads.  With step_fraction < 1 or split > 1 (the GPU-style
  * search, src/gpu/gpu_docking.cpp:200-220): exhaustiveness x split chains,

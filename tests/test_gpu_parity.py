@@ -465,6 +465,24 @@ def test_dock_fp64_matches_oracle(lig, exh, frac, split, qv):
     assert len(r["e"]) > 0 and r["e"] == list(e)
 
 
+@pytest.mark.parametrize("lig,exh", [((1, 24, 6), 4), ((3, 10, 0), 6)])
+def test_dock_fp64_early_stop_matches_oracle(lig, exh):
+    """--gpu_early_stop with --gpu_split 2 and a tenth of the steps through
+    the host facade (3 checkpoints, the rule from the second; units of 2
+    chains) in FP64, against the oracle's docking with the same rule"""
+    rec, l = receptor(), sg.Mol.ligand(*lig)
+    d = sg.Docking(rec, gd_c0(), fp64=True, exhaustiveness=exh, seed=5, step_fraction=0.1, chain_split=2,
+                   early_stop=True)
+    r = d.dock([l])[0]
+    o, ids, _, _ = oracle_for([l])
+    p = sg.mc_params()
+    p.early_stop_checkpoints, p.early_stop_first, p.early_stop_fraction, p.early_stop_eps = 3, 2, 0.5, 0.05
+    e, c, ev = o.dock(ids[0], p, seed=5, exhaustiveness=exh, threads=8, step_fraction=0.1, split=2)
+    full = o.dock(ids[0], sg.mc_params(), seed=5, exhaustiveness=exh, threads=8, step_fraction=0.1, split=2)[2]
+    print(f"early stop {lig}: {r['e'][:3]}, oracle evals {ev} of {full} without stopping", flush=True)
+    assert len(r["e"]) > 0 and r["e"] == list(e)
+
+
 @pytest.mark.parametrize("prec", [sg.MIXED, sg.FP32])
 def test_search_energies_match_fp64_oracle(env, prec):
     """the served searches: FP32 throughout, and mixed (FP32 evaluation,
