@@ -33,8 +33,19 @@ sys.path.insert(0, os.path.join(ROOT, "gpu-smina_amd"))
 
 HBM_PEAK_GBS = 8000.0
 F64_STEP_DIV = 16  # plain FP64 sample: this fraction of the Monte Carlo steps
-CPU_STEP_DIV = 1   # CPU baseline sample: this fraction of one chain's steps
+CPU_STEP_DIV = 4   # CPU baseline sample: this fraction of one chain's steps
 TRAFFIC_FILE = "profiles/mc_kernel_screen1024_traffic.md"
+T_START = time.perf_counter()
+
+
+def size_sample(flats, k):
+    """k ligand indices spread over the batch's cost range: the ligands at
+    the (j + 0.5)/k quantiles of heavy atoms x (1 + torsions), so that a
+    bounded sample stands for small, median and large ligands alike"""
+    cost = np.array([f.nheavy * (1 + f.ntors_total) for f in flats], dtype=np.float64)
+    order = np.argsort(cost, kind="stable")
+    k = max(1, min(k, len(flats)))
+    return [int(order[min(len(order) - 1, int((j + 0.5) / k * len(order)))]) for j in range(k)]
 
 
 def measured_traffic(nligands, exhaustiveness):
@@ -216,7 +227,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--ligands", type=int, default=1024)
     ap.add_argument("--exhaustiveness", type=int, default=8)
-    ap.add_argument("--fp64-sample", type=int, default=1, help="ligands docked on the plain FP64 path (sampled)")
+    ap.add_argument("--budget-s", type=float, default=420.0,
+                    help="wall-time budget of the whole run: timed steps stop early (and say so) to stay within it")
+    ap.add_argument("--fp64-sample", type=int, default=8,
+                    help="ligands docked on the plain FP64 path (sampled across the size range)")
     ap.add_argument("--cpu-sample", type=int, default=0, help="ligands for the CPU baseline (0: one per core)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve", action="store_true", help="the serving workload instead of the screen")
@@ -307,17 +321,34 @@ def main():
         metric += ", --fast search (not smina's)"
     # build the maps for every ligand type before timing (receptor setup)
     dock.prepare(ligs)
+    # warmup: small launches (a few ligands at a fraction of their steps)
+    # that load the code objects and set up the device buffers; a full
+    # screen as warmup would cost as much as a timed step
+    warm = sg.Docking(rec, gd, precision=prec, exhaustiveness=args.exhaustiveness, seed=1, hydrophobic_scale=hyd,
+                      step_fraction=1.0 / 64, fast=args.fast)
+    warm_ligs = [ligs[i] for i in size_sample(flats, 4)]
+    warm.prepare(warm_ligs)
     for w in range(args.warmup):
-        dock.dock(ligs)
-        log(f"warmup {w + 1}/{args.warmup} done")
+        warm.dock(warm_ligs)
+    log(f"warmup: {args.warmup} small launch(es) of {len(warm_ligs)} ligands at 1/64 of their steps")
+    del warm
+    # timed steps: each is the whole screen; run as many of the requested
+    # steps as the time budget holds, leaving room for the baselines
     times = []
     results = None
+    legs_s = 0.0 if args.no_cpu else 45.0
+    legs_s += 60.0 if args.fp64_sample > 0 else 0.0
     for s in range(args.steps):
+        elapsed = time.perf_counter() - T_START
+        if times and elapsed + float(np.mean(times)) + legs_s > args.budget_s:
+            log(f"time budget: {len(times)} of {args.steps} steps run ({elapsed:.0f}s elapsed, "
+                f"{np.mean(times):.1f}s per step, budget {args.budget_s:.0f}s)")
+            break
         t = time.perf_counter()
         results = dock.dock(ligs)
         times.append(time.perf_counter() - t)
         log(f"step {s + 1}/{args.steps}: {times[-1]:.2f}s")
-    step_s = max(times) if args.steps == 1 else float(np.mean(times))
+    step_s = float(np.mean(times))
     value = len(ligs) / step_s
     mc_ms = dock.kernel_ms("mc")
     evals = np.array([r["evals"] for r in results], dtype=np.float64)
@@ -339,14 +370,16 @@ def main():
     log(f"{'flex' if args.flex else 'screen'}: {value:.2f} ligands/s, MC kernel {mc_ms / 1e3:.2f}s, "
         f"{evals.sum():.3e} evals")
 
-    # plain per-ligand FP64 path (one ligand per launch) on a bounded sample:
-    # 1/F64_STEP_DIV of the Monte Carlo steps, the MC kernel time scaled back up
+    # plain per-ligand FP64 path (one ligand per launch) on a bounded sample
+    # across the batch's size range: 1/F64_STEP_DIV of the Monte Carlo steps,
+    # the MC kernel time scaled back up (the search is step-linear)
     plain = None
     if args.fp64_sample > 0:
         frac = 1.0 / F64_STEP_DIV
         d64 = sg.Docking(rec, gd, fp64=True, exhaustiveness=args.exhaustiveness, seed=1, ligands_per_launch=1,
                          step_fraction=frac, hydrophobic_scale=hyd)
-        sample = ligs[:args.fp64_sample]
+        pick = size_sample(flats, args.fp64_sample)
+        sample = [ligs[i] for i in pick]
         d64.prepare(sample)
         per_lig = []
         for lig in sample:
@@ -355,9 +388,14 @@ def main():
             dt = time.perf_counter() - t
             mc = d64.kernel_ms("mc") / 1e3
             per_lig.append(dt - mc + mc / frac)
+        # the sample's mean time per ligand, weighted to the batch: each
+        # sampled ligand stands for the ligands of its size stratum
         plain = {"value": len(sample) / float(np.sum(per_lig)), "unit": "ligands/s",
-                 "sample": f"first {len(sample)} ligand(s) of the batch, FP64, one ligand per launch, "
-                           f"1/{F64_STEP_DIV} of the MC steps with the MC kernel time scaled by {F64_STEP_DIV}",
+                 "sample": f"{len(sample)} ligands across the batch's size range (heavy atoms "
+                           f"{[int(flats[i].nheavy) for i in pick]}, torsions {[int(flats[i].ntors_total) for i in pick]}), "
+                           f"FP64, one ligand per launch, 1/{F64_STEP_DIV} of the MC steps with the MC kernel time "
+                           f"scaled by {F64_STEP_DIV}",
+                 "ligands": [int(i) for i in pick],
                  "seconds_per_ligand": per_lig}
         log(f"plain FP64: {plain['value']:.4f} ligands/s (extrapolated)")
         del d64
@@ -375,9 +413,12 @@ def main():
         "value": value,
         "unit": "ligands/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": len(times),
+        "steps_requested": args.steps,
         "warmup": args.warmup,
+        "warmup_kind": "small launches: 4 ligands across the size range at 1/64 of their steps",
         "ms_per_step": step_s * 1e3,
+        "step_ms": [t * 1e3 for t in times],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -387,8 +428,10 @@ def main():
                    "chains": len(ligs) * args.exhaustiveness,
                    "box_points": int((gd.n[0] + 1) * (gd.n[1] + 1) * (gd.n[2] + 1))},
         "evals_per_s": float(evals.sum() / (mc_ms / 1e3)),
-        "latency_ms": {"p50": step_s * 1e3, "p99": step_s * 1e3,
-                       "note": "one launch: every ligand of the batch completes with it"},
+        "latency_ms": {"p50": float(np.percentile(times, 50)) * 1e3, "p99": float(np.percentile(times, 99)) * 1e3,
+                       "note": "per step over the steps run: a batched screen returns every ligand with its "
+                               "launch, so a ligand's latency is its step's time; job latency under Poisson "
+                               "arrivals is the serving run's (--serve)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "mc_kernel<float, float, false, false>" if prec == sg.FP32 else
@@ -481,6 +524,7 @@ def cpu_baseline(sg, args, log, rec, ligs, flats, gd, hyd):
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     cores = min(cores, 16)
     nsample = min(args.cpu_sample or cores, len(ligs))
+    pick = size_sample(flats, nsample)
     step_div = CPU_STEP_DIV * (16 if args.flex else 1)
     sf = sg.vina_scoring(hyd)
     params = sg.mc_params()
@@ -500,11 +544,12 @@ def cpu_baseline(sg, args, log, rec, ligs, flats, gd, hyd):
 
     t = time.perf_counter()
     with cf.ThreadPoolExecutor(cores) as ex:
-        out = list(ex.map(one, range(nsample)))
+        out = list(ex.map(one, pick))
     dt = time.perf_counter() - t
     est = [o[0] for o in out]
     cpu = {"value": cores / float(np.mean(est)), "unit": "ligands/s", "cores": cores, "kind": "port",
-           "sample": f"first {nsample} ligands of the batch, one per core on {cores} threads: receptor maps plus "
+           "sample": f"{nsample} ligands across the batch's size range (heavy atoms "
+                     f"{[int(flats[i].nheavy) for i in pick]}), one per core on {cores} threads: receptor maps plus "
                      f"one chain at 1/{step_div} of its Monte Carlo steps, per-ligand time = maps + "
                      f"{args.exhaustiveness} chains x {step_div} ({dt:.1f}s wall)",
            "evals_per_s_per_core": float(np.mean([o[1] / o[2] for o in out]))}

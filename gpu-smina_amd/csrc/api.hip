@@ -166,6 +166,9 @@ struct sgpu_ctx
 	static constexpr int NCLASS = 4;
 	hipStream_t class_stream[NCLASS - 1] = {};
 	hipEvent_t class_done[NCLASS - 1] = {};
+	// recorded on `stream` at the start of every part of a launch: the side
+	// streams wait on it, so they see that part's step limits
+	hipEvent_t part_start = nullptr;
 	std::mutex mu;
 	std::string err;
 	Tables tab;
@@ -806,6 +809,8 @@ sgpu_status sgpu_create(int device, sgpu_ctx** out)
 		if (hipStreamCreateWithFlags(&c->class_stream[k], hipStreamNonBlocking) != hipSuccess ||
 				hipEventCreateWithFlags(&c->class_done[k], hipEventDisableTiming) != hipSuccess)
 			return SGPU_E_HIP;
+	if (hipEventCreateWithFlags(&c->part_start, hipEventDisableTiming) != hipSuccess)
+		return SGPU_E_HIP;
 	*out = c.release();
 	return SGPU_OK;
 }
@@ -832,6 +837,7 @@ void sgpu_destroy(sgpu_ctx* c)
 	(void)hipStreamSynchronize(c->stream);
 	(void)hipEventDestroy(c->ev0);
 	(void)hipEventDestroy(c->ev1);
+	(void)hipEventDestroy(c->part_start);
 	for (int k = 0; k < sgpu_ctx::NCLASS - 1; k++)
 	{
 		(void)hipStreamSynchronize(c->class_stream[k]);
@@ -1461,6 +1467,10 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 	for (int part = part0; part <= parts; part++)
 	{
 		const int lo = part == part0 ? 0 : part - 1, hi = part;
+		// everything enqueued on the main stream so far (the chain upload,
+		// and after a checkpoint the early-stop step limits) comes before
+		// this part's launches on every stream
+		hip_check(hipEventRecord(c->part_start, c->stream), "event");
 		std::vector<hipStream_t> side;
 		for (int k = sgpu_ctx::NCLASS - 1; k >= 0; k--)
 		{
@@ -1476,7 +1486,7 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 			{
 				st = c->class_stream[side.size()];
 				side.push_back(st);
-				hip_check(hipStreamWaitEvent(st, c->ev0, 0), "stream wait");
+				hip_check(hipStreamWaitEvent(st, c->part_start, 0), "stream wait");
 			}
 			hip_check(launch_mc<T, S>(ptrs<T>(b), ec, cl.plan, P, L, dchains.as<char>(), int(cl.blks.size()),
 							  cl.waves, dbm.as<int>() + cl.block_off, dsc.as<int>() + cl.slot_off, dsteps.as<int>(),

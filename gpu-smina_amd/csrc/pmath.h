@@ -1,21 +1,30 @@
 /*
  * Portable double-precision sin, cos, exp and acos, compiled identically for
  * the host (g++) and for gfx950 (hipcc), so that the FP64 docking search gives
- * the same bits on the GPU as the CPU restatement of smina's search.
+ * the same bits on the GPU as a CPU smina build with SMINA_PORTABLE_MATH.
  *
- * Method: Sun fdlibm 5.3 (freely distributable; the published algorithms of
- * k_sin.c, k_cos.c, e_rem_pio2.c (medium-argument Cody-Waite path), e_exp.c
- * and e_acos.c with their published minimax coefficients).  The reference
- * switches smina's search to such functions for its bit-exact validation
- * build (src/lib/portable_math.h:1-34, used by src/lib/monte_carlo.cpp:28-35
- * and src/lib/quaternion.cpp:25-32).
+ * Method: Sun fdlibm 5.3 (freely distributable): the minimax kernels of
+ * k_sin.c and k_cos.c, e_exp.c and e_acos.c with their published
+ * coefficients.  The variant is the one smina's portable build uses
+ * (src/lib/portable_math.h:89-264), which departs from stock fdlibm:
+ *   - the kernels take no tail word (the reduced argument is one double);
+ *   - cos on 0.3 <= |x| <= 0.78125 splits off qx = (x*x)/4 with its low
+ *     word cleared (src/lib/portable_math.h:112-128);
+ *   - reduction by pi/2 is one Cody-Waite step with a 33-bit head and its
+ *     tail, the quotient rounded to nearest (:131-141), with no second or
+ *     third refinement;
+ *   - exp reduces every |x| > ln2/2 by k = (int)(x/ln2 +- 1/2) and scales
+ *     by 2^k through multiplications (:62-86, :232-264); acos has no tiny-x
+ *     shortcut (:191-230).
+ * These choices fix the result bits; changing any of them changes sin or cos
+ * by one ulp on a sizeable share of arguments.
  *
- * Domain: sin/cos are exact to fdlibm accuracy for |x| < 2^19 * pi/2 (the
- * search only calls them with |x| <= pi); larger arguments are reduced with
- * the same three-stage Cody-Waite scheme, which loses accuracy there.
+ * Domain: the search calls sin and cos with |x| <= pi; the reduction is good
+ * to full accuracy for |x| up to about 2^20 (its 33-bit head times the
+ * quotient is exact).
  *
  * Every expression is written so that no fused multiply-add may change it:
- * build with -ffp-contract=off.
+ * build with -ffp-contract=off where the bits matter.
  */
 #ifndef SGPU_PMATH_H
 #define SGPU_PMATH_H
@@ -33,277 +42,182 @@ namespace pmath
 
 SGPU_HD uint64_t bits(double x) { return __builtin_bit_cast(uint64_t, x); }
 SGPU_HD double from_bits(uint64_t u) { return __builtin_bit_cast(double, u); }
-SGPU_HD int32_t hi_word(double x) { return int32_t(bits(x) >> 32); }
-SGPU_HD uint32_t lo_word(double x) { return uint32_t(bits(x)); }
-SGPU_HD double make(int32_t hi, uint32_t lo)
-{
-	return from_bits((uint64_t(uint32_t(hi)) << 32) | uint64_t(lo));
-}
-SGPU_HD double with_hi(double x, int32_t hi) { return make(hi, lo_word(x)); }
+SGPU_HD double magnitude(double x) { return from_bits(bits(x) & 0x7fffffffffffffffull); }
+// x with its low 32 mantissa bits cleared
+SGPU_HD double high_part(double x) { return from_bits(bits(x) & 0xffffffff00000000ull); }
+// 2^e for a normal exponent e
+SGPU_HD double pow2(int e) { return from_bits(uint64_t(e + 1023) << 52); }
 
-// k_sin.c: sin on [-pi/4, pi/4]; y is the tail of x, iy = 0 means y = 0
-SGPU_HD double kernel_sin(double x, double y, int iy)
+// x * 2^k by at most three exact power-of-two factors (results that are
+// subnormal round once, in the last factor)
+SGPU_HD double ldexp_mul(double x, int k)
 {
-	const double S1 = -1.66666666666666324348e-01;
-	const double S2 = 8.33333333332248946124e-03;
-	const double S3 = -1.98412698298579493134e-04;
-	const double S4 = 2.75573137070700676789e-06;
-	const double S5 = -2.50507602534068634195e-08;
-	const double S6 = 1.58969099521155010221e-10;
-	const int32_t ix = hi_word(x) & 0x7fffffff;
-	if (ix < 0x3e400000) // |x| < 2^-27
-		if (int(x) == 0)
-			return x;
-	const double z = x * x;
-	const double v = z * x;
-	const double r = S2 + z * (S3 + z * (S4 + z * (S5 + z * S6)));
-	if (iy == 0)
-		return x + v * (S1 + z * r);
-	return x - ((z * (0.5 * y - v * r) - y) - v * S1);
-}
-
-// k_cos.c: cos on [-pi/4, pi/4]
-SGPU_HD double kernel_cos(double x, double y)
-{
-	const double C1 = 4.16666666666666019037e-02;
-	const double C2 = -1.38888888888741095749e-03;
-	const double C3 = 2.48015872894767294178e-05;
-	const double C4 = -2.75573143513906633035e-07;
-	const double C5 = 2.08757232129817482790e-09;
-	const double C6 = -1.13596475577881948265e-11;
-	const int32_t ix = hi_word(x) & 0x7fffffff;
-	if (ix < 0x3e400000) // |x| < 2^-27
-		if (int(x) == 0)
-			return 1.0;
-	const double z = x * x;
-	const double r = z * (C1 + z * (C2 + z * (C3 + z * (C4 + z * (C5 + z * C6)))));
-	if (ix < 0x3FD33333) // |x| < 0.3
-		return 1.0 - (0.5 * z - (z * r - x * y));
-	double qx;
-	if (ix > 0x3fe90000) // |x| > 0.78125
-		qx = 0.28125;
-	else
-		qx = make(ix - 0x00200000, 0); // x/4
-	const double hz = 0.5 * z - qx;
-	const double a = 1.0 - qx;
-	return a - (hz - (z * r - x * y));
-}
-
-// e_rem_pio2.c, medium path: x = n*pi/2 + (y0 + y1), |y0 + y1| <= pi/4
-SGPU_HD int rem_pio2(double x, double& y0, double& y1)
-{
-	const double invpio2 = 6.36619772367581382433e-01;
-	const double pio2_1 = 1.57079632673412561417e+00;
-	const double pio2_1t = 6.07710050650619224932e-11;
-	const double pio2_2 = 6.07710050630396597660e-11;
-	const double pio2_2t = 2.02226624879595063154e-21;
-	const double pio2_3 = 2.02226624871116645580e-21;
-	const double pio2_3t = 8.47842766036889956997e-32;
-	const int32_t hx = hi_word(x);
-	const int32_t ix = hx & 0x7fffffff;
-	const double t = x < 0 ? -x : x;
-	const int n = int(t * invpio2 + 0.5);
-	const double fn = double(n);
-	double r = t - fn * pio2_1;
-	double w = fn * pio2_1t; // first round, good to 85 bits
-	const int32_t j = ix >> 20;
-	y0 = r - w;
-	int32_t i = j - ((hi_word(y0) >> 20) & 0x7ff);
-	if (i > 16)
-	{ // second iteration, good to 118 bits
-		double tt = r;
-		w = fn * pio2_2;
-		r = tt - w;
-		w = fn * pio2_2t - ((tt - r) - w);
-		y0 = r - w;
-		i = j - ((hi_word(y0) >> 20) & 0x7ff);
-		if (i > 49)
-		{ // third iteration, 151 bits
-			tt = r;
-			w = fn * pio2_3;
-			r = tt - w;
-			w = fn * pio2_3t - ((tt - r) - w);
-			y0 = r - w;
+	if (k > 1023)
+	{
+		x = x * pow2(1023);
+		k -= 1023;
+	}
+	else if (k < -1022)
+	{
+		x = x * pow2(-1000);
+		k += 1000;
+		if (k < -1022)
+		{
+			x = x * pow2(-1000);
+			k += 1000;
 		}
 	}
-	y1 = (r - y0) - w;
-	if (hx < 0)
-	{
-		y0 = -y0;
-		y1 = -y1;
-		return -n;
-	}
-	return n;
+	return x * pow2(k);
+}
+
+// nearest integer, ties to even, by the 1.5 * 2^52 shift (|x| < 2^51)
+SGPU_HD double round_even(double x)
+{
+	const double shift = 6755399441055744.0;
+	return x >= 0 ? (x + shift) - shift : -((shift - x) - shift);
+}
+
+// sin on [-pi/4, pi/4]: x + x^3 (S1 + x^2 P(x^2))
+SGPU_HD double sin_kernel(double x)
+{
+	const double s1 = -1.66666666666666324348e-01, s2 = 8.33333333332248946124e-03,
+			s3 = -1.98412698298579493134e-04, s4 = 2.75573137070700676789e-06,
+			s5 = -2.50507602534068634195e-08, s6 = 1.58969099521155010221e-10;
+	const double z = x * x;
+	const double x3 = z * x;
+	const double poly = s2 + z * (s3 + z * (s4 + z * (s5 + z * s6)));
+	return x + x3 * (s1 + z * poly);
+}
+
+// cos on [-pi/4, pi/4]: 1 - x^2/2 + x^4 Q(x^2); above |x| = 0.3 the 1 - x^2/2
+// part is split at the dyadic qx so that 1 - qx is exact
+SGPU_HD double cos_kernel(double x)
+{
+	const double c1 = 4.16666666666666019037e-02, c2 = -1.38888888888741095749e-03,
+			c3 = 2.48015872894767294178e-05, c4 = -2.75573143513906633035e-07,
+			c5 = 2.08757232129817482790e-09, c6 = -1.13596475577881948265e-11;
+	const double z = x * x;
+	const double zq = z * (c1 + z * (c2 + z * (c3 + z * (c4 + z * (c5 + z * c6)))));
+	const double ax = magnitude(x);
+	if (ax < 0.3)
+		return 1.0 - (0.5 * z - z * zq);
+	const double qx = ax > 0.78125 ? 0.28125 : high_part(z * 0.25);
+	const double half_z = 0.5 * z - qx;
+	const double one_minus_q = 1.0 - qx;
+	return one_minus_q - (half_z - z * zq);
+}
+
+// x = q pi/2 + r: returns r, and q through *quadrant
+SGPU_HD double reduce_half_pi(double x, int* quadrant)
+{
+	const double two_over_pi = 6.36619772367581382433e-01;
+	const double half_pi_head = 1.57079632673412561417e+00; // 33 leading bits
+	const double half_pi_tail = 6.07710050650619224932e-11; // pi/2 - head
+	const double q = round_even(x * two_over_pi);
+	*quadrant = int(q);
+	return (x - q * half_pi_head) - q * half_pi_tail;
 }
 
 SGPU_HD double sin(double x)
 {
-	const int32_t ix = hi_word(x) & 0x7fffffff;
-	if (ix <= 0x3fe921fb)
-		return kernel_sin(x, 0.0, 0);
-	if (ix >= 0x7ff00000)
-		return x - x; // NaN
-	double y0, y1;
-	const int n = rem_pio2(x, y0, y1);
-	switch (n & 3)
-	{
-	case 0:
-		return kernel_sin(y0, y1, 1);
-	case 1:
-		return kernel_cos(y0, y1);
-	case 2:
-		return -kernel_sin(y0, y1, 1);
-	default:
-		return -kernel_cos(y0, y1);
-	}
+	if (magnitude(x) <= 7.85398163397448278999e-01)
+		return sin_kernel(x);
+	int q = 0;
+	const double r = reduce_half_pi(x, &q);
+	const int quad = q & 3;
+	const double v = (quad & 1) ? cos_kernel(r) : sin_kernel(r);
+	return (quad & 2) ? -v : v;
 }
 
 SGPU_HD double cos(double x)
 {
-	const int32_t ix = hi_word(x) & 0x7fffffff;
-	if (ix <= 0x3fe921fb)
-		return kernel_cos(x, 0.0);
-	if (ix >= 0x7ff00000)
-		return x - x; // NaN
-	double y0, y1;
-	const int n = rem_pio2(x, y0, y1);
-	switch (n & 3)
-	{
-	case 0:
-		return kernel_cos(y0, y1);
-	case 1:
-		return -kernel_sin(y0, y1, 1);
-	case 2:
-		return -kernel_cos(y0, y1);
-	default:
-		return kernel_sin(y0, y1, 1);
-	}
+	if (magnitude(x) <= 7.85398163397448278999e-01)
+		return cos_kernel(x);
+	int q = 0;
+	const double r = reduce_half_pi(x, &q);
+	const int quad = q & 3;
+	const double v = (quad & 1) ? sin_kernel(r) : cos_kernel(r);
+	return (quad == 1 || quad == 2) ? -v : v;
 }
 
-// e_exp.c
+// e^x = 2^k e^r, r = x - k ln2 in two parts; e^r by the Remez rational form
 SGPU_HD double exp(double x)
 {
-	const double o_threshold = 7.09782712893383973096e+02;
-	const double u_threshold = -7.45133219101941108420e+02;
-	const double ln2HI = 6.93147180369123816490e-01;
-	const double ln2LO = 1.90821492927058770002e-10;
-	const double invln2 = 1.44269504088896338700e+00;
-	const double P1 = 1.66666666666666019037e-01;
-	const double P2 = -2.77777777770155933842e-03;
-	const double P3 = 6.61375632143793436117e-05;
-	const double P4 = -1.65339022054652515390e-06;
-	const double P5 = 4.13813679705723846039e-08;
-	const double twom1000 = 9.33263618503218878990e-302; // 2^-1000
-	int32_t hx = hi_word(x);
-	const int xsb = (hx >> 31) & 1; // sign bit
-	hx &= 0x7fffffff;
-	if (hx >= 0x40862E42)
-	{ // |x| >= 709.78...
-		if (hx >= 0x7ff00000)
-		{
-			if (((hx & 0xfffff) | lo_word(x)) != 0)
-				return x + x; // NaN
-			return xsb == 0 ? x : 0.0; // exp(+-inf) = {inf, 0}
-		}
-		if (x > o_threshold)
-			return from_bits(0x7ff0000000000000ull); // overflow
-		if (x < u_threshold)
-			return 0.0; // underflow
-	}
-	double hi = 0, lo = 0;
+	const double ln2_head = 6.93147180369123816490e-01, ln2_tail = 1.90821492927058770002e-10;
+	const double inv_ln2 = 1.44269504088896338700e+00;
+	const double p1 = 1.66666666666666019037e-01, p2 = -2.77777777770155933842e-03,
+			p3 = 6.61375632143793436117e-05, p4 = -1.65339022054652515390e-06,
+			p5 = 4.13813679705723846039e-08;
+	if (!(x == x))
+		return x;
+	if (x > 7.09782712893383973096e+02)
+		return from_bits(0x7ff0000000000000ull);
+	if (x < -7.45133219101941108420e+02)
+		return 0.0;
+	const double ax = magnitude(x);
 	int k = 0;
-	if (hx > 0x3fd62e42)
-	{ // |x| > 0.5 ln2
-		if (hx < 0x3FF0A2B2)
-		{ // and |x| < 1.5 ln2
-			hi = xsb ? x + ln2HI : x - ln2HI;
-			lo = xsb ? -ln2LO : ln2LO;
-			k = 1 - xsb - xsb;
-		}
-		else
-		{
-			k = int(invln2 * x + (xsb ? -0.5 : 0.5));
-			const double t = double(k);
-			hi = x - t * ln2HI; // t*ln2HI is exact here
-			lo = t * ln2LO;
-		}
-		x = hi - lo;
+	double head = x, tail = 0.0;
+	if (ax > 0.5 * 6.93147180559945286227e-01)
+	{
+		k = int(inv_ln2 * x + (x < 0 ? -0.5 : 0.5));
+		const double kd = double(k);
+		head = x - kd * ln2_head;
+		tail = kd * ln2_tail;
+		x = head - tail;
 	}
-	else if (hx < 0x3e300000)
-	{ // |x| < 2^-28
+	else if (ax < 3.7252902984e-09)
+	{
 		return 1.0 + x;
 	}
-	else
-		k = 0;
 	const double t = x * x;
-	const double c = x - t * (P1 + t * (P2 + t * (P3 + t * (P4 + t * P5))));
+	const double c = x - t * (p1 + t * (p2 + t * (p3 + t * (p4 + t * p5))));
 	if (k == 0)
 		return 1.0 - ((x * c) / (c - 2.0) - x);
-	double y = 1.0 - ((lo - (x * c) / (2.0 - c)) - hi);
-	if (k >= -1021)
-		return with_hi(y, hi_word(y) + (k << 20));
-	y = with_hi(y, hi_word(y) + ((k + 1000) << 20));
-	return y * twom1000;
+	const double y = 1.0 - ((tail - (x * c) / (2.0 - c)) - head);
+	return ldexp_mul(y, k);
 }
 
-// e_acos.c
+// acos: |x| < 1/2 as pi/2 - asin(x) with asin(x) = x + x R(x^2); otherwise
+// through z = (1 -+ x)/2 and s = sqrt(z), with s split into its high part
+// and a correction for x > 1/2
 SGPU_HD double acos(double x)
 {
 	const double pi = 3.14159265358979311600e+00;
-	const double pio2_hi = 1.57079632679489655800e+00;
-	const double pio2_lo = 6.12323399573676603587e-17;
-	const double pS0 = 1.66666666666666657415e-01;
-	const double pS1 = -3.25565818622400915405e-01;
-	const double pS2 = 2.01212532134862925881e-01;
-	const double pS3 = -4.00555345006794114027e-02;
-	const double pS4 = 7.91534994289814532176e-04;
-	const double pS5 = 3.47933107596021167570e-05;
-	const double qS1 = -2.40339491173441421878e+00;
-	const double qS2 = 2.02094576023350569471e+00;
-	const double qS3 = -6.88283971605453293030e-01;
-	const double qS4 = 7.70381505559019352791e-02;
-	const int32_t hx = hi_word(x);
-	const int32_t ix = hx & 0x7fffffff;
-	if (ix >= 0x3ff00000)
-	{ // |x| >= 1
-		if (((ix - 0x3ff00000) | int32_t(lo_word(x))) == 0)
-		{ // |x| == 1
-			if (hx > 0)
-				return 0.0;
-			return pi + 2.0 * pio2_lo;
-		}
-		return (x - x) / (x - x); // NaN
+	const double half_pi_hi = 1.57079632679489655800e+00, half_pi_lo = 6.12323399573676603587e-17;
+	const double a0 = 1.66666666666666657415e-01, a1 = -3.25565818622400915405e-01,
+			a2 = 2.01212532134862925881e-01, a3 = -4.00555345006794114027e-02,
+			a4 = 7.91534994289814532176e-04, a5 = 3.47933107596021167570e-05;
+	const double b1 = -2.40339491173441421878e+00, b2 = 2.02094576023350569471e+00,
+			b3 = -6.88283971605453293030e-01, b4 = 7.70381505559019352791e-02;
+	const double ax = magnitude(x);
+	if (ax >= 1.0)
+	{
+		if (x == 1.0)
+			return 0.0;
+		if (x == -1.0)
+			return pi;
+		return (x - x) / (x - x);
 	}
-	if (ix < 0x3fe00000)
-	{ // |x| < 0.5
-		if (ix <= 0x3c600000)
-			return pio2_hi + pio2_lo; // |x| < 2^-57
+	if (ax < 0.5)
+	{
 		const double z = x * x;
-		const double p = z * (pS0 + z * (pS1 + z * (pS2 + z * (pS3 + z * (pS4 + z * pS5)))));
-		const double q = 1.0 + z * (qS1 + z * (qS2 + z * (qS3 + z * qS4)));
-		const double r = p / q;
-		return pio2_hi - (x - (pio2_lo - x * r));
+		const double num = z * (a0 + z * (a1 + z * (a2 + z * (a3 + z * (a4 + z * a5)))));
+		const double den = 1.0 + z * (b1 + z * (b2 + z * (b3 + z * b4)));
+		return half_pi_hi - (x - (half_pi_lo - x * (num / den)));
 	}
-	if (hx < 0)
-	{ // x < -0.5
-		const double z = (1.0 + x) * 0.5;
-		const double p = z * (pS0 + z * (pS1 + z * (pS2 + z * (pS3 + z * (pS4 + z * pS5)))));
-		const double q = 1.0 + z * (qS1 + z * (qS2 + z * (qS3 + z * qS4)));
-		const double s = __builtin_sqrt(z);
-		const double r = p / q;
-		const double w = r * s - pio2_lo;
+	const double z = x < 0 ? (1.0 + x) * 0.5 : (1.0 - x) * 0.5;
+	const double num = z * (a0 + z * (a1 + z * (a2 + z * (a3 + z * (a4 + z * a5)))));
+	const double den = 1.0 + z * (b1 + z * (b2 + z * (b3 + z * b4)));
+	const double s = __builtin_sqrt(z);
+	if (x < 0)
+	{
+		const double w = (num / den) * s - half_pi_lo;
 		return pi - 2.0 * (s + w);
 	}
-	// x > 0.5
-	const double z = (1.0 - x) * 0.5;
-	const double s = __builtin_sqrt(z);
-	const double df = make(hi_word(s), 0);
-	const double c = (z - df * df) / (s + df);
-	const double p = z * (pS0 + z * (pS1 + z * (pS2 + z * (pS3 + z * (pS4 + z * pS5)))));
-	const double q = 1.0 + z * (qS1 + z * (qS2 + z * (qS3 + z * qS4)));
-	const double r = p / q;
-	const double w = r * s + c;
-	return 2.0 * (df + w);
+	const double s_hi = high_part(s);
+	const double corr = (z - s_hi * s_hi) / (s + s_hi);
+	const double w = (num / den) * s + corr;
+	return 2.0 * (s_hi + w);
 }
 
 } // namespace pmath

@@ -149,20 +149,39 @@ void DockingServer::run(Worker* w)
 			dj[k].exhaustiveness = batch[k]->exhaustiveness;
 			dp.push_back(&dj[k]);
 		}
-		std::string err;
+		// a failing launch (an atom overlap, a HIP error, no memory) fails
+		// only the jobs that cause it: its jobs are docked again one at a
+		// time (a job's search depends on the job alone, so a job docked
+		// alone gives the result it would have had in the batch)
+		std::vector<std::string> errs(batch.size());
 		try
 		{
 			w->receptors[size_t(receptor)]->dock_batch(dp);
 		}
 		catch (std::exception& e)
 		{
-			err = e.what();
+			if (batch.size() == 1)
+				errs[0] = e.what();
+			else
+				for (size_t k = 0; k < batch.size(); k++)
+				{
+					dj[k].out.clear();
+					dj[k].evals = 0;
+					std::vector<GpuDocking::DockJob*> one{&dj[k]};
+					try
+					{
+						w->receptors[size_t(receptor)]->dock_batch(one);
+					}
+					catch (std::exception& e1)
+					{
+						errs[k] = e1.what();
+						dj[k].out.clear();
+					}
+				}
 		}
 		{
 			std::lock_guard<std::mutex> lock(mu_);
 			const double t = now();
-			if (!err.empty())
-				err_ = err;
 			for (size_t k = 0; k < batch.size(); k++)
 			{
 				Job* j = batch[k];
@@ -172,6 +191,9 @@ void DockingServer::run(Worker* w)
 				j->result.latency_s = j->t_done - j->t_submit;
 				j->result.queued_s = j->t_admit - j->t_submit;
 				j->result.batch_size = int(batch.size());
+				j->result.error = errs[k];
+				if (!errs[k].empty())
+					err_ = "job " + std::to_string(j->id) + ": " + errs[k];
 				j->done = true;
 			}
 		}
@@ -217,7 +239,14 @@ struct sgpu_server
 	std::unique_ptr<DockingServer> s;
 	std::vector<sgpu_term> terms;
 	std::vector<sgpu_ci_term> ci;
+	// the message of the last failing call on this server (any thread)
+	std::mutex emu;
 	std::string err;
+	void fail(const std::string& m)
+	{
+		std::lock_guard<std::mutex> lock(emu);
+		err = m;
+	}
 };
 
 extern "C" {
@@ -262,7 +291,7 @@ sgpu_server* sgpu_server_create(const sgpu_server_settings* s)
 	}
 	catch (std::exception& e)
 	{
-		sv->err = e.what();
+		sv->fail(e.what());
 	}
 	return sv;
 }
@@ -271,10 +300,9 @@ void sgpu_server_free(sgpu_server* sv) { delete sv; }
 
 const char* sgpu_server_error(const sgpu_server* sv)
 {
-	if (!sv->err.empty() || !sv->s)
-		return sv->err.c_str();
 	static thread_local std::string e;
-	e = sv->s->error();
+	std::lock_guard<std::mutex> lock(const_cast<sgpu_server*>(sv)->emu);
+	e = sv->err;
 	return e.c_str();
 }
 
@@ -291,7 +319,7 @@ int sgpu_server_add_receptor(sgpu_server* sv, const sgpu_mol* receptor, const sg
 	}
 	catch (std::exception& e)
 	{
-		sv->err = e.what();
+		sv->fail(e.what());
 		return -1;
 	}
 }
@@ -306,7 +334,7 @@ int sgpu_server_submit(sgpu_server* sv, int receptor, const sgpu_mol* ligand, in
 	}
 	catch (std::exception& e)
 	{
-		sv->err = e.what();
+		sv->fail(e.what());
 		return -1;
 	}
 }
@@ -318,6 +346,16 @@ int sgpu_server_wait(sgpu_server* sv, int job, sgpu_serve_result* out)
 	try
 	{
 		DockingServer::Result r = sv->s->wait(job);
+		out->evals = r.evals;
+		out->latency_s = r.latency_s;
+		out->queued_s = r.queued_s;
+		out->batch_size = r.batch_size;
+		if (!r.error.empty())
+		{
+			out->nposes = 0;
+			sv->fail("job " + std::to_string(job) + ": " + r.error);
+			return -1;
+		}
 		out->nposes = int(std::min<size_t>(32, r.poses.size()));
 		for (int k = 0; k < out->nposes; k++)
 			out->e[k] = r.poses[size_t(k)].e;
@@ -329,7 +367,7 @@ int sgpu_server_wait(sgpu_server* sv, int job, sgpu_serve_result* out)
 	}
 	catch (std::exception& e)
 	{
-		sv->err = e.what();
+		sv->fail(e.what());
 		return -1;
 	}
 }

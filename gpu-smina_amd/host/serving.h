@@ -63,6 +63,7 @@ public:
 		double latency_s = 0; // submit to result
 		double queued_s = 0;  // submit to admission
 		int batch_size = 0;   // jobs in its launch
+		std::string error;    // empty: docked; else why this job failed
 	};
 	// block until job `id` completes
 	Result wait(int id);

@@ -67,3 +67,25 @@ def test_fast_serving_does_not_depend_on_the_batch():
     mixed = run(ligs)[0]
     print("fast serving", alone[:3], mixed[:3], flush=True)
     assert len(alone) > 0 and alone == mixed
+
+
+def test_a_failing_job_fails_alone():
+    """a job that cannot be docked (a 300-heavy-atom ligand, past the
+    layout's 255 atoms per model) beside good jobs in one launch: its wait
+    raises with the reason, and the good jobs' results equal their results
+    docked alone"""
+    rec = sg.Mol.receptor(11, 1200, 8.0)
+    gd = sg.grid_dims((0.0, 0.0, 0.0), (20.0, 20.0, 20.0))
+    good = [sg.Mol.ligand(400 + k, 12 + 2 * k, 2) for k in range(3)]
+    bad = sg.Mol.ligand(500, 300, 4)
+    srv = sg.Server(workers=1, max_batch=8)
+    r = srv.add_receptor(rec, gd, list(range(2, 14)))
+    jobs = [srv.submit(r, good[0], 2), srv.submit(r, bad, 2), srv.submit(r, good[1], 2), srv.submit(r, good[2], 2)]
+    with pytest.raises(RuntimeError, match="atoms out of range"):
+        srv.wait(jobs[1])
+    mixed = [srv.wait(j) for j in (jobs[0], jobs[2], jobs[3])]
+    assert max(m["batch_size"] for m in mixed) > 1
+    for lig, m in zip(good, mixed):
+        alone = srv.wait(srv.submit(r, lig, 2))
+        assert len(m["e"]) > 0 and alone["e"] == m["e"] and alone["evals"] == m["evals"]
+    del srv
