@@ -154,6 +154,33 @@ struct Batch
 	std::unique_ptr<Batch> fp64;
 };
 
+// Scratch of the allocation-free entry point (sgpu_eval_confs_noalloc): one
+// array per GPU, held until the process ends.  Host code never uses this
+// name as an address: each device's address comes from hipGetSymbolAddress
+// (noalloc_scratch_of), taken once per device with that device current.
+static __device__ __align__(256) unsigned char scratch[SGPU_NOALLOC_SCRATCH_BYTES];
+
+std::mutex noalloc_mu;
+std::vector<unsigned char*> noalloc_ptr; // per device, null until taken
+
+// the device address of `scratch` on `device` (current), taken on first use
+hipError_t noalloc_scratch_of(int device, unsigned char** out)
+{
+	std::lock_guard<std::mutex> lock(noalloc_mu);
+	if (device >= int(noalloc_ptr.size()))
+		noalloc_ptr.resize(size_t(device) + 1, nullptr);
+	if (!noalloc_ptr[size_t(device)])
+	{
+		void* ptr = nullptr;
+		const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+		if (e != hipSuccess)
+			return e;
+		noalloc_ptr[size_t(device)] = static_cast<unsigned char*>(ptr);
+	}
+	*out = noalloc_ptr[size_t(device)];
+	return hipSuccess;
+}
+
 } // namespace
 
 struct sgpu_ctx
@@ -183,6 +210,8 @@ struct sgpu_ctx
 	int progress_chunks = 1;
 	sgpu_progress_fn progress_fn = nullptr;
 	void* progress_user = nullptr;
+	// this device's copy of the allocation-free entry point's scratch
+	unsigned char* noalloc_scratch = nullptr;
 };
 
 namespace
@@ -811,6 +840,13 @@ sgpu_status sgpu_create(int device, sgpu_ctx** out)
 			return SGPU_E_HIP;
 	if (hipEventCreateWithFlags(&c->part_start, hipEventDisableTiming) != hipSuccess)
 		return SGPU_E_HIP;
+	// the allocation-free entry point's scratch address on this device (a
+	// failure here disables that entry point only: it reports the error)
+	if (noalloc_scratch_of(device, &c->noalloc_scratch) != hipSuccess)
+	{
+		c->noalloc_scratch = nullptr;
+		(void)hipGetLastError();
+	}
 	*out = c.release();
 	return SGPU_OK;
 }
@@ -1660,6 +1696,82 @@ void run_score(sgpu_ctx* c, const sgpu_score_request* q)
 	hip_check(hipStreamSynchronize(c->stream), "score");
 }
 
+
+template<typename T>
+void run_eval_noalloc(sgpu_ctx* c, const sgpu_eval_request_dev* q)
+{
+	Batch& b = find_batch(c, q->batch_id);
+	const int n = q->nconfs;
+	require(n > 0 && q->model && q->conf && q->e, "eval request incomplete");
+	const WavePlan& plan = b.h.plan;
+	require(q->conf_stride >= plan.cmax, "conf stride below the batch's largest conf");
+	require(!q->g || q->change_stride >= plan.gmax, "change stride below the batch's largest change size");
+	require(!q->coords || q->coord_stride >= 3 * plan.nmax, "coord stride below 3 x the batch's largest model");
+	const NoallocLayout L = noalloc_layout(n, plan.cmax, plan.gmax, plan.nmax, plan.pmax, sizeof(T));
+	if (L.bytes > size_t(SGPU_NOALLOC_SCRATCH_BYTES))
+		throw Fail{SGPU_E_INVALID, "request needs " + std::to_string(L.bytes) + " bytes of scratch, above the " +
+						std::to_string(SGPU_NOALLOC_SCRATCH_BYTES) + " of the allocation-free entry point"};
+	require(c->noalloc_scratch != nullptr, "hipGetSymbolAddress gave no scratch address for this device");
+	EvalCtx<T> ec{};
+	ec.tab = table_ptrs<T>(c);
+	for (int d = 0; d < 3; d++)
+		ec.v[d] = T(q->v[d]);
+	if (q->grid_id >= 0)
+	{
+		Grid& g = find_grid(c, q->grid_id);
+		require(g.precision == eval_precision(b.precision), "grid and model precision differ");
+		for (int mi = 0; mi < b.n; mi++)
+			check_maps(b, g, &mi, 1);
+		ec.backend = 0;
+		ec.grid = grid_ptrs<T>(g, q->slope);
+		ec.slope = T(q->slope);
+	}
+	else
+	{
+		require(q->gd != nullptr, "no box for the direct receptor term");
+		check_gd(q->gd);
+		Receptor& r = find_receptor(c, q->receptor_id);
+		std::vector<double> key;
+		for (int i = 0; i < 3; i++)
+		{
+			key.push_back(q->gd->begin[i]);
+			key.push_back(q->gd->end[i]);
+			key.push_back(q->gd->n[i]);
+		}
+		auto it = r.cells.find(key);
+		require(it != r.cells.end(), "the receptor's cells for this box are not built: an sgpu_eval_confs, "
+		                             "sgpu_run_refine or sgpu_score call with this box builds them");
+		ec.backend = 1;
+		ec.rec = receptor_ptrs<T>(r, *it->second);
+		ec.slope = T(q->slope);
+	}
+	NoallocRequest nq{};
+	nq.n = n;
+	nq.model = q->model;
+	nq.conf = q->conf;
+	nq.conf_stride = q->conf_stride;
+	nq.e = q->e;
+	nq.g = q->g;
+	nq.change_stride = q->change_stride;
+	nq.coords = q->coords;
+	nq.coord_stride = q->coord_stride;
+	nq.status = q->status;
+	hipStream_t st = q->stream ? static_cast<hipStream_t>(q->stream) : c->stream;
+	hip_check(launch_eval_noalloc<T>(ptrs<T>(b), ec, plan, b.n, nq, c->noalloc_scratch,
+					  size_t(SGPU_NOALLOC_SCRATCH_BYTES), st),
+			"eval kernel");
+	if (q->status)
+		return;
+	// no status word given: wait, and report the status as the call's
+	int word = 0;
+	hip_check(hipMemcpyAsync(&word, c->noalloc_scratch + L.status, sizeof(int), hipMemcpyDeviceToHost, st),
+			"status");
+	hip_check(hipStreamSynchronize(st), "eval");
+	if (word & NOALLOC_BAD_MODEL)
+		throw Fail{SGPU_E_INVALID, "model index out of range"};
+	if (word & NOALLOC_ATOM_OVERLAP)
+		throw Fail{SGPU_E_ATOM_OVERLAP, "Ligand atom exactly overlaps receptor atom.  I can't deal with this."};
+}
 } // namespace
 
 extern "C" {
@@ -1674,6 +1786,19 @@ sgpu_status sgpu_eval_confs(sgpu_ctx* c, const sgpu_eval_request* q)
 			run_eval<double>(c, q);
 		else
 			run_eval<float>(c, q);
+	});
+}
+
+sgpu_status sgpu_eval_confs_noalloc(sgpu_ctx* c, const sgpu_eval_request_dev* q)
+{
+	return guarded(c, [&]() {
+		require(q != nullptr, "no request");
+		require(c->tab.ok, "tables not uploaded");
+		Batch& b = find_batch(c, q->batch_id);
+		if (b.precision == SGPU_FP64)
+			run_eval_noalloc<double>(c, q);
+		else
+			run_eval_noalloc<float>(c, q);
 	});
 }
 

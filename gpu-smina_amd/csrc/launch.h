@@ -38,6 +38,54 @@ template<typename T, typename S>
 hipError_t launch_refine(const BatchPtrs<T>& B, const EvalCtx<T>& ec, const WavePlan& plan, int n, const int* model,
 		S* conf, int conf_stride, const int* maxiters, int accurate, int early_term, S* e, T* x, int x_stride,
 		T* scratch, int pstride, int64_t* evals, const double* boxes, hipStream_t st);
+// sgpu_eval_confs_noalloc: the caller's device arrays (double) and the
+// layout of the library's scratch region for n confs of a batch
+struct NoallocRequest
+{
+	int n;
+	const int* model;
+	const double* conf;
+	int conf_stride;
+	double* e;
+	double* g;
+	int change_stride;
+	double* coords;
+	int coord_stride;
+	int* status;
+};
+// bits of the status word
+constexpr int NOALLOC_ATOM_OVERLAP = 1; // the direct receptor term met an exact overlap
+constexpr int NOALLOC_BAD_MODEL = 2;    // a model index outside the batch
+struct NoallocLayout
+{
+	int cs, gs, xs, ps;
+	size_t status, model, conf, e, g, x, pairs, bytes;
+};
+inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+// n confs of a batch whose largest model has cmax conf entries, gmax change
+// entries, nmax atoms and pmax pairs, evaluated in reals of `real` bytes
+inline NoallocLayout noalloc_layout(int n, int cmax, int gmax, int nmax, int pmax, size_t real)
+{
+	NoallocLayout L{};
+	L.cs = cmax;
+	L.gs = gmax;
+	L.xs = 3 * nmax;
+	L.ps = pmax;
+	const size_t nn = size_t(n > 0 ? n : 0);
+	L.status = 0;
+	L.model = 256;
+	L.conf = L.model + align256(nn * sizeof(int));
+	L.e = L.conf + align256(nn * size_t(L.cs) * real);
+	L.g = L.e + align256(nn * real);
+	L.x = L.g + align256(nn * size_t(L.gs) * real);
+	L.pairs = L.x + align256(nn * size_t(L.xs) * real);
+	L.bytes = L.pairs + align256(nn * size_t(L.ps) * real);
+	return L;
+}
+template<typename T>
+hipError_t launch_eval_noalloc(const BatchPtrs<T>& B, const EvalCtx<T>& ec, const WavePlan& plan, int nmodels,
+		const NoallocRequest& q, unsigned char* scratch, size_t scratch_bytes, hipStream_t st);
+
 template<typename T>
 hipError_t launch_score(const BatchPtrs<T>& B, const EvalCtx<T>& ec, const WavePlan& plan, int mode, int n,
 		const int* model, const T* conf, int conf_stride, T* e, T* scratch, int pstride, hipStream_t st);

@@ -624,7 +624,7 @@ __device__ inline T direct_term(const EvalCtx<T>& ec, int t1, V3<T> ac, V3<T>& o
 		if (r2 < T(ec.tab.cutoff_sqr))
 		{
 			if (r2 < Real<T>::eps())
-				*ec.error = 1;
+				atomicOr(ec.error, 1); // bit 0: exact overlap (other bits belong to the caller)
 			T pe, pd;
 			table_deriv(ec.tab, tri_index_permissive(t1, r.type[j]), r2, pe, pd);
 			this_e += pe;

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(HERE, "libsgpu.so")
 
 FP32, FP64, MIXED = 0, 1, 2  # sgpu_precision
 OK = 0
+E_INVALID, E_HIP, E_OOM, E_ATOM_OVERLAP, E_NO_DEVICE, E_NOT_FOUND = 1, 2, 3, 4, 5, 6  # sgpu_status
 STATUS = {0: "OK", 1: "INVALID", 2: "HIP", 3: "OOM", 4: "ATOM_OVERLAP", 5: "NO_DEVICE", 6: "NOT_FOUND"}
 
 I32P = C.POINTER(C.c_int)

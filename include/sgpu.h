@@ -260,6 +260,49 @@ typedef struct sgpu_eval_request
 } sgpu_eval_request;
 sgpu_status sgpu_eval_confs(sgpu_ctx* ctx, const sgpu_eval_request* req);
 
+/* sgpu_eval_confs for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid allocation).
+ * It has no counterpart in the reference.  Inputs and outputs are the
+ * caller's DEVICE arrays in double: model[nconfs], conf[nconfs][conf_stride],
+ * e[nconfs], g[nconfs][change_stride] and coords[nconfs][coord_stride]
+ * (g and coords may be NULL; entries past a model's own sizes are written as
+ * 0).  The strides must cover the batch's largest model.  The staged confs,
+ * the evaluation's outputs, the pair scratch and the status word live in one
+ * static device array of SGPU_NOALLOC_SCRATCH_BYTES per GPU, held until the
+ * process ends: a request needing more is refused (SGPU_E_INVALID).  The
+ * call makes no device allocation.  The batch, tables and maps come from
+ * earlier calls; for the direct receptor term (grid_id < 0) the receptor's
+ * cells for the box must already exist (any sgpu_eval_confs, sgpu_run_refine
+ * or sgpu_score call with that box builds them).
+ * stream: a hipStream_t, or NULL for the context's stream.  status: a device
+ * int that receives the status bits (1: an exact atom overlap, 2: a model
+ * index out of range) and the call returns once its work is enqueued; with
+ * status NULL the call waits and returns SGPU_E_ATOM_OVERLAP or
+ * SGPU_E_INVALID for those.  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (other streams, threads or contexts) race. */
+#define SGPU_NOALLOC_SCRATCH_BYTES (32 * 1024 * 1024)
+typedef struct sgpu_eval_request_dev
+{
+	int batch_id;
+	int grid_id;
+	int receptor_id;
+	const sgpu_grid_dims* gd; /* host pointer: the direct term's box */
+	double slope;
+	double v[3];
+	int nconfs;
+	const int* model;
+	const double* conf;
+	int conf_stride;
+	double* e;
+	double* g;
+	int change_stride;
+	double* coords;
+	int coord_stride;
+	int* status;
+	void* stream;
+} sgpu_eval_request_dev;
+sgpu_status sgpu_eval_confs_noalloc(sgpu_ctx* ctx, const sgpu_eval_request_dev* req);
+
 /* run_monte_carlo (src/gpu/gpu_kernels.h:66-72): one chain per wavefront,
  * monte_carlo::operator() (src/lib/monte_carlo.cpp:95-130) for each chain */
 sgpu_status sgpu_run_mc(sgpu_ctx* ctx, int batch_id, int grid_id, const sgpu_mc_params* p,

@@ -2431,6 +2431,37 @@ int orc_mc_chain(orc_ctx* c, int model, const sgpu_mc_params* p, int num_steps, 
 	write_out(m, out, max_out, e, conf, coords);
 	return int(out.size());
 }
+
+// add_to_output_container on a caller-held container of n poses (energies
+// e[n], coordinates coords[n][3 * natoms], capacity max_size): adds the pose
+// (te, tcoords) and returns the new count; conf slots carry each pose's
+// original index (tag) so the caller sees which pose went where
+int orc_container_add(int n, int natoms, int max_size, double min_rmsd, double* e, double* coords, int* tag,
+		double te, const double* tcoords, int ttag)
+{
+	std::vector<Output> out;
+	for (int i = 0; i < n; i++)
+	{
+		Output o{{double(tag[i])}, e[i], {}};
+		for (int a = 0; a < natoms; a++)
+			o.coords.push_back(vec{coords[(size_t(i) * natoms + a) * 3], coords[(size_t(i) * natoms + a) * 3 + 1],
+					coords[(size_t(i) * natoms + a) * 3 + 2]});
+		out.push_back(o);
+	}
+	Output t{{double(ttag)}, te, {}};
+	for (int a = 0; a < natoms; a++)
+		t.coords.push_back(vec{tcoords[3 * a], tcoords[3 * a + 1], tcoords[3 * a + 2]});
+	add_to_output_container(out, t, min_rmsd, size_t(max_size));
+	for (size_t i = 0; i < out.size(); i++)
+	{
+		e[i] = out[i].e;
+		tag[i] = int(out[i].c[0]);
+		for (int a = 0; a < natoms; a++)
+			for (int d = 0; d < 3; d++)
+				coords[(i * natoms + a) * 3 + d] = out[i].coords[size_t(a)][d];
+	}
+	return int(out.size());
+}
 }
 
 extern "C" {

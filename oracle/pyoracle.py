@@ -53,6 +53,8 @@ def _load():
         "orc_visited_check": (C.c_int, [C.c_int, C.c_int, F64P, F64P, F64P, F64P, C.c_double, F64P]),
         "orc_mc_chain": (C.c_int, [vp, C.c_int, vp, C.c_int, C.c_int, F64P, U32P, C.c_int, F64P, F64P, F64P,
                                    I64P]),
+        "orc_container_add": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, F64P, F64P, I32P, C.c_double,
+                                        F64P, C.c_int]),
         "orc_refine": (C.c_double, [vp, C.c_int, F64P, C.c_int, F64P, F64P]),
         "orc_refine_opts": (C.c_double, [vp, C.c_int, F64P, C.c_int, C.c_int, F64P, F64P]),
         "orc_dock": (C.c_int, [vp, C.c_int, vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double,
@@ -83,6 +85,22 @@ def visited_check(points, energies, grads, x, f, g):
     n = x.size
     return bool(lib.orc_visited_check(n, len(energies), _p(points, F64P), _p(energies, F64P), _p(grads, F64P),
                                       _p(x, F64P), float(f), _p(g, F64P)))
+
+
+def container_add(poses, pose, max_size=20, min_rmsd=1.0):
+    """add_to_output_container: poses is a list of (tag, e, coords[natoms][3]),
+    pose one more; returns the new list of (tag, e, coords)"""
+    natoms = len(pose[2])
+    n = len(poses)
+    e = np.zeros(max_size + 1)
+    xyz = np.zeros((max_size + 1, natoms, 3))
+    tag = np.zeros(max_size + 1, np.int32)
+    for i, (t, en, x) in enumerate(poses):
+        tag[i], e[i], xyz[i] = t, en, x
+    tx = np.ascontiguousarray(pose[2], dtype=np.float64)
+    k = lib.orc_container_add(n, natoms, max_size, min_rmsd, _p(e, F64P), _p(xyz, F64P), _p(tag, I32P), pose[1],
+                              _p(tx, F64P), pose[0])
+    return [(int(tag[i]), float(e[i]), xyz[i].copy()) for i in range(k)]
 
 
 def bfgs_quadratic(A, b, x0, maxiters, accurate=False):
