@@ -1704,9 +1704,18 @@ void run_eval_noalloc(sgpu_ctx* c, const sgpu_eval_request_dev* q)
 	const int n = q->nconfs;
 	require(n > 0 && q->model && q->conf && q->e, "eval request incomplete");
 	const WavePlan& plan = b.h.plan;
-	require(q->conf_stride >= plan.cmax, "conf stride below the batch's largest conf");
-	require(!q->g || q->change_stride >= plan.gmax, "change stride below the batch's largest change size");
-	require(!q->coords || q->coord_stride >= 3 * plan.nmax, "coord stride below 3 x the batch's largest model");
+	// the caller's strides cover the batch's largest model (its own sizes,
+	// not the padded widths of the device plan)
+	int csize = 0, gsize = 0, natoms = 0;
+	for (const ModelHdr& h : b.h.hdr)
+	{
+		csize = std::max(csize, h.conf_size);
+		gsize = std::max(gsize, h.change_size);
+		natoms = std::max(natoms, h.natoms);
+	}
+	require(q->conf_stride >= csize, "conf stride below the batch's largest conf");
+	require(!q->g || q->change_stride >= gsize, "change stride below the batch's largest change size");
+	require(!q->coords || q->coord_stride >= 3 * natoms, "coord stride below 3 x the batch's largest model");
 	const NoallocLayout L = noalloc_layout(n, plan.cmax, plan.gmax, plan.nmax, plan.pmax, sizeof(T));
 	if (L.bytes > size_t(SGPU_NOALLOC_SCRATCH_BYTES))
 		throw Fail{SGPU_E_INVALID, "request needs " + std::to_string(L.bytes) + " bytes of scratch, above the " +
