@@ -258,6 +258,8 @@ def main():
                          "stopping, two-pass screening), compared with the default search on the same ligands")
     ap.add_argument("--no-compare", action="store_true",
                     help="with --fast: skip the default-search run that the fast screen is compared with")
+    ap.add_argument("--no-tune", action="store_true", help="skip the launch-plan autotuner (default plans)")
+    ap.add_argument("--tune-steps", type=int, default=16, help="autotuner: Monte Carlo steps per probe")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only a 3 x 1 GiB device copy (the reference for HBM byte counters)")
     args = ap.parse_args()
@@ -340,6 +342,15 @@ def main():
         warm.dock(warm_ligs)
     log(f"warmup: {args.warmup} small launch(es) of {len(warm_ligs)} ligands at 1/64 of their steps")
     del warm
+    # the launch-plan autotuner on the screen's own chains (untimed): a few
+    # Monte Carlo steps per candidate plan and size class; the timed steps
+    # use the fastest plans
+    tuned = None
+    if not args.no_tune:
+        t = time.perf_counter()
+        tuned = dock.tune(ligs, probe_steps=args.tune_steps)
+        log(f"autotuner: {time.perf_counter() - t:.1f}s, plans "
+            f"{[(c['class'], c['chosen']['waves'], c['chosen']['persistent']) for c in tuned]}")
     # timed steps: each is the whole screen; run as many of the requested
     # steps as the time budget holds, leaving room for the baselines
     times = []
@@ -451,9 +462,13 @@ def main():
                              "(calibrated FETCH_SIZE + WRITE_SIZE bytes per launch) from " + TRAFFIC_FILE +
                              " when this is the profiled workload, else null"},
         "cpu_baseline": cpu,
+        "launch_plan": {"autotuned": tuned is not None, "classes": tuned,
+                        "note": "per size class (pairs <= 256, <= 512, <= 1024, more): chains per workgroup and "
+                                "one-shot or persistent work-queue grid, each candidate's probe time in ms"},
         "plain_fp64": plain,
         "speedup_vs_plain_fp64": (value / plain["value"]) if plain else None,
         "top_score_mean": float(np.nanmean(top)),
+        "fp64_error_top_pose": fp64_error_summary(results),
     }
     if accuracy:
         line["top_pose_accuracy"] = accuracy
@@ -463,6 +478,22 @@ def main():
         line["roofline"] = None
         line["fast"] = fast
     print(json.dumps(line), flush=True)
+
+
+def fp64_error_summary(results):
+    """per-ligand FP64 error of an FP32 or mixed screen: each ligand's top
+    pose, its refined energy in the search precision against the FP64 value
+    of the same function at the same conf (None for an FP64 screen)"""
+    err = np.array([r["fp64_error"] for r in results if r.get("fp64_error") is not None])
+    if not len(err):
+        return None
+    rel = np.array([r["fp64_error"] / max(1.0, abs(r["e_fp64"])) for r in results
+                    if r.get("fp64_error") is not None])
+    return {"ligands": int(len(err)), "median_kcal": float(np.median(err)), "p99_kcal": float(np.percentile(err, 99)),
+            "max_kcal": float(err.max()), "max_rel": float(rel.max()),
+            "note": "|E_search - E_FP64| of each ligand's top pose: the refined energy in the search precision "
+                    "against model::eval_deriv in FP64 (non_cache back end, bit-equal to the CPU oracle) at the "
+                    "same conf"}
 
 
 def fast_vs_full(sg, args, log, rec, ligs, gd, hyd, prec, chunks, top_fast):

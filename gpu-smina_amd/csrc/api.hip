@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <array>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -200,7 +201,7 @@ struct sgpu_ctx
 	std::string err;
 	Tables tab;
 	std::map<int, std::unique_ptr<Receptor>> receptors;
-	std::map<int, std::unique_ptr<Grid>> grids;
+	std::map<int, std::shared_ptr<Grid>> grids; // shared between contexts by sgpu_share_grid
 	std::map<int, std::unique_ptr<Batch>> batches;
 	int next_id = 1;
 	std::map<std::string, double> last_ms;
@@ -212,6 +213,27 @@ struct sgpu_ctx
 	void* progress_user = nullptr;
 	// this device's copy of the allocation-free entry point's scratch
 	unsigned char* noalloc_scratch = nullptr;
+	// Monte Carlo launch plans per size class: chains per workgroup and the
+	// persistent form, as the autotuner measured them (keyed by precision,
+	// kernel variant and class), and a one-call override the autotuner's
+	// probes set
+	struct ClassPlan
+	{
+		int waves = 0;
+		bool persist = false;
+	};
+	std::map<std::array<int, 3>, ClassPlan> tuned;
+	ClassPlan plan_override[NCLASS];
+	bool override_on[NCLASS] = {};
+	// the user grid (--user_grid): its geometry and corner-packed values
+	struct UserGrid
+	{
+		bool on = false;
+		int dim[3] = {0, 0, 0};
+		double init[3], factor[3], finv[3], dm1[3];
+		double map_slope = 1e6;
+		DBuf cells32, cells64;
+	} ug;
 };
 
 namespace
@@ -587,6 +609,7 @@ int plan_bytes(const WavePlan& p, int prec, int waves)
 // the precision of a batch's model evaluation (and so of its maps)
 int eval_precision(int prec) { return prec == SGPU_FP64 ? SGPU_FP64 : SGPU_FP32; }
 constexpr int LDS_LIMIT = 160 * 1024;
+constexpr int LDS_STATIC = 64; // the persistent Monte Carlo kernel's own LDS (its work item)
 
 template<typename T>
 TablePtrs<T> table_ptrs(const sgpu_ctx* c)
@@ -637,6 +660,28 @@ void grid_geometry(const sgpu_grid_dims& gd, int* dim, double* init, double* fac
 		factor[i] = dm1[i] / range;
 		finv[i] = 1 / factor[i];
 	}
+}
+
+// the user grid as a one-map GridPtrs (slot 0); has_ug 0 without one
+template<typename T>
+void attach_user_grid(const sgpu_ctx* c, EvalCtx<T>& ec)
+{
+	ec.has_ug = c->ug.on ? 1 : 0;
+	if (!c->ug.on)
+		return;
+	GridPtrs<T>& p = ec.ug;
+	p.cells = (sizeof(T) == 8 ? c->ug.cells64 : c->ug.cells32).template as<T>();
+	p.slot_of = nullptr;
+	for (int d = 0; d < 3; d++)
+	{
+		p.dim[d] = c->ug.dim[d];
+		p.init[d] = c->ug.init[d];
+		p.factor[d] = c->ug.factor[d];
+		p.factor_inv[d] = c->ug.finv[d];
+		p.dim_fl_minus_1[d] = c->ug.dm1[d];
+	}
+	p.ncells = (p.dim[0] - 1) * (p.dim[1] - 1) * (p.dim[2] - 1);
+	p.slope = 0;
 }
 
 void check_gd(const sgpu_grid_dims* gd)
@@ -870,6 +915,8 @@ void sgpu_destroy(sgpu_ctx* c)
 	c->scratch.reset();
 	c->scratch2.reset();
 	c->err_flag.reset();
+	c->ug.cells32.reset();
+	c->ug.cells64.reset();
 	(void)hipStreamSynchronize(c->stream);
 	(void)hipEventDestroy(c->ev0);
 	(void)hipEventDestroy(c->ev1);
@@ -895,6 +942,14 @@ sgpu_status sgpu_device_info(sgpu_ctx* c, char* name, int name_len, int* cus)
 			std::snprintf(name, size_t(name_len), "%s (%s)", p.name, p.gcnArchName);
 		if (cus)
 			*cus = p.multiProcessorCount;
+	});
+}
+
+sgpu_status sgpu_device_memory(sgpu_ctx* c, size_t* free_bytes, size_t* total_bytes)
+{
+	return guarded(c, [&]() {
+		require(free_bytes && total_bytes, "no output");
+		hip_check(hipMemGetInfo(free_bytes, total_bytes), "hipMemGetInfo");
 	});
 }
 
@@ -949,6 +1004,41 @@ sgpu_status sgpu_upload_tables(sgpu_ctx* c, const sgpu_tables* t)
 	});
 }
 
+sgpu_status sgpu_set_user_grid(sgpu_ctx* c, const sgpu_user_grid* ug)
+{
+	return guarded(c, [&]() {
+		if (!ug)
+		{
+			c->ug.on = false;
+			return;
+		}
+		check_gd(&ug->gd);
+		require(ug->data != nullptr, "user grid values missing");
+		require(ug->map_slope >= 0, "user grid map slope");
+		sgpu_ctx::UserGrid& u = c->ug;
+		grid_geometry(ug->gd, u.dim, u.init, u.factor, u.finv, u.dm1);
+		u.map_slope = ug->map_slope;
+		// corner-packed cells, as the receptor maps (x fastest)
+		const int d0 = u.dim[0], d1 = u.dim[1], d2 = u.dim[2];
+		auto at = [&](int x, int y, int z) { return ug->data[size_t(x) + size_t(d0) * (size_t(y) + size_t(d1) * z)]; };
+		std::vector<double> cells(size_t(d0 - 1) * (d1 - 1) * (d2 - 1) * 8);
+		size_t k = 0;
+		for (int z = 0; z < d2 - 1; z++)
+			for (int y = 0; y < d1 - 1; y++)
+				for (int x = 0; x < d0 - 1; x++)
+				{
+					const double v[8] = {at(x, y, z), at(x + 1, y, z), at(x, y + 1, z), at(x + 1, y + 1, z),
+							at(x, y, z + 1), at(x + 1, y, z + 1), at(x, y + 1, z + 1), at(x + 1, y + 1, z + 1)};
+					for (int j = 0; j < 8; j++)
+						cells[k++] = v[j];
+				}
+		upload(u.cells64, cells, c->stream);
+		upload(u.cells32, cast<float>(cells), c->stream);
+		hip_check(hipStreamSynchronize(c->stream), "user grid");
+		u.on = true;
+	});
+}
+
 sgpu_status sgpu_upload_receptor(sgpu_ctx* c, const sgpu_receptor* r, int* id)
 {
 	return guarded(c, [&]() {
@@ -977,7 +1067,7 @@ sgpu_status sgpu_build_grids(sgpu_ctx* c, int receptor_id, const sgpu_grid_dims*
 		require(grid_id && types && ntypes > 0 && ntypes <= 16, "1..16 map types");
 		require(precision == SGPU_FP32 || precision == SGPU_FP64, "invalid precision");
 		Receptor& r = find_receptor(c, receptor_id);
-		std::unique_ptr<Grid> g(new Grid);
+		std::shared_ptr<Grid> g(new Grid);
 		g->receptor = receptor_id;
 		g->precision = precision;
 		g->gd = *gd;
@@ -1042,6 +1132,11 @@ sgpu_status sgpu_build_grids(sgpu_ctx* c, int receptor_id, const sgpu_grid_dims*
 		a.cell_hi = dhi.as<double>();
 		a.types = dtypes.as<int>();
 		a.ntypes = ntypes;
+		EvalCtx<double> ugc{};
+		attach_user_grid(c, ugc);
+		a.has_ug = ugc.has_ug;
+		a.ug = ugc.ug;
+		a.ug_slope = c->ug.map_slope;
 		const int ncells = a.cell_dim[0] * a.cell_dim[1] * a.cell_dim[2];
 		dstart.alloc(size_t(ncells + 1) * sizeof(int));
 		a.cell_start = dstart.as<int>();
@@ -1075,6 +1170,29 @@ sgpu_status sgpu_build_grids(sgpu_ctx* c, int receptor_id, const sgpu_grid_dims*
 		hip_check(hipStreamSynchronize(c->stream), "build grids");
 		*grid_id = c->next_id++;
 		c->grids[*grid_id] = std::move(g);
+	});
+}
+
+sgpu_status sgpu_share_grid(sgpu_ctx* dst, sgpu_ctx* src, int src_grid, int* grid_id)
+{
+	if (!src || src == dst)
+		return SGPU_E_INVALID;
+	std::shared_ptr<Grid> g;
+	{
+		std::lock_guard<std::mutex> lock(src->mu);
+		auto it = src->grids.find(src_grid);
+		if (it == src->grids.end())
+		{
+			src->err = "grid not found";
+			return SGPU_E_NOT_FOUND;
+		}
+		g = it->second;
+	}
+	return guarded(dst, [&]() {
+		require(grid_id != nullptr, "no grid id");
+		require(src->device == dst->device, "the contexts are on different devices");
+		*grid_id = dst->next_id++;
+		dst->grids[*grid_id] = g;
 	});
 }
 
@@ -1161,6 +1279,7 @@ void run_eval(sgpu_ctx* c, const sgpu_eval_request* q)
 	}
 	EvalCtx<T> ec{};
 	ec.tab = table_ptrs<T>(c);
+	attach_user_grid(c, ec);
 	for (int d = 0; d < 3; d++)
 		ec.v[d] = T(q->v[d]);
 	ensure_scratch(c->err_flag, sizeof(int));
@@ -1224,62 +1343,75 @@ void run_eval(sgpu_ctx* c, const sgpu_eval_request* q)
 		throw Fail{SGPU_E_ATOM_OVERLAP, "Ligand atom exactly overlaps receptor atom.  I can't deal with this."};
 }
 
-// Early stopping between the checkpoint launches (stop_agreeing,
-// src/gpu/gpu_kernels_impl.cuh:607-657): each model's search units take the
-// best energy of their chains; a model stops once at least max(2, ceil(
-// fraction x units)) units are within eps of its best.  A stopped model's
-// chains get no further steps (step limit 0: every later part is empty).
-template<typename S>
-void stop_agreeing(sgpu_ctx* c, const sgpu_mc_params& p, const sgpu_chains* ch, const ChainLayout& L,
-		const DBuf& dchains, std::vector<int>& steps_now, std::vector<char>& stopped, DBuf& dsteps)
+// the size class of a model in a Monte Carlo launch, by its pair count
+int mc_class(const ModelHdr& h)
 {
-	const int n = ch->nchains;
-	std::vector<S> best(static_cast<size_t>(n));
-	hip_check(hipMemcpy2DAsync(best.data(), sizeof(S), dchains.as<char>() + L.off_scal + sizeof(S), L.bytes,
-					  sizeof(S), size_t(n), hipMemcpyDeviceToHost, c->stream),
-			"best energies");
-	hip_check(hipStreamSynchronize(c->stream), "best energies");
-	std::map<int, std::map<int, S>> unit_best;
-	std::map<int, int> chains_of_model;
-	for (int i = 0; i < n; i++)
+	const int np = h.npairs;
+	return np <= 256 ? 0 : np <= 512 ? 1 : np <= 1024 ? 2 : 3;
+}
+
+// the Monte Carlo launch's work-queue counters, cleared by a kernel (a
+// memset node would sit in the captured graph)
+__global__ void mc_queue_init(int* q, int n)
+{
+	const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+	if (i < n)
+		q[i] = 0;
+}
+
+// stop_agreeing on the device, one thread per model: the best energy of
+// each search unit (the lowest over its chains' bests), the lowest of them,
+// and the units within eps of it; with at least need[m] agreeing, the model
+// stops for good (its chains' step limits become 0)
+template<typename S>
+__global__ void early_stop_kernel(int nmodels, const int* mstart, const int* mchain, const int* munit,
+		const int* nunits, const int* need, const char* chains, size_t state_bytes, size_t off_best, S eps,
+		int* stopped, int* steps)
+{
+	const int m = int(blockIdx.x * blockDim.x + threadIdx.x);
+	if (m >= nmodels || stopped[m] || nunits[m] < 2)
+		return;
+	auto best = [&](int k) { return *reinterpret_cast<const S*>(chains + size_t(mchain[k]) * state_bytes + off_best); };
+	S lowest = 0;
+	bool have = false;
+	for (int u = 0; u < nunits[m]; u++)
 	{
-		const int m = ch->model[i];
-		if (stopped[size_t(m)])
-			continue;
-		const int u = ch->unit ? ch->unit[i] : chains_of_model[m]++;
-		auto it = unit_best[m].find(u);
-		if (it == unit_best[m].end())
-			unit_best[m][u] = best[size_t(i)];
-		else
-			it->second = std::min(it->second, best[size_t(i)]);
-	}
-	bool any = false;
-	for (auto& kv : unit_best)
-	{
-		const auto& ub = kv.second;
-		if (ub.size() < 2)
-			continue;
-		S lowest = ub.begin()->second;
-		for (auto& u : ub)
-			lowest = std::min(lowest, u.second);
-		int agree = 0;
-		for (auto& u : ub)
-			agree += (u.second - lowest <= S(p.early_stop_eps));
-		const int need = std::max(2, int(std::ceil(p.early_stop_fraction * double(ub.size()) - 1e-9)));
-		if (agree >= need)
+		S ub = 0;
+		bool any = false;
+		for (int k = mstart[m]; k < mstart[m + 1]; k++)
+			if (munit[k] == u)
+			{
+				const S b = best(k);
+				ub = any ? (b < ub ? b : ub) : b;
+				any = true;
+			}
+		if (any)
 		{
-			stopped[size_t(kv.first)] = 1;
-			any = true;
+			lowest = have ? (ub < lowest ? ub : lowest) : ub;
+			have = true;
 		}
 	}
-	if (!any)
-		return;
-	for (int i = 0; i < n; i++)
-		if (stopped[size_t(ch->model[i])])
-			steps_now[size_t(i)] = 0;
-	hip_check(hipMemcpyAsync(dsteps.p, steps_now.data(), steps_now.size() * sizeof(int), hipMemcpyHostToDevice,
-					  c->stream),
-			"step limits");
+	int agree = 0;
+	for (int u = 0; u < nunits[m]; u++)
+	{
+		S ub = 0;
+		bool any = false;
+		for (int k = mstart[m]; k < mstart[m + 1]; k++)
+			if (munit[k] == u)
+			{
+				const S b = best(k);
+				ub = any ? (b < ub ? b : ub) : b;
+				any = true;
+			}
+		if (any && ub - lowest <= eps)
+			agree++;
+	}
+	if (agree >= need[m])
+	{
+		stopped[m] = 1;
+		for (int k = mstart[m]; k < mstart[m + 1]; k++)
+			steps[mchain[k]] = 0;
+	}
 }
 
 template<typename T, typename S>
@@ -1370,10 +1502,7 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 	std::map<int, std::vector<int>> by_model;
 	for (int i = 0; i < n; i++)
 		by_model[ch->model[i]].push_back(i);
-	auto class_of = [&](int mi) {
-		const int np = b.h.hdr[mi].npairs;
-		return np <= 256 ? 0 : np <= 512 ? 1 : np <= 1024 ? 2 : 3;
-	};
+	auto class_of = [&](int mi) { return mc_class(b.h.hdr[mi]); };
 	struct Blk
 	{
 		double cost;
@@ -1386,6 +1515,9 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 		WavePlan plan;
 		int waves = 1;
 		int resident = 0; // waves per CU
+		int grid = 0;     // workgroups of the class's launch
+		bool persist = false; // persistent grid over a work queue
+		bool tuned = false;   // waves and form from the autotuner
 		std::vector<Blk> blks;
 		size_t block_off = 0, slot_off = 0;
 	};
@@ -1406,12 +1538,12 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 		// workgroup
 		k.waves = 0;
 		int best = 0;
+		const int variant = c->tab.kind == 1 ? 2 : p->visited ? 1 : 0;
 		for (int w = 1; w <= std::min(MAX_CHAINS_PER_GROUP, maxper); w++)
 		{
 			const int bytes = plan_bytes(k.plan, b.precision, w);
-			if (bytes > LDS_LIMIT)
+			if (bytes + LDS_STATIC > LDS_LIMIT)
 				break;
-			const int variant = c->tab.kind == 1 ? 2 : p->visited ? 1 : 0;
 			const int resident = w * mc_blocks_per_cu<T, S>(w, size_t(bytes), variant);
 			if (resident >= best && resident > 0)
 			{
@@ -1421,6 +1553,26 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 		}
 		k.resident = best;
 		require(k.waves > 0, "chain state exceeds LDS");
+		// a probe's override, else the tuned plan of this class, where it fits
+		const int ki = int(&k - cls);
+		const sgpu_ctx::ClassPlan* cp = nullptr;
+		if (c->override_on[ki])
+			cp = &c->plan_override[ki];
+		else
+		{
+			auto it = c->tuned.find({b.precision, variant, ki});
+			if (it != c->tuned.end())
+				cp = &it->second;
+		}
+		if (cp && cp->waves >= 1 && cp->waves <= std::min(MAX_CHAINS_PER_GROUP, maxper) &&
+				plan_bytes(k.plan, b.precision, cp->waves) + LDS_STATIC <= LDS_LIMIT &&
+				mc_blocks_per_cu<T, S>(cp->waves, size_t(plan_bytes(k.plan, b.precision, cp->waves)), variant,
+						cp->persist) > 0)
+		{
+			k.waves = cp->waves;
+			k.persist = cp->persist;
+			k.tuned = true;
+		}
 		if (std::getenv("SGPU_LOG_PLAN"))
 		{
 			const int bytes = plan_bytes(k.plan, b.precision, k.waves);
@@ -1464,6 +1616,7 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 	upload(dmax, std::vector<int>(ch->maxiters, ch->maxiters + n), c->stream);
 	EvalCtx<T> ec{};
 	ec.tab = table_ptrs<T>(c);
+	attach_user_grid(c, ec);
 	ec.backend = 0;
 	ec.grid = grid_ptrs<T>(g, p->cache_slope);
 	ec.slope = T(p->cache_slope);
@@ -1497,50 +1650,170 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 		parts = p->early_stop_checkpoints;
 		part0 = std::min(parts, p->early_stop_first);
 	}
-	std::vector<int> steps_now(ch->num_steps, ch->num_steps + n);
-	std::vector<char> stopped(size_t(b.n), 0);
-	Timer tm(c, "mc");
-	for (int part = part0; part <= parts; part++)
+	// Persistent grids: each class's grid holds as many workgroups as the
+	// device keeps resident at once for its plan; they take the class's
+	// items (costliest first) from a work-queue counter, one counter per
+	// class and part, cleared by a kernel at the start.
+	hipDeviceProp_t prop;
+	hip_check(hipGetDeviceProperties(&prop, c->device), "hipGetDeviceProperties");
+	const int variant = c->tab.kind == 1 ? 2 : p->visited ? 1 : 0;
+	// the persistent form unless it spills where the one-shot form does not
+	// (or a tuned plan says otherwise); SGPU_QUEUE=0 / 1 forces either
+	const char* qenv = std::getenv("SGPU_QUEUE");
+	const bool persist_default = qenv ? std::atoi(qenv) != 0
+	                                  : mc_private_bytes<T, S>(variant, true) <= mc_private_bytes<T, S>(variant, false);
+	for (Cls& k : cls)
+		if (!k.blks.empty())
+		{
+			if (!k.tuned)
+				k.persist = persist_default;
+			const int per_cu = mc_blocks_per_cu<T, S>(k.waves, size_t(plan_bytes(k.plan, b.precision, k.waves)),
+					variant, k.persist);
+			k.grid = k.persist ? std::max(1, std::min(int(k.blks.size()), per_cu * prop.multiProcessorCount))
+			                   : int(k.blks.size());
+		}
+	DBuf dqueue;
+	const int nqueue = sgpu_ctx::NCLASS * (parts + 1);
+	dqueue.alloc(size_t(nqueue) * sizeof(int));
+	// early stopping on the device (stop_agreeing, src/gpu/gpu_kernels_impl.cuh:
+	// 607-657): per model, its chains grouped by search unit, the units' count
+	// and the agreeing units needed, max(2, ceil(fraction x units))
+	DBuf dms, dmc, dmu, dnu, dneed, dstop;
+	if (early_stop)
 	{
-		const int lo = part == part0 ? 0 : part - 1, hi = part;
-		// everything enqueued on the main stream so far (the chain upload,
-		// and after a checkpoint the early-stop step limits) comes before
-		// this part's launches on every stream
-		hip_check(hipEventRecord(c->part_start, c->stream), "event");
-		std::vector<hipStream_t> side;
-		for (int k = sgpu_ctx::NCLASS - 1; k >= 0; k--)
+		std::vector<int> mstart(size_t(b.n) + 1, 0), mchain, munit, nunits(size_t(b.n), 0), need(size_t(b.n), 0);
+		for (int mi = 0; mi < b.n; mi++)
 		{
-			const Cls& cl = cls[k];
-			if (cl.blks.empty())
+			mstart[size_t(mi)] = int(mchain.size());
+			auto it = by_model.find(mi);
+			if (it == by_model.end())
 				continue;
-			hipStream_t st = c->stream;
-			bool first = true;
-			for (int j = k + 1; j < sgpu_ctx::NCLASS; j++)
-				if (!cls[j].blks.empty())
-					first = false;
-			if (!first)
+			std::map<int, int> dense;
+			int seq = 0;
+			for (int ci : it->second)
 			{
-				st = c->class_stream[side.size()];
-				side.push_back(st);
-				hip_check(hipStreamWaitEvent(st, c->part_start, 0), "stream wait");
+				const int u = ch->unit ? ch->unit[ci] : seq++;
+				auto d = dense.find(u);
+				if (d == dense.end())
+					d = dense.emplace(u, int(dense.size())).first;
+				mchain.push_back(ci);
+				munit.push_back(d->second);
 			}
-			hip_check(launch_mc<T, S>(ptrs<T>(b), ec, cl.plan, P, L, dchains.as<char>(), int(cl.blks.size()),
-							  cl.waves, dbm.as<int>() + cl.block_off, dsc.as<int>() + cl.slot_off, dsteps.as<int>(),
-							  dmax.as<int>(), lo, hi, parts, st),
-					"mc kernel");
+			nunits[size_t(mi)] = int(dense.size());
+			need[size_t(mi)] =
+					std::max(2, int(std::ceil(p->early_stop_fraction * double(dense.size()) - 1e-9)));
 		}
-		for (size_t j = 0; j < side.size(); j++)
+		mstart[size_t(b.n)] = int(mchain.size());
+		upload(dms, mstart, c->stream);
+		upload(dmc, mchain, c->stream);
+		upload(dmu, munit, c->stream);
+		upload(dnu, nunits, c->stream);
+		upload(dneed, need, c->stream);
+		upload(dstop, std::vector<int>(size_t(b.n), 0), c->stream);
+	}
+	// the progress of each part, for host nodes that report it
+	struct Progress
+	{
+		sgpu_ctx* c;
+		double frac;
+	};
+	std::vector<Progress> prog(size_t(parts) + 1);
+	auto report = [](void* user) {
+		const Progress* pr = static_cast<const Progress*>(user);
+		pr->c->progress_fn(pr->c->progress_user, pr->frac);
+	};
+	// The whole search (every part, the early-stopping rule between parts,
+	// the progress reports) is captured once into a HIP graph and launched
+	// as one: no host round trip between parts.
+	const bool use_graph = !std::getenv("SGPU_NO_GRAPH");
+	Timer tm(c, "mc");
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+	if (use_graph)
+		hip_check(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed), "graph capture");
+	auto enqueue = [&]() {
+		hipLaunchKernelGGL(mc_queue_init, dim3(unsigned((nqueue + 255) / 256)), dim3(256), 0, c->stream,
+				dqueue.as<int>(), nqueue);
+		hip_check(hipGetLastError(), "queue init");
+		for (int part = part0; part <= parts; part++)
 		{
-			hip_check(hipEventRecord(c->class_done[j], side[j]), "event");
-			hip_check(hipStreamWaitEvent(c->stream, c->class_done[j], 0), "stream wait");
+			const int lo = part == part0 ? 0 : part - 1, hi = part;
+			// everything enqueued on the main stream so far (the chain upload,
+			// and after a checkpoint the early-stop step limits) comes before
+			// this part's launches on every stream
+			hip_check(hipEventRecord(c->part_start, c->stream), "event");
+			std::vector<hipStream_t> side;
+			for (int k = sgpu_ctx::NCLASS - 1; k >= 0; k--)
+			{
+				const Cls& cl = cls[k];
+				if (cl.blks.empty())
+					continue;
+				hipStream_t st = c->stream;
+				bool first = true;
+				for (int j = k + 1; j < sgpu_ctx::NCLASS; j++)
+					if (!cls[j].blks.empty())
+						first = false;
+				if (!first)
+				{
+					st = c->class_stream[side.size()];
+					side.push_back(st);
+					hip_check(hipStreamWaitEvent(st, c->part_start, 0), "stream wait");
+				}
+				int* q = cl.persist ? dqueue.as<int>() + size_t(part) * sgpu_ctx::NCLASS + size_t(k) : nullptr;
+				hip_check(launch_mc<T, S>(ptrs<T>(b), ec, cl.plan, P, L, dchains.as<char>(), int(cl.blks.size()),
+								  cl.waves, dbm.as<int>() + cl.block_off, dsc.as<int>() + cl.slot_off,
+								  dsteps.as<int>(), dmax.as<int>(), lo, hi, parts, cl.grid, q, st),
+						"mc kernel");
+			}
+			for (size_t j = 0; j < side.size(); j++)
+			{
+				hip_check(hipEventRecord(c->class_done[j], side[j]), "event");
+				hip_check(hipStreamWaitEvent(c->stream, c->class_done[j], 0), "stream wait");
+			}
+			if (part < parts && early_stop)
+			{
+				hipLaunchKernelGGL((early_stop_kernel<S>), dim3(unsigned((b.n + 63) / 64)), dim3(64), 0, c->stream,
+						b.n, dms.as<int>(), dmc.as<int>(), dmu.as<int>(), dnu.as<int>(), dneed.as<int>(),
+						dchains.as<char>(), L.bytes, L.off_scal + sizeof(S), S(p->early_stop_eps), dstop.as<int>(),
+						dsteps.as<int>());
+				hip_check(hipGetLastError(), "early stop kernel");
+			}
+			if (part < parts && c->progress_fn)
+			{
+				prog[size_t(part)] = {c, double(hi) / double(parts)};
+				hip_check(hipLaunchHostFunc(c->stream, report, &prog[size_t(part)]), "progress");
+			}
 		}
-		if ((c->progress_fn || early_stop) && part < parts)
+	};
+	try
+	{
+		enqueue();
+	}
+	catch (...)
+	{
+		if (use_graph)
 		{
-			hip_check(hipStreamSynchronize(c->stream), "mc part");
-			if (early_stop)
-				stop_agreeing<S>(c, *p, ch, L, dchains, steps_now, stopped, dsteps);
-			if (c->progress_fn)
-				c->progress_fn(c->progress_user, double(hi) / double(parts));
+			hipStreamEndCapture(c->stream, &graph);
+			if (graph)
+				hipGraphDestroy(graph);
+		}
+		throw;
+	}
+	if (use_graph)
+	{
+		hip_check(hipStreamEndCapture(c->stream, &graph), "graph capture");
+		const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+		if (ie == hipSuccess)
+		{
+			const hipError_t le = hipGraphLaunch(exec, c->stream);
+			hipGraphExecDestroy(exec);
+			hipGraphDestroy(graph);
+			hip_check(le, "graph launch");
+		}
+		else
+		{
+			hipGraphDestroy(graph);
+			hip_check(ie, "graph instantiate");
 		}
 	}
 	tm.stop();
@@ -1570,6 +1843,116 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 		}
 		if (out->evals)
 			out->evals[i] = *reinterpret_cast<const int64_t*>(base + L.off_ev);
+	}
+}
+
+// The launch-plan autotuner: for each size class of the request, the
+// class's chains alone run probe_steps Monte Carlo steps from their start
+// state under every candidate plan (chains per workgroup 1, 2, 4, 8 where
+// LDS and the chains per ligand allow; one-shot or persistent grid), timed
+// with HIP events; the fastest plan is kept for the context's later
+// launches of that precision, kernel variant and class.  The chains'
+// trajectories do not depend on the plan, so the probes time the same work.
+template<typename T, typename S>
+void run_tune(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, const sgpu_chains* ch,
+		int probe_steps, sgpu_mc_tune_report* rep)
+{
+	Batch& b = find_batch(c, batch_id);
+	require(p && ch && ch->nchains > 0 && probe_steps >= 1 && rep, "tune request incomplete");
+	const int variant = c->tab.kind == 1 ? 2 : p->visited ? 1 : 0;
+	std::vector<int> members[sgpu_ctx::NCLASS];
+	for (int i = 0; i < ch->nchains; i++)
+	{
+		require(ch->model[i] >= 0 && ch->model[i] < b.n, "chain model out of range");
+		members[mc_class(b.h.hdr[ch->model[i]])].push_back(i);
+	}
+	*rep = sgpu_mc_tune_report{};
+	sgpu_mc_params probe = *p;
+	probe.early_stop_checkpoints = 0;
+	const sgpu_progress_fn saved_fn = c->progress_fn;
+	c->progress_fn = nullptr;
+	struct Restore
+	{
+		sgpu_ctx* c;
+		sgpu_progress_fn fn;
+		~Restore()
+		{
+			c->progress_fn = fn;
+			for (bool& o : c->override_on)
+				o = false;
+		}
+	} restore{c, saved_fn};
+	for (int k = 0; k < sgpu_ctx::NCLASS; k++)
+	{
+		if (members[k].empty())
+			continue;
+		const std::vector<int>& mem = members[k];
+		const int n = int(mem.size());
+		std::vector<int> model, steps(n, probe_steps), iters, unit;
+		std::vector<double> conf;
+		std::vector<uint32_t> rng;
+		std::map<int, int> per_model;
+		std::vector<int> models;
+		int hmax = 1;
+		for (int i : mem)
+		{
+			model.push_back(ch->model[i]);
+			iters.push_back(ch->maxiters[i]);
+			unit.push_back(ch->unit ? ch->unit[i] : 0);
+			conf.insert(conf.end(), ch->conf + size_t(i) * ch->conf_stride,
+					ch->conf + size_t(i + 1) * ch->conf_stride);
+			rng.insert(rng.end(), ch->rng + size_t(i) * 625, ch->rng + size_t(i + 1) * 625);
+			if (per_model[ch->model[i]]++ == 0)
+				models.push_back(ch->model[i]);
+			hmax = std::max(hmax, b.h.hdr[ch->model[i]].nheavy);
+		}
+		int maxper = 1;
+		for (auto& kv : per_model)
+			maxper = std::max(maxper, kv.second);
+		const WavePlan plan = plan_for(b.h, models);
+		sgpu_chains sub{n, model.data(), conf.data(), ch->conf_stride, rng.data(), steps.data(), iters.data(),
+				unit.data()};
+		const int maxp = p->num_saved_mins;
+		std::vector<int> cnt(n);
+		std::vector<double> oe(size_t(n) * maxp), oc(size_t(n) * maxp * ch->conf_stride),
+				ox(size_t(n) * maxp * 3 * hmax);
+		sgpu_chain_results res{maxp, ch->conf_stride, 3 * hmax, cnt.data(), oe.data(), oc.data(), ox.data(), nullptr};
+		std::vector<sgpu_ctx::ClassPlan> cands;
+		for (int w = 1; w <= std::min(MAX_CHAINS_PER_GROUP, maxper); w *= 2)
+		{
+			const int bytes = plan_bytes(plan, b.precision, w);
+			if (bytes + LDS_STATIC > LDS_LIMIT)
+				break;
+			for (int pers = 0; pers < 2; pers++)
+				if (mc_blocks_per_cu<T, S>(w, size_t(bytes), variant, pers != 0) > 0)
+					cands.push_back({w, pers != 0});
+		}
+		require(!cands.empty(), "no launch plan fits this class");
+		const int ci = rep->nclasses++;
+		rep->class_index[ci] = k;
+		c->override_on[k] = true;
+		double best_ms = 0;
+		for (int j = -1; j < int(cands.size()) && j < SGPU_TUNE_MAX_CANDIDATES; j++)
+		{
+			// j = -1: an untimed first run (code objects, buffers)
+			c->plan_override[k] = cands[size_t(std::max(0, j))];
+			run_mc<T, S>(c, batch_id, grid_id, &probe, &sub, &res);
+			if (j < 0)
+				continue;
+			const double ms = c->last_ms["mc"];
+			rep->cand_waves[ci][j] = cands[size_t(j)].waves;
+			rep->cand_persist[ci][j] = cands[size_t(j)].persist ? 1 : 0;
+			rep->cand_ms[ci][j] = ms;
+			rep->ncand[ci] = j + 1;
+			if (j == 0 || ms < best_ms)
+			{
+				best_ms = ms;
+				rep->chosen_waves[ci] = cands[size_t(j)].waves;
+				rep->chosen_persist[ci] = cands[size_t(j)].persist ? 1 : 0;
+			}
+		}
+		c->override_on[k] = false;
+		c->tuned[{b.precision, variant, k}] = {rep->chosen_waves[ci], rep->chosen_persist[ci] != 0};
 	}
 }
 
@@ -1604,6 +1987,7 @@ void run_refine(sgpu_ctx* c, const sgpu_refine_request* q)
 	Cells& cl = receptor_cells(c, r, *q->gd);
 	EvalCtx<T> ec{};
 	ec.tab = table_ptrs<T>(c);
+	attach_user_grid(c, ec);
 	ec.backend = 1;
 	ec.rec = receptor_ptrs<T>(r, cl);
 	for (int d = 0; d < 3; d++)
@@ -1620,18 +2004,19 @@ void run_refine(sgpu_ctx* c, const sgpu_refine_request* q)
 	for (int i = 0; i < n; i++)
 		for (int k = 0; k < b.h.hdr[q->model[i]].conf_size; k++)
 			conf[size_t(i) * cs + k] = S(q->conf[size_t(i) * q->conf_stride + k]);
-	DBuf dm, dc, dmi, de, dx, dev;
+	DBuf dm, dc, dmi, de, dx, dev, dtr;
 	upload(dm, std::vector<int>(q->model, q->model + n), c->stream);
 	upload(dc, conf, c->stream);
 	upload(dmi, std::vector<int>(q->maxiters, q->maxiters + n), c->stream);
 	de.alloc(size_t(n) * sizeof(S));
 	dx.alloc(size_t(n) * xs * sizeof(T));
 	dev.alloc(size_t(n) * sizeof(int64_t));
+	dtr.alloc(size_t(n) * sizeof(int));
 	ensure_scratch(c->scratch, size_t(n) * ps * sizeof(T));
 	Timer tm(c, "refine");
 	hip_check(launch_refine<T, S>(ptrs<T>(b), ec, b.h.plan, n, dm.as<int>(), dc.as<S>(), cs, dmi.as<int>(),
 					  q->accurate_line_search, q->early_term, de.as<S>(), dx.as<T>(), xs, c->scratch.as<T>(), ps,
-					  dev.as<int64_t>(), q->boxes ? dboxes.as<double>() : nullptr, c->stream),
+					  dev.as<int64_t>(), q->boxes ? dboxes.as<double>() : nullptr, dtr.as<int>(), c->stream),
 			"refine kernel");
 	tm.stop();
 	std::vector<S> e(n);
@@ -1641,6 +2026,9 @@ void run_refine(sgpu_ctx* c, const sgpu_refine_request* q)
 	hip_check(hipMemcpyAsync(e.data(), de.p, e.size() * sizeof(S), hipMemcpyDeviceToHost, c->stream), "download");
 	hip_check(hipMemcpyAsync(x.data(), dx.p, x.size() * sizeof(T), hipMemcpyDeviceToHost, c->stream), "download");
 	hip_check(hipMemcpyAsync(&flag, c->err_flag.p, sizeof(int), hipMemcpyDeviceToHost, c->stream), "download");
+	if (q->tries)
+		hip_check(hipMemcpyAsync(q->tries, dtr.p, size_t(n) * sizeof(int), hipMemcpyDeviceToHost, c->stream),
+				"download");
 	hip_check(hipStreamSynchronize(c->stream), "refine");
 	for (int i = 0; i < n; i++)
 	{
@@ -1656,6 +2044,83 @@ void run_refine(sgpu_ctx* c, const sgpu_refine_request* q)
 		throw Fail{SGPU_E_ATOM_OVERLAP, "Ligand atom exactly overlaps receptor atom.  I can't deal with this."};
 }
 
+template<typename T, typename S>
+void run_bfgs(sgpu_ctx* c, const sgpu_bfgs_request* q)
+{
+	Batch& b = find_batch(c, q->batch_id);
+	const int n = q->nconfs;
+	require(n > 0 && q->model && q->conf && q->maxiters && q->e, "bfgs request incomplete");
+	for (int i = 0; i < n; i++)
+	{
+		require(q->model[i] >= 0 && q->model[i] < b.n, "conf model out of range");
+		require(q->conf_stride >= b.h.hdr[q->model[i]].conf_size, "conf stride");
+		require(q->maxiters[i] >= 0, "maxiters");
+	}
+	EvalCtx<T> ec{};
+	ec.tab = table_ptrs<T>(c);
+	attach_user_grid(c, ec);
+	for (int d = 0; d < 3; d++)
+		ec.v[d] = T(q->v[d]);
+	ensure_scratch(c->err_flag, sizeof(int));
+	hip_check(hipMemsetAsync(c->err_flag.p, 0, sizeof(int), c->stream), "memset");
+	ec.error = c->err_flag.as<int>();
+	ec.slope = T(q->slope);
+	if (q->grid_id >= 0)
+	{
+		Grid& g = find_grid(c, q->grid_id);
+		require(g.precision == eval_precision(b.precision), "grid and model precision differ");
+		check_maps(b, g, q->model, n);
+		ec.backend = 0;
+		ec.grid = grid_ptrs<T>(g, q->slope);
+	}
+	else
+	{
+		check_gd(q->gd);
+		Receptor& r = find_receptor(c, q->receptor_id);
+		Cells& cl = receptor_cells(c, r, *q->gd);
+		ec.backend = 1;
+		ec.rec = receptor_ptrs<T>(r, cl);
+	}
+	const int cs = b.h.plan.cmax, ps = b.h.plan.pmax;
+	std::vector<S> conf(size_t(n) * cs, S(0));
+	for (int i = 0; i < n; i++)
+		for (int k = 0; k < b.h.hdr[q->model[i]].conf_size; k++)
+			conf[size_t(i) * cs + k] = S(q->conf[size_t(i) * q->conf_stride + k]);
+	DBuf dm, dc, dmi, de, dev;
+	upload(dm, std::vector<int>(q->model, q->model + n), c->stream);
+	upload(dc, conf, c->stream);
+	upload(dmi, std::vector<int>(q->maxiters, q->maxiters + n), c->stream);
+	de.alloc(size_t(n) * sizeof(S));
+	dev.alloc(size_t(n) * sizeof(int64_t));
+	ensure_scratch(c->scratch, size_t(n) * ps * sizeof(T));
+	Timer tm(c, "bfgs");
+	hip_check(launch_bfgs<T, S>(ptrs<T>(b), ec, b.h.plan, n, dm.as<int>(), dc.as<S>(), cs, dmi.as<int>(),
+					  q->accurate_line_search, q->early_term, de.as<S>(), c->scratch.as<T>(), ps, dev.as<int64_t>(),
+					  c->stream),
+			"bfgs kernel");
+	tm.stop();
+	std::vector<S> e(n);
+	std::vector<int64_t> ev(n);
+	int flag = 0;
+	hip_check(hipMemcpyAsync(conf.data(), dc.p, conf.size() * sizeof(S), hipMemcpyDeviceToHost, c->stream), "download");
+	hip_check(hipMemcpyAsync(e.data(), de.p, e.size() * sizeof(S), hipMemcpyDeviceToHost, c->stream), "download");
+	hip_check(hipMemcpyAsync(ev.data(), dev.p, ev.size() * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream),
+			"download");
+	hip_check(hipMemcpyAsync(&flag, c->err_flag.p, sizeof(int), hipMemcpyDeviceToHost, c->stream), "download");
+	hip_check(hipStreamSynchronize(c->stream), "bfgs");
+	for (int i = 0; i < n; i++)
+	{
+		const ModelHdr& h = b.h.hdr[q->model[i]];
+		for (int k = 0; k < h.conf_size; k++)
+			q->conf[size_t(i) * q->conf_stride + k] = double(conf[size_t(i) * cs + k]);
+		q->e[i] = double(e[i]);
+		if (q->evals)
+			q->evals[i] = ev[i];
+	}
+	if (flag)
+		throw Fail{SGPU_E_ATOM_OVERLAP, "Ligand atom exactly overlaps receptor atom.  I can't deal with this."};
+}
+
 void run_score(sgpu_ctx* c, const sgpu_score_request* q)
 {
 	Batch& b0 = find_batch(c, q->batch_id);
@@ -1663,7 +2128,7 @@ void run_score(sgpu_ctx* c, const sgpu_score_request* q)
 	Receptor& r = find_receptor(c, q->receptor_id);
 	check_gd(q->gd);
 	const int n = q->nconfs;
-	require(n > 0 && q->model && q->conf && q->e && (q->mode == 0 || q->mode == 1), "score request incomplete");
+	require(n > 0 && q->model && q->conf && q->e && q->mode >= 0 && q->mode <= 2, "score request incomplete");
 	for (int i = 0; i < n; i++)
 	{
 		require(q->model[i] >= 0 && q->model[i] < b.n, "model index out of range");
@@ -1672,6 +2137,7 @@ void run_score(sgpu_ctx* c, const sgpu_score_request* q)
 	Cells& cl = receptor_cells(c, r, *q->gd);
 	EvalCtx<double> ec{};
 	ec.tab = table_ptrs<double>(c);
+	attach_user_grid(c, ec);
 	ec.backend = 1;
 	ec.rec = receptor_ptrs<double>(r, cl);
 	ec.slope = q->slope;
@@ -1687,13 +2153,29 @@ void run_score(sgpu_ctx* c, const sgpu_score_request* q)
 	upload(dc, conf, c->stream);
 	de.alloc(size_t(n) * sizeof(double));
 	ensure_scratch(c->scratch2, size_t(n) * ps * sizeof(double));
+	int flag = 0;
 	Timer tm(c, "score");
-	hip_check(launch_score<double>(ptrs<double>(b), ec, b.h.plan, q->mode, n, dm.as<int>(), dc.as<double>(), cs,
-					  de.as<double>(), c->scratch2.as<double>(), ps, c->stream),
-			"score kernel");
+	if (q->mode == 2)
+	{
+		// model::eval_deriv's energy on the non_cache back end with the
+		// search's tables: the function the refinement minimises, in FP64
+		ensure_scratch(c->err_flag, sizeof(int));
+		hip_check(hipMemsetAsync(c->err_flag.p, 0, sizeof(int), c->stream), "memset");
+		ec.error = c->err_flag.as<int>();
+		hip_check(launch_eval<double>(ptrs<double>(b), ec, b.h.plan, n, dm.as<int>(), dc.as<double>(), cs,
+						  de.as<double>(), nullptr, 0, nullptr, 0, c->scratch2.as<double>(), ps, c->stream),
+				"eval kernel");
+		hip_check(hipMemcpyAsync(&flag, c->err_flag.p, sizeof(int), hipMemcpyDeviceToHost, c->stream), "download");
+	}
+	else
+		hip_check(launch_score<double>(ptrs<double>(b), ec, b.h.plan, q->mode, n, dm.as<int>(), dc.as<double>(), cs,
+						  de.as<double>(), c->scratch2.as<double>(), ps, c->stream),
+				"score kernel");
 	tm.stop();
 	hip_check(hipMemcpyAsync(q->e, de.p, size_t(n) * sizeof(double), hipMemcpyDeviceToHost, c->stream), "download");
 	hip_check(hipStreamSynchronize(c->stream), "score");
+	if (flag)
+		throw Fail{SGPU_E_ATOM_OVERLAP, "Ligand atom exactly overlaps receptor atom.  I can't deal with this."};
 }
 
 
@@ -1723,6 +2205,7 @@ void run_eval_noalloc(sgpu_ctx* c, const sgpu_eval_request_dev* q)
 	require(c->noalloc_scratch != nullptr, "hipGetSymbolAddress gave no scratch address for this device");
 	EvalCtx<T> ec{};
 	ec.tab = table_ptrs<T>(c);
+	attach_user_grid(c, ec);
 	for (int d = 0; d < 3; d++)
 		ec.v[d] = T(q->v[d]);
 	if (q->grid_id >= 0)
@@ -1798,6 +2281,21 @@ sgpu_status sgpu_eval_confs(sgpu_ctx* c, const sgpu_eval_request* q)
 	});
 }
 
+sgpu_status sgpu_run_bfgs(sgpu_ctx* c, const sgpu_bfgs_request* q)
+{
+	return guarded(c, [&]() {
+		require(q != nullptr, "no request");
+		require(c->tab.ok, "tables not uploaded");
+		Batch& b = find_batch(c, q->batch_id);
+		if (b.precision == SGPU_FP64)
+			run_bfgs<double, double>(c, q);
+		else if (b.precision == SGPU_MIXED)
+			run_bfgs<float, double>(c, q);
+		else
+			run_bfgs<float, float>(c, q);
+	});
+}
+
 sgpu_status sgpu_eval_confs_noalloc(sgpu_ctx* c, const sgpu_eval_request_dev* q)
 {
 	return guarded(c, [&]() {
@@ -1823,6 +2321,21 @@ sgpu_status sgpu_run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_pa
 			run_mc<float, double>(c, batch_id, grid_id, p, ch, out);
 		else
 			run_mc<float, float>(c, batch_id, grid_id, p, ch, out);
+	});
+}
+
+sgpu_status sgpu_tune_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, const sgpu_chains* ch,
+		int probe_steps, sgpu_mc_tune_report* out)
+{
+	return guarded(c, [&]() {
+		require(c->tab.ok, "tables not uploaded");
+		Batch& b = find_batch(c, batch_id);
+		if (b.precision == SGPU_FP64)
+			run_tune<double, double>(c, batch_id, grid_id, p, ch, probe_steps, out);
+		else if (b.precision == SGPU_MIXED)
+			run_tune<float, double>(c, batch_id, grid_id, p, ch, probe_steps, out);
+		else
+			run_tune<float, float>(c, batch_id, grid_id, p, ch, probe_steps, out);
 	});
 }
 

@@ -13,12 +13,15 @@
  * 2. points: one thread per grid point (x fastest, so a wavefront's probes
  *    share a cell and read the same atoms), FP64 sums of the fast table over
  *    the cell's atoms for every needed ligand type.
+ *    With a user grid (--user_grid), each point adds its value as
+ *    cache::populate does (src/lib/cache.cpp:204-205).
  * 3. pack: point maps -> corner-packed cells (8 corner values per cell).
  */
 #include <hip/hip_runtime.h>
 #include "launch.h"
 #include "atom_types.h"
 #include "spline.h"
+#include "gridterm.h"
 
 namespace sgpu
 {
@@ -175,9 +178,12 @@ __global__ __launch_bounds__(256) void grid_points_kernel(GridBuildArgs a, doubl
 					aff[j] += a.fast[size_t(tri_index_permissive(t1, tt[j])) * a.table_n + idx];
 		}
 	}
+	// the user grid at (x, y, z) as a location in A: the reference's own
+	// expression, kept as it is
+	const double uge = a.has_ug ? grid_value(a.ug, v3(double(x), double(y), double(z)), 1000.0, a.ug_slope) : 0.0;
 	for (int j = 0; j < MAX_GRID_TYPES; j++)
 		if (j < a.ntypes)
-			points[size_t(j) * npts + p] = aff[j];
+			points[size_t(j) * npts + p] = a.has_ug ? aff[j] + uge : aff[j];
 }
 
 template<typename T>

@@ -19,25 +19,36 @@ template<typename T>
 hipError_t launch_eval(const BatchPtrs<T>& B, const EvalCtx<T>& ec, const WavePlan& plan, int n, const int* model,
 		const T* conf, int conf_stride, T* e, T* g, int g_stride, T* x, int x_stride, T* scratch, int pstride,
 		hipStream_t st);
-// nblocks workgroups of `waves` (1..8) chains of one ligand each, running
-// the steps [ceil(steps x part_lo / parts), ceil(steps x part_hi / parts))
-// of every chain; T is the
+// nblocks workgroup items of `waves` (1..8) chains of one ligand each,
+// running the steps [ceil(steps x part_lo / parts), ceil(steps x part_hi /
+// parts)) of every chain: a persistent grid of `grid` workgroups taking
+// items from the work-queue counter *queue (cleared), or with queue null
+// one workgroup per item (grid = nblocks); T is the
 // evaluation's real type, S the search state's (kernels_mixed.hip: float,
 // double)
 template<typename T, typename S>
 hipError_t launch_mc(const BatchPtrs<T>& B, const EvalCtx<T>& ec, const WavePlan& plan, const McParams& P,
 		const ChainLayout& L, char* chains, int nblocks, int waves, const int* block_model, const int* slot_chain,
-		const int* chain_steps, const int* chain_maxiters, int part_lo, int part_hi, int parts, hipStream_t st);
+		const int* chain_steps, const int* chain_maxiters, int part_lo, int part_hi, int parts, int grid, int* queue,
+		hipStream_t st);
 // workgroups of `waves` chains with `lds` bytes of LDS that one CU holds at
 // once (the runtime's occupancy calculator: registers and LDS)
 template<typename T, typename S>
 // variant: 0 smina's search, 1 with QuickVina 2's history, 2 with the
 // spline tables (and the history)
-int mc_blocks_per_cu(int waves, size_t lds, int variant = 0);
+int mc_blocks_per_cu(int waves, size_t lds, int variant = 0, bool persist = false);
+// per-lane private (scratch) bytes of a Monte Carlo kernel form: register
+// spills, which the persistent form must not add
+template<typename T, typename S>
+size_t mc_private_bytes(int variant, bool persist);
 template<typename T, typename S>
 hipError_t launch_refine(const BatchPtrs<T>& B, const EvalCtx<T>& ec, const WavePlan& plan, int n, const int* model,
 		S* conf, int conf_stride, const int* maxiters, int accurate, int early_term, S* e, T* x, int x_stride,
-		T* scratch, int pstride, int64_t* evals, const double* boxes, hipStream_t st);
+		T* scratch, int pstride, int64_t* evals, const double* boxes, int* tries, hipStream_t st);
+template<typename T, typename S>
+hipError_t launch_bfgs(const BatchPtrs<T>& B, const EvalCtx<T>& ec, const WavePlan& plan, int n, const int* model,
+		S* conf, int conf_stride, const int* maxiters, int accurate, int early_term, S* e, T* scratch, int pstride,
+		int64_t* evals, hipStream_t st);
 // sgpu_eval_confs_noalloc: the caller's device arrays (double) and the
 // layout of the library's scratch region for n confs of a batch
 struct NoallocRequest
@@ -119,6 +130,12 @@ struct GridBuildArgs
 	double factor_inv[3];
 	const int* types;      // [ntypes] ligand types
 	int ntypes;
+	// the user grid, added at every point as cache::populate adds it
+	// (src/lib/cache.cpp:204-205): evaluated at the point's integer
+	// indices taken as a location, with the map slope and cap 1000
+	GridPtrs<double> ug;
+	int has_ug;
+	double ug_slope;
 };
 hipError_t launch_cell_count(const GridBuildArgs& a, hipStream_t st);
 hipError_t launch_cell_fill(const GridBuildArgs& a, hipStream_t st);

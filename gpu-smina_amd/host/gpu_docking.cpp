@@ -723,6 +723,9 @@ GpuDocking::GpuDocking(const GpuSettings& s, const sgpu_scoring& sf, const sgpu_
 		check(ctx_, sgpu_upload_tables(ctx_, &tt));
 	}
 	check(ctx_, sgpu_upload_receptor(ctx_, &rec, &receptor_));
+	if (s.user_grid)
+		check(ctx_, sgpu_set_user_grid(ctx_, s.user_grid));
+	s_.user_grid = nullptr; // the device holds its copy
 	check(ctx_, sgpu_set_progress(ctx_, s.progress_chunks, s.progress, s.progress_user));
 	check(ctx_, sgpu_mc_capacity(ctx_, s.precision, &capacity_));
 }
@@ -776,6 +779,17 @@ std::string GpuDocking::description() const
 	return std::string(name) + ", " + std::to_string(cus) + " CUs, " + (s_.precision == SGPU_FP64 ? "FP64" : s_.precision == SGPU_MIXED ? "FP32 evaluation, FP64 search" : "FP32");
 }
 
+void GpuDocking::adopt_grid(sgpu_ctx* store, int store_grid, const std::vector<int>& types)
+{
+	if (grid_ >= 0)
+		sgpu_release_grid(ctx_, grid_);
+	grid_ = -1;
+	check(ctx_, sgpu_share_grid(ctx_, store, store_grid, &grid_));
+	grid_types_ = types;
+	std::sort(grid_types_.begin(), grid_types_.end());
+	shared_grid_ = true;
+}
+
 void GpuDocking::ensure_grids(const std::vector<int>& types)
 {
 	bool have = grid_ >= 0;
@@ -783,6 +797,8 @@ void GpuDocking::ensure_grids(const std::vector<int>& types)
 		have = have && std::find(grid_types_.begin(), grid_types_.end(), t) != grid_types_.end();
 	if (have)
 		return;
+	if (shared_grid_)
+		throw std::invalid_argument("a ligand atom type is not among the served receptor's map types");
 	std::vector<int> all = grid_types_;
 	for (int t : types)
 		if (std::find(all.begin(), all.end(), t) == all.end())
@@ -868,6 +884,20 @@ double full_search_cutoff(std::vector<double> history, double fast_top, double m
 // in (0, 1) and at least 20 ligands seen, a first pass at fast_first of the
 // steps whose top scores join the history, then the full search for the
 // ligands within fast_top_margin of the history's best fast_top
+void GpuDocking::tune(std::vector<DockJob*>& jobs, int probe_steps, sgpu_mc_tune_report& report)
+{
+	if (probe_steps < 1)
+		throw std::invalid_argument("probe_steps must be positive");
+	tune_steps_ = probe_steps;
+	tune_report_ = &report;
+	struct Reset
+	{
+		GpuDocking* d;
+		~Reset() { d->tune_report_ = nullptr; }
+	} reset{this};
+	dock_pass(jobs, 0);
+}
+
 void GpuDocking::dock_batch(std::vector<DockJob*>& jobs)
 {
 	full_searches_ = jobs.size();
@@ -1032,6 +1062,58 @@ void GpuDocking::minimize_batch(std::vector<MinimizeJob*>& jobs)
 	check(ctx_, sgpu_release_models(ctx_, batch));
 }
 
+// the FP64 error of an FP32 or mixed search per job: its top pose's
+// refined energy against the FP64 value of the refinement's own function at
+// the same conf and box slope (sgpu_score mode 2)
+void GpuDocking::fp64_errors(DockJob* const* jobs, size_t nj, int batch, int cstride)
+{
+	for (size_t j = 0; j < nj; j++)
+		jobs[j]->fp64_error = -1;
+	if (s_.precision == SGPU_FP64)
+		return;
+	for (int tries = 0; tries < 5; tries++)
+	{
+		std::vector<int> model;
+		std::vector<double> conf;
+		std::vector<size_t> which;
+		for (size_t j = 0; j < nj; j++)
+		{
+			const auto& out = jobs[j]->out;
+			if (out.empty() || out[0].tries != tries)
+				continue;
+			model.push_back(int(j));
+			std::vector<double> c(size_t(cstride), 0.0);
+			std::copy(out[0].c.begin(), out[0].c.end(), c.begin());
+			conf.insert(conf.end(), c.begin(), c.end());
+			which.push_back(j);
+		}
+		if (model.empty())
+			continue;
+		std::vector<double> e(model.size());
+		sgpu_score_request sq{};
+		sq.batch_id = batch;
+		sq.receptor_id = receptor_;
+		sq.gd = &gd_;
+		sq.slope = 10 * std::pow(10.0, tries);
+		for (int k = 0; k < 3; k++)
+			sq.v[k] = 1000;
+		sq.mode = 2;
+		sq.nconfs = int(model.size());
+		sq.model = model.data();
+		sq.conf = conf.data();
+		sq.conf_stride = cstride;
+		sq.e = e.data();
+		check(ctx_, sgpu_score(ctx_, &sq));
+		for (size_t q = 0; q < which.size(); q++)
+		{
+			DockJob* job = jobs[which[q]];
+			job->e_search = job->out[0].e_search;
+			job->e_fp64 = e[q];
+			job->fp64_error = std::fabs(job->e_search - job->e_fp64);
+		}
+	}
+}
+
 void GpuDocking::dock_sub(DockJob* const* jobs, size_t nj, double budget)
 {
 	if (nj == 0)
@@ -1126,6 +1208,13 @@ void GpuDocking::dock_sub(DockJob* const* jobs, size_t nj, double budget)
 			ox(size_t(nchains) * maxp * hstride);
 	std::vector<int64_t> evals(nchains, 0);
 	sgpu_chain_results res{maxp, cstride, hstride, cnt.data(), oe.data(), oc.data(), ox.data(), evals.data()};
+	if (tune_report_)
+	{
+		// a tuning call: probe the launch plans on these chains, search nothing
+		check(ctx_, sgpu_tune_mc(ctx_, batch, grid_, &mp, &ch, tune_steps_, tune_report_));
+		sgpu_release_models(ctx_, batch);
+		return;
+	}
 	check(ctx_, sgpu_run_mc(ctx_, batch, grid_, &mp, &ch, &res));
 	// merge_output_containers with min_rmsd 2 (src/gpu/gpu_docking.cpp:576-598)
 	std::vector<int> pose_model, pose_iters;
@@ -1164,6 +1253,7 @@ void GpuDocking::dock_sub(DockJob* const* jobs, size_t nj, double budget)
 	if (np > 0)
 	{
 		std::vector<double> pe(np), px(size_t(np) * hstride);
+		std::vector<int> ptries(np, 0);
 		sgpu_refine_request rq{};
 		rq.batch_id = batch;
 		rq.receptor_id = receptor_;
@@ -1178,6 +1268,7 @@ void GpuDocking::dock_sub(DockJob* const* jobs, size_t nj, double budget)
 		rq.e = pe.data();
 		rq.coords = px.data();
 		rq.coord_stride = hstride;
+		rq.tries = ptries.data();
 		check(ctx_, sgpu_run_refine(ctx_, &rq));
 		for (int i = 0; i < np; i++)
 		{
@@ -1185,6 +1276,8 @@ void GpuDocking::dock_sub(DockJob* const* jobs, size_t nj, double budget)
 			const FlatModel& f = fms[pose_ref[i].first];
 			p.c.assign(pose_conf.begin() + size_t(i) * cstride, pose_conf.begin() + size_t(i) * cstride + f.view.conf_size);
 			p.e = pe[i];
+			p.e_search = pe[i];
+			p.tries = ptries[i];
 			p.coords.assign(px.begin() + size_t(i) * hstride, px.begin() + size_t(i) * hstride + 3 * f.nheavy);
 		}
 	}
@@ -1272,6 +1365,7 @@ void GpuDocking::dock_sub(DockJob* const* jobs, size_t nj, double budget)
 		}
 		out.swap(fin);
 	}
+	fp64_errors(jobs, nj, batch, cstride);
 	sgpu_release_models(ctx_, batch);
 	const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 	for (size_t j = 0; j < nj; j++)
@@ -1458,6 +1552,7 @@ sgpu_docking* sgpu_docking_create(const sgpu_dock_settings* s, const sgpu_mol* r
 		gs.progress_chunks = s->progress_chunks > 0 ? s->progress_chunks : 1;
 		gs.progress = s->progress;
 		gs.progress_user = s->progress_user;
+		gs.user_grid = s->user_grid;
 		d->terms.resize(SGPU_MAX_TERMS);
 		d->ci.resize(SGPU_MAX_CI);
 		sgpu_scoring sf;
@@ -1524,6 +1619,30 @@ int sgpu_docking_prepare(sgpu_docking* d, const sgpu_mol* const* ligands, int n)
 	}
 }
 
+int sgpu_docking_tune(sgpu_docking* d, const sgpu_mol* const* ligands, int n, int probe_steps,
+		sgpu_mc_tune_report* report)
+{
+	if (!d->d || !report)
+		return -1;
+	try
+	{
+		std::vector<GpuDocking::DockJob> jobs(n);
+		std::vector<GpuDocking::DockJob*> jp;
+		for (int i = 0; i < n; i++)
+		{
+			jobs[i].m = sgpu_mol_desc(ligands[i]);
+			jp.push_back(&jobs[i]);
+		}
+		d->d->tune(jp, probe_steps, *report);
+		return 0;
+	}
+	catch (std::exception& e)
+	{
+		d->err = e.what();
+		return -1;
+	}
+}
+
 int sgpu_dock(sgpu_docking* d, const sgpu_mol* const* ligands, int n, sgpu_dock_result* out)
 {
 	if (!d->d)
@@ -1545,6 +1664,9 @@ int sgpu_dock(sgpu_docking* d, const sgpu_mol* const* ligands, int n, sgpu_dock_
 				out[i].e[k] = jobs[i].out[k].e;
 			out[i].evals = jobs[i].evals;
 			out[i].seconds = jobs[i].seconds;
+			out[i].fp64_error = jobs[i].fp64_error;
+			out[i].e_search = jobs[i].e_search;
+			out[i].e_fp64 = jobs[i].e_fp64;
 			out[i].conf_size = 0;
 			if (out[i].confs && !jobs[i].out.empty())
 			{

@@ -30,6 +30,10 @@ struct Pose
 	std::vector<double> c;
 	double e = 0;
 	std::vector<double> coords;
+	// the refinement's energy in the search precision, and the box-slope
+	// increases it took (its last BFGS run used slope 10 x 10^tries)
+	double e_search = 0;
+	int tries = 0;
 };
 
 // smina's model after flattening (FlatModel, src/gpu/gpu_flatten_data.h)
@@ -125,6 +129,9 @@ struct GpuSettings
 	int progress_chunks = 1;
 	sgpu_progress_fn progress = nullptr;
 	void* progress_user = nullptr;
+	// --user_grid: copied to the device when the docking object is made
+	// (its maps, direct receptor term and rescoring then carry it)
+	const sgpu_user_grid* user_grid = nullptr;
 };
 
 // the settings --fast selects (src/main/main.cpp:1451-1461, :1426-1431):
@@ -167,11 +174,22 @@ public:
 		std::vector<Pose> out; // final, rescored poses (finish_docking order)
 		int64_t evals = 0;
 		double seconds = 0;
+		// FP32 and mixed searches: the top pose's refined energy in the
+		// search precision against the FP64 value of the same function at
+		// the same conf (model::eval_deriv on the non_cache back end, bit-
+		// equal to the CPU's), |e_search - e_fp64|; -1 for an FP64 search
+		// or a job without poses
+		double fp64_error = -1;
+		double e_search = 0, e_fp64 = 0;
 	};
 	// for each job: parallel_mc seeded with rng(seed) (do_search), refine of
 	// every pose, FP64 rescoring, clustering (src/gpu/gpu_docking.cpp:756-831,
 	// src/main/main.cpp:256-324)
 	void dock_batch(std::vector<DockJob*>& jobs);
+	// the launch-plan autotuner (sgpu_tune_mc) on the chains these jobs'
+	// search would run, probe_steps Monte Carlo steps per candidate plan;
+	// the context keeps the fastest plan per size class for later searches
+	void tune(std::vector<DockJob*>& jobs, int probe_steps, sgpu_mc_tune_report& report);
 	// the ligands of the last dock_batch that got the full search of a
 	// two-pass screen
 	size_t full_searches() const { return full_searches_; }
@@ -198,6 +216,10 @@ public:
 	void prepare(const std::vector<const sgpu_model_desc*>& models);
 	// receptor maps for these ligand atom types
 	void prepare_types(const std::vector<int>& types) { ensure_grids(types); }
+	// use the maps `store` holds (grid store_grid, covering `types`) instead of
+	// building them here: one copy per device for every context serving them
+	void adopt_grid(sgpu_ctx* store, int store_grid, const std::vector<int>& types);
+	int grid_id() const { return grid_; }
 	// device bytes of the maps built so far
 	size_t map_bytes() const;
 	sgpu_ctx* ctx() { return ctx_; }
@@ -210,10 +232,13 @@ private:
 	// (the first pass of a two-pass screen)
 	void dock_sub(DockJob* const* jobs, size_t n, double budget);
 	void dock_pass(std::vector<DockJob*>& jobs, double budget);
+	void fp64_errors(DockJob* const* jobs, size_t nj, int batch, int cstride);
 	GpuSettings s_;
 	int capacity_ = 1;                // chains the device holds at once
 	std::vector<double> top_history_; // two-pass: first-pass top scores so far
 	size_t full_searches_ = 0;
+	int tune_steps_ = 0;
+	sgpu_mc_tune_report* tune_report_ = nullptr; // set during tune()
 	sgpu_ctx* ctx_ = nullptr;
 	bool owns_ctx_ = true;
 	sgpu_scoring sf_;
@@ -223,6 +248,7 @@ private:
 	int receptor_ = -1;
 	int grid_ = -1;
 	std::vector<int> grid_types_;
+	bool shared_grid_ = false; // maps adopted from another context
 };
 
 } // namespace sgpu_host

@@ -1050,7 +1050,96 @@ sgpu_mol* guarded_read(sgpu_mol* (*fn)(const char*), const char* text, char* err
 }
 } // namespace
 
+namespace
+{
+// the reference splits header lines on every single whitespace character
+// (boost::algorithm::split with is_space, no compression): "A  B" gives
+// "A", "", "B"
+std::vector<std::string> split_each_space(const std::string& line)
+{
+	std::vector<std::string> out(1);
+	for (char ch : line)
+		if (ch == ' ' || ch == '\t' || ch == '\n' || ch == '\r' || ch == '\v' || ch == '\f')
+			out.emplace_back();
+		else
+			out.back() += ch;
+	return out;
+}
+
+double field(const std::vector<std::string>& t, size_t i)
+{
+	return i < t.size() ? std::atof(t[i].c_str()) : 0.0;
+}
+
+// setup_user_gd (src/main/main.cpp:772-810) then grid::init's value loop
+// (src/lib/grid.cpp:77-108): three lines skipped; SPACING s; NELEMENTS
+// nx ny nz; CENTER cx cy cz; the box spans (N + 1) s per axis around the
+// center moved by s / 2, n = ceil(span / s) intervals; then n0 n1 n2 values,
+// one per line, z slowest, each stored as -(value x scaling); the last
+// plane of each axis stays 0; a missing line reads as 0
+int read_user_grid(const std::string& text, double scaling, sgpu_grid_dims& gd, std::vector<double>& data)
+{
+	std::istringstream in(text);
+	std::string line;
+	for (int k = 0; k < 3; k++)
+		std::getline(in, line);
+	std::getline(in, line);
+	const double gran = field(split_each_space(line), 1);
+	std::getline(in, line);
+	std::vector<std::string> t = split_each_space(line);
+	const double size[3] = {(field(t, 1) + 1) * gran, (field(t, 2) + 1) * gran, (field(t, 3) + 1) * gran};
+	std::getline(in, line);
+	t = split_each_space(line);
+	const double center[3] = {field(t, 1) + 0.5 * gran, field(t, 2) + 0.5 * gran, field(t, 3) + 0.5 * gran};
+	if (!(gran > 0))
+		throw std::invalid_argument("user grid: SPACING must be positive");
+	for (int i = 0; i < 3; i++)
+	{
+		const double n = std::ceil(size[i] / gran);
+		if (!(n >= 1 && n <= 4096))
+			throw std::invalid_argument("user grid: NELEMENTS out of range");
+		gd.n[i] = int(n);
+		const double real_span = gran * gd.n[i];
+		gd.begin[i] = center[i] - real_span / 2;
+		gd.end[i] = gd.begin[i] + real_span;
+	}
+	const int d0 = gd.n[0] + 1, d1 = gd.n[1] + 1, d2 = gd.n[2] + 1;
+	data.assign(size_t(d0) * d1 * d2, 0.0);
+	for (int z = 0; z < gd.n[2]; z++)
+		for (int y = 0; y < gd.n[1]; y++)
+			for (int x = 0; x < gd.n[0]; x++)
+			{
+				if (!std::getline(in, line))
+					line.clear();
+				data[size_t(x) + size_t(d0) * (size_t(y) + size_t(d1) * z)] = -(std::atof(line.c_str()) * scaling);
+			}
+	return int(data.size());
+}
+} // namespace
+
 extern "C" {
+
+int sgpu_read_user_grid(const char* text, double scaling, sgpu_grid_dims* gd, double* data, int capacity, char* err,
+		int err_len)
+{
+	try
+	{
+		if (!text || !gd)
+			throw std::invalid_argument("user grid: no text or box");
+		std::vector<double> v;
+		const int n = read_user_grid(text, scaling, *gd, v);
+		if (data)
+			std::copy(v.begin(), v.begin() + std::min<size_t>(v.size(), size_t(std::max(0, capacity))), data);
+		return n;
+	}
+	catch (std::exception& e)
+	{
+		if (err && err_len > 0)
+			std::snprintf(err, size_t(err_len), "%s", e.what());
+		return -1;
+	}
+}
+
 
 sgpu_mol* sgpu_read_pdbqt_ligand(const char* text, char* err, int err_len)
 {

@@ -9,8 +9,14 @@ namespace sgpu_host
 
 DockingServer::DockingServer(const ServerSettings& s, const sgpu_scoring& sf) : s_(s)
 {
-	if (s_.workers < 1 || s_.max_batch < 1 || !(s_.budget_bytes > 0))
+	if (s_.workers < 1 || s_.max_batch < 1 || s_.budget_bytes < 0 || !(s_.budget_share > 0 && s_.budget_share <= 1))
 		throw std::invalid_argument("server: workers, max_batch and budget must be positive");
+	if (sgpu_create(s_.device, &store_) != SGPU_OK)
+		throw std::runtime_error("server: no HIP device");
+	size_t free_b = 0, total_b = 0;
+	if (sgpu_device_memory(store_, &free_b, &total_b) != SGPU_OK)
+		throw std::runtime_error("server: device memory query failed");
+	budget_ = s_.budget_bytes > 0 ? s_.budget_bytes : s_.budget_share * double(total_b);
 	terms_.assign(sf.terms, sf.terms + sf.nterms);
 	ci_.assign(sf.ci, sf.ci + sf.nci);
 	sf_ = sf;
@@ -48,6 +54,15 @@ DockingServer::~DockingServer()
 		w->receptors.clear();
 		sgpu_destroy(w->ctx);
 	}
+	// the maps' owner goes last: the workers' contexts used them in place
+	store_recs_.clear();
+	sgpu_destroy(store_);
+}
+
+double DockingServer::resident_bytes() const
+{
+	std::lock_guard<std::mutex> lock(mu_);
+	return maps_bytes_;
 }
 
 double DockingServer::now() const
@@ -70,13 +85,22 @@ int DockingServer::add_receptor(const sgpu_receptor& rec, const sgpu_grid_dims& 
 	r.types = types;
 	recs_.push_back(std::move(r));
 	const Rec& rr = recs_.back();
+	const sgpu_receptor rec_view{rr.rec.natoms, rr.xyz.data(), rr.type.data()};
+	// the maps, once for the device, in the store context
+	std::unique_ptr<GpuDocking> owner(new GpuDocking(s_.dock, sf_, rec_view, rr.gd, store_));
+	owner->prepare_types(rr.types);
+	const double bytes = double(owner->map_bytes());
+	if (maps_bytes_ + bytes > budget_)
+		throw std::runtime_error("server: the resident maps exceed the memory budget");
+	maps_bytes_ += bytes;
 	for (auto& w : workers_)
 	{
-		std::unique_ptr<GpuDocking> d(new GpuDocking(s_.dock, sf_, sgpu_receptor{rr.rec.natoms, rr.xyz.data(),
-				rr.type.data()}, rr.gd, w->ctx));
-		d->prepare_types(rr.types);
+		// each worker: its tables and receptor atoms, the store's maps in place
+		std::unique_ptr<GpuDocking> d(new GpuDocking(s_.dock, sf_, rec_view, rr.gd, w->ctx));
+		d->adopt_grid(store_, owner->grid_id(), rr.types);
 		w->receptors.push_back(std::move(d));
 	}
+	store_recs_.push_back(std::move(owner));
 	return int(recs_.size()) - 1;
 }
 
@@ -100,10 +124,16 @@ int DockingServer::submit(int receptor, const sgpu_model_desc* ligand, int exhau
 
 size_t DockingServer::job_bytes(const Job& j) const
 {
-	// chain state (MT words, output container, coordinates) and the model
-	const int n = j.ligand->natoms;
-	const size_t chain = 2600 + 64 * size_t(8 * 64 + 4 * 3 * n) + 16 * size_t(n);
-	return size_t(j.exhaustiveness) * chain + 64 * size_t(n) * size_t(n);
+	// the device memory a job holds while its launch runs: per chain the
+	// state sgpu_run_mc keeps (generator words, the output container of 64
+	// poses with confs and heavy-atom coordinates, the model's coordinates,
+	// in FP64 the pair energies), per job the model blobs and its FP64 copy
+	const size_t n = size_t(j.ligand->natoms);
+	const size_t real = s_.dock.precision == SGPU_FP64 ? 8 : 4;
+	const size_t conf = 7 + n; // at most one torsion per atom
+	const size_t chain = 4096 + 64 * (8 * conf + real * 3 * n) + real * 3 * n + (real == 8 ? 8 * n * n / 2 : 0);
+	const size_t chains = size_t(j.exhaustiveness) * size_t(s_.dock.chain_split > 1 ? s_.dock.chain_split : 1);
+	return chains * chain + 48 * n * n + (1u << 20);
 }
 
 void DockingServer::run(Worker* w)
@@ -112,30 +142,63 @@ void DockingServer::run(Worker* w)
 	{
 		std::vector<Job*> batch;
 		int receptor = -1;
+		double batch_bytes = 0;
 		{
 			std::unique_lock<std::mutex> lock(mu_);
 			cv_work_.wait(lock, [this]() { return stop_ || !queue_.empty(); });
 			if (stop_ && queue_.empty())
 				return;
 			// the receptor of the job that has waited longest, then its queue in
-			// arrival order, within the batch size and the memory budget
-			receptor = queue_.front()->receptor;
-			const size_t maps = w->receptors[size_t(receptor)]->map_bytes();
-			size_t bytes = maps;
-			for (auto it = queue_.begin(); it != queue_.end() && int(batch.size()) < s_.max_batch;)
+			// arrival order, within the batch size and the device's memory
+			// account (resident maps + every launch in flight)
+			for (;;)
 			{
-				Job* j = *it;
-				if (j->receptor != receptor)
+				if (stop_ && queue_.empty())
+					return;
+				receptor = queue_.front()->receptor;
+				double bytes = 0;
+				for (auto it = queue_.begin(); it != queue_.end() && int(batch.size()) < s_.max_batch;)
 				{
-					++it;
+					Job* j = *it;
+					if (j->receptor != receptor)
+					{
+						++it;
+						continue;
+					}
+					const double b = double(job_bytes(*j));
+					if (maps_bytes_ + inflight_bytes_ + bytes + b > budget_)
+						break;
+					bytes += b;
+					batch.push_back(j);
+					it = queue_.erase(it);
+				}
+				if (!batch.empty())
+				{
+					inflight_bytes_ += bytes;
+					peak_inflight_ = std::max(peak_inflight_, inflight_bytes_);
+					batch_bytes = bytes;
+					break;
+				}
+				Job* front = queue_.front();
+				if (inflight_bytes_ == 0 && maps_bytes_ + double(job_bytes(*front)) > budget_)
+				{
+					// it can never fit: fail this job alone
+					queue_.pop_front();
+					front->t_admit = front->t_done = now();
+					front->result.error = "job needs " + std::to_string(job_bytes(*front)) +
+							" bytes of device memory beside the resident maps, above the server's budget";
+					front->result.latency_s = front->t_done - front->t_submit;
+					err_ = "job " + std::to_string(front->id) + ": " + front->result.error;
+					front->done = true;
+					cv_done_.notify_all();
+					if (queue_.empty())
+						cv_work_.wait(lock, [this]() { return stop_ || !queue_.empty(); });
 					continue;
 				}
-				const size_t b = job_bytes(*j);
-				if (!batch.empty() && double(bytes + b) > s_.budget_bytes)
-					break;
-				bytes += b;
-				batch.push_back(j);
-				it = queue_.erase(it);
+				// wait for a launch in flight to give its memory back
+				cv_work_.wait(lock);
+				if (queue_.empty())
+					cv_work_.wait(lock, [this]() { return stop_ || !queue_.empty(); });
 			}
 			const double t = now();
 			for (Job* j : batch)
@@ -181,6 +244,7 @@ void DockingServer::run(Worker* w)
 		}
 		{
 			std::lock_guard<std::mutex> lock(mu_);
+			inflight_bytes_ -= batch_bytes;
 			const double t = now();
 			for (size_t k = 0; k < batch.size(); k++)
 			{
@@ -188,6 +252,7 @@ void DockingServer::run(Worker* w)
 				j->t_done = t;
 				j->result.poses = std::move(dj[k].out);
 				j->result.evals = dj[k].evals;
+				j->result.fp64_error = dj[k].fp64_error;
 				j->result.latency_s = j->t_done - j->t_submit;
 				j->result.queued_s = j->t_admit - j->t_submit;
 				j->result.batch_size = int(batch.size());
@@ -198,6 +263,7 @@ void DockingServer::run(Worker* w)
 			}
 		}
 		cv_done_.notify_all();
+		cv_work_.notify_all(); // its memory is free again
 	}
 }
 
@@ -260,7 +326,7 @@ sgpu_server* sgpu_server_create(const sgpu_server_settings* s)
 		ss.device = s->device;
 		ss.workers = s->workers;
 		ss.max_batch = s->max_batch;
-		ss.budget_bytes = s->budget_gb * 1e9;
+		ss.budget_bytes = s->budget_gb * 1e9; // 0: 75% of the device's memory
 		ss.dock.device = s->device;
 		if (s->precision != SGPU_FP32 && s->precision != SGPU_FP64 && s->precision != SGPU_MIXED)
 			throw std::invalid_argument("precision must be SGPU_FP32, SGPU_FP64 or SGPU_MIXED");
@@ -271,6 +337,9 @@ sgpu_server* sgpu_server_create(const sgpu_server_settings* s)
 		ss.dock.progress_chunks = s->progress_chunks > 0 ? s->progress_chunks : 1;
 		ss.dock.progress = s->progress;
 		ss.dock.progress_user = s->progress_user;
+		if (s->step_fraction < 0 || s->step_fraction > 1)
+			throw std::invalid_argument("step_fraction must lie in (0, 1] (0: 1)");
+		ss.dock.step_fraction = s->step_fraction > 0 ? s->step_fraction : 1;
 		if (s->fast)
 		{
 			// auto_split's choice when the batch leaves the device idle
@@ -339,6 +408,19 @@ int sgpu_server_submit(sgpu_server* sv, int receptor, const sgpu_mol* ligand, in
 	}
 }
 
+int sgpu_server_memory(sgpu_server* sv, double* budget_bytes, double* resident_bytes, double* peak_inflight_bytes)
+{
+	if (!sv->s)
+		return -1;
+	if (budget_bytes)
+		*budget_bytes = sv->s->budget();
+	if (resident_bytes)
+		*resident_bytes = sv->s->resident_bytes();
+	if (peak_inflight_bytes)
+		*peak_inflight_bytes = sv->s->peak_inflight_bytes();
+	return 0;
+}
+
 int sgpu_server_wait(sgpu_server* sv, int job, sgpu_serve_result* out)
 {
 	if (!sv->s)
@@ -350,6 +432,7 @@ int sgpu_server_wait(sgpu_server* sv, int job, sgpu_serve_result* out)
 		out->latency_s = r.latency_s;
 		out->queued_s = r.queued_s;
 		out->batch_size = r.batch_size;
+		out->fp64_error = r.fp64_error;
 		if (!r.error.empty())
 		{
 			out->nposes = 0;

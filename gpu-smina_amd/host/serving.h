@@ -1,16 +1,20 @@
 // Docking server: several receptors resident on one GPU, jobs admitted as
 // they arrive and batched continuously.
 //
-// Each worker thread owns a device context; every receptor is resident in
-// every context (its atoms and its maps, built once for the ligand types
-// served).  A free worker takes the waiting jobs of the receptor whose
-// oldest job waited longest, up to a batch size and within the worker's
-// device-memory budget, and docks them as one launch (GpuDocking::dock_batch
-// semantics per job: search, merge, refinement, FP64 rescoring).  With W
-// workers, up to W launches are on the device at once, so jobs that arrive
-// while others run start as soon as a worker frees up, and their work
-// overlaps the running launches.  A job's chains are seeded by the job
-// alone, so its result does not depend on what it was batched with.
+// Each worker thread owns a device context (one launch at a time per
+// context); every receptor's maps are built once, in the server's store
+// context, and every worker uses them in place (sgpu_share_grid), so the
+// maps are resident once per device.  A free worker takes the waiting jobs
+// of the receptor whose oldest job waited longest, up to a batch size, and
+// admits them only while one device-wide account holds them: the resident
+// maps plus the chain state and models of every launch in flight, within
+// the server's budget (a stated share of the device's memory).  It docks
+// them as one launch (GpuDocking::dock_batch semantics per job: search,
+// merge, refinement, FP64 rescoring).  With W workers, up to W launches
+// are on the device at once, so jobs that arrive while others run start as
+// soon as a worker and the memory free up, and their work overlaps the
+// running launches.  A job's chains are seeded by the job alone, so its
+// result does not depend on what it was batched with.
 //
 // Each worker's context drives up to four streams.  Streams that share a
 // hardware queue run their kernels in FIFO order, so the process needs
@@ -38,7 +42,10 @@ struct ServerSettings
 	int device = 0;
 	int workers = 2;            // concurrent launches
 	int max_batch = 64;         // ligands per launch
-	double budget_bytes = 64e9; // device memory per worker for maps and jobs
+	// device memory for resident maps and in-flight jobs, one account for
+	// the device; 0: budget_share of the device's total memory
+	double budget_bytes = 0;
+	double budget_share = 0.75;
 	GpuSettings dock;           // precision, seed, modes, energy range
 };
 
@@ -63,6 +70,7 @@ public:
 		double latency_s = 0; // submit to result
 		double queued_s = 0;  // submit to admission
 		int batch_size = 0;   // jobs in its launch
+		double fp64_error = -1; // DockJob::fp64_error of the job
 		std::string error;    // empty: docked; else why this job failed
 	};
 	// block until job `id` completes
@@ -70,6 +78,15 @@ public:
 	// block until every submitted job completes
 	void drain();
 	std::string error() const;
+	// the device-wide memory account: its budget, the resident maps (once
+	// per device) and the peak of the launches in flight
+	double budget() const { return budget_; }
+	double resident_bytes() const;
+	double peak_inflight_bytes() const
+	{
+		std::lock_guard<std::mutex> lock(mu_);
+		return peak_inflight_;
+	}
 
 private:
 	struct Job
@@ -88,6 +105,7 @@ private:
 		std::vector<std::unique_ptr<GpuDocking>> receptors;
 		std::thread thread;
 	};
+
 	void run(Worker* w);
 	double now() const;
 	size_t job_bytes(const Job& j) const;
@@ -105,6 +123,15 @@ private:
 		std::vector<int> types;
 	};
 	std::vector<Rec> recs_;
+	// the store context holds every receptor's maps (destroyed after the
+	// workers, whose contexts use them)
+	sgpu_ctx* store_ = nullptr;
+	std::vector<std::unique_ptr<GpuDocking>> store_recs_;
+	double budget_ = 0;
+	double maps_bytes_ = 0;
+	double inflight_bytes_ = 0;
+	double peak_inflight_ = 0;
+	std::condition_variable cv_mem_;
 	std::vector<std::unique_ptr<Worker>> workers_;
 	mutable std::mutex mu_;
 	std::condition_variable cv_work_, cv_done_;

@@ -83,7 +83,14 @@ class RefineRequest(C.Structure):
                 ("v", C.c_double * 3), ("accurate_line_search", C.c_int), ("early_term", C.c_int),
                 ("nposes", C.c_int), ("model", I32P), ("conf", F64P), ("conf_stride", C.c_int),
                 ("maxiters", I32P), ("e", F64P), ("coords", F64P), ("coord_stride", C.c_int),
-                ("boxes", F64P)]
+                ("boxes", F64P), ("tries", I32P)]
+
+
+class BfgsRequest(C.Structure):
+    _fields_ = [("batch_id", C.c_int), ("grid_id", C.c_int), ("receptor_id", C.c_int), ("gd", C.POINTER(GridDims)),
+                ("slope", C.c_double), ("v", C.c_double * 3), ("accurate_line_search", C.c_int),
+                ("early_term", C.c_int), ("nconfs", C.c_int), ("model", I32P), ("conf", F64P),
+                ("conf_stride", C.c_int), ("maxiters", I32P), ("e", F64P), ("evals", I64P)]
 
 
 class ScoreRequest(C.Structure):
@@ -116,25 +123,37 @@ class Scoring(C.Structure):
 PROGRESS_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_double)
 
 
+class TuneReport(C.Structure):
+    _fields_ = [("nclasses", C.c_int), ("class_index", C.c_int * 4), ("ncand", C.c_int * 4),
+                ("cand_waves", (C.c_int * 8) * 4), ("cand_persist", (C.c_int * 8) * 4),
+                ("cand_ms", (C.c_double * 8) * 4), ("chosen_waves", C.c_int * 4), ("chosen_persist", C.c_int * 4)]
+
+
 class DockSettings(C.Structure):
     _fields_ = [("device", C.c_int), ("precision", C.c_int), ("exhaustiveness", C.c_int), ("seed", C.c_int),
                 ("num_modes", C.c_int), ("energy_range", C.c_double), ("hydrophobic_scale", C.c_double),
                 ("ligands_per_launch", C.c_int), ("step_fraction", C.c_double), ("progress_chunks", C.c_int),
                 ("progress", PROGRESS_FN), ("progress_user", C.c_void_p), ("scoring", C.c_char_p),
                 ("chain_split", C.c_int), ("early_stop", C.c_int), ("fast", C.c_int), ("skip_visited", C.c_int),
-                ("approximation", C.c_int), ("approx_factor", C.c_double)]
+                ("approximation", C.c_int), ("approx_factor", C.c_double),
+                ("user_grid", C.c_void_p)]
+
+
+class UserGrid(C.Structure):
+    _fields_ = [("gd", GridDims), ("data", F64P), ("map_slope", C.c_double)]
 
 
 class ServerSettings(C.Structure):
     _fields_ = [("device", C.c_int), ("workers", C.c_int), ("max_batch", C.c_int), ("budget_gb", C.c_double),
                 ("precision", C.c_int), ("seed", C.c_int), ("num_modes", C.c_int), ("energy_range", C.c_double),
                 ("hydrophobic_scale", C.c_double), ("scoring", C.c_char_p), ("progress_chunks", C.c_int),
-                ("progress", PROGRESS_FN), ("progress_user", C.c_void_p), ("fast", C.c_int)]
+                ("progress", PROGRESS_FN), ("progress_user", C.c_void_p), ("fast", C.c_int),
+                ("step_fraction", C.c_double)]
 
 
 class ServeResult(C.Structure):
     _fields_ = [("nposes", C.c_int), ("e", C.c_double * 32), ("evals", C.c_int64), ("latency_s", C.c_double),
-                ("queued_s", C.c_double), ("batch_size", C.c_int)]
+                ("queued_s", C.c_double), ("batch_size", C.c_int), ("fp64_error", C.c_double)]
 
 
 class MinimizeResult(C.Structure):
@@ -144,7 +163,8 @@ class MinimizeResult(C.Structure):
 class DockResult(C.Structure):
     _fields_ = [("nposes", C.c_int), ("e", C.c_double * 32), ("evals", C.c_int64), ("seconds", C.c_double),
                 ("coords", F64P), ("coords_capacity", C.c_int), ("ncoords", C.c_int),
-                ("confs", F64P), ("confs_capacity", C.c_int), ("conf_size", C.c_int)]
+                ("confs", F64P), ("confs_capacity", C.c_int), ("conf_size", C.c_int),
+                ("fp64_error", C.c_double), ("e_search", C.c_double), ("e_fp64", C.c_double)]
 
 
 def _load():
@@ -169,6 +189,13 @@ def _load():
         "sgpu_run_mc": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(McParams), C.POINTER(Chains),
                                   C.POINTER(ChainResults)]),
         "sgpu_run_refine": (C.c_int, [vp, C.POINTER(RefineRequest)]),
+        "sgpu_run_bfgs": (C.c_int, [vp, C.POINTER(BfgsRequest)]),
+        "sgpu_tune_mc": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(McParams), C.POINTER(Chains), C.c_int,
+                                   C.POINTER(TuneReport)]),
+        "sgpu_docking_tune": (C.c_int, [vp, C.POINTER(vp), C.c_int, C.c_int, C.POINTER(TuneReport)]),
+        "sgpu_set_user_grid": (C.c_int, [vp, vp]),
+        "sgpu_read_user_grid": (C.c_int, [C.c_char_p, C.c_double, C.POINTER(GridDims), F64P, C.c_int, C.c_char_p,
+                                          C.c_int]),
         "sgpu_score": (C.c_int, [vp, C.POINTER(ScoreRequest)]),
         "sgpu_last_kernel_ms": (C.c_int, [vp, C.c_char_p, F64P]),
         "sgpu_set_progress": (C.c_int, [vp, C.c_int, PROGRESS_FN, vp]),
@@ -204,6 +231,8 @@ def _load():
         "sgpu_server_add_receptor": (C.c_int, [vp, vp, C.POINTER(GridDims), I32P, C.c_int]),
         "sgpu_server_submit": (C.c_int, [vp, C.c_int, vp, C.c_int]),
         "sgpu_server_wait": (C.c_int, [vp, C.c_int, C.POINTER(ServeResult)]),
+        "sgpu_server_memory": (C.c_int, [vp, F64P, F64P, F64P]),
+        "sgpu_device_memory": (C.c_int, [vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
         "sgpu_docking_prepare": (C.c_int, [vp, C.POINTER(vp), C.c_int]),
         "sgpu_docking_ctx": (vp, [vp]),
         "sgpu_docking_receptor_id": (C.c_int, [vp]),
@@ -224,7 +253,7 @@ lib = _load()
 EXPORTED = ["sgpu_create", "sgpu_destroy", "sgpu_last_error", "sgpu_device_info", "sgpu_upload_tables",
             "sgpu_upload_receptor", "sgpu_build_grids", "sgpu_download_grid", "sgpu_release_grid",
             "sgpu_upload_models", "sgpu_release_models", "sgpu_eval_confs", "sgpu_run_mc", "sgpu_run_refine",
-            "sgpu_score", "sgpu_last_kernel_ms", "sgpu_set_progress"]
+            "sgpu_run_bfgs", "sgpu_score", "sgpu_last_kernel_ms", "sgpu_set_progress", "sgpu_eval_confs_noalloc"]
 
 
 def ptr(a, t):
@@ -389,6 +418,30 @@ def grid_dims(center, size, spacing=0.375):
     return gd
 
 
+def tune_dict(rep):
+    """an sgpu_mc_tune_report as a list of per-class dicts"""
+    out = []
+    for i in range(rep.nclasses):
+        out.append({"class": rep.class_index[i], "chosen": {"waves": rep.chosen_waves[i],
+                                                             "persistent": bool(rep.chosen_persist[i])},
+                    "candidates": [{"waves": rep.cand_waves[i][j], "persistent": bool(rep.cand_persist[i][j]),
+                                    "ms": rep.cand_ms[i][j]} for j in range(rep.ncand[i])]})
+    return out
+
+
+def read_user_grid(text, scaling=1.0):
+    """a --user_grid map file (sgpu_read_user_grid): (GridDims of its box,
+    the (n0+1)(n1+1)(n2+1) values grid::init stores, x fastest)"""
+    gd = GridDims()
+    err = C.create_string_buffer(512)
+    n = lib.sgpu_read_user_grid(text.encode(), scaling, C.byref(gd), None, 0, err, 512)
+    if n < 0:
+        raise ValueError(err.value.decode())
+    vals = np.zeros(n)
+    lib.sgpu_read_user_grid(text.encode(), scaling, C.byref(gd), ptr(vals, F64P), n, err, 512)
+    return gd, vals
+
+
 class Flat:
     """A flattened model (sgpu_flat) for the device layer."""
 
@@ -444,7 +497,7 @@ class Context:
         self.h = h
 
     def close(self):
-        if self.h:
+        if getattr(self, "h", None):
             lib.sgpu_destroy(self.h)
             self.h = None
 
@@ -470,6 +523,15 @@ class Context:
         self._splines = (f64(coef).reshape(-1), f64(cut))
         s = Splines(nseg, cutoff / nseg, cutoff * cutoff, ptr(self._splines[0], F64P), ptr(self._splines[1], F64P))
         self.check(lib.sgpu_upload_splines(self.h, C.byref(s)))
+
+    def set_user_grid(self, ugd=None, values=None, map_slope=1e6):
+        """sgpu_set_user_grid: the --user_grid map (None: remove it)"""
+        if ugd is None:
+            self.check(lib.sgpu_set_user_grid(self.h, None))
+            return
+        vals = f64(values)
+        ug = UserGrid(ugd, ptr(vals, F64P), map_slope)
+        self.check(lib.sgpu_set_user_grid(self.h, C.byref(ug)))
 
     def upload_receptor(self, xyz, types):
         xyz = f64(xyz).reshape(-1)
@@ -543,10 +605,24 @@ class Context:
         self.check(lib.sgpu_run_mc(self.h, bid, gid, C.byref(params), C.byref(ch), C.byref(res)))
         return cnt, e, c, x, ev
 
+    def tune_mc(self, bid, gid, params, models, confs, rngs, maxiters, probe_steps=8):
+        """sgpu_tune_mc on these chains; returns tune_dict's list"""
+        models = i32(models)
+        confs = f64(confs)
+        rngs = np.ascontiguousarray(rngs, dtype=np.uint32)
+        steps = np.full(len(models), probe_steps, np.int32)
+        maxiters = i32(maxiters)
+        ch = Chains(len(models), ptr(models, I32P), ptr(confs, F64P), confs.shape[1], ptr(rngs, U32P),
+                    ptr(steps, I32P), ptr(maxiters, I32P), None)
+        rep = TuneReport()
+        self.check(lib.sgpu_tune_mc(self.h, bid, gid, C.byref(params), C.byref(ch), probe_steps, C.byref(rep)))
+        return tune_dict(rep)
+
     def refine(self, bid, rid, gd, models, confs, maxiters, v=(1000.0, 1000.0, 1000.0), coord_stride=3 * 256,
-               boxes=None, accurate=False):
+               boxes=None, accurate=False, tries=False):
         """refine_structure of each pose; boxes: optional [n, 6] per-pose
-        (begin, end) inside gd"""
+        (begin, end) inside gd; tries: also return the box-slope increases
+        each refinement took (its last slope is 10 x 10^tries)"""
         models = i32(models)
         confs = f64(confs).copy()
         maxiters = i32(maxiters)
@@ -563,8 +639,33 @@ class Context:
         q.maxiters, q.e, q.coords, q.coord_stride = ptr(maxiters, I32P), ptr(e, F64P), ptr(x, F64P), coord_stride
         q.accurate_line_search = int(accurate)
         q.boxes = ptr(boxes, F64P) if boxes is not None else None
+        nt = np.zeros(n, np.int32)
+        q.tries = ptr(nt, I32P) if tries else None
         self.check(lib.sgpu_run_refine(self.h, C.byref(q)))
-        return e, confs, x
+        return (e, confs, x, nt) if tries else (e, confs, x)
+
+    def bfgs(self, bid, models, confs, maxiters, v, grid_id=-1, receptor_id=0, gd=None, slope=1e6, accurate=False):
+        """sgpu_run_bfgs: one BFGS minimisation per conf on the maps (grid_id)
+        or the direct receptor term (receptor_id, gd); returns (e, confs,
+        evals)"""
+        models = i32(models)
+        confs = f64(confs).copy()
+        maxiters = i32(maxiters)
+        n = len(models)
+        e = np.zeros(n)
+        ev = np.zeros(n, np.int64)
+        q = BfgsRequest()
+        q.batch_id, q.grid_id, q.receptor_id = bid, grid_id, receptor_id
+        q.gd = C.pointer(gd) if gd is not None else None
+        q.slope = slope
+        for k in range(3):
+            q.v[k] = v[k]
+        q.accurate_line_search = int(accurate)
+        q.nconfs = n
+        q.model, q.conf, q.conf_stride = ptr(models, I32P), ptr(confs, F64P), confs.shape[1]
+        q.maxiters, q.e, q.evals = ptr(maxiters, I32P), ptr(e, F64P), ptr(ev, I64P)
+        self.check(lib.sgpu_run_bfgs(self.h, C.byref(q)))
+        return e, confs, ev
 
     def score(self, bid, rid, gd, models, confs, mode, v=(1000.0, 1000.0, 1000.0), slope=1e6):
         models = i32(models)
@@ -619,13 +720,14 @@ class Docking:
     def __init__(self, receptor, gd, fp64=False, exhaustiveness=8, seed=1, num_modes=9, energy_range=3.0,
                  hydrophobic_scale=1.0, ligands_per_launch=0, device=0, step_fraction=1.0, precision=None,
                  progress=None, progress_chunks=1, scoring="vina", chain_split=0, early_stop=False, fast=False,
-                 skip_visited=False, approximation="linear", approx_factor=0.0):
+                 skip_visited=False, approximation="linear", approx_factor=0.0, user_grid=None):
         """precision: FP32, FP64 or MIXED (default: FP64 if fp64 else FP32);
         progress(fraction): called after each of progress_chunks launches
         of the Monte Carlo search; chain_split, early_stop, skip_visited,
         fast: the GPU-style search options --gpu_split, --gpu_early_stop,
         --gpu_qvina, --fast; approximation "linear" or "spline" (--minimize's
-        default, with factor 10 unless approx_factor is given)"""
+        default, with factor 10 unless approx_factor is given); user_grid:
+        (GridDims, values) of --user_grid (read_user_grid)"""
         prec = precision if precision is not None else (FP64 if fp64 else FP32)
         # keep the callback alive as long as the docking object
         self._progress = PROGRESS_FN(lambda user, frac: progress(frac)) if progress else PROGRESS_FN()
@@ -633,6 +735,12 @@ class Docking:
                          ligands_per_launch, step_fraction, progress_chunks, self._progress, None,
                          scoring.encode(), chain_split, int(early_stop), int(fast), int(skip_visited),
                          {"linear": 0, "spline": 1}[approximation], approx_factor)
+        ug = None
+        if user_grid is not None:
+            ugd, vals = user_grid
+            vals = f64(vals)
+            ug = UserGrid(ugd, ptr(vals, F64P), 1e6)
+            s.user_grid = C.cast(C.pointer(ug), C.c_void_p)
         self.gd = gd
         self.h = C.c_void_p(lib.sgpu_docking_create(C.byref(s), receptor.h, C.byref(gd)))
         err = lib.sgpu_docking_error(self.h)
@@ -680,13 +788,27 @@ class Docking:
                 r.confs_capacity = b.size
         if lib.sgpu_dock(self.h, arr, len(ligands), out) != 0:
             raise RuntimeError(lib.sgpu_docking_error(self.h).decode())
-        res = [dict(e=list(r.e[:r.nposes]), evals=r.evals, seconds=r.seconds) for r in out]
+        # fp64_error: |e_search - e_fp64| of the top pose (FP32 and mixed
+        # searches; None for FP64 or a ligand without poses)
+        res = [dict(e=list(r.e[:r.nposes]), evals=r.evals, seconds=r.seconds,
+                    fp64_error=r.fp64_error if r.fp64_error >= 0 else None, e_search=r.e_search, e_fp64=r.e_fp64)
+               for r in out]
         for k, b in enumerate(bufs):
             res[k]["coords"] = b[:out[k].ncoords].reshape(-1, 3).copy()
         for k, b in enumerate(cbufs):
             cs = out[k].conf_size
             res[k]["confs"] = b[:out[k].nposes * cs].reshape(-1, cs).copy() if cs else np.zeros((0, 0))
         return res
+
+    def tune(self, ligands, probe_steps=16):
+        """the launch-plan autotuner on the chains a docking of these
+        ligands would run: per size class, every candidate plan's probe time
+        and the plan kept for later dockings"""
+        arr = (C.c_void_p * len(ligands))(*[m.h.value for m in ligands])
+        rep = TuneReport()
+        if lib.sgpu_docking_tune(self.h, arr, len(ligands), probe_steps, C.byref(rep)) != 0:
+            raise RuntimeError(lib.sgpu_docking_error(self.h).decode())
+        return tune_dict(rep)
 
     def full_searches(self):
         """ligands of the last dock() that got the full search"""
@@ -706,15 +828,17 @@ class Server:
     admitted as they arrive and batched continuously over `workers`
     concurrent launches (sgpu_server_*)."""
 
-    def __init__(self, workers=2, max_batch=64, budget_gb=64.0, fp64=False, seed=1, num_modes=9,
+    def __init__(self, workers=2, max_batch=64, budget_gb=0.0, fp64=False, seed=1, num_modes=9,
                  energy_range=3.0, hydrophobic_scale=1.0, device=0, precision=None, scoring="vina", progress=None,
-                 progress_chunks=1, fast=False):
+                 progress_chunks=1, fast=False, step_fraction=1.0):
         """progress(fraction): after each of progress_chunks parts of every
-        launch's Monte Carlo search, on the worker's thread"""
+        launch's Monte Carlo search, on the worker's thread; budget_gb: the
+        device-wide memory account (0: 75% of the device's memory)"""
         prec = precision if precision is not None else (FP64 if fp64 else FP32)
         self._progress = PROGRESS_FN(lambda user, frac: progress(frac)) if progress else PROGRESS_FN()
         s = ServerSettings(device, workers, max_batch, budget_gb, prec, seed, num_modes, energy_range,
-                           hydrophobic_scale, scoring.encode(), progress_chunks, self._progress, None, int(fast))
+                           hydrophobic_scale, scoring.encode(), progress_chunks, self._progress, None, int(fast),
+                           step_fraction)
         self.h = C.c_void_p(lib.sgpu_server_create(C.byref(s)))
         err = lib.sgpu_server_error(self.h)
         if err:
@@ -744,9 +868,16 @@ class Server:
         self._keep[job] = ligand
         return job
 
+    def memory(self):
+        """the device-wide memory account: budget, resident maps (once per
+        device), peak bytes of the launches in flight"""
+        v = np.zeros(3)
+        self._check(lib.sgpu_server_memory(self.h, ptr(v[0:], F64P), ptr(v[1:], F64P), ptr(v[2:], F64P)))
+        return {"budget": v[0], "resident_maps": v[1], "peak_inflight": v[2]}
+
     def wait(self, job):
         r = ServeResult()
         self._check(lib.sgpu_server_wait(self.h, job, C.byref(r)))
         self._keep.pop(job, None)
         return dict(e=list(r.e[:r.nposes]), evals=r.evals, latency_s=r.latency_s, queued_s=r.queued_s,
-                    batch_size=r.batch_size)
+                    batch_size=r.batch_size, fp64_error=r.fp64_error if r.fp64_error >= 0 else None)

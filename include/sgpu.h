@@ -17,6 +17,7 @@
 #ifndef SGPU_H
 #define SGPU_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -202,6 +203,9 @@ void sgpu_destroy(sgpu_ctx* ctx);
 const char* sgpu_last_error(const sgpu_ctx* ctx);
 /* name and compute-unit count of the context's device */
 sgpu_status sgpu_device_info(sgpu_ctx* ctx, char* name, int name_len, int* cus);
+/* the device's free and total memory (hipMemGetInfo), for admission by
+ * memory footprint */
+sgpu_status sgpu_device_memory(sgpu_ctx* ctx, size_t* free_bytes, size_t* total_bytes);
 /* Monte Carlo chains of this precision the device holds at once (one per
  * wave, as the occupancy calculator gives it from the search kernel's
  * registers): EngineBase::capacity, src/gpu/gpu_docking.cpp:223-226 */
@@ -221,11 +225,43 @@ sgpu_status sgpu_upload_splines(sgpu_ctx* ctx, const sgpu_splines* s);
  * device; cutoff_sqr as the tables' */
 sgpu_status sgpu_upload_receptor(sgpu_ctx* ctx, const sgpu_receptor* r, int* receptor_id);
 
+/* --user_grid (grid::init from a map file, src/lib/grid.cpp:77-108; the
+ * reference's FlatUserGrid, src/gpu/gpu_device.h:95-99): a user-supplied
+ * map added to the score.  data holds (n0+1)(n1+1)(n2+1) values, x fastest,
+ * as grid::init stores them (the file's values times -scaling, the last
+ * plane of each axis 0; sgpu_read_user_grid in sgpu_host.h builds them from
+ * the file).  Once set, on this context:
+ *   - maps built afterwards add it at every point as cache::populate does
+ *     (src/lib/cache.cpp:204-205): evaluated at the point's integer indices
+ *     taken as a location in A, with map_slope (the cache slope, 1e6) and
+ *     cap 1000;
+ *   - the direct receptor term adds it at each heavy movable atom with that
+ *     evaluation's slope and cap 1000, value and derivative
+ *     (src/lib/non_cache.cpp:180-184);
+ *   - the final rescoring (sgpu_score mode 0) adds it at every ligand atom
+ *     with slope and cap 1000 (model::eval, src/lib/model.cpp:953-964).
+ * NULL removes it. */
+typedef struct sgpu_user_grid
+{
+	sgpu_grid_dims gd;   /* the user grid's own box, n intervals per axis */
+	const double* data;
+	double map_slope;
+} sgpu_user_grid;
+sgpu_status sgpu_set_user_grid(sgpu_ctx* ctx, const sgpu_user_grid* ug);
+
 /* DeviceGrids::build (src/gpu/gpu_device.cu:199-250) = cache::populate
  * (src/lib/cache.cpp:116-210): one map per listed ligand type, sums in FP64,
  * stored in `precision`.  slope is the cache slope (1e6). */
 sgpu_status sgpu_build_grids(sgpu_ctx* ctx, int receptor_id, const sgpu_grid_dims* gd,
 		const int* types, int ntypes, int precision, int* grid_id);
+/* Another context on the same device uses the maps of src's grid src_grid
+ * without a copy: the maps stay resident once per device for every context
+ * that serves their receptor (serving: one context per concurrent launch).
+ * The memory stays with src, so src is destroyed after every context that
+ * shares its grids; releasing the grid in one context leaves the others'
+ * use of it intact.  The sharer's own receptor id serves the direct
+ * receptor term; the maps carry only what was built into them. */
+sgpu_status sgpu_share_grid(sgpu_ctx* dst, sgpu_ctx* src, int src_grid, int* grid_id);
 /* copy one type's map back, point layout x fastest, (n0+1)(n1+1)(n2+1) values */
 sgpu_status sgpu_download_grid(sgpu_ctx* ctx, int grid_id, int type, double* out);
 sgpu_status sgpu_release_grid(sgpu_ctx* ctx, int grid_id);
@@ -308,6 +344,32 @@ sgpu_status sgpu_eval_confs_noalloc(sgpu_ctx* ctx, const sgpu_eval_request_dev* 
 sgpu_status sgpu_run_mc(sgpu_ctx* ctx, int batch_id, int grid_id, const sgpu_mc_params* p,
 		const sgpu_chains* chains, sgpu_chain_results* out);
 
+/* The launch-plan autotuner.  It has no counterpart in the reference,
+ * whose threads-per-chain policy is a fixed rule
+ * (src/gpu/gpu_kernels_impl.cuh:552-605).  For each size class of these
+ * chains (by pair count), the class's chains alone run probe_steps Monte
+ * Carlo steps from their start state under every candidate plan: chains
+ * per workgroup 1, 2, 4 or 8 (where LDS and the chains per ligand allow),
+ * one-shot grid or persistent grid over a device work queue.  Each probe
+ * is timed with HIP events; the fastest plan is kept by the context for
+ * its later sgpu_run_mc calls of the same precision, kernel variant and
+ * class.  The chains' trajectories do not depend on the plan.  The report
+ * lists every candidate's time and the choice per class. */
+#define SGPU_TUNE_MAX_CANDIDATES 8
+typedef struct sgpu_mc_tune_report
+{
+	int nclasses;
+	int class_index[4];   /* size class: pairs <= 256, <= 512, <= 1024, more */
+	int ncand[4];
+	int cand_waves[4][SGPU_TUNE_MAX_CANDIDATES];
+	int cand_persist[4][SGPU_TUNE_MAX_CANDIDATES];
+	double cand_ms[4][SGPU_TUNE_MAX_CANDIDATES];
+	int chosen_waves[4];
+	int chosen_persist[4];
+} sgpu_mc_tune_report;
+sgpu_status sgpu_tune_mc(sgpu_ctx* ctx, int batch_id, int grid_id, const sgpu_mc_params* p,
+		const sgpu_chains* chains, int probe_steps, sgpu_mc_tune_report* out);
+
 /* run_refine (src/gpu/gpu_kernels.h:100-110): refine_structure
  * (src/main/main.cpp:165-191) of every pose on the non_cache back end with
  * box gd; conf is updated in place; e = max double if the pose stays out of
@@ -333,13 +395,47 @@ typedef struct sgpu_refine_request
 	 * = gd for every pose.  Each box must lie inside gd, which spans the
 	 * receptor cells. */
 	const double* boxes;
+	/* optional out [nposes]: the box-slope increases the refinement took
+	 * (the last BFGS run used slope 10 x 10^tries; 5: the pose never got
+	 * inside the box), or NULL */
+	int* tries;
 } sgpu_refine_request;
 sgpu_status sgpu_run_refine(sgpu_ctx* ctx, const sgpu_refine_request* req);
+
+/* run_bfgs (src/gpu/gpu_kernels.h:104-111): quasi_newton::operator()
+ * (bfgs, src/lib/bfgs.h:31-282) on every conf in place, one wavefront each,
+ * on the receptor maps (grid_id >= 0, map slope) or on the direct receptor
+ * term (grid_id < 0: receptor_id's atoms in box gd, non_cache slope), with
+ * caps v; e receives the minimised energies and evals (optional) the
+ * evaluations each minimisation took. */
+typedef struct sgpu_bfgs_request
+{
+	int batch_id;
+	int grid_id;
+	int receptor_id;
+	const sgpu_grid_dims* gd;
+	double slope;
+	double v[3];
+	int accurate_line_search;
+	int early_term;
+	int nconfs;
+	const int* model;
+	double* conf;
+	int conf_stride;
+	const int* maxiters;
+	double* e;
+	int64_t* evals;
+} sgpu_bfgs_request;
+sgpu_status sgpu_run_bfgs(sgpu_ctx* ctx, const sgpu_bfgs_request* req);
 
 /* final rescoring of finish_docking (src/main/main.cpp:256-324) in FP64:
  * mode 0: model::eval (src/lib/model.cpp:931-953) with the non_cache back
  *         end (slope) and fast tables;
- * mode 1: model::eval_intramolecular (src/lib/model.cpp:1028-1083). */
+ * mode 1: model::eval_intramolecular (src/lib/model.cpp:1028-1083);
+ * mode 2: model::eval_deriv's energy (src/lib/model.cpp:969-983) with the
+ *         non_cache back end (slope) and the search's smooth tables: the
+ *         function refine_structure minimises, in FP64 (the FP64 value of a
+ *         refined pose's energy, for the error of an FP32 search). */
 typedef struct sgpu_score_request
 {
 	int batch_id;

@@ -76,6 +76,17 @@ sgpu_mol* sgpu_read_pdbqt_flex(const sgpu_mol* ligand, const sgpu_mol* rigid, co
 int sgpu_write_pdbqt_poses(const sgpu_mol* ligand, int nposes, const double* conf, int conf_stride,
 		const double* e, const double* rmsd, char* out, int out_len);
 
+/* --user_grid map file (setup_user_gd, src/main/main.cpp:772-810, and
+ * grid::init, src/lib/grid.cpp:77-108): three lines skipped, "SPACING s",
+ * "NELEMENTS nx ny nz", "CENTER cx cy cz", then one value per line, x
+ * fastest.  Fills gd with the user grid's box ((N + 1) s per axis around the
+ * center moved by s / 2) and data with the (n0+1)(n1+1)(n2+1) values
+ * grid::init stores (each file value times -scaling; scaling is 1, or
+ * 1 - --user_grid_lambda), at most `capacity` of them.  Returns their count
+ * (call with data NULL to get it), or -1 with the reason in err. */
+int sgpu_read_user_grid(const char* text, double scaling, sgpu_grid_dims* gd, double* data, int capacity, char* err,
+		int err_len);
+
 /* ---- scoring */
 /* A built-in scoring function by smina's name (builtin_scoring,
  * src/lib/builtinscoring.cpp:65-113): "vina" (= "default"), "vinardo",
@@ -133,6 +144,7 @@ typedef struct sgpu_dock_settings
 	int approximation;      /* --approximation: 0 linear, 1 spline (the default of --minimize,
 	                         * src/main/main.cpp:1225-1228) */
 	double approx_factor;   /* --factor (0: 32 for linear, 10 for spline as --minimize sets it) */
+	const sgpu_user_grid* user_grid; /* --user_grid (NULL: none); copied at creation */
 } sgpu_dock_settings;
 
 sgpu_docking* sgpu_docking_create(const sgpu_dock_settings* s, const sgpu_mol* receptor,
@@ -152,11 +164,23 @@ typedef struct sgpu_dock_result
 	double* confs;   /* in: the caller's array for the poses' confs (nposes x conf_size), or NULL */
 	int confs_capacity; /* in: its length in doubles */
 	int conf_size;   /* out: doubles per conf (poses that did not fit are not written) */
+	/* FP32 and mixed searches: the top pose's refined energy in the search
+	 * precision (e_search), the FP64 value of the same function at the same
+	 * conf (e_fp64) and |e_search - e_fp64| (fp64_error); fp64_error = -1
+	 * for an FP64 search or a ligand without poses */
+	double fp64_error;
+	double e_search;
+	double e_fp64;
 } sgpu_dock_result;
 
 /* build the receptor maps for every heavy-atom type of these ligands (the
  * maps are otherwise built on first use) */
 int sgpu_docking_prepare(sgpu_docking* d, const sgpu_mol* const* ligands, int n);
+/* the launch-plan autotuner (sgpu_tune_mc) on the chains a docking of these
+ * ligands would run, probe_steps Monte Carlo steps per candidate plan; the
+ * docking object's later searches use the fastest plan per size class */
+int sgpu_docking_tune(sgpu_docking* d, const sgpu_mol* const* ligands, int n, int probe_steps,
+		sgpu_mc_tune_report* report);
 /* dock each ligand model (search, merge, refine, final rescoring) */
 int sgpu_dock(sgpu_docking* d, const sgpu_mol* const* ligands, int n, sgpu_dock_result* out);
 
@@ -201,7 +225,9 @@ typedef struct sgpu_server_settings
 	int device;
 	int workers;           /* concurrent launches (each its own device context) */
 	int max_batch;         /* ligands per launch */
-	double budget_gb;      /* device memory per worker for maps and jobs */
+	double budget_gb;      /* device memory for the resident maps and every job in flight, one account
+	                        * for the device (0: 75% of the device's memory); a job that cannot fit beside
+	                        * the maps fails with an error, jobs that fit wait for memory */
 	int precision;         /* SGPU_FP32, SGPU_FP64 or SGPU_MIXED */
 	int seed;              /* --seed of every job */
 	int num_modes;
@@ -216,6 +242,7 @@ typedef struct sgpu_server_settings
 	                        * with 1/32 of the steps, and QuickVina 2's history (no early stopping at
 	                        * that split).  The split is fixed so that a job's search does not depend
 	                        * on the jobs it is batched with; no two-pass screening. */
+	double step_fraction;  /* fraction of smina's Monte Carlo steps per job (0 or 1: smina's search) */
 } sgpu_server_settings;
 typedef struct sgpu_serve_result
 {
@@ -225,6 +252,7 @@ typedef struct sgpu_serve_result
 	double latency_s;      /* submit to result */
 	double queued_s;       /* submit to admission */
 	int batch_size;        /* jobs in its launch */
+	double fp64_error;     /* as sgpu_dock_result's: -1 for FP64 or no poses */
 } sgpu_serve_result;
 sgpu_server* sgpu_server_create(const sgpu_server_settings* s);
 void sgpu_server_free(sgpu_server* sv);
@@ -236,6 +264,10 @@ int sgpu_server_add_receptor(sgpu_server* sv, const sgpu_mol* receptor, const sg
 int sgpu_server_submit(sgpu_server* sv, int receptor, const sgpu_mol* ligand, int exhaustiveness);
 /* blocks until the job completes */
 int sgpu_server_wait(sgpu_server* sv, int job, sgpu_serve_result* out);
+/* the device-wide memory account: its budget, the resident maps (built once
+ * per device and used in place by every worker) and the peak bytes of the
+ * launches in flight so far */
+int sgpu_server_memory(sgpu_server* sv, double* budget_bytes, double* resident_bytes, double* peak_inflight_bytes);
 
 sgpu_ctx* sgpu_docking_ctx(sgpu_docking* d);
 int sgpu_docking_receptor_id(sgpu_docking* d);

@@ -1259,6 +1259,12 @@ struct Ctx
 	std::vector<RecAtom> rec; // all receptor atoms (incl. H)
 	std::vector<Grid> grids;  // per type
 	std::vector<bool> have_grid;
+	// --user_grid (grid::init from a map file, src/lib/grid.cpp:77-108)
+	Grid ug;
+	bool has_ug = false;
+	fl ug_map_slope = 1e6;
+	// grid::evaluate_user (src/lib/grid.cpp:50-53): evaluate_aux with cap 1000
+	fl user_grid(const vec& location, fl slope, vec* deriv) const { return ug.evaluate(location, slope, 1000, deriv); }
 	std::vector<std::unique_ptr<Model>> models;
 	// non_cache cells (full 3 A bricks over the relevant atoms)
 	int cell_off[3], cell_dim[3];
@@ -1362,7 +1368,13 @@ struct Ctx
 								aff[j] += tab.fast_value(a.type, needed[j], r2);
 					}
 					for (size_t j = 0; j < needed.size(); j++)
+					{
 						grids[needed[j]].at(x, y, z) = aff[j];
+						// src/lib/cache.cpp:204-205: the user grid at the
+						// point's integer indices taken as a location
+						if (has_ug)
+							grids[needed[j]].at(x, y, z) += user_grid(vec(x, y, z), ug_map_slope, nullptr);
+					}
 				}
 	}
 
@@ -1440,6 +1452,8 @@ struct Ctx
 					deriv += pd * r_ba;
 				}
 			}
+			if (has_ug)
+				this_e += user_grid(ac, slope, &ug_deriv);
 			deriv += ug_deriv;
 			curl(this_e, deriv, v);
 			m.minus_forces[i] = deriv + oob_deriv;
@@ -1547,6 +1561,12 @@ struct Ctx
 		e += pairs_eval(m, v[2], m.other_pairs);
 		for (int l = 0; l < m.nligands; l++)
 			e += pairs_eval(m, v[0], m.lig_pairs[l]);
+		// src/lib/model.cpp:953-964: the user grid at every atom of the
+		// single ligand (get_ligand_coords), slope 1000
+		if (has_ug)
+			for (int i = 0; i < m.N; i++)
+				if (m.find_ligand(i) == 0)
+					e += user_grid(m.coords[i], 1000, nullptr);
 		return e;
 	}
 	// model::eval_intramolecular (src/lib/model.cpp:1028-1083)
@@ -2194,6 +2214,28 @@ orc_ctx* orc_create(const sgpu_scoring* sf, const sgpu_receptor* rec, const sgpu
 	c.have_grid.assign(NUM_TYPES, false);
 	c.setup_noncache();
 	return o;
+}
+// --user_grid: values as grid::init stores them ((n+1)^3, x fastest); the
+// maps built afterwards carry it
+void orc_set_user_grid(orc_ctx* c, const sgpu_grid_dims* gd, const double* data, double map_slope)
+{
+	Ctx& x = c->c;
+	if (!gd)
+	{
+		x.has_ug = false;
+		return;
+	}
+	GridDims g;
+	for (int i = 0; i < 3; i++)
+	{
+		g.begin[i] = gd->begin[i];
+		g.end[i] = gd->end[i];
+		g.n[i] = gd->n[i];
+	}
+	x.ug.setup(g);
+	x.ug.data.assign(data, data + size_t(x.ug.dim[0]) * x.ug.dim[1] * x.ug.dim[2]);
+	x.ug_map_slope = map_slope;
+	x.has_ug = true;
 }
 void orc_destroy(orc_ctx* c) { delete c; }
 const char* orc_error(orc_ctx* c) { return c->c.err.c_str(); }

@@ -37,6 +37,9 @@ typedef struct orc_ctx orc_ctx;
  * src/main/main.cpp:1411-1421) */
 orc_ctx* orc_create(const sgpu_scoring* sf, const sgpu_receptor* rec, const sgpu_grid_dims* gd);
 void orc_destroy(orc_ctx* c);
+/* --user_grid: (n+1)^3 values, x fastest, as grid::init stores them; the
+ * maps built afterwards, the non_cache term and model::eval carry it */
+void orc_set_user_grid(orc_ctx* c, const sgpu_grid_dims* gd, const double* data, double map_slope);
 const char* orc_error(orc_ctx* c);
 
 /* table access: smooth[(tri*n + i)*2 + {0,1}], fast[tri*n + i] */

@@ -31,6 +31,7 @@ def _load():
     sig = {
         "orc_create": (vp, [vp, vp, vp]),
         "orc_destroy": (None, [vp]),
+        "orc_set_user_grid": (None, [vp, vp, F64P, C.c_double]),
         "orc_error": (C.c_char_p, [vp]),
         "orc_table_n": (C.c_int, [vp]),
         "orc_tables": (None, [vp, F64P, F64P]),
@@ -160,6 +161,13 @@ class Oracle:
 
     def eval_terms(self, t1, t2, r):
         return lib.orc_eval_terms(self.h, t1, t2, r)
+
+    def set_user_grid(self, ugd, values, map_slope=1e6):
+        """--user_grid: values ((n+1)^3, x fastest) as grid::init stores
+        them; call before build_grids (the maps carry it)"""
+        self._ug = np.ascontiguousarray(values, dtype=np.float64)
+        self._ugd = ugd
+        lib.orc_set_user_grid(self.h, C.addressof(ugd), _p(self._ug, F64P), map_slope)
 
     def build_grids(self, types):
         t = np.ascontiguousarray(types, dtype=np.int32)

@@ -305,6 +305,82 @@ def test_mc_fp64_batched_models(env):
         assert cnt[i] == len(eo) and np.array_equal(e[i, :cnt[i]], eo) and ev[i] == evo
 
 
+def batched_chains(env, seed0=3000):
+    models, cs, ss, steps, iters = [], [], [], [], []
+    for li, f in enumerate(env["flats"]):
+        for k in range(2):
+            c, st = f.make_chain(seed0 + 10 * li + k, env["gd"])
+            cc = np.zeros(64)
+            cc[:f.conf_size] = c
+            models.append(li)
+            cs.append(cc)
+            ss.append(st)
+            steps.append(5 + li)
+            iters.append((25 + f.natoms) // 3)
+    return models, np.array(cs), np.array(ss), steps, iters
+
+
+def check_against_oracle(env, models, cs, ss, steps, iters, out):
+    cnt, e, c, x, ev = out
+    for i, li in enumerate(models):
+        f = env["flats"][li]
+        eo, co, xo, evo = env["o"].mc_chain(env["ids"][li], sg.mc_params(), steps[i], iters[i], cs[i][:f.conf_size],
+                                            ss[i])
+        assert cnt[i] == len(eo) and np.array_equal(e[i, :cnt[i]], eo) and ev[i] == evo, (i, li)
+        assert np.array_equal(c[i, :cnt[i], :co.shape[1]], co)
+
+
+@pytest.mark.parametrize("form", ["graph-persistent", "graph-oneshot", "eager-persistent", "eager-oneshot"])
+def test_mc_fp64_launch_forms(env, form, monkeypatch):
+    """the Monte Carlo launch in each of its forms: persistent grids over the
+    device work queue or one workgroup per item, the whole search captured
+    in a HIP graph or enqueued directly, over three parts (progress
+    reports as graph host nodes).  The graph is built and replayed on every
+    call: two calls in a row with an eager call between them; every
+    chain of every call follows the oracle exactly"""
+    monkeypatch.setenv("SGPU_QUEUE", "1" if "persistent" in form else "0")
+    if form.startswith("eager"):
+        monkeypatch.setenv("SGPU_NO_GRAPH", "1")
+    ctx = env["ctx"]
+    models, cs, ss, steps, iters = batched_chains(env)
+    seen = []
+    ctx.set_progress(3, seen.append)
+    try:
+        for call in range(2):
+            print(f"launch form {form}, call {call}", flush=True)
+            out = ctx.run_mc(env["bids"][sg.FP64], env["gids"][sg.FP64], sg.mc_params(), models, cs, ss, steps, iters)
+            check_against_oracle(env, models, cs, ss, steps, iters, out)
+            with monkeypatch.context() as m:
+                m.setenv("SGPU_NO_GRAPH", "1")
+                eager = ctx.run_mc(env["bids"][sg.FP64], env["gids"][sg.FP64], sg.mc_params(), models, cs, ss, steps,
+                                   iters)
+            for u, w in zip(out, eager):
+                assert np.array_equal(u, w)
+    finally:
+        ctx.set_progress(1)
+    assert seen.count(1.0) == 4 and len(seen) == 12
+
+
+def test_tune_mc_keeps_the_fastest_plan(env):
+    """the launch-plan autotuner: every size class of the request gets its
+    candidates timed (chains per workgroup, one-shot or persistent grid),
+    the fastest is kept, and later searches with the tuned plans still
+    follow the oracle exactly"""
+    ctx = env["ctx"]
+    models, cs, ss, steps, iters = batched_chains(env, seed0=3100)
+    rep = ctx.tune_mc(env["bids"][sg.FP64], env["gids"][sg.FP64], sg.mc_params(), models, cs, ss, iters,
+                      probe_steps=3)
+    print("tuned:", rep, flush=True)
+    assert len(rep) >= 2
+    for cl in rep:
+        ms = [cd["ms"] for cd in cl["candidates"]]
+        assert len(ms) >= 2 and min(ms) > 0
+        best = cl["candidates"][int(np.argmin(ms))]
+        assert cl["chosen"] == {"waves": best["waves"], "persistent": best["persistent"]}
+    out = ctx.run_mc(env["bids"][sg.FP64], env["gids"][sg.FP64], sg.mc_params(), models, cs, ss, steps, iters)
+    check_against_oracle(env, models, cs, ss, steps, iters, out)
+
+
 def test_fp32_search_is_deterministic(env):
     a = run_chains(env, 1, sg.FP32, 20)[3]
     b = run_chains(env, 1, sg.FP32, 20)[3]
@@ -327,6 +403,44 @@ def test_refine_and_rescore_fp64_bit_exact(env):
             assert e[k] == eo and np.array_equal(cr[k], co) and np.array_equal(xr[k], xo)
             assert s0[k] == o.eval(env["ids"][li], confs[k])
             assert s1[k] == o.eval_intramolecular(env["ids"][li], confs[k])
+
+
+@pytest.mark.parametrize("backend", [0, 1])
+@pytest.mark.parametrize("accurate", [False, True])
+def test_run_bfgs_fp64_bit_exact(env, backend, accurate):
+    """sgpu_run_bfgs (run_bfgs, src/gpu/gpu_kernels.h:104-111): one
+    quasi_newton minimisation per conf on the maps or on the direct receptor
+    term, for every ligand size; FP64 energies, confs and evaluation counts
+    equal the oracle's bfgs (src/lib/bfgs.h:31-282) exactly; FP32 ends at a
+    conf whose FP64 energy is within the served tolerance of its own"""
+    ctx, o = env["ctx"], env["o"]
+    models, confs, iters = [], [], []
+    for li, f in enumerate(env["flats"]):
+        c = eval_confs_mix(f, env["gd"], 4, seed=900 + li)
+        for k in range(len(c)):
+            row = np.zeros(max(x.conf_size for x in env["flats"]))
+            row[:f.conf_size] = c[k]
+            confs.append(row)
+            models.append(li)
+            iters.append((25 + f.natoms) // 3)
+    confs = np.array(confs)
+    v = (10.0, 10.0, 10.0) if backend == 0 else (1000.0, 1000.0, 1000.0)
+    slope = 1e6 if backend == 0 else 10.0
+    kw = dict(grid_id=env["gids"][sg.FP64]) if backend == 0 else dict(receptor_id=env["rid"], gd=env["gd"])
+    e, c, ev = ctx.bfgs(env["bids"][sg.FP64], models, confs, iters, v, slope=slope, accurate=accurate, **kw)
+    for k, li in enumerate(models):
+        cs = env["flats"][li].conf_size
+        eo, co, evo = o.bfgs(env["ids"][li], confs[k, :cs], v, iters[k], backend=backend, slope=slope,
+                             accurate=accurate)
+        assert e[k] == eo and np.array_equal(c[k, :cs], co) and ev[k] == evo, (k, li, e[k], eo, ev[k], evo)
+    kw32 = dict(grid_id=env["gids"][sg.FP32]) if backend == 0 else kw
+    e32, c32, _ = ctx.bfgs(env["bids"][sg.FP32], models, confs, iters, v, slope=slope, accurate=accurate, **kw32)
+    for k, li in enumerate(models):
+        cs = env["flats"][li].conf_size
+        eo, _, xo = o.eval_deriv(env["ids"][li], c32[k, :cs], v, backend=backend, slope=slope)
+        if discontinuity_flags(env["flats"][li], env["gd"], env["rec"], [xo], backend)[0]:
+            continue
+        assert abs(e32[k] - eo) <= 1e-3 * max(1.0, abs(eo) / 10), (k, li, e32[k], eo)
 
 
 def test_atom_overlap_is_reported(env):
@@ -518,3 +632,28 @@ def test_search_energies_match_fp64_oracle(env, prec):
         assert abs(er[k] - eo) <= 1e-3 * max(1.0, abs(eo) / 10), (k, er[k], eo)
     print(f"precision {prec}: worst saved-pose |dE| vs FP64 oracle {worst:.2e}", flush=True)
     ctx.release_models(bid)
+
+
+@pytest.mark.parametrize("prec", [sg.FP32, sg.MIXED], ids=["fp32", "mixed"])
+def test_dock_reports_fp64_error_per_result(prec):
+    """every FP32 or mixed docking result carries the FP64 error of its top
+    pose: the refined energy in the search precision (e_search) against the
+    FP64 value of the refinement's own function at that conf (e_fp64, equal
+    to the oracle's model::eval_deriv on the non_cache back end), within the
+    served tolerance; an FP64 docking reports none"""
+    rec = receptor()
+    ligs = [sg.Mol.ligand(1, 24, 6), sg.Mol.ligand(3, 10, 0)]
+    d = sg.Docking(rec, gd_c0(), precision=prec, exhaustiveness=2, seed=5, step_fraction=0.05)
+    res = d.dock(ligs, confs=True)
+    o, ids, _, _ = oracle_for(ligs)
+    for k, r in enumerate(res):
+        assert r["fp64_error"] is not None
+        top = r["confs"][0]
+        refs = [o.eval_deriv(ids[k], top, (1000.0,) * 3, backend=1, slope=10.0 ** p)[0] for p in range(1, 6)]
+        print(f"ligand {k}: e_search {r['e_search']:.6f} e_fp64 {r['e_fp64']:.6f} error {r['fp64_error']:.2e}",
+              flush=True)
+        assert r["e_fp64"] in refs
+        assert r["fp64_error"] == abs(r["e_search"] - r["e_fp64"])
+        assert r["fp64_error"] <= 1e-3 * max(1.0, abs(r["e_fp64"]) / 10)
+    d64 = sg.Docking(rec, gd_c0(), fp64=True, exhaustiveness=1, seed=5, step_fraction=0.01)
+    assert d64.dock(ligs[1:])[0]["fp64_error"] is None
