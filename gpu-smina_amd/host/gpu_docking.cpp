@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <sstream>
 #include <cstring>
 #include <functional>
 #include <limits>
@@ -198,6 +199,60 @@ double normal(Gen& g)
 	}
 }
 } // namespace
+
+double term_value(const sgpu_term& t, int t1, int t2, double r, int atom_params)
+{
+	return term(t, t1, t2, r, atom_params);
+}
+
+// each term's cutoff: the last parameter its name carries (c)
+double term_cutoff(const sgpu_term& t)
+{
+	switch (t.kind)
+	{
+	case SGPU_TERM_REPULSION:
+	case SGPU_TERM_NON_DIR_ANTI_H_BOND_QUADRATIC:
+	case SGPU_TERM_DONOR_DONOR_QUADRATIC:
+	case SGPU_TERM_ACCEPTOR_ACCEPTOR_QUADRATIC:
+		return t.p[1];
+	case SGPU_TERM_VDW:
+		return t.p[4];
+	default:
+		return t.p[2];
+	}
+}
+
+// the term's name as smina prints it (src/lib/everything.h: parameters with
+// the stream's default formatting, vdw's powers as integers)
+std::string term_name(const sgpu_term& t)
+{
+	auto f = [](double x) {
+		std::ostringstream o;
+		o << x;
+		return o.str();
+	};
+	switch (t.kind)
+	{
+	case SGPU_TERM_GAUSS: return "gauss(o=" + f(t.p[0]) + ",_w=" + f(t.p[1]) + ",_c=" + f(t.p[2]) + ")";
+	case SGPU_TERM_REPULSION: return "repulsion(o=" + f(t.p[0]) + ",_c=" + f(t.p[1]) + ")";
+	case SGPU_TERM_HYDROPHOBIC: return "hydrophobic(g=" + f(t.p[0]) + ",_b=" + f(t.p[1]) + ",_c=" + f(t.p[2]) + ")";
+	case SGPU_TERM_NON_HYDROPHOBIC:
+		return "non_hydrophobic(g=" + f(t.p[0]) + ",_b=" + f(t.p[1]) + ",_c=" + f(t.p[2]) + ")";
+	case SGPU_TERM_NON_DIR_H_BOND:
+		return "non_dir_h_bond(g=" + f(t.p[0]) + ",_b=" + f(t.p[1]) + ",_c=" + f(t.p[2]) + ")";
+	case SGPU_TERM_NON_DIR_ANTI_H_BOND_QUADRATIC:
+		return "non_dir_anti_h_bond_quadratic(o=" + f(t.p[0]) + ",_c=" + f(t.p[1]) + ")";
+	case SGPU_TERM_DONOR_DONOR_QUADRATIC: return "donor_donor_quadratic(o=" + f(t.p[0]) + ",_c=" + f(t.p[1]) + ")";
+	case SGPU_TERM_ACCEPTOR_ACCEPTOR_QUADRATIC:
+		return "acceptor_acceptor_quadratic(o=" + f(t.p[0]) + ",_c=" + f(t.p[1]) + ")";
+	case SGPU_TERM_VDW:
+		return "vdw(i=" + f(double(int(t.p[0]))) + ",_j=" + f(double(int(t.p[1]))) + ",_s=" + f(t.p[2]) +
+				",_^=" + f(t.p[3]) + ",_c=" + f(t.p[4]) + ")";
+	case SGPU_TERM_NON_DIR_H_BOND_LJ:
+		return "non_dir_h_bond_lj(o=" + f(t.p[0]) + ",_^=" + f(t.p[1]) + ",_c=" + f(t.p[2]) + ")";
+	}
+	throw std::runtime_error("unknown scoring term");
+}
 
 Tables build_tables(const sgpu_scoring& sf, double factor)
 {
@@ -931,11 +986,16 @@ void GpuDocking::dock_batch(std::vector<DockJob*>& jobs)
 
 sgpu_grid_dims movable_atoms_box(const FlatModel& m, double add, double granularity)
 {
+	return movable_atoms_box_at(m.atom_init.data(), m.view.natoms, add, granularity);
+}
+
+sgpu_grid_dims movable_atoms_box_at(const double* xyz, int n, double add, double granularity)
+{
 	double c1[3] = {0, 0, 0}, c2[3] = {0, 0, 0};
-	for (int i = 0; i < m.view.natoms; i++)
+	for (int i = 0; i < n; i++)
 		for (int j = 0; j < 3; j++)
 		{
-			const double v = m.atom_init[size_t(3 * i + j)];
+			const double v = xyz[size_t(3 * i + j)];
 			if (i == 0 || v < c1[j])
 				c1[j] = v;
 			if (i == 0 || v > c2[j])

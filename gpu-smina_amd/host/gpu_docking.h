@@ -89,6 +89,15 @@ Splines build_splines(const sgpu_scoring& sf, double factor);
 
 void add_to_output_container(std::vector<Pose>& out, const Pose& t, double min_rmsd, size_t max_size);
 
+// one scoring term between atom types t1 and t2 at distance r, unweighted
+// (the term's own eval, src/lib/everything.h), its cutoff and its name as
+// smina prints it
+double term_value(const sgpu_term& t, int t1, int t2, double r, int atom_params);
+double term_cutoff(const sgpu_term& t);
+std::string term_name(const sgpu_term& t);
+// model::movable_atoms_box (src/lib/model.cpp:677-707) of n atoms at xyz
+sgpu_grid_dims movable_atoms_box_at(const double* xyz, int n, double add, double granularity);
+
 struct GpuSettings
 {
 	int device = 0;

@@ -1027,6 +1027,178 @@ std::string write_pdbqt_poses(const sgpu_mol& lig, int nposes, const double* con
 	return out.str();
 }
 
+namespace
+{
+// element symbols of the smina types (src/lib/atom_constants.h:150-216)
+const char* element_of(int t)
+{
+	static const char* el[NUM_ATOM_TYPES] = {"H", "H", "C", "C", "C", "C", "N", "N", "N", "N", "O", "O", "O", "O",
+			"S", "S", "P", "F", "Cl", "Br", "I", "Mg", "Mn", "Zn", "Ca", "Fe", "X"};
+	return t >= 0 && t < NUM_ATOM_TYPES ? el[t] : "X";
+}
+
+// the ligand's atoms: those of its trees (flexible side chains and fixed
+// atoms follow them)
+int ligand_atoms(const sgpu_mol& m)
+{
+	int end = 0;
+	for (int t = 0; t < m.nligands && t < int(m.tree_frame.size()); t++)
+	{
+		const int f1 = t + 1 < int(m.tree_frame.size()) ? m.tree_frame[size_t(t) + 1] : int(m.frame_parent.size());
+		for (int f = m.tree_frame[size_t(t)]; f < f1; f++)
+			end = std::max(end, m.frame_end[size_t(f)]);
+	}
+	return end;
+}
+
+// columns 7-15 of an atom's PDBQT line (model::ligand_atom_str,
+// src/lib/model.cpp:845-865): its serial number and name
+std::string atom_str(const sgpu_mol& m, int i)
+{
+	for (size_t k = 0; k < m.text.size(); k++)
+		if (m.text_atom[k] == i)
+			return m.text[k].size() > 6 ? m.text[k].substr(6, 9) : std::string();
+	return std::string();
+}
+
+double brick_d2(const double* lo, const double* hi, const double* v)
+{
+	double s = 0;
+	for (int k = 0; k < 3; k++)
+	{
+		const double c = v[k] <= lo[k] ? lo[k] : v[k] >= hi[k] ? hi[k] : v[k];
+		s += (c - v[k]) * (c - v[k]);
+	}
+	return s;
+}
+} // namespace
+
+// --atom_terms: each ligand atom's share of every term (weighted) at the
+// pose, as result_info::setAtomValues and writeAtomValues write it
+// (src/lib/result_info.cpp:41-76) from terms::evale_robust
+// (src/lib/terms.cpp:235-307): the receptor's heavy atoms and the model's
+// side-chain and fixed heavy atoms within the terms' largest cutoff of the
+// movable atoms' box; each term exact at the pair's distance (no tables)
+// where the distance is below its own cutoff
+std::string atom_terms(const sgpu_scoring& sf, const sgpu_mol& rec, const sgpu_mol& lig, const double* conf)
+{
+	FlatModel f = flatten(lig.desc);
+	const std::vector<double> x = pose_coords(f, conf);
+	const int nmov = lig.nmovable, nall = lig.natoms(), nlig = ligand_atoms(lig);
+	std::vector<double> xa(3 * size_t(nall));
+	for (int i = 0; i < nall; i++)
+		for (int k = 0; k < 3; k++)
+			xa[3 * size_t(i) + k] = i < nmov ? x[3 * size_t(i) + k] : lig.xyz[3 * size_t(i) + k];
+	double maxc = 0;
+	for (int k = 0; k < sf.nterms; k++)
+		maxc = std::max(maxc, term_cutoff(sf.terms[k]));
+	const double maxc2 = maxc * maxc;
+	const sgpu_grid_dims box = movable_atoms_box_at(xa.data(), nmov, 0, 0.375);
+	auto usable = [](int t) { return t < NUM_ATOM_TYPES && !is_hydrogen_type(t); };
+	// (coordinates, type) of the relevant atoms: receptor first, then the
+	// model's non-ligand atoms
+	std::vector<const double*> rx;
+	std::vector<int> rt;
+	for (int j = 0; j < rec.natoms(); j++)
+		if (usable(rec.type[size_t(j)]) && brick_d2(box.begin, box.end, &rec.xyz[3 * size_t(j)]) < maxc2)
+		{
+			rx.push_back(&rec.xyz[3 * size_t(j)]);
+			rt.push_back(rec.type[size_t(j)]);
+		}
+	for (int j = nlig; j < nall; j++)
+		if (usable(lig.type[size_t(j)]) && brick_d2(box.begin, box.end, &xa[3 * size_t(j)]) < maxc2)
+		{
+			rx.push_back(&xa[3 * size_t(j)]);
+			rt.push_back(lig.type[size_t(j)]);
+		}
+	std::ostringstream out;
+	out << "atomid el pos";
+	for (int k = 0; k < sf.nterms; k++)
+		out << " " << term_name(sf.terms[k]);
+	out << "\n";
+	for (int i = 0; i < nlig; i++)
+	{
+		std::vector<double> v(size_t(sf.nterms), 0.0);
+		const double* xi = &xa[3 * size_t(i)];
+		if (usable(lig.type[size_t(i)]))
+			for (size_t j = 0; j < rx.size(); j++)
+			{
+				const double d2 = (xi[0] - rx[j][0]) * (xi[0] - rx[j][0]) + (xi[1] - rx[j][1]) * (xi[1] - rx[j][1]) +
+						(xi[2] - rx[j][2]) * (xi[2] - rx[j][2]);
+				if (d2 > maxc2)
+					continue;
+				const double d = std::sqrt(d2);
+				for (int k = 0; k < sf.nterms; k++)
+					if (d < term_cutoff(sf.terms[k]))
+						v[size_t(k)] += term_value(sf.terms[k], lig.type[size_t(i)], rt[j], d, sf.atom_params);
+			}
+		out << atom_str(lig, i) << " <" << xi[0] << "," << xi[1] << "," << xi[2] << ">";
+		for (int k = 0; k < sf.nterms; k++)
+			out << " " << v[size_t(k)] * sf.terms[k].weight;
+		out << "\n";
+	}
+	out << "END\n";
+	return out.str();
+}
+
+// SDF poses (result_info::write's native SDF branch,
+// src/lib/result_info.cpp:139-163): per pose a molfile of the ligand's atoms
+// at the pose with the model's covalent bonds (single bonds: orders are not
+// perceived without OpenBabel), then the minimizedAffinity and
+// minimizedRMSD data items to 5 decimals, the atomic_interaction_terms item
+// when asked for, and $$$$
+std::string write_sdf_poses(const sgpu_mol& lig, int nposes, const double* conf, int conf_stride, const double* e,
+		const double* rmsd, const sgpu_scoring* sf, const sgpu_mol* rec)
+{
+	FlatModel f = flatten(lig.desc);
+	if (conf_stride < f.view.conf_size)
+		throw std::invalid_argument("conf stride below the conf size");
+	const int nlig = ligand_atoms(lig);
+	if (nlig > 999)
+		throw std::invalid_argument("SDF V2000 holds at most 999 atoms");
+	std::string name;
+	for (const std::string& l : lig.text)
+		if (l.compare(0, 15, "REMARK  Name = ") == 0)
+			name = l.substr(15);
+	std::vector<std::pair<int, int>> bonds;
+	for (size_t k = 0; k + 1 < lig.bonds.size(); k += 2)
+		if (lig.bonds[k] < nlig && lig.bonds[k + 1] < nlig)
+			bonds.push_back({lig.bonds[k], lig.bonds[k + 1]});
+	std::ostringstream out;
+	char buf[128];
+	for (int p = 0; p < nposes; p++)
+	{
+		const double* c = conf + size_t(p) * conf_stride;
+		const std::vector<double> x = pose_coords(f, c);
+		out << name << "\n  sgpu\n\n";
+		std::snprintf(buf, sizeof buf, "%3d%3d  0  0  0  0  0  0  0  0999 V2000", nlig, int(bonds.size()));
+		out << buf << "\n";
+		for (int i = 0; i < nlig; i++)
+		{
+			std::snprintf(buf, sizeof buf, "%10.4f%10.4f%10.4f %-3s 0  0  0  0  0  0  0  0  0  0  0  0",
+					x[3 * size_t(i)], x[3 * size_t(i) + 1], x[3 * size_t(i) + 2], element_of(lig.type[size_t(i)]));
+			out << buf << "\n";
+		}
+		for (const auto& b : bonds)
+		{
+			std::snprintf(buf, sizeof buf, "%3d%3d  1  0  0  0  0", b.first + 1, b.second + 1);
+			out << buf << "\n";
+		}
+		out << "M  END\n";
+		std::snprintf(buf, sizeof buf, "%.5f", e[p]);
+		out << "> <minimizedAffinity>\n" << buf << "\n\n";
+		if (rmsd && rmsd[p] >= 0)
+		{
+			std::snprintf(buf, sizeof buf, "%.5f", rmsd[p]);
+			out << "> <minimizedRMSD>\n" << buf << "\n\n";
+		}
+		if (sf && rec)
+			out << "> <atomic_interaction_terms>\n" << atom_terms(*sf, *rec, lig, c) << "\n\n";
+		out << "$$$$\n";
+	}
+	return out.str();
+}
+
 } // namespace sgpu_host
 
 using namespace sgpu_host;
@@ -1178,6 +1350,42 @@ int sgpu_write_pdbqt_poses(const sgpu_mol* lig, int nposes, const double* conf, 
 		if (!lig || nposes < 0 || (nposes > 0 && (!conf || !e)))
 			return -1;
 		const std::string s = write_pdbqt_poses(*lig, nposes, conf, conf_stride, e, rmsd);
+		if (out && out_len > 0)
+			std::snprintf(out, size_t(out_len), "%s", s.c_str());
+		return int(s.size());
+	}
+	catch (...)
+	{
+		return -1;
+	}
+}
+
+int sgpu_atom_terms(const sgpu_scoring* sf, const sgpu_mol* receptor, const sgpu_mol* ligand, const double* conf,
+		char* out, int out_len)
+{
+	try
+	{
+		if (!sf || !receptor || !ligand || !conf)
+			return -1;
+		const std::string s = atom_terms(*sf, *receptor, *ligand, conf);
+		if (out && out_len > 0)
+			std::snprintf(out, size_t(out_len), "%s", s.c_str());
+		return int(s.size());
+	}
+	catch (...)
+	{
+		return -1;
+	}
+}
+
+int sgpu_write_sdf_poses(const sgpu_mol* lig, int nposes, const double* conf, int conf_stride, const double* e,
+		const double* rmsd, const sgpu_scoring* atom_terms_sf, const sgpu_mol* receptor, char* out, int out_len)
+{
+	try
+	{
+		if (!lig || nposes < 0 || (nposes > 0 && (!conf || !e)) || (atom_terms_sf && !receptor))
+			return -1;
+		const std::string s = write_sdf_poses(*lig, nposes, conf, conf_stride, e, rmsd, atom_terms_sf, receptor);
 		if (out && out_len > 0)
 			std::snprintf(out, size_t(out_len), "%s", s.c_str());
 		return int(s.size());

@@ -207,6 +207,9 @@ def _load():
         "sgpu_read_pdbqt_receptor": (vp, [C.c_char_p, C.c_char_p, C.c_int]),
         "sgpu_read_pdbqt_flex": (vp, [vp, vp, C.c_char_p, C.c_char_p, C.c_int]),
         "sgpu_write_pdbqt_poses": (C.c_int, [vp, C.c_int, F64P, C.c_int, F64P, F64P, C.c_char_p, C.c_int]),
+        "sgpu_write_sdf_poses": (C.c_int, [vp, C.c_int, F64P, C.c_int, F64P, F64P, C.POINTER(Scoring), vp,
+                                           C.c_char_p, C.c_int]),
+        "sgpu_atom_terms": (C.c_int, [C.POINTER(Scoring), vp, vp, F64P, C.c_char_p, C.c_int]),
         "sgpu_synth_flex": (vp, [vp, vp, C.c_uint32, C.c_int, C.c_int, C.c_int, F64P, C.POINTER(vp)]),
         "sgpu_vina_scoring": (None, [C.POINTER(Term), C.POINTER(CITerm), C.POINTER(Scoring), C.c_double]),
         "sgpu_conf_independent": (C.c_double, [C.POINTER(Scoring), C.c_double, C.c_double]),
@@ -342,6 +345,32 @@ class Mol:
         buf = C.create_string_buffer(n + 1)
         lib.sgpu_write_pdbqt_poses(self.h, len(e), ptr(confs, F64P), confs.shape[1], ptr(e, F64P),
                                    None if r is None else ptr(r, F64P), buf, n + 1)
+        return buf.value.decode()
+
+    def write_sdf(self, confs, energies, rmsds=None, atom_terms=None):
+        """the poses as SDF (sgpu_write_sdf_poses); atom_terms: (scoring,
+        receptor) to add each pose's atomic_interaction_terms item"""
+        confs = np.atleast_2d(f64(confs))
+        e = f64(energies)
+        r = None if rmsds is None else f64(rmsds)
+        sf, rec = (C.byref(atom_terms[0]), atom_terms[1].h) if atom_terms else (None, None)
+        args = (self.h, len(e), ptr(confs, F64P), confs.shape[1], ptr(e, F64P), None if r is None else ptr(r, F64P),
+                sf, rec)
+        n = lib.sgpu_write_sdf_poses(*args, None, 0)
+        if n < 0:
+            raise ValueError("cannot write these poses")
+        buf = C.create_string_buffer(n + 1)
+        lib.sgpu_write_sdf_poses(*args, buf, n + 1)
+        return buf.value.decode()
+
+    def atom_terms(self, scoring, receptor, conf):
+        """--atom_terms: the per-atom term table of this pose (sgpu_atom_terms)"""
+        c = f64(conf)
+        n = lib.sgpu_atom_terms(C.byref(scoring), receptor.h, self.h, ptr(c, F64P), None, 0)
+        if n < 0:
+            raise ValueError("cannot compute atom terms")
+        buf = C.create_string_buffer(n + 1)
+        lib.sgpu_atom_terms(C.byref(scoring), receptor.h, self.h, ptr(c, F64P), buf, n + 1)
         return buf.value.decode()
 
     @staticmethod

@@ -87,6 +87,25 @@ int sgpu_write_pdbqt_poses(const sgpu_mol* ligand, int nposes, const double* con
 int sgpu_read_user_grid(const char* text, double scaling, sgpu_grid_dims* gd, double* data, int capacity, char* err,
 		int err_len);
 
+/* --atom_terms (result_info::setAtomValues / writeAtomValues,
+ * src/lib/result_info.cpp:41-76, from terms::evale_robust,
+ * src/lib/terms.cpp:235-307): the table smina writes for a pose: a header
+ * "atomid el pos <term names>", then per ligand atom its PDBQT serial and
+ * name, "<x,y,z>" and each term's weighted share (the exact term at each
+ * pair's distance against the receptor's heavy atoms and the model's
+ * side-chain and fixed heavy atoms near the movable atoms' box), then
+ * "END".  Returns the full length (writes at most out_len bytes), or -1. */
+int sgpu_atom_terms(const sgpu_scoring* sf, const sgpu_mol* receptor, const sgpu_mol* ligand, const double* conf,
+		char* out, int out_len);
+/* SDF poses (result_info::write's native SDF branch, src/lib/result_info.cpp:
+ * 139-163): per pose a V2000 molfile of the ligand's atoms at the pose with
+ * its covalent bonds (all single: bond orders are not perceived without
+ * OpenBabel), the minimizedAffinity and minimizedRMSD data items (rmsd NULL
+ * or negative: none), the atomic_interaction_terms item when atom_terms_sf
+ * and receptor are given, and $$$$.  Returns the full length, or -1. */
+int sgpu_write_sdf_poses(const sgpu_mol* ligand, int nposes, const double* conf, int conf_stride, const double* e,
+		const double* rmsd, const sgpu_scoring* atom_terms_sf, const sgpu_mol* receptor, char* out, int out_len);
+
 /* ---- scoring */
 /* A built-in scoring function by smina's name (builtin_scoring,
  * src/lib/builtinscoring.cpp:65-113): "vina" (= "default"), "vinardo",
