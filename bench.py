@@ -259,7 +259,8 @@ def main():
     ap.add_argument("--no-compare", action="store_true",
                     help="with --fast: skip the default-search run that the fast screen is compared with")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-plan autotuner (default plans)")
-    ap.add_argument("--tune-steps", type=int, default=16, help="autotuner: Monte Carlo steps per probe")
+    ap.add_argument("--tune-steps", type=int, default=512,
+                    help="autotuner: Monte Carlo steps of the longest chain per probe (the others scaled alike)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only a 3 x 1 GiB device copy (the reference for HBM byte counters)")
     args = ap.parse_args()

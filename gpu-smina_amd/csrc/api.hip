@@ -97,12 +97,21 @@ struct Tables
 	int n = 0;
 	double factor = 0, cutoff_sqr = 0;
 	DBuf smooth32, smooth64, fast;
+	// charge-dependent terms: the other three components
+	int ncomp = 1;
+	DBuf qsmooth32, qsmooth64, qfast;
 	bool ok = false;
 	// precalculate_splines (kind 1)
 	int kind = 0;
 	DBuf spl, spl_cut;
 	double fraction = 0;
 	int nseg = 0;
+	// free every buffer (their frees are ordered on the context's stream)
+	void release()
+	{
+		for (DBuf* b : {&smooth32, &smooth64, &fast, &qsmooth32, &qsmooth64, &qfast, &spl, &spl_cut})
+			b->reset();
+	}
 };
 
 struct Cells
@@ -118,7 +127,8 @@ struct Receptor
 	int n = 0;
 	std::vector<double> xyz;
 	std::vector<int> type;
-	DBuf xyz64, xyz32, typ;
+	std::vector<double> charge;
+	DBuf xyz64, xyz32, typ, q64, q32;
 	// direct-receptor cells per box (full 3 A bricks)
 	std::map<std::vector<double>, std::unique_ptr<Cells>> cells;
 };
@@ -134,6 +144,9 @@ struct Grid
 	int slot_of[NUM_ATOM_TYPES];
 	DBuf points; // FP64 point maps
 	DBuf cells;  // corner-packed, precision
+	// charge maps (four-component tables), same layouts; empty otherwise
+	bool charged = false;
+	DBuf qpoints, qcells;
 	DBuf slot_dev;
 };
 
@@ -546,6 +559,11 @@ void pack_batch(const sgpu_flat_model* ms, int n, HostBatch& B)
 		R.insert(R.end(), m.frame_origin, m.frame_origin + 3 * m.nframes);
 		h.o_faxis = int(R.size());
 		R.insert(R.end(), m.frame_axis, m.frame_axis + 3 * m.nframes);
+		h.o_charge = int(R.size());
+		if (m.atom_charge)
+			R.insert(R.end(), m.atom_charge, m.atom_charge + na);
+		else
+			R.insert(R.end(), size_t(na), 0.0);
 		// append
 		h.iblob_off = int(B.iblob.size());
 		h.iblob_len = int(I.size());
@@ -610,6 +628,7 @@ int plan_bytes(const WavePlan& p, int prec, int waves)
 int eval_precision(int prec) { return prec == SGPU_FP64 ? SGPU_FP64 : SGPU_FP32; }
 constexpr int LDS_LIMIT = 160 * 1024;
 constexpr int LDS_STATIC = 64; // the persistent Monte Carlo kernel's own LDS (its work item)
+constexpr double TUNE_MARGIN = 0.03; // a tuned plan must beat the default by this fraction
 
 template<typename T>
 TablePtrs<T> table_ptrs(const sgpu_ctx* c)
@@ -617,6 +636,9 @@ TablePtrs<T> table_ptrs(const sgpu_ctx* c)
 	TablePtrs<T> t;
 	t.smooth = (sizeof(T) == 8 ? c->tab.smooth64 : c->tab.smooth32).template as<T>();
 	t.fast = c->tab.fast.as<double>();
+	t.ncomp = c->tab.ncomp;
+	t.qsmooth = (sizeof(T) == 8 ? c->tab.qsmooth64 : c->tab.qsmooth32).template as<T>();
+	t.qfast = c->tab.qfast.as<double>();
 	t.n = c->tab.n;
 	t.factor = c->tab.factor;
 	t.cutoff_sqr = c->tab.cutoff_sqr;
@@ -634,6 +656,7 @@ GridPtrs<T> grid_ptrs(const Grid& g, double slope)
 {
 	GridPtrs<T> p;
 	p.cells = g.cells.as<T>();
+	p.qcells = g.charged ? g.qcells.as<T>() : nullptr;
 	p.slot_of = g.slot_dev.as<int>();
 	for (int d = 0; d < 3; d++)
 	{
@@ -671,6 +694,7 @@ void attach_user_grid(const sgpu_ctx* c, EvalCtx<T>& ec)
 		return;
 	GridPtrs<T>& p = ec.ug;
 	p.cells = (sizeof(T) == 8 ? c->ug.cells64 : c->ug.cells32).template as<T>();
+	p.qcells = nullptr;
 	p.slot_of = nullptr;
 	for (int d = 0; d < 3; d++)
 	{
@@ -759,6 +783,7 @@ ReceptorPtrs<T> receptor_ptrs(const Receptor& r, const Cells& cl)
 	ReceptorPtrs<T> p;
 	p.xyz = (sizeof(T) == 8 ? r.xyz64 : r.xyz32).template as<T>();
 	p.type = r.typ.as<int>();
+	p.charge = (sizeof(T) == 8 ? r.q64 : r.q32).template as<T>();
 	p.natoms = r.n;
 	p.cell_start = cl.start.as<int>();
 	p.cell_atoms = cl.atoms.as<int>();
@@ -907,11 +932,7 @@ void sgpu_destroy(sgpu_ctx* c)
 	c->batches.clear();
 	c->grids.clear();
 	c->receptors.clear();
-	c->tab.smooth32.reset();
-	c->tab.smooth64.reset();
-	c->tab.fast.reset();
-	c->tab.spl.reset();
-	c->tab.spl_cut.reset();
+	c->tab.release();
 	c->scratch.reset();
 	c->scratch2.reset();
 	c->err_flag.reset();
@@ -980,6 +1001,7 @@ sgpu_status sgpu_upload_splines(sgpu_ctx* c, const sgpu_splines* s)
 		c->tab.nseg = s->nseg;
 		c->tab.cutoff_sqr = s->cutoff_sqr;
 		c->tab.kind = 1;
+		c->tab.ncomp = 1; // one component: charge-dependent terms are refused with splines
 		c->tab.ok = true;
 		hip_check(hipStreamSynchronize(c->stream), "upload splines");
 	});
@@ -995,6 +1017,16 @@ sgpu_status sgpu_upload_tables(sgpu_ctx* c, const sgpu_tables* t)
 		upload(c->tab.smooth64, sm, c->stream);
 		upload(c->tab.smooth32, cast<float>(sm), c->stream);
 		upload(c->tab.fast, fa, c->stream);
+		require(t->ncomp == 1 || t->ncomp == 4, "tables have 1 or 4 components");
+		require(t->ncomp == 1 || (t->qsmooth && t->qfast), "charge components missing");
+		if (t->ncomp == 4)
+		{
+			std::vector<double> qs(t->qsmooth, t->qsmooth + 6 * ne), qf(t->qfast, t->qfast + 3 * ne);
+			upload(c->tab.qsmooth64, qs, c->stream);
+			upload(c->tab.qsmooth32, cast<float>(qs), c->stream);
+			upload(c->tab.qfast, qf, c->stream);
+		}
+		c->tab.ncomp = t->ncomp;
 		c->tab.n = t->n;
 		c->tab.factor = t->factor;
 		c->tab.cutoff_sqr = t->cutoff_sqr;
@@ -1049,9 +1081,15 @@ sgpu_status sgpu_upload_receptor(sgpu_ctx* c, const sgpu_receptor* r, int* id)
 		R->type.assign(r->type, r->type + r->natoms);
 		for (int t : R->type)
 			require(t >= 0 && t < SGPU_NUM_TYPES, "receptor atom type out of range");
+		if (r->charge)
+			R->charge.assign(r->charge, r->charge + r->natoms);
+		else
+			R->charge.assign(size_t(r->natoms), 0.0);
 		upload(R->xyz64, R->xyz, c->stream);
 		upload(R->xyz32, cast<float>(R->xyz), c->stream);
 		upload(R->typ, R->type, c->stream);
+		upload(R->q64, R->charge, c->stream);
+		upload(R->q32, cast<float>(R->charge), c->stream);
 		hip_check(hipStreamSynchronize(c->stream), "upload receptor");
 		*id = c->next_id++;
 		c->receptors[*id] = std::move(R);
@@ -1084,8 +1122,11 @@ sgpu_status sgpu_build_grids(sgpu_ctx* c, int receptor_id, const sgpu_grid_dims*
 		GridBuildArgs a{};
 		a.rec_xyz = r.xyz64.as<double>();
 		a.rec_type = r.typ.as<int>();
+		a.rec_q = r.q64.as<double>();
 		a.rec_n = r.n;
 		a.fast = c->tab.fast.as<double>();
+		a.ncomp = c->tab.kind == 0 ? c->tab.ncomp : 1;
+		a.qfast = c->tab.qfast.as<double>();
 		a.table_n = c->tab.n;
 		a.factor = c->tab.factor;
 		a.cutoff_sqr = c->tab.cutoff_sqr;
@@ -1150,19 +1191,27 @@ sgpu_status sgpu_build_grids(sgpu_ctx* c, int receptor_id, const sgpu_grid_dims*
 		hip_check(launch_cell_fill(a, c->stream), "cell fill");
 		const size_t npts = size_t(g->dim[0]) * g->dim[1] * g->dim[2];
 		g->points.alloc(npts * ntypes * sizeof(double));
-		hip_check(launch_grid_points(a, g->points.as<double>(), c->stream), "grid points");
+		g->charged = a.ncomp == 4;
+		if (g->charged)
+			g->qpoints.alloc(npts * ntypes * sizeof(double));
+		hip_check(launch_grid_points(a, g->points.as<double>(), g->qpoints.as<double>(), c->stream), "grid points");
 		const size_t ncell = size_t(g->dim[0] - 1) * (g->dim[1] - 1) * (g->dim[2] - 1) * ntypes * 8;
-		if (precision == SGPU_FP64)
+		for (int q = 0; q < (g->charged ? 2 : 1); q++)
 		{
-			g->cells.alloc(ncell * sizeof(double));
-			hip_check(launch_grid_pack<double>(g->points.as<double>(), ntypes, g->dim, g->cells.as<double>(), c->stream),
-					"grid pack");
-		}
-		else
-		{
-			g->cells.alloc(ncell * sizeof(float));
-			hip_check(launch_grid_pack<float>(g->points.as<double>(), ntypes, g->dim, g->cells.as<float>(), c->stream),
-					"grid pack");
+			DBuf& pts = q ? g->qpoints : g->points;
+			DBuf& cells = q ? g->qcells : g->cells;
+			if (precision == SGPU_FP64)
+			{
+				cells.alloc(ncell * sizeof(double));
+				hip_check(launch_grid_pack<double>(pts.as<double>(), ntypes, g->dim, cells.as<double>(), c->stream),
+						"grid pack");
+			}
+			else
+			{
+				cells.alloc(ncell * sizeof(float));
+				hip_check(launch_grid_pack<float>(pts.as<double>(), ntypes, g->dim, cells.as<float>(), c->stream),
+						"grid pack");
+			}
 		}
 		std::vector<int> slots(g->slot_of, g->slot_of + NUM_ATOM_TYPES);
 		upload(g->slot_dev, slots, c->stream);
@@ -1196,21 +1245,33 @@ sgpu_status sgpu_share_grid(sgpu_ctx* dst, sgpu_ctx* src, int src_grid, int* gri
 	});
 }
 
+namespace
+{
+void download_map(sgpu_ctx* c, int grid_id, int type, bool charge, double* out)
+{
+	Grid& g = find_grid(c, grid_id);
+	require(out && type >= 0 && type < NUM_ATOM_TYPES && g.slot_of[type] >= 0, "type has no map");
+	require(!charge || g.charged, "the maps have no charge maps (tables without charge-dependent terms)");
+	const size_t npts = size_t(g.dim[0]) * g.dim[1] * g.dim[2];
+	std::vector<double> pts(npts);
+	hip_check(hipMemcpyAsync(pts.data(), (charge ? g.qpoints : g.points).as<double>() + size_t(g.slot_of[type]) * npts,
+					  npts * sizeof(double), hipMemcpyDeviceToHost, c->stream),
+			"download grid");
+	hip_check(hipStreamSynchronize(c->stream), "download grid");
+	// stored precision: round through it
+	for (size_t k = 0; k < npts; k++)
+		out[k] = g.precision == SGPU_FP64 ? pts[k] : double(float(pts[k]));
+}
+} // namespace
+
+sgpu_status sgpu_download_charge_grid(sgpu_ctx* c, int grid_id, int type, double* out)
+{
+	return guarded(c, [&]() { download_map(c, grid_id, type, true, out); });
+}
+
 sgpu_status sgpu_download_grid(sgpu_ctx* c, int grid_id, int type, double* out)
 {
-	return guarded(c, [&]() {
-		Grid& g = find_grid(c, grid_id);
-		require(out && type >= 0 && type < NUM_ATOM_TYPES && g.slot_of[type] >= 0, "type has no map");
-		const size_t npts = size_t(g.dim[0]) * g.dim[1] * g.dim[2];
-		std::vector<double> pts(npts);
-		hip_check(hipMemcpyAsync(pts.data(), g.points.as<double>() + size_t(g.slot_of[type]) * npts, npts * sizeof(double),
-						  hipMemcpyDeviceToHost, c->stream),
-				"download grid");
-		hip_check(hipStreamSynchronize(c->stream), "download grid");
-		// stored precision: round through it
-		for (size_t k = 0; k < npts; k++)
-			out[k] = g.precision == SGPU_FP64 ? pts[k] : double(float(pts[k]));
-	});
+	return guarded(c, [&]() { download_map(c, grid_id, type, false, out); });
 }
 
 sgpu_status sgpu_release_grid(sgpu_ctx* c, int grid_id)
@@ -1414,6 +1475,39 @@ __global__ void early_stop_kernel(int nmodels, const int* mstart, const int* mch
 	}
 }
 
+// the occupancy rule of a size class: chains per workgroup for the most
+// resident waves per CU (registers and LDS, from the runtime's occupancy
+// calculator), then the larger workgroup; 0 when nothing fits
+template<typename T, typename S>
+int occupancy_waves(const WavePlan& plan, int precision, int variant, int maxper, int* resident)
+{
+	int waves = 0, best = 0;
+	for (int w = 1; w <= std::min(MAX_CHAINS_PER_GROUP, maxper); w++)
+	{
+		const int bytes = plan_bytes(plan, precision, w);
+		if (bytes + LDS_STATIC > LDS_LIMIT)
+			break;
+		const int r = w * mc_blocks_per_cu<T, S>(w, size_t(bytes), variant);
+		if (r >= best && r > 0)
+		{
+			best = r;
+			waves = w;
+		}
+	}
+	if (resident)
+		*resident = best;
+	return waves;
+}
+
+// the default form: the persistent grid unless its kernel spills where the
+// one-shot kernel does not; SGPU_QUEUE=0 / 1 forces either
+template<typename T, typename S>
+bool persist_default(int variant)
+{
+	const char* qenv = std::getenv("SGPU_QUEUE");
+	return qenv ? std::atoi(qenv) != 0 : mc_private_bytes<T, S>(variant, true) <= mc_private_bytes<T, S>(variant, false);
+}
+
 template<typename T, typename S>
 void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, const sgpu_chains* ch,
 		sgpu_chain_results* out)
@@ -1533,25 +1627,9 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 		int maxper = 1;
 		for (int mi : k.models)
 			maxper = std::max(maxper, int(by_model[mi].size()));
-		// chains per workgroup: the most resident waves per CU (registers
-		// and LDS, from the runtime's occupancy calculator), then the larger
-		// workgroup
-		k.waves = 0;
-		int best = 0;
-		const int variant = c->tab.kind == 1 ? 2 : p->visited ? 1 : 0;
-		for (int w = 1; w <= std::min(MAX_CHAINS_PER_GROUP, maxper); w++)
-		{
-			const int bytes = plan_bytes(k.plan, b.precision, w);
-			if (bytes + LDS_STATIC > LDS_LIMIT)
-				break;
-			const int resident = w * mc_blocks_per_cu<T, S>(w, size_t(bytes), variant);
-			if (resident >= best && resident > 0)
-			{
-				best = resident;
-				k.waves = w;
-			}
-		}
-		k.resident = best;
+		// chains per workgroup by the occupancy rule
+		const int variant = (c->tab.kind == 1 || c->tab.ncomp == 4) ? 2 : p->visited ? 1 : 0;
+		k.waves = occupancy_waves<T, S>(k.plan, b.precision, variant, maxper, &k.resident);
 		require(k.waves > 0, "chain state exceeds LDS");
 		// a probe's override, else the tuned plan of this class, where it fits
 		const int ki = int(&k - cls);
@@ -1656,17 +1734,14 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 	// class and part, cleared by a kernel at the start.
 	hipDeviceProp_t prop;
 	hip_check(hipGetDeviceProperties(&prop, c->device), "hipGetDeviceProperties");
-	const int variant = c->tab.kind == 1 ? 2 : p->visited ? 1 : 0;
-	// the persistent form unless it spills where the one-shot form does not
-	// (or a tuned plan says otherwise); SGPU_QUEUE=0 / 1 forces either
-	const char* qenv = std::getenv("SGPU_QUEUE");
-	const bool persist_default = qenv ? std::atoi(qenv) != 0
-	                                  : mc_private_bytes<T, S>(variant, true) <= mc_private_bytes<T, S>(variant, false);
+	const int variant = (c->tab.kind == 1 || c->tab.ncomp == 4) ? 2 : p->visited ? 1 : 0;
+	// the default form unless a tuned plan says otherwise
+	const bool pdef = persist_default<T, S>(variant);
 	for (Cls& k : cls)
 		if (!k.blks.empty())
 		{
 			if (!k.tuned)
-				k.persist = persist_default;
+				k.persist = pdef;
 			const int per_cu = mc_blocks_per_cu<T, S>(k.waves, size_t(plan_bytes(k.plan, b.precision, k.waves)),
 					variant, k.persist);
 			k.grid = k.persist ? std::max(1, std::min(int(k.blks.size()), per_cu * prop.multiProcessorCount))
@@ -1859,12 +1934,14 @@ void run_tune(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, c
 {
 	Batch& b = find_batch(c, batch_id);
 	require(p && ch && ch->nchains > 0 && probe_steps >= 1 && rep, "tune request incomplete");
-	const int variant = c->tab.kind == 1 ? 2 : p->visited ? 1 : 0;
+	const int variant = (c->tab.kind == 1 || c->tab.ncomp == 4) ? 2 : p->visited ? 1 : 0;
 	std::vector<int> members[sgpu_ctx::NCLASS];
+	int hmax = 1;
 	for (int i = 0; i < ch->nchains; i++)
 	{
 		require(ch->model[i] >= 0 && ch->model[i] < b.n, "chain model out of range");
 		members[mc_class(b.h.hdr[ch->model[i]])].push_back(i);
+		hmax = std::max(hmax, b.h.hdr[ch->model[i]].nheavy);
 	}
 	*rep = sgpu_mc_tune_report{};
 	sgpu_mc_params probe = *p;
@@ -1882,77 +1959,133 @@ void run_tune(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, c
 				o = false;
 		}
 	} restore{c, saved_fn};
+	// per class: the largest chains per workgroup up to a target that its
+	// LDS plan, its chains per ligand and the kernel form allow
+	struct ClassInfo
+	{
+		WavePlan plan;
+		int maxper = 1;
+	};
+	ClassInfo info[sgpu_ctx::NCLASS];
 	for (int k = 0; k < sgpu_ctx::NCLASS; k++)
 	{
 		if (members[k].empty())
 			continue;
-		const std::vector<int>& mem = members[k];
-		const int n = int(mem.size());
-		std::vector<int> model, steps(n, probe_steps), iters, unit;
-		std::vector<double> conf;
-		std::vector<uint32_t> rng;
 		std::map<int, int> per_model;
 		std::vector<int> models;
-		int hmax = 1;
-		for (int i : mem)
-		{
-			model.push_back(ch->model[i]);
-			iters.push_back(ch->maxiters[i]);
-			unit.push_back(ch->unit ? ch->unit[i] : 0);
-			conf.insert(conf.end(), ch->conf + size_t(i) * ch->conf_stride,
-					ch->conf + size_t(i + 1) * ch->conf_stride);
-			rng.insert(rng.end(), ch->rng + size_t(i) * 625, ch->rng + size_t(i + 1) * 625);
+		for (int i : members[k])
 			if (per_model[ch->model[i]]++ == 0)
 				models.push_back(ch->model[i]);
-			hmax = std::max(hmax, b.h.hdr[ch->model[i]].nheavy);
-		}
-		int maxper = 1;
+		info[k].plan = plan_for(b.h, models);
 		for (auto& kv : per_model)
-			maxper = std::max(maxper, kv.second);
-		const WavePlan plan = plan_for(b.h, models);
-		sgpu_chains sub{n, model.data(), conf.data(), ch->conf_stride, rng.data(), steps.data(), iters.data(),
-				unit.data()};
-		const int maxp = p->num_saved_mins;
-		std::vector<int> cnt(n);
-		std::vector<double> oe(size_t(n) * maxp), oc(size_t(n) * maxp * ch->conf_stride),
-				ox(size_t(n) * maxp * 3 * hmax);
-		sgpu_chain_results res{maxp, ch->conf_stride, 3 * hmax, cnt.data(), oe.data(), oc.data(), ox.data(), nullptr};
-		std::vector<sgpu_ctx::ClassPlan> cands;
-		for (int w = 1; w <= std::min(MAX_CHAINS_PER_GROUP, maxper); w *= 2)
+			info[k].maxper = std::max(info[k].maxper, kv.second);
+		rep->class_index[rep->nclasses++] = k;
+	}
+	auto fit = [&](int k, int target, bool pers) {
+		for (int w = std::min({target, MAX_CHAINS_PER_GROUP, info[k].maxper}); w >= 1; w--)
 		{
-			const int bytes = plan_bytes(plan, b.precision, w);
-			if (bytes + LDS_STATIC > LDS_LIMIT)
-				break;
-			for (int pers = 0; pers < 2; pers++)
-				if (mc_blocks_per_cu<T, S>(w, size_t(bytes), variant, pers != 0) > 0)
-					cands.push_back({w, pers != 0});
+			const int bytes = plan_bytes(info[k].plan, b.precision, w);
+			if (bytes + LDS_STATIC <= LDS_LIMIT && mc_blocks_per_cu<T, S>(w, size_t(bytes), variant, pers) > 0)
+				return w;
 		}
-		require(!cands.empty(), "no launch plan fits this class");
-		const int ci = rep->nclasses++;
-		rep->class_index[ci] = k;
-		c->override_on[k] = true;
-		double best_ms = 0;
-		for (int j = -1; j < int(cands.size()) && j < SGPU_TUNE_MAX_CANDIDATES; j++)
+		return 0;
+	};
+	// joint candidates: one plan for every class at once, probed on the
+	// whole batch, since the classes share the device (workgroups of one
+	// class that leave part of a CU free can keep the larger workgroups of
+	// another waiting).  The first is the default (the occupancy rule and
+	// the default form), then a target of 8, 4, 2 or 1 chains per workgroup
+	// in either form.
+	struct Joint
+	{
+		sgpu_ctx::ClassPlan plan[sgpu_ctx::NCLASS];
+	};
+	std::vector<Joint> cands;
+	{
+		Joint d{};
+		for (int k = 0; k < sgpu_ctx::NCLASS; k++)
+			if (!members[k].empty())
+				d.plan[k] = {occupancy_waves<T, S>(info[k].plan, b.precision, variant, info[k].maxper, nullptr),
+						persist_default<T, S>(variant)};
+		cands.push_back(d);
+	}
+	for (int target : {8, 4, 2, 1})
+		for (int pers = 0; pers < 2; pers++)
 		{
-			// j = -1: an untimed first run (code objects, buffers)
-			c->plan_override[k] = cands[size_t(std::max(0, j))];
-			run_mc<T, S>(c, batch_id, grid_id, &probe, &sub, &res);
-			if (j < 0)
-				continue;
-			const double ms = c->last_ms["mc"];
-			rep->cand_waves[ci][j] = cands[size_t(j)].waves;
-			rep->cand_persist[ci][j] = cands[size_t(j)].persist ? 1 : 0;
+			Joint j{};
+			bool ok = true;
+			for (int k = 0; k < sgpu_ctx::NCLASS; k++)
+				if (!members[k].empty())
+				{
+					j.plan[k] = {fit(k, target, pers != 0), pers != 0};
+					ok = ok && j.plan[k].waves > 0;
+				}
+			bool dup = false;
+			for (const Joint& o : cands)
+			{
+				bool same = true;
+				for (int k = 0; k < sgpu_ctx::NCLASS; k++)
+					same = same && o.plan[k].waves == j.plan[k].waves && o.plan[k].persist == j.plan[k].persist;
+				dup = dup || same;
+			}
+			if (ok && !dup && int(cands.size()) < SGPU_TUNE_MAX_CANDIDATES)
+				cands.push_back(j);
+		}
+	require(!cands.empty(), "no launch plan fits these chains");
+	// each chain's steps scaled by one factor, the longest to probe_steps:
+	// the probe keeps the batch's spread of chain lengths, which sets how
+	// the classes' workgroups finish and refill the device
+	const int n = ch->nchains;
+	int longest = 1;
+	for (int i = 0; i < n; i++)
+		longest = std::max(longest, ch->num_steps[i]);
+	std::vector<int> steps(n);
+	for (int i = 0; i < n; i++)
+		steps[size_t(i)] = std::max(1, int(std::ceil(double(ch->num_steps[i]) * std::min(probe_steps, longest) / longest)));
+	sgpu_chains all = *ch;
+	all.num_steps = steps.data();
+	const int maxp = p->num_saved_mins;
+	std::vector<int> cnt(n);
+	std::vector<double> oe(size_t(n) * maxp), oc(size_t(n) * maxp * ch->conf_stride), ox(size_t(n) * maxp * 3 * hmax);
+	sgpu_chain_results res{maxp, ch->conf_stride, 3 * hmax, cnt.data(), oe.data(), oc.data(), ox.data(), nullptr};
+	double best_ms = 0;
+	int best = 0;
+	for (int j = -1; j < int(cands.size()); j++)
+	{
+		// j = -1: an untimed first run (code objects, buffers)
+		const Joint& cj = cands[size_t(std::max(0, j))];
+		for (int k = 0; k < sgpu_ctx::NCLASS; k++)
+		{
+			c->override_on[k] = !members[k].empty();
+			c->plan_override[k] = cj.plan[k];
+		}
+		run_mc<T, S>(c, batch_id, grid_id, &probe, &all, &res);
+		if (j < 0)
+			continue;
+		const double ms = c->last_ms["mc"];
+		for (int ci = 0; ci < rep->nclasses; ci++)
+		{
+			const int k = rep->class_index[ci];
+			rep->cand_waves[ci][j] = cj.plan[k].waves;
+			rep->cand_persist[ci][j] = cj.plan[k].persist ? 1 : 0;
 			rep->cand_ms[ci][j] = ms;
 			rep->ncand[ci] = j + 1;
-			if (j == 0 || ms < best_ms)
-			{
-				best_ms = ms;
-				rep->chosen_waves[ci] = cands[size_t(j)].waves;
-				rep->chosen_persist[ci] = cands[size_t(j)].persist ? 1 : 0;
-			}
 		}
-		c->override_on[k] = false;
-		c->tuned[{b.precision, variant, k}] = {rep->chosen_waves[ci], rep->chosen_persist[ci] != 0};
+		if (j == 0)
+			best_ms = ms;
+		// the default stays unless beaten by more than the probe's noise
+		else if (ms < best_ms && ms < rep->cand_ms[0][0] * (1 - TUNE_MARGIN))
+		{
+			best_ms = ms;
+			best = j;
+		}
+	}
+	for (int ci = 0; ci < rep->nclasses; ci++)
+	{
+		const int k = rep->class_index[ci];
+		rep->chosen_waves[ci] = cands[size_t(best)].plan[k].waves;
+		rep->chosen_persist[ci] = cands[size_t(best)].plan[k].persist ? 1 : 0;
+		c->tuned[{b.precision, variant, k}] = cands[size_t(best)].plan[k];
 	}
 }
 

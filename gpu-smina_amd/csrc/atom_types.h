@@ -112,6 +112,66 @@ SGPU_HD AtomTypeInfo atom_type_info(int t, int params)
 	return params == ATOM_PARAMS_VINARDO ? atom_type_info_vinardo(t) : atom_type_info(t);
 }
 
+// the AutoDock 4 solvation parameter and atomic volume of each type (the
+// ad_solvation and ad_volume columns of the default table,
+// src/lib/atom_constants.h:150-216), read by ad4_solvation
+SGPU_HD double ad_solvation(int t)
+{
+	switch (t)
+	{
+	case T_Hydrogen:
+	case T_PolarHydrogen: return 0.000510;
+	case T_CAliph_H:
+	case T_CAliph_P: return -0.001430;
+	case T_CArom_H:
+	case T_CArom_P: return -0.000520;
+	case T_N_P:
+	case T_N_D:
+	case T_N_DA:
+	case T_N_A: return -0.001620;
+	case T_O_P:
+	case T_O_D:
+	case T_O_DA:
+	case T_O_A: return -0.002510;
+	case T_S_P:
+	case T_S_A: return -0.002140;
+	default: return -0.001100; // P, halogens, metals
+	}
+}
+SGPU_HD double ad_volume(int t)
+{
+	switch (t)
+	{
+	case T_Hydrogen:
+	case T_PolarHydrogen: return 0.0;
+	case T_CAliph_H:
+	case T_CAliph_P:
+	case T_CArom_H:
+	case T_CArom_P:
+	case T_S_P:
+	case T_S_A: return 33.510300;
+	case T_N_P:
+	case T_N_D:
+	case T_N_DA:
+	case T_N_A: return 22.449300;
+	case T_O_P:
+	case T_O_D:
+	case T_O_DA:
+	case T_O_A: return 17.157300;
+	case T_P_P: return 38.792400;
+	case T_F_H: return 15.448000;
+	case T_Cl_H: return 35.823500;
+	case T_Br_H: return 42.566100;
+	case T_I_H: return 55.058500;
+	case T_Mg: return 1.560000;
+	case T_Mn: return 2.140000;
+	case T_Zn: return 1.700000;
+	case T_Ca: return 2.770000;
+	case T_Fe: return 1.840000;
+	default: return 22.449300; // generic metal
+	}
+}
+
 SGPU_HD bool is_hydrogen_type(int t) { return t == T_Hydrogen || t == T_PolarHydrogen; }
 
 // triangular_matrix_index (src/lib/triangular_matrix_index.h) of i <= j

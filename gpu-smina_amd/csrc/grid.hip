@@ -14,7 +14,11 @@
  *    share a cell and read the same atoms), FP64 sums of the fast table over
  *    the cell's atoms for every needed ligand type.
  *    With a user grid (--user_grid), each point adds its value as
- *    cache::populate does (src/lib/cache.cpp:204-205).
+ *    cache::populate does (src/lib/cache.cpp:204-205).  With
+ *    charge-dependent terms the point also sums the charge map
+ *    (src/lib/cache.cpp:172-187): the map gets type-only plus |q_rec|
+ *    components, the charge map the |q_lig| plus q_rec q_lig components
+ *    that the evaluation scales by the ligand atom's charge.
  * 3. pack: point maps -> corner-packed cells (8 corner values per cell).
  */
 #include <hip/hip_runtime.h>
@@ -127,7 +131,7 @@ __global__ __launch_bounds__(CELL_THREADS) void cell_fill_kernel(GridBuildArgs a
 	}
 }
 
-__global__ __launch_bounds__(256) void grid_points_kernel(GridBuildArgs a, double* points)
+__global__ __launch_bounds__(256) void grid_points_kernel(GridBuildArgs a, double* points, double* qpoints)
 {
 	const size_t npts = size_t(a.dim[0]) * a.dim[1] * a.dim[2];
 	const size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -143,11 +147,12 @@ __global__ __launch_bounds__(256) void grid_points_kernel(GridBuildArgs a, doubl
 	for (int d = 0; d < 3; d++)
 		ix[d] = int(__builtin_floor(probe[d] / 3.0)) - a.cell_off[d];
 	const int cell = ix[0] + a.cell_dim[0] * (ix[1] + a.cell_dim[1] * ix[2]);
-	double aff[MAX_GRID_TYPES];
+	double aff[MAX_GRID_TYPES], caff[MAX_GRID_TYPES];
 	int tt[MAX_GRID_TYPES];
 	for (int j = 0; j < MAX_GRID_TYPES; j++)
 	{
 		aff[j] = 0;
+		caff[j] = 0;
 		tt[j] = j < a.ntypes ? a.types[j] : 0;
 	}
 	for (int k = a.cell_start[cell]; k < a.cell_start[cell + 1]; k++)
@@ -173,6 +178,24 @@ __global__ __launch_bounds__(256) void grid_points_kernel(GridBuildArgs a, doubl
 				continue;
 			}
 			const size_t idx = size_t(a.factor * r2);
+			if (a.ncomp == 4)
+			{
+				// precalculate_linear::eval_fast_data of (receptor type,
+				// ligand type): the components swapped when t1 > t2
+				const double q = a.rec_q[i];
+				const double aq = q < 0 ? -q : q;
+				for (int j = 0; j < MAX_GRID_TYPES; j++)
+					if (j < a.ntypes)
+					{
+						const size_t k = size_t(tri_index_permissive(t1, tt[j])) * a.table_n + idx;
+						const double* qf = a.qfast + 3 * k;
+						const bool in_order = t1 <= tt[j];
+						const double c1 = in_order ? qf[0] : qf[1], c2 = in_order ? qf[1] : qf[0];
+						aff[j] += a.fast[k] + c1 * aq;
+						caff[j] += c2 + qf[2] * q;
+					}
+				continue;
+			}
 			for (int j = 0; j < MAX_GRID_TYPES; j++)
 				if (j < a.ntypes)
 					aff[j] += a.fast[size_t(tri_index_permissive(t1, tt[j])) * a.table_n + idx];
@@ -183,7 +206,11 @@ __global__ __launch_bounds__(256) void grid_points_kernel(GridBuildArgs a, doubl
 	const double uge = a.has_ug ? grid_value(a.ug, v3(double(x), double(y), double(z)), 1000.0, a.ug_slope) : 0.0;
 	for (int j = 0; j < MAX_GRID_TYPES; j++)
 		if (j < a.ntypes)
+		{
 			points[size_t(j) * npts + p] = a.has_ug ? aff[j] + uge : aff[j];
+			if (a.ncomp == 4)
+				qpoints[size_t(j) * npts + p] = caff[j];
+		}
 }
 
 template<typename T>
@@ -229,12 +256,12 @@ hipError_t launch_cell_fill(const GridBuildArgs& a, hipStream_t st)
 	return hipGetLastError();
 }
 
-hipError_t launch_grid_points(const GridBuildArgs& a, double* points, hipStream_t st)
+hipError_t launch_grid_points(const GridBuildArgs& a, double* points, double* qpoints, hipStream_t st)
 {
-	if (a.ntypes > MAX_GRID_TYPES)
+	if (a.ntypes > MAX_GRID_TYPES || (a.ncomp == 4 && (!qpoints || !a.qfast || !a.rec_q || a.kind != 0)))
 		return hipErrorInvalidValue;
 	const size_t npts = size_t(a.dim[0]) * a.dim[1] * a.dim[2];
-	hipLaunchKernelGGL(grid_points_kernel, dim3(unsigned((npts + 255) / 256)), dim3(256), 0, st, a, points);
+	hipLaunchKernelGGL(grid_points_kernel, dim3(unsigned((npts + 255) / 256)), dim3(256), 0, st, a, points, qpoints);
 	return hipGetLastError();
 }
 

@@ -23,8 +23,10 @@ struct GridSample
 	int region[3];
 };
 
+// the corners from `cells` (a map set laid out as g.cells: the receptor
+// maps or, with charge-dependent terms, the charge maps)
 template<typename T>
-__device__ inline GridSample<T> grid_fetch(const GridPtrs<T>& g, int slot, V3<T> loc)
+__device__ inline GridSample<T> grid_fetch(const GridPtrs<T>& g, const T* cells, int slot, V3<T> loc)
 {
 	GridSample<T> r;
 	r.s[0] = (loc.x - T(g.init[0])) * T(g.factor[0]);
@@ -59,10 +61,16 @@ __device__ inline GridSample<T> grid_fetch(const GridPtrs<T>& g, int slot, V3<T>
 		a[i] = a[i] < 0 ? 0 : (a[i] > g.dim[i] - 2 ? g.dim[i] - 2 : a[i]);
 	}
 	const size_t cell = size_t(a[0]) + size_t(g.dim[0] - 1) * (size_t(a[1]) + size_t(g.dim[1] - 1) * a[2]);
-	const T* c = g.cells + (size_t(slot) * g.ncells + cell) * 8;
+	const T* c = cells + (size_t(slot) * g.ncells + cell) * 8;
 	for (int k = 0; k < 8; k++)
 		r.c[k] = c[k];
 	return r;
+}
+
+template<typename T>
+__device__ inline GridSample<T> grid_fetch(const GridPtrs<T>& g, int slot, V3<T> loc)
+{
+	return grid_fetch(g, g.cells, slot, loc);
 }
 
 template<typename T>
@@ -106,6 +114,18 @@ template<typename T>
 __device__ inline T grid_term(const GridPtrs<T>& g, int slot, V3<T> loc, T v, V3<T>& deriv)
 {
 	return grid_finish(g, grid_fetch(g, slot, loc), v, deriv);
+}
+
+// grid::evaluate's charge-dependent part (src/lib/grid.cpp:28-48): for an
+// atom of charge q != 0, q times the charge map's term (value, derivative)
+// added to the map term's
+template<typename T>
+__device__ inline void grid_charge_term(const GridPtrs<T>& g, int slot, V3<T> loc, T v, T q, T& e, V3<T>& deriv)
+{
+	V3<T> cd;
+	const T ce = grid_finish(g, grid_fetch(g, g.qcells, slot, loc), v, cd);
+	e += q * ce;
+	deriv = v3(deriv.x + q * cd.x, deriv.y + q * cd.y, deriv.z + q * cd.z);
 }
 
 // evaluate_aux without a derivative (deriv NULL): the value curled by v

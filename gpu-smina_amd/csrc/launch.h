@@ -107,8 +107,11 @@ struct GridBuildArgs
 {
 	const double* rec_xyz;
 	const int* rec_type;
+	const double* rec_q;   // receptor partial charges (ncomp 4)
 	int rec_n;
 	const double* fast;
+	const double* qfast;   // [378][n][3] charge components, ncomp 4
+	int ncomp;             // 4: the point pass also writes the charge maps
 	int table_n;
 	double factor;
 	double cutoff_sqr;
@@ -139,7 +142,8 @@ struct GridBuildArgs
 };
 hipError_t launch_cell_count(const GridBuildArgs& a, hipStream_t st);
 hipError_t launch_cell_fill(const GridBuildArgs& a, hipStream_t st);
-hipError_t launch_grid_points(const GridBuildArgs& a, double* points, hipStream_t st);
+// points: [ntypes][npts]; qpoints: the charge maps, same layout (ncomp 4)
+hipError_t launch_grid_points(const GridBuildArgs& a, double* points, double* qpoints, hipStream_t st);
 template<typename T>
 hipError_t launch_grid_pack(const double* points, int ntypes, const int* dim, T* cells, hipStream_t st);
 

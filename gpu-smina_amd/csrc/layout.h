@@ -14,7 +14,8 @@
  *            atom range, conf / change index, children), tree levels, heavy
  *            atoms, trees, conf / change maps;
  * real blob: initial coordinates (movable then fixed), local coordinates,
- *            frame origins and axes (segment: relative to the parent).
+ *            frame origins and axes (segment: relative to the parent),
+ *            partial charges (movable then fixed).
  *
  * A neighbour code is one 32-bit word: the partner atom (bits 0-7), the
  * type-pair table index (8-16), "this atom is the pair's b" (17), "the pair
@@ -94,7 +95,7 @@ struct ModelHdr
 			o_ccount, o_clist, o_lstart, o_lframes, o_heavy, o_troot, o_tconf, o_tchange, o_tlig, o_tntors,
 			o_tab, o_tae, o_tpb, o_tpe, o_maps;
 	// offsets inside the real blob
-	int o_xyz0, o_local, o_forig, o_faxis;
+	int o_xyz0, o_local, o_forig, o_faxis, o_charge;
 };
 
 template<typename T>
@@ -109,12 +110,18 @@ struct BatchPtrs
 };
 
 // scoring tables on the device: smooth as (e, dor) pairs per point; fast
-// values kept in FP64 for the grid build and the final rescoring
+// values kept in FP64 for the grid build and the final rescoring.  With
+// charge-dependent terms (ncomp 4) smooth and fast hold the first (type
+// only) component and qsmooth / qfast the other three (|q_a|, |q_b| and
+// q_a q_b dependent) of the pair (min type, max type).
 template<typename T>
 struct TablePtrs
 {
 	const T* smooth; // [378][n][2]
 	const double* fast; // [378][n]
+	int ncomp;           // 1 or 4
+	const T* qsmooth;    // [378][n][3][2] (ncomp 4)
+	const double* qfast; // [378][n][3] (ncomp 4)
 	int n;
 	double factor;
 	double cutoff_sqr;
@@ -131,6 +138,7 @@ template<typename T>
 struct GridPtrs
 {
 	const T* cells;      // [type slot][cells][8]
+	const T* qcells;     // charge maps, same slots (ncomp 4), else null
 	const int* slot_of;  // [27] -> map slot or -1
 	int dim[3];          // points per axis
 	int ncells;          // (dim0-1)(dim1-1)(dim2-1)
@@ -149,6 +157,7 @@ struct ReceptorPtrs
 {
 	const T* xyz;        // [natoms*3] all atoms (H included)
 	const int* type;
+	const T* charge;     // [natoms] partial charges
 	int natoms;
 	const int* cell_start; // [ncells+1]
 	const int* cell_atoms;

@@ -227,6 +227,7 @@ struct MV
 	const int* clist;
 	const T* forig;
 	const T* faxis;
+	const T* charge; // partial charges, movable then fixed atoms
 	const int* lstart;
 	const int* lframes;
 	const int* pa;
@@ -293,6 +294,7 @@ __device__ inline MV<T> stage_model(const BatchPtrs<T>& B, int m, unsigned char*
 	v.clist = li + h.o_clist;
 	v.forig = lt + h.o_forig;
 	v.faxis = lt + h.o_faxis;
+	v.charge = lt + h.o_charge;
 	v.lstart = li + h.o_lstart;
 	v.lframes = li + h.o_lframes;
 	v.pa = B.pair_a + h.pair_off;
@@ -490,10 +492,43 @@ __device__ inline void table_deriv(const TablePtrs<T>& tab, int tri, T r2, T& e,
 	dor = d1 + rem * (d2 - d1);
 }
 
-// non_cache::eval_deriv of one atom (src/lib/non_cache.cpp:109-191): box
-// clamp with penalty, then the receptor atoms of the clamped point's cell
+// the four-component knots of a pair with charges qa, qb in table order
+// (the atom of the smaller type first): result_components::eval
+// (src/lib/result_components.h:73-79) of each knot's value and derivative,
+// c0 + |qa| c1 + |qb| c2 + qa qb c3 summed in that order.  p: the pair's
+// (e, dor) row at knot i1 in smooth, q: its row in qsmooth.
 template<typename T>
-__device__ inline T direct_term(const EvalCtx<T>& ec, int t1, V3<T> ac, V3<T>& out)
+__device__ inline void charged_knots(const T* p, const T* q, T qa, T qb, T& e1, T& d1, T& e2, T& d2)
+{
+	const T aa = Real<T>::abs(qa), ab = Real<T>::abs(qb), qq = qa * qb;
+	e1 = p[0] + aa * q[0] + ab * q[2] + qq * q[4];
+	d1 = p[1] + aa * q[1] + ab * q[3] + qq * q[5];
+	e2 = p[2] + aa * q[6] + ab * q[8] + qq * q[10];
+	d2 = p[3] + aa * q[7] + ab * q[9] + qq * q[11];
+}
+
+// precalculate_linear::eval_deriv (src/lib/precalculate.h:245-293) of
+// atoms of types t1, t2 and charges q1, q2 with four-component tables: the
+// element of (min type, max type) with the atoms in that order
+template<typename T>
+__device__ inline void table_deriv_charged(const TablePtrs<T>& tab, int t1, int t2, T q1, T q2, T r2, T& e, T& dor)
+{
+	const T r2f = T(tab.factor) * r2;
+	const int i1 = int(r2f);
+	const T rem = r2f - T(i1);
+	const size_t k = size_t(tri_index_permissive(t1, t2)) * tab.n + i1;
+	const bool in_order = t1 <= t2;
+	T e1, d1, e2, d2;
+	charged_knots(tab.smooth + 2 * k, tab.qsmooth + 6 * k, in_order ? q1 : q2, in_order ? q2 : q1, e1, d1, e2, d2);
+	e = e1 + rem * (e2 - e1);
+	dor = d1 + rem * (d2 - d1);
+}
+
+// non_cache::eval_deriv of one atom (src/lib/non_cache.cpp:109-191): box
+// clamp with penalty, then the receptor atoms of the clamped point's cell.
+// q1: the atom's partial charge (read with four-component tables only).
+template<typename T>
+__device__ inline T direct_term(const EvalCtx<T>& ec, int t1, T q1, V3<T> ac, V3<T>& out)
 {
 	const ReceptorPtrs<T>& r = ec.rec;
 	T a[3] = {ac.x, ac.y, ac.z};
@@ -539,7 +574,10 @@ __device__ inline T direct_term(const EvalCtx<T>& ec, int t1, V3<T> ac, V3<T>& o
 			if (r2 < Real<T>::eps())
 				atomicOr(ec.error, 1); // bit 0: exact overlap (other bits belong to the caller)
 			T pe, pd;
-			table_deriv(ec.tab, tri_index_permissive(t1, r.type[j]), r2, pe, pd);
+			if (ec.tab.ncomp == 4)
+				table_deriv_charged(ec.tab, t1, r.type[j], q1, r.charge[j], r2, pe, pd);
+			else
+				table_deriv(ec.tab, tri_index_permissive(t1, r.type[j]), r2, pe, pd);
 			this_e += pe;
 			deriv.x += pd * rba.x;
 			deriv.y += pd * rba.y;
@@ -586,9 +624,13 @@ __device__ __forceinline__ T eval_deriv(const MV<T>& m, const EvalCtx<T>& ec, Wa
 			if (!is_hydrogen_type(t))
 			{
 				if constexpr (BK == BK_MAPS)
+				{
 					e = grid_term(ec.grid, m.slot[i], xi, ec.v[1], d);
+					if (SPL && ec.grid.qcells && m.charge[i] != T(0))
+						grid_charge_term(ec.grid, m.slot[i], xi, ec.v[1], m.charge[i], e, d);
+				}
 				else
-					e = direct_term(ec, t, xi, d);
+					e = direct_term(ec, t, m.charge[i], xi, d);
 			}
 			if constexpr (Real<T>::ordered)
 				W.ebuf[i] = e;
@@ -635,7 +677,17 @@ __device__ __forceinline__ T eval_deriv(const MV<T>& m, const EvalCtx<T>& ec, Wa
 					const T r2f = T(ec.tab.factor) * r2[u];
 					const int i1 = r2[u] < cutoff_sqr ? int(r2f) : 0;
 					rem[u] = r2f - T(i1);
-					const T* tp = ec.tab.smooth + (size_t(code_tri(code)) * ec.tab.n + i1) * 2;
+					const size_t kn = size_t(code_tri(code)) * ec.tab.n + i1;
+					const T* tp = ec.tab.smooth + kn * 2;
+					if (SPL && ec.tab.ncomp == 4)
+					{
+						// four components: the pair's charges in table order
+						const int ia = is_b ? j : i, ib = is_b ? i : j;
+						const bool in_order = m.type[ia] <= m.type[ib];
+						const T qa = m.charge[in_order ? ia : ib], qb = m.charge[in_order ? ib : ia];
+						charged_knots(tp, ec.tab.qsmooth + kn * 6, qa, qb, e1[u], d1[u], e2[u], d2[u]);
+						continue;
+					}
 					e1[u] = tp[0];
 					d1[u] = tp[1];
 					e2[u] = tp[2];
@@ -738,6 +790,15 @@ __device__ __forceinline__ T eval_deriv(const MV<T>& m, const EvalCtx<T>& ec, Wa
 				const int i1 = min(int(r2f), last_knot);
 				rem[u] = r2f - T(i1);
 				const T* tp = ec.tab.smooth + int(__umul24(unsigned(prec_tri(rc[u].y)), unsigned(row_stride)) + 2 * i1);
+				if (SPL && ec.tab.ncomp == 4)
+				{
+					// four components: the pair's charges in table order
+					const int ia = prec_xa(rc[u].x) / 3, ib = prec_xb(rc[u].x) / 3;
+					const bool in_order = m.type[ia] <= m.type[ib];
+					const T qa = m.charge[in_order ? ia : ib], qb = m.charge[in_order ? ib : ia];
+					charged_knots(tp, ec.tab.qsmooth + 3 * (tp - ec.tab.smooth), qa, qb, e1[u], d1[u], e2[u], d2[u]);
+					continue;
+				}
 				e1[u] = tp[0];
 				d1[u] = tp[1];
 				e2[u] = tp[2];
@@ -794,9 +855,14 @@ __device__ __forceinline__ T eval_deriv(const MV<T>& m, const EvalCtx<T>& ec, Wa
 			if (heavy)
 			{
 				if constexpr (BK == BK_MAPS)
-					e_part += grid_finish(ec.grid, gs, ec.v[1], d);
+				{
+					T ea = grid_finish(ec.grid, gs, ec.v[1], d);
+					if (SPL && ec.grid.qcells && m.charge[i] != T(0))
+						grid_charge_term(ec.grid, m.slot[i], xi, ec.v[1], m.charge[i], ea, d);
+					e_part += ea;
+				}
 				else
-					e_part += direct_term(ec, t, xi, d);
+					e_part += direct_term(ec, t, m.charge[i], xi, d);
 			}
 			st3(W.forces + 3 * i, d + fp);
 		}

@@ -135,6 +135,37 @@ double term(const sgpu_term& t, int t1, int t2, double r, int ap)
 	throw std::runtime_error("unknown scoring term");
 }
 
+} // namespace
+
+bool charge_dependent(const sgpu_term& t)
+{
+	return t.kind == SGPU_TERM_ELECTROSTATIC || t.kind == SGPU_TERM_AD4_SOLVATION;
+}
+
+namespace
+{
+
+// the four result components of a charge-dependent term (type only, |q_a|,
+// |q_b| and q_a q_b dependent): electrostatic and ad4_solvation's
+// eval_components (src/lib/everything.h:64-156)
+void components(const sgpu_term& t, int t1, int t2, double r, double* out)
+{
+	out[0] = out[1] = out[2] = out[3] = 0;
+	if (t.kind == SGPU_TERM_ELECTROSTATIC)
+	{
+		const double p = ipow(r, int(t.p[0]));
+		out[3] = p < kEps ? t.p[1] : std::min(t.p[1], 1 / p);
+		return;
+	}
+	if (t.kind != SGPU_TERM_AD4_SOLVATION)
+		throw std::runtime_error("not a charge-dependent term");
+	const double df = std::exp(-sq(r / (2 * t.p[0])));
+	out[0] += ad_solvation(t1) * ad_volume(t2) * df;
+	out[1] += t.p[1] * ad_volume(t2) * df;
+	out[0] += ad_solvation(t2) * ad_volume(t1) * df;
+	out[2] += t.p[1] * ad_volume(t1) * df;
+}
+
 // smooth_div (src/lib/everything.cpp)
 double smooth_div(double x, double y)
 {
@@ -200,9 +231,15 @@ double normal(Gen& g)
 }
 } // namespace
 
-double term_value(const sgpu_term& t, int t1, int t2, double r, int atom_params)
+double term_value(const sgpu_term& t, int t1, int t2, double r, int atom_params, double q1, double q2)
 {
-	return term(t, t1, t2, r, atom_params);
+	if (!charge_dependent(t))
+		return term(t, t1, t2, r, atom_params);
+	// charge_dependent::eval (src/lib/terms.h:90-94): the components of
+	// (t1, t2) as given, then result_components::eval
+	double c[4];
+	components(t, t1, t2, r, c);
+	return c[0] + std::abs(q1) * c[1] + std::abs(q2) * c[2] + q1 * q2 * c[3];
 }
 
 // each term's cutoff: the last parameter its name carries (c)
@@ -250,6 +287,10 @@ std::string term_name(const sgpu_term& t)
 				",_^=" + f(t.p[3]) + ",_c=" + f(t.p[4]) + ")";
 	case SGPU_TERM_NON_DIR_H_BOND_LJ:
 		return "non_dir_h_bond_lj(o=" + f(t.p[0]) + ",_^=" + f(t.p[1]) + ",_c=" + f(t.p[2]) + ")";
+	case SGPU_TERM_ELECTROSTATIC:
+		return "electrostatic(i=" + f(double(int(t.p[0]))) + ",_^=" + f(t.p[1]) + ",_c=" + f(t.p[2]) + ")";
+	case SGPU_TERM_AD4_SOLVATION:
+		return "ad4_solvation(d-sigma=" + f(t.p[0]) + ",_s/q=" + f(t.p[1]) + ",_c=" + f(t.p[2]) + ")";
 	}
 	throw std::runtime_error("unknown scoring term");
 }
@@ -265,34 +306,68 @@ Tables build_tables(const sgpu_scoring& sf, double factor)
 	std::vector<double> rs(n + 2);
 	for (int i = 0; i < n + 2; i++)
 		rs[i] = std::sqrt(i / factor);
+	bool charged = false;
+	for (int k = 0; k < sf.nterms; k++)
+		charged = charged || charge_dependent(sf.terms[k]);
+	t.ncomp = charged ? 4 : 1;
 	t.smooth.assign(size_t(SGPU_NUM_TYPE_PAIRS) * n * 2, 0);
 	t.fast.assign(size_t(SGPU_NUM_TYPE_PAIRS) * n, 0);
-	std::vector<double> e(n);
+	if (charged)
+	{
+		t.qsmooth.assign(size_t(SGPU_NUM_TYPE_PAIRS) * n * 6, 0);
+		t.qfast.assign(size_t(SGPU_NUM_TYPE_PAIRS) * n * 3, 0);
+	}
+	// per knot the four components: the charge-independent terms summed
+	// into the first, then each charge-dependent term's components times
+	// its weight (weighted_terms::eval_fast, src/lib/weighted_terms.cpp:52-62)
+	std::vector<double> e(size_t(n) * 4);
 	for (int t1 = 0; t1 < NUM_ATOM_TYPES; t1++)
 		for (int t2 = t1; t2 < NUM_ATOM_TYPES; t2++)
 		{
 			for (int i = 0; i < n; i++)
 			{
-				double acc = 0;
+				double* acc = &e[4 * size_t(i)];
+				acc[0] = acc[1] = acc[2] = acc[3] = 0;
 				for (int k = 0; k < sf.nterms; k++)
-					acc += sf.terms[k].weight * term(sf.terms[k], t1, t2, rs[i], sf.atom_params);
-				e[i] = acc;
+					if (!charge_dependent(sf.terms[k]))
+						acc[0] += sf.terms[k].weight * term(sf.terms[k], t1, t2, rs[i], sf.atom_params);
+				for (int k = 0; k < sf.nterms; k++)
+					if (charge_dependent(sf.terms[k]))
+					{
+						double c[4];
+						components(sf.terms[k], t1, t2, rs[i], c);
+						for (int j = 0; j < 4; j++)
+							acc[j] += c[j] * sf.terms[k].weight;
+					}
 			}
 			const size_t base = size_t(tri_index(t1, t2)) * n;
-			for (int i = 0; i < n; i++)
+			for (int c = 0; c < t.ncomp; c++)
 			{
-				double dor = 0;
-				if (!(i == 0 || i == n - 1))
+				auto val = [&](int i) { return e[4 * size_t(i) + c]; };
+				for (int i = 0; i < n; i++)
 				{
-					const double delta = rs[i + 1] - rs[i - 1];
-					const double r = rs[i];
-					dor = (e[i + 1] - e[i - 1]) / (delta * r);
+					double dor = 0;
+					if (!(i == 0 || i == n - 1))
+					{
+						const double delta = rs[i + 1] - rs[i - 1];
+						const double r = rs[i];
+						dor = (val(i + 1) - val(i - 1)) / (delta * r);
+					}
+					const double f1 = val(i);
+					const double f2 = (i + 1 >= n) ? 0 : val(i + 1);
+					if (c == 0)
+					{
+						t.smooth[2 * (base + i)] = val(i);
+						t.smooth[2 * (base + i) + 1] = dor;
+						t.fast[base + i] = (f2 + f1) / 2;
+					}
+					else
+					{
+						t.qsmooth[2 * ((base + i) * 3 + c - 1)] = val(i);
+						t.qsmooth[2 * ((base + i) * 3 + c - 1) + 1] = dor;
+						t.qfast[(base + i) * 3 + c - 1] = (f2 + f1) / 2;
+					}
 				}
-				t.smooth[2 * (base + i)] = e[i];
-				t.smooth[2 * (base + i) + 1] = dor;
-				const double f1 = e[i];
-				const double f2 = (i + 1 >= n) ? 0 : e[i + 1];
-				t.fast[base + i] = (f2 + f1) / 2;
 			}
 		}
 	return t;
@@ -402,6 +477,7 @@ void FlatModel::publish()
 	view.pair_a = pair_a.data();
 	view.pair_b = pair_b.data();
 	view.pair_tri = pair_tri.data();
+	view.atom_charge = atom_charge.empty() ? nullptr : atom_charge.data();
 }
 
 FlatModel flatten(const sgpu_model_desc& d)
@@ -415,6 +491,8 @@ FlatModel flatten(const sgpu_model_desc& d)
 		if (t < 0 || t >= NUM_ATOM_TYPES)
 			throw std::runtime_error("atom type out of range");
 	m.atom_init.assign(d.xyz, d.xyz + 3 * size_t(NA));
+	if (d.charge)
+		m.atom_charge.assign(d.charge, d.charge + NA);
 	auto X = [&](int a, int k) { return d.xyz[3 * a + k]; };
 	std::vector<int> tree_of(d.nframes, -1);
 	for (int t = 0; t < d.ntrees; t++)
@@ -762,6 +840,10 @@ GpuDocking::GpuDocking(const GpuSettings& s, const sgpu_scoring& sf, const sgpu_
 	sf_.ci = ci_.data();
 	if (s.approximation != GpuSettings::LINEAR && s.approximation != GpuSettings::SPLINE)
 		throw std::invalid_argument("approximation must be LINEAR or SPLINE");
+	if (s.approximation == GpuSettings::SPLINE)
+		for (const sgpu_term& t : terms_)
+			if (charge_dependent(t))
+				throw std::invalid_argument("charge-dependent terms need the linear tables (approximation LINEAR)");
 	sgpu_status st = sgpu_create(s.device, &ctx_);
 	if (st != SGPU_OK)
 		throw std::runtime_error("sgpu_create failed: no HIP device");
@@ -774,7 +856,8 @@ GpuDocking::GpuDocking(const GpuSettings& s, const sgpu_scoring& sf, const sgpu_
 	else
 	{
 		Tables t = build_tables(sf_, s.approx_factor);
-		sgpu_tables tt{t.n, t.factor, t.cutoff_sqr, t.smooth.data(), t.fast.data()};
+		sgpu_tables tt{t.n, t.factor, t.cutoff_sqr, t.smooth.data(), t.fast.data(), t.ncomp,
+				t.ncomp == 4 ? t.qsmooth.data() : nullptr, t.ncomp == 4 ? t.qfast.data() : nullptr};
 		check(ctx_, sgpu_upload_tables(ctx_, &tt));
 	}
 	check(ctx_, sgpu_upload_receptor(ctx_, &rec, &receptor_));
@@ -801,7 +884,8 @@ GpuDocking::GpuDocking(const GpuSettings& s, const sgpu_scoring& sf, const sgpu_
 	if (s.approximation != GpuSettings::LINEAR)
 		throw std::invalid_argument("a shared context carries the linear tables");
 	Tables t = build_tables(sf_, s.approx_factor);
-	sgpu_tables tt{t.n, t.factor, t.cutoff_sqr, t.smooth.data(), t.fast.data()};
+	sgpu_tables tt{t.n, t.factor, t.cutoff_sqr, t.smooth.data(), t.fast.data(), t.ncomp,
+			t.ncomp == 4 ? t.qsmooth.data() : nullptr, t.ncomp == 4 ? t.qfast.data() : nullptr};
 	check(ctx_, sgpu_upload_tables(ctx_, &tt));
 	check(ctx_, sgpu_upload_receptor(ctx_, &rec, &receptor_));
 	check(ctx_, sgpu_mc_capacity(ctx_, s.precision, &capacity_));
@@ -1490,8 +1574,22 @@ int sgpu_builtin_scoring(const char* name, double hydrophobic_scale, sgpu_term* 
 		ci[c++] = {SGPU_CI_NUM_TORS_SQR, .285035};
 		ci[c++] = {SGPU_CI_CONSTANT, -2.585651};
 	}
-	else if (n == "dkoes_scoring" || n == "ad4_scoring")
-		return -2;
+	else if (n == "dkoes_scoring")
+	{ // with charge-dependent desolvation: four-component tables
+		t[k++] = {SGPU_TERM_VDW, {4, 8, 0, 100, 8}, 0.009900};
+		t[k++] = {SGPU_TERM_NON_DIR_H_BOND, {-0.7, 0, 8, 0, 0}, -0.153055};
+		t[k++] = {SGPU_TERM_AD4_SOLVATION, {3.6, 0.01097, 8, 0, 0}, 0.048934};
+		ci[c++] = {SGPU_CI_NUM_TORS_SQR, 0.317267};
+		ci[c++] = {SGPU_CI_CONSTANT, -2.469020};
+	}
+	else if (n == "ad4_scoring")
+	{
+		t[k++] = {SGPU_TERM_VDW, {6, 12, 0, 100, 8}, 0.1560};
+		t[k++] = {SGPU_TERM_NON_DIR_H_BOND_LJ, {-0.7, 100, 8, 0, 0}, -0.0974};
+		t[k++] = {SGPU_TERM_AD4_SOLVATION, {3.5, 0.01097, 8, 0, 0}, 0.1159};
+		t[k++] = {SGPU_TERM_ELECTROSTATIC, {1, 100, 8, 0, 0}, 0.1465};
+		ci[c++] = {SGPU_CI_NUM_TORS_ADD, 0.2744};
+	}
 	else
 		return -1;
 	out->nterms = k;
@@ -1521,6 +1619,23 @@ int sgpu_build_splines(const sgpu_scoring* sf, double factor, double* coef, doub
 		if (cut)
 			std::copy(s.cut.begin(), s.cut.end(), cut);
 		return s.nseg;
+	}
+	catch (...)
+	{
+		return -1;
+	}
+}
+
+int sgpu_build_qtables(const sgpu_scoring* sf, double factor, double* qsmooth, double* qfast)
+{
+	try
+	{
+		Tables t = build_tables(*sf, factor);
+		if (qsmooth)
+			std::copy(t.qsmooth.begin(), t.qsmooth.end(), qsmooth);
+		if (qfast)
+			std::copy(t.qfast.begin(), t.qfast.end(), qfast);
+		return t.ncomp;
 	}
 	catch (...)
 	{
@@ -1618,10 +1733,9 @@ sgpu_docking* sgpu_docking_create(const sgpu_dock_settings* s, const sgpu_mol* r
 		sgpu_scoring sf;
 		const int rc = sgpu_builtin_scoring(s->scoring, s->hydrophobic_scale, d->terms.data(), d->ci.data(), &sf);
 		if (rc != 0)
-			throw std::invalid_argument(rc == -2 ? "scoring function needs atom charges, which this path does not carry"
-			                                     : "unknown scoring function");
+			throw std::invalid_argument("unknown scoring function");
 		const sgpu_model_desc* rd = sgpu_mol_desc(receptor);
-		sgpu_receptor rec{rd->natoms, rd->xyz, rd->type};
+		sgpu_receptor rec{rd->natoms, rd->xyz, rd->type, rd->charge};
 		d->d.reset(new GpuDocking(gs, sf, rec, *gd));
 	}
 	catch (std::exception& e)

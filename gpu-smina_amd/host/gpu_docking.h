@@ -43,6 +43,7 @@ struct FlatModel
 	std::vector<int> atom_type, atom_frame, frame_parent, frame_kind, frame_begin, frame_end, frame_tree,
 			frame_conf, frame_change, pair_a, pair_b, pair_tri;
 	std::vector<double> atom_local, atom_init, frame_origin, frame_axis;
+	std::vector<double> atom_charge; // empty: every charge 0
 	std::vector<int> tree_ligand, tree_ntors, tree_conf;
 	int nheavy = 0;
 	double num_tors = 0;
@@ -75,6 +76,10 @@ struct Tables
 	double factor = 32;
 	double cutoff_sqr = 64;
 	std::vector<double> smooth, fast;
+	// with charge-dependent terms: 4 components, the three charge-dependent
+	// ones ([378][n][3][2] and [378][n][3])
+	int ncomp = 1;
+	std::vector<double> qsmooth, qfast;
 };
 Tables build_tables(const sgpu_scoring& sf, double factor);
 // precalculate_splines(sf, factor) (src/lib/precalculate.h:313-455)
@@ -92,7 +97,10 @@ void add_to_output_container(std::vector<Pose>& out, const Pose& t, double min_r
 // one scoring term between atom types t1 and t2 at distance r, unweighted
 // (the term's own eval, src/lib/everything.h), its cutoff and its name as
 // smina prints it
-double term_value(const sgpu_term& t, int t1, int t2, double r, int atom_params);
+// a term's unweighted value for atoms of types t1, t2 and charges q1, q2
+// at distance r (charge-dependent terms use the charges)
+double term_value(const sgpu_term& t, int t1, int t2, double r, int atom_params, double q1 = 0, double q2 = 0);
+bool charge_dependent(const sgpu_term& t);
 double term_cutoff(const sgpu_term& t);
 std::string term_name(const sgpu_term& t);
 // model::movable_atoms_box (src/lib/model.cpp:677-707) of n atoms at xyz

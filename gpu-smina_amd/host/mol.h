@@ -13,6 +13,7 @@ struct sgpu_mol
 	std::vector<int> bonds;
 	std::vector<int> frame_parent, frame_begin, frame_end, frame_origin_atom, frame_axis_atom;
 	std::vector<int> tree_frame;
+	std::vector<double> charge; // partial charges (empty: all 0)
 	int nmovable = 0;
 	int nligands = 0;
 	sgpu_model_desc desc{};
@@ -40,6 +41,7 @@ struct sgpu_mol
 		desc.nligands = nligands;
 		desc.ntrees = int(tree_frame.size());
 		desc.tree_frame = tree_frame.data();
+		desc.charge = charge.size() == type.size() ? charge.data() : nullptr;
 	}
 };
 

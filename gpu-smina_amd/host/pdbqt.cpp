@@ -136,6 +136,7 @@ struct Atom
 	int number = 0;
 	int type = 0;
 	double xyz[3] = {0, 0, 0};
+	double charge = 0;
 	int line = 0; // index into the file's lines
 };
 
@@ -167,6 +168,10 @@ Atom parse_atom(const std::string& s, int idx)
 	a.number = int(number);
 	for (int k = 0; k < 3; k++)
 		a.xyz[k] = column_number(s, 31 + 8 * k, 38 + 8 * k, line, "coordinate");
+	// the partial charge, columns 69-76 unless blank
+	const std::string q = s.size() >= 69 ? s.substr(68, std::min<size_t>(8, s.size() - 68)) : std::string();
+	if (!trim(q).empty())
+		a.charge = column_number(s, 69, 76, line, "charge");
 	if (s.size() < 78)
 		throw ParseError(line, "ATOM syntax incorrect: no atom type");
 	const std::string name = trim(s.substr(77));
@@ -508,6 +513,7 @@ sgpu_mol* finish(std::vector<Atom>& atoms, const std::vector<int>& bonds, std::v
 	{
 		m->type.push_back(a.type);
 		m->xyz.insert(m->xyz.end(), a.xyz, a.xyz + 3);
+		m->charge.push_back(a.charge);
 	}
 	m->bonds = bonds;
 	m->text = std::move(lines);
@@ -868,10 +874,13 @@ sgpu_mol* read_flex(const sgpu_mol& lig, const sgpu_mol& rigid, const char* text
 	m->text_atom.clear();
 	const int L = lig.natoms();
 	auto final_index = [&](int loc) { return L + loc; }; // movable then fixed, in local order
+	if (m->charge.size() != size_t(L))
+		m->charge.assign(size_t(L), 0.0);
 	for (int a = 0; a < M + I; a++)
 	{
 		m->type.push_back(atoms[size_t(a)].type);
 		m->xyz.insert(m->xyz.end(), atoms[size_t(a)].xyz, atoms[size_t(a)].xyz + 3);
+		m->charge.push_back(atoms[size_t(a)].charge);
 	}
 	for (size_t k = 0; k + 1 < all_bonds.size(); k += 2)
 		if (all_bonds[k] < M + I && all_bonds[k + 1] < M + I)
@@ -1079,9 +1088,18 @@ double brick_d2(const double* lo, const double* hi, const double* v)
 // (src/lib/terms.cpp:235-307): the receptor's heavy atoms and the model's
 // side-chain and fixed heavy atoms within the terms' largest cutoff of the
 // movable atoms' box; each term exact at the pair's distance (no tables)
-// where the distance is below its own cutoff
+// where the distance is below its own cutoff, charge-dependent terms with
+// the atoms' partial charges.  The columns follow the terms' internal order
+// (terms::eval_additive_aux, src/lib/terms.cpp:202-230): the
+// charge-independent terms, then the charge-dependent ones.
 std::string atom_terms(const sgpu_scoring& sf, const sgpu_mol& rec, const sgpu_mol& lig, const double* conf)
 {
+	std::vector<int> order;
+	for (int pass = 0; pass < 2; pass++)
+		for (int k = 0; k < sf.nterms; k++)
+			if (charge_dependent(sf.terms[k]) == (pass == 1))
+				order.push_back(k);
+	auto q_of = [](const sgpu_mol& m, int i) { return m.charge.empty() ? 0.0 : m.charge[size_t(i)]; };
 	FlatModel f = flatten(lig.desc);
 	const std::vector<double> x = pose_coords(f, conf);
 	const int nmov = lig.nmovable, nall = lig.natoms(), nlig = ligand_atoms(lig);
@@ -1099,21 +1117,24 @@ std::string atom_terms(const sgpu_scoring& sf, const sgpu_mol& rec, const sgpu_m
 	// model's non-ligand atoms
 	std::vector<const double*> rx;
 	std::vector<int> rt;
+	std::vector<double> rq;
 	for (int j = 0; j < rec.natoms(); j++)
 		if (usable(rec.type[size_t(j)]) && brick_d2(box.begin, box.end, &rec.xyz[3 * size_t(j)]) < maxc2)
 		{
 			rx.push_back(&rec.xyz[3 * size_t(j)]);
 			rt.push_back(rec.type[size_t(j)]);
+			rq.push_back(q_of(rec, j));
 		}
 	for (int j = nlig; j < nall; j++)
 		if (usable(lig.type[size_t(j)]) && brick_d2(box.begin, box.end, &xa[3 * size_t(j)]) < maxc2)
 		{
 			rx.push_back(&xa[3 * size_t(j)]);
 			rt.push_back(lig.type[size_t(j)]);
+			rq.push_back(q_of(lig, j));
 		}
 	std::ostringstream out;
 	out << "atomid el pos";
-	for (int k = 0; k < sf.nterms; k++)
+	for (int k : order)
 		out << " " << term_name(sf.terms[k]);
 	out << "\n";
 	for (int i = 0; i < nlig; i++)
@@ -1130,10 +1151,11 @@ std::string atom_terms(const sgpu_scoring& sf, const sgpu_mol& rec, const sgpu_m
 				const double d = std::sqrt(d2);
 				for (int k = 0; k < sf.nterms; k++)
 					if (d < term_cutoff(sf.terms[k]))
-						v[size_t(k)] += term_value(sf.terms[k], lig.type[size_t(i)], rt[j], d, sf.atom_params);
+						v[size_t(k)] += term_value(sf.terms[k], lig.type[size_t(i)], rt[j], d, sf.atom_params,
+								q_of(lig, i), rq[j]);
 			}
 		out << atom_str(lig, i) << " <" << xi[0] << "," << xi[1] << "," << xi[2] << ">";
-		for (int k = 0; k < sf.nterms; k++)
+		for (int k : order)
 			out << " " << v[size_t(k)] * sf.terms[k].weight;
 		out << "\n";
 	}
@@ -1358,6 +1380,18 @@ int sgpu_write_pdbqt_poses(const sgpu_mol* lig, int nposes, const double* conf, 
 	{
 		return -1;
 	}
+}
+
+int sgpu_mol_set_charges(sgpu_mol* m, const double* charges)
+{
+	if (!m)
+		return -1;
+	if (charges)
+		m->charge.assign(charges, charges + m->natoms());
+	else
+		m->charge.clear();
+	m->publish();
+	return 0;
 }
 
 int sgpu_atom_terms(const sgpu_scoring* sf, const sgpu_mol* receptor, const sgpu_mol* ligand, const double* conf,

@@ -80,12 +80,15 @@ int DockingServer::add_receptor(const sgpu_receptor& rec, const sgpu_grid_dims& 
 	Rec r;
 	r.xyz.assign(rec.xyz, rec.xyz + 3 * size_t(rec.natoms));
 	r.type.assign(rec.type, rec.type + rec.natoms);
-	r.rec = sgpu_receptor{rec.natoms, r.xyz.data(), r.type.data()};
+	if (rec.charge)
+		r.charge.assign(rec.charge, rec.charge + rec.natoms);
+	r.rec = sgpu_receptor{rec.natoms, r.xyz.data(), r.type.data(), r.charge.empty() ? nullptr : r.charge.data()};
 	r.gd = gd;
 	r.types = types;
 	recs_.push_back(std::move(r));
 	const Rec& rr = recs_.back();
-	const sgpu_receptor rec_view{rr.rec.natoms, rr.xyz.data(), rr.type.data()};
+	const sgpu_receptor rec_view{rr.rec.natoms, rr.xyz.data(), rr.type.data(),
+			rr.charge.empty() ? nullptr : rr.charge.data()};
 	// the maps, once for the device, in the store context
 	std::unique_ptr<GpuDocking> owner(new GpuDocking(s_.dock, sf_, rec_view, rr.gd, store_));
 	owner->prepare_types(rr.types);
@@ -354,8 +357,7 @@ sgpu_server* sgpu_server_create(const sgpu_server_settings* s)
 		sgpu_scoring sf;
 		const int rc = sgpu_builtin_scoring(s->scoring, s->hydrophobic_scale, sv->terms.data(), sv->ci.data(), &sf);
 		if (rc != 0)
-			throw std::invalid_argument(rc == -2 ? "scoring function needs atom charges, which this path does not carry"
-			                                     : "unknown scoring function");
+			throw std::invalid_argument("unknown scoring function");
 		sv->s.reset(new DockingServer(ss, sf));
 	}
 	catch (std::exception& e)
@@ -383,7 +385,7 @@ int sgpu_server_add_receptor(sgpu_server* sv, const sgpu_mol* receptor, const sg
 	try
 	{
 		const sgpu_model_desc* rd = sgpu_mol_desc(receptor);
-		return sv->s->add_receptor(sgpu_receptor{rd->natoms, rd->xyz, rd->type}, *gd,
+		return sv->s->add_receptor(sgpu_receptor{rd->natoms, rd->xyz, rd->type, rd->charge}, *gd,
 				std::vector<int>(types, types + ntypes));
 	}
 	catch (std::exception& e)

@@ -119,6 +119,7 @@ private:
 		sgpu_receptor rec;
 		std::vector<double> xyz;
 		std::vector<int> type;
+		std::vector<double> charge;
 		sgpu_grid_dims gd;
 		std::vector<int> types;
 	};

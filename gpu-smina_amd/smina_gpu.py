@@ -26,7 +26,7 @@ I64P = C.POINTER(C.c_int64)
 
 class Tables(C.Structure):
     _fields_ = [("n", C.c_int), ("factor", C.c_double), ("cutoff_sqr", C.c_double),
-                ("smooth", F64P), ("fast", F64P)]
+                ("smooth", F64P), ("fast", F64P), ("ncomp", C.c_int), ("qsmooth", F64P), ("qfast", F64P)]
 
 
 class Splines(C.Structure):
@@ -35,7 +35,7 @@ class Splines(C.Structure):
 
 
 class Receptor(C.Structure):
-    _fields_ = [("natoms", C.c_int), ("xyz", F64P), ("type", I32P)]
+    _fields_ = [("natoms", C.c_int), ("xyz", F64P), ("type", I32P), ("charge", F64P)]
 
 
 class GridDims(C.Structure):
@@ -50,7 +50,7 @@ class FlatModel(C.Structure):
                 ("frame_parent", I32P), ("frame_kind", I32P), ("frame_begin", I32P), ("frame_end", I32P),
                 ("frame_tree", I32P), ("frame_conf", I32P), ("frame_change", I32P),
                 ("frame_origin", F64P), ("frame_axis", F64P),
-                ("pair_a", I32P), ("pair_b", I32P), ("pair_tri", I32P)]
+                ("pair_a", I32P), ("pair_b", I32P), ("pair_tri", I32P), ("atom_charge", F64P)]
 
 
 class McParams(C.Structure):
@@ -104,7 +104,7 @@ class ModelDesc(C.Structure):
                 ("nbonds", C.c_int), ("bonds", I32P), ("nframes", C.c_int),
                 ("frame_parent", I32P), ("frame_begin", I32P), ("frame_end", I32P),
                 ("frame_origin_atom", I32P), ("frame_axis_atom", I32P),
-                ("nligands", C.c_int), ("ntrees", C.c_int), ("tree_frame", I32P)]
+                ("nligands", C.c_int), ("ntrees", C.c_int), ("tree_frame", I32P), ("charge", F64P)]
 
 
 class Term(C.Structure):
@@ -182,6 +182,7 @@ def _load():
         "sgpu_upload_receptor": (C.c_int, [vp, C.POINTER(Receptor), I32P]),
         "sgpu_build_grids": (C.c_int, [vp, C.c_int, C.POINTER(GridDims), I32P, C.c_int, C.c_int, I32P]),
         "sgpu_download_grid": (C.c_int, [vp, C.c_int, C.c_int, F64P]),
+        "sgpu_download_charge_grid": (C.c_int, [vp, C.c_int, C.c_int, F64P]),
         "sgpu_release_grid": (C.c_int, [vp, C.c_int]),
         "sgpu_upload_models": (C.c_int, [vp, C.POINTER(FlatModel), C.c_int, C.c_int, I32P]),
         "sgpu_release_models": (C.c_int, [vp, C.c_int]),
@@ -201,6 +202,8 @@ def _load():
         "sgpu_set_progress": (C.c_int, [vp, C.c_int, PROGRESS_FN, vp]),
         "sgpu_mol_free": (None, [vp]),
         "sgpu_mol_desc": (C.POINTER(ModelDesc), [vp]),
+        "sgpu_mol_set_charges": (C.c_int, [vp, F64P]),
+        "sgpu_build_qtables": (C.c_int, [C.POINTER(Scoring), C.c_double, F64P, F64P]),
         "sgpu_synth_receptor": (vp, [C.c_uint32, C.c_int, C.c_double]),
         "sgpu_synth_ligand": (vp, [C.c_uint32, C.c_int, C.c_int, F64P]),
         "sgpu_read_pdbqt_ligand": (vp, [C.c_char_p, C.c_char_p, C.c_int]),
@@ -254,7 +257,8 @@ def _load():
 lib = _load()
 
 EXPORTED = ["sgpu_create", "sgpu_destroy", "sgpu_last_error", "sgpu_device_info", "sgpu_upload_tables",
-            "sgpu_upload_receptor", "sgpu_build_grids", "sgpu_download_grid", "sgpu_release_grid",
+            "sgpu_upload_receptor", "sgpu_build_grids", "sgpu_download_grid", "sgpu_download_charge_grid",
+            "sgpu_release_grid",
             "sgpu_upload_models", "sgpu_release_models", "sgpu_eval_confs", "sgpu_run_mc", "sgpu_run_refine",
             "sgpu_run_bfgs", "sgpu_score", "sgpu_last_kernel_ms", "sgpu_set_progress", "sgpu_eval_confs_noalloc"]
 
@@ -290,6 +294,15 @@ class Mol:
             np.zeros((0, 2), np.int32)
         self.frames = [dict(parent=d.frame_parent[f], begin=d.frame_begin[f], end=d.frame_end[f],
                             origin=d.frame_origin_atom[f], axis=d.frame_axis_atom[f]) for f in range(d.nframes)]
+        self.charge = np.ctypeslib.as_array(d.charge, (d.natoms,)).copy() if d.charge else np.zeros(d.natoms)
+
+    def set_charges(self, q):
+        """the atoms' partial charges (sgpu_mol_set_charges)"""
+        q = f64(q)
+        assert len(q) == self.natoms
+        lib.sgpu_mol_set_charges(self.h, ptr(q, F64P))
+        self.desc_ptr = lib.sgpu_mol_desc(self.h)
+        self.charge = q.copy()
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -390,7 +403,7 @@ def builtin_scoring(name="vina", hydrophobic_scale=1.0):
     terms, ci, sf = (Term * 8)(), (CITerm * 4)(), Scoring()
     rc = lib.sgpu_builtin_scoring(name.encode(), hydrophobic_scale, terms, ci, C.byref(sf))
     if rc != 0:
-        raise ValueError(f"scoring {name!r}: " + ("needs atom charges" if rc == -2 else "unknown"))
+        raise ValueError(f"scoring {name!r}: unknown")
     sf._keep = (terms, ci)  # keep alive
     return sf
 
@@ -405,6 +418,19 @@ def build_tables(sf, factor=32.0):
     fast = np.zeros(378 * n)
     lib.sgpu_build_tables(C.byref(sf), factor, ptr(smooth, F64P), ptr(fast, F64P))
     return n, smooth, fast
+
+
+def build_qtables(sf, factor=32.0):
+    """(ncomp, qsmooth, qfast): the charge-dependent components of the
+    tables (None arrays when the scoring has no charge-dependent terms)"""
+    n = lib.sgpu_build_tables(C.byref(sf), factor, None, None)
+    k = lib.sgpu_build_qtables(C.byref(sf), factor, None, None)
+    if k != 4:
+        return k, None, None
+    qs = np.zeros(378 * n * 6)
+    qf = np.zeros(378 * n * 3)
+    lib.sgpu_build_qtables(C.byref(sf), factor, ptr(qs, F64P), ptr(qf, F64P))
+    return k, qs, qf
 
 
 def search_plan(tasks, capacity, chain_split=0, step_fraction=0.0, early_stop=1, budget=0.0):
@@ -543,9 +569,14 @@ class Context:
         self.check(lib.sgpu_device_info(self.h, name, 256, C.byref(cus)))
         return name.value.decode(), cus.value
 
-    def upload_tables(self, n, smooth, fast, factor=32.0, cutoff_sqr=64.0):
-        self._tables = (f64(smooth), f64(fast))
-        t = Tables(n, factor, cutoff_sqr, ptr(self._tables[0], F64P), ptr(self._tables[1], F64P))
+    def upload_tables(self, n, smooth, fast, factor=32.0, cutoff_sqr=64.0, qsmooth=None, qfast=None):
+        """the linear tables; with charge-dependent terms also their three
+        other components (qsmooth [378, n, 3, 2], qfast [378, n, 3])"""
+        self._tables = (f64(smooth), f64(fast), None if qsmooth is None else f64(qsmooth),
+                        None if qfast is None else f64(qfast))
+        t = Tables(n, factor, cutoff_sqr, ptr(self._tables[0], F64P), ptr(self._tables[1], F64P),
+                   1 if qsmooth is None else 4, None if qsmooth is None else ptr(self._tables[2], F64P),
+                   None if qfast is None else ptr(self._tables[3], F64P))
         self.check(lib.sgpu_upload_tables(self.h, C.byref(t)))
 
     def upload_splines(self, nseg, coef, cut, cutoff=8.0):
@@ -562,10 +593,11 @@ class Context:
         ug = UserGrid(ugd, ptr(vals, F64P), map_slope)
         self.check(lib.sgpu_set_user_grid(self.h, C.byref(ug)))
 
-    def upload_receptor(self, xyz, types):
+    def upload_receptor(self, xyz, types, charges=None):
         xyz = f64(xyz).reshape(-1)
         types = i32(types)
-        r = Receptor(len(types), ptr(xyz, F64P), ptr(types, I32P))
+        q = None if charges is None else f64(charges)
+        r = Receptor(len(types), ptr(xyz, F64P), ptr(types, I32P), None if q is None else ptr(q, F64P))
         rid = C.c_int()
         self.check(lib.sgpu_upload_receptor(self.h, C.byref(r), C.byref(rid)))
         return rid.value
@@ -576,9 +608,11 @@ class Context:
         self.check(lib.sgpu_build_grids(self.h, rid, C.byref(gd), ptr(t, I32P), len(t), precision, C.byref(gid)))
         return gid.value
 
-    def download_grid(self, gid, t, gd):
+    def download_grid(self, gid, t, gd, charge=False):
+        """type t's map (charge=True: its charge map), x fastest"""
         out = np.zeros((gd.n[0] + 1) * (gd.n[1] + 1) * (gd.n[2] + 1))
-        self.check(lib.sgpu_download_grid(self.h, gid, t, ptr(out, F64P)))
+        fn = lib.sgpu_download_charge_grid if charge else lib.sgpu_download_grid
+        self.check(fn(self.h, gid, t, ptr(out, F64P)))
         return out
 
     def upload_models(self, flats, precision):
@@ -829,7 +863,7 @@ class Docking:
             res[k]["confs"] = b[:out[k].nposes * cs].reshape(-1, cs).copy() if cs else np.zeros((0, 0))
         return res
 
-    def tune(self, ligands, probe_steps=16):
+    def tune(self, ligands, probe_steps=512):
         """the launch-plan autotuner on the chains a docking of these
         ligands would run: per size class, every candidate plan's probe time
         and the plan kept for later dockings"""

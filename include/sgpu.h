@@ -58,8 +58,15 @@ typedef struct sgpu_tables
 	int n;               /* control points per type pair (2051 at factor 32, cutoff 8) */
 	double factor;       /* points per A^2 */
 	double cutoff_sqr;
-	const double* smooth; /* [378][n][2]: (e, dor) */
+	const double* smooth; /* [378][n][2]: (e, dor) of the charge-independent component */
 	const double* fast;   /* [378][n] */
+	/* with charge-dependent terms (weighted_terms.h:51-57: four result
+	 * components): 4, and the other three components (|q_a|, |q_b| and
+	 * q_a q_b dependent, for the pair (min type, max type)) in qsmooth
+	 * [378][n][3][2] and qfast [378][n][3]; else 1 and NULL */
+	int ncomp;
+	const double* qsmooth;
+	const double* qfast;
 } sgpu_tables;
 
 /* ---- precalculate_splines (src/lib/precalculate.h:313-455): per type pair,
@@ -79,6 +86,7 @@ typedef struct sgpu_receptor
 	int natoms;
 	const double* xyz; /* [3*natoms] */
 	const int* type;   /* [natoms] smina types */
+	const double* charge; /* [natoms] partial charges (NULL: all 0) */
 } sgpu_receptor;
 
 /* ---- a grid box, grid_dims (src/lib/grid_dim.h): n intervals per axis */
@@ -133,6 +141,7 @@ typedef struct sgpu_flat_model
 	const int* pair_a;        /* [npairs] a < b; b may be a fixed atom */
 	const int* pair_b;
 	const int* pair_tri;      /* [npairs] triangular index of (min type, max type) */
+	const double* atom_charge; /* [natoms+ninflex] partial charges (NULL: all 0) */
 } sgpu_flat_model;
 
 /* ---- search parameters (MCParams, src/gpu/gpu_mc_structs.h:21-34) */
@@ -264,6 +273,10 @@ sgpu_status sgpu_build_grids(sgpu_ctx* ctx, int receptor_id, const sgpu_grid_dim
 sgpu_status sgpu_share_grid(sgpu_ctx* dst, sgpu_ctx* src, int src_grid, int* grid_id);
 /* copy one type's map back, point layout x fastest, (n0+1)(n1+1)(n2+1) values */
 sgpu_status sgpu_download_grid(sgpu_ctx* ctx, int grid_id, int type, double* out);
+/* the same for the type's charge map (grid::chargedata, built when the
+ * tables carry charge-dependent terms: the values the evaluation scales by
+ * the ligand atom's charge, src/lib/cache.cpp:172-187) */
+sgpu_status sgpu_download_charge_grid(sgpu_ctx* ctx, int grid_id, int type, double* out);
 sgpu_status sgpu_release_grid(sgpu_ctx* ctx, int grid_id);
 
 /* DeviceModels::upload (src/gpu/gpu_device.cu:307-423) */
@@ -346,15 +359,23 @@ sgpu_status sgpu_run_mc(sgpu_ctx* ctx, int batch_id, int grid_id, const sgpu_mc_
 
 /* The launch-plan autotuner.  It has no counterpart in the reference,
  * whose threads-per-chain policy is a fixed rule
- * (src/gpu/gpu_kernels_impl.cuh:552-605).  For each size class of these
- * chains (by pair count), the class's chains alone run probe_steps Monte
- * Carlo steps from their start state under every candidate plan: chains
- * per workgroup 1, 2, 4 or 8 (where LDS and the chains per ligand allow),
- * one-shot grid or persistent grid over a device work queue.  Each probe
- * is timed with HIP events; the fastest plan is kept by the context for
- * its later sgpu_run_mc calls of the same precision, kernel variant and
- * class.  The chains' trajectories do not depend on the plan.  The report
- * lists every candidate's time and the choice per class. */
+ * (src/gpu/gpu_kernels_impl.cuh:552-605).  A candidate is one plan for
+ * every size class of these chains (by pair count) at once.  The first is
+ * the default plan (per class the chains per workgroup with the most
+ * resident waves per CU, the larger on ties, and the default form); the
+ * others take a target of 8, 4, 2 or 1 chains per workgroup (each class
+ * the largest count up to the target that its LDS plan and chains per
+ * ligand allow) and one form, one-shot grid or persistent grid over a
+ * device work queue.  Every chain runs from its start state with its steps
+ * scaled by one factor, the longest chain's to probe_steps, under each
+ * candidate, the classes concurrently as in a real launch (they share the
+ * device, so a class's plan is only good together with the others').  Each
+ * probe is timed with HIP events.  A candidate replaces the default only
+ * when it is more than 3% faster; the chosen plans are kept by the context
+ * for its later sgpu_run_mc calls of the same precision, kernel variant
+ * and class.  The chains' trajectories do not depend on the plan.  The
+ * report lists, per class, its plan under every candidate with the
+ * candidate's time, and the choice. */
 #define SGPU_TUNE_MAX_CANDIDATES 8
 typedef struct sgpu_mc_tune_report
 {

@@ -22,6 +22,9 @@ typedef struct sgpu_mol sgpu_mol;
 void sgpu_mol_free(sgpu_mol* m);
 /* the molecule as a model description (pointers valid until freed) */
 const sgpu_model_desc* sgpu_mol_desc(const sgpu_mol* m);
+/* set the atoms' partial charges ([natoms]; NULL: all 0).  PDBQT input
+ * carries them (columns 69-76); the synthetic molecules have none. */
+int sgpu_mol_set_charges(sgpu_mol* m, const double* charges);
 
 /* Receptor: nheavy heavy atoms as self-avoiding random-walk chains (1.5 A
  * steps, >= 1.3 A apart) filling a shell around a spherical pocket of
@@ -112,8 +115,10 @@ int sgpu_write_sdf_poses(const sgpu_mol* ligand, int nposes, const double* conf,
  * "dkoes_fast", "dkoes_scoring_old".  hydrophobic_scale multiplies the
  * weight of the hydrophobic term (1: the built-in weights).  terms and ci
  * hold at least SGPU_MAX_TERMS and SGPU_MAX_CI entries and back `out`.
- * Returns 0, or -1 for an unknown name and -2 for the charge-dependent
- * functions (dkoes_scoring, ad4_scoring), which this path does not carry. */
+ * "dkoes_scoring" and "ad4_scoring" carry charge-dependent terms
+ * (ad4_solvation, electrostatic): their tables have four components and
+ * the atoms' partial charges enter the score.  Returns 0, or -1 for an
+ * unknown name. */
 #define SGPU_MAX_TERMS 8
 #define SGPU_MAX_CI 4
 int sgpu_builtin_scoring(const char* name, double hydrophobic_scale, sgpu_term* terms, sgpu_ci_term* ci,
@@ -128,6 +133,10 @@ void sgpu_vina_scoring(sgpu_term* terms, sgpu_ci_term* ci, sgpu_scoring* out, do
 /* precalculate_linear(sf, factor) tables, FP64: smooth [378][n][2], fast
  * [378][n]; returns n (call with NULL buffers to get n) */
 int sgpu_build_tables(const sgpu_scoring* sf, double factor, double* smooth, double* fast);
+/* the tables' charge-dependent components (sgpu_tables.qsmooth / qfast:
+ * [378][n][3][2], [378][n][3]); returns ncomp: 4 when the scoring function
+ * has charge-dependent terms (the buffers are then filled), else 1 */
+int sgpu_build_qtables(const sgpu_scoring* sf, double factor, double* qsmooth, double* qfast);
 /* precalculate_splines(sf, factor) (src/lib/precalculate.h:313-455): coef
  * [378][nseg][5] (x, a, b, c, d per segment), cut [378]; returns nseg =
  * factor x 8 (call with NULL buffers to get it), -1 for a bad factor */

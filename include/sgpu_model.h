@@ -41,6 +41,8 @@ typedef struct sgpu_model_desc
 	int nligands;                 /* the first nligands trees are ligands */
 	int ntrees;
 	const int* tree_frame;        /* [ntrees] first frame of each tree */
+	const double* charge;         /* [natoms] partial charges (NULL: all 0); only the
+	                                 charge-dependent terms read them */
 } sgpu_model_desc;
 
 /* Scoring terms (src/lib/everything.h).  Parameters p[] in the order of the
@@ -48,7 +50,10 @@ typedef struct sgpu_model_desc
  * non_hydrophobic(g, b, c), non_dir_h_bond(g, b, c),
  * non_dir_anti_h_bond_quadratic(o, c), donor_donor_quadratic(o, c),
  * acceptor_acceptor_quadratic(o, c), vdw(i, j, s, cap, c),
- * non_dir_h_bond_lj(o, cap, c). */
+ * non_dir_h_bond_lj(o, cap, c), electrostatic(i, cap, c),
+ * ad4_solvation(d-sigma, s/q, c).  The charge-independent terms are summed
+ * first, in list order, then the charge-dependent ones (weighted_terms::
+ * eval_fast, src/lib/weighted_terms.cpp:52-62). */
 typedef enum sgpu_term_kind
 {
 	SGPU_TERM_GAUSS = 0,
@@ -60,7 +65,11 @@ typedef enum sgpu_term_kind
 	SGPU_TERM_DONOR_DONOR_QUADRATIC = 6,
 	SGPU_TERM_ACCEPTOR_ACCEPTOR_QUADRATIC = 7,
 	SGPU_TERM_VDW = 8,
-	SGPU_TERM_NON_DIR_H_BOND_LJ = 9
+	SGPU_TERM_NON_DIR_H_BOND_LJ = 9,
+	/* charge-dependent terms (src/lib/everything.h:64-156): four result
+	 * components each, combined with the atoms' charges */
+	SGPU_TERM_ELECTROSTATIC = 10,  /* electrostatic(i, cap, c): q_a q_b min(cap, 1 / r^i), i = 1 or 2 */
+	SGPU_TERM_AD4_SOLVATION = 11   /* ad4_solvation(d-sigma, s/q, c) */
 } sgpu_term_kind;
 
 typedef struct sgpu_term

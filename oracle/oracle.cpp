@@ -441,18 +441,73 @@ inline fl smooth_div(fl x, fl y)
 	return x / y;
 }
 
+inline bool charge_dependent(const sgpu_term& t)
+{
+	return t.kind == SGPU_TERM_ELECTROSTATIC || t.kind == SGPU_TERM_AD4_SOLVATION;
+}
+
+// charge_dependent::eval_components (src/lib/everything.h:64-156): the four
+// result components (type only, |q_a|, |q_b|, q_a q_b dependent)
+void term_components(const sgpu_term& t, int t1, int t2, fl r, fl* out)
+{
+	out[0] = out[1] = out[2] = out[3] = 0;
+	if (t.kind == SGPU_TERM_ELECTROSTATIC)
+	{
+		const fl tmp = int_pow(r, int(t.p[0]));
+		const fl cap = t.p[1];
+		if (tmp < epsilon_fl)
+			out[3] = cap;
+		else
+			out[3] = std::min(cap, 1 / tmp);
+		return;
+	}
+	// ad4_solvation
+	const fl solv1 = sgpu::ad_solvation(t1), solv2 = sgpu::ad_solvation(t2);
+	const fl volume1 = sgpu::ad_volume(t1), volume2 = sgpu::ad_volume(t2);
+	const fl mysolv = t.p[1];
+	const fl distfactor = std::exp(-sqr(r / (2 * t.p[0])));
+	out[0] += solv1 * volume2 * distfactor;
+	out[1] += mysolv * volume2 * distfactor;
+	out[0] += solv2 * volume1 * distfactor;
+	out[2] += mysolv * volume1 * distfactor;
+}
+
 struct Scoring
 {
 	std::vector<sgpu_term> terms;
 	std::vector<sgpu_ci_term> ci;
 	int atom_params = 0;
 	fl cutoff = 8;
+	bool charged() const
+	{
+		for (const auto& t : terms)
+			if (charge_dependent(t))
+				return true;
+		return false;
+	}
+	// weighted_terms::eval_fast (src/lib/weighted_terms.cpp:52-62): the
+	// charge-independent terms into the first component, then each
+	// charge-dependent term's components times its weight
+	void eval_fast4(int t1, int t2, fl r, fl* acc) const
+	{
+		acc[0] = acc[1] = acc[2] = acc[3] = 0;
+		for (const auto& t : terms)
+			if (!charge_dependent(t))
+				acc[0] += t.weight * term_eval(t, t1, t2, r, atom_params);
+		for (const auto& t : terms)
+			if (charge_dependent(t))
+			{
+				fl c[4];
+				term_components(t, t1, t2, r, c);
+				for (int k = 0; k < 4; k++)
+					acc[k] += c[k] * t.weight;
+			}
+	}
 	fl eval_fast(int t1, int t2, fl r) const
 	{
-		fl acc = 0;
-		for (const auto& t : terms)
-			acc += t.weight * term_eval(t, t1, t2, r, atom_params);
-		return acc;
+		fl acc[4];
+		eval_fast4(t1, t2, r, acc);
+		return acc[0];
 	}
 	// weighted_terms::conf_independent (src/lib/weighted_terms.cpp:74-80):
 	// each conf-independent term's eval in order (src/lib/everything.h:730-856,
@@ -488,7 +543,11 @@ struct Tables
 	fl factor = 32;
 	fl cutoff_sqr = 64;
 	std::vector<fl> rs;
-	std::vector<fl> e, dor, fast; // [tri][n]
+	std::vector<fl> e, dor, fast; // [tri][n], the first (charge-independent) component
+	// with charge-dependent terms: 4 components, the other three here
+	// ([tri][n][3]: |q_a|, |q_b|, q_a q_b dependent)
+	int ncomp = 1;
+	std::vector<fl> eq, dorq, fastq;
 	void build(const Scoring& sf, fl factor_)
 	{
 		factor = factor_;
@@ -497,30 +556,52 @@ struct Tables
 		rs.assign(n + 2, 0);
 		for (int i = 0; i < n + 2; i++)
 			rs[i] = std::sqrt(i / factor);
+		ncomp = sf.charged() ? 4 : 1;
 		const int ntri = NUM_TYPES * (NUM_TYPES + 1) / 2;
 		e.assign(size_t(ntri) * n, 0);
 		dor.assign(size_t(ntri) * n, 0);
 		fast.assign(size_t(ntri) * n, 0);
+		eq.assign(ncomp == 4 ? size_t(ntri) * n * 3 : 0, 0);
+		dorq.assign(eq.size(), 0);
+		fastq.assign(eq.size(), 0);
+		std::vector<fl> comp(4 * size_t(n));
 		for (int t1 = 0; t1 < NUM_TYPES; t1++)
 			for (int t2 = t1; t2 < NUM_TYPES; t2++)
 			{
 				const size_t base = size_t(sgpu::tri_index(t1, t2)) * n;
 				for (int i = 0; i < n; i++)
-					e[base + i] = sf.eval_fast(t1, t2, rs[i]);
-				// init_from_smooth_fst
-				for (int i = 0; i < n; i++)
 				{
-					if (i == 0 || i == n - 1)
-						dor[base + i] = 0;
-					else
+					sf.eval_fast4(t1, t2, rs[i], &comp[4 * size_t(i)]);
+					e[base + i] = comp[4 * size_t(i)];
+				}
+				// init_from_smooth_fst, component by component
+				for (int c = 0; c < ncomp; c++)
+				{
+					auto val = [&](int i) { return comp[4 * size_t(i) + c]; };
+					for (int i = 0; i < n; i++)
 					{
-						fl delta = rs[i + 1] - rs[i - 1];
-						fl r = rs[i];
-						dor[base + i] = (e[base + i + 1] - e[base + i - 1]) / (delta * r);
+						fl d = 0;
+						if (!(i == 0 || i == n - 1))
+						{
+							fl delta = rs[i + 1] - rs[i - 1];
+							fl r = rs[i];
+							d = (val(i + 1) - val(i - 1)) / (delta * r);
+						}
+						fl f1 = val(i);
+						fl f2 = (i + 1 >= n) ? 0 : val(i + 1);
+						const fl fa = (f2 + f1) / 2;
+						if (c == 0)
+						{
+							dor[base + i] = d;
+							fast[base + i] = fa;
+						}
+						else
+						{
+							eq[(base + i) * 3 + c - 1] = val(i);
+							dorq[(base + i) * 3 + c - 1] = d;
+							fastq[(base + i) * 3 + c - 1] = fa;
+						}
 					}
-					fl f1 = e[base + i];
-					fl f2 = (i + 1 >= n) ? 0 : e[base + i + 1];
-					fast[base + i] = (f2 + f1) / 2;
 				}
 			}
 	}
@@ -633,7 +714,12 @@ struct Tables
 	// precalculate_linear_element::eval_deriv with one component, or
 	// precalculate_splines::eval_deriv (src/lib/precalculate.h:418-450, no
 	// slow terms): the spline's derivative over r
-	void eval_deriv(int t1, int t2, fl r2, fl& eo, fl& doro) const
+	// precalculate_linear::eval_deriv (src/lib/precalculate.h:245-293) of
+	// atoms of types t1, t2 and charges q1, q2: the element of (min type,
+	// max type) with the atoms in that order; with four components each
+	// knot's value is result_components::eval (c0 + |qa| c1 + |qb| c2 +
+	// qa qb c3, src/lib/result_components.h:73-79) before interpolation
+	void eval_deriv(int t1, int t2, fl r2, fl& eo, fl& doro, fl q1 = 0, fl q2 = 0) const
 	{
 		if (kind == 1)
 		{
@@ -649,18 +735,61 @@ struct Tables
 		size_t i1 = size_t(r2_factored);
 		size_t i2 = i1 + 1;
 		fl rem = r2_factored - i1;
-		fl e1 = e[base + i1], e2 = e[base + i2];
-		fl d1 = dor[base + i1], d2 = dor[base + i2];
+		fl e1, e2, d1, d2;
+		if (ncomp == 1)
+		{
+			e1 = e[base + i1];
+			e2 = e[base + i2];
+			d1 = dor[base + i1];
+			d2 = dor[base + i2];
+		}
+		else
+		{
+			const fl qa = t1 <= t2 ? q1 : q2, qb = t1 <= t2 ? q2 : q1;
+			auto comb = [&](const std::vector<fl>& c0, const std::vector<fl>& cq, size_t i) {
+				const fl* q = &cq[(base + i) * 3];
+				return c0[base + i] + std::abs(qa) * q[0] + std::abs(qb) * q[1] + qa * qb * q[2];
+			};
+			e1 = comb(e, eq, i1);
+			e2 = comb(e, eq, i2);
+			d1 = comb(dor, dorq, i1);
+			d2 = comb(dor, dorq, i2);
+		}
 		eo = e1 + rem * (e2 - e1);
 		doro = d1 + rem * (d2 - d1);
 	}
-	// precalculate::eval (eval_fast(...).eval(a, b) + eval_slow) for
-	// uncharged atoms and no slow terms
-	fl eval(int t1, int t2, fl r2) const
+	// precalculate::eval (eval_fast(...).eval(a, b) + eval_slow, no slow
+	// terms): the fast components of (min type, max type), swapped
+	// (swapOrder) when t1 > t2, then result_components::eval with the atoms
+	// in the given order
+	fl eval(int t1, int t2, fl r2, fl q1 = 0, fl q2 = 0) const
 	{
-		fl c0 = fast_value(t1, t2, r2);
-		fl ret = c0 + 0.0 * 0.0 + 0.0 * 0.0 + 0.0 * 0.0 * 0.0;
+		fl c[4] = {fast_value(t1, t2, r2), 0, 0, 0};
+		if (ncomp == 4)
+		{
+			const size_t base = size_t(sgpu::tri_index_permissive(t1, t2)) * n;
+			const fl* q = &fastq[(base + size_t(factor * r2)) * 3];
+			c[1] = t1 <= t2 ? q[0] : q[1];
+			c[2] = t1 <= t2 ? q[1] : q[0];
+			c[3] = q[2];
+		}
+		fl ret = c[0] + std::abs(q1) * c[1] + std::abs(q2) * c[2] + q1 * q2 * c[3];
 		return ret + 0.0;
+	}
+	// the four fast components of receptor type tr and ligand type tl in
+	// that order (eval_fast_data, swapped when tr > tl), for the maps
+	void fast_components(int tr, int tl, fl r2, fl* c) const
+	{
+		c[0] = fast_value(tr, tl, r2);
+		c[1] = c[2] = c[3] = 0;
+		if (ncomp == 4)
+		{
+			const size_t base = size_t(sgpu::tri_index_permissive(tr, tl)) * n;
+			const fl* q = &fastq[(base + size_t(factor * r2)) * 3];
+			c[1] = tr <= tl ? q[0] : q[1];
+			c[2] = tr <= tl ? q[1] : q[0];
+			c[3] = q[2];
+		}
 	}
 	fl fast_value(int t1, int t2, fl r2) const
 	{
@@ -787,6 +916,7 @@ struct RecAtom
 {
 	vec coords;
 	int type;
+	fl charge;
 };
 
 // ---------------------------------------------------------------- model
@@ -829,6 +959,7 @@ struct Model
 	int natoms = 0;  // movable + inflex
 	int nligands = 0;
 	std::vector<int> type;
+	std::vector<fl> charge;  // partial charges, every atom
 	std::vector<vec> local;  // movable: relative to frame origin at identity; inflex: absolute
 	std::vector<std::vector<int>> bonds;
 	std::vector<Frame> frames;
@@ -865,6 +996,9 @@ struct Model
 		natoms = d.natoms;
 		nligands = d.nligands;
 		type.assign(d.type, d.type + natoms);
+		charge.assign(natoms, 0);
+		if (d.charge)
+			charge.assign(d.charge, d.charge + natoms);
 		std::vector<vec> xyz(natoms);
 		for (int i = 0; i < natoms; i++)
 			xyz[i] = vec(d.xyz[3 * i], d.xyz[3 * i + 1], d.xyz[3 * i + 2]);
@@ -1258,6 +1392,7 @@ struct Ctx
 	Grid geom;
 	std::vector<RecAtom> rec; // all receptor atoms (incl. H)
 	std::vector<Grid> grids;  // per type
+	std::vector<Grid> cgrids; // per type: the charge map (four-component tables)
 	std::vector<bool> have_grid;
 	// --user_grid (grid::init from a map file, src/lib/grid.cpp:77-108)
 	Grid ug;
@@ -1321,6 +1456,11 @@ struct Ctx
 				needed.push_back(t);
 				grids[t].setup(gd);
 				grids[t].data.assign(size_t(grids[t].dim[0]) * grids[t].dim[1] * grids[t].dim[2], 0);
+				if (tab.ncomp == 4)
+				{
+					cgrids[t].setup(gd);
+					cgrids[t].data.assign(grids[t].data.size(), 0);
+				}
 				have_grid[t] = true;
 			}
 		if (needed.empty())
@@ -1334,12 +1474,14 @@ struct Ctx
 				relevant.push_back(int(i));
 		std::map<std::array<int, 3>, std::vector<int>> cache;
 		Grid& g = grids[needed.front()];
-		std::vector<fl> aff(needed.size());
+		std::vector<fl> aff(needed.size()), caff(needed.size());
+		const bool charged = tab.ncomp == 4;
 		for (int x = 0; x < g.dim[0]; x++)
 			for (int y = 0; y < g.dim[1]; y++)
 				for (int z = 0; z < g.dim[2]; z++)
 				{
 					std::fill(aff.begin(), aff.end(), 0);
+					std::fill(caff.begin(), caff.end(), 0);
 					vec probe = g.index_to_argument(x, y, z);
 					std::array<int, 3> key;
 					for (int i = 0; i < 3; i++)
@@ -1365,11 +1507,26 @@ struct Ctx
 						const fl r2 = vec_distance_sqr(a.coords, probe);
 						if (r2 <= cutoff_sqr)
 							for (size_t j = 0; j < needed.size(); j++)
-								aff[j] += tab.fast_value(a.type, needed[j], r2);
+							{
+								if (!charged)
+								{
+									aff[j] += tab.fast_value(a.type, needed[j], r2);
+									continue;
+								}
+								// src/lib/cache.cpp:172-187: the receptor atom's
+								// charge in the map, the ligand atom's applied
+								// to the charge map at evaluation
+								fl val[4];
+								tab.fast_components(a.type, needed[j], r2, val);
+								aff[j] += val[0] + val[1] * std::abs(a.charge);
+								caff[j] += val[2] + val[3] * a.charge;
+							}
 					}
 					for (size_t j = 0; j < needed.size(); j++)
 					{
 						grids[needed[j]].at(x, y, z) = aff[j];
+						if (charged)
+							cgrids[needed[j]].at(x, y, z) = caff[j];
 						// src/lib/cache.cpp:204-205: the user grid at the
 						// point's integer indices taken as a location
 						if (has_ug)
@@ -1393,8 +1550,17 @@ struct Ctx
 			}
 			if (!have_grid[t])
 				throw std::runtime_error("grid for a ligand type was not built");
+			// grid::evaluate (src/lib/grid.cpp:28-48): the charge map scaled by
+			// the atom's charge, value and derivative, when it has one
 			vec deriv;
-			e += grids[t].evaluate(m.coords[i], slope, v, &deriv);
+			fl ge = grids[t].evaluate(m.coords[i], slope, v, &deriv);
+			if (m.charge[i] != 0 && tab.ncomp == 4)
+			{
+				vec cderiv(0, 0, 0);
+				ge += m.charge[i] * cgrids[t].evaluate(m.coords[i], slope, v, &cderiv);
+				deriv += m.charge[i] * cderiv;
+			}
+			e += ge;
 			m.minus_forces[i] = deriv;
 		}
 		return e;
@@ -1447,7 +1613,7 @@ struct Ctx
 					if (r2 < epsilon_fl)
 						throw std::runtime_error("Ligand atom exactly overlaps receptor atom.  I can't deal with this.");
 					fl pe, pd;
-					tab.eval_deriv(t1, b.type, r2, pe, pd);
+					tab.eval_deriv(t1, b.type, r2, pe, pd, m.charge[i], b.charge);
 					this_e += pe;
 					deriv += pd * r_ba;
 				}
@@ -1496,7 +1662,7 @@ struct Ctx
 				vec r_ba = adj - b.coords;
 				fl r2 = r_ba.norm_sqr();
 				if (r2 < cutoff_sqr)
-					this_e += tab.eval(t1, b.type, r2);
+					this_e += tab.eval(t1, b.type, r2, m.charge[i], b.charge);
 			}
 			curl(this_e, v);
 			e += this_e + oob_penalty;
@@ -1515,7 +1681,7 @@ struct Ctx
 			if (r2 < cutoff_sqr)
 			{
 				fl pe, pd;
-				tab.eval_deriv(m.type[ip.a], m.type[ip.b], r2, pe, pd);
+				tab.eval_deriv(m.type[ip.a], m.type[ip.b], r2, pe, pd, m.charge[ip.a], m.charge[ip.b]);
 				vec force = pd * r;
 				curl(pe, force, v);
 				e += pe;
@@ -1535,7 +1701,7 @@ struct Ctx
 			fl r2 = vec_distance_sqr(m.coords[ip.a], m.coords[ip.b]);
 			if (r2 < cutoff_sqr)
 			{
-				fl tmp = tab.eval(m.type[ip.a], m.type[ip.b], r2);
+				fl tmp = tab.eval(m.type[ip.a], m.type[ip.b], r2, m.charge[ip.a], m.charge[ip.b]);
 				curl(tmp, v);
 				e += tmp;
 			}
@@ -1591,7 +1757,7 @@ struct Ctx
 				fl r2 = vec_distance_sqr(m.coords[i], b.coords);
 				if (r2 < cutoff_sqr)
 				{
-					fl this_e = tab.eval(t1, b.type, r2);
+					fl this_e = tab.eval(t1, b.type, r2, m.charge[i], b.charge);
 					curl(this_e, v[1]);
 					e += this_e;
 				}
@@ -1604,7 +1770,7 @@ struct Ctx
 			fl r2 = vec_distance_sqr(m.coords[p.a], m.coords[p.b]);
 			if (r2 < cutoff_sqr)
 			{
-				fl this_e = tab.eval(m.type[p.a], m.type[p.b], r2);
+				fl this_e = tab.eval(m.type[p.a], m.type[p.b], r2, m.charge[p.a], m.charge[p.b]);
 				curl(this_e, v[2]);
 				e += this_e;
 			}
@@ -2209,8 +2375,10 @@ orc_ctx* orc_create(const sgpu_scoring* sf, const sgpu_receptor* rec, const sgpu
 	c.geom.setup(c.gd);
 	c.rec.resize(rec->natoms);
 	for (int i = 0; i < rec->natoms; i++)
-		c.rec[i] = {vec(rec->xyz[3 * i], rec->xyz[3 * i + 1], rec->xyz[3 * i + 2]), rec->type[i]};
+		c.rec[i] = {vec(rec->xyz[3 * i], rec->xyz[3 * i + 1], rec->xyz[3 * i + 2]), rec->type[i],
+				rec->charge ? rec->charge[i] : 0.0};
 	c.grids.resize(NUM_TYPES);
+	c.cgrids.resize(NUM_TYPES);
 	c.have_grid.assign(NUM_TYPES, false);
 	c.setup_noncache();
 	return o;
@@ -2248,6 +2416,11 @@ int orc_use_splines(orc_ctx* c, double factor, double* coef, double* cut)
 		return -1;
 	}
 	Tables& t = c->c.tab;
+	if (c->c.sf.charged())
+	{
+		c->c.err = "the spline tables carry one component: no charge-dependent terms";
+		return -1;
+	}
 	t.build_splines(c->c.sf, factor);
 	if (coef)
 		std::copy(t.spl.begin(), t.spl.end(), coef);
@@ -2273,6 +2446,7 @@ void orc_tables(orc_ctx* c, double* smooth, double* fast)
 	}
 }
 double orc_eval_terms(orc_ctx* c, int t1, int t2, double r) { return c->c.sf.eval_fast(t1, t2, r); }
+void orc_eval_components(orc_ctx* c, int t1, int t2, double r, double* out4) { c->c.sf.eval_fast4(t1, t2, r, out4); }
 double orc_conf_independent(orc_ctx* c, double num_tors, double e) { return c->c.sf.conf_independent(num_tors, e); }
 int orc_build_grids(orc_ctx* c, const int* types, int ntypes)
 {
@@ -2285,6 +2459,29 @@ int orc_build_grids(orc_ctx* c, const int* types, int ntypes)
 		c->c.err = e.what();
 		return 1;
 	}
+	return 0;
+}
+int orc_table_ncomp(orc_ctx* c) { return c->c.tab.ncomp; }
+// the three charge-dependent components: qsmooth [tri][n][3][2], qfast [tri][n][3]
+void orc_qtables(orc_ctx* c, double* qsmooth, double* qfast)
+{
+	const Tables& t = c->c.tab;
+	for (size_t k = 0; k < t.eq.size(); k++)
+	{
+		if (qsmooth)
+		{
+			qsmooth[2 * k] = t.eq[k];
+			qsmooth[2 * k + 1] = t.dorq[k];
+		}
+		if (qfast)
+			qfast[k] = t.fastq[k];
+	}
+}
+int orc_charge_grid_values(orc_ctx* c, int type, double* out)
+{
+	if (!c->c.have_grid[type] || c->c.tab.ncomp != 4)
+		return 1;
+	std::copy(c->c.cgrids[type].data.begin(), c->c.cgrids[type].data.end(), out);
 	return 0;
 }
 int orc_grid_values(orc_ctx* c, int type, double* out)

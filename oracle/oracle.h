@@ -52,11 +52,18 @@ int orc_use_splines(orc_ctx* c, double factor, double* coef, double* cut);
 void orc_tables(orc_ctx* c, double* smooth, double* fast);
 /* one charge-independent weighted sum of terms (weighted_terms::eval_fast) */
 double orc_eval_terms(orc_ctx* c, int t1, int t2, double r);
+/* the four weighted components (type only, |q_a|, |q_b|, q_a q_b) */
+void orc_eval_components(orc_ctx* c, int t1, int t2, double r, double* out4);
 /* weighted_terms::conf_independent: the final score of intermolecular energy e */
 double orc_conf_independent(orc_ctx* c, double num_tors, double e);
 
 /* cache::populate for the listed types; maps in point layout (x fastest) */
 int orc_build_grids(orc_ctx* c, const int* types, int ntypes);
+/* 1, or 4 with charge-dependent terms; their three extra components
+ * (qsmooth [378][n][3][2], qfast [378][n][3]) and a type's charge map */
+int orc_table_ncomp(orc_ctx* c);
+void orc_qtables(orc_ctx* c, double* qsmooth, double* qfast);
+int orc_charge_grid_values(orc_ctx* c, int type, double* out);
 int orc_grid_values(orc_ctx* c, int type, double* out);
 
 /* models */

@@ -36,9 +36,13 @@ def _load():
         "orc_table_n": (C.c_int, [vp]),
         "orc_tables": (None, [vp, F64P, F64P]),
         "orc_eval_terms": (C.c_double, [vp, C.c_int, C.c_int, C.c_double]),
+        "orc_eval_components": (None, [vp, C.c_int, C.c_int, C.c_double, F64P]),
         "orc_conf_independent": (C.c_double, [vp, C.c_double, C.c_double]),
         "orc_build_grids": (C.c_int, [vp, I32P, C.c_int]),
         "orc_grid_values": (C.c_int, [vp, C.c_int, F64P]),
+        "orc_table_ncomp": (C.c_int, [vp]),
+        "orc_qtables": (None, [vp, F64P, F64P]),
+        "orc_charge_grid_values": (C.c_int, [vp, C.c_int, F64P]),
         "orc_add_model": (C.c_int, [vp, vp]),
         "orc_model_info": (None, [vp, C.c_int, I32P]),
         "orc_initial_conf": (None, [vp, C.c_int, F64P]),
@@ -116,15 +120,17 @@ def bfgs_quadratic(A, b, x0, maxiters, accurate=False):
 class Oracle:
     """One scoring function + receptor + box, with models added to it."""
 
-    def __init__(self, scoring, receptor_xyz, receptor_types, gd):
+    def __init__(self, scoring, receptor_xyz, receptor_types, gd, receptor_charges=None):
         # scoring, gd: the product binding's ctypes structs (same C layout)
         self._xyz = np.ascontiguousarray(receptor_xyz, dtype=np.float64).reshape(-1)
         self._types = np.ascontiguousarray(receptor_types, dtype=np.int32)
+        self._q = None if receptor_charges is None else np.ascontiguousarray(receptor_charges, dtype=np.float64)
 
         class Rec(C.Structure):
-            _fields_ = [("natoms", C.c_int), ("xyz", F64P), ("type", I32P)]
+            _fields_ = [("natoms", C.c_int), ("xyz", F64P), ("type", I32P), ("charge", F64P)]
 
-        self._rec = Rec(len(self._types), _p(self._xyz, F64P), _p(self._types, I32P))
+        self._rec = Rec(len(self._types), _p(self._xyz, F64P), _p(self._types, I32P),
+                        None if self._q is None else _p(self._q, F64P))
         self._sf = scoring
         self._gd = gd
         self.h = C.c_void_p(lib.orc_create(C.addressof(scoring), C.addressof(self._rec), C.addressof(gd)))
@@ -156,11 +162,35 @@ class Oracle:
         lib.orc_tables(self.h, _p(smooth, F64P), _p(fast, F64P))
         return smooth, fast
 
+    def qtables(self):
+        """(ncomp, qsmooth [378, n, 3, 2], qfast [378, n, 3]); None arrays
+        without charge-dependent terms"""
+        k = lib.orc_table_ncomp(self.h)
+        if k != 4:
+            return k, None, None
+        qs = np.zeros(378 * self.n * 6)
+        qf = np.zeros(378 * self.n * 3)
+        lib.orc_qtables(self.h, _p(qs, F64P), _p(qf, F64P))
+        return k, qs, qf
+
+    def charge_grid(self, t):
+        gd = self._gd
+        g = np.zeros((gd.n[0] + 1) * (gd.n[1] + 1) * (gd.n[2] + 1))
+        if lib.orc_charge_grid_values(self.h, t, _p(g, F64P)) != 0:
+            raise RuntimeError("no charge map for this type")
+        return g
+
     def conf_independent(self, num_tors, e):
         return lib.orc_conf_independent(self.h, num_tors, e)
 
     def eval_terms(self, t1, t2, r):
         return lib.orc_eval_terms(self.h, t1, t2, r)
+
+    def eval_components(self, t1, t2, r):
+        """weighted_terms::eval_fast's four components at distance r"""
+        out = np.zeros(4)
+        lib.orc_eval_components(self.h, t1, t2, r, _p(out, F64P))
+        return out
 
     def set_user_grid(self, ugd, values, map_slope=1e6):
         """--user_grid: values ((n+1)^3, x fastest) as grid::init stores

@@ -32,10 +32,12 @@ def test_tables_match_oracle(lig, name):
     assert n == o.n and np.array_equal(smooth, osm) and np.array_equal(fast, ofa)
 
 
-def test_charge_dependent_functions_are_refused():
-    for name in ("dkoes_scoring", "ad4_scoring", "no_such_function"):
-        with pytest.raises(ValueError):
-            sg.builtin_scoring(name)
+def test_unknown_functions_are_refused():
+    with pytest.raises(ValueError):
+        sg.builtin_scoring("no_such_function")
+    # the charge-dependent functions are built (their tests hold the charges)
+    for name in ("dkoes_scoring", "ad4_scoring"):
+        assert sg.builtin_scoring(name).nterms > 0
 
 
 def test_vinardo_closed_forms(lig):
