@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <array>
 #include <map>
 #include <memory>
@@ -629,6 +630,12 @@ int eval_precision(int prec) { return prec == SGPU_FP64 ? SGPU_FP64 : SGPU_FP32;
 constexpr int LDS_LIMIT = 160 * 1024;
 constexpr int LDS_STATIC = 64; // the persistent Monte Carlo kernel's own LDS (its work item)
 constexpr double TUNE_MARGIN = 0.03; // a tuned plan must beat the default by this fraction
+// the speculative-chain kernel's own LDS (its request and result exchange)
+template<typename S>
+constexpr int spec_static_bytes()
+{
+	return int(sizeof(S)) * (SPEC_REQ + SPEC_MAX * SPEC_RS) + 64;
+}
 
 template<typename T>
 TablePtrs<T> table_ptrs(const sgpu_ctx* c)
@@ -1612,12 +1619,48 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 		int grid = 0;     // workgroups of the class's launch
 		bool persist = false; // persistent grid over a work queue
 		bool tuned = false;   // waves and form from the autotuner
+		bool spec = false;    // one speculative chain per workgroup of `waves` waves
 		std::vector<Blk> blks;
 		size_t block_off = 0, slot_off = 0;
 	};
 	Cls cls[sgpu_ctx::NCLASS];
 	for (auto& kv : by_model)
 		cls[class_of(kv.first)].models.push_back(kv.first);
+	hipDeviceProp_t prop;
+	hip_check(hipGetDeviceProperties(&prop, c->device), "hipGetDeviceProperties");
+	const int variant = (c->tab.kind == 1 || c->tab.ncomp == 4) ? 2 : p->visited ? 1 : 0;
+	// Speculative chains: with fewer chains than the device has wave slots,
+	// each chain gets `spec` waves (wave 0 runs it, the others evaluate the
+	// next trials of its fast line search).  FP32 evaluation, without
+	// QuickVina 2's history; the widest of 4 and 2 whose n x width waves all
+	// fit at once.  SGPU_SPEC=1 turns it off, 2..4 forces the width.
+	int spec = 1;
+	if (pair_parallel<T>() && !p->visited && !p->accurate_line_search)
+	{
+		const char* senv = std::getenv("SGPU_SPEC");
+		const int forced = senv ? std::atoi(senv) : 0;
+		for (int w : {4, 3, 2})
+		{
+			if (forced ? w != forced : w == 3)
+				continue;
+			int slots = std::numeric_limits<int>::max();
+			for (Cls& k : cls)
+				if (!k.models.empty())
+				{
+					const WavePlan pl = plan_for(b.h, k.models);
+					const int bytes = plan_bytes(pl, b.precision, w);
+					const int per_cu = bytes + spec_static_bytes<S>() <= LDS_LIMIT
+							? mc_blocks_per_cu<T, S>(w, size_t(bytes), variant, false, true)
+							: 0;
+					slots = std::min(slots, per_cu * w * prop.multiProcessorCount);
+				}
+			if (forced ? slots > 0 : int64_t(n) * w <= int64_t(slots))
+			{
+				spec = w;
+				break;
+			}
+		}
+	}
 	std::vector<int> block_model, slot_chain;
 	for (Cls& k : cls)
 	{
@@ -1628,7 +1671,6 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 		for (int mi : k.models)
 			maxper = std::max(maxper, int(by_model[mi].size()));
 		// chains per workgroup by the occupancy rule
-		const int variant = (c->tab.kind == 1 || c->tab.ncomp == 4) ? 2 : p->visited ? 1 : 0;
 		k.waves = occupancy_waves<T, S>(k.plan, b.precision, variant, maxper, &k.resident);
 		require(k.waves > 0, "chain state exceeds LDS");
 		// a probe's override, else the tuned plan of this class, where it fits
@@ -1642,7 +1684,15 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 			if (it != c->tuned.end())
 				cp = &it->second;
 		}
-		if (cp && cp->waves >= 1 && cp->waves <= std::min(MAX_CHAINS_PER_GROUP, maxper) &&
+		if (spec > 1)
+		{
+			// one chain per workgroup of `spec` waves
+			k.waves = spec;
+			k.spec = true;
+			k.resident = spec * mc_blocks_per_cu<T, S>(spec, size_t(plan_bytes(k.plan, b.precision, spec)), variant,
+					false, true);
+		}
+		else if (cp && cp->waves >= 1 && cp->waves <= std::min(MAX_CHAINS_PER_GROUP, maxper) &&
 				plan_bytes(k.plan, b.precision, cp->waves) + LDS_STATIC <= LDS_LIMIT &&
 				mc_blocks_per_cu<T, S>(cp->waves, size_t(plan_bytes(k.plan, b.precision, cp->waves)), variant,
 						cp->persist) > 0)
@@ -1655,19 +1705,19 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 		{
 			const int bytes = plan_bytes(k.plan, b.precision, k.waves);
 			std::fprintf(stderr,
-					"[sgpu] mc class %d: %zu models, %d waves/workgroup, %d LDS bytes/workgroup, %d resident "
+					"[sgpu] mc class %d: %zu models, %d waves/workgroup%s, %d LDS bytes/workgroup, %d resident "
 					"waves/CU (atoms %d, pairs %d, frames %d, change %d)\n",
-					int(&k - cls), k.models.size(), k.waves, bytes,
+					int(&k - cls), k.models.size(), k.waves, k.spec ? " (one speculative chain)" : "", bytes,
 					k.resident, k.plan.nmax, k.plan.pmax, k.plan.fmax,
 					k.plan.gmax);
 		}
 		for (int mi : k.models)
 		{
 			const std::vector<int>& cs = by_model[mi];
-			for (size_t s = 0; s < cs.size(); s += size_t(k.waves))
+			for (size_t s = 0; s < cs.size(); s += size_t(k.spec ? 1 : k.waves))
 			{
 				Blk bk{0, mi, {}};
-				for (size_t j = s; j < std::min(cs.size(), s + size_t(k.waves)); j++)
+				for (size_t j = s; j < std::min(cs.size(), s + size_t(k.spec ? 1 : k.waves)); j++)
 				{
 					bk.chains.push_back(cs[j]);
 					bk.cost = std::max(bk.cost, double(ch->num_steps[cs[j]]) *
@@ -1683,7 +1733,7 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 		{
 			block_model.push_back(bk.model);
 			for (int j = 0; j < k.waves; j++)
-				slot_chain.push_back(j < int(bk.chains.size()) ? bk.chains[j] : -1);
+				slot_chain.push_back(k.spec ? bk.chains[0] : j < int(bk.chains.size()) ? bk.chains[j] : -1);
 		}
 	}
 	DBuf dchains, dbm, dsc, dsteps, dmax;
@@ -1732,18 +1782,15 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 	// device keeps resident at once for its plan; they take the class's
 	// items (costliest first) from a work-queue counter, one counter per
 	// class and part, cleared by a kernel at the start.
-	hipDeviceProp_t prop;
-	hip_check(hipGetDeviceProperties(&prop, c->device), "hipGetDeviceProperties");
-	const int variant = (c->tab.kind == 1 || c->tab.ncomp == 4) ? 2 : p->visited ? 1 : 0;
 	// the default form unless a tuned plan says otherwise
 	const bool pdef = persist_default<T, S>(variant);
 	for (Cls& k : cls)
 		if (!k.blks.empty())
 		{
 			if (!k.tuned)
-				k.persist = pdef;
+				k.persist = pdef && !k.spec;
 			const int per_cu = mc_blocks_per_cu<T, S>(k.waves, size_t(plan_bytes(k.plan, b.precision, k.waves)),
-					variant, k.persist);
+					variant, k.persist, k.spec);
 			k.grid = k.persist ? std::max(1, std::min(int(k.blks.size()), per_cu * prop.multiProcessorCount))
 			                   : int(k.blks.size());
 		}
@@ -1837,7 +1884,7 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 				int* q = cl.persist ? dqueue.as<int>() + size_t(part) * sgpu_ctx::NCLASS + size_t(k) : nullptr;
 				hip_check(launch_mc<T, S>(ptrs<T>(b), ec, cl.plan, P, L, dchains.as<char>(), int(cl.blks.size()),
 								  cl.waves, dbm.as<int>() + cl.block_off, dsc.as<int>() + cl.slot_off,
-								  dsteps.as<int>(), dmax.as<int>(), lo, hi, parts, cl.grid, q, st),
+								  dsteps.as<int>(), dmax.as<int>(), lo, hi, parts, cl.grid, q, cl.spec, st),
 						"mc kernel");
 			}
 			for (size_t j = 0; j < side.size(); j++)

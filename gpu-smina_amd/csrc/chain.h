@@ -288,6 +288,30 @@ struct Visited
 	}
 };
 
+// Speculative trials of the fast line search.  When a launch has fewer
+// chains than the device has wave slots, a workgroup gives one chain
+// `width` waves: wave 0 runs the chain, and at each round of its fast line
+// search the other waves evaluate the trials the round would reach next.
+// Wave 0 posts the staged conf x, the direction p, the step alpha and the
+// curl caps (barrier A); wave w evaluates x + alpha 2^-w p while wave 0
+// evaluates x + alpha p (the halvings are exact, so these are the confs
+// the sequential search evaluates); after barrier B wave 0 takes the
+// trials in order, as the sequential search would, and the first that
+// passes the Armijo test ends the line search.  The chain's confs,
+// energies and evaluation counts are the sequential search's.  The helpers
+// touch no chain state.
+constexpr int SPEC_MAX = 4;
+constexpr int SPEC_REQ = MAX_CONF + MAX_CHANGE + 4;  // x, p, alpha, caps
+constexpr int SPEC_RS = 1 + MAX_CONF + MAX_CHANGE;   // per wave: f, conf, gradient
+template<typename S>
+struct SpecLink
+{
+	int width;   // waves per chain (0 or 1: no speculation)
+	int* cmd;    // 0: a request is posted, 1: the chain is done
+	S* req;      // [SPEC_REQ]
+	S* res;      // [SPEC_MAX][SPEC_RS]
+};
+
 // bfgs (src/lib/bfgs.h:201-282) with fast_line_search (:76-96) or
 // accurate_line_search (:101-190), written as one loop around a single
 // model evaluation: the kernels inline the evaluation once.  Each pass of
@@ -295,7 +319,7 @@ struct Visited
 // point, then the line search's trials; a finished line search runs the
 // BFGS step body and starts the next line search.  The order of every
 // evaluation and every arithmetic operation is the CPU's.
-template<typename T, typename S, int BK, bool VIS = false, bool SPL = true>
+template<typename T, typename S, int BK, bool VIS = false, bool SPL = true, bool SPEC = false>
 struct Searcher
 {
 	const MV<T>& m;
@@ -305,6 +329,25 @@ struct Searcher
 	T* pbuf;
 	int64_t evals;
 	Visited<S> vis; // QuickVina 2's history (vis.x null: none)
+	SpecLink<S> spec{}; // speculative trials (wave 0 of a chain's workgroup)
+
+	// post a round of trials at alpha (barrier A); the helpers start on it
+	__device__ __forceinline__ void spec_post(S x, S p, S alpha)
+	{
+		const int lane = lane_id();
+		if (lane < m.h.conf_size)
+			spec.req[lane] = x;
+		if (lane < m.h.change_size)
+			spec.req[MAX_CONF + lane] = p;
+		if (lane == 0)
+		{
+			spec.req[MAX_CONF + MAX_CHANGE] = alpha;
+			for (int d = 0; d < 3; d++)
+				spec.req[MAX_CONF + MAX_CHANGE + 1 + d] = S(ec.v[d]);
+			*spec.cmd = 0;
+		}
+		__syncthreads();
+	}
 
 	// this lane's change-indexed component of the conf x
 	__device__ __forceinline__ S change_value(S x)
@@ -334,6 +377,7 @@ struct Searcher
 		const int n = m.h.change_size, c = m.h.conf_size;
 		const int lane = lane_id();
 		const bool history = VIS && vis.on(n);
+		const bool speculate = SPEC && spec.width > 1 && !accurate;
 		// identity
 		const HessMap hm = hess_map(n > 0 ? n : 1);
 		for (int r0 = 0; r0 < n; r0 += hm.rows)
@@ -350,12 +394,18 @@ struct Searcher
 		int trial = 0, step = 0;
 		bool initial = true;
 		S xe = x; // the conf evaluated next
+		// speculative rounds: whether the last trial taken in the sequential
+		// order was a helper's (then this wave's coordinates are not that
+		// trial's), and that trial's conf
+		bool helper_last = false;
+		S x_last = 0;
 		for (;;)
 		{
 			++evals;
 			T ge_t;
 			const S fe = S(eval_deriv<T, BK, SPL>(m, ec, W, pbuf, T(xe), ge_t));
 			const S ge = S(ge_t);
+			helper_last = false;
 			bool begin = false;
 			if (initial)
 			{
@@ -384,8 +434,39 @@ struct Searcher
 				f1 = fe;
 				g_new = ge;
 				x_new = xe;
-				bool done;
-				if (!accurate)
+				bool done = false;
+				if constexpr (SPEC)
+					if (speculate)
+				{
+					// barrier B: the helpers' trials of this round are in;
+					// take the round's trials in order
+					__syncthreads();
+					for (int w = 0; w < spec.width && !done; w++)
+					{
+						if (w > 0)
+						{
+							const S* r = spec.res + w * SPEC_RS;
+							++evals;
+							f1 = r[0];
+							x_new = lane < c ? r[1 + lane] : S(0);
+							g_new = lane < n ? r[1 + MAX_CONF + lane] : S(0);
+							helper_last = true;
+							x_last = x_new;
+						}
+						if (f1 - f0 < S(0.0001) * alpha * slope)
+							done = true;
+						else
+						{
+							alpha *= S(0.5);
+							done = ++trial >= 10;
+						}
+					}
+				}
+				if (speculate)
+				{
+					// taken above
+				}
+				else if (!accurate)
 				{
 					if (f1 - f0 < S(0.0001) * alpha * slope)
 						done = true;
@@ -453,6 +534,9 @@ struct Searcher
 				}
 				if (!done)
 				{
+					if constexpr (SPEC)
+						if (speculate)
+							spec_post(x, p, alpha);
 					xe = trial_conf(x, p, alpha);
 					continue;
 				}
@@ -519,9 +603,17 @@ struct Searcher
 					alpha2 = 0;
 					f2 = 0;
 				}
+				if constexpr (SPEC)
+					if (speculate)
+						spec_post(x, p, alpha);
 				xe = trial_conf(x, p, alpha);
 			}
 		}
+		// the model's coordinates are the last evaluated conf's, as after
+		// the sequential search (smina's mutation reads them next)
+		if constexpr (SPEC)
+			if (helper_last)
+				set_conf(m, W, T(x_last));
 		if (!(f0 <= f_orig))
 		{
 			f0 = f_orig;
@@ -531,6 +623,47 @@ struct Searcher
 		return f0;
 	}
 };
+
+// A helper wave of a speculative chain (rank 1..width-1 of the workgroup):
+// until wave 0 reports the chain done, evaluate at each posted round the
+// trial x + alpha 2^-rank p on the receptor maps, as Searcher::trial_conf and
+// the evaluation at the top of Searcher::bfgs do, and leave its energy, conf
+// and gradient for wave 0.  Two workgroup barriers per round (A, B), as wave
+// 0 takes them.
+template<typename T, typename S, bool SPL>
+__device__ inline void spec_helper(const MV<T>& m, const EvalCtx<T>& ec, WaveMem<T>& W, SearchMem<S>& SM,
+		const SpecLink<S>& L, int rank)
+{
+	EvalCtx<T> ev = ec;
+	const int lane = lane_id();
+	const int c = m.h.conf_size, n = m.h.change_size;
+	for (;;)
+	{
+		__syncthreads(); // A: a round (or the end) is posted
+		if (*L.cmd)
+			break;
+		S alpha = L.req[MAX_CONF + MAX_CHANGE];
+		for (int k = 0; k < rank; k++)
+			alpha *= S(0.5);
+		for (int d = 0; d < 3; d++)
+			ev.v[d] = T(L.req[MAX_CONF + MAX_CHANGE + 1 + d]);
+		const S x = lane < c ? L.req[lane] : S(0);
+		const S p = lane < n ? L.req[MAX_CONF + lane] : S(0);
+		stage(SM, x, p, c, n);
+		const S xe = conf_increment(m, SM, alpha);
+		lsync();
+		T g;
+		const S f = S(eval_deriv<T, BK_MAPS, SPL>(m, ev, W, nullptr, T(xe), g));
+		S* r = L.res + rank * SPEC_RS;
+		if (lane == 0)
+			r[0] = f;
+		if (lane < c)
+			r[1 + lane] = xe;
+		if (lane < n)
+			r[1 + MAX_CONF + lane] = S(g);
+		__syncthreads(); // B: this round's trials are in
+	}
+}
 
 // ------------------------------------------------------------------ RNG
 // MT19937's regeneration of the 624 words in place, lane-parallel.  Out of

@@ -30,13 +30,14 @@ template<typename T, typename S>
 hipError_t launch_mc(const BatchPtrs<T>& B, const EvalCtx<T>& ec, const WavePlan& plan, const McParams& P,
 		const ChainLayout& L, char* chains, int nblocks, int waves, const int* block_model, const int* slot_chain,
 		const int* chain_steps, const int* chain_maxiters, int part_lo, int part_hi, int parts, int grid, int* queue,
-		hipStream_t st);
+		bool spec, hipStream_t st);
 // workgroups of `waves` chains with `lds` bytes of LDS that one CU holds at
 // once (the runtime's occupancy calculator: registers and LDS)
 template<typename T, typename S>
 // variant: 0 smina's search, 1 with QuickVina 2's history, 2 with the
-// spline tables (and the history)
-int mc_blocks_per_cu(int waves, size_t lds, int variant = 0, bool persist = false);
+// spline or charge-dependent tables (and the history); spec: the
+// speculative-chain kernel (waves = its width)
+int mc_blocks_per_cu(int waves, size_t lds, int variant = 0, bool persist = false, bool spec = false);
 // per-lane private (scratch) bytes of a Monte Carlo kernel form: register
 // spills, which the persistent form must not add
 template<typename T, typename S>

@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 import smina_gpu as sg
-from common import heavy_types, oracle_for
+from common import discontinuity_flags, heavy_types, oracle_for
 
 pytestmark = pytest.mark.gpu
 
@@ -90,16 +90,23 @@ def test_c2_fp32_saved_poses_within_tolerance(c2):
     models, cs, ss, steps, iters = c2_chains(S, 8100)
     cnt, e, c, x, ev = ctx.run_mc(bid, gid, sg.mc_params(), models, cs, ss, steps, iters,
                                   coord_stride=3 * max(f.nheavy for f in S["flats"]))
-    worst = 0.0
+    worst, flagged, total = 0.0, 0, 0
     for i, li in enumerate(models):
         f = S["flats"][li]
         assert cnt[i] > 0
         for j in range(cnt[i]):
-            eo = o.eval_deriv(S["ids"][li], c[i, j, :f.conf_size], (1000.0,) * 3, backend=0, slope=1e6)[0]
+            eo, go, xo = o.eval_deriv(S["ids"][li], c[i, j, :f.conf_size], (1000.0,) * 3, backend=0, slope=1e6)
             err = abs(e[i, j] - eo)
+            total += 1
+            # the reference's rule leaves out evaluations at a discontinuity
+            # (a map cell face, the cutoff, the box edge)
+            if discontinuity_flags(f, S["gd"], S["rec"], [xo], 0)[0]:
+                flagged += 1
+                print(f"  flagged pose {i}/{j}: |dE| {err:.2e}", flush=True)
+                continue
             worst = max(worst, err / max(1.0, abs(eo) / 10))
             assert err <= 1e-3 * max(1.0, abs(eo) / 10), (i, j, e[i, j], eo)
-    print(f"C2 FP32 saved poses: worst scaled |dE| {worst:.2e}", flush=True)
+    print(f"C2 FP32 saved poses: {total}, {flagged} flagged, worst unflagged scaled |dE| {worst:.2e}", flush=True)
 
 
 def test_c3_flex_dock_fp64_matches_oracle():
