@@ -1542,59 +1542,6 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 						p->early_stop_eps >= 0,
 				"early stopping rule");
 	check_maps(b, g, ch->model, n);
-	ChainLayout L{};
-	L.conf_stride = b.h.plan.cmax;
-	int hmax = 1;
-	for (const ModelHdr& h : b.h.hdr)
-		hmax = std::max(hmax, h.nheavy);
-	L.coord_stride = 3 * hmax;
-	L.atom_stride = 3 * b.h.plan.nmax;
-	L.pair_stride = b.h.plan.pmax;
-	L.max_poses = MAX_POSES;
-	size_t off = 0;
-	auto take = [&](size_t bytes) {
-		off = (off + 63) / 64 * 64;
-		size_t o = off;
-		off += bytes;
-		return o;
-	};
-	L.off_mt = take(MT_N * sizeof(uint32_t));
-	L.off_ints = take((3 + MAX_POSES) * sizeof(int));
-	L.off_tmp = take(L.conf_stride * sizeof(S));
-	L.off_scal = take(2 * sizeof(S));
-	L.off_oute = take(MAX_POSES * sizeof(S));
-	L.off_outc = take(size_t(MAX_POSES) * L.conf_stride * sizeof(S));
-	L.off_outx = take(size_t(MAX_POSES) * L.coord_stride * sizeof(T));
-	L.off_mc = take(L.atom_stride * sizeof(T));
-	L.off_ro = take(MAX_TREES * 3 * sizeof(T));
-	L.off_pbuf = take((sizeof(T) == 8 ? size_t(L.pair_stride) : 1) * sizeof(T));
-	L.off_ev = take(sizeof(int64_t));
-	// QuickVina 2's history: 5n points of n coordinates, n up to the batch's
-	// largest change size and at most 64 (chain_layout,
-	// src/gpu/gpu_mc_structs.h:97-152)
-	L.vis_n = p->visited ? std::min(b.h.plan.gmax, 64) : 0;
-	L.off_vx = take(size_t(5) * L.vis_n * L.vis_n * sizeof(S));
-	L.off_vf = take(size_t(5) * L.vis_n * sizeof(S));
-	L.off_vb = take(size_t(10) * L.vis_n * sizeof(uint64_t));
-	L.off_vs = take(3 * sizeof(int));
-	L.bytes = (off + 63) / 64 * 64;
-	// initial chain state (the history empty: zeros)
-	std::vector<char> init(L.bytes * n, 0);
-	for (int i = 0; i < n; i++)
-	{
-		char* base = init.data() + L.bytes * i;
-		std::memcpy(base + L.off_mt, ch->rng + size_t(i) * 625, MT_N * sizeof(uint32_t));
-		int* ints = reinterpret_cast<int*>(base + L.off_ints);
-		ints[0] = int(ch->rng[size_t(i) * 625 + 624]);
-		ints[1] = 0;
-		ints[2] = 0;
-		S* tmp = reinterpret_cast<S*>(base + L.off_tmp);
-		for (int k = 0; k < b.h.hdr[ch->model[i]].conf_size; k++)
-			tmp[k] = S(ch->conf[size_t(i) * ch->conf_stride + k]);
-		S* scal = reinterpret_cast<S*>(base + L.off_scal);
-		scal[0] = 0;
-		scal[1] = Real_host_max<S>();
-	}
 	// Launch plan.  Ligands fall into size classes by pair count; each class
 	// gets its own LDS plan (sized for its largest member) and runs on its own
 	// stream, concurrently with the others.  Within a class, workgroups hold
@@ -1631,11 +1578,11 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 	const int variant = (c->tab.kind == 1 || c->tab.ncomp == 4) ? 2 : p->visited ? 1 : 0;
 	// Speculative chains: with fewer chains than the device has wave slots,
 	// each chain gets `spec` waves (wave 0 runs it, the others evaluate the
-	// next trials of its fast line search).  FP32 evaluation, without
-	// QuickVina 2's history; the widest of 4 and 2 whose n x width waves all
-	// fit at once.  SGPU_SPEC=1 turns it off, 2..4 forces the width.
+	// next trials of its fast line search), without QuickVina 2's history;
+	// the widest of 4 and 2 whose n x width waves all fit at once.
+	// SGPU_SPEC=1 turns it off, 2..4 forces the width.
 	int spec = 1;
-	if (pair_parallel<T>() && !p->visited && !p->accurate_line_search)
+	if (!p->visited && !p->accurate_line_search)
 	{
 		const char* senv = std::getenv("SGPU_SPEC");
 		const int forced = senv ? std::atoi(senv) : 0;
@@ -1660,6 +1607,61 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 				break;
 			}
 		}
+	}
+	ChainLayout L{};
+	L.conf_stride = b.h.plan.cmax;
+	int hmax = 1;
+	for (const ModelHdr& h : b.h.hdr)
+		hmax = std::max(hmax, h.nheavy);
+	L.coord_stride = 3 * hmax;
+	L.atom_stride = 3 * b.h.plan.nmax;
+	L.pair_stride = b.h.plan.pmax;
+	L.max_poses = MAX_POSES;
+	size_t off = 0;
+	auto take = [&](size_t bytes) {
+		off = (off + 63) / 64 * 64;
+		size_t o = off;
+		off += bytes;
+		return o;
+	};
+	L.off_mt = take(MT_N * sizeof(uint32_t));
+	L.off_ints = take((3 + MAX_POSES) * sizeof(int));
+	L.off_tmp = take(L.conf_stride * sizeof(S));
+	L.off_scal = take(2 * sizeof(S));
+	L.off_oute = take(MAX_POSES * sizeof(S));
+	L.off_outc = take(size_t(MAX_POSES) * L.conf_stride * sizeof(S));
+	L.off_outx = take(size_t(MAX_POSES) * L.coord_stride * sizeof(T));
+	L.off_mc = take(L.atom_stride * sizeof(T));
+	L.off_ro = take(MAX_TREES * 3 * sizeof(T));
+	L.off_pbuf = take((sizeof(T) == 8 ? size_t(L.pair_stride) : 1) * sizeof(T));
+	// the ordered pair loop's scratch of a speculative chain's helper waves
+	L.off_hpbuf = take((sizeof(T) == 8 && spec > 1 ? size_t(spec - 1) * L.pair_stride : 1) * sizeof(T));
+	L.off_ev = take(sizeof(int64_t));
+	// QuickVina 2's history: 5n points of n coordinates, n up to the batch's
+	// largest change size and at most 64 (chain_layout,
+	// src/gpu/gpu_mc_structs.h:97-152)
+	L.vis_n = p->visited ? std::min(b.h.plan.gmax, 64) : 0;
+	L.off_vx = take(size_t(5) * L.vis_n * L.vis_n * sizeof(S));
+	L.off_vf = take(size_t(5) * L.vis_n * sizeof(S));
+	L.off_vb = take(size_t(10) * L.vis_n * sizeof(uint64_t));
+	L.off_vs = take(3 * sizeof(int));
+	L.bytes = (off + 63) / 64 * 64;
+	// initial chain state (the history empty: zeros)
+	std::vector<char> init(L.bytes * n, 0);
+	for (int i = 0; i < n; i++)
+	{
+		char* base = init.data() + L.bytes * i;
+		std::memcpy(base + L.off_mt, ch->rng + size_t(i) * 625, MT_N * sizeof(uint32_t));
+		int* ints = reinterpret_cast<int*>(base + L.off_ints);
+		ints[0] = int(ch->rng[size_t(i) * 625 + 624]);
+		ints[1] = 0;
+		ints[2] = 0;
+		S* tmp = reinterpret_cast<S*>(base + L.off_tmp);
+		for (int k = 0; k < b.h.hdr[ch->model[i]].conf_size; k++)
+			tmp[k] = S(ch->conf[size_t(i) * ch->conf_stride + k]);
+		S* scal = reinterpret_cast<S*>(base + L.off_scal);
+		scal[0] = 0;
+		scal[1] = Real_host_max<S>();
 	}
 	std::vector<int> block_model, slot_chain;
 	for (Cls& k : cls)

@@ -632,7 +632,7 @@ struct Searcher
 // 0 takes them.
 template<typename T, typename S, bool SPL>
 __device__ inline void spec_helper(const MV<T>& m, const EvalCtx<T>& ec, WaveMem<T>& W, SearchMem<S>& SM,
-		const SpecLink<S>& L, int rank)
+		const SpecLink<S>& L, int rank, T* pbuf)
 {
 	EvalCtx<T> ev = ec;
 	const int lane = lane_id();
@@ -653,7 +653,7 @@ __device__ inline void spec_helper(const MV<T>& m, const EvalCtx<T>& ec, WaveMem
 		const S xe = conf_increment(m, SM, alpha);
 		lsync();
 		T g;
-		const S f = S(eval_deriv<T, BK_MAPS, SPL>(m, ev, W, nullptr, T(xe), g));
+		const S f = S(eval_deriv<T, BK_MAPS, SPL>(m, ev, W, pbuf, T(xe), g));
 		S* r = L.res + rank * SPEC_RS;
 		if (lane == 0)
 			r[0] = f;
@@ -772,6 +772,7 @@ struct ChainLayout
 	int max_poses;
 	size_t bytes;      // per chain
 	size_t off_mt, off_ints, off_tmp, off_scal, off_oute, off_outc, off_outx, off_mc, off_ro, off_pbuf, off_ev;
+	size_t off_hpbuf;  // FP64: the ordered pair loop's scratch of a speculative chain's helper waves
 	int vis_n;         // QuickVina 2's history for change sizes up to this (0: none)
 	size_t off_vx, off_vf, off_vb, off_vs;
 };

@@ -335,15 +335,18 @@ def check_against_oracle(env, models, cs, ss, steps, iters, out):
         assert np.array_equal(c[i, :cnt[i], :co.shape[1]], co)
 
 
-@pytest.mark.parametrize("form", ["graph-persistent", "graph-oneshot", "eager-persistent", "eager-oneshot"])
+@pytest.mark.parametrize("form", ["graph-persistent", "graph-oneshot", "eager-persistent", "eager-oneshot",
+                                  "graph-speculative4", "eager-speculative2"])
 def test_mc_fp64_launch_forms(env, form, monkeypatch):
     """the Monte Carlo launch in each of its forms: persistent grids over the
-    device work queue or one workgroup per item, the whole search captured
-    in a HIP graph or enqueued directly, over three parts (progress
+    device work queue, one workgroup per item, or one workgroup of 2 or 4
+    waves per chain (speculative line-search trials), the whole search
+    captured in a HIP graph or enqueued directly, over three parts (progress
     reports as graph host nodes).  The graph is built and replayed on every
     call: two calls in a row with an eager call between them; every
     chain of every call follows the oracle exactly"""
     monkeypatch.setenv("SGPU_QUEUE", "1" if "persistent" in form else "0")
+    monkeypatch.setenv("SGPU_SPEC", form[-1] if "speculative" in form else "1")
     if form.startswith("eager"):
         monkeypatch.setenv("SGPU_NO_GRAPH", "1")
     ctx = env["ctx"]
@@ -630,27 +633,30 @@ def test_fp32_launch_forms_agree(env, prec, monkeypatch):
         ctx.release_models(bid)
 
 
-@pytest.mark.parametrize("prec", [sg.FP32, sg.MIXED], ids=["fp32", "mixed"])
+@pytest.mark.parametrize("prec", [sg.FP32, sg.MIXED, sg.FP64], ids=["fp32", "mixed", "fp64"])
 def test_speculative_chains_equal_sequential(env, prec, monkeypatch):
     """speculative line-search trials: with 2 or 4 waves per chain (wave 0
     runs the chain, the others evaluate the next trial steps of its fast
     line search), every chain of a batch across the size classes gives the
     sequential search's result exactly: saved energies, confs, coordinates
     and evaluation counts, over both passes of each step (hunting and
-    authentic caps)"""
+    authentic caps).  In FP64 the chains also follow the oracle's exactly."""
     ctx = env["ctx"]
     bid = env["bids"][prec] if prec in env["bids"] else ctx.upload_models(env["flats"], prec)
+    gid = env["gids"][sg.FP64 if prec == sg.FP64 else sg.FP32]
     models, cs, ss, steps, iters = batched_chains(env, seed0=5100)
     steps = [30 + 10 * li for li in models]
     outs = {}
     for w in ("1", "2", "4"):
         monkeypatch.setenv("SGPU_SPEC", w)
         print(f"speculative chains: precision {prec}, width {w}", flush=True)
-        outs[w] = ctx.run_mc(bid, env["gids"][sg.FP32], sg.mc_params(), models, cs, ss, steps, iters)
+        outs[w] = ctx.run_mc(bid, gid, sg.mc_params(), models, cs, ss, steps, iters)
     for w in ("2", "4"):
         for a, b in zip(outs["1"], outs[w]):
             assert np.array_equal(a, b), w
     assert min(outs["1"][0]) > 0
+    if prec == sg.FP64:
+        check_against_oracle(env, models, cs, ss, steps, iters, outs["4"])
     if prec not in env["bids"]:
         ctx.release_models(bid)
 
