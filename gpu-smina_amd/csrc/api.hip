@@ -1917,9 +1917,9 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 	{
 		if (use_graph)
 		{
-			hipStreamEndCapture(c->stream, &graph);
+			(void)hipStreamEndCapture(c->stream, &graph);
 			if (graph)
-				hipGraphDestroy(graph);
+				(void)hipGraphDestroy(graph);
 		}
 		throw;
 	}
@@ -1930,13 +1930,13 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 		if (ie == hipSuccess)
 		{
 			const hipError_t le = hipGraphLaunch(exec, c->stream);
-			hipGraphExecDestroy(exec);
-			hipGraphDestroy(graph);
+			(void)hipGraphExecDestroy(exec);
+			(void)hipGraphDestroy(graph);
 			hip_check(le, "graph launch");
 		}
 		else
 		{
-			hipGraphDestroy(graph);
+			(void)hipGraphDestroy(graph);
 			hip_check(ie, "graph instantiate");
 		}
 	}
