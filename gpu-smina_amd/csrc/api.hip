@@ -1785,7 +1785,11 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 	// items (costliest first) from a work-queue counter, one counter per
 	// class and part, cleared by a kernel at the start.
 	// the default form unless a tuned plan says otherwise
+	// SGPU_PERSIST_GRID caps a persistent grid's workgroups (each then runs
+	// more items)
 	const bool pdef = persist_default<T, S>(variant);
+	const char* genv = std::getenv("SGPU_PERSIST_GRID");
+	const int grid_cap = genv ? std::max(1, std::atoi(genv)) : std::numeric_limits<int>::max();
 	for (Cls& k : cls)
 		if (!k.blks.empty())
 		{
@@ -1793,7 +1797,7 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 				k.persist = pdef && !k.spec;
 			const int per_cu = mc_blocks_per_cu<T, S>(k.waves, size_t(plan_bytes(k.plan, b.precision, k.waves)),
 					variant, k.persist, k.spec);
-			k.grid = k.persist ? std::max(1, std::min(int(k.blks.size()), per_cu * prop.multiProcessorCount))
+			k.grid = k.persist ? std::max(1, std::min({int(k.blks.size()), per_cu * prop.multiProcessorCount, grid_cap}))
 			                   : int(k.blks.size());
 		}
 	DBuf dqueue;

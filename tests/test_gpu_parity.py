@@ -336,17 +336,21 @@ def check_against_oracle(env, models, cs, ss, steps, iters, out):
 
 
 @pytest.mark.parametrize("form", ["graph-persistent", "graph-oneshot", "eager-persistent", "eager-oneshot",
+                                  "graph-persistent-grid2", "eager-persistent-grid2",
                                   "graph-speculative4", "eager-speculative2"])
 def test_mc_fp64_launch_forms(env, form, monkeypatch):
     """the Monte Carlo launch in each of its forms: persistent grids over the
-    device work queue, one workgroup per item, or one workgroup of 2 or 4
-    waves per chain (speculative line-search trials), the whole search
-    captured in a HIP graph or enqueued directly, over three parts (progress
-    reports as graph host nodes).  The graph is built and replayed on every
-    call: two calls in a row with an eager call between them; every
-    chain of every call follows the oracle exactly"""
+    device work queue (a grid of 2 workgroups makes each run several items),
+    one workgroup per item, or one workgroup of 2 or 4 waves per chain
+    (speculative line-search trials),
+    the whole search captured in a HIP graph or enqueued directly, over
+    three parts (progress reports as graph host nodes).  The graph is built
+    and replayed on every call: two calls in a row with an eager call
+    between them; every chain of every call follows the oracle exactly"""
     monkeypatch.setenv("SGPU_QUEUE", "1" if "persistent" in form else "0")
     monkeypatch.setenv("SGPU_SPEC", form[-1] if "speculative" in form else "1")
+    if "grid2" in form:
+        monkeypatch.setenv("SGPU_PERSIST_GRID", "2")
     if form.startswith("eager"):
         monkeypatch.setenv("SGPU_NO_GRAPH", "1")
     ctx = env["ctx"]
@@ -619,8 +623,12 @@ def test_fp32_launch_forms_agree(env, prec, monkeypatch):
     models, cs, ss, steps, iters = batched_chains(env, seed0=5300)
     monkeypatch.setenv("SGPU_SPEC", "1")
     outs = []
-    for q, graph in (("1", True), ("0", True), ("0", False)):
+    for q, graph, grid in (("1", True, None), ("0", True, None), ("0", False, None), ("1", False, "2")):
         monkeypatch.setenv("SGPU_QUEUE", q)
+        if grid:
+            monkeypatch.setenv("SGPU_PERSIST_GRID", grid)
+        else:
+            monkeypatch.delenv("SGPU_PERSIST_GRID", raising=False)
         if graph:
             monkeypatch.delenv("SGPU_NO_GRAPH", raising=False)
         else:
