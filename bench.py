@@ -249,6 +249,8 @@ def main():
                     help="one ligand at exhaustiveness 128 in FP64, FP32 and mixed: error and top-pose drift")
     ap.add_argument("--flex", action="store_true",
                     help="the flexible-receptor batch (256 ligands, 4 side chains, mixed precision)")
+    ap.add_argument("--minimize", type=int, default=0,
+                    help="--minimize throughput on this many parsed poses (FP64, spline approximation)")
     ap.add_argument("--accuracy", type=int, default=0,
                     help="top-pose accuracy on this many ligands: served vs FP64, mixed vs FP64, FP64 seed 2 vs 1")
     ap.add_argument("--precision", choices=["fp32", "mixed", "fp64"], default=None,
@@ -289,6 +291,9 @@ def main():
         return
     if args.single:
         single(sg, args, log)
+        return
+    if args.minimize > 0:
+        minimize(sg, args, log)
         return
 
     if args.flex:
@@ -525,6 +530,73 @@ def fast_vs_full(sg, args, log, rec, ligs, gd, hyd, prec, chunks, top_fast):
             "note": "per-ligand top score (kcal/mol, after refinement and FP64 rescoring) of the --fast screen "
                     "minus the default search's, same seed; top10_overlap: share of the default search's best "
                     "10% that the fast screen also ranks in its best 10%"}
+
+
+def minimize(sg, args, log):
+    """--minimize (minimize_batch, src/gpu/gpu_docking.cpp:690-742): every
+    pose refined in FP64 with the accurate line search inside its own
+    movable_atoms_box(4, 0.375), the spline approximation at factor 10 (the
+    option's default), all poses in one launch.  The poses are the screen
+    generator's ligands at their parsed coordinates (not docked: the
+    reference's "random poses" row).  CPU baseline: the oracle's
+    refine_structure on one core over a sample, each pose's result checked
+    against the device's."""
+    n = args.minimize
+    rec = sg.Mol.receptor(1, 1800, 8.0)
+    ligs, heavy, tors = screen_ligands(sg, n, seed=1890)
+    gd = sg.grid_dims((0.0, 0.0, 0.0), (26.0, 26.0, 26.0))
+    d = sg.Docking(rec, gd, fp64=False, approximation="spline")
+    d.minimize(ligs[:4], maxiters=0, accurate=True)  # loads the code objects
+    times, kms = [], []
+    for k in range(max(1, args.steps)):
+        t = time.perf_counter()
+        res = d.minimize(ligs, maxiters=0, accurate=True)
+        times.append(time.perf_counter() - t)
+        kms.append(d.kernel_ms("refine"))
+        log(f"minimize {n} poses: {times[-1]:.3f}s (refine kernel {kms[-1]:.1f} ms)")
+    es = np.array([r["e"] for r in res])
+    wall = float(np.median(times))
+    line = {"metric": "--minimize poses/s (FP64, spline approximation, one launch)", "value": n / wall,
+            "unit": "poses/s", "n_gpus": 1, "steps": len(times), "warmup": 1, "ms_per_step": 1000 * wall,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (seeded receptor and ligands at their parsed poses)",
+            "config": {"workload": f"{n} poses (10-60 heavy atoms, Poisson(5) torsions <= 15) of the screen "
+                                   f"generator, each in its movable_atoms_box(4, 0.375) against the 1,800-atom "
+                                   f"receptor's direct terms, spline tables at factor 10, accurate line search"},
+            "refine_kernel_ms": float(np.median(kms)), "seconds": times,
+            "energy_median": float(np.median(es[np.isfinite(es)])) if np.isfinite(es).any() else None}
+    if not args.no_cpu:
+        line["cpu_baseline"] = minimize_cpu_baseline(sg, args, log, rec, ligs, res)
+    print(json.dumps(line), flush=True)
+
+
+def minimize_cpu_baseline(sg, args, log, rec, ligs, res):
+    """the oracle's refine_structure of a sample of the poses on one core,
+    each in the box the device used; the tables are built once per pose box
+    by the oracle and that set-up is left out, as smina's CPU --minimize
+    builds them once per run"""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    sf = sg.vina_scoring()
+    pick = list(range(0, len(ligs), max(1, len(ligs) // 32)))[:32]
+    spent, same = 0.0, 0
+    for i in pick:
+        r = res[i]
+        g = sg.GridDims()
+        for k in range(3):
+            g.begin[k], g.end[k] = r["box"][0][k], r["box"][1][k]
+            g.n[k] = int(round((g.end[k] - g.begin[k]) / 0.375))
+        o = po.Oracle(sf, rec.xyz, rec.type, g)
+        o.use_splines(10.0)
+        m = o.add_model(ligs[i].desc_ptr)
+        t = time.perf_counter()
+        e, _, _ = o.refine(m, o.initial_conf(m), 10000, accurate=True)
+        spent += time.perf_counter() - t
+        same += int(e == r["e"])
+    log(f"CPU oracle --minimize: {len(pick)} poses in {spent:.2f}s on one core, {same} bit-equal to the device's")
+    return {"value": len(pick) / spent, "unit": "poses/s", "cores": 1, "kind": "port",
+            "sample": f"{len(pick)} poses spread over the set, one core, refine only (the oracle's per-box set-up "
+                      f"left out); {same} of {len(pick)} FP64 energies bit-equal to the device's"}
 
 
 def top_pose_accuracy(sg, args, log, rec, sample, gd, hyd, served):
