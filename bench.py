@@ -459,8 +459,9 @@ def main():
                                "arrivals is the serving run's (--serve)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "mc_kernel<float, float, false, false>" if prec == sg.FP32 else
-                     "mc_kernel<float, double, false, false>" if prec == sg.MIXED else "mc_kernel<double, double, false, false>",
+                     "kernel": "mc_kernel<float, float, false, false, true, false>" if prec == sg.FP32 else
+                     "mc_kernel<float, double, false, false, true, false>" if prec == sg.MIXED else
+                     "mc_kernel<double, double, false, false, true, false>",
                      "kernel_ms": mc_ms,
                      "algorithmic_bytes": alg_bytes,
                      "note": "one launch = the four size-class dispatches of one Monte Carlo launch, run "
