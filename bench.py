@@ -535,9 +535,9 @@ def fast_vs_full(sg, args, log, rec, ligs, gd, hyd, prec, chunks, top_fast):
 
 def minimize(sg, args, log):
     """--minimize (minimize_batch, src/gpu/gpu_docking.cpp:690-742): every
-    pose refined in FP64 with the accurate line search inside its own
-    movable_atoms_box(4, 0.375), the spline approximation at factor 10 (the
-    option's default), all poses in one launch.  The poses are the screen
+    pose refined in FP64 with the accurate line search and the force cap 10
+    inside its own movable_atoms_box(4, 0.375), the spline approximation at
+    factor 10 (the option's defaults), all poses in one launch.  The poses are the screen
     generator's ligands at their parsed coordinates (not docked: the
     reference's "random poses" row).  CPU baseline: the oracle's
     refine_structure on one core over a sample, each pose's result checked
@@ -547,11 +547,11 @@ def minimize(sg, args, log):
     ligs, heavy, tors = screen_ligands(sg, n, seed=1890)
     gd = sg.grid_dims((0.0, 0.0, 0.0), (26.0, 26.0, 26.0))
     d = sg.Docking(rec, gd, fp64=False, approximation="spline")
-    d.minimize(ligs[:4], maxiters=0, accurate=True)  # loads the code objects
+    d.minimize(ligs[:4], maxiters=0, accurate=True, force_cap=10.0)  # loads the code objects
     times, kms = [], []
     for k in range(max(1, args.steps)):
         t = time.perf_counter()
-        res = d.minimize(ligs, maxiters=0, accurate=True)
+        res = d.minimize(ligs, maxiters=0, accurate=True, force_cap=10.0)
         times.append(time.perf_counter() - t)
         kms.append(d.kernel_ms("refine"))
         log(f"minimize {n} poses: {times[-1]:.3f}s (refine kernel {kms[-1]:.1f} ms)")
@@ -563,7 +563,8 @@ def minimize(sg, args, log):
             "data": "synthetic (seeded receptor and ligands at their parsed poses)",
             "config": {"workload": f"{n} poses (10-60 heavy atoms, Poisson(5) torsions <= 15) of the screen "
                                    f"generator, each in its movable_atoms_box(4, 0.375) against the 1,800-atom "
-                                   f"receptor's direct terms, spline tables at factor 10, accurate line search"},
+                                   f"receptor's direct terms, spline tables at factor 10, accurate line search, "
+                                   f"force cap 10"},
             "refine_kernel_ms": float(np.median(kms)), "seconds": times,
             "energy_median": float(np.median(es[np.isfinite(es)])) if np.isfinite(es).any() else None}
     if not args.no_cpu:
@@ -591,7 +592,7 @@ def minimize_cpu_baseline(sg, args, log, rec, ligs, res):
         o.use_splines(10.0)
         m = o.add_model(ligs[i].desc_ptr)
         t = time.perf_counter()
-        e, _, _ = o.refine(m, o.initial_conf(m), 10000, accurate=True)
+        e, _, _ = o.refine(m, o.initial_conf(m), 10000, v=(10.0, 10.0, 10.0), accurate=True)
         spent += time.perf_counter() - t
         same += int(e == r["e"])
     log(f"CPU oracle --minimize: {len(pick)} poses in {spent:.2f}s on one core, {same} bit-equal to the device's")

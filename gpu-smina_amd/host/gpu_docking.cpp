@@ -1102,10 +1102,12 @@ sgpu_grid_dims movable_atoms_box_at(const double* xyz, int n, double add, double
 	return gd;
 }
 
-void GpuDocking::minimize_batch(std::vector<MinimizeJob*>& jobs)
+void GpuDocking::minimize_batch(std::vector<MinimizeJob*>& jobs, double cap)
 {
 	if (jobs.empty())
 		return;
+	if (!(cap > 0))
+		throw std::invalid_argument("minimize: the force cap must be positive");
 	std::vector<FlatModel> fms;
 	for (MinimizeJob* j : jobs)
 		fms.push_back(flatten(*j->m));
@@ -1166,7 +1168,7 @@ void GpuDocking::minimize_batch(std::vector<MinimizeJob*>& jobs)
 		rq.receptor_id = receptor_;
 		rq.gd = &un;
 		for (int k = 0; k < 3; k++)
-			rq.v[k] = 1000;
+			rq.v[k] = cap;
 		rq.accurate_line_search = acc;
 		rq.nposes = n;
 		rq.model = model.data();
@@ -1869,7 +1871,7 @@ int sgpu_dock(sgpu_docking* d, const sgpu_mol* const* ligands, int n, sgpu_dock_
 }
 
 int sgpu_minimize(sgpu_docking* d, const sgpu_mol* const* ligands, int n, int maxiters, int accurate,
-		sgpu_minimize_result* out)
+		double force_cap, sgpu_minimize_result* out)
 {
 	if (!d->d)
 		return -1;
@@ -1884,7 +1886,7 @@ int sgpu_minimize(sgpu_docking* d, const sgpu_mol* const* ligands, int n, int ma
 			jobs[i].accurate = accurate != 0;
 			jp.push_back(&jobs[i]);
 		}
-		d->d->minimize_batch(jp);
+		d->d->minimize_batch(jp, force_cap);
 		for (int i = 0; i < n; i++)
 		{
 			out[i].e = jobs[i].e;

@@ -227,7 +227,10 @@ public:
 		std::vector<double> coords; // heavy movable atoms after refinement
 		double rmsd = 0;            // heavy-atom RMSD from the parsed pose
 	};
-	void minimize_batch(std::vector<MinimizeJob*>& jobs);
+	// cap: every BFGS's force cap (authentic_v's three components;
+	// main.cpp sets 10 for --minimize and keeps --force_cap's 1000 for
+	// --local_only, src/main/main.cpp:1216-1220)
+	void minimize_batch(std::vector<MinimizeJob*>& jobs, double cap);
 
 	// receptor maps for every heavy-atom type of these models
 	void prepare(const std::vector<const sgpu_model_desc*>& models);

@@ -230,7 +230,8 @@ def _load():
         "sgpu_docking_free": (None, [vp]),
         "sgpu_docking_error": (C.c_char_p, [vp]),
         "sgpu_dock": (C.c_int, [vp, C.POINTER(vp), C.c_int, C.POINTER(DockResult)]),
-        "sgpu_minimize": (C.c_int, [vp, C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.POINTER(MinimizeResult)]),
+        "sgpu_minimize": (C.c_int, [vp, C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_double,
+                                    C.POINTER(MinimizeResult)]),
         "sgpu_server_create": (vp, [C.POINTER(ServerSettings)]),
         "sgpu_server_free": (None, [vp]),
         "sgpu_server_error": (C.c_char_p, [vp]),
@@ -877,11 +878,13 @@ class Docking:
         """ligands of the last dock() that got the full search"""
         return lib.sgpu_docking_full_searches(self.h)
 
-    def minimize(self, ligands, maxiters=0, accurate=True):
-        """--minimize: FP64 refine_structure of each parsed pose in its own box"""
+    def minimize(self, ligands, maxiters=0, accurate=True, force_cap=10.0):
+        """--minimize: FP64 refine_structure of each parsed pose in its own box,
+        every BFGS under force_cap (--minimize's 10; --local_only keeps
+        --force_cap's 1000)"""
         arr = (C.c_void_p * len(ligands))(*[m.h.value for m in ligands])
         out = (MinimizeResult * len(ligands))()
-        if lib.sgpu_minimize(self.h, arr, len(ligands), maxiters, int(accurate), out) != 0:
+        if lib.sgpu_minimize(self.h, arr, len(ligands), maxiters, int(accurate), force_cap, out) != 0:
             raise RuntimeError(lib.sgpu_docking_error(self.h).decode())
         return [dict(e=r.e, rmsd=r.rmsd, box=(tuple(r.box_begin), tuple(r.box_end))) for r in out]
 

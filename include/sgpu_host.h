@@ -214,7 +214,9 @@ int sgpu_dock(sgpu_docking* d, const sgpu_mol* const* ligands, int n, sgpu_dock_
 
 /* --minimize / --local_only in FP64 (GpuDocking::minimize_batch,
  * src/gpu/gpu_docking.h:96-109): refine_structure of each ligand's parsed
- * pose in its own movable_atoms_box(4, 0.375); maxiters 0 = 10000 */
+ * pose in its own movable_atoms_box(4, 0.375); maxiters 0 = 10000;
+ * force_cap: every BFGS's cap (main.cpp: 10 for --minimize, --force_cap's
+ * 1000 for --local_only), positive */
 typedef struct sgpu_minimize_result
 {
 	double e;           /* energy of the refined pose (DBL_MAX if it left the box) */
@@ -223,7 +225,7 @@ typedef struct sgpu_minimize_result
 	double box_end[3];
 } sgpu_minimize_result;
 int sgpu_minimize(sgpu_docking* d, const sgpu_mol* const* ligands, int n, int maxiters, int accurate,
-		sgpu_minimize_result* out);
+		double force_cap, sgpu_minimize_result* out);
 /* ligands of the last sgpu_dock that got the full search (all of them
  * unless a two-pass screen ran) */
 int sgpu_docking_full_searches(const sgpu_docking* d);

@@ -487,21 +487,32 @@ def test_invalid_requests_are_rejected(env):
                        grid_id=gid)
 
 
-@pytest.mark.parametrize("approximation", ["linear", "spline"])
-def test_minimize_fp64_matches_oracle(approximation):
-    """--minimize (GpuDocking::minimize_batch): each parsed pose refined in
-    FP64 with the accurate line search inside its own movable_atoms_box,
-    several boxes in one launch; bit-exact against the oracle's
-    refine_structure over that box.  With the spline approximation, the
-    default of --minimize (src/main/main.cpp:1225-1228, factor 10), both
-    sides evaluate precalculate_splines"""
+@pytest.mark.parametrize("mode", ["minimize-linear", "minimize-spline", "local_only"])
+def test_minimize_fp64_matches_oracle(mode):
+    """--minimize and --local_only (GpuDocking::minimize_batch): each parsed
+    pose refined in FP64 inside its own movable_atoms_box, several boxes in
+    one launch; bit-exact against the oracle's refine_structure over that
+    box.  --minimize: the accurate line search, 10,000 iterations and a
+    force cap of 10 (src/main/main.cpp:1216-1229), with the linear tables
+    or the spline approximation, its default (factor 10; both sides
+    evaluate precalculate_splines).  --local_only: the fast line search,
+    setup_search's (25 + movable atoms) / 3 iterations and --force_cap's
+    1000 (:513-522, :85)"""
     rec = receptor()
     ligs = [sg.Mol.ligand(40 + k, 16 + 6 * k, 2 + k, center=(0.5 * k, -0.5 * k, 0.25 * k)) for k in range(4)]
+    approximation = "spline" if mode == "minimize-spline" else "linear"
     d = sg.Docking(rec, gd_c0(), fp64=False, approximation=approximation)
-    res = d.minimize(ligs, maxiters=0, accurate=True)
+    local = mode == "local_only"
+    cap = 1000.0 if local else 10.0
+    if local:
+        iters = [(25 + sg.Flat(lig).natoms) // 3 for lig in ligs]
+        res = [d.minimize([lig], maxiters=it, accurate=False, force_cap=cap)[0] for lig, it in zip(ligs, iters)]
+    else:
+        iters = [10000] * len(ligs)
+        res = d.minimize(ligs, maxiters=0, accurate=True, force_cap=cap)
     sf = sg.vina_scoring()
     import pyoracle as po
-    for lig, r in zip(ligs, res):
+    for lig, r, it in zip(ligs, res, iters):
         gd = sg.GridDims()
         for i in range(3):
             gd.begin[i], gd.end[i] = r["box"][0][i], r["box"][1][i]
@@ -510,8 +521,8 @@ def test_minimize_fp64_matches_oracle(approximation):
         if approximation == "spline":
             o.use_splines(10.0)
         m = o.add_model(lig.desc_ptr)
-        e, _, _ = o.refine(m, o.initial_conf(m), 10000, accurate=True)
-        print(f"minimize ({approximation}): e {r['e']:.6f} rmsd {r['rmsd']:.3f}", flush=True)
+        e, _, _ = o.refine(m, o.initial_conf(m), it, v=(cap, cap, cap), accurate=not local)
+        print(f"minimize ({mode}): e {r['e']:.6f} rmsd {r['rmsd']:.3f}", flush=True)
         assert r["e"] == e, (r["e"], e)
 
 
