@@ -1242,7 +1242,7 @@ void GpuDocking::fp64_errors(DockJob* const* jobs, size_t nj, int batch, int cst
 		sq.gd = &gd_;
 		sq.slope = 10 * std::pow(10.0, tries);
 		for (int k = 0; k < 3; k++)
-			sq.v[k] = 1000;
+			sq.v[k] = s_.force_cap;
 		sq.mode = 2;
 		sq.nconfs = int(model.size());
 		sq.model = model.data();
@@ -1333,7 +1333,10 @@ void GpuDocking::dock_sub(DockJob* const* jobs, size_t nj, double budget)
 	mp.temperature = 1.2;
 	mp.mutation_amplitude = 2;
 	mp.min_rmsd = 1.0;
-	mp.num_saved_mins = 20;
+	// setup_search (src/main/main.cpp:513-529); the chains' second pass
+	// keeps monte_carlo's own cap of 1000 (src/lib/monte_carlo.cpp:96),
+	// whatever --force_cap says
+	mp.num_saved_mins = std::max(20, s_.num_modes);
 	for (int k = 0; k < 3; k++)
 	{
 		mp.hunt_cap[k] = 10;
@@ -1405,7 +1408,7 @@ void GpuDocking::dock_sub(DockJob* const* jobs, size_t nj, double budget)
 		rq.receptor_id = receptor_;
 		rq.gd = &gd_;
 		for (int k = 0; k < 3; k++)
-			rq.v[k] = 1000;
+			rq.v[k] = s_.force_cap;
 		rq.nposes = np;
 		rq.model = pose_model.data();
 		rq.conf = pose_conf.data();
@@ -1445,7 +1448,7 @@ void GpuDocking::dock_sub(DockJob* const* jobs, size_t nj, double budget)
 	}
 	if (!with_poses.empty())
 	{
-		const double v[3] = {1000, 1000, 1000};
+		const double v[3] = {s_.force_cap, s_.force_cap, s_.force_cap};
 		std::vector<double> intra(with_poses.size());
 		sgpu_score_request sq{};
 		sq.batch_id = batch;
@@ -1700,8 +1703,13 @@ sgpu_docking* sgpu_docking_create(const sgpu_dock_settings* s, const sgpu_mol* r
 		gs.precision = s->precision;
 		gs.exhaustiveness = s->exhaustiveness;
 		gs.seed = s->seed;
+		if (s->num_modes < 1 || s->num_modes > 64)
+			throw std::invalid_argument("num_modes must be in 1..64");
 		gs.num_modes = s->num_modes;
 		gs.energy_range = s->energy_range;
+		if (s->force_cap < 0)
+			throw std::invalid_argument("force_cap must be positive (0: 1000)");
+		gs.force_cap = s->force_cap > 0 ? s->force_cap : 1000;
 		gs.ligands_per_launch = s->ligands_per_launch;
 		if (!(s->step_fraction > 0) || s->step_fraction > 1)
 			throw std::invalid_argument("step_fraction must be in (0, 1]");

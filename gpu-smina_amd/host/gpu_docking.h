@@ -114,6 +114,10 @@ struct GpuSettings
 	int seed = 1;
 	int num_modes = 9;
 	double energy_range = 3.0;
+	// --force_cap: authentic_v of the refinement and the final scoring
+	// (main_procedure, src/main/main.cpp:392); the chains keep monte_carlo's
+	// own 1000 (src/lib/monte_carlo.cpp:96)
+	double force_cap = 1000;
 	int ligands_per_launch = 0; // 0: the whole batch in one launch
 	// --approximation: LINEAR or SPLINE (the default of --minimize,
 	// src/main/main.cpp:1225-1228); factor: points per A^2 (linear) or per

@@ -335,6 +335,8 @@ sgpu_server* sgpu_server_create(const sgpu_server_settings* s)
 			throw std::invalid_argument("precision must be SGPU_FP32, SGPU_FP64 or SGPU_MIXED");
 		ss.dock.precision = s->precision;
 		ss.dock.seed = s->seed;
+		if (s->num_modes < 1 || s->num_modes > 64)
+			throw std::invalid_argument("num_modes must be in 1..64");
 		ss.dock.num_modes = s->num_modes;
 		ss.dock.energy_range = s->energy_range;
 		ss.dock.progress_chunks = s->progress_chunks > 0 ? s->progress_chunks : 1;

@@ -173,6 +173,8 @@ typedef struct sgpu_dock_settings
 	                         * src/main/main.cpp:1225-1228) */
 	double approx_factor;   /* --factor (0: 32 for linear, 10 for spline as --minimize sets it) */
 	const sgpu_user_grid* user_grid; /* --user_grid (NULL: none); copied at creation */
+	double force_cap;       /* --force_cap: authentic_v of the refinement and the final scoring
+	                         * (0: 1000, smina's default); the chains keep monte_carlo's 1000 */
 } sgpu_dock_settings;
 
 sgpu_docking* sgpu_docking_create(const sgpu_dock_settings* s, const sgpu_mol* receptor,

@@ -2752,8 +2752,10 @@ double orc_refine_opts(orc_ctx* c, int model, const double* v, int maxiters, int
 
 int orc_dock(orc_ctx* c, int model, const sgpu_mc_params* p, int seed, int exhaustiveness, int num_modes,
 		double energy_range, int threads, double step_fraction, int split, int max_out, double* e, double* conf,
-		int64_t* evals)
+		int64_t* evals, double force_cap)
 {
+	const double cap[3] = {force_cap > 0 ? force_cap : p->authentic_v[0], force_cap > 0 ? force_cap : p->authentic_v[1],
+			force_cap > 0 ? force_cap : p->authentic_v[2]};
 	if (split < 1)
 	{
 		c->c.err = "split must be at least 1";
@@ -2880,18 +2882,18 @@ int orc_dock(orc_ctx* c, int model, const sgpu_mc_params* p, int seed, int exhau
 	{
 		// refine_structure of every pose (do_search, src/main/main.cpp:478-482)
 		for (Output& o : out)
-			refine_structure(c->c, m, o, p->authentic_v, maxiters, p->accurate_line_search != 0,
+			refine_structure(c->c, m, o, cap, maxiters, p->accurate_line_search != 0,
 					p->early_term != 0, &total);
 		// finish_docking (src/main/main.cpp:256-324)
 		const double slope = p->cache_slope;
 		if (!out.empty())
 		{
 			sort_container(out);
-			const double intra = c->c.eval_intramolecular(m, p->authentic_v, out[0].c.data());
+			const double intra = c->c.eval_intramolecular(m, cap, out[0].c.data());
 			for (Output& o : out)
 				if (not_max(o.e))
 				{
-					double ev = c->c.eval(m, slope, p->authentic_v, o.c.data());
+					double ev = c->c.eval(m, slope, cap, o.c.data());
 					o.e = c->c.sf.conf_independent(m.num_tors, ev - intra);
 				}
 			sort_container(out);

@@ -605,6 +605,28 @@ def test_dock_fp64_matches_oracle(lig, exh, frac, split, qv):
     assert len(r["e"]) > 0 and r["e"] == list(e)
 
 
+def test_dock_fp64_force_cap_and_modes_match_oracle():
+    """--force_cap and --num_modes through the host facade: the cap is
+    main_procedure's authentic_v (src/main/main.cpp:392) of the refinement
+    and the final scoring, while the chains keep monte_carlo's own 1000
+    (src/lib/monte_carlo.cpp:96); setup_search keeps max(20, num_modes)
+    minima per chain (:524).  FP64 at a tenth of the steps, against the
+    oracle's dock with the same settings"""
+    rec, l = receptor(), sg.Mol.ligand(1, 24, 6)
+    d = sg.Docking(rec, gd_c0(), fp64=True, exhaustiveness=4, seed=7, step_fraction=0.1, num_modes=25,
+                   energy_range=20.0, force_cap=50.0)
+    r = d.dock([l])[0]
+    o, ids, _, _ = oracle_for([l])
+    e, c, ev = o.dock(ids[0], sg.mc_params(num_saved=25), seed=7, exhaustiveness=4, num_modes=25,
+                      energy_range=20.0, threads=8, step_fraction=0.1, force_cap=50.0)
+    print(f"dock with force cap 50 and 25 modes: {len(r['e'])} modes, {r['e'][:3]}", flush=True)
+    assert len(r["e"]) > 9 and r["e"] == list(e)
+    # the default cap differs: the setting reaches the refinement and scoring
+    e1000 = o.dock(ids[0], sg.mc_params(num_saved=25), seed=7, exhaustiveness=4, num_modes=25, energy_range=20.0,
+                   threads=8, step_fraction=0.1)[0]
+    assert list(e1000) != list(e)
+
+
 @pytest.mark.parametrize("lig,exh", [((1, 24, 6), 4), ((3, 10, 0), 6)])
 def test_dock_fp64_early_stop_matches_oracle(lig, exh):
     """--gpu_early_stop with --gpu_split 2 and a tenth of the steps through
