@@ -1351,7 +1351,7 @@ void GpuDocking::dock_sub(DockJob* const* jobs, size_t nj, double budget)
 		mp.early_stop_fraction = 0.5;
 		mp.early_stop_eps = 0.05;
 	}
-	const int maxp = 20;
+	const int maxp = mp.num_saved_mins;
 	std::vector<int> cnt(nchains);
 	std::vector<double> oe(size_t(nchains) * maxp), oc(size_t(nchains) * maxp * cstride),
 			ox(size_t(nchains) * maxp * hstride);
