@@ -772,9 +772,10 @@ void make_chain(const FlatModel& m, uint32_t seed, const sgpu_grid_dims& gd, dou
 	}
 }
 
-SearchParams setup_search(const FlatModel& m, int exhaustiveness)
+SearchParams setup_search(const FlatModel& m, int exhaustiveness, int num_modes)
 {
 	SearchParams p;
+	p.num_saved_mins = std::max(20, num_modes);
 	int dof = 0;
 	for (size_t t = 0; t < m.tree_ligand.size(); t++)
 		dof += m.tree_ntors[t] + (m.tree_ligand[t] ? 6 : 0);
@@ -1307,7 +1308,7 @@ void GpuDocking::dock_sub(DockJob* const* jobs, size_t nj, double budget)
 	for (size_t j = 0; j < nj; j++)
 	{
 		const int exh = jobs[j]->exhaustiveness > 0 ? jobs[j]->exhaustiveness : s_.exhaustiveness;
-		sp.push_back(setup_search(fms[j], exh));
+		sp.push_back(setup_search(fms[j], exh, s_.num_modes));
 		Gen g(uint32_t(s_.seed));
 		first[j] = cmodel.size();
 		const int steps = std::max(1, int(std::ceil(double(sp[j].num_steps) * fraction / double(split))));

@@ -67,7 +67,7 @@ struct SearchParams
 	double min_rmsd = 1.0;
 	int num_tasks = 8;
 };
-SearchParams setup_search(const FlatModel& m, int exhaustiveness);
+SearchParams setup_search(const FlatModel& m, int exhaustiveness, int num_modes = 9);
 
 // scoring tables: precalculate_linear(sf, factor) (src/lib/precalculate.h:206-237)
 struct Tables
