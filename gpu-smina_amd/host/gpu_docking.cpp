@@ -1344,6 +1344,7 @@ void GpuDocking::dock_sub(DockJob* const* jobs, size_t nj, double budget)
 		mp.authentic_v[k] = 1000;
 	}
 	mp.cache_slope = 1e6;
+	mp.accurate_line_search = s_.accurate_line ? 1 : 0;
 	mp.visited = s_.skip_visited ? 1 : 0;
 	if (plan.checkpoints > 0)
 	{
@@ -1410,6 +1411,7 @@ void GpuDocking::dock_sub(DockJob* const* jobs, size_t nj, double budget)
 		rq.gd = &gd_;
 		for (int k = 0; k < 3; k++)
 			rq.v[k] = s_.force_cap;
+		rq.accurate_line_search = s_.accurate_line ? 1 : 0;
 		rq.nposes = np;
 		rq.model = pose_model.data();
 		rq.conf = pose_conf.data();
@@ -1501,10 +1503,10 @@ void GpuDocking::dock_sub(DockJob* const* jobs, size_t nj, double budget)
 	{
 		auto& out = jobs[j]->out;
 		std::stable_sort(out.begin(), out.end(), [](const Pose& a, const Pose& b) { return a.e < b.e; });
-		// remove_redundant with out_min_rmsd 1 (src/main/main.cpp:204-210)
+		// remove_redundant with --min_rmsd_filter (src/main/main.cpp:204-210, :276)
 		std::vector<Pose> tmp;
 		for (const Pose& p : out)
-			add_to_output_container(tmp, p, 1.0, out.size());
+			add_to_output_container(tmp, p, s_.out_min_rmsd, out.size());
 		// the reported modes: num_modes within energy_range of the best
 		std::vector<Pose> fin;
 		for (size_t i = 0; i < tmp.size(); i++)
@@ -1711,6 +1713,10 @@ sgpu_docking* sgpu_docking_create(const sgpu_dock_settings* s, const sgpu_mol* r
 		if (s->force_cap < 0)
 			throw std::invalid_argument("force_cap must be positive (0: 1000)");
 		gs.force_cap = s->force_cap > 0 ? s->force_cap : 1000;
+		gs.accurate_line = s->accurate_line != 0;
+		if (s->min_rmsd_filter < 0)
+			throw std::invalid_argument("min_rmsd_filter must be positive (0: 1)");
+		gs.out_min_rmsd = s->min_rmsd_filter > 0 ? s->min_rmsd_filter : 1;
 		gs.ligands_per_launch = s->ligands_per_launch;
 		if (!(s->step_fraction > 0) || s->step_fraction > 1)
 			throw std::invalid_argument("step_fraction must be in (0, 1]");

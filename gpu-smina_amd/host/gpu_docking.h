@@ -118,6 +118,11 @@ struct GpuSettings
 	// (main_procedure, src/main/main.cpp:392); the chains keep monte_carlo's
 	// own 1000 (src/lib/monte_carlo.cpp:96)
 	double force_cap = 1000;
+	// --accurate_line: minimization_params::BFGSAccurateLineSearch for
+	// every BFGS, the chains' and the refinement's (src/main/main.cpp:1231-1234)
+	bool accurate_line = false;
+	// --min_rmsd_filter: remove_redundant's RMSD (src/main/main.cpp:276)
+	double out_min_rmsd = 1;
 	int ligands_per_launch = 0; // 0: the whole batch in one launch
 	// --approximation: LINEAR or SPLINE (the default of --minimize,
 	// src/main/main.cpp:1225-1228); factor: points per A^2 (linear) or per

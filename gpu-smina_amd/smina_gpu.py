@@ -136,7 +136,8 @@ class DockSettings(C.Structure):
                 ("progress", PROGRESS_FN), ("progress_user", C.c_void_p), ("scoring", C.c_char_p),
                 ("chain_split", C.c_int), ("early_stop", C.c_int), ("fast", C.c_int), ("skip_visited", C.c_int),
                 ("approximation", C.c_int), ("approx_factor", C.c_double),
-                ("user_grid", C.c_void_p), ("force_cap", C.c_double)]
+                ("user_grid", C.c_void_p), ("force_cap", C.c_double), ("accurate_line", C.c_int),
+                ("min_rmsd_filter", C.c_double)]
 
 
 class UserGrid(C.Structure):
@@ -784,7 +785,8 @@ class Docking:
     def __init__(self, receptor, gd, fp64=False, exhaustiveness=8, seed=1, num_modes=9, energy_range=3.0,
                  hydrophobic_scale=1.0, ligands_per_launch=0, device=0, step_fraction=1.0, precision=None,
                  progress=None, progress_chunks=1, scoring="vina", chain_split=0, early_stop=False, fast=False,
-                 skip_visited=False, approximation="linear", approx_factor=0.0, user_grid=None, force_cap=1000.0):
+                 skip_visited=False, approximation="linear", approx_factor=0.0, user_grid=None, force_cap=1000.0,
+                 accurate_line=False, min_rmsd_filter=1.0):
         """precision: FP32, FP64 or MIXED (default: FP64 if fp64 else FP32);
         progress(fraction): called after each of progress_chunks launches
         of the Monte Carlo search; chain_split, early_stop, skip_visited,
@@ -793,7 +795,9 @@ class Docking:
         default, with factor 10 unless approx_factor is given); user_grid:
         (GridDims, values) of --user_grid (read_user_grid); force_cap:
         --force_cap (authentic_v of the refinement and the final scoring;
-        the chains keep monte_carlo's 1000)"""
+        the chains keep monte_carlo's 1000); accurate_line: --accurate_line
+        (every BFGS with the accurate line search); min_rmsd_filter:
+        --min_rmsd_filter"""
         prec = precision if precision is not None else (FP64 if fp64 else FP32)
         # keep the callback alive as long as the docking object
         self._progress = PROGRESS_FN(lambda user, frac: progress(frac)) if progress else PROGRESS_FN()
@@ -802,6 +806,8 @@ class Docking:
                          scoring.encode(), chain_split, int(early_stop), int(fast), int(skip_visited),
                          {"linear": 0, "spline": 1}[approximation], approx_factor)
         s.force_cap = force_cap
+        s.accurate_line = int(accurate_line)
+        s.min_rmsd_filter = min_rmsd_filter
         ug = None
         if user_grid is not None:
             ugd, vals = user_grid

@@ -175,6 +175,9 @@ typedef struct sgpu_dock_settings
 	const sgpu_user_grid* user_grid; /* --user_grid (NULL: none); copied at creation */
 	double force_cap;       /* --force_cap: authentic_v of the refinement and the final scoring
 	                         * (0: 1000, smina's default); the chains keep monte_carlo's 1000 */
+	int accurate_line;      /* --accurate_line: every BFGS (chains and refinement) with the
+	                         * accurate line search */
+	double min_rmsd_filter; /* --min_rmsd_filter: RMSD below which a worse mode is dropped (0: 1) */
 } sgpu_dock_settings;
 
 sgpu_docking* sgpu_docking_create(const sgpu_dock_settings* s, const sgpu_mol* receptor,

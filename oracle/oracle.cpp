@@ -2752,7 +2752,7 @@ double orc_refine_opts(orc_ctx* c, int model, const double* v, int maxiters, int
 
 int orc_dock(orc_ctx* c, int model, const sgpu_mc_params* p, int seed, int exhaustiveness, int num_modes,
 		double energy_range, int threads, double step_fraction, int split, int max_out, double* e, double* conf,
-		int64_t* evals, double force_cap)
+		int64_t* evals, double force_cap, double out_min_rmsd)
 {
 	const double cap[3] = {force_cap > 0 ? force_cap : p->authentic_v[0], force_cap > 0 ? force_cap : p->authentic_v[1],
 			force_cap > 0 ? force_cap : p->authentic_v[2]};
@@ -2901,7 +2901,7 @@ int orc_dock(orc_ctx* c, int model, const sgpu_mc_params* p, int seed, int exhau
 		// remove_redundant (src/main/main.cpp:204-210), out_min_rmsd 1
 		std::vector<Output> tmp;
 		for (const Output& o : out)
-			add_to_output_container(tmp, o, 1.0, out.size());
+			add_to_output_container(tmp, o, out_min_rmsd > 0 ? out_min_rmsd : 1.0, out.size());
 		out.swap(tmp);
 	}
 	catch (std::exception& ex)

@@ -627,6 +627,27 @@ def test_dock_fp64_force_cap_and_modes_match_oracle():
     assert list(e1000) != list(e)
 
 
+def test_dock_fp64_accurate_line_and_rmsd_filter_match_oracle():
+    """--accurate_line (every BFGS, the chains' and the refinement's, with
+    the accurate line search, src/main/main.cpp:1231-1234) and
+    --min_rmsd_filter (remove_redundant's RMSD, :276) through the host
+    facade: FP64 at a twentieth of the steps against the oracle's dock"""
+    rec, l = receptor(), sg.Mol.ligand(2, 20, 4)
+    d = sg.Docking(rec, gd_c0(), fp64=True, exhaustiveness=4, seed=11, step_fraction=0.05, num_modes=20,
+                   energy_range=20.0, accurate_line=True, min_rmsd_filter=2.5)
+    r = d.dock([l])[0]
+    o, ids, _, _ = oracle_for([l])
+    p = sg.mc_params()
+    p.accurate_line_search = 1
+    e, c, ev = o.dock(ids[0], p, seed=11, exhaustiveness=4, num_modes=20, energy_range=20.0, threads=8,
+                      step_fraction=0.05, min_rmsd_filter=2.5)
+    print(f"dock with the accurate line search, modes 2.5 A apart: {len(r['e'])} modes, {r['e'][:3]}", flush=True)
+    assert len(r["e"]) > 1 and r["e"] == list(e)
+    e1 = o.dock(ids[0], p, seed=11, exhaustiveness=4, num_modes=20, energy_range=20.0, threads=8,
+                step_fraction=0.05)[0]
+    assert len(e1) > len(e)  # the filter reaches the mode selection
+
+
 @pytest.mark.parametrize("lig,exh", [((1, 24, 6), 4), ((3, 10, 0), 6)])
 def test_dock_fp64_early_stop_matches_oracle(lig, exh):
     """--gpu_early_stop with --gpu_split 2 and a tenth of the steps through
