@@ -772,7 +772,7 @@ void make_chain(const FlatModel& m, uint32_t seed, const sgpu_grid_dims& gd, dou
 	}
 }
 
-SearchParams setup_search(const FlatModel& m, int exhaustiveness, int num_modes)
+SearchParams setup_search(const FlatModel& m, int exhaustiveness, int num_modes, int minimize_iters)
 {
 	SearchParams p;
 	p.num_saved_mins = std::max(20, num_modes);
@@ -781,7 +781,7 @@ SearchParams setup_search(const FlatModel& m, int exhaustiveness, int num_modes)
 		dof += m.tree_ntors[t] + (m.tree_ligand[t] ? 6 : 0);
 	const int heuristic = m.view.natoms + 10 * dof;
 	p.num_steps = int(unsigned(70 * 3 * (50 + heuristic) / 2));
-	p.maxiters = (25 + m.view.natoms) / 3;
+	p.maxiters = minimize_iters > 0 ? minimize_iters : (25 + m.view.natoms) / 3;
 	p.num_tasks = exhaustiveness;
 	return p;
 }
@@ -1308,7 +1308,7 @@ void GpuDocking::dock_sub(DockJob* const* jobs, size_t nj, double budget)
 	for (size_t j = 0; j < nj; j++)
 	{
 		const int exh = jobs[j]->exhaustiveness > 0 ? jobs[j]->exhaustiveness : s_.exhaustiveness;
-		sp.push_back(setup_search(fms[j], exh, s_.num_modes));
+		sp.push_back(setup_search(fms[j], exh, s_.num_modes, s_.minimize_iters));
 		Gen g(uint32_t(s_.seed));
 		first[j] = cmodel.size();
 		const int steps = std::max(1, int(std::ceil(double(sp[j].num_steps) * fraction / double(split))));
@@ -1345,6 +1345,7 @@ void GpuDocking::dock_sub(DockJob* const* jobs, size_t nj, double budget)
 	}
 	mp.cache_slope = 1e6;
 	mp.accurate_line_search = s_.accurate_line ? 1 : 0;
+	mp.early_term = s_.early_term ? 1 : 0;
 	mp.visited = s_.skip_visited ? 1 : 0;
 	if (plan.checkpoints > 0)
 	{
@@ -1412,6 +1413,7 @@ void GpuDocking::dock_sub(DockJob* const* jobs, size_t nj, double budget)
 		for (int k = 0; k < 3; k++)
 			rq.v[k] = s_.force_cap;
 		rq.accurate_line_search = s_.accurate_line ? 1 : 0;
+		rq.early_term = s_.early_term ? 1 : 0;
 		rq.nposes = np;
 		rq.model = pose_model.data();
 		rq.conf = pose_conf.data();
@@ -1717,6 +1719,10 @@ sgpu_docking* sgpu_docking_create(const sgpu_dock_settings* s, const sgpu_mol* r
 		if (s->min_rmsd_filter < 0)
 			throw std::invalid_argument("min_rmsd_filter must be positive (0: 1)");
 		gs.out_min_rmsd = s->min_rmsd_filter > 0 ? s->min_rmsd_filter : 1;
+		if (s->minimize_iters < 0)
+			throw std::invalid_argument("minimize_iters must be positive (0: default)");
+		gs.minimize_iters = s->minimize_iters;
+		gs.early_term = s->minimize_early_term != 0;
 		gs.ligands_per_launch = s->ligands_per_launch;
 		if (!(s->step_fraction > 0) || s->step_fraction > 1)
 			throw std::invalid_argument("step_fraction must be in (0, 1]");

@@ -67,7 +67,7 @@ struct SearchParams
 	double min_rmsd = 1.0;
 	int num_tasks = 8;
 };
-SearchParams setup_search(const FlatModel& m, int exhaustiveness, int num_modes = 9);
+SearchParams setup_search(const FlatModel& m, int exhaustiveness, int num_modes = 9, int minimize_iters = 0);
 
 // scoring tables: precalculate_linear(sf, factor) (src/lib/precalculate.h:206-237)
 struct Tables
@@ -123,6 +123,11 @@ struct GpuSettings
 	bool accurate_line = false;
 	// --min_rmsd_filter: remove_redundant's RMSD (src/main/main.cpp:276)
 	double out_min_rmsd = 1;
+	// --minimize_iters (0: setup_search's (25 + movable atoms) / 3) and
+	// --minimize_early_term: minimization_params of every BFGS, the chains'
+	// and the refinement's (src/main/main.cpp:513-522, :1059-1066)
+	int minimize_iters = 0;
+	bool early_term = false;
 	int ligands_per_launch = 0; // 0: the whole batch in one launch
 	// --approximation: LINEAR or SPLINE (the default of --minimize,
 	// src/main/main.cpp:1225-1228); factor: points per A^2 (linear) or per

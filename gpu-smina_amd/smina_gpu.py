@@ -137,7 +137,7 @@ class DockSettings(C.Structure):
                 ("chain_split", C.c_int), ("early_stop", C.c_int), ("fast", C.c_int), ("skip_visited", C.c_int),
                 ("approximation", C.c_int), ("approx_factor", C.c_double),
                 ("user_grid", C.c_void_p), ("force_cap", C.c_double), ("accurate_line", C.c_int),
-                ("min_rmsd_filter", C.c_double)]
+                ("min_rmsd_filter", C.c_double), ("minimize_iters", C.c_int), ("minimize_early_term", C.c_int)]
 
 
 class UserGrid(C.Structure):
@@ -786,7 +786,7 @@ class Docking:
                  hydrophobic_scale=1.0, ligands_per_launch=0, device=0, step_fraction=1.0, precision=None,
                  progress=None, progress_chunks=1, scoring="vina", chain_split=0, early_stop=False, fast=False,
                  skip_visited=False, approximation="linear", approx_factor=0.0, user_grid=None, force_cap=1000.0,
-                 accurate_line=False, min_rmsd_filter=1.0):
+                 accurate_line=False, min_rmsd_filter=1.0, minimize_iters=0, minimize_early_term=False):
         """precision: FP32, FP64 or MIXED (default: FP64 if fp64 else FP32);
         progress(fraction): called after each of progress_chunks launches
         of the Monte Carlo search; chain_split, early_stop, skip_visited,
@@ -797,7 +797,8 @@ class Docking:
         --force_cap (authentic_v of the refinement and the final scoring;
         the chains keep monte_carlo's 1000); accurate_line: --accurate_line
         (every BFGS with the accurate line search); min_rmsd_filter:
-        --min_rmsd_filter"""
+        --min_rmsd_filter; minimize_iters, minimize_early_term: every BFGS's
+        iterations (0: (25 + movable atoms) / 3) and early termination"""
         prec = precision if precision is not None else (FP64 if fp64 else FP32)
         # keep the callback alive as long as the docking object
         self._progress = PROGRESS_FN(lambda user, frac: progress(frac)) if progress else PROGRESS_FN()
@@ -808,6 +809,8 @@ class Docking:
         s.force_cap = force_cap
         s.accurate_line = int(accurate_line)
         s.min_rmsd_filter = min_rmsd_filter
+        s.minimize_iters = minimize_iters
+        s.minimize_early_term = int(minimize_early_term)
         ug = None
         if user_grid is not None:
             ugd, vals = user_grid

@@ -178,6 +178,10 @@ typedef struct sgpu_dock_settings
 	int accurate_line;      /* --accurate_line: every BFGS (chains and refinement) with the
 	                         * accurate line search */
 	double min_rmsd_filter; /* --min_rmsd_filter: RMSD below which a worse mode is dropped (0: 1) */
+	int minimize_iters;     /* --minimize_iters: every BFGS's iterations (0: setup_search's
+	                         * (25 + movable atoms) / 3) */
+	int minimize_early_term; /* --minimize_early_term: every BFGS stops once an iteration gains
+	                          * less than 1e-5 */
 } sgpu_dock_settings;
 
 sgpu_docking* sgpu_docking_create(const sgpu_dock_settings* s, const sgpu_mol* receptor,

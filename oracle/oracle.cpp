@@ -2752,7 +2752,7 @@ double orc_refine_opts(orc_ctx* c, int model, const double* v, int maxiters, int
 
 int orc_dock(orc_ctx* c, int model, const sgpu_mc_params* p, int seed, int exhaustiveness, int num_modes,
 		double energy_range, int threads, double step_fraction, int split, int max_out, double* e, double* conf,
-		int64_t* evals, double force_cap, double out_min_rmsd)
+		int64_t* evals, double force_cap, double out_min_rmsd, int minimize_iters)
 {
 	const double cap[3] = {force_cap > 0 ? force_cap : p->authentic_v[0], force_cap > 0 ? force_cap : p->authentic_v[1],
 			force_cap > 0 ? force_cap : p->authentic_v[2]};
@@ -2776,7 +2776,7 @@ int orc_dock(orc_ctx* c, int model, const sgpu_mc_params* p, int seed, int exhau
 	const int num_steps = step_fraction >= 1 && split == 1
 	                              ? full_steps
 	                              : std::max(1, int(std::ceil(double(full_steps) * step_fraction / double(split))));
-	const int maxiters = (25 + N) / 3;
+	const int maxiters = minimize_iters > 0 ? minimize_iters : (25 + N) / 3;
 	// parallel_mc::operator() (src/lib/parallel_mc.cpp:80-98)
 	const int tasks = exhaustiveness * split;
 	rng generator{uint32_t(seed)};

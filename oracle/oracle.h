@@ -126,11 +126,12 @@ double orc_refine_opts(orc_ctx* c, int model, const double* v, int maxiters, int
  * use p->authentic_v (monte_carlo's own 1000, src/lib/monte_carlo.cpp:96);
  * the refinement and the final scoring use force_cap (main_procedure's
  * authentic_v, src/main/main.cpp:392; <= 0: p->authentic_v);
- * remove_redundant keeps modes out_min_rmsd apart (<= 0: 1).  Results
+ * remove_redundant keeps modes out_min_rmsd apart (<= 0: 1); every BFGS
+ * runs minimize_iters iterations (0: (25 + movable atoms) / 3).  Results
  * sorted: count, energies, confs. */
 int orc_dock(orc_ctx* c, int model, const sgpu_mc_params* p, int seed, int exhaustiveness,
 		int num_modes, double energy_range, int threads, double step_fraction, int split, int max_out, double* e,
-		double* conf, int64_t* evals, double force_cap, double out_min_rmsd);
+		double* conf, int64_t* evals, double force_cap, double out_min_rmsd, int minimize_iters);
 
 #ifdef __cplusplus
 }

@@ -63,7 +63,7 @@ def _load():
         "orc_refine": (C.c_double, [vp, C.c_int, F64P, C.c_int, F64P, F64P]),
         "orc_refine_opts": (C.c_double, [vp, C.c_int, F64P, C.c_int, C.c_int, F64P, F64P]),
         "orc_dock": (C.c_int, [vp, C.c_int, vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double,
-                               C.c_int, C.c_int, F64P, F64P, I64P, C.c_double, C.c_double]),
+                               C.c_int, C.c_int, F64P, F64P, I64P, C.c_double, C.c_double, C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -296,16 +296,17 @@ class Oracle:
         return e, conf, x
 
     def dock(self, m, params, seed=1, exhaustiveness=8, num_modes=9, energy_range=3.0, threads=1, max_out=32,
-             step_fraction=1.0, split=1, force_cap=0.0, min_rmsd_filter=1.0):
+             step_fraction=1.0, split=1, force_cap=0.0, min_rmsd_filter=1.0, minimize_iters=0):
         """force_cap: the refinement's and final scoring's cap (0: the
-        chains' params.authentic_v); min_rmsd_filter: remove_redundant's"""
+        chains' params.authentic_v); min_rmsd_filter: remove_redundant's;
+        minimize_iters: every BFGS's iterations (0: (25 + N) / 3)"""
         info = self.info(m)
         e = np.zeros(max_out)
         c = np.zeros((max_out, info["conf_size"]))
         ev = np.zeros(1, np.int64)
         k = lib.orc_dock(self.h, m, C.addressof(params), seed, exhaustiveness, num_modes, energy_range, threads,
                          step_fraction, split, max_out, _p(e, F64P), _p(c, F64P), _p(ev, I64P), force_cap,
-                         min_rmsd_filter)
+                         min_rmsd_filter, minimize_iters)
         if k < 0:
             raise RuntimeError(self.error())
         return e[:k], c[:k], int(ev[0])

@@ -627,6 +627,25 @@ def test_dock_fp64_force_cap_and_modes_match_oracle():
     assert list(e1000) != list(e)
 
 
+def test_dock_fp64_minimize_settings_match_oracle():
+    """--minimize_iters and --minimize_early_term (every BFGS's iterations
+    and early termination, the chains' and the refinement's,
+    src/main/main.cpp:513-522, :1059-1066) through the host facade: FP64 at
+    a tenth of the steps against the oracle's dock"""
+    rec, l = receptor(), sg.Mol.ligand(3, 22, 5)
+    d = sg.Docking(rec, gd_c0(), fp64=True, exhaustiveness=4, seed=13, step_fraction=0.1, minimize_iters=7,
+                   minimize_early_term=True)
+    r = d.dock([l])[0]
+    o, ids, _, _ = oracle_for([l])
+    p = sg.mc_params()
+    p.early_term = 1
+    e, c, ev = o.dock(ids[0], p, seed=13, exhaustiveness=4, threads=8, step_fraction=0.1, minimize_iters=7)
+    print(f"dock with 7 iterations and early termination: {r['e'][:3]}", flush=True)
+    assert len(r["e"]) > 0 and r["e"] == list(e)
+    e0 = o.dock(ids[0], sg.mc_params(), seed=13, exhaustiveness=4, threads=8, step_fraction=0.1)[0]
+    assert list(e0) != list(e)
+
+
 def test_dock_fp64_accurate_line_and_rmsd_filter_match_oracle():
     """--accurate_line (every BFGS, the chains' and the refinement's, with
     the accurate line search, src/main/main.cpp:1231-1234) and
