@@ -9,6 +9,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <sstream>
 #include <cstring>
 #include <functional>
@@ -1544,6 +1546,126 @@ struct sgpu_docking
 	std::vector<sgpu_ci_term> ci;
 };
 
+// custom_terms::add (src/lib/custom_terms.cpp:17-37) over the creators of
+// src/lib/everything.h:1006-1037: a term's name is its parameterized form,
+// "name(k1=v1,_k2=v2,..)" with the keys in a fixed order, or a bare name
+// for the conf-independent terms.
+namespace sgpu_host
+{
+namespace
+{
+struct TermForm
+{
+	const char* name;
+	int kind;
+	std::vector<std::string> keys;
+	bool charge_dependent;
+};
+
+const std::vector<TermForm>& term_forms()
+{
+	static const std::vector<TermForm> f = {
+			{"electrostatic", SGPU_TERM_ELECTROSTATIC, {"i", "^", "c"}, true},
+			{"ad4_solvation", SGPU_TERM_AD4_SOLVATION, {"d-sigma", "s/q", "c"}, true},
+			{"gauss", SGPU_TERM_GAUSS, {"o", "w", "c"}, false},
+			{"repulsion", SGPU_TERM_REPULSION, {"o", "c"}, false},
+			{"hydrophobic", SGPU_TERM_HYDROPHOBIC, {"g", "b", "c"}, false},
+			{"non_hydrophobic", SGPU_TERM_NON_HYDROPHOBIC, {"g", "b", "c"}, false},
+			{"vdw", SGPU_TERM_VDW, {"i", "j", "s", "^", "c"}, false},
+			{"non_dir_h_bond_lj", SGPU_TERM_NON_DIR_H_BOND_LJ, {"o", "^", "c"}, false},
+			{"non_dir_anti_h_bond_quadratic", SGPU_TERM_NON_DIR_ANTI_H_BOND_QUADRATIC, {"o", "c"}, false},
+			{"non_dir_h_bond", SGPU_TERM_NON_DIR_H_BOND, {"g", "b", "c"}, false},
+			{"acceptor_acceptor_quadratic", SGPU_TERM_ACCEPTOR_ACCEPTOR_QUADRATIC, {"o", "c"}, false},
+			{"donor_donor_quadratic", SGPU_TERM_DONOR_DONOR_QUADRATIC, {"o", "c"}, false},
+	};
+	return f;
+}
+
+// boost::lexical_cast<fl>: the whole string is one number
+double whole_number(const std::string& v, const std::string& term)
+{
+	char* end = nullptr;
+	const double x = v.empty() ? 0 : std::strtod(v.c_str(), &end);
+	if (v.empty() || end != v.c_str() + v.size())
+		throw std::invalid_argument(term + ": could not convert parameters");
+	return x;
+}
+} // namespace
+
+void parse_custom_scoring(const std::string& text, double scale, std::vector<sgpu_term>& ind,
+		std::vector<sgpu_term>& dep, std::vector<sgpu_ci_term>& ci)
+{
+	static const std::vector<std::pair<std::string, int>> ci_forms = {{"num_tors_add", SGPU_CI_NUM_TORS_ADD},
+			{"num_tors_sqr", SGPU_CI_NUM_TORS_SQR}, {"num_tors_sqrt", SGPU_CI_NUM_TORS_SQRT},
+			{"num_tors_div", SGPU_CI_NUM_TORS_DIV}, {"num_tors_div_simple", SGPU_CI_NUM_TORS_DIV_SIMPLE},
+			{"constant_term", SGPU_CI_CONSTANT}};
+	static const char* refused[] = {"atom_type_gaussian", "atom_type_linear", "atom_type_quadratic",
+			"atom_type_inverse_power", "ligand_length", "num_ligands", "num_heavy_atoms_div", "num_heavy_atoms",
+			"num_hydrophobic_atoms"};
+	std::istringstream in(text);
+	std::string line;
+	int lineno = 0;
+	while (std::getline(in, line))
+	{
+		++lineno;
+		if (line.empty() || line[0] == '#')
+			continue;
+		std::istringstream ls(line);
+		double w = 0;
+		std::string name;
+		if (!(ls >> w >> name) || name.empty())
+			continue;
+		const std::string where = "line " + std::to_string(lineno) + ": " + name;
+		bool done = false;
+		for (const auto& c : ci_forms)
+			if (name == c.first)
+			{
+				ci.push_back({c.second, w * scale});
+				done = true;
+				break;
+			}
+		if (done)
+			continue;
+		const size_t open = name.find('(');
+		const std::string base = name.substr(0, open);
+		for (const char* r : refused)
+			if (base == r)
+				throw std::invalid_argument(where + ": a term this build does not evaluate");
+		const TermForm* form = nullptr;
+		for (const TermForm& f : term_forms())
+			if (base == f.name)
+				form = &f;
+		if (!form || open == std::string::npos || name.back() != ')')
+			throw std::invalid_argument(where + ": unknown term");
+		// "k1=v1,_k2=v2,..": each value runs to the next ",_<key>=" or the end
+		const std::string inner = name.substr(open + 1, name.size() - open - 2);
+		sgpu_term t{form->kind, {0, 0, 0, 0, 0}, w * scale};
+		size_t pos = 0;
+		for (size_t k = 0; k < form->keys.size(); k++)
+		{
+			const std::string lead = (k == 0 ? "" : ",_") + form->keys[k] + "=";
+			if (inner.compare(pos, lead.size(), lead) != 0)
+				throw std::invalid_argument(where + ": unknown term");
+			pos += lead.size();
+			size_t end = inner.size();
+			if (k + 1 < form->keys.size())
+			{
+				end = inner.find(",_" + form->keys[k + 1] + "=", pos);
+				if (end == std::string::npos)
+					throw std::invalid_argument(where + ": unknown term");
+			}
+			t.p[k] = whole_number(inner.substr(pos, end - pos), where);
+			pos = end;
+		}
+		if (form->kind == SGPU_TERM_ELECTROSTATIC && t.p[0] != 1 && t.p[0] != 2)
+			throw std::invalid_argument(where + ": invalid exponent (1 or 2 only)");
+		if (form->kind == SGPU_TERM_VDW && !((t.p[0] == 4 && t.p[1] == 8) || (t.p[0] == 6 && t.p[1] == 12)))
+			throw std::invalid_argument(where + ": unsupported LJ exponents (4,8 or 6,12)");
+		(form->charge_dependent ? dep : ind).push_back(t);
+	}
+}
+} // namespace sgpu_host
+
 extern "C" {
 
 int sgpu_builtin_scoring(const char* name, double hydrophobic_scale, sgpu_term* t, sgpu_ci_term* ci,
@@ -1607,6 +1729,40 @@ int sgpu_builtin_scoring(const char* name, double hydrophobic_scale, sgpu_term* 
 	out->nci = c;
 	out->ci = ci;
 	return 0;
+}
+
+int sgpu_parse_custom_scoring(const char* text, double scale, sgpu_term* terms, int max_terms, sgpu_ci_term* ci,
+		int max_ci, sgpu_scoring* out, char* err, int err_len)
+{
+	try
+	{
+		if (!text || !terms || !ci || !out)
+			throw std::invalid_argument("custom scoring: missing input");
+		std::vector<sgpu_term> ind, dep;
+		std::vector<sgpu_ci_term> cis;
+		parse_custom_scoring(text, scale, ind, dep, cis);
+		if (int(ind.size() + dep.size()) > max_terms || int(cis.size()) > max_ci)
+			throw std::invalid_argument("custom scoring: more terms than the arrays hold");
+		int k = 0;
+		for (const sgpu_term& t : ind)
+			terms[k++] = t;
+		for (const sgpu_term& t : dep)
+			terms[k++] = t;
+		for (size_t i = 0; i < cis.size(); i++)
+			ci[i] = cis[i];
+		out->nterms = k;
+		out->terms = terms;
+		out->nci = int(cis.size());
+		out->ci = ci;
+		out->atom_params = SGPU_ATOM_PARAMS_DEFAULT;
+		return 0;
+	}
+	catch (std::exception& e)
+	{
+		if (err && err_len > 0)
+			std::snprintf(err, size_t(err_len), "%s", e.what());
+		return -1;
+	}
 }
 
 double sgpu_conf_independent(const sgpu_scoring* sf, double num_tors, double e)
@@ -1751,12 +1907,31 @@ sgpu_docking* sgpu_docking_create(const sgpu_dock_settings* s, const sgpu_mol* r
 		gs.progress = s->progress;
 		gs.progress_user = s->progress_user;
 		gs.user_grid = s->user_grid;
-		d->terms.resize(SGPU_MAX_TERMS);
-		d->ci.resize(SGPU_MAX_CI);
 		sgpu_scoring sf;
-		const int rc = sgpu_builtin_scoring(s->scoring, s->hydrophobic_scale, d->terms.data(), d->ci.data(), &sf);
-		if (rc != 0)
-			throw std::invalid_argument("unknown scoring function");
+		if (s->custom_scoring)
+		{
+			std::vector<sgpu_term> ind, dep;
+			std::vector<sgpu_ci_term> cis;
+			parse_custom_scoring(s->custom_scoring, 1.0, ind, dep, cis);
+			d->terms = ind;
+			d->terms.insert(d->terms.end(), dep.begin(), dep.end());
+			d->ci = cis;
+			if (d->terms.empty())
+				throw std::invalid_argument("custom scoring: no pairwise terms");
+			sf.nterms = int(d->terms.size());
+			sf.terms = d->terms.data();
+			sf.nci = int(d->ci.size());
+			sf.ci = d->ci.data();
+			sf.atom_params = SGPU_ATOM_PARAMS_DEFAULT;
+		}
+		else
+		{
+			d->terms.resize(SGPU_MAX_TERMS);
+			d->ci.resize(SGPU_MAX_CI);
+			const int rc = sgpu_builtin_scoring(s->scoring, s->hydrophobic_scale, d->terms.data(), d->ci.data(), &sf);
+			if (rc != 0)
+				throw std::invalid_argument("unknown scoring function");
+		}
 		const sgpu_model_desc* rd = sgpu_mol_desc(receptor);
 		sgpu_receptor rec{rd->natoms, rd->xyz, rd->type, rd->charge};
 		d->d.reset(new GpuDocking(gs, sf, rec, *gd));

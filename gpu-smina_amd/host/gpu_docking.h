@@ -106,6 +106,13 @@ std::string term_name(const sgpu_term& t);
 // model::movable_atoms_box (src/lib/model.cpp:677-707) of n atoms at xyz
 sgpu_grid_dims movable_atoms_box_at(const double* xyz, int n, double add, double granularity);
 
+// --custom_scoring (custom_terms::add_terms_from_file, src/lib/custom_terms.cpp:17-71):
+// the file's pairwise terms, charge-independent and charge-dependent, each in
+// file order, and its conf-independent terms, weights times scale; throws
+// std::invalid_argument naming the line of an unknown or unsupported term
+void parse_custom_scoring(const std::string& text, double scale, std::vector<sgpu_term>& ind,
+		std::vector<sgpu_term>& dep, std::vector<sgpu_ci_term>& ci);
+
 struct GpuSettings
 {
 	int device = 0;

@@ -111,6 +111,13 @@ class Term(C.Structure):
     _fields_ = [("kind", C.c_int), ("p", C.c_double * 5), ("weight", C.c_double)]
 
 
+# sgpu_term_kind and sgpu_ci_kind (include/sgpu_model.h)
+(TERM_GAUSS, TERM_REPULSION, TERM_HYDROPHOBIC, TERM_NON_HYDROPHOBIC, TERM_NON_DIR_H_BOND,
+ TERM_NON_DIR_ANTI_H_BOND_QUADRATIC, TERM_DONOR_DONOR_QUADRATIC, TERM_ACCEPTOR_ACCEPTOR_QUADRATIC, TERM_VDW,
+ TERM_NON_DIR_H_BOND_LJ, TERM_ELECTROSTATIC, TERM_AD4_SOLVATION) = range(12)
+(CI_NUM_TORS_DIV, CI_NUM_TORS_ADD, CI_NUM_TORS_SQR, CI_NUM_TORS_SQRT, CI_NUM_TORS_DIV_SIMPLE, CI_CONSTANT) = range(6)
+
+
 class CITerm(C.Structure):
     _fields_ = [("kind", C.c_int), ("weight", C.c_double)]
 
@@ -137,7 +144,8 @@ class DockSettings(C.Structure):
                 ("chain_split", C.c_int), ("early_stop", C.c_int), ("fast", C.c_int), ("skip_visited", C.c_int),
                 ("approximation", C.c_int), ("approx_factor", C.c_double),
                 ("user_grid", C.c_void_p), ("force_cap", C.c_double), ("accurate_line", C.c_int),
-                ("min_rmsd_filter", C.c_double), ("minimize_iters", C.c_int), ("minimize_early_term", C.c_int)]
+                ("min_rmsd_filter", C.c_double), ("minimize_iters", C.c_int), ("minimize_early_term", C.c_int),
+                ("custom_scoring", C.c_char_p)]
 
 
 class UserGrid(C.Structure):
@@ -219,6 +227,8 @@ def _load():
         "sgpu_conf_independent": (C.c_double, [C.POINTER(Scoring), C.c_double, C.c_double]),
         "sgpu_builtin_scoring": (C.c_int, [C.c_char_p, C.c_double, C.POINTER(Term), C.POINTER(CITerm),
                                            C.POINTER(Scoring)]),
+        "sgpu_parse_custom_scoring": (C.c_int, [C.c_char_p, C.c_double, C.POINTER(Term), C.c_int, C.POINTER(CITerm),
+                                                C.c_int, C.POINTER(Scoring), C.c_char_p, C.c_int]),
         "sgpu_build_tables": (C.c_int, [C.POINTER(Scoring), C.c_double, F64P, F64P]),
         "sgpu_build_splines": (C.c_int, [C.POINTER(Scoring), C.c_double, F64P, F64P]),
         "sgpu_upload_splines": (C.c_int, [vp, C.POINTER(Splines)]),
@@ -407,6 +417,18 @@ def builtin_scoring(name="vina", hydrophobic_scale=1.0):
     if rc != 0:
         raise ValueError(f"scoring {name!r}: unknown")
     sf._keep = (terms, ci)  # keep alive
+    return sf
+
+
+def custom_scoring(text, scale=1.0):
+    """--custom_scoring from the file's text (sgpu_parse_custom_scoring);
+    ValueError names the line of an unknown or unsupported term"""
+    n = max(1, text.count("\n") + 1)
+    terms, ci, sf = (Term * n)(), (CITerm * n)(), Scoring()
+    err = C.create_string_buffer(512)
+    if lib.sgpu_parse_custom_scoring(text.encode(), scale, terms, n, ci, n, C.byref(sf), err, 512) != 0:
+        raise ValueError(err.value.decode())
+    sf._keep = (terms, ci)
     return sf
 
 
@@ -786,7 +808,8 @@ class Docking:
                  hydrophobic_scale=1.0, ligands_per_launch=0, device=0, step_fraction=1.0, precision=None,
                  progress=None, progress_chunks=1, scoring="vina", chain_split=0, early_stop=False, fast=False,
                  skip_visited=False, approximation="linear", approx_factor=0.0, user_grid=None, force_cap=1000.0,
-                 accurate_line=False, min_rmsd_filter=1.0, minimize_iters=0, minimize_early_term=False):
+                 accurate_line=False, min_rmsd_filter=1.0, minimize_iters=0, minimize_early_term=False,
+                 custom_scoring=None):
         """precision: FP32, FP64 or MIXED (default: FP64 if fp64 else FP32);
         progress(fraction): called after each of progress_chunks launches
         of the Monte Carlo search; chain_split, early_stop, skip_visited,
@@ -798,7 +821,9 @@ class Docking:
         the chains keep monte_carlo's 1000); accurate_line: --accurate_line
         (every BFGS with the accurate line search); min_rmsd_filter:
         --min_rmsd_filter; minimize_iters, minimize_early_term: every BFGS's
-        iterations (0: (25 + movable atoms) / 3) and early termination"""
+        iterations (0: (25 + movable atoms) / 3) and early termination;
+        custom_scoring: the text of a --custom_scoring file, in place of
+        scoring"""
         prec = precision if precision is not None else (FP64 if fp64 else FP32)
         # keep the callback alive as long as the docking object
         self._progress = PROGRESS_FN(lambda user, frac: progress(frac)) if progress else PROGRESS_FN()
@@ -811,6 +836,8 @@ class Docking:
         s.min_rmsd_filter = min_rmsd_filter
         s.minimize_iters = minimize_iters
         s.minimize_early_term = int(minimize_early_term)
+        self._custom = custom_scoring.encode() if custom_scoring else None
+        s.custom_scoring = self._custom
         ug = None
         if user_grid is not None:
             ugd, vals = user_grid

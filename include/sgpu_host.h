@@ -123,6 +123,23 @@ int sgpu_write_sdf_poses(const sgpu_mol* ligand, int nposes, const double* conf,
 #define SGPU_MAX_CI 4
 int sgpu_builtin_scoring(const char* name, double hydrophobic_scale, sgpu_term* terms, sgpu_ci_term* ci,
 		sgpu_scoring* out);
+/* --custom_scoring (custom_terms::add_terms_from_file and add,
+ * src/lib/custom_terms.cpp:17-71): the file's text, one "weight term" per
+ * line (blank lines and lines starting with '#' skipped, anything after the
+ * term ignored), each term in smina's parameterized form, e.g.
+ * "gauss(o=0,_w=0.5,_c=8)" (src/lib/everything.h).  The pairwise terms are
+ * ordered as smina evaluates them, charge-independent ones, then
+ * charge-dependent ones, each in file order; the conf-independent ones in
+ * file order.  Weights are multiplied by scale (--user_grid_lambda; 1
+ * otherwise).  Supported: the pairwise terms of sgpu_term_kind and
+ * num_tors_add, num_tors_sqr, num_tors_sqrt, num_tors_div,
+ * num_tors_div_simple and constant_term; smina's other terms (atom_type_*,
+ * ligand_length, num_ligands, num_heavy_atoms, num_heavy_atoms_div,
+ * num_hydrophobic_atoms) are refused.  terms and ci hold max_terms and
+ * max_ci entries and back `out`.  Returns 0, or -1 with the line and reason
+ * in err. */
+int sgpu_parse_custom_scoring(const char* text, double scale, sgpu_term* terms, int max_terms, sgpu_ci_term* ci,
+		int max_ci, sgpu_scoring* out, char* err, int err_len);
 /* the final score of a pose (weighted_terms::conf_independent,
  * src/lib/weighted_terms.cpp:74-80): the conf-independent terms applied in
  * order to its intermolecular energy e with the ligand's num_tors */
@@ -182,6 +199,8 @@ typedef struct sgpu_dock_settings
 	                         * (25 + movable atoms) / 3) */
 	int minimize_early_term; /* --minimize_early_term: every BFGS stops once an iteration gains
 	                          * less than 1e-5 */
+	const char* custom_scoring; /* --custom_scoring: the file's text (sgpu_parse_custom_scoring),
+	                             * in place of `scoring` (NULL: the built-in) */
 } sgpu_dock_settings;
 
 sgpu_docking* sgpu_docking_create(const sgpu_dock_settings* s, const sgpu_mol* receptor,
