@@ -2066,6 +2066,42 @@ int sgpu_dock(sgpu_docking* d, const sgpu_mol* const* ligands, int n, sgpu_dock_
 	}
 }
 
+int sgpu_autobox(const sgpu_mol* const* mols, int n, double add, double granularity, const double* size,
+		sgpu_grid_dims* out)
+{
+	if (!mols || n < 1 || !out || !(granularity > 0) || !(add >= 0))
+		return -1;
+	double lo[3], hi[3];
+	bool any = false;
+	for (int k = 0; k < n; k++)
+	{
+		const sgpu_model_desc* m = mols[k] ? sgpu_mol_desc(mols[k]) : nullptr;
+		if (!m)
+			return -1;
+		for (int a = 0; a < m->natoms; a++)
+			for (int d = 0; d < 3; d++)
+			{
+				const double x = m->xyz[3 * a + d];
+				lo[d] = any ? std::min(lo[d], x) : x;
+				hi[d] = any ? std::max(hi[d], x) : x;
+				if (d == 2)
+					any = true;
+			}
+	}
+	if (!any)
+		return -1;
+	for (int d = 0; d < 3; d++)
+	{
+		const double center = (hi[d] + lo[d]) / 2.0;
+		const double span = size && size[d] > 0 ? size[d] : (hi[d] + add) - (lo[d] - add);
+		out->n[d] = int(std::ceil(span / granularity));
+		const double real_span = granularity * out->n[d];
+		out->begin[d] = center - real_span / 2;
+		out->end[d] = out->begin[d] + real_span;
+	}
+	return 0;
+}
+
 int sgpu_minimize(sgpu_docking* d, const sgpu_mol* const* ligands, int n, int maxiters, int accurate,
 		double force_cap, sgpu_minimize_result* out)
 {

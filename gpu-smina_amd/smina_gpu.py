@@ -241,6 +241,7 @@ def _load():
         "sgpu_docking_free": (None, [vp]),
         "sgpu_docking_error": (C.c_char_p, [vp]),
         "sgpu_dock": (C.c_int, [vp, C.POINTER(vp), C.c_int, C.POINTER(DockResult)]),
+        "sgpu_autobox": (C.c_int, [C.POINTER(vp), C.c_int, C.c_double, C.c_double, F64P, C.c_void_p]),
         "sgpu_minimize": (C.c_int, [vp, C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_double,
                                     C.POINTER(MinimizeResult)]),
         "sgpu_server_create": (vp, [C.POINTER(ServerSettings)]),
@@ -494,6 +495,18 @@ def grid_dims(center, size, spacing=0.375):
         gd.begin[i] = center[i] - real / 2
         gd.end[i] = gd.begin[i] + real
         gd.n[i] = n
+    return gd
+
+
+def autobox(mols, add=4.0, spacing=0.375, size=None):
+    """--autobox_ligand / --autobox_add (sgpu_autobox): the box around every
+    atom of mols, each side grown by add; size components > 0 replace the
+    grown extent (--size_x/y/z)"""
+    arr = (C.c_void_p * len(mols))(*[m.h.value for m in mols])
+    sz = f64(size if size is not None else (0.0, 0.0, 0.0))
+    gd = GridDims()
+    if lib.sgpu_autobox(arr, len(mols), add, spacing, ptr(sz, F64P), C.addressof(gd)) != 0:
+        raise ValueError("autobox: no atoms or a bad argument")
     return gd
 
 

@@ -254,6 +254,14 @@ typedef struct sgpu_minimize_result
 } sgpu_minimize_result;
 int sgpu_minimize(sgpu_docking* d, const sgpu_mol* const* ligands, int n, int maxiters, int accurate,
 		double force_cap, sgpu_minimize_result* out);
+/* --autobox_ligand / --autobox_add (setup_autobox, src/main/main.cpp:648-687,
+ * and the box of :800-809): the bounding box of every atom of the n
+ * molecules, centred on its middle, each side grown by add (4 by default);
+ * a size component > 0 in `size` (may be NULL) replaces the grown extent,
+ * as --size_x/y/z do; then n = ceil(size / granularity) intervals spread
+ * around the centre.  Returns 0, or -1 for no atoms or a bad argument. */
+int sgpu_autobox(const sgpu_mol* const* mols, int n, double add, double granularity, const double* size,
+		sgpu_grid_dims* out);
 /* ligands of the last sgpu_dock that got the full search (all of them
  * unless a two-pass screen ran) */
 int sgpu_docking_full_searches(const sgpu_docking* d);
