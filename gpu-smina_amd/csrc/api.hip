@@ -1579,14 +1579,14 @@ void run_mc(sgpu_ctx* c, int batch_id, int grid_id, const sgpu_mc_params* p, con
 	// Speculative chains: with fewer chains than the device has wave slots,
 	// each chain gets `spec` waves (wave 0 runs it, the others evaluate the
 	// next trials of its fast line search), without QuickVina 2's history;
-	// the widest of 4 and 2 whose n x width waves all fit at once.
-	// SGPU_SPEC=1 turns it off, 2..4 forces the width.
+	// the widest of 8, 4 and 2 whose n x width waves all fit at once.
+	// SGPU_SPEC=1 turns it off, 2..8 forces the width.
 	int spec = 1;
 	if (!p->visited && !p->accurate_line_search)
 	{
 		const char* senv = std::getenv("SGPU_SPEC");
 		const int forced = senv ? std::atoi(senv) : 0;
-		for (int w : {4, 3, 2})
+		for (int w : {8, 4, 3, 2})
 		{
 			if (forced ? w != forced : w == 3)
 				continue;

@@ -300,7 +300,7 @@ struct Visited
 // passes the Armijo test ends the line search.  The chain's confs,
 // energies and evaluation counts are the sequential search's.  The helpers
 // touch no chain state.
-constexpr int SPEC_MAX = 4;
+constexpr int SPEC_MAX = 8;
 constexpr int SPEC_REQ = MAX_CONF + MAX_CHANGE + 4;  // x, p, alpha, caps
 constexpr int SPEC_RS = 1 + MAX_CONF + MAX_CHANGE;   // per wave: f, conf, gradient
 template<typename S>

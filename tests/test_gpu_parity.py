@@ -337,7 +337,7 @@ def check_against_oracle(env, models, cs, ss, steps, iters, out):
 
 @pytest.mark.parametrize("form", ["graph-persistent", "graph-oneshot", "eager-persistent", "eager-oneshot",
                                   "graph-persistent-grid2", "eager-persistent-grid2",
-                                  "graph-speculative4", "eager-speculative2"])
+                                  "graph-speculative4", "eager-speculative2", "graph-speculative8"])
 def test_mc_fp64_launch_forms(env, form, monkeypatch):
     """the Monte Carlo launch in each of its forms: persistent grids over the
     device work queue (a grid of 2 workgroups makes each run several items),
@@ -716,7 +716,7 @@ def test_fp32_launch_forms_agree(env, prec, monkeypatch):
 
 @pytest.mark.parametrize("prec", [sg.FP32, sg.MIXED, sg.FP64], ids=["fp32", "mixed", "fp64"])
 def test_speculative_chains_equal_sequential(env, prec, monkeypatch):
-    """speculative line-search trials: with 2 or 4 waves per chain (wave 0
+    """speculative line-search trials: with 2, 4 or 8 waves per chain (wave 0
     runs the chain, the others evaluate the next trial steps of its fast
     line search), every chain of a batch across the size classes gives the
     sequential search's result exactly: saved energies, confs, coordinates
@@ -728,16 +728,16 @@ def test_speculative_chains_equal_sequential(env, prec, monkeypatch):
     models, cs, ss, steps, iters = batched_chains(env, seed0=5100)
     steps = [30 + 10 * li for li in models]
     outs = {}
-    for w in ("1", "2", "4"):
+    for w in ("1", "2", "4", "8"):
         monkeypatch.setenv("SGPU_SPEC", w)
         print(f"speculative chains: precision {prec}, width {w}", flush=True)
         outs[w] = ctx.run_mc(bid, gid, sg.mc_params(), models, cs, ss, steps, iters)
-    for w in ("2", "4"):
+    for w in ("2", "4", "8"):
         for a, b in zip(outs["1"], outs[w]):
             assert np.array_equal(a, b), w
     assert min(outs["1"][0]) > 0
     if prec == sg.FP64:
-        check_against_oracle(env, models, cs, ss, steps, iters, outs["4"])
+        check_against_oracle(env, models, cs, ss, steps, iters, outs["8"])
     if prec not in env["bids"]:
         ctx.release_models(bid)
 
