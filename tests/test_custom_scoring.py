@@ -109,3 +109,21 @@ def test_dock_fp64_custom_scoring_matches_oracle():
     e, c, ev = o.dock(m, sg.mc_params(), seed=17, exhaustiveness=4, threads=8, step_fraction=0.1)
     print(f"dock with a custom scoring file: {r['e'][:3]}", flush=True)
     assert len(r["e"]) > 0 and r["e"] == list(e)
+
+
+def test_names_written_by_atom_terms_read_back():
+    """the term names the --atom_terms table writes in its header (smina's
+    parameterized forms) are read back into the same terms"""
+    from common import receptor
+    text = CUSTOM + "0.0489 ad4_solvation(d-sigma=3.6,_s/q=0.01097,_c=8)\n0.1465 electrostatic(i=2,_^=100,_c=8)\n" \
+        "0.0089 vdw(i=4,_j=8,_s=0,_^=100,_c=8)\n-0.0974 non_dir_h_bond_lj(o=-0.7,_^=100,_c=8)\n"
+    sf = sg.custom_scoring(text)
+    lig = sg.Mol.ligand(1, 24, 6)
+    f = sg.Flat(lig)
+    head = lig.atom_terms(sf, receptor(), f.initial_conf()).strip().split("\n")[0].split()
+    names = [h for h in head if "(" in h]
+    assert len(names) == sf.nterms
+    back = sg.custom_scoring("".join(f"1.0 {n}\n" for n in names))
+    got = sorted((back.terms[i].kind, tuple(back.terms[i].p)) for i in range(back.nterms))
+    want = sorted((sf.terms[i].kind, tuple(sf.terms[i].p)) for i in range(sf.nterms))
+    assert got == want
